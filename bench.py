@@ -1,0 +1,149 @@
+"""Throughput of the explicit MLS-MPM/APIC substep on one MI355X.
+
+Workload: the 192^3 Newtonian honey pour (340k particles at 8 per cell, two revolved-SDF
+glasses, the source tilting at 1 rad/s about its rim, 432 substeps per 1/60 s frame).
+A "step" is one frame (one control tick of 432 substeps). Prints one JSON line:
+value = particle-substeps/s over the timed frames (particles resident on the GPU),
+plus the roofline of the dominant kernel (the fused g2p+stress+p2g particle pass),
+timed with HIP events on its stream during the timed frames, and the CPU oracle's
+throughput on a bounded sample of the same workload.
+
+    python bench.py [--gpus 1] [--steps 10] [--warmup 2] [--no-cpu]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(REPO, "mpm-engine_amd"))
+sys.path.insert(0, REPO)
+
+HBM_PEAK_GBS = 8000.0
+# algorithmic bytes (reference-state semantics): per particle-substep the fused pass reads
+# x 12, F 36, selection 4, material 4, mass 4, vol 4, mu 4, lam 4, yield 4 (76 B) and the
+# reference's kernel writes x 12, v 12, C 36, L 36, F_trial 36, F 36, stress 36 (204 B);
+# per active grid node it writes the P2G result (16 B) and reads the G2P velocity (12 B).
+# The grid update moves the remaining 56 B per active node (zero m+v_in 16, normalise
+# read 16 + write 12, zero v_out 12): 280 B/particle + 84 B/active node per substep.
+PASS_B_PER_PARTICLE = 280.0
+PASS_B_PER_NODE = 28.0
+GRID_B_PER_NODE = 56.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-substeps", type=int, default=6)
+    ap.add_argument("--graph", action="store_true", help="replay ticks as hipGraphs")
+    return ap.parse_args()
+
+
+def cpu_baseline(substeps: int):
+    """The oracle (single-threaded C restatement of the reference kernels, driven by the
+    restated host schedule) on the same pour, for `substeps` substeps of one frame."""
+    from mpm_amd.scenes import PourScene
+    from oracle.sim import OracleSolver
+
+    sc = PourScene()
+    s = sc.build(OracleSolver(sc.grid))
+    sc.substeps_saved = sc.substeps
+    t0 = time.perf_counter()
+    s.set_cup(sc.src, center=sc.center, quat=sc.quat, velocity=[0, 0, 0], omega=sc.omega)
+    s.step(sc.dt, substeps)
+    el = time.perf_counter() - t0
+    n = len(sc.pos)
+    return {"value": n * substeps / el, "unit": "particle-substeps/s", "cores": 1, "kind": "port",
+            "sample": f"honey pour 192^3, {n} particles, {substeps} substeps of one frame "
+                      f"({el:.1f} s, single-threaded C oracle)"}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
+                                   "(multi-GPU sharding is a design only)"}))
+        sys.exit(2)
+    import numpy as np
+    import torch
+
+    from mpm_amd import Solver
+    from mpm_amd.core import native
+    from mpm_amd.scenes import PourScene
+
+    sc = PourScene()
+    s = sc.build(Solver(grid=sc.grid, device="cuda:0"))
+    sim = s._sim
+    sim.use_graph = bool(args.graph)
+    n = s.n_particles
+    for _ in range(args.warmup):
+        sc.frame(s)
+    torch.cuda.synchronize()
+
+    # dominant-kernel timing: HIP events around every fused pass launch, on its stream
+    events = []
+    orig_pass = sim._pass
+    fused_mode = native.PASS_G2P | native.PASS_STRESS | native.PASS_P2G
+
+    def timed_pass(mode, dt, stream):
+        if (mode & fused_mode) == fused_mode and not sim.use_graph:
+            a = torch.cuda.Event(enable_timing=True)
+            b = torch.cuda.Event(enable_timing=True)
+            a.record()
+            orig_pass(mode, dt, stream)
+            b.record()
+            events.append((a, b))
+        else:
+            orig_pass(mode, dt, stream)
+
+    sim._pass = timed_pass
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        sc.frame(s)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    sim._pass = orig_pass
+    if not np.isfinite(s.x()).all():
+        raise RuntimeError("non-finite particle state after the timed frames")
+
+    substeps = args.steps * sc.substeps
+    value = n * substeps / el
+    # active nodes: nodes receiving mass in one scatter of the current state
+    sim.node_in.zero_()
+    sim._pass(native.PASS_STRESS | native.PASS_P2G, sc.dt, sim.stream())
+    active = int((sim.node_in.view(-1, 4)[:, 3] > 0).sum().item())
+    sim.node_in.zero_()
+    roof = None
+    if events:
+        ms = [a.elapsed_time(b) for a, b in events]
+        avg_s = float(np.mean(ms)) / 1e3
+        algo = PASS_B_PER_PARTICLE * n + PASS_B_PER_NODE * active
+        ach = algo / avg_s / 1e9
+        roof = {"bound": "hbm", "achieved": round(ach, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
+                "kernel": "k_particle_pass (fused g2p+stress+p2g)",
+                "avg_launch_us": round(avg_s * 1e6, 2), "launches": len(ms),
+                "algorithmic_bytes_per_launch": int(algo), "active_nodes": active}
+    cpu = None if args.no_cpu else cpu_baseline(args.cpu_substeps)
+    out = {"metric": "particle-substeps/s at 340k pour", "value": round(value, 1),
+           "unit": "particle-substeps/s", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": round(el / args.steps * 1e3, 3),
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+           "data": "synthetic",
+           "config": {"workload": "honey pour 192^3, eta 10 Pa s, 2 revolved-SDF glasses, "
+                                  "source tilting 1 rad/s", "particles": n,
+                      "substeps_per_step": sc.substeps, "dt": sc.dt, "grid": 192,
+                      "graph": bool(args.graph)},
+           "roofline": roof, "cpu_baseline": cpu}
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,218 @@
+/*
+ * mpm_hip.h - C ABI of libmpm_hip.so, the gfx950 implementation of warpmpm's explicit
+ * MLS-MPM/APIC substep.
+ *
+ * The reference drives its Warp kernels from Python (MPM_Simulator_WARP in
+ * src/warpmpm/kernels/mpm_solver_warp.py); every kernel takes the MPMStateStruct /
+ * MPMModelStruct by value (src/warpmpm/kernels/warp_utils.py:23-157). This header is the
+ * replacement boundary: plain structs of device pointers and sizes, no C++ or torch types.
+ * The caller (the Python Solver of this repository, or a ctypes binding a maintainer adds
+ * to the reference) owns every allocation; the library never allocates persistent state.
+ * Every function returns an mpm_status (0 = ok) and enqueues work on `stream` (a
+ * hipStream_t passed as void*; NULL = the legacy default stream).
+ *
+ * Memory layout (device):
+ *  - particles are SoA planes: x[c*stride + p]; mat33 fields hold 9 planes, component
+ *    (r, c) in plane 3*r + c (row-major like Warp's mat33).
+ *  - the grid is dense but blocked: 4x4x4-node blocks, block (bi, bj, bk) at
+ *    ((bi*nb[1] + bj)*nb[2] + bk), node (i, j, k) at block*64 + (i&3)*16 + (j&3)*4 + (k&3).
+ *    node_in holds (momentum x, y, z, mass) and node_out (velocity x, y, z, 0) as float4.
+ */
+#ifndef MPM_HIP_H
+#define MPM_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MPM_MAX_MATERIALS 16
+#define MPM_ABI_VERSION 1
+
+typedef enum mpm_status {
+    MPM_OK = 0,
+    MPM_ERR_ARG = 1,    /* bad argument (null pointer, size mismatch) */
+    MPM_ERR_HIP = 2,    /* a HIP runtime call failed */
+    MPM_ERR_SIZE = 3,   /* size beyond a documented maximum */
+} mpm_status;
+
+/* particle-pass mode bits (mpm_particle_pass) */
+#define MPM_PASS_G2P 1          /* gather: g2p_particle, mpm_utils.py:944-1043 */
+#define MPM_PASS_STRESS 2       /* stress_update_particle, mpm_utils.py:1056-1149 */
+#define MPM_PASS_P2G 4          /* scatter: p2g_particle, mpm_utils.py:819-916 */
+#define MPM_PASS_LEAN 8         /* do not store v, C, L, F_trial (dead between fused substeps) */
+#define MPM_PASS_KEEP_STRESS 16 /* with LEAN: still store the stress */
+
+/* grid-update flag bits (mpm_grid_update) */
+#define MPM_GRID_ZERO_IN 1      /* clear node_in after it is consumed */
+
+/* collider kinds (the reference's grid_postprocess plugins) */
+#define MPM_COLLIDER_PLANE 1      /* mpm_solver_warp.py:1929-1992 */
+#define MPM_COLLIDER_BOX 2        /* mpm_solver_warp.py:2053-2088 */
+#define MPM_COLLIDER_CUP 3        /* revolved SDF glass, mpm_solver_warp.py:2161-2225 */
+#define MPM_COLLIDER_WALLS 4      /* mpm_solver_warp.py:2294-2359 */
+#define MPM_COLLIDER_SDF 5        /* mesh SDF, mpm_solver_warp.py:2680-2736 */
+#define MPM_COLLIDER_POINTCLOUD 6 /* mpm_solver_warp.py:2586-2605 */
+
+/* Particle state (MPMStateStruct particle_* arrays, warp_utils.py:87-123, plus the
+ * per-particle model arrays mu / lam / yield_stress, warp_utils.py:31-40). */
+typedef struct mpm_particles {
+    int32_t n;       /* particle count */
+    int32_t stride;  /* plane stride in elements, >= n */
+    float *x, *v;                          /* 3 planes */
+    float *F, *F_trial, *C, *stress, *L;   /* 9 planes each */
+    float *vol, *mass, *mu, *lam, *yield_stress, *Jp;
+    int32_t *material, *selection;
+    float *cov; /* 6 planes (xx, xy, xz, yy, yz, zz) or NULL when covariance is off */
+} mpm_particles_t;
+
+typedef struct mpm_grid {
+    float *node_in;       /* float4 per node: momentum xyz, mass */
+    float *node_out;      /* float4 per node: velocity xyz, 0 */
+    int32_t n[3];         /* logical node counts */
+    int32_t nb[3];        /* blocks per axis, ceil(n / 4) */
+    float dx, inv_dx;     /* fp32 like the reference's MPMModelStruct.dx / inv_dx */
+    int32_t *error_flag;  /* device int, OR-ed bits: 1 = a stencil left the grid,
+                             2 = a non-finite particle position */
+} mpm_grid_t;
+
+/* Model parameters (MPMModelStruct, warp_utils.py:23-84). Lives in device memory; the
+ * per-type arrays are indexed by material id, exactly as in the reference. */
+typedef struct mpm_model {
+    float gravity[3];
+    float rpic_damping;
+    float grid_v_damping_scale;
+    int32_t update_cov_with_F;
+    int32_t periodic_x;
+    int32_t eta_table_n;
+    float eta_table_smin, eta_table_smax;
+    const float *eta_table;   /* device, eta_table_n samples */
+    float E[MPM_MAX_MATERIALS], nu[MPM_MAX_MATERIALS], bulk[MPM_MAX_MATERIALS];
+    float hardening[MPM_MAX_MATERIALS], xi[MPM_MAX_MATERIALS];
+    float plastic_viscosity[MPM_MAX_MATERIALS], softening[MPM_MAX_MATERIALS];
+    float alpha[MPM_MAX_MATERIALS];
+    float muI_mu_s[MPM_MAX_MATERIALS], muI_delta_mu[MPM_MAX_MATERIALS];
+    float muI_I0[MPM_MAX_MATERIALS], muI_d[MPM_MAX_MATERIALS], muI_rho_s[MPM_MAX_MATERIALS];
+    float muI_phi_init[MPM_MAX_MATERIALS], muI_phi_chi[MPM_MAX_MATERIALS];
+} mpm_model_t;
+
+/* One collider at one substep (the reference's per-collider param struct plus the pose
+ * its host modify_bc callable integrated to, mpm_solver_warp.py:1283-1336). A tick's
+ * schedule is a device table [substeps][ncoll] the host fills before the tick. */
+typedef struct mpm_collider {
+    int32_t kind;        /* MPM_COLLIDER_* */
+    int32_t surface;     /* plane 0 sticky / 1 slip / 2 separable / 11 cut; sdf 0 / 1 / 2 */
+    int32_t reset;       /* box reset flag (mpm_solver_warp.py:2086-2088) */
+    int32_t slot;        /* wrench slot (force at 6*slot, torque at 6*slot + 3); -1 none */
+    int32_t lo[3], hi[3];/* node box the collider acts on (its restricted launch box) */
+    int32_t active;      /* 0: launch skipped this substep (empty box) */
+    int32_t res;         /* sdf voxel resolution */
+    float time, dt;      /* fp32 simulation time of this substep, substep dt */
+    float start_time, end_time;
+    float point[3], normal[3], size[3], velocity[3], omega[3];
+    float rot[9];        /* cup rotation, row-major */
+    float quat[4];       /* sdf orientation (x, y, z, w) */
+    float friction, sticky_depth, contact_band, band;
+    float outer_radius, inner_radius, half_height, inner_floor_z, fillet_radius;
+    float origin[3], cell;
+    const float *sdf_val;      /* device [res^3] */
+    const float *sdf_grad;     /* device [res^3][3] */
+    const int32_t *occupancy;  /* device dense [n0*n1*n2], z fastest */
+} mpm_collider_t;
+
+/* Device-resident per-tick control read by the grid kernels, so a captured hipGraph
+ * replays with this tick's launch box without recapture. */
+typedef struct mpm_tick_ctl {
+    int32_t box_lo[3], box_hi[3]; /* node box of normalisation and BCs (the particle box) */
+    int32_t nblocks;              /* entries valid in the block list */
+    int32_t pad;
+} mpm_tick_ctl_t;
+
+/* ---- introspection (no GPU work) ------------------------------------------------- */
+int32_t mpm_abi_version(void);
+const char *mpm_status_string(int32_t status);
+/* sizeof of the ABI structs, so bindings can verify their mirrors */
+int32_t mpm_struct_size(int32_t which); /* 0 particles, 1 grid, 2 model, 3 collider, 4 tick_ctl */
+
+/* ---- particle pass --------------------------------------------------------------- */
+/* One pass over all particles with selection == 0 doing the stages in `mode`:
+ * G2P|STRESS|P2G is the fused interior substep g2p_stress_p2g (mpm_utils.py:1161-1178);
+ * STRESS|P2G is the tick prologue (compute_stress_from_F_trial + p2g_apic_with_stress,
+ * mpm_utils.py:919-923, 1152-1158); G2P alone is the tick epilogue (g2p, :1046-1052).
+ * Scatter accumulates into node_in with float atomics; a stencil node outside the grid
+ * is skipped and sets error bit 1. `model` is a device pointer. */
+int32_t mpm_particle_pass(const mpm_particles_t *P, const mpm_grid_t *G, const mpm_model_t *model,
+                          float dt, int32_t mode, void *stream);
+
+/* ---- grid -------------------------------------------------------------------------- */
+/* Clear node_in (which & 1) and / or node_out (which & 2) over a node box [lo, hi).
+ * Replaces zero_grid / zero_grid_m_vin / zero_grid_vout (mpm_utils.py:590-598,1181-1202). */
+int32_t mpm_grid_zero_box(const mpm_grid_t *G, const int32_t *lo, const int32_t *hi,
+                          int32_t which, void *stream);
+
+/* Same over a device block list (zero_grid_blocks, mpm_utils.py:1556-1564). */
+int32_t mpm_grid_zero_blocks(const mpm_grid_t *G, const int32_t *block_list,
+                             const int32_t *block_count, int32_t capacity, int32_t which,
+                             void *stream);
+
+/* Build the list of blocks intersecting ctl->box (ctl is device memory) into
+ * block_list and set ctl->nblocks. */
+int32_t mpm_blocks_from_box(const mpm_grid_t *G, mpm_tick_ctl_t *ctl, int32_t *block_list,
+                            void *stream);
+
+/* Grid update of one substep over the listed blocks, restricted to ctl->box:
+ * normalisation + gravity (grid_normalization_and_gravity, mpm_utils.py:927-941),
+ * damping (add_damping_via_grid, :1287-1295), then every collider of `colliders`
+ * (ncoll entries, device) in order, each acting inside its own box on nodes with mass,
+ * accumulating its reaction impulse / torque into wrench[6*slot]. Nodes without mass
+ * above 1e-15 read velocity 0 (the split-zero contract, mpm_utils.py:1193-1202).
+ * With MPM_GRID_ZERO_IN, node_in is cleared afterwards. */
+int32_t mpm_grid_update(const mpm_grid_t *G, const mpm_model_t *model, const mpm_tick_ctl_t *ctl,
+                        const int32_t *block_list, const int32_t *block_count, int32_t capacity,
+                        const mpm_collider_t *colliders, int32_t ncoll, float *wrench, float dt,
+                        int32_t flags, void *stream);
+
+/* ---- sparse active blocks (mpm_utils.py:1506-1544, mpm_solver_warp.py:989-1026) ----- */
+/* raw must be zero on entry; marks the stencil-base block of every particle. */
+int32_t mpm_blocks_mark(const mpm_particles_t *P, const mpm_grid_t *G, int32_t *raw, void *stream);
+int32_t mpm_blocks_dilate(const mpm_grid_t *G, const int32_t *raw, int32_t *cur, void *stream);
+/* counts (device, 2 ints) must be zero on entry; cur_list / union_list get block codes. */
+int32_t mpm_blocks_compact(const mpm_grid_t *G, const int32_t *cur, const int32_t *prev,
+                           int32_t *cur_list, int32_t *union_list, int32_t *counts, void *stream);
+
+/* ---- guard / launch box (solver.py:485-518) ---------------------------------------- */
+/* out (device, 8 floats): min x, y, z, max x, y, z, max |v| component, non-finite count.
+ * Works on all particles, like the reference's host readback. */
+int32_t mpm_bounds(const mpm_particles_t *P, float *out, void *stream);
+
+/* ---- block sort (solver.py:454-467, mpm_solver_warp.py:1548-1619) ------------------- */
+/* keys[p] = ((bx*nb + by)*nb + bz) with b = floor(x/dx - 0.5) >> 2, nb = n0/4 + 2;
+ * vals[p] = p; *unsorted (device int) gets 1 if keys are not non-decreasing. */
+int32_t mpm_sort_keys(const mpm_particles_t *P, const mpm_grid_t *G, int32_t *keys,
+                      int32_t *vals, int32_t *unsorted, void *stream);
+/* temp storage needed by mpm_sort_pairs for n keys */
+int64_t mpm_sort_temp_bytes(int32_t n);
+/* stable radix sort of (keys, vals) pairs into (keys_out, vals_out) */
+int32_t mpm_sort_pairs(const int32_t *keys, int32_t *keys_out, const int32_t *vals,
+                       int32_t *vals_out, int32_t n, void *temp, int64_t temp_bytes, void *stream);
+/* gather every particle array by perm (dst[i] = src[perm[i]]) through `scratch`
+ * (>= 9 * stride floats) and copy back in place, so device pointers stay stable. */
+int32_t mpm_permute(const mpm_particles_t *P, const int32_t *perm, float *scratch, void *stream);
+
+/* ---- exports ------------------------------------------------------------------------ */
+/* compute_R_from_F (mpm_utils.py:1261-1285): R_out[9*p + ...] = (U V^T)^T from F_trial,
+ * row-major AoS [n][3][3], as the reference stores it. */
+int32_t mpm_export_R(const mpm_particles_t *P, float *R_out, void *stream);
+/* compute_cov_from_F (mpm_utils.py:1234-1258): cov_out[6*p + ...] = F init F^T from
+ * F_trial and init_cov (AoS [n][6]). */
+int32_t mpm_export_cov(const mpm_particles_t *P, const float *init_cov, float *cov_out,
+                       void *stream);
+/* dense copies of the grid in the reference's layout (z fastest): m [n^3], v_in and
+ * v_out [n^3][3]; any output may be NULL. */
+int32_t mpm_grid_to_dense(const mpm_grid_t *G, float *m, float *v_in, float *v_out, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

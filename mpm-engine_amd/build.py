@@ -1,0 +1,60 @@
+"""Build recipe for libmpm_hip.so (gfx950) and the CPU oracle.
+
+    python mpm-engine_amd/build.py          # both
+    python mpm-engine_amd/build.py --hip    # only the HIP library
+
+The HIP library is compiled with hipcc for gfx950 only, hardware float atomics
+(-munsafe-fp-atomics) and packed-FP32 instructions disabled. Objects go to
+mpm-engine_amd/_build, the library to mpm-engine_amd/lib (both git-ignored).
+"""
+from __future__ import annotations
+
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+REPO = os.path.dirname(HERE)
+CSRC = os.path.join(HERE, "csrc")
+BUILD = os.path.join(HERE, "_build")
+LIB = os.path.join(HERE, "lib", "libmpm_hip.so")
+SOURCES = ["mpm_kernels.hip", "mpm_sort.hip"]
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-munsafe-fp-atomics",
+         "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+         "-I" + os.path.join(REPO, "include")]
+
+
+def _run(cmd):
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    noise = "packed-fp32-ops"
+    err = "\n".join(l for l in r.stderr.splitlines() if noise not in l)
+    if r.returncode != 0:
+        raise RuntimeError(f"build failed: {' '.join(cmd)}\n{err}")
+
+
+def build_hip():
+    os.makedirs(BUILD, exist_ok=True)
+    os.makedirs(os.path.dirname(LIB), exist_ok=True)
+    objs = []
+    for src in SOURCES:
+        obj = os.path.join(BUILD, src.replace(".hip", ".o"))
+        _run([HIPCC, *FLAGS, "-c", os.path.join(CSRC, src), "-o", obj])
+        objs.append(obj)
+    tmp = LIB + ".tmp"
+    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-o", tmp, *objs])
+    os.replace(tmp, LIB)
+    return LIB
+
+
+def build_oracle():
+    _run(["make", "-s", "-C", os.path.join(REPO, "oracle")])
+
+
+if __name__ == "__main__":
+    args = set(sys.argv[1:])
+    if "--oracle" not in args:
+        print("built", build_hip())
+    if "--hip" not in args:
+        build_oracle()
+        print("built oracle")

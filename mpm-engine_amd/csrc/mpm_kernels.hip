@@ -1,0 +1,860 @@
+// gfx950 kernels and C-ABI entry points of libmpm_hip.so (declared in include/mpm_hip.h).
+//
+// Particle state is SoA planes, the grid is a dense array of 4x4x4-node blocks holding
+// float4 (momentum, mass) in and float4 velocity out. All launches go to the caller's
+// stream; nothing here allocates.
+#include <hip/hip_runtime.h>
+
+#include "mpm_device.h"
+
+namespace {
+
+constexpr int kThreads = 256;
+
+MPM_DEV int node_index(const mpm_grid_t& G, int i, int j, int k) {
+    return ((((i >> 2) * G.nb[1] + (j >> 2)) * G.nb[2] + (k >> 2)) << 6) + ((i & 3) << 4) +
+           ((j & 3) << 2) + (k & 3);
+}
+
+MPM_DEV void decode_block(const mpm_grid_t& G, int code, int& bi, int& bj, int& bk) {
+    bk = code % G.nb[2];
+    int r = code / G.nb[2];
+    bj = r % G.nb[1];
+    bi = r / G.nb[1];
+}
+
+MPM_DEV v3 ld3(const float* base, int stride, int p) {
+    return mk3(base[p], base[stride + p], base[2 * stride + p]);
+}
+MPM_DEV void st3(float* base, int stride, int p, v3 v) {
+    base[p] = v.x; base[stride + p] = v.y; base[2 * stride + p] = v.z;
+}
+MPM_DEV m33 ld9(const float* base, int stride, int p) {
+    m33 m;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) m.a[i] = base[i * stride + p];
+    return m;
+}
+MPM_DEV void st9(float* base, int stride, int p, const m33& m) {
+#pragma unroll
+    for (int i = 0; i < 9; ++i) base[i * stride + p] = m.a[i];
+}
+
+// ------------------------------------------------------------------ particle pass
+// g2p_particle (mpm_utils.py:944-1043), stress_update_particle (:1056-1149) and
+// p2g_particle (:819-916) as one per-particle pass; MODE selects the stages.
+template <int MODE>
+__global__ __launch_bounds__(kThreads) void k_particle_pass(mpm_particles_t P, mpm_grid_t G,
+                                                            const mpm_model_t* __restrict__ md,
+                                                            float dt) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= P.n) return;
+    if (P.selection[p] != 0) return;
+    const int S = P.stride;
+    const int mat = P.material[p];
+    constexpr bool kG2P = (MODE & MPM_PASS_G2P) != 0;
+    constexpr bool kStress = (MODE & MPM_PASS_STRESS) != 0;
+    constexpr bool kP2G = (MODE & MPM_PASS_P2G) != 0;
+    constexpr bool kLean = (MODE & MPM_PASS_LEAN) != 0;
+    constexpr bool kKeepStress = (MODE & MPM_PASS_KEEP_STRESS) != 0;
+    const int periodic = md->periodic_x;
+
+    v3 x = ld3(P.x, S, p);
+    v3 v = mk3(0.f, 0.f, 0.f);
+    m33 Cm = mzero(), L = mzero(), Ft = mzero();
+    const bool gather = kG2P && mat != 7 && mat != 8;
+    if (kG2P && !gather && !kStress && !kP2G) return;
+    int err = 0;
+
+    if (gather) {
+        Stencil st = make_stencil(x, G.inv_dx);
+        v3 nv = mk3(0.f, 0.f, 0.f);
+        m33 nC = mzero(), nF = mzero();
+        const float4* __restrict__ out = reinterpret_cast<const float4*>(G.node_out);
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            int ix = st.base[0] + i;
+            if (periodic) ix = ((ix % G.n[0]) + G.n[0]) % G.n[0];
+#pragma unroll
+            for (int j = 0; j < 3; ++j) {
+                int iy = st.base[1] + j;
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    int iz = st.base[2] + k;
+                    float4 gv4 = make_float4(0.f, 0.f, 0.f, 0.f);
+                    if ((unsigned)ix < (unsigned)G.n[0] && (unsigned)iy < (unsigned)G.n[1] &&
+                        (unsigned)iz < (unsigned)G.n[2])
+                        gv4 = out[node_index(G, ix, iy, iz)];
+                    else
+                        err = 1;
+                    v3 gv = mk3(gv4.x, gv4.y, gv4.z);
+                    float weight = st.w[0][i] * st.w[1][j] * st.w[2][k];
+                    nv = nv + gv * weight;
+                    v3 dpos = mk3((float)i - st.fx[0], (float)j - st.fx[1], (float)k - st.fx[2]);
+                    float cw = weight * G.inv_dx * 4.0f;
+                    v3 dwt = mk3(st.dw[0][i] * st.w[1][j] * st.w[2][k], st.w[0][i] * st.dw[1][j] * st.w[2][k],
+                                 st.w[0][i] * st.w[1][j] * st.dw[2][k]) * G.inv_dx;
+                    float g[3] = {gv.x, gv.y, gv.z};
+                    float dp[3] = {dpos.x, dpos.y, dpos.z};
+                    float dw3[3] = {dwt.x, dwt.y, dwt.z};
+#pragma unroll
+                    for (int r = 0; r < 3; ++r)
+#pragma unroll
+                        for (int c = 0; c < 3; ++c) {
+                            nC.a[3 * r + c] += g[r] * dp[c] * cw;
+                            nF.a[3 * r + c] += g[r] * dw3[c];
+                        }
+                }
+            }
+        }
+        v = nv;
+        Cm = nC;
+        L = nF;
+        x = x + v * dt;
+        if (periodic) {
+            float lx = (float)G.n[0] * G.dx;
+            if (x.x < 0.0f) x.x = x.x + lx;
+            if (x.x >= lx) x.x = x.x - lx;
+        }
+        m33 F = ld9(P.F, S, p);
+        m33 A = mscale(nF, dt);
+        A.a[0] += 1.f; A.a[4] += 1.f; A.a[8] += 1.f;
+        Ft = mmul(A, F);
+        if (md->update_cov_with_F && P.cov) {  // update_cov, mpm_utils.py:613-633
+            float c[6];
+#pragma unroll
+            for (int q = 0; q < 6; ++q) c[q] = P.cov[q * S + p];
+            m33 cn;
+            cn.a[0] = c[0]; cn.a[1] = c[1]; cn.a[2] = c[2];
+            cn.a[3] = c[1]; cn.a[4] = c[3]; cn.a[5] = c[4];
+            cn.a[6] = c[2]; cn.a[7] = c[4]; cn.a[8] = c[5];
+            m33 n1 = madd(cn, mscale(madd(mmul(nF, cn), mmul(cn, mT(nF))), dt));
+            P.cov[p] = n1.a[0]; P.cov[S + p] = n1.a[1]; P.cov[2 * S + p] = n1.a[2];
+            P.cov[3 * S + p] = n1.a[4]; P.cov[4 * S + p] = n1.a[5]; P.cov[5 * S + p] = n1.a[8];
+        }
+        st3(P.x, S, p, x);
+        if (!kLean) {
+            st3(P.v, S, p, v);
+            st9(P.C, S, p, Cm);
+            st9(P.L, S, p, L);
+            st9(P.F_trial, S, p, Ft);
+        }
+    } else {
+        if (kP2G) {
+            v = ld3(P.v, S, p);
+            Cm = ld9(P.C, S, p);
+        }
+        if (kStress) {
+            Ft = ld9(P.F_trial, S, p);
+            if (mat == 10 || mat == 12) L = ld9(P.L, S, p);
+        }
+    }
+
+    m33 stress;
+    if (kStress) {
+        PState ps;
+        ps.mu = P.mu[p];
+        ps.lam = P.lam[p];
+        ps.yield = P.yield_stress[p];
+        ps.Jp = P.Jp[p];
+        ps.dirty_mu = ps.dirty_yield = ps.dirty_Jp = false;
+        m33 F;
+        stress_update(Ft, L, mat, md, dt, ps, F, stress);
+        st9(P.F, S, p, F);
+        if (!kLean || kKeepStress) st9(P.stress, S, p, stress);
+        if (ps.dirty_yield) P.yield_stress[p] = ps.yield;
+        if (ps.dirty_mu) { P.mu[p] = ps.mu; P.lam[p] = ps.lam; }
+        if (ps.dirty_Jp) P.Jp[p] = ps.Jp;
+    } else if (kP2G) {
+        stress = ld9(P.stress, S, p);
+    }
+
+    if (kP2G) {
+        Stencil st = make_stencil(x, G.inv_dx);
+        const float r = md->rpic_damping;
+        if (r != 0.0f) {
+            m33 Ct = mT(Cm);
+            Cm = madd(mscale(Cm, 1.0f - r), mscale(msub(Cm, Ct), r / 2.0f));
+            if (r < -0.001f) Cm = mzero();
+        }
+        const float vol = P.vol[p], mass = P.mass[p];
+        const m33 fs = mscale(stress, -vol * dt);
+        float* __restrict__ gin = G.node_in;
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            int ix = st.base[0] + i;
+            if (periodic) ix = ((ix % G.n[0]) + G.n[0]) % G.n[0];
+#pragma unroll
+            for (int j = 0; j < 3; ++j) {
+                int iy = st.base[1] + j;
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    int iz = st.base[2] + k;
+                    if (!((unsigned)ix < (unsigned)G.n[0] && (unsigned)iy < (unsigned)G.n[1] &&
+                          (unsigned)iz < (unsigned)G.n[2])) {
+                        err = 1;
+                        continue;
+                    }
+                    float weight = st.w[0][i] * st.w[1][j] * st.w[2][k];
+                    v3 dpos = mk3(((float)i - st.fx[0]) * G.dx, ((float)j - st.fx[1]) * G.dx,
+                                  ((float)k - st.fx[2]) * G.dx);
+                    v3 dwt = mk3(st.dw[0][i] * st.w[1][j] * st.w[2][k], st.w[0][i] * st.dw[1][j] * st.w[2][k],
+                                 st.w[0][i] * st.w[1][j] * st.dw[2][k]) * G.inv_dx;
+                    float wm = weight * mass;
+                    v3 add = (v + mvmul(Cm, dpos)) * wm + mvmul(fs, dwt);
+                    size_t o = 4 * (size_t)node_index(G, ix, iy, iz);
+                    atomicAdd(gin + o + 0, add.x);
+                    atomicAdd(gin + o + 1, add.y);
+                    atomicAdd(gin + o + 2, add.z);
+                    atomicAdd(gin + o + 3, wm);
+                }
+            }
+        }
+    }
+    if (!(x.x == x.x && x.y == x.y && x.z == x.z)) err |= 2;
+    if (err && G.error_flag) atomicOr(G.error_flag, err);
+}
+
+// ------------------------------------------------------------------------ colliders
+// capped_cylinder_sdf / revolved_glass_sdf, mpm_solver_warp.py:29-57
+MPM_DEV float capped_cylinder_sdf(float r, float z, float radius, float z0, float z1) {
+    float zc = 0.5f * (z0 + z1), hh = 0.5f * (z1 - z0);
+    float dr = r - radius, dz = fabsf(z - zc) - hh;
+    float dro = fmaxf(dr, 0.0f), dzo = fmaxf(dz, 0.0f);
+    return sqrtf(dro * dro + dzo * dzo) + fminf(fmaxf(dr, dz), 0.0f);
+}
+
+MPM_DEV float glass_sdf(v3 l, const mpm_collider_t& c) {
+    float r = sqrtf(l.x * l.x + l.y * l.y);
+    float d_outer = capped_cylinder_sdf(r, l.z, c.outer_radius, -c.half_height, c.half_height);
+    float d_cav = capped_cylinder_sdf(r, l.z, c.inner_radius - c.fillet_radius,
+                                      c.inner_floor_z + c.fillet_radius, c.half_height + c.outer_radius) -
+                  c.fillet_radius;
+    return fmaxf(d_outer, -d_cav);
+}
+
+// sdf_trilerp(_vec), mpm_solver_warp.py:60-91
+MPM_DEV void trilerp(const float* __restrict__ vals, int nc, int res, v3 f, float* out) {
+    float hi = (float)res - 1.0001f;
+    float fi = fminf(fmaxf(f.x, 0.0f), hi), fj = fminf(fmaxf(f.y, 0.0f), hi), fk = fminf(fmaxf(f.z, 0.0f), hi);
+    int i = (int)floorf(fi), j = (int)floorf(fj), k = (int)floorf(fk);
+    float tx = fi - (float)i, ty = fj - (float)j, tz = fk - (float)k;
+    auto V = [&](int a, int b, int c, int ch) { return vals[(((size_t)a * res + b) * res + c) * nc + ch]; };
+    for (int ch = 0; ch < nc; ++ch) {
+        float c00 = V(i, j, k, ch) * (1.0f - tx) + V(i + 1, j, k, ch) * tx;
+        float c01 = V(i, j, k + 1, ch) * (1.0f - tx) + V(i + 1, j, k + 1, ch) * tx;
+        float c10 = V(i, j + 1, k, ch) * (1.0f - tx) + V(i + 1, j + 1, k, ch) * tx;
+        float c11 = V(i, j + 1, k + 1, ch) * (1.0f - tx) + V(i + 1, j + 1, k + 1, ch) * tx;
+        float c0 = c00 * (1.0f - ty) + c10 * ty;
+        float c1 = c01 * (1.0f - ty) + c11 * ty;
+        out[ch] = c0 * (1.0f - tz) + c1 * tz;
+    }
+}
+
+// v' = v(2w^2 - 1) + 2(q.v) q +/- 2w (q x v): rotation by a unit quaternion (x, y, z, w)
+MPM_DEV v3 quat_rotate(const float* q, v3 v, bool inverse) {
+    v3 qv = mk3(q[0], q[1], q[2]);
+    float w = q[3];
+    return v * (2.0f * w * w - 1.0f) + qv * (2.0f * dot(qv, v)) + cross(qv, v) * ((inverse ? -w : w) * 2.0f);
+}
+
+// Applies collider c to one node; returns the reaction impulse in imp (zero if untouched).
+MPM_DEV void apply_collider(const mpm_collider_t& c, const mpm_grid_t& G, int gi, int gj, int gk,
+                            float m, v3& vo, v3& imp, v3& torque) {
+    imp = mk3(0.f, 0.f, 0.f);
+    torque = mk3(0.f, 0.f, 0.f);
+    const float time = c.time;
+    const bool live = time >= c.start_time && time < c.end_time;
+    const v3 xn = mk3((float)gi * G.dx, (float)gj * G.dx, (float)gk * G.dx);
+    switch (c.kind) {
+        case MPM_COLLIDER_PLANE: {  // mpm_solver_warp.py:1929-1992
+            if (!live) return;
+            v3 n = mk3(c.normal[0], c.normal[1], c.normal[2]);
+            v3 off = mk3((float)gi * G.dx - c.point[0], (float)gj * G.dx - c.point[1], (float)gk * G.dx - c.point[2]);
+            if (!(dot(off, n) < 0.0f)) return;
+            if (c.surface == 0) {
+                vo = mk3(0.f, 0.f, 0.f);
+            } else if (c.surface == 11) {
+                float z = (float)gk * G.dx;
+                if (z < 0.4f || z > 0.53f) vo = mk3(0.f, 0.f, 0.f);
+                else vo = mk3(vo.x, 0.0f, vo.z) * 0.3f;
+            } else {
+                float nc = dot(vo, n);
+                v3 w = (c.surface == 1) ? vo - n * nc : vo - n * fminf(nc, 0.0f);
+                if (nc < 0.0f && len(w) > 1e-20f) w = normalize(w) * fmaxf(0.0f, len(w) + nc * c.friction);
+                vo = w;
+            }
+        } break;
+        case MPM_COLLIDER_BOX: {  // mpm_solver_warp.py:2053-2088
+            v3 vb = mk3(c.velocity[0], c.velocity[1], c.velocity[2]);
+            if (live) {
+                v3 off = mk3((float)gi * G.dx - c.point[0], (float)gj * G.dx - c.point[1], (float)gk * G.dx - c.point[2]);
+                if (fabsf(off.x) < c.size[0] && fabsf(off.y) < c.size[1] && fabsf(off.z) < c.size[2]) {
+                    imp = (vo - vb) * m;
+                    vo = vb;
+                }
+            } else if (c.reset == 1) {
+                if (time < c.end_time + 15.0f * c.dt) vo = mk3(0.f, 0.f, 0.f);
+            }
+        } break;
+        case MPM_COLLIDER_CUP: {  // mpm_solver_warp.py:2161-2225
+            if (!live) return;
+            m33 R;
+#pragma unroll
+            for (int q = 0; q < 9; ++q) R.a[q] = c.rot[q];
+            v3 rel = xn - mk3(c.point[0], c.point[1], c.point[2]);
+            v3 loc = mTvmul(R, rel);
+            float sdf = glass_sdf(loc, c);
+            if (!(sdf < c.contact_band)) return;
+            v3 vs = mk3(c.velocity[0], c.velocity[1], c.velocity[2]) + cross(mk3(c.omega[0], c.omega[1], c.omega[2]), rel);
+            v3 vin = vo;
+            v3 vnew = vs;
+            if (sdf > -c.sticky_depth) {
+                vnew = vin;
+                float h = 0.25f * G.dx;
+                v3 nl = mk3(glass_sdf(loc + mk3(h, 0, 0), c) - glass_sdf(loc - mk3(h, 0, 0), c),
+                            glass_sdf(loc + mk3(0, h, 0), c) - glass_sdf(loc - mk3(0, h, 0), c),
+                            glass_sdf(loc + mk3(0, 0, h), c) - glass_sdf(loc - mk3(0, 0, h), c));
+                float nlen = len(nl);
+                if (nlen > 1.0e-12f) {
+                    v3 n = mvmul(R, vdiv(nl, nlen));
+                    v3 vr = vin - vs;
+                    float vn = dot(vr, n);
+                    if (vn < 0.0f) {
+                        v3 vt = vr - n * vn;
+                        float vtl = len(vt);
+                        if (vtl > 1.0e-20f) vt = vdiv(vt, vtl) * fmaxf(vtl + c.friction * vn, 0.0f);
+                        vnew = vs + vt;
+                    }
+                }
+            }
+            imp = (vin - vnew) * m;
+            torque = cross(rel, imp);
+            vo = vnew;
+        } break;
+        case MPM_COLLIDER_WALLS: {  // mpm_solver_warp.py:2294-2359
+            if (!live) return;
+            const int pad = 3;
+            if (gi < pad && vo.x < 0) vo.x = 0.0f;
+            if (gi >= G.n[0] - pad && vo.x > 0) vo.x = 0.0f;
+            if (gj < pad && vo.y < 0) vo.y = 0.0f;
+            if (gj >= G.n[1] - pad && vo.y > 0) vo.y = 0.0f;
+            if (gk < pad && vo.z < 0) vo.z = 0.0f;
+            if (gk >= G.n[2] - pad && vo.z > 0) vo.z = 0.0f;
+        } break;
+        case MPM_COLLIDER_SDF: {  // mpm_solver_warp.py:2680-2736
+            if (!live) return;
+            v3 rel = xn - mk3(c.point[0], c.point[1], c.point[2]);
+            v3 body = quat_rotate(c.quat, rel, true);
+            v3 f = mk3((body.x - c.origin[0]) / c.cell, (body.y - c.origin[1]) / c.cell, (body.z - c.origin[2]) / c.cell);
+            float rf = (float)c.res - 1.0f;
+            if (!(f.x >= 0.0f && f.y >= 0.0f && f.z >= 0.0f && f.x <= rf && f.y <= rf && f.z <= rf)) return;
+            float sd;
+            trilerp(c.sdf_val, 1, c.res, f, &sd);
+            if (!(sd <= c.band)) return;
+            float g[3];
+            trilerp(c.sdf_grad, 3, c.res, f, g);
+            v3 gr = mk3(g[0], g[1], g[2]);
+            if (!(len(gr) > 1.0e-12f)) return;
+            v3 nw = quat_rotate(c.quat, normalize(gr), false);
+            v3 vsurf = mk3(c.velocity[0], c.velocity[1], c.velocity[2]) + cross(mk3(c.omega[0], c.omega[1], c.omega[2]), rel);
+            v3 vf = vo;
+            v3 vr = vf - vsurf;
+            float vn = dot(vr, nw);
+            v3 vnew = vf;
+            if (c.surface == 0) vnew = vsurf;
+            else if (c.surface == 1) vnew = vsurf + (vr - nw * vn);
+            else if (vn < 0.0f) {
+                v3 vt = vr - nw * vn;
+                float tl = len(vt);
+                if (tl > 1.0e-12f) vt = vt * (fmaxf(0.0f, tl + c.friction * vn) / tl);
+                vnew = vsurf + vt;
+            }
+            imp = (vf - vnew) * m;
+            torque = cross(rel, imp);
+            vo = vnew;
+        } break;
+        case MPM_COLLIDER_POINTCLOUD: {  // mpm_solver_warp.py:2586-2605
+            if (!live) return;
+            if (c.occupancy[((size_t)gi * G.n[1] + gj) * G.n[2] + gk] == 1) vo = mk3(0.f, 0.f, 0.f);
+        } break;
+        default: break;
+    }
+}
+
+MPM_DEV float wave_sum(float v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// One substep's grid update over listed blocks (one 64-node block per wavefront).
+__global__ __launch_bounds__(kThreads) void k_grid_update(mpm_grid_t G, const mpm_model_t* __restrict__ md,
+                                                          const mpm_tick_ctl_t* __restrict__ ctl,
+                                                          const int32_t* __restrict__ list,
+                                                          const int32_t* __restrict__ count,
+                                                          const mpm_collider_t* __restrict__ col, int ncoll,
+                                                          float* __restrict__ wrench, float dt, int flags) {
+    const int nblk = *count;
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int waves_total = gridDim.x * (kThreads / 64);
+    const float g0 = md->gravity[0], g1 = md->gravity[1], g2 = md->gravity[2];
+    const float damp = md->grid_v_damping_scale;
+    const int blo0 = ctl->box_lo[0], blo1 = ctl->box_lo[1], blo2 = ctl->box_lo[2];
+    const int bhi0 = ctl->box_hi[0], bhi1 = ctl->box_hi[1], bhi2 = ctl->box_hi[2];
+    for (int b = blockIdx.x * (kThreads / 64) + wave; b < nblk; b += waves_total) {
+        const int code = list[b];
+        int bi, bj, bk;
+        decode_block(G, code, bi, bj, bk);
+        const int gi = bi * 4 + (lane >> 4), gj = bj * 4 + ((lane >> 2) & 3), gk = bk * 4 + (lane & 3);
+        const size_t o = 4 * ((size_t)code * 64 + lane);
+        const bool valid = gi < G.n[0] && gj < G.n[1] && gk < G.n[2];
+        const bool inbox = valid && gi >= blo0 && gi < bhi0 && gj >= blo1 && gj < bhi1 && gk >= blo2 && gk < bhi2;
+        float4 in4 = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (valid) in4 = *reinterpret_cast<const float4*>(G.node_in + o);
+        const float m = in4.w;
+        v3 vo = mk3(0.f, 0.f, 0.f);
+        if (m > 1e-15f) {
+            float inv = 1.0f / m;
+            vo = mk3(in4.x * inv + dt * g0, in4.y * inv + dt * g1, in4.z * inv + dt * g2);
+        }
+        if (damp < 1.0f) vo = vo * damp;
+        for (int q = 0; q < ncoll; ++q) {
+            const mpm_collider_t& c = col[q];
+            v3 imp = mk3(0.f, 0.f, 0.f), tq = mk3(0.f, 0.f, 0.f);
+            const bool act = inbox && c.active && m > 0.0f && gi >= c.lo[0] && gi < c.hi[0] && gj >= c.lo[1] &&
+                             gj < c.hi[1] && gk >= c.lo[2] && gk < c.hi[2];
+            if (act) apply_collider(c, G, gi, gj, gk, m, vo, imp, tq);
+            if (c.slot >= 0 && __any(act)) {
+                float s0 = wave_sum(imp.x), s1 = wave_sum(imp.y), s2 = wave_sum(imp.z);
+                float s3 = wave_sum(tq.x), s4 = wave_sum(tq.y), s5 = wave_sum(tq.z);
+                if (lane == 0) {
+                    float* w = wrench + 6 * c.slot;
+                    atomicAdd(w + 0, s0); atomicAdd(w + 1, s1); atomicAdd(w + 2, s2);
+                    atomicAdd(w + 3, s3); atomicAdd(w + 4, s4); atomicAdd(w + 5, s5);
+                }
+            }
+        }
+        if (inbox) *reinterpret_cast<float4*>(G.node_out + o) = make_float4(vo.x, vo.y, vo.z, 0.0f);
+        if (valid && (flags & MPM_GRID_ZERO_IN)) *reinterpret_cast<float4*>(G.node_in + o) = make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+}
+
+__global__ void k_grid_zero_box(mpm_grid_t G, int lo0, int lo1, int lo2, int d0, int d1, int d2, int which) {
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    long total = (long)d0 * d1 * d2;
+    if (t >= total) return;
+    int k = lo2 + (int)(t % d2);
+    long r = t / d2;
+    int j = lo1 + (int)(r % d1);
+    int i = lo0 + (int)(r / d1);
+    size_t o = 4 * (size_t)node_index(G, i, j, k);
+    const float4 z = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (which & 1) *reinterpret_cast<float4*>(G.node_in + o) = z;
+    if (which & 2) *reinterpret_cast<float4*>(G.node_out + o) = z;
+}
+
+__global__ void k_grid_zero_blocks(mpm_grid_t G, const int32_t* list, const int32_t* count, int which) {
+    const int nblk = *count;
+    for (int b = blockIdx.x * (kThreads / 64) + (threadIdx.x >> 6); b < nblk; b += gridDim.x * (kThreads / 64)) {
+        size_t o = 4 * ((size_t)list[b] * 64 + (threadIdx.x & 63));
+        const float4 z = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (which & 1) *reinterpret_cast<float4*>(G.node_in + o) = z;
+        if (which & 2) *reinterpret_cast<float4*>(G.node_out + o) = z;
+    }
+}
+
+__global__ void k_blocks_from_box(mpm_grid_t G, mpm_tick_ctl_t* ctl, int32_t* list) {
+    const int bl0 = ctl->box_lo[0] >> 2, bl1 = ctl->box_lo[1] >> 2, bl2 = ctl->box_lo[2] >> 2;
+    const int bh0 = (ctl->box_hi[0] + 3) >> 2, bh1 = (ctl->box_hi[1] + 3) >> 2, bh2 = (ctl->box_hi[2] + 3) >> 2;
+    const int d0 = max(bh0 - bl0, 0), d1 = max(bh1 - bl1, 0), d2 = max(bh2 - bl2, 0);
+    const int total = d0 * d1 * d2;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t == 0) ctl->nblocks = total;
+    if (t >= total) return;
+    int bk = bl2 + t % d2;
+    int r = t / d2;
+    int bj = bl1 + r % d1;
+    int bi = bl0 + r / d1;
+    list[t] = (bi * G.nb[1] + bj) * G.nb[2] + bk;
+}
+
+// ------------------------------------------------------------------ sparse blocks
+// blocks_mark, mpm_utils.py:1506-1513 (int() truncation toward zero, >> 2, clamp)
+__global__ void k_blocks_mark(mpm_particles_t P, mpm_grid_t G, int32_t* raw) {
+    int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= P.n) return;
+    int b[3];
+    for (int a = 0; a < 3; ++a) {
+        int t = (int)(P.x[a * P.stride + p] * G.inv_dx - 0.5f);
+        t >>= 2;
+        b[a] = min(max(t, 0), G.nb[a] - 1);
+    }
+    raw[(b[0] * G.nb[1] + b[1]) * G.nb[2] + b[2]] = 1;
+}
+
+// blocks_dilate, mpm_utils.py:1516-1529
+__global__ void k_blocks_dilate(mpm_grid_t G, const int32_t* raw, int32_t* cur) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    int total = G.nb[0] * G.nb[1] * G.nb[2];
+    if (t >= total) return;
+    int z = t % G.nb[2], r = t / G.nb[2], y = r % G.nb[1], x = r / G.nb[1];
+    int on = 0;
+    for (int i = -1; i <= 1; ++i)
+        for (int j = -1; j <= 1; ++j)
+            for (int k = -1; k <= 1; ++k) {
+                int xi = min(max(x + i, 0), G.nb[0] - 1), yj = min(max(y + j, 0), G.nb[1] - 1),
+                    zk = min(max(z + k, 0), G.nb[2] - 1);
+                if (raw[(xi * G.nb[1] + yj) * G.nb[2] + zk] == 1) on = 1;
+            }
+    cur[t] = on;
+}
+
+// blocks_compact, mpm_utils.py:1532-1544
+__global__ void k_blocks_compact(mpm_grid_t G, const int32_t* cur, const int32_t* prev, int32_t* cur_list,
+                                 int32_t* union_list, int32_t* counts) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    int total = G.nb[0] * G.nb[1] * G.nb[2];
+    if (t >= total) return;
+    int c = cur[t];
+    if (c == 1) cur_list[atomicAdd(counts, 1)] = t;
+    if (c == 1 || prev[t] == 1) union_list[atomicAdd(counts + 1, 1)] = t;
+}
+
+// ---------------------------------------------------------------------- bounds
+MPM_DEV int ord_key(float f) {
+    int i = __float_as_int(f);
+    return i >= 0 ? i : i ^ 0x7fffffff;
+}
+MPM_DEV float ord_val(int k) { return __int_as_float(k >= 0 ? k : k ^ 0x7fffffff); }
+
+__global__ void k_bounds_init(int* out) {
+    int t = threadIdx.x;
+    if (t < 3) out[t] = 0x7fffffff;                          // min keys start at +max
+    else if (t < 6) out[t] = (int)0x80000000;                // max keys start at -max
+    else if (t == 6) out[t] = 0;                             // max |v| key (>= 0)
+    else if (t == 7) out[t] = 0;                             // non-finite count
+}
+
+__global__ __launch_bounds__(kThreads) void k_bounds(mpm_particles_t P, int* out) {
+    __shared__ int sh[8][kThreads / 64];
+    int p = blockIdx.x * blockDim.x + threadIdx.x;
+    int kmin[3] = {0x7fffffff, 0x7fffffff, 0x7fffffff};
+    int kmax[3] = {(int)0x80000000, (int)0x80000000, (int)0x80000000};
+    int kv = 0, bad = 0;
+    if (p < P.n) {
+        for (int a = 0; a < 3; ++a) {
+            float x = P.x[a * P.stride + p];
+            if (!isfinite(x)) bad = 1;
+            int k = ord_key(x);
+            kmin[a] = k; kmax[a] = k;
+            float av = fabsf(P.v[a * P.stride + p]);
+            kv = max(kv, __float_as_int(av));
+        }
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        for (int a = 0; a < 3; ++a) {
+            kmin[a] = min(kmin[a], __shfl_xor(kmin[a], off, 64));
+            kmax[a] = max(kmax[a], __shfl_xor(kmax[a], off, 64));
+        }
+        kv = max(kv, __shfl_xor(kv, off, 64));
+        bad += __shfl_xor(bad, off, 64);
+    }
+    int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    if (lane == 0) {
+        for (int a = 0; a < 3; ++a) { sh[a][wv] = kmin[a]; sh[3 + a][wv] = kmax[a]; }
+        sh[6][wv] = kv; sh[7][wv] = bad;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < kThreads / 64; ++w) {
+            for (int a = 0; a < 3; ++a) { sh[a][0] = min(sh[a][0], sh[a][w]); sh[3 + a][0] = max(sh[3 + a][0], sh[3 + a][w]); }
+            sh[6][0] = max(sh[6][0], sh[6][w]); sh[7][0] += sh[7][w];
+        }
+        for (int a = 0; a < 3; ++a) { atomicMin(out + a, sh[a][0]); atomicMax(out + 3 + a, sh[3 + a][0]); }
+        atomicMax(out + 6, sh[6][0]);
+        if (sh[7][0]) atomicAdd(out + 7, sh[7][0]);
+    }
+}
+
+__global__ void k_bounds_finish(int* out) {
+    int t = threadIdx.x;
+    float* f = reinterpret_cast<float*>(out);
+    if (t < 6) f[t] = ord_val(out[t]);
+    else if (t == 6) f[t] = __int_as_float(out[t]);
+    else if (t == 7) f[t] = (float)out[t];
+}
+
+// ------------------------------------------------------------------------ sorting
+// key of solver.py:458-461: base = floor(x / dx - 0.5) >> 2, nb = n/4 + 2
+__global__ void k_sort_keys(mpm_particles_t P, mpm_grid_t G, int32_t* keys, int32_t* vals, int32_t* unsorted) {
+    int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= P.n) return;
+    const int nb = (G.n[0] >> 2) + 2;
+    auto key_of = [&](int q) {
+        int b[3];
+        for (int a = 0; a < 3; ++a) b[a] = ((int)floorf(P.x[a * P.stride + q] / G.dx - 0.5f)) >> 2;
+        return (b[0] * nb + b[1]) * nb + b[2];
+    };
+    int k = key_of(p);
+    keys[p] = k;
+    vals[p] = p;
+    if (p + 1 < P.n && key_of(p + 1) < k) atomicOr(unsorted, 1);
+}
+
+template <typename T>
+__global__ void k_gather(const T* __restrict__ src, const int32_t* __restrict__ perm, T* __restrict__ dst, int n,
+                         int stride, int planes) {
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long)n * planes) return;
+    int q = (int)(t / n), i = (int)(t % n);
+    dst[(size_t)q * stride + i] = src[(size_t)q * stride + perm[i]];
+}
+
+// ------------------------------------------------------------------------ exports
+__global__ void k_export_R(mpm_particles_t P, float* R) {
+    int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= P.n) return;
+    m33 F = ld9(P.F_trial, P.stride, p), U, V;
+    v3 s;
+    svd3(F, U, s, V);
+    if (mdet(U) < 0.0f) { U.a[2] = -U.a[2]; U.a[5] = -U.a[5]; U.a[8] = -U.a[8]; }
+    if (mdet(V) < 0.0f) { V.a[2] = -V.a[2]; V.a[5] = -V.a[5]; V.a[8] = -V.a[8]; }
+    m33 Rt = mT(mmul(U, mT(V)));
+    for (int i = 0; i < 9; ++i) R[9 * (size_t)p + i] = Rt.a[i];
+}
+
+__global__ void k_export_cov(mpm_particles_t P, const float* init_cov, float* cov) {
+    int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= P.n) return;
+    m33 F = ld9(P.F_trial, P.stride, p);
+    const float* c = init_cov + 6 * (size_t)p;
+    m33 ic;
+    ic.a[0] = c[0]; ic.a[1] = c[1]; ic.a[2] = c[2];
+    ic.a[3] = c[1]; ic.a[4] = c[3]; ic.a[5] = c[4];
+    ic.a[6] = c[2]; ic.a[7] = c[4]; ic.a[8] = c[5];
+    m33 r = mmul(mmul(F, ic), mT(F));
+    float* o = cov + 6 * (size_t)p;
+    o[0] = r.a[0]; o[1] = r.a[1]; o[2] = r.a[2]; o[3] = r.a[4]; o[4] = r.a[5]; o[5] = r.a[8];
+}
+
+__global__ void k_grid_to_dense(mpm_grid_t G, float* m, float* vin, float* vout) {
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    long total = (long)G.n[0] * G.n[1] * G.n[2];
+    if (t >= total) return;
+    int k = (int)(t % G.n[2]);
+    long r = t / G.n[2];
+    int j = (int)(r % G.n[1]);
+    int i = (int)(r / G.n[1]);
+    size_t o = 4 * (size_t)node_index(G, i, j, k);
+    if (m) m[t] = G.node_in[o + 3];
+    if (vin) { vin[3 * t] = G.node_in[o]; vin[3 * t + 1] = G.node_in[o + 1]; vin[3 * t + 2] = G.node_in[o + 2]; }
+    if (vout) { vout[3 * t] = G.node_out[o]; vout[3 * t + 1] = G.node_out[o + 1]; vout[3 * t + 2] = G.node_out[o + 2]; }
+}
+
+inline unsigned blocks_for(long n, int threads = kThreads) { return (unsigned)((n + threads - 1) / threads); }
+
+inline int32_t launch_status() { return hipGetLastError() == hipSuccess ? MPM_OK : MPM_ERR_HIP; }
+
+inline int grid_ok(const mpm_grid_t* G) {
+    return G && G->node_in && G->node_out && G->n[0] > 0 && G->n[1] > 0 && G->n[2] > 0 &&
+           G->nb[0] == (G->n[0] + 3) / 4 && G->nb[1] == (G->n[1] + 3) / 4 && G->nb[2] == (G->n[2] + 3) / 4;
+}
+
+inline int particles_ok(const mpm_particles_t* P) {
+    return P && P->n >= 0 && P->stride >= P->n && P->x && P->v && P->F && P->F_trial && P->C && P->stress &&
+           P->L && P->vol && P->mass && P->mu && P->lam && P->yield_stress && P->Jp && P->material && P->selection;
+}
+
+}  // namespace
+
+// ======================================================================== C ABI
+extern "C" {
+
+int32_t mpm_abi_version(void) { return MPM_ABI_VERSION; }
+
+const char* mpm_status_string(int32_t s) {
+    switch (s) {
+        case MPM_OK: return "ok";
+        case MPM_ERR_ARG: return "invalid argument";
+        case MPM_ERR_HIP: return "HIP runtime error";
+        case MPM_ERR_SIZE: return "size beyond the documented maximum";
+        default: return "unknown status";
+    }
+}
+
+int32_t mpm_struct_size(int32_t which) {
+    switch (which) {
+        case 0: return (int32_t)sizeof(mpm_particles_t);
+        case 1: return (int32_t)sizeof(mpm_grid_t);
+        case 2: return (int32_t)sizeof(mpm_model_t);
+        case 3: return (int32_t)sizeof(mpm_collider_t);
+        case 4: return (int32_t)sizeof(mpm_tick_ctl_t);
+        default: return -1;
+    }
+}
+
+int32_t mpm_particle_pass(const mpm_particles_t* P, const mpm_grid_t* G, const mpm_model_t* model, float dt,
+                          int32_t mode, void* stream) {
+    if (!particles_ok(P) || !grid_ok(G) || !model) return MPM_ERR_ARG;
+    if (P->n == 0) return MPM_OK;
+    hipStream_t s = (hipStream_t)stream;
+    dim3 grid(blocks_for(P->n)), block(kThreads);
+    const int m = mode & (MPM_PASS_G2P | MPM_PASS_STRESS | MPM_PASS_P2G | MPM_PASS_LEAN | MPM_PASS_KEEP_STRESS);
+    switch (m) {
+#define CASE(M) case (M): hipLaunchKernelGGL(k_particle_pass<(M)>, grid, block, 0, s, *P, *G, model, dt); break;
+        CASE(MPM_PASS_G2P)
+        CASE(MPM_PASS_STRESS)
+        CASE(MPM_PASS_P2G)
+        CASE(MPM_PASS_STRESS | MPM_PASS_P2G)
+        CASE(MPM_PASS_G2P | MPM_PASS_STRESS | MPM_PASS_P2G)
+        CASE(MPM_PASS_G2P | MPM_PASS_STRESS | MPM_PASS_P2G | MPM_PASS_LEAN)
+        CASE(MPM_PASS_G2P | MPM_PASS_STRESS | MPM_PASS_P2G | MPM_PASS_LEAN | MPM_PASS_KEEP_STRESS)
+#undef CASE
+        default: return MPM_ERR_ARG;
+    }
+    return launch_status();
+}
+
+int32_t mpm_grid_zero_box(const mpm_grid_t* G, const int32_t* lo, const int32_t* hi, int32_t which, void* stream) {
+    if (!grid_ok(G) || !lo || !hi) return MPM_ERR_ARG;
+    int l[3], d[3];
+    for (int a = 0; a < 3; ++a) {
+        l[a] = lo[a] < 0 ? 0 : lo[a];
+        int h = hi[a] > G->n[a] ? G->n[a] : hi[a];
+        d[a] = h - l[a];
+        if (d[a] <= 0) return MPM_OK;
+    }
+    long total = (long)d[0] * d[1] * d[2];
+    hipLaunchKernelGGL(k_grid_zero_box, dim3(blocks_for(total)), dim3(kThreads), 0, (hipStream_t)stream, *G, l[0],
+                       l[1], l[2], d[0], d[1], d[2], which);
+    return launch_status();
+}
+
+int32_t mpm_grid_zero_blocks(const mpm_grid_t* G, const int32_t* list, const int32_t* count, int32_t capacity,
+                             int32_t which, void* stream) {
+    if (!grid_ok(G) || !list || !count || capacity < 0) return MPM_ERR_ARG;
+    if (capacity == 0) return MPM_OK;
+    unsigned nwg = (unsigned)min((capacity + 3) / 4, 8192);
+    hipLaunchKernelGGL(k_grid_zero_blocks, dim3(nwg), dim3(kThreads), 0, (hipStream_t)stream, *G, list, count, which);
+    return launch_status();
+}
+
+int32_t mpm_blocks_from_box(const mpm_grid_t* G, mpm_tick_ctl_t* ctl, int32_t* list, void* stream) {
+    if (!grid_ok(G) || !ctl || !list) return MPM_ERR_ARG;
+    long total = (long)G->nb[0] * G->nb[1] * G->nb[2];
+    hipLaunchKernelGGL(k_blocks_from_box, dim3(blocks_for(total)), dim3(kThreads), 0, (hipStream_t)stream, *G, ctl,
+                       list);
+    return launch_status();
+}
+
+int32_t mpm_grid_update(const mpm_grid_t* G, const mpm_model_t* model, const mpm_tick_ctl_t* ctl,
+                        const int32_t* list, const int32_t* count, int32_t capacity, const mpm_collider_t* colliders,
+                        int32_t ncoll, float* wrench, float dt, int32_t flags, void* stream) {
+    if (!grid_ok(G) || !model || !ctl || !list || !count || capacity < 0 || ncoll < 0) return MPM_ERR_ARG;
+    if (ncoll > 0 && !colliders) return MPM_ERR_ARG;
+    if (capacity == 0) return MPM_OK;
+    unsigned nwg = (unsigned)min((capacity + 3) / 4, 8192);
+    hipLaunchKernelGGL(k_grid_update, dim3(nwg), dim3(kThreads), 0, (hipStream_t)stream, *G, model, ctl, list, count,
+                       colliders, ncoll, wrench, dt, flags);
+    return launch_status();
+}
+
+int32_t mpm_blocks_mark(const mpm_particles_t* P, const mpm_grid_t* G, int32_t* raw, void* stream) {
+    if (!particles_ok(P) || !grid_ok(G) || !raw) return MPM_ERR_ARG;
+    if (P->n == 0) return MPM_OK;
+    hipLaunchKernelGGL(k_blocks_mark, dim3(blocks_for(P->n)), dim3(kThreads), 0, (hipStream_t)stream, *P, *G, raw);
+    return launch_status();
+}
+
+int32_t mpm_blocks_dilate(const mpm_grid_t* G, const int32_t* raw, int32_t* cur, void* stream) {
+    if (!grid_ok(G) || !raw || !cur) return MPM_ERR_ARG;
+    long total = (long)G->nb[0] * G->nb[1] * G->nb[2];
+    hipLaunchKernelGGL(k_blocks_dilate, dim3(blocks_for(total)), dim3(kThreads), 0, (hipStream_t)stream, *G, raw, cur);
+    return launch_status();
+}
+
+int32_t mpm_blocks_compact(const mpm_grid_t* G, const int32_t* cur, const int32_t* prev, int32_t* cur_list,
+                           int32_t* union_list, int32_t* counts, void* stream) {
+    if (!grid_ok(G) || !cur || !prev || !cur_list || !union_list || !counts) return MPM_ERR_ARG;
+    long total = (long)G->nb[0] * G->nb[1] * G->nb[2];
+    hipLaunchKernelGGL(k_blocks_compact, dim3(blocks_for(total)), dim3(kThreads), 0, (hipStream_t)stream, *G, cur,
+                       prev, cur_list, union_list, counts);
+    return launch_status();
+}
+
+int32_t mpm_bounds(const mpm_particles_t* P, float* out, void* stream) {
+    if (!particles_ok(P) || !out) return MPM_ERR_ARG;
+    hipStream_t s = (hipStream_t)stream;
+    int* o = reinterpret_cast<int*>(out);
+    hipLaunchKernelGGL(k_bounds_init, dim3(1), dim3(64), 0, s, o);
+    if (P->n > 0) hipLaunchKernelGGL(k_bounds, dim3(blocks_for(P->n)), dim3(kThreads), 0, s, *P, o);
+    hipLaunchKernelGGL(k_bounds_finish, dim3(1), dim3(64), 0, s, o);
+    return launch_status();
+}
+
+int32_t mpm_sort_keys(const mpm_particles_t* P, const mpm_grid_t* G, int32_t* keys, int32_t* vals, int32_t* unsorted,
+                      void* stream) {
+    if (!particles_ok(P) || !grid_ok(G) || !keys || !vals || !unsorted) return MPM_ERR_ARG;
+    if (P->n == 0) return MPM_OK;
+    hipLaunchKernelGGL(k_sort_keys, dim3(blocks_for(P->n)), dim3(kThreads), 0, (hipStream_t)stream, *P, *G, keys, vals,
+                       unsorted);
+    return launch_status();
+}
+
+int32_t mpm_permute(const mpm_particles_t* P, const int32_t* perm, float* scratch, void* stream) {
+    if (!particles_ok(P) || !perm || !scratch) return MPM_ERR_ARG;
+    if (P->n == 0) return MPM_OK;
+    hipStream_t s = (hipStream_t)stream;
+    const int n = P->n, S = P->stride;
+    auto one = [&](float* a, int planes) -> int {
+        hipLaunchKernelGGL(k_gather<float>, dim3(blocks_for((long)n * planes)), dim3(kThreads), 0, s, a, perm, scratch,
+                           n, S, planes);
+        for (int q = 0; q < planes; ++q)
+            if (hipMemcpyAsync(a + (size_t)q * S, scratch + (size_t)q * S, sizeof(float) * n, hipMemcpyDeviceToDevice,
+                               s) != hipSuccess)
+                return MPM_ERR_HIP;
+        return MPM_OK;
+    };
+    auto onei = [&](int32_t* a) -> int {
+        int32_t* sc = reinterpret_cast<int32_t*>(scratch);
+        hipLaunchKernelGGL(k_gather<int32_t>, dim3(blocks_for(n)), dim3(kThreads), 0, s, a, perm, sc, n, S, 1);
+        return hipMemcpyAsync(a, sc, sizeof(int32_t) * n, hipMemcpyDeviceToDevice, s) == hipSuccess ? MPM_OK
+                                                                                                    : MPM_ERR_HIP;
+    };
+    int r = MPM_OK;
+    r |= one(P->x, 3); r |= one(P->v, 3);
+    r |= one(P->F, 9); r |= one(P->F_trial, 9); r |= one(P->C, 9); r |= one(P->stress, 9); r |= one(P->L, 9);
+    r |= one(P->vol, 1); r |= one(P->mass, 1); r |= one(P->mu, 1); r |= one(P->lam, 1);
+    r |= one(P->yield_stress, 1); r |= one(P->Jp, 1);
+    if (P->cov) r |= one(P->cov, 6);
+    r |= onei(P->material); r |= onei(P->selection);
+    if (r != MPM_OK) return MPM_ERR_HIP;
+    return launch_status();
+}
+
+int32_t mpm_export_R(const mpm_particles_t* P, float* R_out, void* stream) {
+    if (!particles_ok(P) || !R_out) return MPM_ERR_ARG;
+    if (P->n == 0) return MPM_OK;
+    hipLaunchKernelGGL(k_export_R, dim3(blocks_for(P->n)), dim3(kThreads), 0, (hipStream_t)stream, *P, R_out);
+    return launch_status();
+}
+
+int32_t mpm_export_cov(const mpm_particles_t* P, const float* init_cov, float* cov_out, void* stream) {
+    if (!particles_ok(P) || !init_cov || !cov_out) return MPM_ERR_ARG;
+    if (P->n == 0) return MPM_OK;
+    hipLaunchKernelGGL(k_export_cov, dim3(blocks_for(P->n)), dim3(kThreads), 0, (hipStream_t)stream, *P, init_cov,
+                       cov_out);
+    return launch_status();
+}
+
+int32_t mpm_grid_to_dense(const mpm_grid_t* G, float* m, float* v_in, float* v_out, void* stream) {
+    if (!grid_ok(G)) return MPM_ERR_ARG;
+    long total = (long)G->n[0] * G->n[1] * G->n[2];
+    hipLaunchKernelGGL(k_grid_to_dense, dim3(blocks_for(total)), dim3(kThreads), 0, (hipStream_t)stream, *G, m, v_in,
+                       v_out);
+    return launch_status();
+}
+
+}  // extern "C"

@@ -1,0 +1,134 @@
+"""ctypes binding of libmpm_hip.so (the C ABI declared in include/mpm_hip.h).
+
+The library is built in-tree by mpm-engine_amd/build.py. Loading fails loudly when it is
+missing: there is no fallback path, the HIP kernels are the only implementation.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_PKG_ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+LIB_PATH = os.path.join(_PKG_ROOT, "lib", "libmpm_hip.so")
+
+MAX_MATERIALS = 16
+
+# mode / flag bits, mirrored from include/mpm_hip.h
+PASS_G2P, PASS_STRESS, PASS_P2G, PASS_LEAN, PASS_KEEP_STRESS = 1, 2, 4, 8, 16
+GRID_ZERO_IN = 1
+COLLIDER_PLANE, COLLIDER_BOX, COLLIDER_CUP, COLLIDER_WALLS, COLLIDER_SDF, COLLIDER_POINTCLOUD = \
+    1, 2, 3, 4, 5, 6
+
+_FP = C.c_void_p
+
+
+class Particles(C.Structure):
+    _fields_ = [("n", C.c_int32), ("stride", C.c_int32),
+                ("x", _FP), ("v", _FP), ("F", _FP), ("F_trial", _FP), ("C", _FP),
+                ("stress", _FP), ("L", _FP), ("vol", _FP), ("mass", _FP), ("mu", _FP),
+                ("lam", _FP), ("yield_stress", _FP), ("Jp", _FP), ("material", _FP),
+                ("selection", _FP), ("cov", _FP)]
+
+
+class Grid(C.Structure):
+    _fields_ = [("node_in", _FP), ("node_out", _FP), ("n", C.c_int32 * 3),
+                ("nb", C.c_int32 * 3), ("dx", C.c_float), ("inv_dx", C.c_float),
+                ("error_flag", _FP)]
+
+
+_M16 = (np.float32, (MAX_MATERIALS,))
+MODEL_DTYPE = np.dtype([
+    ("gravity", np.float32, (3,)), ("rpic_damping", np.float32),
+    ("grid_v_damping_scale", np.float32), ("update_cov_with_F", np.int32),
+    ("periodic_x", np.int32), ("eta_table_n", np.int32), ("eta_table_smin", np.float32),
+    ("eta_table_smax", np.float32), ("eta_table", np.uint64),
+    ("E",) + _M16, ("nu",) + _M16, ("bulk",) + _M16, ("hardening",) + _M16, ("xi",) + _M16,
+    ("plastic_viscosity",) + _M16, ("softening",) + _M16, ("alpha",) + _M16,
+    ("muI_mu_s",) + _M16, ("muI_delta_mu",) + _M16, ("muI_I0",) + _M16, ("muI_d",) + _M16,
+    ("muI_rho_s",) + _M16, ("muI_phi_init",) + _M16, ("muI_phi_chi",) + _M16,
+], align=True)
+
+COLLIDER_DTYPE = np.dtype([
+    ("kind", np.int32), ("surface", np.int32), ("reset", np.int32), ("slot", np.int32),
+    ("lo", np.int32, (3,)), ("hi", np.int32, (3,)), ("active", np.int32), ("res", np.int32),
+    ("time", np.float32), ("dt", np.float32), ("start_time", np.float32),
+    ("end_time", np.float32), ("point", np.float32, (3,)), ("normal", np.float32, (3,)),
+    ("size", np.float32, (3,)), ("velocity", np.float32, (3,)), ("omega", np.float32, (3,)),
+    ("rot", np.float32, (9,)), ("quat", np.float32, (4,)), ("friction", np.float32),
+    ("sticky_depth", np.float32), ("contact_band", np.float32), ("band", np.float32),
+    ("outer_radius", np.float32), ("inner_radius", np.float32), ("half_height", np.float32),
+    ("inner_floor_z", np.float32), ("fillet_radius", np.float32),
+    ("origin", np.float32, (3,)), ("cell", np.float32),
+    ("sdf_val", np.uint64), ("sdf_grad", np.uint64), ("occupancy", np.uint64),
+], align=True)
+
+TICK_CTL_DTYPE = np.dtype([("box_lo", np.int32, (3,)), ("box_hi", np.int32, (3,)),
+                           ("nblocks", np.int32), ("pad", np.int32)])
+
+# every entry point declared in include/mpm_hip.h: name -> (restype, argtypes)
+_P, _I32, _I64, _F = C.c_void_p, C.c_int32, C.c_int64, C.c_float
+SIGNATURES = {
+    "mpm_abi_version": (_I32, []),
+    "mpm_status_string": (C.c_char_p, [_I32]),
+    "mpm_struct_size": (_I32, [_I32]),
+    "mpm_particle_pass": (_I32, [_P, _P, _P, _F, _I32, _P]),
+    "mpm_grid_zero_box": (_I32, [_P, _P, _P, _I32, _P]),
+    "mpm_grid_zero_blocks": (_I32, [_P, _P, _P, _I32, _I32, _P]),
+    "mpm_blocks_from_box": (_I32, [_P, _P, _P, _P]),
+    "mpm_grid_update": (_I32, [_P, _P, _P, _P, _P, _I32, _P, _I32, _P, _F, _I32, _P]),
+    "mpm_blocks_mark": (_I32, [_P, _P, _P, _P]),
+    "mpm_blocks_dilate": (_I32, [_P, _P, _P, _P]),
+    "mpm_blocks_compact": (_I32, [_P, _P, _P, _P, _P, _P, _P]),
+    "mpm_bounds": (_I32, [_P, _P, _P]),
+    "mpm_sort_keys": (_I32, [_P, _P, _P, _P, _P, _P]),
+    "mpm_sort_temp_bytes": (_I64, [_I32]),
+    "mpm_sort_pairs": (_I32, [_P, _P, _P, _P, _I32, _P, _I64, _P]),
+    "mpm_permute": (_I32, [_P, _P, _P, _P]),
+    "mpm_export_R": (_I32, [_P, _P, _P]),
+    "mpm_export_cov": (_I32, [_P, _P, _P, _P]),
+    "mpm_grid_to_dense": (_I32, [_P, _P, _P, _P, _P]),
+}
+
+_LIB = None
+
+
+class NativeError(RuntimeError):
+    """A C-ABI call returned a non-zero mpm_status."""
+
+
+def load():
+    """Load libmpm_hip.so once; raise ImportError (no fallback) when it is not built."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(
+            f"libmpm_hip.so not found at {LIB_PATH}: build it with "
+            f"`python mpm-engine_amd/build.py` (there is no non-HIP fallback)")
+    lib = C.CDLL(LIB_PATH)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    sizes = {0: C.sizeof(Particles), 1: C.sizeof(Grid), 2: MODEL_DTYPE.itemsize,
+             3: COLLIDER_DTYPE.itemsize, 4: TICK_CTL_DTYPE.itemsize}
+    for which, size in sizes.items():
+        got = lib.mpm_struct_size(which)
+        if got != size:
+            raise ImportError(f"libmpm_hip ABI mismatch: struct {which} is {got} bytes in C, "
+                              f"{size} in the binding")
+    _LIB = lib
+    return lib
+
+
+def check(status: int, what: str = "") -> None:
+    if status != 0:
+        msg = _LIB.mpm_status_string(status).decode() if _LIB else str(status)
+        raise NativeError(f"{what}: {msg} (status {status})")
+
+
+def ptr(t) -> int:
+    """Device (or host) address of a torch tensor, 0 for None."""
+    return 0 if t is None else int(t.data_ptr())

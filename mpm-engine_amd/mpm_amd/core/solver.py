@@ -1,0 +1,991 @@
+"""Typed solver API on MI355X: the drop-in for warpmpm's ``Solver`` (src/warpmpm/core/
+solver.py:47-642) with ``device="hip"``.
+
+Same constructor flags (grid, device, guard_interval, sparse, fused, sort_interval,
+profile, inversion_policy, periodic_x), the same load / material / collider / step /
+export methods and readouts, and the same exceptions. Underneath, ``HipSimulator`` keeps
+every array as a torch tensor on the GPU and calls the gfx950 kernels of libmpm_hip.so
+through the C ABI (include/mpm_hip.h):
+
+* one tick = a prologue particle pass, S grid updates, S - 1 fused g2p+stress+p2g passes
+  and an epilogue gather, enqueued back to back on one stream with no host work between
+  substeps: collider poses for the whole tick are precomputed into a device table
+  (core.colliders) and the launch box lives in device memory;
+* the whole tick can therefore be captured once into a hipGraph and replayed;
+* the per-tick guard reads back 8 floats from a device reduction instead of copying
+  all positions and velocities to the host.
+
+There is no CPU execution path: ``device="cpu"`` raises, and a missing libmpm_hip.so
+raises at load.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+import warnings
+from dataclasses import dataclass, field
+from typing import Any
+
+import numpy as np
+
+from . import native
+from .colliders import Collider, ColliderSet, grid_box
+
+f32 = np.float32
+MATERIAL_NAME_TO_ID = {
+    "jelly": 0, "metal": 1, "sand": 2, "plasticine": 5, "fluid": 6, "stationary": 7,
+    "rigid": 8, "mu_i_sand": 9, "newtonian": 10, "mu_i_phi": 11, "tabulated_viscous": 12,
+    "tabulated_mu_i": 13,
+}
+_COV6_IDX = ((0, 0), (0, 1), (0, 2), (1, 1), (1, 2), (2, 2))
+
+
+def _cov_to_six(cov) -> np.ndarray:
+    cov = np.asarray(cov)
+    if cov.ndim == 2 and cov.shape[1] == 6:
+        return np.ascontiguousarray(cov, dtype=np.float32)
+    if cov.ndim == 3 and cov.shape[1:] == (3, 3):
+        return np.ascontiguousarray(np.stack([cov[:, i, j] for i, j in _COV6_IDX], 1), np.float32)
+    raise ValueError(f"cov must be (N, 6) or (N, 3, 3), got shape {cov.shape}")
+
+
+@dataclass
+class GridConfig:
+    n_grid: int = 64
+    grid_lim: float = 0.4
+
+    @property
+    def dx(self) -> float:
+        return self.grid_lim / self.n_grid
+
+
+def _resolve_device(device: str) -> str:
+    import torch
+
+    if device in ("cpu",):
+        raise NotImplementedError(
+            "this backend runs only on the GPU (HIP); device='cpu' has no implementation here")
+    if not torch.cuda.is_available():
+        raise RuntimeError("no HIP device is visible: the HIP Solver needs a GPU")
+    if device in ("auto", "hip", "cuda", "gpu"):
+        return "cuda:0"
+    if device.startswith("hip:"):
+        return "cuda:" + device.split(":", 1)[1]
+    if device.startswith("cuda"):
+        return device
+    raise ValueError(f"unknown device {device!r}")
+
+
+class HipSimulator:
+    """Device state and kernel orchestration (the role of MPM_Simulator_WARP)."""
+
+    def __init__(self, pos, vol, n_grid: int, grid_lim: float, device: str, cov=None):
+        import torch
+
+        self.torch = torch
+        self.lib = native.load()
+        self.device = torch.device(device)
+        n = int(len(pos))
+        self.n_particles = n
+        self.n_grid = int(n_grid)
+        self.dx = float(f32(grid_lim / n_grid))
+        self.inv_dx = float(f32(float(n_grid / grid_lim)))
+        self.grid_lim = float(grid_lim)
+        dev = self.device
+        t = lambda *s, dt=torch.float32: torch.zeros(*s, dtype=dt, device=dev)  # noqa: E731
+        self.x = torch.from_numpy(np.ascontiguousarray(np.asarray(pos, np.float32).T)).to(dev).contiguous()
+        self.v = t(3, n)
+        eye = torch.zeros(9, n, dtype=torch.float32, device=dev)
+        eye[0] = eye[4] = eye[8] = 1.0
+        self.F = eye.clone()
+        self.F_trial = eye.clone()
+        self.C = t(9, n)
+        self.stress = t(9, n)
+        self.L = t(9, n)
+        self.vol = torch.from_numpy(np.asarray(vol, np.float32).copy()).to(dev)
+        self.mass = t(n)
+        self.density = t(n)
+        self.mu = t(n)
+        self.lam = t(n)
+        self.yield_stress = t(n)
+        self.Jp = t(n)
+        self.material = t(n, dt=torch.int32)
+        self.selection = t(n, dt=torch.int32)
+        self.init_cov = None
+        self.cov = None
+        if cov is not None:
+            self.init_cov = torch.from_numpy(np.asarray(cov, np.float32).reshape(n, 6).copy()).to(dev)
+        self.update_cov_with_F = 0
+        # blocked dense grid
+        nb = (self.n_grid + 3) // 4
+        self.nb = (nb, nb, nb)
+        self.n_blocks = nb ** 3
+        self.node_in = t(self.n_blocks * 64 * 4)
+        self.node_out = t(self.n_blocks * 64 * 4)
+        self.error_flag = t(1, dt=torch.int32)
+        self.bounds = t(8)
+        # block lists and device tick control
+        self.box_list = t(self.n_blocks, dt=torch.int32)
+        self.ctl = t(native.TICK_CTL_DTYPE.itemsize // 4, dt=torch.int32)
+        self._ctl_host = torch.zeros_like(self.ctl, device="cpu").pin_memory()
+        self._h2d_done = None  # event after the last pinned-host upload
+        self._blk = None
+        self.sparse_ready = False
+        # model
+        self.model = np.zeros(1, dtype=native.MODEL_DTYPE)
+        m = self.model[0]
+        m["muI_I0"][:] = 1.0
+        m["muI_d"][:] = 1.0
+        m["muI_rho_s"][:] = 1.0
+        m["muI_phi_init"][:] = 1.0
+        m["softening"][:] = 0.1
+        sin_phi = f32(np.sin(f32(25.0 / 180.0 * 3.14159265)))
+        m["alpha"][:] = f32(np.sqrt(f32(2.0 / 3.0)) * f32(2.0) * sin_phi / (f32(3.0) - sin_phi))
+        m["grid_v_damping_scale"] = 1.1
+        m["eta_table_smin"], m["eta_table_smax"], m["eta_table_n"] = -1.0, 2.0, 2
+        self.eta_table = t(2)
+        m["eta_table"] = self.eta_table.data_ptr()
+        self.model_dev = torch.zeros(native.MODEL_DTYPE.itemsize, dtype=torch.uint8, device=dev)
+        self._model_dirty = True
+        self.material_id = 0
+        # colliders
+        self.colliders = ColliderSet(self.n_grid, self.dx)
+        self.wrench = t(6 * 64)
+        self.sched_dev = None
+        self._sched_host = None
+        self.time = 0.0
+        self.grid_launch_box = None
+        self._prev_grid_box = None
+        self.restrict_grid = True
+        self.profile = False
+        self.time_profile: dict[str, list[float]] = {}
+        self.use_graph = True
+        self._graph = None
+        self._graph_key = None
+        self._ticks = 0
+        self._structs()
+
+    # --------------------------------------------------------------- ABI structs
+    def _structs(self):
+        P = native.Particles()
+        P.n, P.stride = self.n_particles, self.n_particles
+        for nm in ("x", "v", "F", "F_trial", "C", "stress", "L", "vol", "mass", "mu", "lam",
+                   "yield_stress", "Jp", "material", "selection"):
+            setattr(P, nm, native.ptr(getattr(self, nm)))
+        P.cov = native.ptr(self.cov)
+        G = native.Grid()
+        G.node_in, G.node_out = native.ptr(self.node_in), native.ptr(self.node_out)
+        G.n[:] = [self.n_grid] * 3
+        G.nb[:] = list(self.nb)
+        G.dx, G.inv_dx = self.dx, self.inv_dx
+        G.error_flag = native.ptr(self.error_flag)
+        self.P, self.G = P, G
+        self.pP, self.pG = C.byref(P), C.byref(G)
+
+    def stream(self) -> int:
+        return self.torch.cuda.current_stream(self.device).cuda_stream
+
+    def _upload_model(self):
+        if self._model_dirty:
+            host = self.torch.from_numpy(self.model.view(np.uint8).copy())
+            self.model_dev.copy_(host, non_blocking=False)
+            self._model_dirty = False
+
+    # --------------------------------------------------------------- parameters
+    def _material_name_to_id(self, name):
+        if isinstance(name, (int, np.integer)):
+            return int(name)
+        if name not in MATERIAL_NAME_TO_ID:
+            raise TypeError(f"Undefined material type: {name}")
+        return MATERIAL_NAME_TO_ID[name]
+
+    def _set_type_params(self, m: int, kw: dict):
+        md = self.model[0]
+        for key, fld in (("E", "E"), ("nu", "nu"), ("bulk_modulus", "bulk"),
+                         ("xi", "xi"), ("plastic_viscosity", "plastic_viscosity"),
+                         ("softening", "softening"), ("mu_s", "muI_mu_s"),
+                         ("delta_mu", "muI_delta_mu"), ("I0", "muI_I0"),
+                         ("grain_diameter", "muI_d"), ("grain_density", "muI_rho_s"),
+                         ("phi_init", "muI_phi_init"), ("phi_chi", "muI_phi_chi")):
+            if key in kw:
+                md[fld][m] = float(kw[key])
+        if "hardening" in kw:
+            md["hardening"][m] = float(kw["hardening"])
+        if "friction_angle" in kw:
+            sin_phi = float(np.sin(kw["friction_angle"] / 180.0 * 3.14159265))
+            md["alpha"][m] = np.sqrt(2.0 / 3.0) * 2.0 * sin_phi / (3.0 - sin_phi)
+        self._model_dirty = True
+
+    def set_parameters_dict(self, kw: dict):
+        """set_parameters_dict (mpm_solver_warp.py:634-800)."""
+        torch = self.torch
+        if "material" in kw:
+            self.material_id = self._material_name_to_id(kw["material"])
+            self.material.fill_(self.material_id)
+        # the reference reallocates (zeroes) the grid on every call
+        self.node_in.zero_()
+        self.node_out.zero_()
+        m = self.material_id
+        self._set_type_params(m, kw)
+        md = self.model[0]
+        if "yield_stress" in kw:
+            self.yield_stress.fill_(float(kw["yield_stress"]))
+        if "eta_table" in kw:
+            tab = np.asarray(kw["eta_table"], dtype=np.float32).reshape(-1)
+            self.eta_table = torch.from_numpy(tab.copy()).to(self.device)
+            md["eta_table"] = self.eta_table.data_ptr()
+            md["eta_table_n"] = int(tab.shape[0])
+        if "eta_table_smin" in kw:
+            md["eta_table_smin"] = float(kw["eta_table_smin"])
+        if "eta_table_smax" in kw:
+            md["eta_table_smax"] = float(kw["eta_table_smax"])
+        if "g" in kw:
+            md["gravity"] = np.asarray(kw["g"], dtype=np.float64).astype(f32)
+        if "density" in kw:
+            self.density.fill_(float(kw["density"]))
+            torch.mul(self.density, self.vol, out=self.mass)
+        if "E" in kw or "nu" in kw:
+            self.finalize_mu_lam()
+        if "rpic_damping" in kw:
+            md["rpic_damping"] = kw["rpic_damping"]
+        if "grid_v_damping_scale" in kw:
+            md["grid_v_damping_scale"] = kw["grid_v_damping_scale"]
+        self._model_dirty = True
+
+    def finalize_mu_lam(self):
+        """compute_mu_lam_from_E_nu (mpm_utils.py:583-588), fp32 on device."""
+        torch = self.torch
+        md = self.model[0]
+        E = torch.from_numpy(md["E"].copy()).to(self.device)[self.material.long()]
+        nu = torch.from_numpy(md["nu"].copy()).to(self.device)[self.material.long()]
+        self.mu.copy_(E / (2.0 * (1.0 + nu)))
+        self.lam.copy_(E * nu / ((1.0 + nu) * (1.0 - 2.0 * nu)))
+
+    def set_parameters_for_particles(self, a: int, b: int, kw: dict):
+        """set_parameters_for_particles (mpm_solver_warp.py:1700-1783)."""
+        m = None
+        if "material" in kw:
+            m = self._material_name_to_id(kw["material"])
+            self.material[a:b] = m
+        if m is None:
+            m = self.material_id
+        self._set_type_params(m, kw)
+        if "density" in kw:
+            self.density[a:b] = float(kw["density"])
+            self.mass[a:b] = self.density[a:b] * self.vol[a:b]
+        if "yield_stress" in kw:
+            self.yield_stress[a:b] = float(kw["yield_stress"])
+        if "E" in kw or "nu" in kw:
+            md = self.model[0]
+            E, nu = float(md["E"][m]), float(md["nu"][m])
+            self.mu[a:b] = E / (2.0 * (1.0 + nu))
+            self.lam[a:b] = E * nu / ((1.0 + nu) * (1.0 - 2.0 * nu))
+
+    def set_gravity(self, g):
+        self.model[0]["gravity"] = np.asarray(g, dtype=np.float64).astype(f32)
+        self._model_dirty = True
+
+    def enable_cov_step(self):
+        self.update_cov_with_F = 1
+        self.model[0]["update_cov_with_F"] = 1
+        self._model_dirty = True
+        self.cov = self.init_cov.t().contiguous()
+        self._structs()
+        self._graph = None
+
+    # --------------------------------------------------------------- grid boxes
+    def _grid_box(self, lo_w, hi_w, halo=1):
+        return grid_box(lo_w, hi_w, self.dx, (self.n_grid,) * 3, halo)
+
+    def _boxes(self):
+        """Current particle box and the zero box (union with the previous tick's),
+        mpm_solver_warp.py:1415-1432."""
+        full = ((0, 0, 0), (self.n_grid,) * 3)
+        gbox = self.grid_launch_box if (self.restrict_grid and self.grid_launch_box) else None
+        if gbox is None:
+            self._prev_grid_box = None
+            return full, full
+        prev = self._prev_grid_box
+        if prev is None:
+            z = gbox
+        else:
+            zl = tuple(min(prev[0][i], gbox[0][i]) for i in range(3))
+            zh = tuple(max(prev[0][i] + prev[1][i], gbox[0][i] + gbox[1][i]) for i in range(3))
+            z = (zl, tuple(zh[i] - zl[i] for i in range(3)))
+        self._prev_grid_box = gbox
+        return gbox, z
+
+    # --------------------------------------------------------------- guard
+    def read_bounds(self):
+        """(min xyz, max xyz, max |v|, non-finite count, error flag) via a device
+        reduction; one small device->host copy."""
+        native.check(self.lib.mpm_bounds(self.pP, native.ptr(self.bounds), self.stream()), "bounds")
+        b = self.bounds.cpu().numpy()
+        flag = int(self.error_flag.item())
+        return b[0:3].copy(), b[3:6].copy(), float(b[6]), int(b[7]), flag
+
+    # --------------------------------------------------------------- sparse blocks
+    def rebuild_active_blocks(self):
+        """rebuild_active_blocks (mpm_solver_warp.py:989-1026), fully on device: the
+        counts stay on the GPU and size nothing on the host."""
+        torch = self.torch
+        if self._blk is None:
+            nb = self.n_blocks
+            z = lambda: torch.zeros(nb, dtype=torch.int32, device=self.device)  # noqa: E731
+            self._blk = {"raw": z(), "cur": z(), "prev": z(), "cur_list": z(),
+                         "union_list": z(),
+                         "counts": torch.zeros(2, dtype=torch.int32, device=self.device)}
+        b = self._blk
+        b["prev"], b["cur"] = b["cur"], b["prev"]
+        b["raw"].zero_()
+        b["counts"].zero_()
+        s = self.stream()
+        L = self.lib
+        native.check(L.mpm_blocks_mark(self.pP, self.pG, native.ptr(b["raw"]), s), "blocks_mark")
+        native.check(L.mpm_blocks_dilate(self.pG, native.ptr(b["raw"]), native.ptr(b["cur"]), s),
+                     "blocks_dilate")
+        native.check(L.mpm_blocks_compact(self.pG, native.ptr(b["cur"]), native.ptr(b["prev"]),
+                                          native.ptr(b["cur_list"]), native.ptr(b["union_list"]),
+                                          native.ptr(b["counts"]), s), "blocks_compact")
+        self.sparse_ready = True
+
+    def active_block_fraction(self) -> float:
+        if not self.sparse_ready:
+            return 1.0
+        return int(self._blk["counts"][0].item()) / float(self.n_blocks)
+
+    # --------------------------------------------------------------- sort
+    def sort_by_block(self) -> bool:
+        """Stable block-key sort on device (solver.py:454-467, mpm_solver_warp.py:
+        1548-1619); returns False (no permutation) when already ordered."""
+        torch = self.torch
+        n = self.n_particles
+        if n < 2:
+            return False
+        dev = self.device
+        keys = torch.empty(n, dtype=torch.int32, device=dev)
+        vals = torch.empty(n, dtype=torch.int32, device=dev)
+        flag = torch.zeros(1, dtype=torch.int32, device=dev)
+        s = self.stream()
+        native.check(self.lib.mpm_sort_keys(self.pP, self.pG, native.ptr(keys), native.ptr(vals),
+                                            native.ptr(flag), s), "sort_keys")
+        if int(flag.item()) == 0:
+            return False
+        tb = int(self.lib.mpm_sort_temp_bytes(n))
+        temp = torch.empty(max(tb, 1), dtype=torch.uint8, device=dev)
+        keys_o = torch.empty_like(keys)
+        perm = torch.empty_like(vals)
+        native.check(self.lib.mpm_sort_pairs(native.ptr(keys), native.ptr(keys_o), native.ptr(vals),
+                                             native.ptr(perm), n, native.ptr(temp), tb, s),
+                     "sort_pairs")
+        scratch = torch.empty(9 * n, dtype=torch.float32, device=dev)
+        native.check(self.lib.mpm_permute(self.pP, native.ptr(perm), native.ptr(scratch), s), "permute")
+        pl = perm.long()
+        self.density.copy_(self.density[pl])
+        if self.init_cov is not None:
+            self.init_cov.copy_(self.init_cov[pl])
+        self._last_perm = pl
+        return True
+
+    # --------------------------------------------------------------- ticks
+    def _ensure_sched(self, nbytes: int):
+        torch = self.torch
+        if self.sched_dev is None or self.sched_dev.numel() < nbytes:
+            self.sched_dev = torch.zeros(max(nbytes, 256), dtype=torch.uint8, device=self.device)
+            self._sched_host = torch.zeros(max(nbytes, 256), dtype=torch.uint8).pin_memory()
+            self._graph = None
+
+    def run_tick(self, dt: float, substeps: int, fused: bool, sparse: bool):
+        torch = self.torch
+        S = int(substeps)
+        if S <= 0:
+            return
+        self._upload_model()
+        L = self.lib
+        s = self.stream()
+        tab, t_end = self.colliders.schedule(self.time, dt, S)
+        K = tab.shape[1]
+        if self.colliders.n_slots > self.wrench.numel() // 6:
+            raise ValueError("at most 64 wrench-reading colliders are supported")
+        nbytes = tab.nbytes
+        self._ensure_sched(nbytes)
+        if self._h2d_done is not None:  # the pinned staging buffers are reused
+            self._h2d_done.synchronize()
+        if nbytes:
+            self._sched_host[:nbytes].numpy()[:] = tab.view(np.uint8).reshape(-1)
+            self.sched_dev[:nbytes].copy_(self._sched_host[:nbytes], non_blocking=True)
+        ctl = np.zeros(1, dtype=native.TICK_CTL_DTYPE)
+        if sparse:
+            ctl["box_lo"], ctl["box_hi"] = (0, 0, 0), (self.n_grid,) * 3
+            b = self._blk
+            zl, zc = b["union_list"], b["counts"][1:]
+            glist, gcount = b["cur_list"], b["counts"][0:]
+            self._prev_grid_box = None
+        else:
+            g, z = self._boxes()
+            ctl["box_lo"] = g[0]
+            ctl["box_hi"] = [g[0][i] + g[1][i] for i in range(3)]
+        self._ctl_host.numpy()[:] = ctl.view(np.int32).reshape(-1)
+        self.ctl.copy_(self._ctl_host, non_blocking=True)
+        self._h2d_done = torch.cuda.Event()
+        self._h2d_done.record()
+        # clear the zero box (union of this tick's and the previous tick's box)
+        if sparse:
+            native.check(L.mpm_grid_zero_blocks(self.pG, native.ptr(zl), native.ptr(zc),
+                                                self.n_blocks, 3, s), "zero_blocks")
+        else:
+            lo = (C.c_int32 * 3)(*z[0])
+            hi = (C.c_int32 * 3)(*[z[0][i] + z[1][i] for i in range(3)])
+            native.check(L.mpm_grid_zero_box(self.pG, lo, hi, 3, s), "zero_box")
+            native.check(L.mpm_blocks_from_box(self.pG, native.ptr(self.ctl), native.ptr(self.box_list), s),
+                         "blocks_from_box")
+            glist, gcount = self.box_list, self.ctl[6:]
+        lists = (native.ptr(glist), native.ptr(gcount))
+        if self.profile:
+            self._enqueue_tick_profiled(dt, S, K, fused, lists)
+        elif self.use_graph and self._ticks > 0:
+            key = (S, float(dt), bool(fused), K, lists, native.ptr(self.sched_dev),
+                   self.update_cov_with_F)
+            if self._graph is None or self._graph_key != key:
+                self._capture(dt, S, K, fused, lists)
+                self._graph_key = key
+            self._graph.replay()
+        else:
+            self._enqueue_tick(dt, S, K, fused, lists, s)
+        self._ticks += 1
+        self.time = t_end
+
+    def _capture(self, dt, S, K, fused, lists):
+        torch = self.torch
+        g = torch.cuda.CUDAGraph()
+        torch.cuda.synchronize(self.device)
+        with torch.cuda.graph(g):
+            self._enqueue_tick(dt, S, K, fused, lists, self.stream())
+        self._graph = g
+
+    def _pass(self, mode, dt, s):
+        native.check(self.lib.mpm_particle_pass(self.pP, self.pG, native.ptr(self.model_dev), dt, mode, s),
+                     "particle_pass")
+
+    def _grid(self, dt, k_row, K, lists, s):
+        sched = native.ptr(self.sched_dev) + k_row * K * native.COLLIDER_DTYPE.itemsize
+        native.check(self.lib.mpm_grid_update(
+            self.pG, native.ptr(self.model_dev), native.ptr(self.ctl), lists[0], lists[1],
+            self.n_blocks, sched if K else None, K, native.ptr(self.wrench), dt,
+            native.GRID_ZERO_IN, s), "grid_update")
+
+    def _enqueue_tick(self, dt, S, K, fused, lists, s):
+        G2P, ST, P2G = native.PASS_G2P, native.PASS_STRESS, native.PASS_P2G
+        if fused:
+            self._pass(ST | P2G, dt, s)
+            self._grid(dt, 0, K, lists, s)
+            for k in range(1, S):
+                mode = G2P | ST | P2G | native.PASS_LEAN
+                if k == S - 1:
+                    mode |= native.PASS_KEEP_STRESS
+                self._pass(mode, dt, s)
+                self._grid(dt, k, K, lists, s)
+            self._pass(G2P, dt, s)
+        else:
+            for k in range(S):
+                self._pass(ST | P2G, dt, s)
+                self._grid(dt, k, K, lists, s)
+                self._pass(G2P, dt, s)
+
+    def _enqueue_tick_profiled(self, dt, S, K, fused, lists):
+        torch = self.torch
+        s = self.stream()
+        ev = []
+
+        def timed(name, fn):
+            a = torch.cuda.Event(enable_timing=True)
+            b = torch.cuda.Event(enable_timing=True)
+            a.record()
+            fn()
+            b.record()
+            ev.append((name, a, b))
+
+        G2P, ST, P2G = native.PASS_G2P, native.PASS_STRESS, native.PASS_P2G
+        for k in range(S):
+            if fused and k > 0:
+                timed("g2p2g_fused", lambda: self._pass(G2P | ST | P2G, dt, s))
+            else:
+                timed("stress_p2g", lambda: self._pass(ST | P2G, dt, s))
+            timed("grid_update+BC", lambda k=k: self._grid(dt, k, K, lists, s))
+            if not fused:
+                timed("g2p", lambda: self._pass(G2P, dt, s))
+        if fused:
+            timed("g2p", lambda: self._pass(G2P, dt, s))
+        torch.cuda.synchronize(self.device)
+        for name, a, b in ev:
+            self.time_profile.setdefault(name, []).append(a.elapsed_time(b))
+
+    # --------------------------------------------------------------- exports
+    def export_R(self):
+        torch = self.torch
+        R = torch.empty(self.n_particles, 9, dtype=torch.float32, device=self.device)
+        native.check(self.lib.mpm_export_R(self.pP, native.ptr(R), self.stream()), "export_R")
+        return R
+
+    def export_cov(self):
+        torch = self.torch
+        if self.update_cov_with_F:
+            return self.cov.t()
+        out = torch.empty(self.n_particles, 6, dtype=torch.float32, device=self.device)
+        if self.init_cov is None:
+            return out.zero_()
+        native.check(self.lib.mpm_export_cov(self.pP, native.ptr(self.init_cov), native.ptr(out),
+                                             self.stream()), "export_cov")
+        return out
+
+    def grid_dense(self):
+        """Dense (n, n, n) mass and (n, n, n, 3) v_in / v_out copies (z fastest)."""
+        torch = self.torch
+        n = self.n_grid
+        m = torch.empty(n, n, n, dtype=torch.float32, device=self.device)
+        vi = torch.empty(n, n, n, 3, dtype=torch.float32, device=self.device)
+        vo = torch.empty(n, n, n, 3, dtype=torch.float32, device=self.device)
+        native.check(self.lib.mpm_grid_to_dense(self.pG, native.ptr(m), native.ptr(vi), native.ptr(vo),
+                                                self.stream()), "grid_to_dense")
+        return m, vi, vo
+
+
+@dataclass
+class Solver:
+    """warpmpm's Solver on the HIP backend. Device selection occurs at load time."""
+
+    grid: GridConfig = field(default_factory=GridConfig)
+    device: str = "auto"
+    guard_interval: int = 1
+    sparse: bool = False
+    fused: bool = True
+    sort_interval: int = 0
+    profile: bool = False
+    inversion_policy: str = "warn"
+    periodic_x: bool = False
+    _sim: Any = field(default=None, init=False, repr=False)
+    _step: int = field(default=0, init=False, repr=False)
+    _tick: int = field(default=0, init=False, repr=False)
+    _vol0: Any = field(default=None, init=False, repr=False)
+
+    def __post_init__(self) -> None:
+        if self.inversion_policy not in {"warn", "raise", "nan"}:
+            raise ValueError("inversion_policy must be 'warn', 'raise', or 'nan'")
+
+    # ------------------------------------------------------------------ setup
+    def load_particles(self, pos: np.ndarray, vol: np.ndarray, cov: np.ndarray | None = None,
+                       cov_mode: str = "step") -> "Solver":
+        self.device = _resolve_device(self.device)
+        pos = np.asarray(pos)
+        self._vol0 = np.asarray(vol, np.float32).copy()
+        cov6 = None
+        if cov is not None:
+            if cov_mode not in ("step", "from_F"):
+                raise ValueError(f"cov_mode must be 'step' or 'from_F', got {cov_mode!r}")
+            cov6 = _cov_to_six(cov)
+            if cov6.shape[0] != len(pos):
+                raise ValueError(f"cov has {cov6.shape[0]} rows, expected {len(pos)}")
+        self._sim = HipSimulator(pos.astype(np.float32), vol, self.grid.n_grid, self.grid.grid_lim,
+                                 self.device, cov=cov6)
+        if self.periodic_x:
+            self._sim.model[0]["periodic_x"] = 1
+            self._sim._model_dirty = True
+        if cov6 is not None and cov_mode == "step":
+            self._sim.enable_cov_step()
+        return self
+
+    def set_material(self, material, **overrides: float) -> "Solver":
+        if hasattr(material, "resolve"):
+            name, params = material.resolve()
+        else:
+            name, params = str(material), {}
+        self._sim.set_parameters_dict({"material": name, "g": [0.0, 0.0, -9.81], **params,
+                                       **overrides})
+        self._sim.finalize_mu_lam()
+        return self
+
+    def set_material_range(self, start: int, end: int, material, **overrides) -> "Solver":
+        if hasattr(material, "resolve"):
+            name, params = material.resolve()
+        else:
+            name, params = str(material), {}
+        if name in ("tabulated_viscous", "tabulated_mu_i"):
+            raise NotImplementedError(
+                "tabulated materials are global-only (one eta table on the model); use set_material")
+        self._sim.set_parameters_for_particles(start, end, {"material": name, **params, **overrides})
+        return self
+
+    def finalize_rigid_bodies(self) -> "Solver":
+        raise NotImplementedError("rigid bodies run the reference's split pipeline only; "
+                                  "they are not part of the HIP substep")
+
+    # ------------------------------------------------------------------ colliders
+    def _cs(self) -> ColliderSet:
+        return self._sim.colliders
+
+    def add_plane(self, point, normal, surface: str = "sticky", friction: float = 0.0,
+                  restitution: float = 0.0) -> "Solver":
+        """Plane collider (mpm_solver_warp.py:1880-2018)."""
+        if restitution != 0.0:
+            raise NotImplementedError("plane restitution acts on rigid bodies, which the HIP "
+                                      "substep does not carry")
+        if surface == "sticky" and friction != 0:
+            raise ValueError("friction must be 0 on sticky surfaces.")
+        point = [float(v) for v in point]
+        scale = 1.0 / float(f32(math.sqrt(float(sum(float(x) ** 2 for x in normal)))))
+        normal = [scale * float(x) for x in normal]
+        c = Collider(native.COLLIDER_PLANE, "plane", 0.0, 999.0)
+        c.point = [float(f32(v)) for v in point]
+        c.normal = [float(f32(v)) for v in normal]
+        c.surface = {"sticky": 0, "slip": 1, "cut": 11}.get(surface, 2)
+        c.friction = float(f32(friction))
+        axis = next((i for i in range(3) if abs(abs(normal[i]) - 1.0) < 1e-6), -1)
+        if axis >= 0:
+            c.aabb_kind = "plane"
+            c.plane_axis = axis
+            c.plane_offset = float(point[axis])
+            c.plane_sign = float(normal[axis])
+        else:
+            c.label = "plane_free"
+        self._cs().add(c)
+        return self
+
+    def add_box(self, center, half_size, velocity=(0.0, 0.0, 0.0), start_time: float = 0.0,
+                end_time: float = 1.0e9) -> int:
+        """Kinematic axis-aligned box (set_velocity_on_cuboid, mpm_solver_warp.py:2028-2106)."""
+        c = Collider(native.COLLIDER_BOX, "cuboid_velocity", start_time, end_time)
+        c.point = [float(f32(v)) for v in center]
+        c.size = [float(f32(v)) for v in half_size]
+        c.velocity = [float(f32(v)) for v in velocity]
+        c.aabb_kind = "box"
+        return self._cs().add(c, wrench=True)
+
+    def set_box(self, handle: int, center=None, velocity=None) -> "Solver":
+        c = self._cs().items[handle]
+        if center is not None:
+            c.point = [float(f32(v)) for v in center]
+        if velocity is not None:
+            c.velocity = [float(f32(v)) for v in velocity]
+        return self
+
+    def _wrench(self, handle: int):
+        c = self._cs().items[handle]
+        if c.slot < 0:
+            raise ValueError(f"collider {handle} ({c.label}) has no reaction accumulator")
+        return c
+
+    def reset_tool_force(self, handle: int) -> "Solver":
+        c = self._wrench(handle)
+        self._sim.wrench[6 * c.slot:6 * c.slot + 6].zero_()
+        return self
+
+    def tool_force(self, handle: int, dt: float) -> np.ndarray:
+        c = self._wrench(handle)
+        return self._sim.wrench[6 * c.slot:6 * c.slot + 3].cpu().numpy().astype(float) / dt
+
+    def add_cup(self, profile, center, quat=(1.0, 0.0, 0.0, 0.0), velocity=(0.0, 0.0, 0.0),
+                omega=(0.0, 0.0, 0.0), friction: float = 0.05, sticky_cells: float = 1.5,
+                contact_cells: float = 0.5, start_time: float = 0.0, end_time: float = 1.0e9) -> int:
+        """Revolved-SDF glass (add_revolved_sdf_collider, mpm_solver_warp.py:2121-2269)."""
+        from ..colliders.glass import quat_to_mat
+
+        sticky_depth = min(sticky_cells * self.grid.dx, 0.45 * profile.wall_thickness)
+        band = contact_cells * self.grid.dx
+        c = Collider(native.COLLIDER_CUP, "revolved_cup", start_time, end_time)
+        c.point = [float(f32(v)) for v in center]
+        c.rot = np.asarray(quat_to_mat(quat), dtype=np.float64).reshape(3, 3).copy()
+        c.velocity = [float(f32(v)) for v in velocity]
+        c.omega = [float(f32(v)) for v in omega]
+        c.friction = float(f32(friction))
+        c.sticky_depth = float(f32(sticky_depth))
+        c.contact_band = float(f32(band))
+        c.shape = tuple(float(f32(v)) for v in (profile.outer_radius, profile.inner_radius,
+                                                 profile.half_height, profile.inner_floor_z,
+                                                 profile.fillet_radius))
+        c.aabb_kind = "ball"
+        c.ball_r = float(np.sqrt(profile.outer_radius ** 2 + profile.half_height ** 2)) + float(band)
+        return self._cs().add(c, wrench=True)
+
+    def set_cup(self, handle: int, center=None, quat=None, velocity=None, omega=None) -> "Solver":
+        from ..colliders.glass import quat_to_mat
+
+        c = self._cs().items[handle]
+        if center is not None:
+            c.point = [float(f32(v)) for v in center]
+        if quat is not None:
+            c.rot = np.asarray(quat_to_mat(quat), dtype=np.float64).reshape(3, 3).copy()
+        if velocity is not None:
+            c.velocity = [float(f32(v)) for v in velocity]
+        if omega is not None:
+            c.omega = [float(f32(v)) for v in omega]
+        return self
+
+    def reset_cup_wrench(self, handle: int) -> "Solver":
+        c = self._wrench(handle)
+        self._sim.wrench[6 * c.slot:6 * c.slot + 6].zero_()
+        return self
+
+    def cup_wrench(self, handle: int, dt: float) -> dict:
+        c = self._wrench(handle)
+        w = self._sim.wrench[6 * c.slot:6 * c.slot + 6].cpu().numpy().astype(float)
+        return {"force": w[:3] / dt, "torque": w[3:] / dt}
+
+    def add_domain_walls(self, start_time: float = 0.0, end_time: float = 1.0e9) -> "Solver":
+        """Outward-velocity clamp within 3 cells of each face, as six face shells sharing
+        one parameter block (add_bounding_box, mpm_solver_warp.py:2287-2380)."""
+        n, pad = self.grid.n_grid, 3
+        shells = [((0, 0, 0), (pad, n, n)), ((n - pad, 0, 0), (pad, n, n)),
+                  ((0, 0, 0), (n, pad, n)), ((0, n - pad, 0), (n, pad, n)),
+                  ((0, 0, 0), (n, n, pad)), ((0, 0, n - pad), (n, n, pad))]
+        for box in shells:
+            c = Collider(native.COLLIDER_WALLS, "domain_walls", start_time, end_time)
+            c.aabb_kind = "fixed"
+            c.fixed_box = box
+            self._cs().add(c)
+        return self
+
+    def add_sdf_collider(self, sdf, center, quat=(0.0, 0.0, 0.0, 1.0), velocity=(0.0, 0.0, 0.0),
+                         omega=(0.0, 0.0, 0.0), band=None, surface: str = "separable",
+                         friction: float = 0.4, start_time: float = 0.0,
+                         end_time: float = 1.0e9) -> int:
+        """Mesh-SDF collider (mpm_solver_warp.py:2621-2777)."""
+        torch = self._sim.torch
+        vals = np.asarray(sdf.values)
+        res = int(vals.shape[0])
+        if band is None:
+            band = float(self._sim.dx)
+        sid = {"sticky": 0, "slip": 1, "separable": 2}[surface]
+        bmin = float(min(vals[0].min(), vals[-1].min(), vals[:, 0, :].min(), vals[:, -1, :].min(),
+                         vals[:, :, 0].min(), vals[:, :, -1].min()))
+        if band >= bmin:
+            raise ValueError(
+                f"SDF collider contact band ({band * 1e3:.1f} mm) reaches or exceeds the SDF "
+                f"grid margin ({bmin * 1e3:.1f} mm): near-surface space outside the stored grid "
+                f"would get no collision. Rebuild the SDF with more margin_cells or reduce band.")
+        ext = (res - 1) * float(sdf.cell)
+        o = [float(v) for v in sdf.origin]
+        corners = np.array([[o[0] + a * ext, o[1] + b * ext, o[2] + d * ext]
+                            for a in (0, 1) for b in (0, 1) for d in (0, 1)])
+        r_max = float(np.linalg.norm(corners, axis=1).max())
+        c = Collider(native.COLLIDER_SDF, "mesh_sdf", start_time, end_time)
+        val_d = torch.from_numpy(np.ascontiguousarray(vals, dtype=np.float32)).to(self._sim.device)
+        grad_d = torch.from_numpy(np.ascontiguousarray(sdf.grads, dtype=np.float32)).to(self._sim.device)
+        c.sdf = (val_d, grad_d, res, [float(f32(v)) for v in o], float(f32(sdf.cell)), r_max)
+        c.point = [float(f32(v)) for v in center]
+        c.quat = [float(f32(v)) for v in quat]
+        c.velocity = [float(f32(v)) for v in velocity]
+        c.omega = [float(f32(v)) for v in omega]
+        c.band = float(f32(band))
+        c.surface = sid
+        c.friction = float(f32(friction))
+        c.aabb_kind = "sdf"
+        c.sdf_corners = corners
+        c.guard = {"r_max": r_max, "band": float(band), "warned": False}
+        return self._cs().add(c, wrench=True)
+
+    def set_sdf_pose(self, handle: int, center=None, quat=None, velocity=None, omega=None) -> "Solver":
+        c = self._cs().items[handle]
+        g = c.guard
+        if g is not None and not g["warned"]:
+            jump = 0.0
+            if center is not None:
+                jump += float(np.linalg.norm(np.array([float(center[i]) - c.point[i] for i in range(3)])))
+            if quat is not None:
+                dot = abs(sum(float(quat[i]) * c.quat[i] for i in range(4)))
+                jump += 2.0 * float(np.arccos(min(1.0, dot))) * g["r_max"]
+            if jump > g["band"]:
+                g["warned"] = True
+                warnings.warn(
+                    f"set_sdf_pose jumped the collider surface by up to {jump * 1e3:.2f} mm in "
+                    f"one tick, more than the contact band ({g['band'] * 1e3:.2f} mm): material "
+                    f"inside the swept region can be tunnelled through. Drive the pose in "
+                    f"smaller per-tick steps (or via velocity/omega).", RuntimeWarning, stacklevel=2)
+        if center is not None:
+            c.point = [float(f32(v)) for v in center]
+        if quat is not None:
+            c.quat = [float(f32(v)) for v in quat]
+        if velocity is not None:
+            c.velocity = [float(f32(v)) for v in velocity]
+        if omega is not None:
+            c.omega = [float(f32(v)) for v in omega]
+        return self
+
+    def reset_sdf_force(self, handle: int) -> "Solver":
+        return self.reset_cup_wrench(handle)
+
+    def sdf_wrench(self, handle: int, dt: float) -> dict:
+        return self.cup_wrench(handle, dt)
+
+    def add_point_cloud_collider(self, points, padding: int = 1, start_time: float = 0.0,
+                                 end_time: float = 1.0e9) -> "Solver":
+        """Static occupancy collider (mpm_solver_warp.py:2545-2616)."""
+        torch = self._sim.torch
+        n = self.grid.n_grid
+        idx = np.clip((np.asarray(points) * self._sim.inv_dx).astype(np.int32), 0, n - 1)
+        occ = np.zeros((n, n, n), np.int32)
+        occ[idx[:, 0], idx[:, 1], idx[:, 2]] = 1
+        for _ in range(padding):
+            d = occ.copy()
+            d[1:] |= occ[:-1]; d[:-1] |= occ[1:]
+            d[:, 1:] |= occ[:, :-1]; d[:, :-1] |= occ[:, 1:]
+            d[:, :, 1:] |= occ[:, :, :-1]; d[:, :, :-1] |= occ[:, :, 1:]
+            occ = d
+        c = Collider(native.COLLIDER_POINTCLOUD, "pointcloud", start_time, end_time)
+        c.occupancy = torch.from_numpy(occ).to(self._sim.device)
+        a = np.argwhere(occ == 1)
+        c.aabb_kind = "fixed"
+        c.fixed_box = ((tuple(int(v) for v in a.min(0)),
+                        tuple(int(a.max(0)[i] - a.min(0)[i] + 1) for i in range(3))) if len(a) else None)
+        self._cs().add(c)
+        return self
+
+    def add_cdf_collider(self, *args, **kwargs) -> int:
+        raise NotImplementedError("CPIC thin-boundary (CDF) colliders are not implemented on the "
+                                  "HIP substep yet")
+
+    # ------------------------------------------------------------------ stepping
+    def step(self, dt: float, substeps: int = 1) -> "Solver":
+        """solver.py:429-452: guard + launch box, optional block sort, optional sparse
+        rebuild, then one control tick of `substeps` substeps."""
+        sim = self._sim
+        if self._tick % max(1, self.guard_interval) == 0:
+            self._update_grid_box(dt, substeps)
+        if self.sort_interval and self._tick % self.sort_interval == 0:
+            self._sort_particles()
+        if self.sparse:
+            sim.rebuild_active_blocks()
+        sim.profile = self.profile
+        self._tick += 1
+        sim.run_tick(dt, substeps, fused=self.fused, sparse=self.sparse)
+        self._step += substeps
+        return self
+
+    def _sort_particles(self) -> bool:
+        moved = self._sim.sort_by_block()
+        if moved:
+            self._vol0 = self._vol0[self._sim._last_perm.cpu().numpy()]
+        return moved
+
+    def _update_grid_box(self, dt: float, substeps: int) -> None:
+        """solver.py:485-518 on device-reduced bounds."""
+        lo_x, hi_x, vmax, bad, flag = self._sim.read_bounds()
+        dx = self.grid.dx
+        lim = self.grid.grid_lim
+        if flag & 1:
+            raise RuntimeError("a particle stencil left the grid during the last tick (out of "
+                               "bounds P2G was skipped); enlarge grid_lim or add a wall collider")
+        gmin = lo_x[1:].min() if self.periodic_x else lo_x.min()
+        gmax = hi_x[1:].max() if self.periodic_x else hi_x.max()
+        if bad or gmin < 1.5 * dx or gmax > lim - 2.5 * dx:
+            raise RuntimeError(
+                f"particles within 2 cells of the grid edge (x in [{gmin:.4f}, {gmax:.4f}] m, "
+                f"domain [0, {lim}] m, dx={dx:.4f}): the P2G stencil would write out of bounds. "
+                f"Enlarge grid_lim or add a bounding box / wall collider.")
+        pad = 3.0 * dx + 1.5 * float(vmax) * dt * substeps
+        lo = lo_x - pad
+        hi = hi_x + pad
+        if self.periodic_x:
+            lo[0], hi[0] = 0.0, lim
+        self._sim.grid_launch_box = self._sim._grid_box(lo, hi, halo=0)
+
+    def profile_report(self) -> str:
+        prof = self._sim.time_profile if self._sim is not None else {}
+        rows = [(k, sum(v) / 1000.0, len(v)) for k, v in prof.items() if v]
+        if not rows:
+            return "profile_report: no samples (set solver.profile = True and step)"
+        timed = sum(t for _, t, _ in rows)
+        rows.sort(key=lambda r: -r[1])
+        lines = [f"substep profile over {self._step} substeps (timed device work {timed:.3f}s; "
+                 f"hipEvent-timed live launches):"]
+        for name, tot, n in rows:
+            lines.append(f"  {name:<28s} {tot:7.3f}s  {tot / max(n, 1) * 1000:8.3f} ms/launch"
+                         f"  {tot / timed * 100:5.1f}%")
+        return "\n".join(lines)
+
+    # ------------------------------------------------------------------ imports / exports
+    def set_x(self, pos: np.ndarray) -> "Solver":
+        t = self._sim.torch.from_numpy(np.ascontiguousarray(pos, dtype=np.float32))
+        self._sim.x.copy_(t.t())
+        return self
+
+    def set_v(self, vel: np.ndarray) -> "Solver":
+        t = self._sim.torch.from_numpy(np.ascontiguousarray(vel, dtype=np.float32))
+        self._sim.v.copy_(t.t())
+        return self
+
+    def x(self) -> np.ndarray:
+        return self._sim.x.t().cpu().numpy()
+
+    def v(self) -> np.ndarray:
+        return self._sim.v.t().cpu().numpy()
+
+    def F(self) -> np.ndarray:
+        return self._sim.F.t().cpu().numpy().reshape(-1, 3, 3)
+
+    def stress(self) -> np.ndarray:
+        return self._sim.stress.t().cpu().numpy().reshape(-1, 3, 3)
+
+    def L(self) -> np.ndarray:
+        return self._sim.L.t().cpu().numpy().reshape(-1, 3, 3)
+
+    def inverted_count(self) -> int:
+        J = np.linalg.det(self.F())
+        return int(np.count_nonzero(~np.isfinite(J) | (J <= 0.0)))
+
+    def _checked_jacobian(self) -> np.ndarray:
+        J = np.linalg.det(self.F())
+        bad = ~np.isfinite(J) | (J <= 0.0)
+        n = int(np.count_nonzero(bad))
+        if not n:
+            return J
+        msg = (f"{n} particles have non-finite or non-positive det(F); current volume and "
+               "Cauchy stress are undefined for an inverted configuration")
+        if self.inversion_policy == "raise":
+            raise RuntimeError(msg)
+        if self.inversion_policy == "warn":
+            if not getattr(self, "_warned_inverted", False):
+                self._warned_inverted = True
+                warnings.warn(f"{msg}; using |det(F)| for compatibility. Set inversion_policy="
+                              "'raise' to reject or 'nan' to mark these particles.",
+                              RuntimeWarning, stacklevel=3)
+            out = np.abs(J)
+            out[~np.isfinite(out)] = np.nan
+            return out
+        J = J.copy()
+        J[bad] = np.nan
+        return J
+
+    def vol(self) -> np.ndarray:
+        return self._vol0 * self._checked_jacobian()
+
+    def cauchy(self) -> np.ndarray:
+        J = self._checked_jacobian()
+        if self.inversion_policy == "warn":
+            J = np.clip(J, 1e-9, None)
+        return self.stress() / J[:, None, None]
+
+    def cov(self) -> np.ndarray:
+        return self._sim.export_cov().cpu().numpy().reshape(-1, 6)
+
+    def R(self) -> np.ndarray:
+        R = self._sim.export_R().cpu().numpy().reshape(-1, 3, 3)
+        return np.transpose(R, (0, 2, 1))
+
+    def x_torch(self):
+        return self._sim.x.t()
+
+    def v_torch(self):
+        return self._sim.v.t()
+
+    def F_torch(self):
+        return self._sim.F.t().reshape(-1, 3, 3)
+
+    def cov_torch(self):
+        return self._sim.export_cov().reshape(-1, 6)
+
+    def R_torch(self):
+        return self._sim.export_R().reshape(-1, 3, 3).transpose(1, 2)
+
+    @property
+    def n_particles(self) -> int:
+        return 0 if self._sim is None else self._sim.n_particles

@@ -1,0 +1,107 @@
+"""Scene builders shared by tests and the bench (reference: src/warpmpm/scenes.py)."""
+from __future__ import annotations
+
+import numpy as np
+
+from .core.solver import GridConfig
+from .materials import Material, newtonian
+
+
+def dough(eta: float = 40.0, tau_y: float = 200.0) -> Material:
+    """Bingham dough: Newtonian with a yield stress."""
+    return newtonian(eta=eta, density=1000.0).with_yield(tau_y)
+
+
+def block(grid: GridConfig, size=(0.12, 0.05, 0.06), center=None, ppc: int = 2, seed: int = 0):
+    """Jittered particle box resting on the floor (3 cells up). Returns (pos, vol, floor)."""
+    dx = grid.dx
+    floor = 3 * dx
+    h = dx / ppc
+    cx = grid.grid_lim * 0.5 if center is None else center[0]
+    cy = grid.grid_lim * 0.5 if center is None else center[1]
+    sx, sy, sz = size
+    axes = (np.arange(cx - 0.5 * sx + 0.5 * h, cx + 0.5 * sx, h),
+            np.arange(cy - 0.5 * sy + 0.5 * h, cy + 0.5 * sy, h),
+            np.arange(floor + 0.5 * h, floor + sz, h))
+    g = np.stack(np.meshgrid(*axes, indexing="ij"), axis=-1).reshape(-1, 3)
+    g = g + np.random.default_rng(seed).uniform(-0.25 * h, 0.25 * h, size=g.shape)
+    return g.astype(np.float32), np.full(len(g), h ** 3, dtype=np.float32), floor
+
+
+def lattice_box(lo, hi, h: float, seed: int = 0):
+    """Jittered lattice (spacing h, jitter +-h/4) filling the box [lo, hi)."""
+    axes = [np.arange(a + 0.5 * h, b, h) for a, b in zip(lo, hi)]
+    g = np.stack(np.meshgrid(*axes, indexing="ij"), axis=-1).reshape(-1, 3)
+    g = g + np.random.default_rng(seed).uniform(-0.25 * h, 0.25 * h, size=g.shape)
+    return g.astype(np.float32), np.full(len(g), h ** 3, dtype=np.float32)
+
+
+def cylinder(center, radius: float, height: float, h: float, seed: int = 0):
+    """Jittered lattice filling an upright (z-axis) cylinder with its base at center."""
+    cx, cy, z0 = center
+    g, vol = lattice_box((cx - radius, cy - radius, z0), (cx + radius, cy + radius, z0 + height),
+                         h, seed)
+    keep = np.hypot(g[:, 0] - cx, g[:, 1] - cy) < radius
+    return g[keep], vol[keep]
+
+
+class PourScene:
+    """Newtonian honey pour on a 192^3 grid over [0, 1]^3 (the production workload):
+    honey (eta 10 Pa s, bulk 1e4 Pa, rho 1400) fills a cylinder of radius 0.08 at 8
+    particles per cell inside a source glass whose wall is 2 cells thick; a static target
+    glass stands beside it. The source tilts at omega_tilt about a pivot on its rim. One
+    frame is 1/60 s in `substeps` substeps (dt = 1 / (60 * substeps))."""
+
+    def __init__(self, n_grid: int = 192, height: float = 0.30, radius: float = 0.08,
+                 substeps: int = 432, omega_tilt: float = 1.0, seed: int = 0):
+        from .colliders.glass import GlassProfile
+
+        self.grid = GridConfig(n_grid=n_grid, grid_lim=1.0)
+        dx = self.grid.dx
+        self.substeps = substeps
+        self.dt = 1.0 / (60.0 * substeps)
+        inner = radius + 2 * dx
+        self.profile = GlassProfile(outer_radius=inner + 2 * dx, inner_radius=inner,
+                                    height=height + 0.08, base_thickness=0.03, fillet_radius=0.01)
+        p = self.profile
+        self.source_center = np.array([0.32, 0.5, 0.06 + p.half_height])
+        self.target_center = np.array([0.72, 0.5, 0.06 + p.half_height])
+        floor = self.source_center[2] + p.inner_floor_z
+        self.pos, self.vol = cylinder((0.32, 0.5, floor + dx), radius, height, dx / 2, seed)
+        # rim pivot on the +x side of the source glass; tilt about +y
+        self.pivot = self.source_center + np.array([p.outer_radius, 0.0, p.half_height])
+        self.omega = np.array([0.0, omega_tilt, 0.0])
+        self.center = self.source_center.astype(np.float64).copy()
+        self.quat = np.array([1.0, 0.0, 0.0, 0.0])
+
+    def material(self):
+        return newtonian(eta=10.0, density=1400.0, bulk_modulus=1.0e4)
+
+    def build(self, solver):
+        solver.load_particles(self.pos, self.vol)
+        solver.set_material(self.material())
+        solver.add_plane((0, 0, 3 * self.grid.dx), (0, 0, 1), "separate", friction=0.3)
+        solver.add_domain_walls()
+        self.src = solver.add_cup(self.profile, self.source_center)
+        self.dst = solver.add_cup(self.profile, self.target_center)
+        return solver
+
+    def frame(self, solver):
+        """Command the source glass for one frame (start-of-tick pose, rigid velocity of
+        the tilt about the pivot), then advance one frame."""
+        from .colliders.glass import quat_from_axis_angle, quat_mul
+
+        fdt = self.dt * self.substeps
+        vel = np.cross(self.omega, self.center - self.pivot)
+        solver.set_cup(self.src, center=self.center, quat=self.quat, velocity=vel, omega=self.omega)
+        solver.step(self.dt, self.substeps)
+        # exact rigid motion of the commanded pose over the frame
+        ang = float(np.linalg.norm(self.omega)) * fdt
+        if ang > 0:
+            ax = self.omega / np.linalg.norm(self.omega)
+            q = quat_from_axis_angle(ax, ang)
+            from .colliders.glass import quat_to_mat
+
+            R = quat_to_mat(q)
+            self.center = self.pivot + R @ (self.center - self.pivot)
+            self.quat = quat_mul(q, self.quat)

@@ -1,0 +1,1003 @@
+"""Oracle orchestration: numpy restatement of warpmpm's simulator host logic.
+
+TEST INFRASTRUCTURE ONLY (the checker). Only tests/, the smoke check and the bench's
+cpu_baseline leg import this module; the product Solver never does.
+
+`OracleSim` restates ``MPM_Simulator_WARP`` (src/warpmpm/kernels/mpm_solver_warp.py): the
+split substep ``p2g2p`` (:1092-1281), the fused tick ``p2g2p_fused_tick`` (:1379-1546), the
+per-collider BC launches and host pose integration ``_apply_grid_bc`` (:1283-1336), the
+collider registry (:1880-2814), sparse blocks (:989-1026) and the permutation (:1548-1619).
+`OracleSolver` restates the typed wrapper ``Solver`` (src/warpmpm/core/solver.py:57-642).
+Device kernels are the C functions in mpm_oracle.c, called through ctypes on numpy
+arrays laid out like the reference's Warp arrays (AoS vec3 / row-major mat33, dense grid
+with z fastest). Host-side numerics follow the reference: poses integrate in float64 and
+are stored as float32 (Warp struct fields are fp32), kernel `time` arguments are fp32.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import warnings
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+MAX_MATERIALS = 16
+# material ids, mpm_solver_warp.py:14-19
+MATERIAL_NAME_TO_ID = {
+    "jelly": 0, "metal": 1, "sand": 2, "plasticine": 5, "fluid": 6, "stationary": 7,
+    "rigid": 8, "mu_i_sand": 9, "newtonian": 10, "mu_i_phi": 11, "tabulated_viscous": 12,
+    "tabulated_mu_i": 13,
+}
+
+f32 = np.float32
+FP = C.POINTER(C.c_float)
+IP = C.POINTER(C.c_int32)
+
+
+class OModel(C.Structure):
+    _fields_ = [
+        ("dx", C.c_float), ("inv_dx", C.c_float),
+        ("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32),
+        ("gravity", C.c_float * 3), ("rpic_damping", C.c_float),
+        ("grid_v_damping_scale", C.c_float), ("update_cov_with_F", C.c_int32),
+        ("periodic_x", C.c_int32),
+        ("E", FP), ("nu", FP), ("bulk", FP), ("hardening", FP), ("xi", FP),
+        ("plastic_viscosity", FP), ("softening", FP), ("alpha", FP),
+        ("muI_mu_s", FP), ("muI_delta_mu", FP), ("muI_I0", FP), ("muI_d", FP),
+        ("muI_rho_s", FP), ("muI_phi_init", FP), ("muI_phi_chi", FP),
+        ("eta_table", FP), ("eta_table_smin", C.c_float), ("eta_table_smax", C.c_float),
+        ("eta_table_n", C.c_int32),
+        ("mu", FP), ("lam", FP), ("yield_stress", FP),
+    ]
+
+
+class OState(C.Structure):
+    _fields_ = [
+        ("n", C.c_int32),
+        ("x", FP), ("v", FP), ("F", FP), ("F_trial", FP), ("C", FP), ("stress", FP),
+        ("L", FP), ("R", FP), ("vol", FP), ("mass", FP), ("density", FP), ("Jp", FP),
+        ("cov", FP), ("init_cov", FP), ("selection", IP), ("material", IP),
+        ("grid_m", FP), ("grid_v_in", FP), ("grid_v_out", FP),
+    ]
+
+
+class OPlane(C.Structure):
+    _fields_ = [("start_time", C.c_float), ("end_time", C.c_float),
+                ("point", C.c_float * 3), ("normal", C.c_float * 3),
+                ("surface_type", C.c_int32), ("friction", C.c_float)]
+
+
+class OBox(C.Structure):
+    _fields_ = [("start_time", C.c_float), ("end_time", C.c_float),
+                ("point", C.c_float * 3), ("size", C.c_float * 3),
+                ("velocity", C.c_float * 3), ("reset", C.c_int32), ("force", FP)]
+
+
+class OCup(C.Structure):
+    _fields_ = [("start_time", C.c_float), ("end_time", C.c_float),
+                ("point", C.c_float * 3), ("rot", C.c_float * 9),
+                ("velocity", C.c_float * 3), ("omega", C.c_float * 3),
+                ("friction", C.c_float), ("outer_radius", C.c_float),
+                ("inner_radius", C.c_float), ("half_height", C.c_float),
+                ("inner_floor_z", C.c_float), ("fillet_radius", C.c_float),
+                ("sticky_depth", C.c_float), ("contact_band", C.c_float),
+                ("force", FP), ("torque", FP)]
+
+
+class OWalls(C.Structure):
+    _fields_ = [("start_time", C.c_float), ("end_time", C.c_float)]
+
+
+class OSdf(C.Structure):
+    _fields_ = [("sdf_val", FP), ("sdf_grad", FP), ("res", C.c_int32),
+                ("origin", C.c_float * 3), ("cell", C.c_float), ("center", C.c_float * 3),
+                ("quat", C.c_float * 4), ("velocity", C.c_float * 3),
+                ("omega", C.c_float * 3), ("band", C.c_float), ("surface_type", C.c_int32),
+                ("friction", C.c_float), ("start_time", C.c_float), ("end_time", C.c_float),
+                ("force", FP), ("torque", FP)]
+
+
+class OPointCloud(C.Structure):
+    _fields_ = [("occupancy", IP), ("start_time", C.c_float), ("end_time", C.c_float)]
+
+
+_LIB = None
+
+
+def lib():
+    """Load (building on first use if needed) the oracle shared library."""
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "libmpm_oracle.so")
+        if not os.path.exists(path):
+            import subprocess
+            subprocess.run(["make", "-C", _HERE], check=True, capture_output=True)
+        L = C.CDLL(path)
+        S, M = C.POINTER(OState), C.POINTER(OModel)
+        L.o_svd3.argtypes = [FP, FP, FP, FP]
+        L.o_compute_mu_lam.argtypes = [S, M]
+        for nm in ("o_compute_stress", "o_p2g", "o_g2p", "o_g2p_stress_p2g"):
+            getattr(L, nm).argtypes = [S, M, C.c_float]
+        L.o_stress_particle.argtypes = [S, M, C.c_float, C.c_int]
+        L.o_zero_grid.argtypes = [S, M, IP, IP, C.c_int]
+        L.o_grid_normalize.argtypes = [S, M, C.c_float, IP, IP]
+        L.o_grid_damping.argtypes = [S, M, C.c_float, IP, IP]
+        for nm, st in (("o_collide_plane", OPlane), ("o_collide_box", OBox),
+                       ("o_collide_cup", OCup), ("o_collide_walls", OWalls),
+                       ("o_collide_sdf", OSdf), ("o_collide_pointcloud", OPointCloud)):
+            getattr(L, nm).argtypes = [C.c_float, C.c_float, S, M, C.POINTER(st), IP, IP]
+        L.o_blocks_mark.argtypes = [FP, C.c_int, C.c_float, IP, IP]
+        L.o_blocks_dilate.argtypes = [IP, IP, IP]
+        L.o_blocks_compact.argtypes = [IP, IP, IP, IP, IP, IP]
+        L.o_blocks_grid_op.argtypes = [S, M, C.c_int, C.c_float, C.c_float, IP, C.c_int, IP]
+        L.o_compute_R.argtypes = [S]
+        L.o_compute_cov.argtypes = [S]
+        _LIB = L
+    return _LIB
+
+
+def fptr(a):
+    assert a.dtype == np.float32 and a.flags.c_contiguous
+    return a.ctypes.data_as(FP)
+
+
+def iptr(a):
+    assert a.dtype == np.int32 and a.flags.c_contiguous
+    return a.ctypes.data_as(IP)
+
+
+def i3(t):
+    return (C.c_int32 * 3)(*[int(v) for v in t])
+
+
+def svd3(A):
+    """Oracle 3x3 SVD (float32 in / out): A = U diag(s) V^T."""
+    A = np.ascontiguousarray(A, dtype=f32).reshape(9)
+    U = np.zeros(9, f32); s = np.zeros(3, f32); V = np.zeros(9, f32)
+    lib().o_svd3(fptr(A), fptr(U), fptr(s), fptr(V))
+    return U.reshape(3, 3), s, V.reshape(3, 3)
+
+
+def quat_to_mat(q):
+    """Rotation matrix of a wxyz quaternion (restates colliders/glass.py:24-34)."""
+    w, x, y, z = (float(v) for v in q)
+    return np.array([
+        [1 - 2 * (y * y + z * z), 2 * (x * y - z * w), 2 * (x * z + y * w)],
+        [2 * (x * y + z * w), 1 - 2 * (x * x + z * z), 2 * (y * z - x * w)],
+        [2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)]], dtype=np.float64)
+
+
+def _omega_step_quat(q, omega, dt):
+    """xyzw exp-map step, world omega left-multiplies (mpm_solver_warp.py:186-210)."""
+    w = np.asarray(omega, dtype=float)
+    speed = float(np.linalg.norm(w))
+    if speed * dt < 1e-12:
+        return q
+    half = 0.5 * speed * dt
+    ax = w / speed
+    dq = np.array([ax[0] * np.sin(half), ax[1] * np.sin(half), ax[2] * np.sin(half),
+                   np.cos(half)])
+    x1, y1, z1, w1 = dq
+    x2, y2, z2, w2 = q
+    qn = np.array([w1 * x2 + x1 * w2 + y1 * z2 - z1 * y2,
+                   w1 * y2 - x1 * z2 + y1 * w2 + z1 * x2,
+                   w1 * z2 + x1 * y2 - y1 * x2 + z1 * w2,
+                   w1 * w2 - x1 * x2 - y1 * y2 - z1 * z2])
+    return qn / np.linalg.norm(qn)
+
+
+def _fa(vals):
+    return np.asarray(vals, dtype=np.float64).astype(f32)
+
+
+class OracleSim:
+    """Restatement of MPM_Simulator_WARP (mpm_solver_warp.py:213-504 allocations)."""
+
+    def __init__(self, pos, vol, n_grid, grid_lim, cov=None):
+        n = len(pos)
+        self.n_particles = n
+        self.n_grid = int(n_grid)
+        # model dx/inv_dx are fp32 struct fields (:225-235)
+        self.dx = float(f32(grid_lim / n_grid))
+        self.inv_dx = float(f32(float(n_grid / grid_lim)))
+        self.grid_lim = float(f32(grid_lim))
+        self.x = np.ascontiguousarray(pos, dtype=f32).reshape(n, 3).copy()
+        self.v = np.zeros((n, 3), f32)
+        eye = np.tile(np.eye(3, dtype=f32), (n, 1, 1))
+        self.F = eye.copy()
+        self.F_trial = eye.copy()
+        self.C = np.zeros((n, 3, 3), f32)
+        self.stress = np.zeros((n, 3, 3), f32)
+        self.L = np.zeros((n, 3, 3), f32)
+        self.R = np.zeros((n, 3, 3), f32)
+        self.vol = np.ascontiguousarray(vol, dtype=f32).copy()
+        self.mass = np.zeros(n, f32)
+        self.density = np.zeros(n, f32)
+        self.Jp = np.zeros(n, f32)
+        self.init_cov = np.zeros(6 * n, f32)
+        self.cov = np.zeros(6 * n, f32)
+        if cov is not None:
+            self.init_cov[:] = np.asarray(cov, f32).reshape(-1)
+        self.selection = np.zeros(n, np.int32)
+        self.material = np.zeros(n, np.int32)
+        g = self.n_grid
+        self.grid_m = np.zeros((g, g, g), f32)
+        self.grid_v_in = np.zeros((g, g, g, 3), f32)
+        self.grid_v_out = np.zeros((g, g, g, 3), f32)
+        # per-type arrays with the reference defaults (:238-293)
+        z = lambda: np.zeros(MAX_MATERIALS, f32)  # noqa: E731
+        self.E, self.nu, self.bulk, self.hardening = z(), z(), z(), z()
+        self.xi, self.plastic_viscosity = z(), z()
+        sin_phi = float(f32(np.sin(f32(25.0 / 180.0 * 3.14159265))))
+        alpha0 = float(f32(np.sqrt(f32(2.0 / 3.0)) * f32(2.0) * f32(sin_phi) / f32(3.0 - sin_phi)))
+        self.softening = np.full(MAX_MATERIALS, 0.1, f32)
+        self.alpha = np.full(MAX_MATERIALS, alpha0, f32)
+        self.muI_mu_s, self.muI_delta_mu = z(), z()
+        self.muI_I0 = np.ones(MAX_MATERIALS, f32)
+        self.muI_d = np.ones(MAX_MATERIALS, f32)
+        self.muI_rho_s = np.ones(MAX_MATERIALS, f32)
+        self.muI_phi_init = np.ones(MAX_MATERIALS, f32)
+        self.muI_phi_chi = z()
+        self.eta_table = np.zeros(2, f32)
+        self.eta_table_smin, self.eta_table_smax, self.eta_table_n = -1.0, 2.0, 2
+        self.mu = np.zeros(n, f32)
+        self.lam = np.zeros(n, f32)
+        self.yield_stress = np.zeros(n, f32)
+        self.gravity = np.zeros(3, f32)
+        self.rpic_damping = 0.0
+        self.grid_v_damping_scale = 1.1
+        self.update_cov_with_F = 0
+        self.periodic_x = 0
+        self.material_id = 0
+        self.time = 0.0
+        # collider registry: five parallel lists (:440-449)
+        self.colliders = []      # (kind, ctypes param)
+        self.modify_bc = []
+        self.collider_aabbs = []
+        self.collider_labels = []
+        self._keep = []          # keep accumulator arrays alive
+        self._bc_box_cache = {}
+        self.restrict_bc = True
+        self.restrict_grid = True
+        self.grid_launch_box = None
+        self._prev_grid_box = None
+        self.sparse_ready = False
+        self._blk = None
+        self._revolved_rot = {}
+        self._sdf_guard = {}
+
+    # ------------------------------------------------------------------ ctypes views
+    def _state(self):
+        s = OState()
+        s.n = self.n_particles
+        for nm in ("x", "v", "F", "F_trial", "C", "stress", "L", "R", "vol", "mass",
+                   "density", "Jp", "cov", "init_cov", "grid_m", "grid_v_in", "grid_v_out"):
+            setattr(s, nm, fptr(getattr(self, nm)))
+        s.selection = iptr(self.selection)
+        s.material = iptr(self.material)
+        self._sref = s
+        return C.byref(s)
+
+    def _model(self):
+        m = OModel()
+        m.dx, m.inv_dx = self.dx, self.inv_dx
+        m.nx = m.ny = m.nz = self.n_grid
+        m.gravity[:] = [float(v) for v in self.gravity]
+        m.rpic_damping = self.rpic_damping
+        m.grid_v_damping_scale = self.grid_v_damping_scale
+        m.update_cov_with_F = int(self.update_cov_with_F)
+        m.periodic_x = int(self.periodic_x)
+        for nm in ("E", "nu", "bulk", "hardening", "xi", "plastic_viscosity", "softening",
+                   "alpha", "muI_mu_s", "muI_delta_mu", "muI_I0", "muI_d", "muI_rho_s",
+                   "muI_phi_init", "muI_phi_chi", "eta_table", "mu", "lam", "yield_stress"):
+            setattr(m, nm, fptr(getattr(self, nm)))
+        m.eta_table_smin = self.eta_table_smin
+        m.eta_table_smax = self.eta_table_smax
+        m.eta_table_n = self.eta_table_n
+        self._mref = m
+        return C.byref(m)
+
+    # ------------------------------------------------------------------ parameters
+    def _material_name_to_id(self, name):
+        if isinstance(name, (int, np.integer)):
+            return int(name)
+        if name not in MATERIAL_NAME_TO_ID:
+            raise TypeError(f"Undefined material type: {name}")
+        return MATERIAL_NAME_TO_ID[name]
+
+    def set_parameters_dict(self, kw):
+        """Restates set_parameters_dict, mpm_solver_warp.py:634-800."""
+        if "material" in kw:
+            self.material_id = self._material_name_to_id(kw["material"])
+            self.material[:] = self.material_id
+        # grid arrays are reallocated (zeroed) by every call (:659-673)
+        self.grid_m[:] = 0
+        self.grid_v_in[:] = 0
+        self.grid_v_out[:] = 0
+        m = self.material_id
+        for key, arr in (("E", self.E), ("nu", self.nu), ("bulk_modulus", self.bulk)):
+            if key in kw:
+                arr[m] = kw[key]
+        if "yield_stress" in kw:
+            self.yield_stress[:] = kw["yield_stress"]
+        if "hardening" in kw:
+            self.hardening[m] = float(kw["hardening"])
+        if "xi" in kw:
+            self.xi[m] = kw["xi"]
+        if "friction_angle" in kw:
+            sin_phi = float(np.sin(kw["friction_angle"] / 180.0 * 3.14159265))
+            self.alpha[m] = np.sqrt(2.0 / 3.0) * 2.0 * sin_phi / (3.0 - sin_phi)
+        if "plastic_viscosity" in kw:
+            self.plastic_viscosity[m] = kw["plastic_viscosity"]
+        if "softening" in kw:
+            self.softening[m] = kw["softening"]
+        if "eta_table" in kw:
+            self.eta_table = np.asarray(kw["eta_table"], dtype=f32).reshape(-1).copy()
+            self.eta_table_n = int(self.eta_table.shape[0])
+        if "eta_table_smin" in kw:
+            self.eta_table_smin = float(kw["eta_table_smin"])
+        if "eta_table_smax" in kw:
+            self.eta_table_smax = float(kw["eta_table_smax"])
+        for key, arr in (("mu_s", self.muI_mu_s), ("delta_mu", self.muI_delta_mu),
+                         ("I0", self.muI_I0), ("grain_diameter", self.muI_d),
+                         ("grain_density", self.muI_rho_s), ("phi_init", self.muI_phi_init),
+                         ("phi_chi", self.muI_phi_chi)):
+            if key in kw:
+                arr[m] = float(kw[key])
+        if "g" in kw:
+            self.gravity[:] = _fa(kw["g"])
+        if "density" in kw:
+            self.density[:] = kw["density"]
+            self.mass[:] = self.density * self.vol
+        if "E" in kw or "nu" in kw:
+            lib().o_compute_mu_lam(self._state(), self._model())
+        if "rpic_damping" in kw:
+            self.rpic_damping = kw["rpic_damping"]
+        if "grid_v_damping_scale" in kw:
+            self.grid_v_damping_scale = kw["grid_v_damping_scale"]
+
+    def finalize_mu_lam(self):
+        lib().o_compute_mu_lam(self._state(), self._model())
+
+    def set_parameters_for_particles(self, a, b, kw):
+        """Restates set_parameters_for_particles, mpm_solver_warp.py:1700-1783."""
+        m = None
+        if "material" in kw:
+            m = self._material_name_to_id(kw["material"])
+            self.material[a:b] = m
+        if m is None:
+            m = self.material_id
+        for key, arr in (("E", self.E), ("nu", self.nu), ("bulk_modulus", self.bulk)):
+            if key in kw:
+                arr[m] = kw[key]
+        if "hardening" in kw:
+            self.hardening[m] = float(kw["hardening"])
+        if "xi" in kw:
+            self.xi[m] = kw["xi"]
+        if "friction_angle" in kw:
+            sin_phi = float(np.sin(kw["friction_angle"] / 180.0 * 3.14159265))
+            self.alpha[m] = np.sqrt(2.0 / 3.0) * 2.0 * sin_phi / (3.0 - sin_phi)
+        if "plastic_viscosity" in kw:
+            self.plastic_viscosity[m] = kw["plastic_viscosity"]
+        if "softening" in kw:
+            self.softening[m] = kw["softening"]
+        for key, arr in (("mu_s", self.muI_mu_s), ("delta_mu", self.muI_delta_mu),
+                         ("I0", self.muI_I0), ("grain_diameter", self.muI_d),
+                         ("grain_density", self.muI_rho_s), ("phi_init", self.muI_phi_init),
+                         ("phi_chi", self.muI_phi_chi)):
+            if key in kw:
+                arr[m] = float(kw[key])
+        if "density" in kw:
+            self.density[a:b] = kw["density"]
+            self.mass[a:b] = self.density[a:b] * self.vol[a:b]
+        if "yield_stress" in kw:
+            self.yield_stress[a:b] = kw["yield_stress"]
+        if "E" in kw or "nu" in kw:
+            E, nu = float(self.E[m]), float(self.nu[m])
+            self.mu[a:b] = E / (2.0 * (1.0 + nu))
+            self.lam[a:b] = E * nu / ((1.0 + nu) * (1.0 - 2.0 * nu))
+
+    # ------------------------------------------------------------------ grid boxes
+    def _grid_box(self, lo_w, hi_w, halo=1):
+        """mpm_solver_warp.py:1868-1878."""
+        dx = self.dx
+        n = (self.n_grid,) * 3
+        lo = [max(0, int(np.floor(float(lo_w[i]) / dx)) - halo) for i in range(3)]
+        hi = [min(n[i], int(np.ceil(float(hi_w[i]) / dx)) + halo + 1) for i in range(3)]
+        dims = (hi[0] - lo[0], hi[1] - lo[1], hi[2] - lo[2])
+        if dims[0] <= 0 or dims[1] <= 0 or dims[2] <= 0:
+            return None
+        return (tuple(lo), dims)
+
+    # ------------------------------------------------------------------ colliders
+    def _register(self, kind, param, modify, aabb, label):
+        self.colliders.append((kind, param))
+        self.modify_bc.append(modify)
+        self.collider_aabbs.append(aabb)
+        self.collider_labels.append(label)
+
+    def add_surface_collider(self, point, normal, surface="sticky", friction=0.0,
+                             start_time=0.0, end_time=999.0):
+        """mpm_solver_warp.py:1880-2018."""
+        point = [float(v) for v in point]
+        scale = 1.0 / float(f32(np.sqrt(float(sum(x ** 2 for x in normal)))))
+        normal = [scale * x for x in normal]
+        p = OPlane()
+        p.start_time, p.end_time = start_time, end_time
+        p.point[:] = point
+        p.normal[:] = normal
+        if surface == "sticky" and friction != 0:
+            raise ValueError("friction must be 0 on sticky surfaces.")
+        p.surface_type = {"sticky": 0, "slip": 1, "cut": 11}.get(surface, 2)
+        p.friction = friction
+        axis = next((i for i in range(3) if abs(abs(normal[i]) - 1.0) < 1e-6), -1)
+        aabb = None
+        if axis >= 0:
+            p_ax, sgn = float(point[axis]), float(normal[axis])
+
+            def aabb(axis=axis, p_ax=p_ax, sgn=sgn):
+                big = 1.0e9
+                lo_w = [-big] * 3
+                hi_w = [big] * 3
+                if sgn > 0.0:
+                    hi_w[axis] = p_ax
+                else:
+                    lo_w[axis] = p_ax
+                return self._grid_box(lo_w, hi_w, halo=1)
+        self._register("plane", p, None, aabb, "plane")
+
+    def set_velocity_on_cuboid(self, point, size, velocity, start_time=0.0, end_time=999.0,
+                               reset=0):
+        """mpm_solver_warp.py:2028-2106."""
+        p = OBox()
+        p.start_time, p.end_time = start_time, end_time
+        p.point[:] = [float(v) for v in point]
+        p.size[:] = [float(v) for v in size]
+        p.velocity[:] = [float(v) for v in velocity]
+        p.reset = reset
+        force = np.zeros(3, f32)
+        self._keep.append(force)
+        p.force = fptr(force)
+        p._force = force
+
+        def modify(time, dt, param):
+            if time >= param.start_time and time < param.end_time:
+                param.point[:] = [float(f32(param.point[i] + dt * param.velocity[i]))
+                                  for i in range(3)]
+
+        def aabb(param=p):
+            pt, s = param.point, param.size
+            return self._grid_box((pt[0] - s[0], pt[1] - s[1], pt[2] - s[2]),
+                                  (pt[0] + s[0], pt[1] + s[1], pt[2] + s[2]), halo=1)
+        self._register("box", p, modify, aabb, "cuboid_velocity")
+
+    def add_revolved_sdf_collider(self, point, rot, velocity=(0, 0, 0), omega=(0, 0, 0),
+                                  outer_radius=0.089, inner_radius=0.069, half_height=0.12,
+                                  inner_floor_z=-0.07, fillet_radius=0.012, friction=0.0,
+                                  sticky_depth=0.0, contact_band=0.0, start_time=0.0,
+                                  end_time=999.0):
+        """mpm_solver_warp.py:2121-2269."""
+        p = OCup()
+        p.point[:] = [float(v) for v in point]
+        rot_np = np.asarray(rot, dtype=np.float64).reshape(3, 3)
+        p.rot[:] = [float(v) for v in rot_np.astype(f32).ravel()]
+        p.velocity[:] = [float(v) for v in velocity]
+        p.omega[:] = [float(v) for v in omega]
+        p.start_time, p.end_time = start_time, end_time
+        p.friction = friction
+        p.outer_radius, p.inner_radius = outer_radius, inner_radius
+        p.half_height, p.inner_floor_z, p.fillet_radius = half_height, inner_floor_z, fillet_radius
+        p.sticky_depth, p.contact_band = sticky_depth, contact_band
+        force, torque = np.zeros(3, f32), np.zeros(3, f32)
+        self._keep += [force, torque]
+        p.force, p.torque = fptr(force), fptr(torque)
+        p._force, p._torque = force, torque
+        idx = len(self.colliders)
+        self._revolved_rot[idx] = rot_np.copy()
+
+        def modify(time, dt, param, idx=idx):
+            if time >= param.start_time and time < param.end_time:
+                param.point[:] = [float(f32(param.point[i] + dt * param.velocity[i]))
+                                  for i in range(3)]
+                w = np.array([param.omega[0], param.omega[1], param.omega[2]], dtype=np.float64)
+                w_norm = float(np.linalg.norm(w))
+                ang = w_norm * dt
+                if ang > 1.0e-12:
+                    ax = w / w_norm
+                    kx = np.array([[0.0, -ax[2], ax[1]], [ax[2], 0.0, -ax[0]],
+                                   [-ax[1], ax[0], 0.0]])
+                    inc = np.eye(3) + np.sin(ang) * kx + (1.0 - np.cos(ang)) * (kx @ kx)
+                    self._revolved_rot[idx] = inc @ self._revolved_rot[idx]
+                    param.rot[:] = [float(v) for v in self._revolved_rot[idx].astype(f32).ravel()]
+
+        r_ball = float(np.sqrt(outer_radius ** 2 + half_height ** 2)) + float(contact_band)
+
+        def aabb(param=p, r=r_ball):
+            c = param.point
+            return self._grid_box((c[0] - r, c[1] - r, c[2] - r), (c[0] + r, c[1] + r, c[2] + r),
+                                  halo=1)
+        self._register("cup", p, modify, aabb, "revolved_cup")
+        return idx
+
+    def set_revolved_collider_pose(self, idx, point=None, rot=None, velocity=None, omega=None):
+        """mpm_solver_warp.py:2271-2285."""
+        self._bc_box_cache = {}
+        p = self.colliders[idx][1]
+        if point is not None:
+            p.point[:] = [float(v) for v in point]
+        if rot is not None:
+            rot_np = np.asarray(rot, dtype=np.float64).reshape(3, 3)
+            self._revolved_rot[idx] = rot_np.copy()
+            p.rot[:] = [float(v) for v in rot_np.astype(f32).ravel()]
+        if velocity is not None:
+            p.velocity[:] = [float(v) for v in velocity]
+        if omega is not None:
+            p.omega[:] = [float(v) for v in omega]
+
+    def add_bounding_box(self, start_time=0.0, end_time=999.0):
+        """mpm_solver_warp.py:2287-2380: six face shells sharing one param."""
+        p = OWalls()
+        p.start_time, p.end_time = start_time, end_time
+        n, pad = self.n_grid, 3
+        shells = [((0, 0, 0), (pad, n, n)), ((n - pad, 0, 0), (pad, n, n)),
+                  ((0, 0, 0), (n, pad, n)), ((0, n - pad, 0), (n, pad, n)),
+                  ((0, 0, 0), (n, n, pad)), ((0, 0, n - pad), (n, n, pad))]
+        for box in shells:
+            self._register("walls", p, None, (lambda box=box: box), "domain_walls")
+
+    def add_sdf_collider(self, sdf_values, sdf_grads, origin, cell, center, quat=(0, 0, 0, 1),
+                         velocity=(0, 0, 0), omega=(0, 0, 0), band=None, surface="separable",
+                         friction=0.4, start_time=0.0, end_time=999.0):
+        """mpm_solver_warp.py:2621-2777."""
+        res = int(sdf_values.shape[0])
+        if band is None:
+            band = float(self.dx)
+        sid = {"sticky": 0, "slip": 1, "separable": 2}[surface]
+        vals = np.asarray(sdf_values)
+        bmin = float(min(vals[0].min(), vals[-1].min(), vals[:, 0, :].min(),
+                         vals[:, -1, :].min(), vals[:, :, 0].min(), vals[:, :, -1].min()))
+        if band >= bmin:
+            raise ValueError("SDF collider contact band reaches or exceeds the SDF grid margin")
+        ext = (res - 1) * float(cell)
+        corners = np.array([[float(origin[0]) + a * ext, float(origin[1]) + b * ext,
+                             float(origin[2]) + c * ext]
+                            for a in (0, 1) for b in (0, 1) for c in (0, 1)])
+        r_max = float(np.linalg.norm(corners, axis=1).max())
+        idx = len(self.colliders)
+        self._sdf_guard[idx] = {"r_max": r_max, "band": float(band), "warned": False}
+        p = OSdf()
+        sv = np.ascontiguousarray(sdf_values, dtype=f32)
+        sg = np.ascontiguousarray(sdf_grads, dtype=f32)
+        self._keep += [sv, sg]
+        p.sdf_val, p.sdf_grad, p.res = fptr(sv), fptr(sg), res
+        p.origin[:] = [float(v) for v in origin]
+        p.cell = float(cell)
+        p.center[:] = [float(v) for v in center]
+        p.quat[:] = [float(v) for v in quat]
+        p.velocity[:] = [float(v) for v in velocity]
+        p.omega[:] = [float(v) for v in omega]
+        p.band, p.surface_type, p.friction = float(band), sid, float(friction)
+        p.start_time, p.end_time = start_time, end_time
+        force, torque = np.zeros(3, f32), np.zeros(3, f32)
+        self._keep += [force, torque]
+        p.force, p.torque = fptr(force), fptr(torque)
+        p._force, p._torque = force, torque
+        guard = self._sdf_guard[idx]
+
+        def modify(time, dt, param):
+            if time >= param.start_time and time < param.end_time:
+                v = param.velocity
+                w = np.array([param.omega[0], param.omega[1], param.omega[2]], dtype=float)
+                speed = float(np.hypot(np.hypot(v[0], v[1]), v[2])) + \
+                    float(np.linalg.norm(w)) * guard["r_max"]
+                if speed * dt > guard["band"] and not guard["warned"]:
+                    guard["warned"] = True
+                    warnings.warn("SDF collider surface sweep exceeds the contact band: "
+                                  "material can tunnel through", RuntimeWarning, stacklevel=2)
+                param.center[:] = [float(f32(param.center[i] + dt * v[i])) for i in range(3)]
+                q = np.array([param.quat[0], param.quat[1], param.quat[2], param.quat[3]],
+                             dtype=float)
+                q = _omega_step_quat(q, w, dt)
+                param.quat[:] = [float(f32(t)) for t in q]
+
+        def aabb(param=p, corners=corners):
+            qx, qy, qz, qw = param.quat[0], param.quat[1], param.quat[2], param.quat[3]
+            R = np.array([
+                [1 - 2 * (qy * qy + qz * qz), 2 * (qx * qy - qz * qw), 2 * (qx * qz + qy * qw)],
+                [2 * (qx * qy + qz * qw), 1 - 2 * (qx * qx + qz * qz), 2 * (qy * qz - qx * qw)],
+                [2 * (qx * qz - qy * qw), 2 * (qy * qz + qx * qw), 1 - 2 * (qx * qx + qy * qy)]])
+            c = np.array([param.center[0], param.center[1], param.center[2]])
+            pts = c + corners @ R.T
+            return self._grid_box(pts.min(axis=0), pts.max(axis=0), halo=1)
+        self._register("sdf", p, modify, aabb, "mesh_sdf")
+        return idx
+
+    def set_sdf_pose(self, handle, center=None, quat=None, velocity=None, omega=None):
+        """mpm_solver_warp.py:2779-2814 (jump warning restated)."""
+        self._bc_box_cache = {}
+        p = self.colliders[handle][1]
+        guard = self._sdf_guard.get(handle)
+        if guard is not None and not guard["warned"]:
+            jump = 0.0
+            if center is not None:
+                jump += float(np.linalg.norm(np.array([float(center[i]) - p.center[i]
+                                                       for i in range(3)])))
+            if quat is not None:
+                dot = abs(sum(float(quat[i]) * p.quat[i] for i in range(4)))
+                jump += 2.0 * float(np.arccos(min(1.0, dot))) * guard["r_max"]
+            if jump > guard["band"]:
+                guard["warned"] = True
+                warnings.warn("set_sdf_pose jump exceeds the contact band: tunnel risk",
+                              RuntimeWarning, stacklevel=2)
+        if center is not None:
+            p.center[:] = [float(v) for v in center]
+        if quat is not None:
+            p.quat[:] = [float(v) for v in quat]
+        if velocity is not None:
+            p.velocity[:] = [float(v) for v in velocity]
+        if omega is not None:
+            p.omega[:] = [float(v) for v in omega]
+
+    def add_point_cloud_collider(self, pts, padding=1, start_time=0.0, end_time=999.0):
+        """mpm_solver_warp.py:2545-2616."""
+        n = self.n_grid
+        idx = np.clip((pts * self.inv_dx).astype(np.int32), 0, n - 1)
+        occ = np.zeros((n, n, n), np.int32)
+        occ[idx[:, 0], idx[:, 1], idx[:, 2]] = 1
+        for _ in range(padding):
+            dil = occ.copy()
+            for ax in range(3):
+                s_src = [slice(None)] * 3
+                s_dst = [slice(None)] * 3
+                s_src[ax], s_dst[ax] = slice(0, -1), slice(1, None)
+                dil[tuple(s_dst)] |= occ[tuple(s_src)]
+                s_src = [slice(None)] * 3
+                s_dst = [slice(None)] * 3
+                s_src[ax], s_dst[ax] = slice(1, None), slice(0, -1)
+                dil[tuple(s_dst)] |= occ[tuple(s_src)]
+            occ = dil
+        self._keep.append(occ)
+        p = OPointCloud()
+        p.occupancy = iptr(occ)
+        p.start_time, p.end_time = start_time, end_time
+        a = np.argwhere(occ == 1)
+        box = ((tuple(int(v) for v in a.min(0)),
+                tuple(int(a.max(0)[i] - a.min(0)[i] + 1) for i in range(3))) if len(a) else None)
+        self._register("pointcloud", p, None, (lambda box=box: box), "pointcloud")
+
+    # ------------------------------------------------------------------ BC application
+    _COLLIDE = {"plane": "o_collide_plane", "box": "o_collide_box", "cup": "o_collide_cup",
+                "walls": "o_collide_walls", "sdf": "o_collide_sdf",
+                "pointcloud": "o_collide_pointcloud"}
+
+    def _apply_grid_bc(self, dt):
+        """mpm_solver_warp.py:1283-1336."""
+        full = ((0, 0, 0), (self.n_grid,) * 3)
+        st, md = self._state(), self._model()
+        for k, (kind, param) in enumerate(self.colliders):
+            lo, dims = full
+            skip = False
+            fn = self.collider_aabbs[k]
+            if self.restrict_bc and fn is not None:
+                static = self.modify_bc[k] is None
+                if static and k in self._bc_box_cache:
+                    skip, lo, dims = self._bc_box_cache[k]
+                else:
+                    box = fn()
+                    if box is None:
+                        skip = True
+                    else:
+                        lo, dims = box
+                    if static:
+                        self._bc_box_cache[k] = (skip, lo, dims)
+            if not skip:
+                getattr(lib(), self._COLLIDE[kind])(float(f32(self.time)), dt, st, md,
+                                                    C.byref(param), i3(lo), i3(dims))
+            if self.modify_bc[k] is not None:
+                self.modify_bc[k](self.time, dt, param)
+
+    # ------------------------------------------------------------------ substeps
+    def _boxes(self):
+        full = ((0, 0, 0), (self.n_grid,) * 3)
+        gbox = self.grid_launch_box if (self.restrict_grid and self.grid_launch_box) else None
+        if gbox is None:
+            self._prev_grid_box = None
+            return full, full
+        prev = self._prev_grid_box
+        if prev is None:
+            z = gbox
+        else:
+            zl = tuple(min(prev[0][i], gbox[0][i]) for i in range(3))
+            zh = tuple(max(prev[0][i] + prev[1][i], gbox[0][i] + gbox[1][i]) for i in range(3))
+            z = (zl, tuple(zh[i] - zl[i] for i in range(3)))
+        self._prev_grid_box = gbox
+        return gbox, z
+
+    def p2g2p(self, dt):
+        """Split substep, mpm_solver_warp.py:1092-1281 (CPU path: no graphs)."""
+        L = lib()
+        st, md = self._state(), self._model()
+        sparse = self.sparse_ready and self.restrict_grid
+        if sparse:
+            self._prev_grid_box = None
+            b = self._blk
+            if b["union_n"] > 0:
+                L.o_blocks_grid_op(st, md, 0, dt, 0.0, iptr(b["union_list"]), b["union_n"],
+                                   i3(b["bd"]))
+        else:
+            g, z = self._boxes()
+            L.o_zero_grid(st, md, i3(z[0]), i3(z[1]), 7)
+        L.o_compute_stress(st, md, dt)
+        L.o_p2g(st, md, dt)
+        if sparse:
+            b = self._blk
+            if b["cur_n"] > 0:
+                L.o_blocks_grid_op(st, md, 1, dt, 0.0, iptr(b["cur_list"]), b["cur_n"],
+                                   i3(b["bd"]))
+        else:
+            L.o_grid_normalize(st, md, dt, i3(g[0]), i3(g[1]))
+        if self.grid_v_damping_scale < 1.0:
+            if sparse:
+                b = self._blk
+                if b["cur_n"] > 0:
+                    L.o_blocks_grid_op(st, md, 2, dt, self.grid_v_damping_scale,
+                                       iptr(b["cur_list"]), b["cur_n"], i3(b["bd"]))
+            else:
+                L.o_grid_damping(st, md, self.grid_v_damping_scale, i3(g[0]), i3(g[1]))
+        self._apply_grid_bc(dt)
+        L.o_g2p(self._state(), self._model(), dt)
+        self.time = self.time + dt
+
+    def p2g2p_fused_tick(self, dt, substeps):
+        """Fused tick, mpm_solver_warp.py:1379-1546 (CPU path: no graphs)."""
+        L = lib()
+        for s in range(substeps):
+            st, md = self._state(), self._model()
+            g, z = self._boxes()
+            if s == 0:
+                L.o_zero_grid(st, md, i3(z[0]), i3(z[1]), 7)
+                L.o_compute_stress(st, md, dt)
+                L.o_p2g(st, md, dt)
+            else:
+                L.o_zero_grid(st, md, i3(z[0]), i3(z[1]), 3)
+                L.o_g2p_stress_p2g(st, md, dt)
+                L.o_zero_grid(st, md, i3(z[0]), i3(z[1]), 4)
+            L.o_grid_normalize(st, md, dt, i3(g[0]), i3(g[1]))
+            if self.grid_v_damping_scale < 1.0:
+                L.o_grid_damping(st, md, self.grid_v_damping_scale, i3(g[0]), i3(g[1]))
+            self._apply_grid_bc(dt)
+            self.time = self.time + dt
+        L.o_g2p(self._state(), self._model(), dt)
+
+    # ------------------------------------------------------------------ sparse blocks
+    def rebuild_active_blocks(self):
+        """mpm_solver_warp.py:989-1026."""
+        if self._blk is None:
+            n = self.n_grid
+            bd = (int(np.ceil(n / 4)),) * 3
+            nb = bd[0] * bd[1] * bd[2]
+            self._blk = {"bd": bd, "raw": np.zeros(bd, np.int32), "cur": np.zeros(bd, np.int32),
+                         "prev": np.zeros(bd, np.int32), "cur_list": np.zeros(nb, np.int32),
+                         "union_list": np.zeros(nb, np.int32), "cur_n": 0, "union_n": 0}
+        b = self._blk
+        b["prev"], b["cur"] = b["cur"], b["prev"]
+        b["raw"][:] = 0
+        L = lib()
+        L.o_blocks_mark(fptr(self.x), self.n_particles, self.inv_dx, i3(b["bd"]), iptr(b["raw"]))
+        L.o_blocks_dilate(iptr(b["raw"]), i3(b["bd"]), iptr(b["cur"]))
+        counts = np.zeros(2, np.int32)
+        L.o_blocks_compact(iptr(b["cur"]), iptr(b["prev"]), i3(b["bd"]), iptr(b["cur_list"]),
+                           iptr(b["union_list"]), iptr(counts))
+        b["cur_n"], b["union_n"] = int(counts[0]), int(counts[1])
+        self.sparse_ready = True
+
+    def active_block_fraction(self):
+        if not self.sparse_ready:
+            return 1.0
+        bd = self._blk["bd"]
+        return self._blk["cur_n"] / float(bd[0] * bd[1] * bd[2])
+
+    def permute_particles(self, perm):
+        """mpm_solver_warp.py:1548-1619 (every per-particle array, incl. mu/lam/yield)."""
+        perm = np.asarray(perm)
+        for nm in ("x", "v", "F", "F_trial", "R", "stress", "C", "L", "vol", "mass",
+                   "density", "Jp", "mu", "lam", "yield_stress", "selection", "material"):
+            a = getattr(self, nm)
+            a[...] = a[perm]
+        for nm in ("init_cov", "cov"):
+            a = getattr(self, nm).reshape(-1, 6)
+            a[...] = a[perm]
+
+    def compute_R(self):
+        lib().o_compute_R(self._state())
+        return self.R
+
+    def compute_cov(self):
+        lib().o_compute_cov(self._state())
+        return self.cov
+
+
+class OracleSolver:
+    """Restatement of the typed wrapper Solver (src/warpmpm/core/solver.py:57-642) over
+    OracleSim. Accepts the same Material objects (anything with .resolve())."""
+
+    def __init__(self, grid, guard_interval=1, sparse=False, fused=True, sort_interval=0,
+                 periodic_x=False):
+        self.grid = grid
+        self.guard_interval = guard_interval
+        self.sparse = sparse
+        self.fused = fused
+        self.sort_interval = sort_interval
+        self.periodic_x = periodic_x
+        self._sim = None
+        self._step = 0
+        self._tick = 0
+
+    def load_particles(self, pos, vol, cov=None, cov_mode="step"):
+        self._vol0 = np.asarray(vol, f32).copy()
+        self._sim = OracleSim(pos, vol, self.grid.n_grid, self.grid.grid_lim, cov=cov)
+        if self.periodic_x:
+            self._sim.periodic_x = 1
+        if cov is not None and cov_mode == "step":
+            self._sim.update_cov_with_F = 1
+            self._sim.cov[:] = self._sim.init_cov
+        return self
+
+    def set_material(self, material, **overrides):
+        if hasattr(material, "resolve"):
+            name, params = material.resolve()
+        else:
+            name, params = str(material), {}
+        params = {**params, **overrides}
+        self._sim.set_parameters_dict({"material": name, "g": [0.0, 0.0, -9.81], **params})
+        self._sim.finalize_mu_lam()
+        return self
+
+    def set_material_range(self, start, end, material, **overrides):
+        if hasattr(material, "resolve"):
+            name, params = material.resolve()
+        else:
+            name, params = str(material), {}
+        self._sim.set_parameters_for_particles(start, end,
+                                               {"material": name, **params, **overrides})
+        return self
+
+    def add_plane(self, point, normal, surface="sticky", friction=0.0):
+        self._sim.add_surface_collider(tuple(point), tuple(normal), surface, friction=friction)
+        return self
+
+    def add_box(self, center, half_size, velocity=(0.0, 0.0, 0.0), start_time=0.0,
+                end_time=1.0e9):
+        self._sim.set_velocity_on_cuboid(tuple(center), tuple(half_size), tuple(velocity),
+                                         start_time=start_time, end_time=end_time)
+        return len(self._sim.colliders) - 1
+
+    def set_box(self, handle, center=None, velocity=None):
+        p = self._sim.colliders[handle][1]
+        self._sim._bc_box_cache = {}
+        if center is not None:
+            p.point[:] = [float(v) for v in center]
+        if velocity is not None:
+            p.velocity[:] = [float(v) for v in velocity]
+        return self
+
+    def reset_tool_force(self, handle):
+        self._sim.colliders[handle][1]._force[:] = 0
+        return self
+
+    def tool_force(self, handle, dt):
+        return np.asarray(self._sim.colliders[handle][1]._force, dtype=float) / dt
+
+    def add_cup(self, profile, center, quat=(1.0, 0.0, 0.0, 0.0), velocity=(0, 0, 0),
+                omega=(0, 0, 0), friction=0.05, sticky_cells=1.5, contact_cells=0.5,
+                start_time=0.0, end_time=1.0e9):
+        sticky = min(sticky_cells * self.grid.dx, 0.45 * profile.wall_thickness)
+        return self._sim.add_revolved_sdf_collider(
+            tuple(center), quat_to_mat(quat), velocity=tuple(velocity), omega=tuple(omega),
+            outer_radius=profile.outer_radius, inner_radius=profile.inner_radius,
+            half_height=profile.half_height, inner_floor_z=profile.inner_floor_z,
+            fillet_radius=profile.fillet_radius, friction=friction, sticky_depth=sticky,
+            contact_band=contact_cells * self.grid.dx, start_time=start_time,
+            end_time=end_time)
+
+    def set_cup(self, handle, center=None, quat=None, velocity=None, omega=None):
+        rot = None if quat is None else quat_to_mat(quat)
+        self._sim.set_revolved_collider_pose(handle, point=center, rot=rot,
+                                             velocity=velocity, omega=omega)
+        return self
+
+    def reset_cup_wrench(self, handle):
+        p = self._sim.colliders[handle][1]
+        p._force[:] = 0
+        p._torque[:] = 0
+        return self
+
+    def cup_wrench(self, handle, dt):
+        p = self._sim.colliders[handle][1]
+        return {"force": np.asarray(p._force, float) / dt, "torque": np.asarray(p._torque, float) / dt}
+
+    reset_sdf_force = reset_cup_wrench
+    sdf_wrench = cup_wrench
+
+    def add_domain_walls(self, start_time=0.0, end_time=1.0e9):
+        self._sim.add_bounding_box(start_time=start_time, end_time=end_time)
+        return self
+
+    def add_sdf_collider(self, sdf, center, quat=(0.0, 0.0, 0.0, 1.0), velocity=(0, 0, 0),
+                         omega=(0, 0, 0), band=None, surface="separable", friction=0.4,
+                         start_time=0.0, end_time=1.0e9):
+        return self._sim.add_sdf_collider(sdf.values, sdf.grads, sdf.origin, sdf.cell, center,
+                                          quat=quat, velocity=velocity, omega=omega, band=band,
+                                          surface=surface, friction=friction,
+                                          start_time=start_time, end_time=end_time)
+
+    def set_sdf_pose(self, handle, center=None, quat=None, velocity=None, omega=None):
+        self._sim.set_sdf_pose(handle, center=center, quat=quat, velocity=velocity, omega=omega)
+        return self
+
+    def step(self, dt, substeps=1):
+        """solver.py:429-452."""
+        if self._tick % max(1, self.guard_interval) == 0:
+            self._update_grid_box(dt, substeps)
+        if self.sort_interval and self._tick % self.sort_interval == 0:
+            self._sort_particles()
+        if self.sparse:
+            self._sim.rebuild_active_blocks()
+        self._tick += 1
+        if self.fused and not self.sparse:
+            self._sim.p2g2p_fused_tick(dt, substeps)
+            self._step += substeps
+        else:
+            for _ in range(substeps):
+                self._sim.p2g2p(dt)
+                self._step += 1
+        return self
+
+    def _sort_particles(self):
+        """solver.py:454-467."""
+        x = self.x()
+        base = (np.floor(x / self.grid.dx - 0.5).astype(np.int64)) >> 2
+        nb = (self.grid.n_grid >> 2) + 2
+        keys = (base[:, 0] * nb + base[:, 1]) * nb + base[:, 2]
+        if np.all(np.diff(keys) >= 0):
+            return False
+        perm = np.argsort(keys, kind="stable")
+        self._sim.permute_particles(perm)
+        self._vol0 = self._vol0[perm]
+        return True
+
+    def _update_grid_box(self, dt, substeps):
+        """solver.py:485-518."""
+        x, v = self.x(), self.v()
+        dx = self.grid.dx
+        lim = self.grid.grid_lim
+        g = x[:, 1:] if self.periodic_x else x
+        if g.min() < 1.5 * dx or g.max() > lim - 2.5 * dx:
+            raise RuntimeError("particles within 2 cells of the grid edge: the P2G stencil "
+                               "would write out of bounds")
+        pad = 3.0 * dx + 1.5 * float(np.abs(v).max()) * dt * substeps
+        lo = x.min(0) - pad
+        hi = x.max(0) + pad
+        if self.periodic_x:
+            lo[0], hi[0] = 0.0, lim
+        self._sim.grid_launch_box = self._sim._grid_box(lo, hi, halo=0)
+
+    def x(self):
+        return self._sim.x.copy()
+
+    def v(self):
+        return self._sim.v.copy()
+
+    def F(self):
+        return self._sim.F.copy()
+
+    def stress(self):
+        return self._sim.stress.copy()
+
+    def L(self):
+        return self._sim.L.copy()
+
+    @property
+    def n_particles(self):
+        return 0 if self._sim is None else self._sim.n_particles

@@ -1,0 +1,285 @@
+"""HIP substep vs the CPU oracle on the same seeded inputs (GPU).
+
+Tolerances follow the stated parity bar: grid mass / momentum after one substep within
+1e-5 relative, particle positions after 100 substeps within 1e-4 dx, conservation to fp32
+round-off; integer work (sort permutation, active blocks) bit-exact. Float atomics on the
+GPU accumulate in a different order than the oracle's sequential loop, which is the only
+expected source of difference for the transfers; the SVD differs at fp32 round-off."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from mpm_amd import GlassProfile, GridConfig, Solver
+from mpm_amd.geometry import sphere_sdf
+from mpm_amd.materials import elastic, granular, newtonian, tabulated_viscous, vonmises
+from oracle.sim import OracleSolver
+
+pytestmark = pytest.mark.gpu
+
+
+def c1_scene():
+    """Elastic cube drop: 32^3 grid on [0,1]^3, side-0.25 cube centred at (0.5,0.6,0.5),
+    8 particles per cell with seeded jitter (4096 particles), E=1e5, nu=0.3, rho=1000,
+    gravity -9.8 y, six sticky walls 3 cells in."""
+    grid = GridConfig(n_grid=32, grid_lim=1.0)
+    h = grid.dx / 2
+    lo = np.array([0.375, 0.475, 0.375])
+    ax = [np.arange(lo[i] + 0.5 * h, lo[i] + 0.25, h) for i in range(3)]
+    g = np.stack(np.meshgrid(*ax, indexing="ij"), -1).reshape(-1, 3)
+    g = g + np.random.default_rng(0).uniform(-0.25 * h, 0.25 * h, size=g.shape)
+    pos = g.astype(np.float32)
+    vol = np.full(len(pos), h ** 3, np.float32)
+    assert len(pos) == 4096
+    return grid, pos, vol
+
+
+def _c1(solver_cls, **kw):
+    grid, pos, vol = c1_scene()
+    s = solver_cls(grid=grid, **kw) if solver_cls is Solver else solver_cls(grid, **kw)
+    s.load_particles(pos, vol)
+    s.set_material(elastic(E=1e5, nu=0.3, density=1000.0), g=[0.0, -9.8, 0.0])
+    d = 3 * grid.dx
+    for ax in range(3):
+        for sgn in (1.0, -1.0):
+            p = [0.5, 0.5, 0.5]
+            p[ax] = d if sgn > 0 else 1.0 - d
+            n = [0.0, 0.0, 0.0]
+            n[ax] = sgn
+            s.add_plane(p, n, "sticky")
+    return s
+
+
+def test_c1_grid_after_one_substep():
+    """Grid mass and momentum after the first substep's scatter, 1e-5 relative."""
+    from oracle import sim as O
+
+    hip = _c1(Solver, device="hip")
+    ora = _c1(OracleSolver)
+    hip.step(1e-4, 1)
+    ora.step(1e-4, 1)
+    # state after the substep's scatter: rerun the prologue scatter on both
+    sim = hip._sim
+    sim.node_in.zero_()
+    sim._upload_model()
+    sim._pass(6, 1e-4, sim.stream())
+    m, vi, _ = sim.grid_dense()
+    o = ora._sim
+    o.grid_m[:] = 0
+    o.grid_v_in[:] = 0
+    O.lib().o_compute_stress(o._state(), o._model(), 1e-4)
+    O.lib().o_p2g(o._state(), o._model(), 1e-4)
+    m = m.cpu().numpy()
+    vi = vi.cpu().numpy()
+    assert np.abs(m - o.grid_m).max() <= 1e-5 * np.abs(o.grid_m).max()
+    scale = max(np.abs(o.grid_v_in).max(), 1e-30)
+    assert np.abs(vi - o.grid_v_in).max() <= 1e-5 * scale
+    np.testing.assert_allclose(m.sum(dtype=np.float64), o.mass.sum(dtype=np.float64), rtol=1e-6)
+
+
+def test_c1_positions_after_100_substeps():
+    hip = _c1(Solver, device="hip")
+    ora = _c1(OracleSolver)
+    hip.step(1e-4, 100)
+    ora.step(1e-4, 100)
+    dx = 1.0 / 32
+    err = np.abs(hip.x() - ora.x()).max()
+    assert err <= 1e-4 * dx, f"max |dx| = {err / dx:.3e} dx"
+    np.testing.assert_allclose(hip.v(), ora.v(), atol=1e-5)
+    np.testing.assert_allclose(hip.F(), ora.F(), atol=1e-5)
+
+
+def _blob(n=3000, seed=0):
+    rng = np.random.default_rng(seed)
+    pts = (rng.random((n, 3), dtype=np.float32) * 0.10 + 0.15).astype(np.float32)
+    return GridConfig(n_grid=40, grid_lim=0.4), pts, np.full(n, 1.0e-7, np.float32)
+
+
+MATERIALS = [
+    ("jelly", elastic(E=5.0e4, nu=0.3, density=800.0)),
+    ("metal", vonmises(E=1.0e5, nu=0.3, yield_stress=2.0e3, density=1200.0)),
+    ("metal_hardening", vonmises(E=1.0e5, nu=0.3, yield_stress=5.0e2, hardening_xi=1e3)),
+    ("newtonian", newtonian(eta=2.0, density=1000.0, bulk_modulus=2.0e5)),
+    ("herschel_bulkley", newtonian(eta=0.0, density=1000.0, bulk_modulus=2.0e5)
+     .with_yield(200.0).with_powerlaw(K=50.0, n=0.5)),
+    ("mu_i", granular(mu_s=0.38, delta_mu=0.26, I0=0.28, density=1550.0, E=2e5)),
+    ("mu_i_phi", granular(mu_s=0.38, delta_mu=0.26, I0=0.28, E=2e5).with_dilatancy(0.6, 0.2)),
+    ("tabulated", tabulated_viscous(np.linspace(5.0, 1.0, 16), density=1000.0, bulk_modulus=2e5)),
+]
+
+
+def _mat_run(cls, material, fused=True, **kw):
+    grid, pts, vol = _blob()
+    s = cls(grid=grid, **kw) if cls is Solver else cls(grid, fused=fused)
+    if cls is Solver:
+        s.fused = fused
+    s.load_particles(pts, vol)
+    s.set_material(material)
+    s.add_plane((0, 0, 3 * grid.dx), (0, 0, 1), "separate", friction=0.3)
+    s.add_domain_walls()
+    for _ in range(3):
+        s.step(2.0e-4, 8)
+    return s
+
+
+@pytest.mark.parametrize("label,material", MATERIALS, ids=[m[0] for m in MATERIALS])
+def test_materials_match_oracle(label, material):
+    hip = _mat_run(Solver, material, device="hip")
+    ora = _mat_run(OracleSolver, material)
+    dx = 0.4 / 40
+    assert np.abs(hip.x() - ora.x()).max() <= 1e-4 * dx
+    np.testing.assert_allclose(hip.F(), ora.F(), rtol=1e-4, atol=1e-5)
+    sref = np.abs(ora.stress()).max()
+    np.testing.assert_allclose(hip.stress(), ora.stress(), atol=2e-4 * sref + 1e-3)
+    np.testing.assert_allclose(hip._sim.yield_stress.cpu().numpy(), ora._sim.yield_stress,
+                               rtol=1e-4)
+
+
+@pytest.mark.parametrize("material", [MATERIALS[0][1], MATERIALS[3][1]], ids=["jelly", "newtonian"])
+def test_split_pipeline_matches_oracle(material):
+    hip = _mat_run(Solver, material, fused=False, device="hip")
+    ora = _mat_run(OracleSolver, material, fused=False)
+    assert np.abs(hip.x() - ora.x()).max() <= 1e-4 * 0.01
+
+
+PROF = GlassProfile(outer_radius=0.05, inner_radius=0.04, height=0.12, base_thickness=0.02,
+                    fillet_radius=0.006)
+
+
+def _collider_run(cls, restrict=True, **kw):
+    grid, pts, vol = _blob(seed=3)
+    s = cls(grid=grid, **kw) if cls is Solver else cls(grid)
+    s.load_particles(pts, vol)
+    s.set_material(newtonian(eta=5.0, density=1000.0, bulk_modulus=5.0e5))
+    s.add_plane((0, 0, 3 * grid.dx), (0, 0, 1), "slip", friction=0.2)
+    cup = s.add_cup(PROF, np.array([0.2, 0.2, 0.2]))
+    s.set_cup(cup, center=(0.2, 0.2, 0.2), quat=(1, 0, 0, 0), velocity=(0.02, 0, 0),
+              omega=(0.3, 0.0, 1.0))
+    sdf = s.add_sdf_collider(sphere_sdf(0.03, res=24), center=(0.2, 0.26, 0.2),
+                             velocity=(0.0, -0.05, 0.0), omega=(0.0, 0.0, 2.0))
+    box = s.add_box((0.16, 0.2, 0.2), (0.02, 0.02, 0.06), velocity=(0.0, 0.0, -0.05))
+    s.add_domain_walls()
+    out = []
+    for _ in range(4):
+        for h in (cup, sdf):
+            s.reset_cup_wrench(h)
+        s.reset_tool_force(box)
+        s.step(2.0e-4, 6)
+        out.append((s.cup_wrench(cup, 1.2e-3), s.sdf_wrench(sdf, 1.2e-3), s.tool_force(box, 1.2e-3)))
+    return s, out
+
+
+def test_colliders_and_wrench_match_oracle():
+    hip, wh = _collider_run(Solver, device="hip")
+    ora, wo = _collider_run(OracleSolver)
+    assert np.abs(hip.x() - ora.x()).max() <= 1e-4 * 0.01
+    for (ch, sh, bh), (co, so, bo) in zip(wh, wo):
+        for a, b in ((ch["force"], co["force"]), (ch["torque"], co["torque"]),
+                     (sh["force"], so["force"]), (sh["torque"], so["torque"]), (bh, bo)):
+            assert np.abs(b).max() > 0
+            np.testing.assert_allclose(a, b, rtol=1e-3, atol=1e-4 * np.abs(b).max())
+
+
+def test_sparse_matches_dense_and_oracle():
+    def run(cls, sparse, **kw):
+        grid = GridConfig(n_grid=48, grid_lim=0.4)
+        h = grid.dx / 2
+        pts = []
+        for cx in (0.08, 0.32):
+            ax = np.arange(-0.015, 0.015, h)
+            g = np.stack(np.meshgrid(ax, ax, np.arange(0, 0.03, h), indexing="ij"), -1).reshape(-1, 3)
+            pts.append(g + [cx, 0.2, 0.06])
+        pts = np.concatenate(pts).astype(np.float32)
+        s = cls(grid=grid, sparse=sparse, **kw) if cls is Solver else cls(grid, sparse=sparse)
+        s.load_particles(pts, np.full(len(pts), h ** 3, np.float32))
+        s.set_material(newtonian(eta=5.0, density=1000.0, bulk_modulus=5.0e5))
+        s.add_plane((0, 0, 3 * grid.dx), (0, 0, 1), "slip", friction=0.2)
+        for _ in range(5):
+            s.step(2e-4, 3)
+        return s
+
+    hs = run(Solver, True, device="hip")
+    hd = run(Solver, False, device="hip")
+    os_ = run(OracleSolver, True)
+    dx = 0.4 / 48
+    assert np.abs(hs.x() - hd.x()).max() <= 1e-5 * dx
+    assert np.abs(hs.x() - os_.x()).max() <= 1e-4 * dx
+    assert hs._sim.active_block_fraction() == pytest.approx(os_._sim.active_block_fraction())
+
+
+def test_block_sort_permutation_is_exact():
+    grid, pts, vol = _blob(n=4000, seed=5)
+    pts = pts[np.random.default_rng(2).permutation(len(pts))]
+    hip = Solver(grid=grid, device="hip").load_particles(pts, vol)
+    ora = OracleSolver(grid).load_particles(pts, vol)
+    for s in (hip, ora):
+        s.set_material(newtonian(eta=2.0, density=1000.0, bulk_modulus=2.0e5))
+    assert hip._sort_particles() is True
+    assert ora._sort_particles() is True
+    np.testing.assert_array_equal(hip.x(), ora.x())
+    assert hip._sort_particles() is False
+    ptr = hip._sim.x.data_ptr()
+    hip.step(2e-4, 4)
+    assert hip._sim.x.data_ptr() == ptr
+
+
+def test_graph_replay_matches_eager():
+    def run(graph):
+        grid, pts, vol = _blob(seed=6)
+        s = Solver(grid=grid, device="hip").load_particles(pts, vol)
+        s._sim.use_graph = graph
+        s.set_material(newtonian(eta=5.0, density=1000.0, bulk_modulus=5.0e5))
+        s.add_plane((0, 0, 3 * grid.dx), (0, 0, 1), "separate", friction=0.3)
+        c = s.add_cup(PROF, np.array([0.2, 0.2, 0.2]), velocity=(0.01, 0, 0))
+        for _ in range(5):
+            s.step(2e-4, 8)
+        return s.x(), s.cup_wrench(c, 1.0)["force"]
+
+    xg, fg = run(True)
+    xe, fe = run(False)
+    assert np.abs(xg - xe).max() <= 1e-6
+    np.testing.assert_allclose(fg, fe, rtol=1e-4, atol=1e-6)
+
+
+def test_momentum_balance_full_pipeline():
+    """Size-independent invariant: with only impulse-accounted colliders, the particle
+    momentum change over a tick equals M g t minus the collider impulses (G2P and P2G
+    conserve momentum exactly up to round-off)."""
+    grid, pts, vol = _blob(n=20000, seed=7)
+    s = Solver(grid=grid, device="hip").load_particles(pts, vol)
+    s.set_material(newtonian(eta=5.0, density=1000.0, bulk_modulus=5.0e5))
+    box = s.add_box((0.2, 0.2, 0.16), (0.08, 0.08, 0.015), velocity=(0.0, 0.0, 0.0))
+    s.step(1e-4, 2)
+    m = s._sim.mass.cpu().numpy().astype(np.float64)
+    p0 = (m[:, None] * s.v()).sum(0)
+    s.reset_tool_force(box)
+    S, dt = 20, 1e-4
+    s.step(dt, S)
+    p1 = (m[:, None] * s.v()).sum(0)
+    imp = s.tool_force(box, 1.0)
+    expect = p0 + m.sum() * np.array([0, 0, -9.81]) * S * dt - imp
+    assert np.abs(p1 - expect).max() <= 1e-4 * m.sum() * 9.81 * S * dt
+
+
+def test_exports_R_and_cov_match_oracle():
+    grid, pts, vol = _blob(n=500, seed=8)
+    cov = np.tile(np.array([1e-6, 0, 0, 2e-6, 0, 3e-6], np.float32), (500, 1))
+    hip = Solver(grid=grid, device="hip").load_particles(pts, vol, cov=cov, cov_mode="from_F")
+    ora = OracleSolver(grid).load_particles(pts, vol, cov=cov, cov_mode="from_F")
+    for s in (hip, ora):
+        s.set_material(elastic(E=5e4, nu=0.3))
+        s.add_plane((0, 0, 0.19), (0, 0, 1), "sticky")
+        s.step(2e-4, 10)
+    ora._sim.compute_R()
+    np.testing.assert_allclose(hip.R(), np.transpose(ora._sim.R, (0, 2, 1)), atol=2e-5)
+    ora._sim.compute_cov()
+    np.testing.assert_allclose(hip.cov(), ora._sim.cov.reshape(-1, 6), rtol=1e-4, atol=1e-12)
+
+
+def test_grid_edge_raises_on_gpu():
+    grid = GridConfig(n_grid=32, grid_lim=0.4)
+    s = Solver(grid=grid, device="hip").load_particles(
+        np.array([[0.2, 0.2, 0.4 - 2 * grid.dx]], np.float32), np.full(1, 1e-6, np.float32))
+    s.set_material(newtonian(eta=5.0))
+    with pytest.raises(RuntimeError, match="out of bounds"):
+        s.step(2e-4, 1)
