@@ -42,6 +42,7 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-substeps", type=int, default=6)
     ap.add_argument("--graph", action="store_true", help="replay ticks as hipGraphs")
+    ap.add_argument("--sort-interval", type=int, default=1)
     return ap.parse_args()
 
 
@@ -78,7 +79,9 @@ def main():
     from mpm_amd.scenes import PourScene
 
     sc = PourScene()
-    s = sc.build(Solver(grid=sc.grid, device="cuda:0"))
+    # block sort at the start of every frame keeps each 256-particle chunk's stencil box
+    # inside the LDS arena of the particle pass
+    s = sc.build(Solver(grid=sc.grid, device="cuda:0", sort_interval=args.sort_interval))
     sim = s._sim
     sim.use_graph = bool(args.graph)
     n = s.n_particles
@@ -140,7 +143,7 @@ def main():
            "config": {"workload": "honey pour 192^3, eta 10 Pa s, 2 revolved-SDF glasses, "
                                   "source tilting 1 rad/s", "particles": n,
                       "substeps_per_step": sc.substeps, "dt": sc.dt, "grid": 192,
-                      "graph": bool(args.graph)},
+                      "graph": bool(args.graph), "sort_interval": args.sort_interval},
            "roofline": roof, "cpu_baseline": cpu}
     print(json.dumps(out))
 

@@ -43,6 +43,8 @@ typedef enum mpm_status {
 #define MPM_PASS_P2G 4          /* scatter: p2g_particle, mpm_utils.py:819-916 */
 #define MPM_PASS_LEAN 8         /* do not store v, C, L, F_trial (dead between fused substeps) */
 #define MPM_PASS_KEEP_STRESS 16 /* with LEAN: still store the stress */
+#define MPM_PASS_VISCOUS_ONLY 32 /* every particle is a viscous / rigid id (6, 7, 8, 10, 12):
+                                    launch the specialisation without the SVD paths */
 
 /* grid-update flag bits (mpm_grid_update) */
 #define MPM_GRID_ZERO_IN 1      /* clear node_in after it is consumed */
@@ -74,7 +76,8 @@ typedef struct mpm_grid {
     int32_t nb[3];        /* blocks per axis, ceil(n / 4) */
     float dx, inv_dx;     /* fp32 like the reference's MPMModelStruct.dx / inv_dx */
     int32_t *error_flag;  /* device int, OR-ed bits: 1 = a stencil left the grid,
-                             2 = a non-finite particle position */
+                             2 = a non-finite particle position, 4 = a solid material
+                             met a MPM_PASS_VISCOUS_ONLY pass */
 } mpm_grid_t;
 
 /* Model parameters (MPMModelStruct, warp_utils.py:23-84). Lives in device memory; the

@@ -428,3 +428,34 @@ MPM_DEV void stress_update(const m33& Ft, const m33& L, int mat, const mpm_model
     // enforce symmetry
     S = mscale(madd(S, mT(S)), 0.5f);
 }
+
+// ------------------------------------------------------------- layout helpers
+MPM_DEV int node_index(const mpm_grid_t& G, int i, int j, int k) {
+    return ((((i >> 2) * G.nb[1] + (j >> 2)) * G.nb[2] + (k >> 2)) << 6) + ((i & 3) << 4) +
+           ((j & 3) << 2) + (k & 3);
+}
+
+MPM_DEV void decode_block(const mpm_grid_t& G, int code, int& bi, int& bj, int& bk) {
+    bk = code % G.nb[2];
+    int r = code / G.nb[2];
+    bj = r % G.nb[1];
+    bi = r / G.nb[1];
+}
+
+MPM_DEV v3 ld3(const float* base, int stride, int p) {
+    return mk3(base[p], base[stride + p], base[2 * stride + p]);
+}
+MPM_DEV void st3(float* base, int stride, int p, v3 v) {
+    base[p] = v.x; base[stride + p] = v.y; base[2 * stride + p] = v.z;
+}
+MPM_DEV m33 ld9(const float* base, int stride, int p) {
+    m33 m;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) m.a[i] = base[i * stride + p];
+    return m;
+}
+MPM_DEV void st9(float* base, int stride, int p, const m33& m) {
+#pragma unroll
+    for (int i = 0; i < 9; ++i) base[i * stride + p] = m.a[i];
+}
+

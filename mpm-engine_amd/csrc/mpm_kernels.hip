@@ -11,210 +11,6 @@ namespace {
 
 constexpr int kThreads = 256;
 
-MPM_DEV int node_index(const mpm_grid_t& G, int i, int j, int k) {
-    return ((((i >> 2) * G.nb[1] + (j >> 2)) * G.nb[2] + (k >> 2)) << 6) + ((i & 3) << 4) +
-           ((j & 3) << 2) + (k & 3);
-}
-
-MPM_DEV void decode_block(const mpm_grid_t& G, int code, int& bi, int& bj, int& bk) {
-    bk = code % G.nb[2];
-    int r = code / G.nb[2];
-    bj = r % G.nb[1];
-    bi = r / G.nb[1];
-}
-
-MPM_DEV v3 ld3(const float* base, int stride, int p) {
-    return mk3(base[p], base[stride + p], base[2 * stride + p]);
-}
-MPM_DEV void st3(float* base, int stride, int p, v3 v) {
-    base[p] = v.x; base[stride + p] = v.y; base[2 * stride + p] = v.z;
-}
-MPM_DEV m33 ld9(const float* base, int stride, int p) {
-    m33 m;
-#pragma unroll
-    for (int i = 0; i < 9; ++i) m.a[i] = base[i * stride + p];
-    return m;
-}
-MPM_DEV void st9(float* base, int stride, int p, const m33& m) {
-#pragma unroll
-    for (int i = 0; i < 9; ++i) base[i * stride + p] = m.a[i];
-}
-
-// ------------------------------------------------------------------ particle pass
-// g2p_particle (mpm_utils.py:944-1043), stress_update_particle (:1056-1149) and
-// p2g_particle (:819-916) as one per-particle pass; MODE selects the stages.
-template <int MODE>
-__global__ __launch_bounds__(kThreads) void k_particle_pass(mpm_particles_t P, mpm_grid_t G,
-                                                            const mpm_model_t* __restrict__ md,
-                                                            float dt) {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= P.n) return;
-    if (P.selection[p] != 0) return;
-    const int S = P.stride;
-    const int mat = P.material[p];
-    constexpr bool kG2P = (MODE & MPM_PASS_G2P) != 0;
-    constexpr bool kStress = (MODE & MPM_PASS_STRESS) != 0;
-    constexpr bool kP2G = (MODE & MPM_PASS_P2G) != 0;
-    constexpr bool kLean = (MODE & MPM_PASS_LEAN) != 0;
-    constexpr bool kKeepStress = (MODE & MPM_PASS_KEEP_STRESS) != 0;
-    const int periodic = md->periodic_x;
-
-    v3 x = ld3(P.x, S, p);
-    v3 v = mk3(0.f, 0.f, 0.f);
-    m33 Cm = mzero(), L = mzero(), Ft = mzero();
-    const bool gather = kG2P && mat != 7 && mat != 8;
-    if (kG2P && !gather && !kStress && !kP2G) return;
-    int err = 0;
-
-    if (gather) {
-        Stencil st = make_stencil(x, G.inv_dx);
-        v3 nv = mk3(0.f, 0.f, 0.f);
-        m33 nC = mzero(), nF = mzero();
-        const float4* __restrict__ out = reinterpret_cast<const float4*>(G.node_out);
-#pragma unroll
-        for (int i = 0; i < 3; ++i) {
-            int ix = st.base[0] + i;
-            if (periodic) ix = ((ix % G.n[0]) + G.n[0]) % G.n[0];
-#pragma unroll
-            for (int j = 0; j < 3; ++j) {
-                int iy = st.base[1] + j;
-#pragma unroll
-                for (int k = 0; k < 3; ++k) {
-                    int iz = st.base[2] + k;
-                    float4 gv4 = make_float4(0.f, 0.f, 0.f, 0.f);
-                    if ((unsigned)ix < (unsigned)G.n[0] && (unsigned)iy < (unsigned)G.n[1] &&
-                        (unsigned)iz < (unsigned)G.n[2])
-                        gv4 = out[node_index(G, ix, iy, iz)];
-                    else
-                        err = 1;
-                    v3 gv = mk3(gv4.x, gv4.y, gv4.z);
-                    float weight = st.w[0][i] * st.w[1][j] * st.w[2][k];
-                    nv = nv + gv * weight;
-                    v3 dpos = mk3((float)i - st.fx[0], (float)j - st.fx[1], (float)k - st.fx[2]);
-                    float cw = weight * G.inv_dx * 4.0f;
-                    v3 dwt = mk3(st.dw[0][i] * st.w[1][j] * st.w[2][k], st.w[0][i] * st.dw[1][j] * st.w[2][k],
-                                 st.w[0][i] * st.w[1][j] * st.dw[2][k]) * G.inv_dx;
-                    float g[3] = {gv.x, gv.y, gv.z};
-                    float dp[3] = {dpos.x, dpos.y, dpos.z};
-                    float dw3[3] = {dwt.x, dwt.y, dwt.z};
-#pragma unroll
-                    for (int r = 0; r < 3; ++r)
-#pragma unroll
-                        for (int c = 0; c < 3; ++c) {
-                            nC.a[3 * r + c] += g[r] * dp[c] * cw;
-                            nF.a[3 * r + c] += g[r] * dw3[c];
-                        }
-                }
-            }
-        }
-        v = nv;
-        Cm = nC;
-        L = nF;
-        x = x + v * dt;
-        if (periodic) {
-            float lx = (float)G.n[0] * G.dx;
-            if (x.x < 0.0f) x.x = x.x + lx;
-            if (x.x >= lx) x.x = x.x - lx;
-        }
-        m33 F = ld9(P.F, S, p);
-        m33 A = mscale(nF, dt);
-        A.a[0] += 1.f; A.a[4] += 1.f; A.a[8] += 1.f;
-        Ft = mmul(A, F);
-        if (md->update_cov_with_F && P.cov) {  // update_cov, mpm_utils.py:613-633
-            float c[6];
-#pragma unroll
-            for (int q = 0; q < 6; ++q) c[q] = P.cov[q * S + p];
-            m33 cn;
-            cn.a[0] = c[0]; cn.a[1] = c[1]; cn.a[2] = c[2];
-            cn.a[3] = c[1]; cn.a[4] = c[3]; cn.a[5] = c[4];
-            cn.a[6] = c[2]; cn.a[7] = c[4]; cn.a[8] = c[5];
-            m33 n1 = madd(cn, mscale(madd(mmul(nF, cn), mmul(cn, mT(nF))), dt));
-            P.cov[p] = n1.a[0]; P.cov[S + p] = n1.a[1]; P.cov[2 * S + p] = n1.a[2];
-            P.cov[3 * S + p] = n1.a[4]; P.cov[4 * S + p] = n1.a[5]; P.cov[5 * S + p] = n1.a[8];
-        }
-        st3(P.x, S, p, x);
-        if (!kLean) {
-            st3(P.v, S, p, v);
-            st9(P.C, S, p, Cm);
-            st9(P.L, S, p, L);
-            st9(P.F_trial, S, p, Ft);
-        }
-    } else {
-        if (kP2G) {
-            v = ld3(P.v, S, p);
-            Cm = ld9(P.C, S, p);
-        }
-        if (kStress) {
-            Ft = ld9(P.F_trial, S, p);
-            if (mat == 10 || mat == 12) L = ld9(P.L, S, p);
-        }
-    }
-
-    m33 stress;
-    if (kStress) {
-        PState ps;
-        ps.mu = P.mu[p];
-        ps.lam = P.lam[p];
-        ps.yield = P.yield_stress[p];
-        ps.Jp = P.Jp[p];
-        ps.dirty_mu = ps.dirty_yield = ps.dirty_Jp = false;
-        m33 F;
-        stress_update(Ft, L, mat, md, dt, ps, F, stress);
-        st9(P.F, S, p, F);
-        if (!kLean || kKeepStress) st9(P.stress, S, p, stress);
-        if (ps.dirty_yield) P.yield_stress[p] = ps.yield;
-        if (ps.dirty_mu) { P.mu[p] = ps.mu; P.lam[p] = ps.lam; }
-        if (ps.dirty_Jp) P.Jp[p] = ps.Jp;
-    } else if (kP2G) {
-        stress = ld9(P.stress, S, p);
-    }
-
-    if (kP2G) {
-        Stencil st = make_stencil(x, G.inv_dx);
-        const float r = md->rpic_damping;
-        if (r != 0.0f) {
-            m33 Ct = mT(Cm);
-            Cm = madd(mscale(Cm, 1.0f - r), mscale(msub(Cm, Ct), r / 2.0f));
-            if (r < -0.001f) Cm = mzero();
-        }
-        const float vol = P.vol[p], mass = P.mass[p];
-        const m33 fs = mscale(stress, -vol * dt);
-        float* __restrict__ gin = G.node_in;
-#pragma unroll
-        for (int i = 0; i < 3; ++i) {
-            int ix = st.base[0] + i;
-            if (periodic) ix = ((ix % G.n[0]) + G.n[0]) % G.n[0];
-#pragma unroll
-            for (int j = 0; j < 3; ++j) {
-                int iy = st.base[1] + j;
-#pragma unroll
-                for (int k = 0; k < 3; ++k) {
-                    int iz = st.base[2] + k;
-                    if (!((unsigned)ix < (unsigned)G.n[0] && (unsigned)iy < (unsigned)G.n[1] &&
-                          (unsigned)iz < (unsigned)G.n[2])) {
-                        err = 1;
-                        continue;
-                    }
-                    float weight = st.w[0][i] * st.w[1][j] * st.w[2][k];
-                    v3 dpos = mk3(((float)i - st.fx[0]) * G.dx, ((float)j - st.fx[1]) * G.dx,
-                                  ((float)k - st.fx[2]) * G.dx);
-                    v3 dwt = mk3(st.dw[0][i] * st.w[1][j] * st.w[2][k], st.w[0][i] * st.dw[1][j] * st.w[2][k],
-                                 st.w[0][i] * st.w[1][j] * st.dw[2][k]) * G.inv_dx;
-                    float wm = weight * mass;
-                    v3 add = (v + mvmul(Cm, dpos)) * wm + mvmul(fs, dwt);
-                    size_t o = 4 * (size_t)node_index(G, ix, iy, iz);
-                    atomicAdd(gin + o + 0, add.x);
-                    atomicAdd(gin + o + 1, add.y);
-                    atomicAdd(gin + o + 2, add.z);
-                    atomicAdd(gin + o + 3, wm);
-                }
-            }
-        }
-    }
-    if (!(x.x == x.x && x.y == x.y && x.z == x.z)) err |= 2;
-    if (err && G.error_flag) atomicOr(G.error_flag, err);
-}
-
 // ------------------------------------------------------------------------ colliders
 // capped_cylinder_sdf / revolved_glass_sdf, mpm_solver_warp.py:29-57
 MPM_DEV float capped_cylinder_sdf(float r, float z, float radius, float z0, float z1) {
@@ -693,28 +489,6 @@ int32_t mpm_struct_size(int32_t which) {
         case 4: return (int32_t)sizeof(mpm_tick_ctl_t);
         default: return -1;
     }
-}
-
-int32_t mpm_particle_pass(const mpm_particles_t* P, const mpm_grid_t* G, const mpm_model_t* model, float dt,
-                          int32_t mode, void* stream) {
-    if (!particles_ok(P) || !grid_ok(G) || !model) return MPM_ERR_ARG;
-    if (P->n == 0) return MPM_OK;
-    hipStream_t s = (hipStream_t)stream;
-    dim3 grid(blocks_for(P->n)), block(kThreads);
-    const int m = mode & (MPM_PASS_G2P | MPM_PASS_STRESS | MPM_PASS_P2G | MPM_PASS_LEAN | MPM_PASS_KEEP_STRESS);
-    switch (m) {
-#define CASE(M) case (M): hipLaunchKernelGGL(k_particle_pass<(M)>, grid, block, 0, s, *P, *G, model, dt); break;
-        CASE(MPM_PASS_G2P)
-        CASE(MPM_PASS_STRESS)
-        CASE(MPM_PASS_P2G)
-        CASE(MPM_PASS_STRESS | MPM_PASS_P2G)
-        CASE(MPM_PASS_G2P | MPM_PASS_STRESS | MPM_PASS_P2G)
-        CASE(MPM_PASS_G2P | MPM_PASS_STRESS | MPM_PASS_P2G | MPM_PASS_LEAN)
-        CASE(MPM_PASS_G2P | MPM_PASS_STRESS | MPM_PASS_P2G | MPM_PASS_LEAN | MPM_PASS_KEEP_STRESS)
-#undef CASE
-        default: return MPM_ERR_ARG;
-    }
-    return launch_status();
 }
 
 int32_t mpm_grid_zero_box(const mpm_grid_t* G, const int32_t* lo, const int32_t* hi, int32_t which, void* stream) {

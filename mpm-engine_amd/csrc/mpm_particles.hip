@@ -1,0 +1,466 @@
+// The fused particle pass: gather (g2p_particle, mpm_utils.py:944-1043), return map +
+// Kirchhoff stress (stress_update_particle, :1056-1149) and scatter (p2g_particle,
+// :819-916) for 256 particles per workgroup.
+//
+// LDS staging (claymore-style, on block-sorted particles): the workgroup reduces the
+// stencil bounding box of its 256 particles; when the box holds at most kArena nodes it
+// loads the gather box of node_out into LDS once (one 16-byte read per node instead of
+// 27 per particle) and scatters into an LDS arena with ds_add_f32, then flushes each
+// touched arena node to node_in with four global float atomics. Chunks whose box is
+// larger (unsorted or scattered particles) fall back to direct global gathers and
+// atomics, so results never depend on the particle order. Everything that decides a
+// barrier is reduced first, so every branch around __syncthreads is workgroup-uniform.
+#include <hip/hip_runtime.h>
+#include <limits.h>
+
+#include "mpm_device.h"
+
+namespace {
+
+constexpr int kT = 256;        // particles (threads) per workgroup
+constexpr int kArena = 768;    // LDS arena capacity in nodes (12 KB gather + 12 KB scatter)
+
+// block-wide bounding box of per-thread [lo, hi); every thread receives the result
+MPM_DEV void reduce_box(int lo[3], int hi[3], int (*red)[6], int out[6]) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1)
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            lo[a] = min(lo[a], __shfl_xor(lo[a], off, 64));
+            hi[a] = max(hi[a], __shfl_xor(hi[a], off, 64));
+        }
+    const int wave = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0)
+        for (int a = 0; a < 3; ++a) {
+            red[wave][a] = lo[a];
+            red[wave][3 + a] = hi[a];
+        }
+    __syncthreads();
+    for (int a = 0; a < 3; ++a) {
+        int l = red[0][a], h = red[0][3 + a];
+        for (int w = 1; w < kT / 64; ++w) {
+            l = min(l, red[w][a]);
+            h = max(h, red[w][3 + a]);
+        }
+        out[a] = l;
+        out[3 + a] = h;
+    }
+}
+
+MPM_DEV bool box_fits(const int b[6], int& d0, int& d1, int& d2) {
+    d0 = d1 = d2 = 0;
+    if (b[3] <= b[0] || b[4] <= b[1] || b[5] <= b[2]) return false;  // empty (sentinels)
+    d0 = b[3] - b[0];
+    d1 = b[4] - b[1];
+    d2 = b[5] - b[2];
+    if (d0 <= 0 || d1 <= 0 || d2 <= 0 || d0 > kArena || d1 > kArena || d2 > kArena) return false;
+    return d0 * d1 * d2 <= kArena;
+}
+
+MPM_DEV bool in_grid(const mpm_grid_t& G, int i, int j, int k) {
+    return (unsigned)i < (unsigned)G.n[0] && (unsigned)j < (unsigned)G.n[1] && (unsigned)k < (unsigned)G.n[2];
+}
+
+// Stress update restricted to the viscous / rigid ids (6, 7, 8, 10, 12): the SVD-free
+// branches of stress_update_particle. Other ids flag error bit 4 and keep F_trial.
+MPM_DEV bool stress_update_viscous(const m33& Ft, const m33& L, int mat, const mpm_model_t* md, PState& ps,
+                                   m33& F, m33& S) {
+    S = mzero();
+    if (mat == 6 || mat == 10 || mat == 12) {
+        float Jc = powf(mdet(Ft), 1.0f / 3.0f);
+        F = mdiag(Jc, Jc, Jc);
+        float J = mdet(F);
+        if (mat == 6) {
+            float pr = -md->bulk[mat] * (powf(J, -1.1f) - 1.0f);
+            S = mdiag(pr * J, pr * J, pr * J);
+        } else {
+            S = stress_viscous(J, md->bulk[mat], L, md->plastic_viscosity[mat], ps.yield, md->hardening[mat],
+                               md->softening[mat], md, mat == 12);
+        }
+        S = mscale(madd(S, mT(S)), 0.5f);
+        return true;
+    }
+    if (mat == 7 || mat == 8) {
+        F = meye();
+        return true;
+    }
+    F = Ft;
+    return false;
+}
+
+// Tensor-product form of the 27-node gather: per (i, j) row sum over k, then over j,
+// then over i. Same sums as the reference's triple loop (v = sum w v_i, C = sum w v_i
+// (x_i - x_p)^T 4/dx^2, L = sum v_i grad w^T) with fewer products and live values.
+// fetch(i, j, k) returns the node velocity for stencil offsets (i, j, k).
+template <class Fetch>
+MPM_DEV void gather_tp(const Stencil& st, float inv_dx, Fetch fetch, v3& nv, m33& nC, m33& nF) {
+    nv = mk3(0.f, 0.f, 0.f);
+    v3 cX = nv, cY = nv, cZ = nv;  // columns of C (by dpos component)
+    v3 fX = nv, fY = nv, fZ = nv;  // columns of L (by gradient component)
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        v3 Ai = mk3(0.f, 0.f, 0.f), Ay = Ai, Bi = Ai, Cz = Ai, Dy = Ai;
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            v3 A = mk3(0.f, 0.f, 0.f), B = A, Z = A;
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                const v3 g = fetch(i, j, k);
+                A = A + g * st.w[2][k];
+                B = B + g * st.dw[2][k];
+                Z = Z + g * (st.w[2][k] * ((float)k - st.fx[2]));
+            }
+            const float wj = st.w[1][j];
+            Ai = Ai + A * wj;
+            Ay = Ay + A * (wj * ((float)j - st.fx[1]));
+            Bi = Bi + B * wj;
+            Cz = Cz + Z * wj;
+            Dy = Dy + A * st.dw[1][j];
+        }
+        const float wi = st.w[0][i];
+        nv = nv + Ai * wi;
+        cX = cX + Ai * (wi * ((float)i - st.fx[0]));
+        cY = cY + Ay * wi;
+        cZ = cZ + Cz * wi;
+        fX = fX + Ai * st.dw[0][i];
+        fY = fY + Dy * wi;
+        fZ = fZ + Bi * wi;
+    }
+    const float c4 = inv_dx * 4.0f;
+    nC.a[0] = cX.x * c4; nC.a[1] = cY.x * c4; nC.a[2] = cZ.x * c4;
+    nC.a[3] = cX.y * c4; nC.a[4] = cY.y * c4; nC.a[5] = cZ.y * c4;
+    nC.a[6] = cX.z * c4; nC.a[7] = cY.z * c4; nC.a[8] = cZ.z * c4;
+    nF.a[0] = fX.x * inv_dx; nF.a[1] = fY.x * inv_dx; nF.a[2] = fZ.x * inv_dx;
+    nF.a[3] = fX.y * inv_dx; nF.a[4] = fY.y * inv_dx; nF.a[5] = fZ.y * inv_dx;
+    nF.a[6] = fX.z * inv_dx; nF.a[7] = fY.z * inv_dx; nF.a[8] = fZ.z * inv_dx;
+}
+
+// one particle's 27 scatter contributions: (v + C dpos) w m + fs grad w, fs = -vol dt tau;
+// add(i, j, k, value, mass) receives each node's deposit
+template <class Add>
+MPM_DEV void scatter_p(const Stencil& st, float dx, float inv_dx, float mass, v3 v, const m33& Cm,
+                       const m33& fs, Add add) {
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                const float w = st.w[0][i] * st.w[1][j] * st.w[2][k];
+                const v3 dpos = mk3(((float)i - st.fx[0]) * dx, ((float)j - st.fx[1]) * dx,
+                                    ((float)k - st.fx[2]) * dx);
+                const v3 dwt = mk3(st.dw[0][i] * st.w[1][j] * st.w[2][k], st.w[0][i] * st.dw[1][j] * st.w[2][k],
+                                   st.w[0][i] * st.w[1][j] * st.dw[2][k]) * inv_dx;
+                const float wm = w * mass;
+                add(i, j, k, (v + mvmul(Cm, dpos)) * wm + mvmul(fs, dwt), wm);
+            }
+}
+
+MPM_DEV float sel3(int i, const float* a) { return i == 0 ? a[0] : (i == 1 ? a[1] : a[2]); }
+
+// Global-memory gather / scatter for chunks whose stencil box does not fit the LDS arena
+// (unsorted or spread-out particles): a rolled loop over the 27 nodes with the weights
+// picked by selects, so this rare path stays small next to the LDS fast path it shares a
+// kernel (and a register budget) with.
+MPM_DEV void gather_global(const mpm_grid_t& G, const Stencil& st, bool periodic, v3& nv, m33& nC, m33& nF) {
+    const float4* __restrict__ out = reinterpret_cast<const float4*>(G.node_out);
+    nv = mk3(0.f, 0.f, 0.f);
+    nC = mzero();
+    nF = mzero();
+    const float c4 = G.inv_dx * 4.0f;
+#pragma unroll 1
+    for (int n = 0; n < 27; ++n) {
+        const int i = n / 9, j = (n / 3) % 3, k = n % 3;
+        int ix = st.base[0] + i;
+        if (periodic) ix = ((ix % G.n[0]) + G.n[0]) % G.n[0];
+        const int iy = st.base[1] + j, iz = st.base[2] + k;
+        if (!in_grid(G, ix, iy, iz)) continue;
+        const float4 g4 = out[node_index(G, ix, iy, iz)];
+        const float w0 = sel3(i, st.w[0]), w1 = sel3(j, st.w[1]), w2 = sel3(k, st.w[2]);
+        const float d0 = sel3(i, st.dw[0]), d1 = sel3(j, st.dw[1]), d2 = sel3(k, st.dw[2]);
+        const float w = w0 * w1 * w2;
+        const float g[3] = {g4.x, g4.y, g4.z};
+        const float dp[3] = {((float)i - st.fx[0]) * w * c4, ((float)j - st.fx[1]) * w * c4,
+                             ((float)k - st.fx[2]) * w * c4};
+        const float dw[3] = {d0 * w1 * w2 * G.inv_dx, w0 * d1 * w2 * G.inv_dx, w0 * w1 * d2 * G.inv_dx};
+        nv = nv + mk3(g4.x, g4.y, g4.z) * w;
+#pragma unroll
+        for (int r = 0; r < 3; ++r)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                nC.a[3 * r + c] += g[r] * dp[c];
+                nF.a[3 * r + c] += g[r] * dw[c];
+            }
+    }
+}
+
+MPM_DEV void scatter_global(const mpm_grid_t& G, const Stencil& st, bool periodic, float mass, v3 v,
+                            const m33& Cm, const m33& fs) {
+    float* __restrict__ gin = G.node_in;
+#pragma unroll 1
+    for (int n = 0; n < 27; ++n) {
+        const int i = n / 9, j = (n / 3) % 3, k = n % 3;
+        int ix = st.base[0] + i;
+        if (periodic) ix = ((ix % G.n[0]) + G.n[0]) % G.n[0];
+        const int iy = st.base[1] + j, iz = st.base[2] + k;
+        if (!in_grid(G, ix, iy, iz)) continue;
+        const float w0 = sel3(i, st.w[0]), w1 = sel3(j, st.w[1]), w2 = sel3(k, st.w[2]);
+        const float d0 = sel3(i, st.dw[0]), d1 = sel3(j, st.dw[1]), d2 = sel3(k, st.dw[2]);
+        const float wm = w0 * w1 * w2 * mass;
+        const v3 dpos = mk3(((float)i - st.fx[0]) * G.dx, ((float)j - st.fx[1]) * G.dx, ((float)k - st.fx[2]) * G.dx);
+        const v3 dwt = mk3(d0 * w1 * w2, w0 * d1 * w2, w0 * w1 * d2) * G.inv_dx;
+        const v3 add = (v + mvmul(Cm, dpos)) * wm + mvmul(fs, dwt);
+        const size_t o = 4 * (size_t)node_index(G, ix, iy, iz);
+        atomicAdd(gin + o + 0, add.x);
+        atomicAdd(gin + o + 1, add.y);
+        atomicAdd(gin + o + 2, add.z);
+        atomicAdd(gin + o + 3, wm);
+    }
+}
+
+MPM_DEV bool stencil_in_grid(const mpm_grid_t& G, const Stencil& st, bool periodic) {
+    bool ok = true;
+#pragma unroll
+    for (int a = periodic ? 1 : 0; a < 3; ++a) ok = ok && st.base[a] >= 0 && st.base[a] + 2 < G.n[a];
+    return ok;
+}
+
+template <int MODE, bool kSolid>
+__global__ __launch_bounds__(kT) void k_particle_pass(mpm_particles_t P, mpm_grid_t G,
+                                                      const mpm_model_t* __restrict__ md, float dt) {
+    constexpr bool kG2P = (MODE & MPM_PASS_G2P) != 0;
+    constexpr bool kStress = (MODE & MPM_PASS_STRESS) != 0;
+    constexpr bool kP2G = (MODE & MPM_PASS_P2G) != 0;
+    constexpr bool kLean = (MODE & MPM_PASS_LEAN) != 0;
+    constexpr bool kKeepStress = (MODE & MPM_PASS_KEEP_STRESS) != 0;
+    __shared__ float4 s_gather[kG2P ? kArena : 1];
+    __shared__ float s_scatter[kP2G ? 4 : 1][kP2G ? kArena : 1];
+    __shared__ int s_red0[kT / 64][6];
+    __shared__ int s_red1[kT / 64][6];
+
+    const int S = P.stride;
+    const int p = blockIdx.x * kT + threadIdx.x;
+    const bool alive = p < P.n && P.selection[p] == 0;
+    const int mat = alive ? P.material[p] : 0;
+    const bool periodic = md->periodic_x != 0;
+    int err = 0;
+
+    v3 x = alive ? ld3(P.x, S, p) : mk3(0.f, 0.f, 0.f);
+    v3 v = mk3(0.f, 0.f, 0.f);
+    m33 Cm = mzero(), L = mzero(), Ft = mzero();
+
+    if constexpr (kG2P) {
+        const bool gather = alive && mat != 7 && mat != 8;
+        const Stencil st = make_stencil(x, G.inv_dx);
+        int lo[3], hi[3];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            lo[a] = gather ? st.base[a] : INT_MAX;
+            hi[a] = gather ? st.base[a] + 3 : INT_MIN;
+        }
+        int bx[6], d0, d1, d2;
+        reduce_box(lo, hi, s_red0, bx);
+        const bool arena = !periodic && box_fits(bx, d0, d1, d2);
+        if (arena) {
+            const float4* __restrict__ out = reinterpret_cast<const float4*>(G.node_out);
+            const int vol = d0 * d1 * d2;
+            for (int t = threadIdx.x; t < vol; t += kT) {
+                int k = t % d2, r = t / d2;
+                int j = r % d1, i = r / d1;
+                int gi = bx[0] + i, gj = bx[1] + j, gk = bx[2] + k;
+                s_gather[t] = in_grid(G, gi, gj, gk) ? out[node_index(G, gi, gj, gk)] : make_float4(0.f, 0.f, 0.f, 0.f);
+            }
+            __syncthreads();
+        }
+        if (gather) {
+            if (!stencil_in_grid(G, st, periodic)) err |= 1;
+            v3 nv;
+            m33 nC, nF;
+            if (arena) {
+                const int o0 = st.base[0] - bx[0], o1 = st.base[1] - bx[1], o2 = st.base[2] - bx[2];
+                gather_tp(st, G.inv_dx,
+                          [&](int i, int j, int k) {
+                              const float4 g = s_gather[((o0 + i) * d1 + (o1 + j)) * d2 + (o2 + k)];
+                              return mk3(g.x, g.y, g.z);
+                          },
+                          nv, nC, nF);
+            } else {
+                gather_global(G, st, periodic, nv, nC, nF);
+            }
+            v = nv;
+            Cm = nC;
+            L = nF;
+            x = x + v * dt;
+            if (periodic) {
+                const float lx = (float)G.n[0] * G.dx;
+                if (x.x < 0.0f) x.x = x.x + lx;
+                if (x.x >= lx) x.x = x.x - lx;
+            }
+            m33 A = mscale(nF, dt);
+            A.a[0] += 1.f; A.a[4] += 1.f; A.a[8] += 1.f;
+            Ft = mmul(A, ld9(P.F, S, p));
+            if (md->update_cov_with_F && P.cov) {  // update_cov, mpm_utils.py:613-633
+                float c[6];
+#pragma unroll
+                for (int q = 0; q < 6; ++q) c[q] = P.cov[q * S + p];
+                m33 cn;
+                cn.a[0] = c[0]; cn.a[1] = c[1]; cn.a[2] = c[2];
+                cn.a[3] = c[1]; cn.a[4] = c[3]; cn.a[5] = c[4];
+                cn.a[6] = c[2]; cn.a[7] = c[4]; cn.a[8] = c[5];
+                m33 n1 = madd(cn, mscale(madd(mmul(nF, cn), mmul(cn, mT(nF))), dt));
+                P.cov[p] = n1.a[0]; P.cov[S + p] = n1.a[1]; P.cov[2 * S + p] = n1.a[2];
+                P.cov[3 * S + p] = n1.a[4]; P.cov[4 * S + p] = n1.a[5]; P.cov[5 * S + p] = n1.a[8];
+            }
+            st3(P.x, S, p, x);
+            if (!kLean) {
+                st3(P.v, S, p, v);
+                st9(P.C, S, p, Cm);
+                st9(P.L, S, p, L);
+                st9(P.F_trial, S, p, Ft);
+            }
+        } else if (alive) {
+            if (kP2G) {
+                v = ld3(P.v, S, p);
+                Cm = ld9(P.C, S, p);
+            }
+            if (kStress) {
+                Ft = ld9(P.F_trial, S, p);
+                if (mat == 10 || mat == 12) L = ld9(P.L, S, p);
+            }
+        }
+    } else if (alive) {
+        if (kP2G) {
+            v = ld3(P.v, S, p);
+            Cm = ld9(P.C, S, p);
+        }
+        if (kStress) {
+            Ft = ld9(P.F_trial, S, p);
+            if (mat == 10 || mat == 12) L = ld9(P.L, S, p);
+        }
+    }
+
+    m33 stress = mzero();
+    if constexpr (kStress) {
+        if (alive) {
+            PState ps;
+            ps.mu = P.mu[p];
+            ps.lam = P.lam[p];
+            ps.yield = P.yield_stress[p];
+            ps.Jp = P.Jp[p];
+            ps.dirty_mu = ps.dirty_yield = ps.dirty_Jp = false;
+            m33 F;
+            if constexpr (kSolid) {
+                stress_update(Ft, L, mat, md, dt, ps, F, stress);
+            } else {
+                if (!stress_update_viscous(Ft, L, mat, md, ps, F, stress)) err |= 4;
+            }
+            st9(P.F, S, p, F);
+            if (!kLean || kKeepStress) st9(P.stress, S, p, stress);
+            if (ps.dirty_yield) P.yield_stress[p] = ps.yield;
+            if (ps.dirty_mu) { P.mu[p] = ps.mu; P.lam[p] = ps.lam; }
+            if (ps.dirty_Jp) P.Jp[p] = ps.Jp;
+        }
+    } else if (kP2G && alive) {
+        stress = ld9(P.stress, S, p);
+    }
+
+    if constexpr (kP2G) {
+        const Stencil st = make_stencil(x, G.inv_dx);
+        int lo[3], hi[3];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            lo[a] = alive ? st.base[a] : INT_MAX;
+            hi[a] = alive ? st.base[a] + 3 : INT_MIN;
+        }
+        int bx[6], d0, d1, d2;
+        reduce_box(lo, hi, s_red1, bx);
+        const bool arena = !periodic && box_fits(bx, d0, d1, d2);
+        const int vol = arena ? d0 * d1 * d2 : 0;
+        if (arena) {
+            for (int t = threadIdx.x; t < vol; t += kT) {
+                s_scatter[0][t] = 0.f;
+                s_scatter[1][t] = 0.f;
+                s_scatter[2][t] = 0.f;
+                s_scatter[3][t] = 0.f;
+            }
+            __syncthreads();
+        }
+        if (alive) {
+            const float r = md->rpic_damping;
+            if (r != 0.0f) {
+                const m33 Ct = mT(Cm);
+                Cm = madd(mscale(Cm, 1.0f - r), mscale(msub(Cm, Ct), r / 2.0f));
+                if (r < -0.001f) Cm = mzero();
+            }
+            const m33 fs = mscale(stress, -P.vol[p] * dt);
+            if (!stencil_in_grid(G, st, periodic)) err |= 1;
+            if (arena) {
+                const int o0 = st.base[0] - bx[0], o1 = st.base[1] - bx[1], o2 = st.base[2] - bx[2];
+                scatter_p(st, G.dx, G.inv_dx, P.mass[p], v, Cm, fs, [&](int i, int j, int k, v3 add, float wm) {
+                    const int a = ((o0 + i) * d1 + (o1 + j)) * d2 + (o2 + k);
+                    atomicAdd(&s_scatter[0][a], add.x);
+                    atomicAdd(&s_scatter[1][a], add.y);
+                    atomicAdd(&s_scatter[2][a], add.z);
+                    atomicAdd(&s_scatter[3][a], wm);
+                });
+            } else {
+                scatter_global(G, st, periodic, P.mass[p], v, Cm, fs);
+            }
+        }
+        if (arena) {
+            __syncthreads();
+            float* __restrict__ gin = G.node_in;
+            for (int t = threadIdx.x; t < vol; t += kT) {
+                const float a0 = s_scatter[0][t], a1 = s_scatter[1][t], a2 = s_scatter[2][t], a3 = s_scatter[3][t];
+                if (a0 == 0.f && a1 == 0.f && a2 == 0.f && a3 == 0.f) continue;
+                int k = t % d2, rr = t / d2;
+                int j = rr % d1, i = rr / d1;
+                const int gi = bx[0] + i, gj = bx[1] + j, gk = bx[2] + k;
+                if (!in_grid(G, gi, gj, gk)) continue;
+                const size_t o = 4 * (size_t)node_index(G, gi, gj, gk);
+                atomicAdd(gin + o + 0, a0);
+                atomicAdd(gin + o + 1, a1);
+                atomicAdd(gin + o + 2, a2);
+                atomicAdd(gin + o + 3, a3);
+            }
+        }
+    }
+    if (alive && !(x.x == x.x && x.y == x.y && x.z == x.z)) err |= 2;
+    if (err && G.error_flag) atomicOr(G.error_flag, err);
+}
+
+inline int32_t launch_status() { return hipGetLastError() == hipSuccess ? MPM_OK : MPM_ERR_HIP; }
+
+}  // namespace
+
+extern "C" int32_t mpm_particle_pass(const mpm_particles_t* P, const mpm_grid_t* G, const mpm_model_t* model,
+                                     float dt, int32_t mode, void* stream) {
+    if (!P || !G || !model || P->n < 0 || P->stride < P->n || !P->x || !P->v || !P->F || !P->F_trial || !P->C ||
+        !P->stress || !P->L || !P->vol || !P->mass || !P->mu || !P->lam || !P->yield_stress || !P->Jp ||
+        !P->material || !P->selection || !G->node_in || !G->node_out || G->n[0] <= 0 || G->n[1] <= 0 ||
+        G->n[2] <= 0 || G->nb[0] != (G->n[0] + 3) / 4 || G->nb[1] != (G->n[1] + 3) / 4 ||
+        G->nb[2] != (G->n[2] + 3) / 4)
+        return MPM_ERR_ARG;
+    if (P->n == 0) return MPM_OK;
+    hipStream_t s = (hipStream_t)stream;
+    const dim3 grid((unsigned)((P->n + kT - 1) / kT)), block(kT);
+    const bool solid = (mode & MPM_PASS_VISCOUS_ONLY) == 0;
+    const int m = mode & (MPM_PASS_G2P | MPM_PASS_STRESS | MPM_PASS_P2G | MPM_PASS_LEAN | MPM_PASS_KEEP_STRESS);
+#define CASE(M)                                                                                   \
+    case (M):                                                                                     \
+        if (solid) hipLaunchKernelGGL((k_particle_pass<(M), true>), grid, block, 0, s, *P, *G, model, dt); \
+        else hipLaunchKernelGGL((k_particle_pass<(M), false>), grid, block, 0, s, *P, *G, model, dt);     \
+        break;
+    switch (m) {
+        CASE(MPM_PASS_G2P)
+        CASE(MPM_PASS_STRESS)
+        CASE(MPM_PASS_P2G)
+        CASE(MPM_PASS_STRESS | MPM_PASS_P2G)
+        CASE(MPM_PASS_G2P | MPM_PASS_STRESS | MPM_PASS_P2G)
+        CASE(MPM_PASS_G2P | MPM_PASS_STRESS | MPM_PASS_P2G | MPM_PASS_LEAN)
+        CASE(MPM_PASS_G2P | MPM_PASS_STRESS | MPM_PASS_P2G | MPM_PASS_LEAN | MPM_PASS_KEEP_STRESS)
+        default: return MPM_ERR_ARG;
+    }
+#undef CASE
+    return launch_status();
+}

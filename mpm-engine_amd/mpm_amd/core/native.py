@@ -16,7 +16,8 @@ LIB_PATH = os.path.join(_PKG_ROOT, "lib", "libmpm_hip.so")
 MAX_MATERIALS = 16
 
 # mode / flag bits, mirrored from include/mpm_hip.h
-PASS_G2P, PASS_STRESS, PASS_P2G, PASS_LEAN, PASS_KEEP_STRESS = 1, 2, 4, 8, 16
+PASS_G2P, PASS_STRESS, PASS_P2G, PASS_LEAN, PASS_KEEP_STRESS, PASS_VISCOUS_ONLY = 1, 2, 4, 8, 16, 32
+VISCOUS_IDS = frozenset((6, 7, 8, 10, 12))
 GRID_ZERO_IN = 1
 COLLIDER_PLANE, COLLIDER_BOX, COLLIDER_CUP, COLLIDER_WALLS, COLLIDER_SDF, COLLIDER_POINTCLOUD = \
     1, 2, 3, 4, 5, 6

@@ -148,6 +148,7 @@ class HipSimulator:
         self.model_dev = torch.zeros(native.MODEL_DTYPE.itemsize, dtype=torch.uint8, device=dev)
         self._model_dirty = True
         self.material_id = 0
+        self._mat_ids = {0}  # material ids present (selects the pass specialisation)
         # colliders
         self.colliders = ColliderSet(self.n_grid, self.dx)
         self.wrench = t(6 * 64)
@@ -222,6 +223,7 @@ class HipSimulator:
         if "material" in kw:
             self.material_id = self._material_name_to_id(kw["material"])
             self.material.fill_(self.material_id)
+            self._mat_ids = {self.material_id}
         # the reference reallocates (zeroes) the grid on every call
         self.node_in.zero_()
         self.node_out.zero_()
@@ -267,6 +269,7 @@ class HipSimulator:
         if "material" in kw:
             m = self._material_name_to_id(kw["material"])
             self.material[a:b] = m
+            self._mat_ids.add(m)
         if m is None:
             m = self.material_id
         self._set_type_params(m, kw)
@@ -464,6 +467,8 @@ class HipSimulator:
         self._graph = g
 
     def _pass(self, mode, dt, s):
+        if self._mat_ids <= native.VISCOUS_IDS:
+            mode |= native.PASS_VISCOUS_ONLY
         native.check(self.lib.mpm_particle_pass(self.pP, self.pG, native.ptr(self.model_dev), dt, mode, s),
                      "particle_pass")
 
@@ -871,6 +876,8 @@ class Solver:
         lo_x, hi_x, vmax, bad, flag = self._sim.read_bounds()
         dx = self.grid.dx
         lim = self.grid.grid_lim
+        if flag & 4:
+            raise RuntimeError("internal: a solid material ran in the viscous-only particle pass")
         if flag & 1:
             raise RuntimeError("a particle stencil left the grid during the last tick (out of "
                                "bounds P2G was skipped); enlarge grid_lim or add a wall collider")

@@ -51,30 +51,42 @@ def _c1(solver_cls, **kw):
 
 
 def test_c1_grid_after_one_substep():
-    """Grid mass and momentum after the first substep's scatter, 1e-5 relative."""
+    """Grid mass and momentum after one substep's scatter, within 1e-5 relative, from a
+    non-trivial seeded state of the C1 cube (random velocity, affine C and deformation F,
+    so the momentum, APIC and stress terms all contribute)."""
     from oracle import sim as O
 
     hip = _c1(Solver, device="hip")
     ora = _c1(OracleSolver)
-    hip.step(1e-4, 1)
-    ora.step(1e-4, 1)
-    # state after the substep's scatter: rerun the prologue scatter on both
+    n = hip.n_particles
+    rng = np.random.default_rng(11)
+    v = rng.uniform(-0.1, 0.1, (n, 3)).astype(np.float32)
+    Cm = rng.uniform(-1.0, 1.0, (n, 3, 3)).astype(np.float32)
+    Ft = (np.eye(3) + 0.05 * rng.uniform(-1, 1, (n, 3, 3))).astype(np.float32)
+    import torch
     sim = hip._sim
+    sim.v.copy_(torch.from_numpy(v.T.copy()))
+    sim.C.copy_(torch.from_numpy(Cm.reshape(n, 9).T.copy()))
+    sim.F_trial.copy_(torch.from_numpy(Ft.reshape(n, 9).T.copy()))
+    o = ora._sim
+    o.v[:] = v
+    o.C[:] = Cm
+    o.F_trial[:] = Ft
+    dt = 1e-4
     sim.node_in.zero_()
     sim._upload_model()
-    sim._pass(6, 1e-4, sim.stream())
+    sim._pass(6, dt, sim.stream())  # stress + P2G, the tick prologue
     m, vi, _ = sim.grid_dense()
-    o = ora._sim
-    o.grid_m[:] = 0
-    o.grid_v_in[:] = 0
-    O.lib().o_compute_stress(o._state(), o._model(), 1e-4)
-    O.lib().o_p2g(o._state(), o._model(), 1e-4)
+    O.lib().o_compute_stress(o._state(), o._model(), dt)
+    O.lib().o_p2g(o._state(), o._model(), dt)
     m = m.cpu().numpy()
     vi = vi.cpu().numpy()
     assert np.abs(m - o.grid_m).max() <= 1e-5 * np.abs(o.grid_m).max()
-    scale = max(np.abs(o.grid_v_in).max(), 1e-30)
-    assert np.abs(vi - o.grid_v_in).max() <= 1e-5 * scale
+    assert np.abs(vi - o.grid_v_in).max() <= 1e-5 * np.abs(o.grid_v_in).max()
     np.testing.assert_allclose(m.sum(dtype=np.float64), o.mass.sum(dtype=np.float64), rtol=1e-6)
+    mom = (o.mass[:, None].astype(np.float64) * v).sum(0)
+    np.testing.assert_allclose(vi.reshape(-1, 3).sum(0, dtype=np.float64), mom, rtol=1e-5,
+                               atol=1e-6 * np.abs(mom).max())
 
 
 def test_c1_positions_after_100_substeps():
@@ -101,7 +113,7 @@ MATERIALS = [
     ("metal_hardening", vonmises(E=1.0e5, nu=0.3, yield_stress=5.0e2, hardening_xi=1e3)),
     ("newtonian", newtonian(eta=2.0, density=1000.0, bulk_modulus=2.0e5)),
     ("herschel_bulkley", newtonian(eta=0.0, density=1000.0, bulk_modulus=2.0e5)
-     .with_yield(200.0).with_powerlaw(K=50.0, n=0.5)),
+     .with_yield(1.0).with_powerlaw(K=0.5, n=0.5)),
     ("mu_i", granular(mu_s=0.38, delta_mu=0.26, I0=0.28, density=1550.0, E=2e5)),
     ("mu_i_phi", granular(mu_s=0.38, delta_mu=0.26, I0=0.28, E=2e5).with_dilatancy(0.6, 0.2)),
     ("tabulated", tabulated_viscous(np.linspace(5.0, 1.0, 16), density=1000.0, bulk_modulus=2e5)),
@@ -129,8 +141,11 @@ def test_materials_match_oracle(label, material):
     dx = 0.4 / 40
     assert np.abs(hip.x() - ora.x()).max() <= 1e-4 * dx
     np.testing.assert_allclose(hip.F(), ora.F(), rtol=1e-4, atol=1e-5)
-    sref = np.abs(ora.stress()).max()
-    np.testing.assert_allclose(hip.stress(), ora.stress(), atol=2e-4 * sref + 1e-3)
+    # stress tolerance: the fp32 SVD resolves singular values near 1 to a few ulps, so
+    # log-strain / (F - R) terms carry ~1e-6 absolute noise scaled by the moduli
+    mu, lam = ora._sim.mu.max(), np.abs(ora._sim.lam).max()
+    tol = (2.0 * mu + 3.0 * lam) * 2e-6 + 1e-4 * np.abs(ora.stress()).max() + 1e-3
+    np.testing.assert_allclose(hip.stress(), ora.stress(), atol=tol)
     np.testing.assert_allclose(hip._sim.yield_stress.cpu().numpy(), ora._sim.yield_stress,
                                rtol=1e-4)
 
