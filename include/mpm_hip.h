@@ -167,14 +167,20 @@ int32_t mpm_blocks_from_box(const mpm_grid_t *G, mpm_tick_ctl_t *ctl, int32_t *b
 /* Grid update of one substep over the listed blocks, restricted to ctl->box:
  * normalisation + gravity (grid_normalization_and_gravity, mpm_utils.py:927-941),
  * damping (add_damping_via_grid, :1287-1295), then every collider of `colliders`
- * (ncoll entries, device) in order, each acting inside its own box on nodes with mass,
- * accumulating its reaction impulse / torque into wrench[6*slot]. Nodes without mass
- * above 1e-15 read velocity 0 (the split-zero contract, mpm_utils.py:1193-1202).
- * With MPM_GRID_ZERO_IN, node_in is cleared afterwards. */
+ * (ncoll entries, device) in order, each acting inside its own box on nodes with mass
+ * (the per-collider grid_postprocess launches of _apply_grid_bc, mpm_solver_warp.py:
+ * 1283-1336). Reaction impulse / torque of the collider with wrench slot k is added to
+ * wrench[6k .. 6k+5] (wrench_slots <= 64) through `workspace` (device floats, at least
+ * mpm_grid_workspace_floats(capacity, wrench_slots)) with a fixed-order reduction, so the
+ * readout is bit-reproducible. Nodes without mass above 1e-15 read velocity 0 (the
+ * split-zero contract, mpm_utils.py:1193-1202). With MPM_GRID_ZERO_IN, node_in is
+ * cleared afterwards. */
+int32_t mpm_grid_workspace_floats(int32_t capacity, int32_t wrench_slots);
 int32_t mpm_grid_update(const mpm_grid_t *G, const mpm_model_t *model, const mpm_tick_ctl_t *ctl,
                         const int32_t *block_list, const int32_t *block_count, int32_t capacity,
-                        const mpm_collider_t *colliders, int32_t ncoll, float *wrench, float dt,
-                        int32_t flags, void *stream);
+                        const mpm_collider_t *colliders, int32_t ncoll, float *wrench,
+                        int32_t wrench_slots, float *workspace, float dt, int32_t flags,
+                        void *stream);
 
 /* ---- sparse active blocks (mpm_utils.py:1506-1544, mpm_solver_warp.py:989-1026) ----- */
 /* raw must be zero on entry; marks the stencil-base block of every particle. */

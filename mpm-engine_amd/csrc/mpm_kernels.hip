@@ -184,16 +184,27 @@ MPM_DEV float wave_sum(float v) {
     return v;
 }
 
-// One substep's grid update over listed blocks (one 64-node block per wavefront).
+constexpr int kMaxSlots = 64;    // wrench slots per launch (documented in mpm_hip.h)
+constexpr int kGridWgs = 256;    // persistent workgroups of the grid update
+
+// One substep's grid update over listed blocks (one 64-node block per wavefront per
+// iteration, persistent workgroups). Collider reaction impulses are summed per
+// wavefront (shuffles) into that wavefront's own LDS row, the rows are added in a fixed
+// order at the end and written to this workgroup's slice of `partials`; k_wrench_reduce
+// then adds the slices in workgroup order. No float atomics: the wrench readout is
+// bit-reproducible for a given block list.
 __global__ __launch_bounds__(kThreads) void k_grid_update(mpm_grid_t G, const mpm_model_t* __restrict__ md,
                                                           const mpm_tick_ctl_t* __restrict__ ctl,
                                                           const int32_t* __restrict__ list,
                                                           const int32_t* __restrict__ count,
                                                           const mpm_collider_t* __restrict__ col, int ncoll,
-                                                          float* __restrict__ wrench, float dt, int flags) {
-    const int nblk = *count;
+                                                          int nslots, float* __restrict__ partials, float dt,
+                                                          int flags) {
+    __shared__ float s_w[kThreads / 64][kMaxSlots * 6];
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
+    for (int t = lane; t < nslots * 6; t += 64) s_w[wave][t] = 0.0f;
+    const int nblk = *count;
     const int waves_total = gridDim.x * (kThreads / 64);
     const float g0 = md->gravity[0], g1 = md->gravity[1], g2 = md->gravity[2];
     const float damp = md->grid_v_damping_scale;
@@ -212,7 +223,7 @@ __global__ __launch_bounds__(kThreads) void k_grid_update(mpm_grid_t G, const mp
         const float m = in4.w;
         v3 vo = mk3(0.f, 0.f, 0.f);
         if (m > 1e-15f) {
-            float inv = 1.0f / m;
+            const float inv = 1.0f / m;
             vo = mk3(in4.x * inv + dt * g0, in4.y * inv + dt * g1, in4.z * inv + dt * g2);
         }
         if (damp < 1.0f) vo = vo * damp;
@@ -222,20 +233,46 @@ __global__ __launch_bounds__(kThreads) void k_grid_update(mpm_grid_t G, const mp
             const bool act = inbox && c.active && m > 0.0f && gi >= c.lo[0] && gi < c.hi[0] && gj >= c.lo[1] &&
                              gj < c.hi[1] && gk >= c.lo[2] && gk < c.hi[2];
             if (act) apply_collider(c, G, gi, gj, gk, m, vo, imp, tq);
-            if (c.slot >= 0 && __any(act)) {
-                float s0 = wave_sum(imp.x), s1 = wave_sum(imp.y), s2 = wave_sum(imp.z);
-                float s3 = wave_sum(tq.x), s4 = wave_sum(tq.y), s5 = wave_sum(tq.z);
+            if (c.slot >= 0 && c.slot < nslots && __any(act)) {
+                const float s0 = wave_sum(imp.x), s1 = wave_sum(imp.y), s2 = wave_sum(imp.z);
+                const float s3 = wave_sum(tq.x), s4 = wave_sum(tq.y), s5 = wave_sum(tq.z);
                 if (lane == 0) {
-                    float* w = wrench + 6 * c.slot;
-                    atomicAdd(w + 0, s0); atomicAdd(w + 1, s1); atomicAdd(w + 2, s2);
-                    atomicAdd(w + 3, s3); atomicAdd(w + 4, s4); atomicAdd(w + 5, s5);
+                    float* w = &s_w[wave][6 * c.slot];
+                    w[0] += s0; w[1] += s1; w[2] += s2; w[3] += s3; w[4] += s4; w[5] += s5;
                 }
             }
         }
         if (inbox) *reinterpret_cast<float4*>(G.node_out + o) = make_float4(vo.x, vo.y, vo.z, 0.0f);
         if (valid && (flags & MPM_GRID_ZERO_IN)) *reinterpret_cast<float4*>(G.node_in + o) = make_float4(0.f, 0.f, 0.f, 0.f);
     }
+    if (nslots > 0) {
+        __syncthreads();
+        for (int t = threadIdx.x; t < nslots * 6; t += kThreads) {
+            float acc = s_w[0][t];
+            for (int w = 1; w < kThreads / 64; ++w) acc += s_w[w][t];
+            partials[(size_t)blockIdx.x * nslots * 6 + t] = acc;
+        }
+    }
 }
+
+// wrench[v] += sum over workgroups of partials[wg][v]: one workgroup per value, strided
+// per-thread sums then an LDS tree, both in a fixed order (bit-reproducible)
+__global__ __launch_bounds__(kThreads) void k_wrench_reduce(const float* __restrict__ partials, int nwg, int nvals,
+                                                            float* __restrict__ wrench) {
+    __shared__ float sh[kThreads];
+    const int v = blockIdx.x;
+    float acc = 0.0f;
+    for (int b = threadIdx.x; b < nwg; b += kThreads) acc += partials[(size_t)b * nvals + v];
+    sh[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = kThreads / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) wrench[v] += sh[0];
+}
+
+inline unsigned grid_update_wgs(int capacity) { return (unsigned)min((capacity + 3) / 4, kGridWgs); }
 
 __global__ void k_grid_zero_box(mpm_grid_t G, int lo0, int lo1, int lo2, int d0, int d1, int d2, int which) {
     long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -523,15 +560,28 @@ int32_t mpm_blocks_from_box(const mpm_grid_t* G, mpm_tick_ctl_t* ctl, int32_t* l
     return launch_status();
 }
 
+int32_t mpm_grid_workspace_floats(int32_t capacity, int32_t wrench_slots) {
+    if (capacity < 0 || wrench_slots < 0 || wrench_slots > kMaxSlots) return -1;
+    return (int32_t)(grid_update_wgs(capacity) * wrench_slots * 6);
+}
+
 int32_t mpm_grid_update(const mpm_grid_t* G, const mpm_model_t* model, const mpm_tick_ctl_t* ctl,
                         const int32_t* list, const int32_t* count, int32_t capacity, const mpm_collider_t* colliders,
-                        int32_t ncoll, float* wrench, float dt, int32_t flags, void* stream) {
+                        int32_t ncoll, float* wrench, int32_t wrench_slots, float* workspace, float dt,
+                        int32_t flags, void* stream) {
     if (!grid_ok(G) || !model || !ctl || !list || !count || capacity < 0 || ncoll < 0) return MPM_ERR_ARG;
     if (ncoll > 0 && !colliders) return MPM_ERR_ARG;
+    if (wrench_slots < 0) return MPM_ERR_ARG;
+    if (wrench_slots > kMaxSlots) return MPM_ERR_SIZE;
+    if (wrench_slots > 0 && (!wrench || !workspace)) return MPM_ERR_ARG;
     if (capacity == 0) return MPM_OK;
-    unsigned nwg = (unsigned)min((capacity + 3) / 4, 8192);
-    hipLaunchKernelGGL(k_grid_update, dim3(nwg), dim3(kThreads), 0, (hipStream_t)stream, *G, model, ctl, list, count,
-                       colliders, ncoll, wrench, dt, flags);
+    const unsigned nwg = grid_update_wgs(capacity);
+    hipStream_t s = (hipStream_t)stream;
+    hipLaunchKernelGGL(k_grid_update, dim3(nwg), dim3(kThreads), 0, s, *G, model, ctl, list, count, colliders, ncoll,
+                       wrench_slots, workspace, dt, flags);
+    if (wrench_slots > 0)
+        hipLaunchKernelGGL(k_wrench_reduce, dim3(wrench_slots * 6), dim3(kThreads), 0, s, workspace, (int)nwg,
+                           wrench_slots * 6, wrench);
     return launch_status();
 }
 

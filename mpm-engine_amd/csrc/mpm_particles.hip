@@ -20,8 +20,30 @@ namespace {
 constexpr int kT = 256;        // particles (threads) per workgroup
 constexpr int kArena = 768;    // LDS arena capacity in nodes (12 KB gather + 12 KB scatter)
 
-// block-wide bounding box of per-thread [lo, hi); every thread receives the result
-MPM_DEV void reduce_box(int lo[3], int hi[3], int (*red)[6], int out[6]) {
+// An LDS tile: the node box [lo, lo + (d0, d1, d2)) staged at arena offset `base`.
+// Tier 1: the stencil box of all 256 particles of the workgroup fits kArena nodes (the
+// usual case for block-sorted particles). Tier 2 (`wg` false): each wavefront stages the
+// box of its own 64 particles in its quarter of the arena; a wavefront whose box still
+// does not fit (`lds` false) gathers and scatters through global memory.
+struct Tile {
+    int lo[3];
+    int d1, d2, vol, base;
+    bool wg, lds;
+};
+
+MPM_DEV bool fits(const int lo[3], const int hi[3], int cap, int& d0, int& d1, int& d2) {
+    d0 = d1 = d2 = 0;
+    if (hi[0] <= lo[0] || hi[1] <= lo[1] || hi[2] <= lo[2]) return false;  // empty (sentinels)
+    d0 = hi[0] - lo[0];
+    d1 = hi[1] - lo[1];
+    d2 = hi[2] - lo[2];
+    if (d0 > cap || d1 > cap || d2 > cap) return false;
+    return d0 * d1 * d2 <= cap;
+}
+
+// per-thread stencil box [lo, hi) (sentinels INT_MAX / INT_MIN when inactive) -> tile;
+// contains one __syncthreads, so every thread of the workgroup must call it
+MPM_DEV Tile make_tile(int lo[3], int hi[3], int (*red)[6], bool periodic) {
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1)
 #pragma unroll
@@ -36,25 +58,47 @@ MPM_DEV void reduce_box(int lo[3], int hi[3], int (*red)[6], int out[6]) {
             red[wave][3 + a] = hi[a];
         }
     __syncthreads();
+    int wlo[3], whi[3];
     for (int a = 0; a < 3; ++a) {
         int l = red[0][a], h = red[0][3 + a];
         for (int w = 1; w < kT / 64; ++w) {
             l = min(l, red[w][a]);
             h = max(h, red[w][3 + a]);
         }
-        out[a] = l;
-        out[3 + a] = h;
+        wlo[a] = l;
+        whi[a] = h;
     }
+    Tile t;
+    int d0, d1, d2;
+    if (!periodic && fits(wlo, whi, kArena, d0, d1, d2)) {
+        t.wg = true;
+        t.lds = true;
+        t.base = 0;
+        for (int a = 0; a < 3; ++a) t.lo[a] = wlo[a];
+    } else {
+        t.wg = false;
+        t.lds = !periodic && fits(lo, hi, kArena / (kT / 64), d0, d1, d2);
+        t.base = wave * (kArena / (kT / 64));
+        for (int a = 0; a < 3; ++a) t.lo[a] = lo[a];
+    }
+    t.d1 = d1;
+    t.d2 = d2;
+    t.vol = t.lds ? d0 * d1 * d2 : 0;
+    return t;
 }
 
-MPM_DEV bool box_fits(const int b[6], int& d0, int& d1, int& d2) {
-    d0 = d1 = d2 = 0;
-    if (b[3] <= b[0] || b[4] <= b[1] || b[5] <= b[2]) return false;  // empty (sentinels)
-    d0 = b[3] - b[0];
-    d1 = b[4] - b[1];
-    d2 = b[5] - b[2];
-    if (d0 <= 0 || d1 <= 0 || d2 <= 0 || d0 > kArena || d1 > kArena || d2 > kArena) return false;
-    return d0 * d1 * d2 <= kArena;
+// cooperative loop over the tile's nodes: the whole workgroup for a workgroup tile, the
+// owning wavefront for a wavefront tile
+template <class F>
+MPM_DEV void tile_for(const Tile& t, F f) {
+    if (!t.lds) return;
+    const int start = t.wg ? (int)threadIdx.x : (int)(threadIdx.x & 63);
+    const int step = t.wg ? kT : 64;
+    for (int n = start; n < t.vol; n += step) {
+        const int k = n % t.d2, r = n / t.d2;
+        const int j = r % t.d1, i = r / t.d1;
+        f(t.base + n, t.lo[0] + i, t.lo[1] + j, t.lo[2] + k);
+    }
 }
 
 MPM_DEV bool in_grid(const mpm_grid_t& G, int i, int j, int k) {
@@ -135,27 +179,6 @@ MPM_DEV void gather_tp(const Stencil& st, float inv_dx, Fetch fetch, v3& nv, m33
     nF.a[6] = fX.z * inv_dx; nF.a[7] = fY.z * inv_dx; nF.a[8] = fZ.z * inv_dx;
 }
 
-// one particle's 27 scatter contributions: (v + C dpos) w m + fs grad w, fs = -vol dt tau;
-// add(i, j, k, value, mass) receives each node's deposit
-template <class Add>
-MPM_DEV void scatter_p(const Stencil& st, float dx, float inv_dx, float mass, v3 v, const m33& Cm,
-                       const m33& fs, Add add) {
-#pragma unroll
-    for (int i = 0; i < 3; ++i)
-#pragma unroll
-        for (int j = 0; j < 3; ++j)
-#pragma unroll
-            for (int k = 0; k < 3; ++k) {
-                const float w = st.w[0][i] * st.w[1][j] * st.w[2][k];
-                const v3 dpos = mk3(((float)i - st.fx[0]) * dx, ((float)j - st.fx[1]) * dx,
-                                    ((float)k - st.fx[2]) * dx);
-                const v3 dwt = mk3(st.dw[0][i] * st.w[1][j] * st.w[2][k], st.w[0][i] * st.dw[1][j] * st.w[2][k],
-                                   st.w[0][i] * st.w[1][j] * st.dw[2][k]) * inv_dx;
-                const float wm = w * mass;
-                add(i, j, k, (v + mvmul(Cm, dpos)) * wm + mvmul(fs, dwt), wm);
-            }
-}
-
 MPM_DEV float sel3(int i, const float* a) { return i == 0 ? a[0] : (i == 1 ? a[1] : a[2]); }
 
 // Global-memory gather / scatter for chunks whose stencil box does not fit the LDS arena
@@ -218,6 +241,28 @@ MPM_DEV void scatter_global(const mpm_grid_t& G, const Stencil& st, bool periodi
     }
 }
 
+// One particle's 27 scatter contributions (v + C dpos) w m + fs grad w (fs = -vol dt tau)
+// handed to add(i, j, k, value, w m). Lane-rotated node order: step s visits node
+// (s + rot) mod 27, so lanes whose particles share a cell (and therefore the same 27
+// target nodes) touch different LDS addresses in the same instruction instead of
+// serialising on one address.
+template <class Add>
+MPM_DEV void scatter_rotated(const Stencil& st, float dx, float inv_dx, float mass, v3 v, const m33& Cm,
+                             const m33& fs, int rot, Add add) {
+#pragma unroll 3
+    for (int s = 0; s < 27; ++s) {
+        int n = s + rot;
+        n = n >= 27 ? n - 27 : n;
+        const int i = n / 9, j = (n / 3) % 3, k = n % 3;
+        const float w0 = sel3(i, st.w[0]), w1 = sel3(j, st.w[1]), w2 = sel3(k, st.w[2]);
+        const float e0 = sel3(i, st.dw[0]), e1 = sel3(j, st.dw[1]), e2 = sel3(k, st.dw[2]);
+        const float wm = w0 * w1 * w2 * mass;
+        const v3 dpos = mk3(((float)i - st.fx[0]) * dx, ((float)j - st.fx[1]) * dx, ((float)k - st.fx[2]) * dx);
+        const v3 dwt = mk3(e0 * w1 * w2, w0 * e1 * w2, w0 * w1 * e2) * inv_dx;
+        add(i, j, k, (v + mvmul(Cm, dpos)) * wm + mvmul(fs, dwt), wm);
+    }
+}
+
 MPM_DEV bool stencil_in_grid(const mpm_grid_t& G, const Stencil& st, bool periodic) {
     bool ok = true;
 #pragma unroll
@@ -225,9 +270,16 @@ MPM_DEV bool stencil_in_grid(const mpm_grid_t& G, const Stencil& st, bool period
     return ok;
 }
 
+// Workgroups are dispatched round-robin over the 8 XCDs (each with its own L2). Chunk c of
+// block-sorted particles is handed to workgroup b with c = (b % 8) * q + b / 8, so every
+// XCD sweeps a contiguous range of chunks: spatial neighbours, whose scatter flushes
+// overlap on the same grid nodes, share one L2 instead of bouncing lines between L2s.
+constexpr int kXcds = 8;
+
 template <int MODE, bool kSolid>
 __global__ __launch_bounds__(kT) void k_particle_pass(mpm_particles_t P, mpm_grid_t G,
-                                                      const mpm_model_t* __restrict__ md, float dt) {
+                                                      const mpm_model_t* __restrict__ md, float dt,
+                                                      int xcd_q) {
     constexpr bool kG2P = (MODE & MPM_PASS_G2P) != 0;
     constexpr bool kStress = (MODE & MPM_PASS_STRESS) != 0;
     constexpr bool kP2G = (MODE & MPM_PASS_P2G) != 0;
@@ -239,7 +291,9 @@ __global__ __launch_bounds__(kT) void k_particle_pass(mpm_particles_t P, mpm_gri
     __shared__ int s_red1[kT / 64][6];
 
     const int S = P.stride;
-    const int p = blockIdx.x * kT + threadIdx.x;
+    const int chunk = (int)(blockIdx.x % kXcds) * xcd_q + (int)(blockIdx.x / kXcds);
+    if (chunk * kT >= P.n) return;  // padding workgroup (uniform exit before any barrier)
+    const int p = chunk * kT + threadIdx.x;
     const bool alive = p < P.n && P.selection[p] == 0;
     const int mat = alive ? P.material[p] : 0;
     const bool periodic = md->periodic_x != 0;
@@ -258,29 +312,24 @@ __global__ __launch_bounds__(kT) void k_particle_pass(mpm_particles_t P, mpm_gri
             lo[a] = gather ? st.base[a] : INT_MAX;
             hi[a] = gather ? st.base[a] + 3 : INT_MIN;
         }
-        int bx[6], d0, d1, d2;
-        reduce_box(lo, hi, s_red0, bx);
-        const bool arena = !periodic && box_fits(bx, d0, d1, d2);
-        if (arena) {
+        const Tile tg = make_tile(lo, hi, s_red0, periodic);
+        {
             const float4* __restrict__ out = reinterpret_cast<const float4*>(G.node_out);
-            const int vol = d0 * d1 * d2;
-            for (int t = threadIdx.x; t < vol; t += kT) {
-                int k = t % d2, r = t / d2;
-                int j = r % d1, i = r / d1;
-                int gi = bx[0] + i, gj = bx[1] + j, gk = bx[2] + k;
-                s_gather[t] = in_grid(G, gi, gj, gk) ? out[node_index(G, gi, gj, gk)] : make_float4(0.f, 0.f, 0.f, 0.f);
-            }
-            __syncthreads();
+            tile_for(tg, [&](int a, int gi, int gj, int gk) {
+                s_gather[a] = in_grid(G, gi, gj, gk) ? out[node_index(G, gi, gj, gk)] : make_float4(0.f, 0.f, 0.f, 0.f);
+            });
         }
+        __syncthreads();
         if (gather) {
             if (!stencil_in_grid(G, st, periodic)) err |= 1;
             v3 nv;
             m33 nC, nF;
-            if (arena) {
-                const int o0 = st.base[0] - bx[0], o1 = st.base[1] - bx[1], o2 = st.base[2] - bx[2];
+            if (tg.lds) {
+                const int o0 = st.base[0] - tg.lo[0], o1 = st.base[1] - tg.lo[1], o2 = st.base[2] - tg.lo[2];
+                const int d1 = tg.d1, d2 = tg.d2, b0 = tg.base;
                 gather_tp(st, G.inv_dx,
                           [&](int i, int j, int k) {
-                              const float4 g = s_gather[((o0 + i) * d1 + (o1 + j)) * d2 + (o2 + k)];
+                              const float4 g = s_gather[b0 + ((o0 + i) * d1 + (o1 + j)) * d2 + (o2 + k)];
                               return mk3(g.x, g.y, g.z);
                           },
                           nv, nC, nF);
@@ -372,19 +421,14 @@ __global__ __launch_bounds__(kT) void k_particle_pass(mpm_particles_t P, mpm_gri
             lo[a] = alive ? st.base[a] : INT_MAX;
             hi[a] = alive ? st.base[a] + 3 : INT_MIN;
         }
-        int bx[6], d0, d1, d2;
-        reduce_box(lo, hi, s_red1, bx);
-        const bool arena = !periodic && box_fits(bx, d0, d1, d2);
-        const int vol = arena ? d0 * d1 * d2 : 0;
-        if (arena) {
-            for (int t = threadIdx.x; t < vol; t += kT) {
-                s_scatter[0][t] = 0.f;
-                s_scatter[1][t] = 0.f;
-                s_scatter[2][t] = 0.f;
-                s_scatter[3][t] = 0.f;
-            }
-            __syncthreads();
-        }
+        const Tile ts = make_tile(lo, hi, s_red1, periodic);
+        tile_for(ts, [&](int a, int, int, int) {
+            s_scatter[0][a] = 0.f;
+            s_scatter[1][a] = 0.f;
+            s_scatter[2][a] = 0.f;
+            s_scatter[3][a] = 0.f;
+        });
+        __syncthreads();
         if (alive) {
             const float r = md->rpic_damping;
             if (r != 0.0f) {
@@ -394,35 +438,34 @@ __global__ __launch_bounds__(kT) void k_particle_pass(mpm_particles_t P, mpm_gri
             }
             const m33 fs = mscale(stress, -P.vol[p] * dt);
             if (!stencil_in_grid(G, st, periodic)) err |= 1;
-            if (arena) {
-                const int o0 = st.base[0] - bx[0], o1 = st.base[1] - bx[1], o2 = st.base[2] - bx[2];
-                scatter_p(st, G.dx, G.inv_dx, P.mass[p], v, Cm, fs, [&](int i, int j, int k, v3 add, float wm) {
-                    const int a = ((o0 + i) * d1 + (o1 + j)) * d2 + (o2 + k);
-                    atomicAdd(&s_scatter[0][a], add.x);
-                    atomicAdd(&s_scatter[1][a], add.y);
-                    atomicAdd(&s_scatter[2][a], add.z);
-                    atomicAdd(&s_scatter[3][a], wm);
-                });
+            if (ts.lds) {
+                const int o0 = st.base[0] - ts.lo[0], o1 = st.base[1] - ts.lo[1], o2 = st.base[2] - ts.lo[2];
+                const int d1 = ts.d1, d2 = ts.d2, b0 = ts.base;
+                scatter_rotated(st, G.dx, G.inv_dx, P.mass[p], v, Cm, fs, (int)(threadIdx.x % 27u),
+                                [&](int i, int j, int k, v3 add, float wm) {
+                                    const int a = b0 + ((o0 + i) * d1 + (o1 + j)) * d2 + (o2 + k);
+                                    atomicAdd(&s_scatter[0][a], add.x);
+                                    atomicAdd(&s_scatter[1][a], add.y);
+                                    atomicAdd(&s_scatter[2][a], add.z);
+                                    atomicAdd(&s_scatter[3][a], wm);
+                                });
             } else {
                 scatter_global(G, st, periodic, P.mass[p], v, Cm, fs);
             }
         }
-        if (arena) {
-            __syncthreads();
+        __syncthreads();
+        {
             float* __restrict__ gin = G.node_in;
-            for (int t = threadIdx.x; t < vol; t += kT) {
-                const float a0 = s_scatter[0][t], a1 = s_scatter[1][t], a2 = s_scatter[2][t], a3 = s_scatter[3][t];
-                if (a0 == 0.f && a1 == 0.f && a2 == 0.f && a3 == 0.f) continue;
-                int k = t % d2, rr = t / d2;
-                int j = rr % d1, i = rr / d1;
-                const int gi = bx[0] + i, gj = bx[1] + j, gk = bx[2] + k;
-                if (!in_grid(G, gi, gj, gk)) continue;
+            tile_for(ts, [&](int a, int gi, int gj, int gk) {
+                const float a0 = s_scatter[0][a], a1 = s_scatter[1][a], a2 = s_scatter[2][a], a3 = s_scatter[3][a];
+                if (a0 == 0.f && a1 == 0.f && a2 == 0.f && a3 == 0.f) return;
+                if (!in_grid(G, gi, gj, gk)) return;
                 const size_t o = 4 * (size_t)node_index(G, gi, gj, gk);
                 atomicAdd(gin + o + 0, a0);
                 atomicAdd(gin + o + 1, a1);
                 atomicAdd(gin + o + 2, a2);
                 atomicAdd(gin + o + 3, a3);
-            }
+            });
         }
     }
     if (alive && !(x.x == x.x && x.y == x.y && x.z == x.z)) err |= 2;
@@ -443,13 +486,15 @@ extern "C" int32_t mpm_particle_pass(const mpm_particles_t* P, const mpm_grid_t*
         return MPM_ERR_ARG;
     if (P->n == 0) return MPM_OK;
     hipStream_t s = (hipStream_t)stream;
-    const dim3 grid((unsigned)((P->n + kT - 1) / kT)), block(kT);
+    const int nchunks = (P->n + kT - 1) / kT;
+    const int q = (nchunks + kXcds - 1) / kXcds;
+    const dim3 grid((unsigned)(q * kXcds)), block(kT);
     const bool solid = (mode & MPM_PASS_VISCOUS_ONLY) == 0;
     const int m = mode & (MPM_PASS_G2P | MPM_PASS_STRESS | MPM_PASS_P2G | MPM_PASS_LEAN | MPM_PASS_KEEP_STRESS);
 #define CASE(M)                                                                                   \
     case (M):                                                                                     \
-        if (solid) hipLaunchKernelGGL((k_particle_pass<(M), true>), grid, block, 0, s, *P, *G, model, dt); \
-        else hipLaunchKernelGGL((k_particle_pass<(M), false>), grid, block, 0, s, *P, *G, model, dt);     \
+        if (solid) hipLaunchKernelGGL((k_particle_pass<(M), true>), grid, block, 0, s, *P, *G, model, dt, q); \
+        else hipLaunchKernelGGL((k_particle_pass<(M), false>), grid, block, 0, s, *P, *G, model, dt, q);     \
         break;
     switch (m) {
         CASE(MPM_PASS_G2P)

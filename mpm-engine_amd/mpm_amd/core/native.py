@@ -78,7 +78,8 @@ SIGNATURES = {
     "mpm_grid_zero_box": (_I32, [_P, _P, _P, _I32, _P]),
     "mpm_grid_zero_blocks": (_I32, [_P, _P, _P, _I32, _I32, _P]),
     "mpm_blocks_from_box": (_I32, [_P, _P, _P, _P]),
-    "mpm_grid_update": (_I32, [_P, _P, _P, _P, _P, _I32, _P, _I32, _P, _F, _I32, _P]),
+    "mpm_grid_workspace_floats": (_I32, [_I32, _I32]),
+    "mpm_grid_update": (_I32, [_P, _P, _P, _P, _P, _I32, _P, _I32, _P, _I32, _P, _F, _I32, _P]),
     "mpm_blocks_mark": (_I32, [_P, _P, _P, _P]),
     "mpm_blocks_dilate": (_I32, [_P, _P, _P, _P]),
     "mpm_blocks_compact": (_I32, [_P, _P, _P, _P, _P, _P, _P]),

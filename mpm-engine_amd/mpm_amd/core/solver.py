@@ -152,6 +152,7 @@ class HipSimulator:
         # colliders
         self.colliders = ColliderSet(self.n_grid, self.dx)
         self.wrench = t(6 * 64)
+        self.grid_ws = None
         self.sched_dev = None
         self._sched_host = None
         self.time = 0.0
@@ -412,6 +413,12 @@ class HipSimulator:
             raise ValueError("at most 64 wrench-reading colliders are supported")
         nbytes = tab.nbytes
         self._ensure_sched(nbytes)
+        ws = int(L.mpm_grid_workspace_floats(self.n_blocks, self.colliders.n_slots))
+        if ws < 0:
+            raise ValueError("at most 64 wrench-reading colliders are supported")
+        if self.grid_ws is None or self.grid_ws.numel() < max(ws, 1):
+            self.grid_ws = torch.zeros(max(ws, 1), dtype=torch.float32, device=self.device)
+            self._graph = None
         if self._h2d_done is not None:  # the pinned staging buffers are reused
             self._h2d_done.synchronize()
         if nbytes:
@@ -448,7 +455,8 @@ class HipSimulator:
             self._enqueue_tick_profiled(dt, S, K, fused, lists)
         elif self.use_graph and self._ticks > 0:
             key = (S, float(dt), bool(fused), K, lists, native.ptr(self.sched_dev),
-                   self.update_cov_with_F)
+                   self.update_cov_with_F, self.colliders.n_slots, native.ptr(self.grid_ws),
+                   frozenset(self._mat_ids) <= native.VISCOUS_IDS)
             if self._graph is None or self._graph_key != key:
                 self._capture(dt, S, K, fused, lists)
                 self._graph_key = key
@@ -474,10 +482,11 @@ class HipSimulator:
 
     def _grid(self, dt, k_row, K, lists, s):
         sched = native.ptr(self.sched_dev) + k_row * K * native.COLLIDER_DTYPE.itemsize
+        slots = self.colliders.n_slots
         native.check(self.lib.mpm_grid_update(
             self.pG, native.ptr(self.model_dev), native.ptr(self.ctl), lists[0], lists[1],
-            self.n_blocks, sched if K else None, K, native.ptr(self.wrench), dt,
-            native.GRID_ZERO_IN, s), "grid_update")
+            self.n_blocks, sched if K else None, K, native.ptr(self.wrench), slots,
+            native.ptr(self.grid_ws) if slots else None, dt, native.GRID_ZERO_IN, s), "grid_update")
 
     def _enqueue_tick(self, dt, S, K, fused, lists, s):
         G2P, ST, P2G = native.PASS_G2P, native.PASS_STRESS, native.PASS_P2G

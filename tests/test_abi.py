@@ -40,5 +40,6 @@ def test_struct_layouts_and_introspection():
 def test_entry_points_reject_null_arguments_without_touching_the_gpu():
     lib = native.load()
     assert lib.mpm_particle_pass(None, None, None, 1e-4, 7, None) == 1
-    assert lib.mpm_grid_update(None, None, None, None, None, 0, None, 0, None, 1e-4, 1, None) == 1
+    assert lib.mpm_grid_update(None, None, None, None, None, 0, None, 0, None, 0, None, 1e-4, 1, None) == 1
+    assert lib.mpm_grid_workspace_floats(100, 65) == -1
     assert lib.mpm_permute(None, None, None, None) == 1
