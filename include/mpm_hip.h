@@ -11,6 +11,14 @@
  * Every function returns an mpm_status (0 = ok) and enqueues work on `stream` (a
  * hipStream_t passed as void*; NULL = the legacy default stream).
  *
+ * Batched scenes (no counterpart in the reference, which runs one scene per solver):
+ * n_scenes independent grids of the same shape live back to back in node_in / node_out
+ * and one particle array holds every scene's particles, scene[p] naming the owner. One
+ * launch per phase advances all scenes: block codes in block lists are global
+ * (scene * nb0*nb1*nb2 + local code), collider schedules hold ncoll rows per scene
+ * ([substeps][n_scenes][ncoll]), box arguments (zero box, launch box) apply to every
+ * scene, and the block sort keys are scene-major so each scene stays contiguous.
+ *
  * Memory layout (device):
  *  - particles are SoA planes: x[c*stride + p]; mat33 fields hold 9 planes, component
  *    (r, c) in plane 3*r + c (row-major like Warp's mat33).
@@ -28,7 +36,7 @@ extern "C" {
 #endif
 
 #define MPM_MAX_MATERIALS 16
-#define MPM_ABI_VERSION 1
+#define MPM_ABI_VERSION 2
 
 typedef enum mpm_status {
     MPM_OK = 0,
@@ -45,6 +53,10 @@ typedef enum mpm_status {
 #define MPM_PASS_KEEP_STRESS 16 /* with LEAN: still store the stress */
 #define MPM_PASS_VISCOUS_ONLY 32 /* every particle is a viscous / rigid id (6, 7, 8, 10, 12):
                                     launch the specialisation without the SVD paths */
+/* launch shape (results do not depend on it; an autotuner picks it per workload) */
+#define MPM_PASS_WG128 64       /* 128 threads per workgroup instead of 256 */
+#define MPM_PASS_PPT2 128       /* each thread takes 2 particles in turn */
+#define MPM_PASS_PPT4 256       /* each thread takes 4 particles in turn */
 
 /* grid-update flag bits (mpm_grid_update) */
 #define MPM_GRID_ZERO_IN 1      /* clear node_in after it is consumed */
@@ -67,6 +79,7 @@ typedef struct mpm_particles {
     float *vol, *mass, *mu, *lam, *yield_stress, *Jp;
     int32_t *material, *selection;
     float *cov; /* 6 planes (xx, xy, xz, yy, yz, zz) or NULL when covariance is off */
+    int32_t *scene; /* scene id per particle for batched scenes, or NULL (all scene 0) */
 } mpm_particles_t;
 
 typedef struct mpm_grid {
@@ -77,7 +90,12 @@ typedef struct mpm_grid {
     float dx, inv_dx;     /* fp32 like the reference's MPMModelStruct.dx / inv_dx */
     int32_t *error_flag;  /* device int, OR-ed bits: 1 = a stencil left the grid,
                              2 = a non-finite particle position, 4 = a solid material
-                             met a MPM_PASS_VISCOUS_ONLY pass */
+                             met a MPM_PASS_VISCOUS_ONLY pass, 8 = a scene id outside
+                             [0, n_scenes) (that particle is skipped) */
+    int32_t n_scenes;     /* independent scenes sharing the buffers (>= 1): scene s owns
+                             blocks [s * B, (s + 1) * B), B = nb[0] * nb[1] * nb[2];
+                             block codes in every block list are these global codes */
+    int32_t pad_;
 } mpm_grid_t;
 
 /* Model parameters (MPMModelStruct, warp_utils.py:23-84). Lives in device memory; the
@@ -159,8 +177,8 @@ int32_t mpm_grid_zero_blocks(const mpm_grid_t *G, const int32_t *block_list,
                              const int32_t *block_count, int32_t capacity, int32_t which,
                              void *stream);
 
-/* Build the list of blocks intersecting ctl->box (ctl is device memory) into
- * block_list and set ctl->nblocks. */
+/* Build the list of blocks intersecting ctl->box (ctl is device memory) in every scene
+ * into block_list (capacity nb0*nb1*nb2*n_scenes) and set ctl->nblocks. */
 int32_t mpm_blocks_from_box(const mpm_grid_t *G, mpm_tick_ctl_t *ctl, int32_t *block_list,
                             void *stream);
 
@@ -170,9 +188,11 @@ int32_t mpm_blocks_from_box(const mpm_grid_t *G, mpm_tick_ctl_t *ctl, int32_t *b
  * (ncoll entries, device) in order, each acting inside its own box on nodes with mass
  * (the per-collider grid_postprocess launches of _apply_grid_bc, mpm_solver_warp.py:
  * 1283-1336). Reaction impulse / torque of the collider with wrench slot k is added to
- * wrench[6k .. 6k+5] (wrench_slots <= 64) through `workspace` (device floats, at least
- * mpm_grid_workspace_floats(capacity, wrench_slots)) with a fixed-order reduction, so the
- * readout is bit-reproducible. Nodes without mass above 1e-15 read velocity 0 (the
+ * wrench[6k .. 6k+5]. Up to 64 slots this goes through `workspace` (device floats, at
+ * least mpm_grid_workspace_floats(capacity, wrench_slots)) with a fixed-order reduction,
+ * so the readout is bit-reproducible; with more slots (batched scenes) per-wavefront sums
+ * are added with float atomics, workspace_floats is 0 and `workspace` may be NULL.
+ * A block of scene s applies colliders[s * ncoll .. s * ncoll + ncoll). Nodes without mass above 1e-15 read velocity 0 (the
  * split-zero contract, mpm_utils.py:1193-1202). With MPM_GRID_ZERO_IN, node_in is
  * cleared afterwards. */
 int32_t mpm_grid_workspace_floats(int32_t capacity, int32_t wrench_slots);
@@ -196,8 +216,9 @@ int32_t mpm_blocks_compact(const mpm_grid_t *G, const int32_t *cur, const int32_
 int32_t mpm_bounds(const mpm_particles_t *P, float *out, void *stream);
 
 /* ---- block sort (solver.py:454-467, mpm_solver_warp.py:1548-1619) ------------------- */
-/* keys[p] = ((bx*nb + by)*nb + bz) with b = floor(x/dx - 0.5) >> 2, nb = n0/4 + 2;
- * vals[p] = p; *unsorted (device int) gets 1 if keys are not non-decreasing. */
+/* keys[p] = scene*nb^3 + ((bx*nb + by)*nb + bz) with b = floor(x/dx - 0.5) >> 2,
+ * nb = n0/4 + 2 (scene 0 without batching); vals[p] = p; *unsorted (device int) gets 1
+ * if keys are not non-decreasing. MPM_ERR_SIZE when nb^3 * n_scenes exceeds int32. */
 int32_t mpm_sort_keys(const mpm_particles_t *P, const mpm_grid_t *G, int32_t *keys,
                       int32_t *vals, int32_t *unsorted, void *stream);
 /* temp storage needed by mpm_sort_pairs for n keys */
@@ -218,7 +239,7 @@ int32_t mpm_export_R(const mpm_particles_t *P, float *R_out, void *stream);
 int32_t mpm_export_cov(const mpm_particles_t *P, const float *init_cov, float *cov_out,
                        void *stream);
 /* dense copies of the grid in the reference's layout (z fastest): m [n^3], v_in and
- * v_out [n^3][3]; any output may be NULL. */
+ * v_out [n^3][3], one such copy per scene back to back; any output may be NULL. */
 int32_t mpm_grid_to_dense(const mpm_grid_t *G, float *m, float *v_in, float *v_out, void *stream);
 
 #ifdef __cplusplus

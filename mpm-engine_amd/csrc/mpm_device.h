@@ -435,6 +435,10 @@ MPM_DEV int node_index(const mpm_grid_t& G, int i, int j, int k) {
            ((j & 3) << 2) + (k & 3);
 }
 
+MPM_DEV size_t scene_base(const mpm_grid_t& G, int scene) {
+    return (size_t)scene * (size_t)G.nb[0] * G.nb[1] * G.nb[2] * 64;
+}
+
 MPM_DEV void decode_block(const mpm_grid_t& G, int code, int& bi, int& bj, int& bk) {
     bk = code % G.nb[2];
     int r = code / G.nb[2];

@@ -188,23 +188,31 @@ constexpr int kMaxSlots = 64;    // wrench slots per launch (documented in mpm_h
 constexpr int kGridWgs = 256;    // persistent workgroups of the grid update
 
 // One substep's grid update over listed blocks (one 64-node block per wavefront per
-// iteration, persistent workgroups). Collider reaction impulses are summed per
-// wavefront (shuffles) into that wavefront's own LDS row, the rows are added in a fixed
-// order at the end and written to this workgroup's slice of `partials`; k_wrench_reduce
-// then adds the slices in workgroup order. No float atomics: the wrench readout is
-// bit-reproducible for a given block list.
+// iteration, persistent workgroups). Block codes are global over the batched scenes
+// (scene * blocks-per-scene + local code); a block of scene s applies that scene's row
+// of ncoll colliders. Collider reaction impulses are summed per wavefront (shuffles).
+// Up to kMaxSlots wrench slots, each wavefront adds its sums into its own LDS row, the
+// rows are added in a fixed order at the end and written to this workgroup's slice of
+// `partials`; k_wrench_reduce then adds the slices in workgroup order. No float
+// atomics: the wrench readout is bit-reproducible for a given block list. With more
+// slots (one or more wrench colliders per scene of a large batch) lane 0 adds the
+// wavefront sums to `wrench` with float atomics instead.
 __global__ __launch_bounds__(kThreads) void k_grid_update(mpm_grid_t G, const mpm_model_t* __restrict__ md,
                                                           const mpm_tick_ctl_t* __restrict__ ctl,
                                                           const int32_t* __restrict__ list,
                                                           const int32_t* __restrict__ count,
                                                           const mpm_collider_t* __restrict__ col, int ncoll,
-                                                          int nslots, float* __restrict__ partials, float dt,
-                                                          int flags) {
+                                                          int nslots, float* __restrict__ partials,
+                                                          float* __restrict__ wrench, float dt, int flags) {
     __shared__ float s_w[kThreads / 64][kMaxSlots * 6];
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
-    for (int t = lane; t < nslots * 6; t += 64) s_w[wave][t] = 0.0f;
+    const bool lds_w = nslots <= kMaxSlots;
+    if (lds_w)
+        for (int t = lane; t < nslots * 6; t += 64) s_w[wave][t] = 0.0f;
     const int nblk = *count;
+    const int bps = G.nb[0] * G.nb[1] * G.nb[2];
+    const int ncode = bps * G.n_scenes;
     const int waves_total = gridDim.x * (kThreads / 64);
     const float g0 = md->gravity[0], g1 = md->gravity[1], g2 = md->gravity[2];
     const float damp = md->grid_v_damping_scale;
@@ -212,8 +220,11 @@ __global__ __launch_bounds__(kThreads) void k_grid_update(mpm_grid_t G, const mp
     const int bhi0 = ctl->box_hi[0], bhi1 = ctl->box_hi[1], bhi2 = ctl->box_hi[2];
     for (int b = blockIdx.x * (kThreads / 64) + wave; b < nblk; b += waves_total) {
         const int code = list[b];
+        if ((unsigned)code >= (unsigned)ncode) continue;  // wavefront-uniform
+        const int scene = code / bps;
+        const mpm_collider_t* __restrict__ cs = col + (size_t)scene * ncoll;
         int bi, bj, bk;
-        decode_block(G, code, bi, bj, bk);
+        decode_block(G, code - scene * bps, bi, bj, bk);
         const int gi = bi * 4 + (lane >> 4), gj = bj * 4 + ((lane >> 2) & 3), gk = bk * 4 + (lane & 3);
         const size_t o = 4 * ((size_t)code * 64 + lane);
         const bool valid = gi < G.n[0] && gj < G.n[1] && gk < G.n[2];
@@ -228,7 +239,7 @@ __global__ __launch_bounds__(kThreads) void k_grid_update(mpm_grid_t G, const mp
         }
         if (damp < 1.0f) vo = vo * damp;
         for (int q = 0; q < ncoll; ++q) {
-            const mpm_collider_t& c = col[q];
+            const mpm_collider_t& c = cs[q];
             v3 imp = mk3(0.f, 0.f, 0.f), tq = mk3(0.f, 0.f, 0.f);
             const bool act = inbox && c.active && m > 0.0f && gi >= c.lo[0] && gi < c.hi[0] && gj >= c.lo[1] &&
                              gj < c.hi[1] && gk >= c.lo[2] && gk < c.hi[2];
@@ -237,15 +248,21 @@ __global__ __launch_bounds__(kThreads) void k_grid_update(mpm_grid_t G, const mp
                 const float s0 = wave_sum(imp.x), s1 = wave_sum(imp.y), s2 = wave_sum(imp.z);
                 const float s3 = wave_sum(tq.x), s4 = wave_sum(tq.y), s5 = wave_sum(tq.z);
                 if (lane == 0) {
-                    float* w = &s_w[wave][6 * c.slot];
-                    w[0] += s0; w[1] += s1; w[2] += s2; w[3] += s3; w[4] += s4; w[5] += s5;
+                    if (lds_w) {
+                        float* w = &s_w[wave][6 * c.slot];
+                        w[0] += s0; w[1] += s1; w[2] += s2; w[3] += s3; w[4] += s4; w[5] += s5;
+                    } else {
+                        float* w = wrench + 6 * (size_t)c.slot;
+                        atomicAdd(w + 0, s0); atomicAdd(w + 1, s1); atomicAdd(w + 2, s2);
+                        atomicAdd(w + 3, s3); atomicAdd(w + 4, s4); atomicAdd(w + 5, s5);
+                    }
                 }
             }
         }
         if (inbox) *reinterpret_cast<float4*>(G.node_out + o) = make_float4(vo.x, vo.y, vo.z, 0.0f);
         if (valid && (flags & MPM_GRID_ZERO_IN)) *reinterpret_cast<float4*>(G.node_in + o) = make_float4(0.f, 0.f, 0.f, 0.f);
     }
-    if (nslots > 0) {
+    if (nslots > 0 && lds_w) {
         __syncthreads();
         for (int t = threadIdx.x; t < nslots * 6; t += kThreads) {
             float acc = s_w[0][t];
@@ -276,13 +293,15 @@ inline unsigned grid_update_wgs(int capacity) { return (unsigned)min((capacity +
 
 __global__ void k_grid_zero_box(mpm_grid_t G, int lo0, int lo1, int lo2, int d0, int d1, int d2, int which) {
     long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    long total = (long)d0 * d1 * d2;
-    if (t >= total) return;
+    const long vol = (long)d0 * d1 * d2;
+    if (t >= vol * G.n_scenes) return;
+    const int scene = (int)(t / vol);
+    t -= scene * vol;
     int k = lo2 + (int)(t % d2);
     long r = t / d2;
     int j = lo1 + (int)(r % d1);
     int i = lo0 + (int)(r / d1);
-    size_t o = 4 * (size_t)node_index(G, i, j, k);
+    size_t o = 4 * (scene_base(G, scene) + (size_t)node_index(G, i, j, k));
     const float4 z = make_float4(0.f, 0.f, 0.f, 0.f);
     if (which & 1) *reinterpret_cast<float4*>(G.node_in + o) = z;
     if (which & 2) *reinterpret_cast<float4*>(G.node_out + o) = z;
@@ -302,15 +321,18 @@ __global__ void k_blocks_from_box(mpm_grid_t G, mpm_tick_ctl_t* ctl, int32_t* li
     const int bl0 = ctl->box_lo[0] >> 2, bl1 = ctl->box_lo[1] >> 2, bl2 = ctl->box_lo[2] >> 2;
     const int bh0 = (ctl->box_hi[0] + 3) >> 2, bh1 = (ctl->box_hi[1] + 3) >> 2, bh2 = (ctl->box_hi[2] + 3) >> 2;
     const int d0 = max(bh0 - bl0, 0), d1 = max(bh1 - bl1, 0), d2 = max(bh2 - bl2, 0);
-    const int total = d0 * d1 * d2;
+    const int per = d0 * d1 * d2;
+    const int total = per * G.n_scenes;
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t == 0) ctl->nblocks = total;
     if (t >= total) return;
-    int bk = bl2 + t % d2;
-    int r = t / d2;
+    const int scene = t / per;
+    const int u = t - scene * per;
+    int bk = bl2 + u % d2;
+    int r = u / d2;
     int bj = bl1 + r % d1;
     int bi = bl0 + r / d1;
-    list[t] = (bi * G.nb[1] + bj) * G.nb[2] + bk;
+    list[t] = scene * (G.nb[0] * G.nb[1] * G.nb[2]) + (bi * G.nb[1] + bj) * G.nb[2] + bk;
 }
 
 // ------------------------------------------------------------------ sparse blocks
@@ -318,21 +340,26 @@ __global__ void k_blocks_from_box(mpm_grid_t G, mpm_tick_ctl_t* ctl, int32_t* li
 __global__ void k_blocks_mark(mpm_particles_t P, mpm_grid_t G, int32_t* raw) {
     int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= P.n) return;
+    const int scene = P.scene ? P.scene[p] : 0;
+    if ((unsigned)scene >= (unsigned)G.n_scenes) return;
     int b[3];
     for (int a = 0; a < 3; ++a) {
         int t = (int)(P.x[a * P.stride + p] * G.inv_dx - 0.5f);
         t >>= 2;
         b[a] = min(max(t, 0), G.nb[a] - 1);
     }
-    raw[(b[0] * G.nb[1] + b[1]) * G.nb[2] + b[2]] = 1;
+    raw[(size_t)scene * (G.nb[0] * G.nb[1] * G.nb[2]) + (b[0] * G.nb[1] + b[1]) * G.nb[2] + b[2]] = 1;
 }
 
 // blocks_dilate, mpm_utils.py:1516-1529
 __global__ void k_blocks_dilate(mpm_grid_t G, const int32_t* raw, int32_t* cur) {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
-    int total = G.nb[0] * G.nb[1] * G.nb[2];
-    if (t >= total) return;
-    int z = t % G.nb[2], r = t / G.nb[2], y = r % G.nb[1], x = r / G.nb[1];
+    const int bps = G.nb[0] * G.nb[1] * G.nb[2];
+    if (t >= bps * G.n_scenes) return;
+    const int scene = t / bps;
+    raw += (size_t)scene * bps;  // dilation stays inside the scene
+    const int u = t - scene * bps;
+    int z = u % G.nb[2], r = u / G.nb[2], y = r % G.nb[1], x = r / G.nb[1];
     int on = 0;
     for (int i = -1; i <= 1; ++i)
         for (int j = -1; j <= 1; ++j)
@@ -348,7 +375,7 @@ __global__ void k_blocks_dilate(mpm_grid_t G, const int32_t* raw, int32_t* cur) 
 __global__ void k_blocks_compact(mpm_grid_t G, const int32_t* cur, const int32_t* prev, int32_t* cur_list,
                                  int32_t* union_list, int32_t* counts) {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
-    int total = G.nb[0] * G.nb[1] * G.nb[2];
+    int total = G.nb[0] * G.nb[1] * G.nb[2] * G.n_scenes;
     if (t >= total) return;
     int c = cur[t];
     if (c == 1) cur_list[atomicAdd(counts, 1)] = t;
@@ -420,7 +447,8 @@ __global__ void k_bounds_finish(int* out) {
 }
 
 // ------------------------------------------------------------------------ sorting
-// key of solver.py:458-461: base = floor(x / dx - 0.5) >> 2, nb = n/4 + 2
+// key of solver.py:458-461: base = floor(x / dx - 0.5) >> 2, nb = n/4 + 2; batched
+// scenes are scene-major (scene * nb^3 + key), so each scene stays one contiguous range
 __global__ void k_sort_keys(mpm_particles_t P, mpm_grid_t G, int32_t* keys, int32_t* vals, int32_t* unsorted) {
     int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= P.n) return;
@@ -428,7 +456,8 @@ __global__ void k_sort_keys(mpm_particles_t P, mpm_grid_t G, int32_t* keys, int3
     auto key_of = [&](int q) {
         int b[3];
         for (int a = 0; a < 3; ++a) b[a] = ((int)floorf(P.x[a * P.stride + q] / G.dx - 0.5f)) >> 2;
-        return (b[0] * nb + b[1]) * nb + b[2];
+        const int scene = P.scene ? min(max(P.scene[q], 0), G.n_scenes - 1) : 0;
+        return scene * (nb * nb * nb) + (b[0] * nb + b[1]) * nb + b[2];
     };
     int k = key_of(p);
     keys[p] = k;
@@ -474,13 +503,15 @@ __global__ void k_export_cov(mpm_particles_t P, const float* init_cov, float* co
 
 __global__ void k_grid_to_dense(mpm_grid_t G, float* m, float* vin, float* vout) {
     long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    long total = (long)G.n[0] * G.n[1] * G.n[2];
-    if (t >= total) return;
-    int k = (int)(t % G.n[2]);
-    long r = t / G.n[2];
+    const long per = (long)G.n[0] * G.n[1] * G.n[2];
+    if (t >= per * G.n_scenes) return;
+    const int scene = (int)(t / per);
+    const long u = t - scene * per;
+    int k = (int)(u % G.n[2]);
+    long r = u / G.n[2];
     int j = (int)(r % G.n[1]);
     int i = (int)(r / G.n[1]);
-    size_t o = 4 * (size_t)node_index(G, i, j, k);
+    size_t o = 4 * (scene_base(G, scene) + (size_t)node_index(G, i, j, k));
     if (m) m[t] = G.node_in[o + 3];
     if (vin) { vin[3 * t] = G.node_in[o]; vin[3 * t + 1] = G.node_in[o + 1]; vin[3 * t + 2] = G.node_in[o + 2]; }
     if (vout) { vout[3 * t] = G.node_out[o]; vout[3 * t + 1] = G.node_out[o + 1]; vout[3 * t + 2] = G.node_out[o + 2]; }
@@ -491,7 +522,7 @@ inline unsigned blocks_for(long n, int threads = kThreads) { return (unsigned)((
 inline int32_t launch_status() { return hipGetLastError() == hipSuccess ? MPM_OK : MPM_ERR_HIP; }
 
 inline int grid_ok(const mpm_grid_t* G) {
-    return G && G->node_in && G->node_out && G->n[0] > 0 && G->n[1] > 0 && G->n[2] > 0 &&
+    return G && G->node_in && G->node_out && G->n[0] > 0 && G->n[1] > 0 && G->n[2] > 0 && G->n_scenes >= 1 &&
            G->nb[0] == (G->n[0] + 3) / 4 && G->nb[1] == (G->n[1] + 3) / 4 && G->nb[2] == (G->n[2] + 3) / 4;
 }
 
@@ -537,7 +568,7 @@ int32_t mpm_grid_zero_box(const mpm_grid_t* G, const int32_t* lo, const int32_t*
         d[a] = h - l[a];
         if (d[a] <= 0) return MPM_OK;
     }
-    long total = (long)d[0] * d[1] * d[2];
+    long total = (long)d[0] * d[1] * d[2] * G->n_scenes;
     hipLaunchKernelGGL(k_grid_zero_box, dim3(blocks_for(total)), dim3(kThreads), 0, (hipStream_t)stream, *G, l[0],
                        l[1], l[2], d[0], d[1], d[2], which);
     return launch_status();
@@ -554,14 +585,15 @@ int32_t mpm_grid_zero_blocks(const mpm_grid_t* G, const int32_t* list, const int
 
 int32_t mpm_blocks_from_box(const mpm_grid_t* G, mpm_tick_ctl_t* ctl, int32_t* list, void* stream) {
     if (!grid_ok(G) || !ctl || !list) return MPM_ERR_ARG;
-    long total = (long)G->nb[0] * G->nb[1] * G->nb[2];
+    long total = (long)G->nb[0] * G->nb[1] * G->nb[2] * G->n_scenes;
     hipLaunchKernelGGL(k_blocks_from_box, dim3(blocks_for(total)), dim3(kThreads), 0, (hipStream_t)stream, *G, ctl,
                        list);
     return launch_status();
 }
 
 int32_t mpm_grid_workspace_floats(int32_t capacity, int32_t wrench_slots) {
-    if (capacity < 0 || wrench_slots < 0 || wrench_slots > kMaxSlots) return -1;
+    if (capacity < 0 || wrench_slots < 0) return -1;
+    if (wrench_slots > kMaxSlots) return 0;  // atomic path, no workspace
     return (int32_t)(grid_update_wgs(capacity) * wrench_slots * 6);
 }
 
@@ -572,21 +604,21 @@ int32_t mpm_grid_update(const mpm_grid_t* G, const mpm_model_t* model, const mpm
     if (!grid_ok(G) || !model || !ctl || !list || !count || capacity < 0 || ncoll < 0) return MPM_ERR_ARG;
     if (ncoll > 0 && !colliders) return MPM_ERR_ARG;
     if (wrench_slots < 0) return MPM_ERR_ARG;
-    if (wrench_slots > kMaxSlots) return MPM_ERR_SIZE;
-    if (wrench_slots > 0 && (!wrench || !workspace)) return MPM_ERR_ARG;
+    const bool fixed_order = wrench_slots <= kMaxSlots;
+    if (wrench_slots > 0 && (!wrench || (fixed_order && !workspace))) return MPM_ERR_ARG;
     if (capacity == 0) return MPM_OK;
     const unsigned nwg = grid_update_wgs(capacity);
     hipStream_t s = (hipStream_t)stream;
     hipLaunchKernelGGL(k_grid_update, dim3(nwg), dim3(kThreads), 0, s, *G, model, ctl, list, count, colliders, ncoll,
-                       wrench_slots, workspace, dt, flags);
-    if (wrench_slots > 0)
+                       wrench_slots, workspace, wrench, dt, flags);
+    if (wrench_slots > 0 && fixed_order)
         hipLaunchKernelGGL(k_wrench_reduce, dim3(wrench_slots * 6), dim3(kThreads), 0, s, workspace, (int)nwg,
                            wrench_slots * 6, wrench);
     return launch_status();
 }
 
 int32_t mpm_blocks_mark(const mpm_particles_t* P, const mpm_grid_t* G, int32_t* raw, void* stream) {
-    if (!particles_ok(P) || !grid_ok(G) || !raw) return MPM_ERR_ARG;
+    if (!particles_ok(P) || !grid_ok(G) || !raw || (G->n_scenes > 1 && !P->scene)) return MPM_ERR_ARG;
     if (P->n == 0) return MPM_OK;
     hipLaunchKernelGGL(k_blocks_mark, dim3(blocks_for(P->n)), dim3(kThreads), 0, (hipStream_t)stream, *P, *G, raw);
     return launch_status();
@@ -594,7 +626,7 @@ int32_t mpm_blocks_mark(const mpm_particles_t* P, const mpm_grid_t* G, int32_t* 
 
 int32_t mpm_blocks_dilate(const mpm_grid_t* G, const int32_t* raw, int32_t* cur, void* stream) {
     if (!grid_ok(G) || !raw || !cur) return MPM_ERR_ARG;
-    long total = (long)G->nb[0] * G->nb[1] * G->nb[2];
+    long total = (long)G->nb[0] * G->nb[1] * G->nb[2] * G->n_scenes;
     hipLaunchKernelGGL(k_blocks_dilate, dim3(blocks_for(total)), dim3(kThreads), 0, (hipStream_t)stream, *G, raw, cur);
     return launch_status();
 }
@@ -602,7 +634,7 @@ int32_t mpm_blocks_dilate(const mpm_grid_t* G, const int32_t* raw, int32_t* cur,
 int32_t mpm_blocks_compact(const mpm_grid_t* G, const int32_t* cur, const int32_t* prev, int32_t* cur_list,
                            int32_t* union_list, int32_t* counts, void* stream) {
     if (!grid_ok(G) || !cur || !prev || !cur_list || !union_list || !counts) return MPM_ERR_ARG;
-    long total = (long)G->nb[0] * G->nb[1] * G->nb[2];
+    long total = (long)G->nb[0] * G->nb[1] * G->nb[2] * G->n_scenes;
     hipLaunchKernelGGL(k_blocks_compact, dim3(blocks_for(total)), dim3(kThreads), 0, (hipStream_t)stream, *G, cur,
                        prev, cur_list, union_list, counts);
     return launch_status();
@@ -621,6 +653,8 @@ int32_t mpm_bounds(const mpm_particles_t* P, float* out, void* stream) {
 int32_t mpm_sort_keys(const mpm_particles_t* P, const mpm_grid_t* G, int32_t* keys, int32_t* vals, int32_t* unsorted,
                       void* stream) {
     if (!particles_ok(P) || !grid_ok(G) || !keys || !vals || !unsorted) return MPM_ERR_ARG;
+    const long nbs = (G->n[0] >> 2) + 2;
+    if (nbs * nbs * nbs * G->n_scenes > 0x7fffffffL) return MPM_ERR_SIZE;
     if (P->n == 0) return MPM_OK;
     hipLaunchKernelGGL(k_sort_keys, dim3(blocks_for(P->n)), dim3(kThreads), 0, (hipStream_t)stream, *P, *G, keys, vals,
                        unsorted);
@@ -654,6 +688,7 @@ int32_t mpm_permute(const mpm_particles_t* P, const int32_t* perm, float* scratc
     r |= one(P->yield_stress, 1); r |= one(P->Jp, 1);
     if (P->cov) r |= one(P->cov, 6);
     r |= onei(P->material); r |= onei(P->selection);
+    if (P->scene) r |= onei(P->scene);
     if (r != MPM_OK) return MPM_ERR_HIP;
     return launch_status();
 }
@@ -675,7 +710,7 @@ int32_t mpm_export_cov(const mpm_particles_t* P, const float* init_cov, float* c
 
 int32_t mpm_grid_to_dense(const mpm_grid_t* G, float* m, float* v_in, float* v_out, void* stream) {
     if (!grid_ok(G)) return MPM_ERR_ARG;
-    long total = (long)G->n[0] * G->n[1] * G->n[2];
+    long total = (long)G->n[0] * G->n[1] * G->n[2] * G->n_scenes;
     hipLaunchKernelGGL(k_grid_to_dense, dim3(blocks_for(total)), dim3(kThreads), 0, (hipStream_t)stream, *G, m, v_in,
                        v_out);
     return launch_status();

@@ -1,9 +1,10 @@
 // The fused particle pass: gather (g2p_particle, mpm_utils.py:944-1043), return map +
 // Kirchhoff stress (stress_update_particle, :1056-1149) and scatter (p2g_particle,
-// :819-916) for 256 particles per workgroup.
+// :819-916) for 256 (or 128) particles per workgroup, each thread taking one or more
+// particles in turn (particles per thread).
 //
 // LDS staging (claymore-style, on block-sorted particles): the workgroup reduces the
-// stencil bounding box of its 256 particles; when the box holds at most kArena nodes it
+// stencil bounding box of its particles; when the box holds at most kArena nodes it
 // loads the gather box of node_out into LDS once (one 16-byte read per node instead of
 // 27 per particle) and scatters into an LDS arena with ds_add_f32, then flushes each
 // touched arena node to node_in with four global float atomics. Chunks whose box is
@@ -17,17 +18,22 @@
 
 namespace {
 
-constexpr int kT = 256;        // particles (threads) per workgroup
-constexpr int kArena = 768;    // LDS arena capacity in nodes (12 KB gather + 12 KB scatter)
+// Workgroup size kT (256 or 128, a launch-shape choice) is a template parameter; the LDS
+// arena holds 3 nodes per thread: 768 nodes (12 KB gather + 12 KB scatter) at 256.
+template <int kT>
+constexpr int arena_of() { return 3 * kT; }
 
-// An LDS tile: the node box [lo, lo + (d0, d1, d2)) staged at arena offset `base`.
-// Tier 1: the stencil box of all 256 particles of the workgroup fits kArena nodes (the
-// usual case for block-sorted particles). Tier 2 (`wg` false): each wavefront stages the
-// box of its own 64 particles in its quarter of the arena; a wavefront whose box still
-// does not fit (`lds` false) gathers and scatters through global memory.
+// An LDS tile: the node box [lo, lo + (d0, d1, d2)) of one scene (node offset `soff` in
+// the batched grid) staged at arena offset `base`.
+// Tier 1: the stencil box of all particles of the workgroup fits kArena nodes and they
+// belong to one scene (the usual case for block-sorted particles). Tier 2 (`wg` false):
+// each wavefront stages the box of its own 64 particles in its quarter of the arena; a
+// wavefront whose box still does not fit, or that spans two scenes (`lds` false), gathers
+// and scatters through global memory.
 struct Tile {
     int lo[3];
     int d1, d2, vol, base;
+    size_t soff;
     bool wg, lds;
 };
 
@@ -41,22 +47,30 @@ MPM_DEV bool fits(const int lo[3], const int hi[3], int cap, int& d0, int& d1, i
     return d0 * d1 * d2 <= cap;
 }
 
-// per-thread stencil box [lo, hi) (sentinels INT_MAX / INT_MIN when inactive) -> tile;
-// contains one __syncthreads, so every thread of the workgroup must call it
-MPM_DEV Tile make_tile(int lo[3], int hi[3], int (*red)[6], bool periodic) {
+// per-thread stencil box [lo, hi) and scene range [slo, shi] (sentinels INT_MAX / INT_MIN
+// when inactive) -> tile; contains one __syncthreads, so every thread of the workgroup
+// must call it
+template <int kT>
+MPM_DEV Tile make_tile(const mpm_grid_t& G, int lo[3], int hi[3], int slo, int shi, int (*red)[8], bool periodic) {
 #pragma unroll
-    for (int off = 32; off > 0; off >>= 1)
+    for (int off = 32; off > 0; off >>= 1) {
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
             lo[a] = min(lo[a], __shfl_xor(lo[a], off, 64));
             hi[a] = max(hi[a], __shfl_xor(hi[a], off, 64));
         }
+        slo = min(slo, __shfl_xor(slo, off, 64));
+        shi = max(shi, __shfl_xor(shi, off, 64));
+    }
     const int wave = threadIdx.x >> 6;
-    if ((threadIdx.x & 63) == 0)
+    if ((threadIdx.x & 63) == 0) {
         for (int a = 0; a < 3; ++a) {
             red[wave][a] = lo[a];
             red[wave][3 + a] = hi[a];
         }
+        red[wave][6] = slo;
+        red[wave][7] = shi;
+    }
     __syncthreads();
     int wlo[3], whi[3];
     for (int a = 0; a < 3; ++a) {
@@ -68,17 +82,25 @@ MPM_DEV Tile make_tile(int lo[3], int hi[3], int (*red)[6], bool periodic) {
         wlo[a] = l;
         whi[a] = h;
     }
+    int wslo = red[0][6], wshi = red[0][7];
+    for (int w = 1; w < kT / 64; ++w) {
+        wslo = min(wslo, red[w][6]);
+        wshi = max(wshi, red[w][7]);
+    }
+    constexpr int kArena = arena_of<kT>();
     Tile t;
     int d0, d1, d2;
-    if (!periodic && fits(wlo, whi, kArena, d0, d1, d2)) {
+    if (!periodic && wslo == wshi && fits(wlo, whi, kArena, d0, d1, d2)) {
         t.wg = true;
         t.lds = true;
         t.base = 0;
+        t.soff = scene_base(G, wslo);
         for (int a = 0; a < 3; ++a) t.lo[a] = wlo[a];
     } else {
         t.wg = false;
-        t.lds = !periodic && fits(lo, hi, kArena / (kT / 64), d0, d1, d2);
+        t.lds = !periodic && slo == shi && fits(lo, hi, kArena / (kT / 64), d0, d1, d2);
         t.base = wave * (kArena / (kT / 64));
+        t.soff = t.lds ? scene_base(G, slo) : 0;
         for (int a = 0; a < 3; ++a) t.lo[a] = lo[a];
     }
     t.d1 = d1;
@@ -89,7 +111,7 @@ MPM_DEV Tile make_tile(int lo[3], int hi[3], int (*red)[6], bool periodic) {
 
 // cooperative loop over the tile's nodes: the whole workgroup for a workgroup tile, the
 // owning wavefront for a wavefront tile
-template <class F>
+template <int kT, class F>
 MPM_DEV void tile_for(const Tile& t, F f) {
     if (!t.lds) return;
     const int start = t.wg ? (int)threadIdx.x : (int)(threadIdx.x & 63);
@@ -185,8 +207,9 @@ MPM_DEV float sel3(int i, const float* a) { return i == 0 ? a[0] : (i == 1 ? a[1
 // (unsorted or spread-out particles): a rolled loop over the 27 nodes with the weights
 // picked by selects, so this rare path stays small next to the LDS fast path it shares a
 // kernel (and a register budget) with.
-MPM_DEV void gather_global(const mpm_grid_t& G, const Stencil& st, bool periodic, v3& nv, m33& nC, m33& nF) {
-    const float4* __restrict__ out = reinterpret_cast<const float4*>(G.node_out);
+MPM_DEV void gather_global(const mpm_grid_t& G, size_t soff, const Stencil& st, bool periodic, v3& nv, m33& nC,
+                           m33& nF) {
+    const float4* __restrict__ out = reinterpret_cast<const float4*>(G.node_out) + soff;
     nv = mk3(0.f, 0.f, 0.f);
     nC = mzero();
     nF = mzero();
@@ -217,9 +240,9 @@ MPM_DEV void gather_global(const mpm_grid_t& G, const Stencil& st, bool periodic
     }
 }
 
-MPM_DEV void scatter_global(const mpm_grid_t& G, const Stencil& st, bool periodic, float mass, v3 v,
+MPM_DEV void scatter_global(const mpm_grid_t& G, size_t soff, const Stencil& st, bool periodic, float mass, v3 v,
                             const m33& Cm, const m33& fs) {
-    float* __restrict__ gin = G.node_in;
+    float* __restrict__ gin = G.node_in + 4 * soff;
 #pragma unroll 1
     for (int n = 0; n < 27; ++n) {
         const int i = n / 9, j = (n / 3) % 3, k = n % 3;
@@ -270,16 +293,21 @@ MPM_DEV bool stencil_in_grid(const mpm_grid_t& G, const Stencil& st, bool period
     return ok;
 }
 
-// Workgroups are dispatched round-robin over the 8 XCDs (each with its own L2). Chunk c of
-// block-sorted particles is handed to workgroup b with c = (b % 8) * q + b / 8, so every
-// XCD sweeps a contiguous range of chunks: spatial neighbours, whose scatter flushes
-// overlap on the same grid nodes, share one L2 instead of bouncing lines between L2s.
+// Workgroups are dispatched round-robin over the 8 XCDs (each with its own L2). Group g
+// of `ppt` consecutive chunks of block-sorted particles is handed to workgroup b with
+// g = (b % 8) * q + b / 8, so every XCD sweeps a contiguous range of chunks: spatial
+// neighbours, whose scatter flushes overlap on the same grid nodes, share one L2 instead
+// of bouncing lines between L2s. The workgroup processes its ppt chunks one after the
+// other (one particle per thread per chunk).
 constexpr int kXcds = 8;
 
-template <int MODE, bool kSolid>
-__global__ __launch_bounds__(kT) void k_particle_pass(mpm_particles_t P, mpm_grid_t G,
+// amdgpu_waves_per_eu(4): keep the allocation within 128 VGPRs (4 wavefronts per SIMD);
+// left alone the scheduler lands between 110 and 200 depending on unrelated edits.
+template <int MODE, bool kSolid, int kT>
+__global__ __launch_bounds__(kT) __attribute__((amdgpu_waves_per_eu(4))) void k_particle_pass(mpm_particles_t P, mpm_grid_t G,
                                                       const mpm_model_t* __restrict__ md, float dt,
-                                                      int xcd_q) {
+                                                      int xcd_q, int ppt) {
+    constexpr int kArena = arena_of<kT>();
     constexpr bool kG2P = (MODE & MPM_PASS_G2P) != 0;
     constexpr bool kStress = (MODE & MPM_PASS_STRESS) != 0;
     constexpr bool kP2G = (MODE & MPM_PASS_P2G) != 0;
@@ -287,85 +315,101 @@ __global__ __launch_bounds__(kT) void k_particle_pass(mpm_particles_t P, mpm_gri
     constexpr bool kKeepStress = (MODE & MPM_PASS_KEEP_STRESS) != 0;
     __shared__ float4 s_gather[kG2P ? kArena : 1];
     __shared__ float s_scatter[kP2G ? 4 : 1][kP2G ? kArena : 1];
-    __shared__ int s_red0[kT / 64][6];
-    __shared__ int s_red1[kT / 64][6];
+    __shared__ int s_red0[kT / 64][8];
+    __shared__ int s_red1[kT / 64][8];
 
     const int S = P.stride;
-    const int chunk = (int)(blockIdx.x % kXcds) * xcd_q + (int)(blockIdx.x / kXcds);
-    if (chunk * kT >= P.n) return;  // padding workgroup (uniform exit before any barrier)
-    const int p = chunk * kT + threadIdx.x;
-    const bool alive = p < P.n && P.selection[p] == 0;
-    const int mat = alive ? P.material[p] : 0;
-    const bool periodic = md->periodic_x != 0;
-    int err = 0;
+    const int group = (int)(blockIdx.x % kXcds) * xcd_q + (int)(blockIdx.x / kXcds);
+    for (int it = 0; it < ppt; ++it) {
+        const int chunk = group * ppt + it;
+        if (chunk * kT >= P.n) return;  // past the last chunk (uniform exit before any barrier)
+        if (it > 0) __syncthreads();    // the previous chunk's LDS reads are done
+        const int p = chunk * kT + threadIdx.x;
+        const bool sel = p < P.n && P.selection[p] == 0;
+        const int scene = (sel && P.scene) ? P.scene[p] : 0;
+        const bool alive = sel && (unsigned)scene < (unsigned)G.n_scenes;  // bad scene ids: error bit 8
+        const int mat = alive ? P.material[p] : 0;
+        const size_t soff = scene_base(G, alive ? scene : 0);
+        const bool periodic = md->periodic_x != 0;
+        int err = (sel && !alive) ? 8 : 0;
 
-    v3 x = alive ? ld3(P.x, S, p) : mk3(0.f, 0.f, 0.f);
-    v3 v = mk3(0.f, 0.f, 0.f);
-    m33 Cm = mzero(), L = mzero(), Ft = mzero();
+        v3 x = alive ? ld3(P.x, S, p) : mk3(0.f, 0.f, 0.f);
+        v3 v = mk3(0.f, 0.f, 0.f);
+        m33 Cm = mzero(), L = mzero(), Ft = mzero();
 
-    if constexpr (kG2P) {
-        const bool gather = alive && mat != 7 && mat != 8;
-        const Stencil st = make_stencil(x, G.inv_dx);
-        int lo[3], hi[3];
-#pragma unroll
-        for (int a = 0; a < 3; ++a) {
-            lo[a] = gather ? st.base[a] : INT_MAX;
-            hi[a] = gather ? st.base[a] + 3 : INT_MIN;
-        }
-        const Tile tg = make_tile(lo, hi, s_red0, periodic);
-        {
-            const float4* __restrict__ out = reinterpret_cast<const float4*>(G.node_out);
-            tile_for(tg, [&](int a, int gi, int gj, int gk) {
-                s_gather[a] = in_grid(G, gi, gj, gk) ? out[node_index(G, gi, gj, gk)] : make_float4(0.f, 0.f, 0.f, 0.f);
-            });
-        }
-        __syncthreads();
-        if (gather) {
-            if (!stencil_in_grid(G, st, periodic)) err |= 1;
-            v3 nv;
-            m33 nC, nF;
-            if (tg.lds) {
-                const int o0 = st.base[0] - tg.lo[0], o1 = st.base[1] - tg.lo[1], o2 = st.base[2] - tg.lo[2];
-                const int d1 = tg.d1, d2 = tg.d2, b0 = tg.base;
-                gather_tp(st, G.inv_dx,
-                          [&](int i, int j, int k) {
-                              const float4 g = s_gather[b0 + ((o0 + i) * d1 + (o1 + j)) * d2 + (o2 + k)];
-                              return mk3(g.x, g.y, g.z);
-                          },
-                          nv, nC, nF);
-            } else {
-                gather_global(G, st, periodic, nv, nC, nF);
+        if constexpr (kG2P) {
+            const bool gather = alive && mat != 7 && mat != 8;
+            const Stencil st = make_stencil(x, G.inv_dx);
+            int lo[3], hi[3];
+    #pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                lo[a] = gather ? st.base[a] : INT_MAX;
+                hi[a] = gather ? st.base[a] + 3 : INT_MIN;
             }
-            v = nv;
-            Cm = nC;
-            L = nF;
-            x = x + v * dt;
-            if (periodic) {
-                const float lx = (float)G.n[0] * G.dx;
-                if (x.x < 0.0f) x.x = x.x + lx;
-                if (x.x >= lx) x.x = x.x - lx;
+            const Tile tg = make_tile<kT>(G, lo, hi, gather ? scene : INT_MAX, gather ? scene : INT_MIN, s_red0, periodic);
+            {
+                const float4* __restrict__ out = reinterpret_cast<const float4*>(G.node_out) + tg.soff;
+                tile_for<kT>(tg, [&](int a, int gi, int gj, int gk) {
+                    s_gather[a] = in_grid(G, gi, gj, gk) ? out[node_index(G, gi, gj, gk)] : make_float4(0.f, 0.f, 0.f, 0.f);
+                });
             }
-            m33 A = mscale(nF, dt);
-            A.a[0] += 1.f; A.a[4] += 1.f; A.a[8] += 1.f;
-            Ft = mmul(A, ld9(P.F, S, p));
-            if (md->update_cov_with_F && P.cov) {  // update_cov, mpm_utils.py:613-633
-                float c[6];
-#pragma unroll
-                for (int q = 0; q < 6; ++q) c[q] = P.cov[q * S + p];
-                m33 cn;
-                cn.a[0] = c[0]; cn.a[1] = c[1]; cn.a[2] = c[2];
-                cn.a[3] = c[1]; cn.a[4] = c[3]; cn.a[5] = c[4];
-                cn.a[6] = c[2]; cn.a[7] = c[4]; cn.a[8] = c[5];
-                m33 n1 = madd(cn, mscale(madd(mmul(nF, cn), mmul(cn, mT(nF))), dt));
-                P.cov[p] = n1.a[0]; P.cov[S + p] = n1.a[1]; P.cov[2 * S + p] = n1.a[2];
-                P.cov[3 * S + p] = n1.a[4]; P.cov[4 * S + p] = n1.a[5]; P.cov[5 * S + p] = n1.a[8];
-            }
-            st3(P.x, S, p, x);
-            if (!kLean) {
-                st3(P.v, S, p, v);
-                st9(P.C, S, p, Cm);
-                st9(P.L, S, p, L);
-                st9(P.F_trial, S, p, Ft);
+            __syncthreads();
+            if (gather) {
+                if (!stencil_in_grid(G, st, periodic)) err |= 1;
+                v3 nv;
+                m33 nC, nF;
+                if (tg.lds) {
+                    const int o0 = st.base[0] - tg.lo[0], o1 = st.base[1] - tg.lo[1], o2 = st.base[2] - tg.lo[2];
+                    const int d1 = tg.d1, d2 = tg.d2, b0 = tg.base;
+                    gather_tp(st, G.inv_dx,
+                              [&](int i, int j, int k) {
+                                  const float4 g = s_gather[b0 + ((o0 + i) * d1 + (o1 + j)) * d2 + (o2 + k)];
+                                  return mk3(g.x, g.y, g.z);
+                              },
+                              nv, nC, nF);
+                } else {
+                    gather_global(G, soff, st, periodic, nv, nC, nF);
+                }
+                v = nv;
+                Cm = nC;
+                L = nF;
+                x = x + v * dt;
+                if (periodic) {
+                    const float lx = (float)G.n[0] * G.dx;
+                    if (x.x < 0.0f) x.x = x.x + lx;
+                    if (x.x >= lx) x.x = x.x - lx;
+                }
+                m33 A = mscale(nF, dt);
+                A.a[0] += 1.f; A.a[4] += 1.f; A.a[8] += 1.f;
+                Ft = mmul(A, ld9(P.F, S, p));
+                if (md->update_cov_with_F && P.cov) {  // update_cov, mpm_utils.py:613-633
+                    float c[6];
+    #pragma unroll
+                    for (int q = 0; q < 6; ++q) c[q] = P.cov[q * S + p];
+                    m33 cn;
+                    cn.a[0] = c[0]; cn.a[1] = c[1]; cn.a[2] = c[2];
+                    cn.a[3] = c[1]; cn.a[4] = c[3]; cn.a[5] = c[4];
+                    cn.a[6] = c[2]; cn.a[7] = c[4]; cn.a[8] = c[5];
+                    m33 n1 = madd(cn, mscale(madd(mmul(nF, cn), mmul(cn, mT(nF))), dt));
+                    P.cov[p] = n1.a[0]; P.cov[S + p] = n1.a[1]; P.cov[2 * S + p] = n1.a[2];
+                    P.cov[3 * S + p] = n1.a[4]; P.cov[4 * S + p] = n1.a[5]; P.cov[5 * S + p] = n1.a[8];
+                }
+                st3(P.x, S, p, x);
+                if (!kLean) {
+                    st3(P.v, S, p, v);
+                    st9(P.C, S, p, Cm);
+                    st9(P.L, S, p, L);
+                    st9(P.F_trial, S, p, Ft);
+                }
+            } else if (alive) {
+                if (kP2G) {
+                    v = ld3(P.v, S, p);
+                    Cm = ld9(P.C, S, p);
+                }
+                if (kStress) {
+                    Ft = ld9(P.F_trial, S, p);
+                    if (mat == 10 || mat == 12) L = ld9(P.L, S, p);
+                }
             }
         } else if (alive) {
             if (kP2G) {
@@ -377,99 +421,90 @@ __global__ __launch_bounds__(kT) void k_particle_pass(mpm_particles_t P, mpm_gri
                 if (mat == 10 || mat == 12) L = ld9(P.L, S, p);
             }
         }
-    } else if (alive) {
-        if (kP2G) {
-            v = ld3(P.v, S, p);
-            Cm = ld9(P.C, S, p);
-        }
-        if (kStress) {
-            Ft = ld9(P.F_trial, S, p);
-            if (mat == 10 || mat == 12) L = ld9(P.L, S, p);
-        }
-    }
 
-    m33 stress = mzero();
-    if constexpr (kStress) {
-        if (alive) {
-            PState ps;
-            ps.mu = P.mu[p];
-            ps.lam = P.lam[p];
-            ps.yield = P.yield_stress[p];
-            ps.Jp = P.Jp[p];
-            ps.dirty_mu = ps.dirty_yield = ps.dirty_Jp = false;
-            m33 F;
-            if constexpr (kSolid) {
-                stress_update(Ft, L, mat, md, dt, ps, F, stress);
-            } else {
-                if (!stress_update_viscous(Ft, L, mat, md, ps, F, stress)) err |= 4;
+        m33 stress = mzero();
+        if constexpr (kStress) {
+            if (alive) {
+                PState ps;
+                ps.mu = P.mu[p];
+                ps.lam = P.lam[p];
+                ps.yield = P.yield_stress[p];
+                ps.Jp = P.Jp[p];
+                ps.dirty_mu = ps.dirty_yield = ps.dirty_Jp = false;
+                m33 F;
+                if constexpr (kSolid) {
+                    stress_update(Ft, L, mat, md, dt, ps, F, stress);
+                } else {
+                    if (!stress_update_viscous(Ft, L, mat, md, ps, F, stress)) err |= 4;
+                }
+                st9(P.F, S, p, F);
+                if (!kLean || kKeepStress) st9(P.stress, S, p, stress);
+                if (ps.dirty_yield) P.yield_stress[p] = ps.yield;
+                if (ps.dirty_mu) { P.mu[p] = ps.mu; P.lam[p] = ps.lam; }
+                if (ps.dirty_Jp) P.Jp[p] = ps.Jp;
             }
-            st9(P.F, S, p, F);
-            if (!kLean || kKeepStress) st9(P.stress, S, p, stress);
-            if (ps.dirty_yield) P.yield_stress[p] = ps.yield;
-            if (ps.dirty_mu) { P.mu[p] = ps.mu; P.lam[p] = ps.lam; }
-            if (ps.dirty_Jp) P.Jp[p] = ps.Jp;
+        } else if (kP2G && alive) {
+            stress = ld9(P.stress, S, p);
         }
-    } else if (kP2G && alive) {
-        stress = ld9(P.stress, S, p);
-    }
 
-    if constexpr (kP2G) {
-        const Stencil st = make_stencil(x, G.inv_dx);
-        int lo[3], hi[3];
-#pragma unroll
-        for (int a = 0; a < 3; ++a) {
-            lo[a] = alive ? st.base[a] : INT_MAX;
-            hi[a] = alive ? st.base[a] + 3 : INT_MIN;
-        }
-        const Tile ts = make_tile(lo, hi, s_red1, periodic);
-        tile_for(ts, [&](int a, int, int, int) {
-            s_scatter[0][a] = 0.f;
-            s_scatter[1][a] = 0.f;
-            s_scatter[2][a] = 0.f;
-            s_scatter[3][a] = 0.f;
-        });
-        __syncthreads();
-        if (alive) {
-            const float r = md->rpic_damping;
-            if (r != 0.0f) {
-                const m33 Ct = mT(Cm);
-                Cm = madd(mscale(Cm, 1.0f - r), mscale(msub(Cm, Ct), r / 2.0f));
-                if (r < -0.001f) Cm = mzero();
+        if constexpr (kP2G) {
+            const Stencil st = make_stencil(x, G.inv_dx);
+            int lo[3], hi[3];
+    #pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                lo[a] = alive ? st.base[a] : INT_MAX;
+                hi[a] = alive ? st.base[a] + 3 : INT_MIN;
             }
-            const m33 fs = mscale(stress, -P.vol[p] * dt);
-            if (!stencil_in_grid(G, st, periodic)) err |= 1;
-            if (ts.lds) {
-                const int o0 = st.base[0] - ts.lo[0], o1 = st.base[1] - ts.lo[1], o2 = st.base[2] - ts.lo[2];
-                const int d1 = ts.d1, d2 = ts.d2, b0 = ts.base;
-                scatter_rotated(st, G.dx, G.inv_dx, P.mass[p], v, Cm, fs, (int)(threadIdx.x % 27u),
-                                [&](int i, int j, int k, v3 add, float wm) {
-                                    const int a = b0 + ((o0 + i) * d1 + (o1 + j)) * d2 + (o2 + k);
-                                    atomicAdd(&s_scatter[0][a], add.x);
-                                    atomicAdd(&s_scatter[1][a], add.y);
-                                    atomicAdd(&s_scatter[2][a], add.z);
-                                    atomicAdd(&s_scatter[3][a], wm);
-                                });
-            } else {
-                scatter_global(G, st, periodic, P.mass[p], v, Cm, fs);
-            }
-        }
-        __syncthreads();
-        {
-            float* __restrict__ gin = G.node_in;
-            tile_for(ts, [&](int a, int gi, int gj, int gk) {
-                const float a0 = s_scatter[0][a], a1 = s_scatter[1][a], a2 = s_scatter[2][a], a3 = s_scatter[3][a];
-                if (a0 == 0.f && a1 == 0.f && a2 == 0.f && a3 == 0.f) return;
-                if (!in_grid(G, gi, gj, gk)) return;
-                const size_t o = 4 * (size_t)node_index(G, gi, gj, gk);
-                atomicAdd(gin + o + 0, a0);
-                atomicAdd(gin + o + 1, a1);
-                atomicAdd(gin + o + 2, a2);
-                atomicAdd(gin + o + 3, a3);
+            const Tile ts = make_tile<kT>(G, lo, hi, alive ? scene : INT_MAX, alive ? scene : INT_MIN, s_red1, periodic);
+            tile_for<kT>(ts, [&](int a, int, int, int) {
+                s_scatter[0][a] = 0.f;
+                s_scatter[1][a] = 0.f;
+                s_scatter[2][a] = 0.f;
+                s_scatter[3][a] = 0.f;
             });
+            __syncthreads();
+            if (alive) {
+                const float r = md->rpic_damping;
+                if (r != 0.0f) {
+                    const m33 Ct = mT(Cm);
+                    Cm = madd(mscale(Cm, 1.0f - r), mscale(msub(Cm, Ct), r / 2.0f));
+                    if (r < -0.001f) Cm = mzero();
+                }
+                const m33 fs = mscale(stress, -P.vol[p] * dt);
+                if (!stencil_in_grid(G, st, periodic)) err |= 1;
+                if (ts.lds) {
+                    const int o0 = st.base[0] - ts.lo[0], o1 = st.base[1] - ts.lo[1], o2 = st.base[2] - ts.lo[2];
+                    const int d1 = ts.d1, d2 = ts.d2, b0 = ts.base;
+                    scatter_rotated(st, G.dx, G.inv_dx, P.mass[p], v, Cm, fs, (int)(threadIdx.x % 27u),
+                                    [&](int i, int j, int k, v3 add, float wm) {
+                                        const int a = b0 + ((o0 + i) * d1 + (o1 + j)) * d2 + (o2 + k);
+                                        atomicAdd(&s_scatter[0][a], add.x);
+                                        atomicAdd(&s_scatter[1][a], add.y);
+                                        atomicAdd(&s_scatter[2][a], add.z);
+                                        atomicAdd(&s_scatter[3][a], wm);
+                                    });
+                } else {
+                    scatter_global(G, soff, st, periodic, P.mass[p], v, Cm, fs);
+                }
+            }
+            __syncthreads();
+            {
+                float* __restrict__ gin = G.node_in + 4 * ts.soff;
+                tile_for<kT>(ts, [&](int a, int gi, int gj, int gk) {
+                    const float a0 = s_scatter[0][a], a1 = s_scatter[1][a], a2 = s_scatter[2][a], a3 = s_scatter[3][a];
+                    if (a0 == 0.f && a1 == 0.f && a2 == 0.f && a3 == 0.f) return;
+                    if (!in_grid(G, gi, gj, gk)) return;
+                    const size_t o = 4 * (size_t)node_index(G, gi, gj, gk);
+                    atomicAdd(gin + o + 0, a0);
+                    atomicAdd(gin + o + 1, a1);
+                    atomicAdd(gin + o + 2, a2);
+                    atomicAdd(gin + o + 3, a3);
+                });
+            }
         }
-    }
-    if (alive && !(x.x == x.x && x.y == x.y && x.z == x.z)) err |= 2;
-    if (err && G.error_flag) atomicOr(G.error_flag, err);
+        if (alive && !(x.x == x.x && x.y == x.y && x.z == x.z)) err |= 2;
+        if (err && G.error_flag) atomicOr(G.error_flag, err);
+    }  // chunk loop
 }
 
 inline int32_t launch_status() { return hipGetLastError() == hipSuccess ? MPM_OK : MPM_ERR_HIP; }
@@ -481,20 +516,29 @@ extern "C" int32_t mpm_particle_pass(const mpm_particles_t* P, const mpm_grid_t*
     if (!P || !G || !model || P->n < 0 || P->stride < P->n || !P->x || !P->v || !P->F || !P->F_trial || !P->C ||
         !P->stress || !P->L || !P->vol || !P->mass || !P->mu || !P->lam || !P->yield_stress || !P->Jp ||
         !P->material || !P->selection || !G->node_in || !G->node_out || G->n[0] <= 0 || G->n[1] <= 0 ||
-        G->n[2] <= 0 || G->nb[0] != (G->n[0] + 3) / 4 || G->nb[1] != (G->n[1] + 3) / 4 ||
+        G->n[2] <= 0 || G->n_scenes < 1 || (G->n_scenes > 1 && !P->scene) || G->nb[0] != (G->n[0] + 3) / 4 || G->nb[1] != (G->n[1] + 3) / 4 ||
         G->nb[2] != (G->n[2] + 3) / 4)
         return MPM_ERR_ARG;
     if (P->n == 0) return MPM_OK;
+    const int wg = (mode & MPM_PASS_WG128) ? 128 : 256;
+    const int ppt = (mode & MPM_PASS_PPT4) ? 4 : ((mode & MPM_PASS_PPT2) ? 2 : 1);
     hipStream_t s = (hipStream_t)stream;
-    const int nchunks = (P->n + kT - 1) / kT;
-    const int q = (nchunks + kXcds - 1) / kXcds;
-    const dim3 grid((unsigned)(q * kXcds)), block(kT);
+    const int nchunks = (P->n + wg - 1) / wg;
+    const int ngroups = (nchunks + ppt - 1) / ppt;
+    const int q = (ngroups + kXcds - 1) / kXcds;
+    const dim3 grid((unsigned)(q * kXcds)), block(wg);
     const bool solid = (mode & MPM_PASS_VISCOUS_ONLY) == 0;
     const int m = mode & (MPM_PASS_G2P | MPM_PASS_STRESS | MPM_PASS_P2G | MPM_PASS_LEAN | MPM_PASS_KEEP_STRESS);
-#define CASE(M)                                                                                   \
-    case (M):                                                                                     \
-        if (solid) hipLaunchKernelGGL((k_particle_pass<(M), true>), grid, block, 0, s, *P, *G, model, dt, q); \
-        else hipLaunchKernelGGL((k_particle_pass<(M), false>), grid, block, 0, s, *P, *G, model, dt, q);     \
+#define LAUNCH(M, T)                                                                                               \
+    if (solid) hipLaunchKernelGGL((k_particle_pass<(M), true, T>), grid, block, 0, s, *P, *G, model, dt, q, ppt); \
+    else hipLaunchKernelGGL((k_particle_pass<(M), false, T>), grid, block, 0, s, *P, *G, model, dt, q, ppt);
+#define CASE(M)            \
+    case (M):              \
+        if (wg == 128) {   \
+            LAUNCH(M, 128) \
+        } else {           \
+            LAUNCH(M, 256) \
+        }                  \
         break;
     switch (m) {
         CASE(MPM_PASS_G2P)
@@ -506,6 +550,7 @@ extern "C" int32_t mpm_particle_pass(const mpm_particles_t* P, const mpm_grid_t*
         CASE(MPM_PASS_G2P | MPM_PASS_STRESS | MPM_PASS_P2G | MPM_PASS_LEAN | MPM_PASS_KEEP_STRESS)
         default: return MPM_ERR_ARG;
     }
+#undef LAUNCH
 #undef CASE
     return launch_status();
 }

@@ -159,60 +159,81 @@ class ColliderSet:
         raise AssertionError(c.aabb_kind)
 
     # ------------------------------------------------------------------ schedule
+    @staticmethod
+    def times(time0: float, dt: float, substeps: int):
+        """float64 start time of every substep (time += dt per substep, as the reference's
+        host loop accumulates it) and the end time."""
+        times = np.empty(substeps, dtype=np.float64)
+        t = float(time0)
+        for s in range(substeps):
+            times[s] = t
+            t = t + dt
+        return times, t
+
     def schedule(self, time0: float, dt: float, substeps: int):
         """Rows [substeps, ncoll] of mpm_collider_t for one tick starting at time0, and
         the end time. Advances every collider's pose through the tick exactly like the
         reference's per-substep modify_bc calls."""
         S, K = substeps, len(self.items)
-        times = np.empty(S, dtype=np.float64)
-        t = float(time0)
-        for s in range(S):
-            times[s] = t
-            t = t + dt
+        times, t = self.times(time0, dt, S)
         tab = np.zeros((S, K), dtype=native.COLLIDER_DTYPE)
         if K == 0:
             return tab, t
         tab["time"] = times.astype(f32)[:, None]
         tab["dt"] = dt
         for k, c in enumerate(self.items):
-            col = tab[:, k]
-            col["kind"] = c.kind
-            col["surface"] = c.surface
-            col["reset"] = c.reset
-            col["slot"] = c.slot
-            col["start_time"] = c.start_time
-            col["end_time"] = c.end_time
-            col["normal"] = c.normal
-            col["size"] = c.size
-            col["friction"] = c.friction
-            col["sticky_depth"] = c.sticky_depth
-            col["contact_band"] = c.contact_band
-            col["band"] = c.band
-            (col["outer_radius"], col["inner_radius"], col["half_height"],
-             col["inner_floor_z"], col["fillet_radius"]) = c.shape
-            if c.sdf is not None:
-                val, grad, res, origin, cell, _ = c.sdf
-                col["sdf_val"], col["sdf_grad"], col["res"] = val.data_ptr(), grad.data_ptr(), res
-                col["origin"] = origin
-                col["cell"] = cell
-            if c.occupancy is not None:
-                col["occupancy"] = c.occupancy.data_ptr()
-            if c.moving:
-                self._integrate(c, col, times, dt)
-            else:
-                col["point"] = c.point
-                col["velocity"] = c.velocity
-                col["omega"] = c.omega
-                col["rot"] = np.asarray(c.rot, dtype=f32).ravel()
-                col["quat"] = c.quat
-                box = self.aabb(c)
-                if box is None:
-                    col["active"] = 0
-                else:
-                    col["active"] = 1
-                    col["lo"] = box[0]
-                    col["hi"] = [box[0][i] + box[1][i] for i in range(3)]
+            self.fill(c, tab[:, k], times, dt)
         return tab, t
+
+    def fill(self, c: Collider, col, times, dt):
+        """Write collider c's rows for the substeps at `times` into col (all but the
+        time / dt fields), advancing a moving collider's pose."""
+        col["kind"] = c.kind
+        col["surface"] = c.surface
+        col["reset"] = c.reset
+        col["slot"] = c.slot
+        col["start_time"] = c.start_time
+        col["end_time"] = c.end_time
+        col["normal"] = c.normal
+        col["size"] = c.size
+        col["friction"] = c.friction
+        col["sticky_depth"] = c.sticky_depth
+        col["contact_band"] = c.contact_band
+        col["band"] = c.band
+        (col["outer_radius"], col["inner_radius"], col["half_height"],
+         col["inner_floor_z"], col["fillet_radius"]) = c.shape
+        if c.sdf is not None:
+            val, grad, res, origin, cell, _ = c.sdf
+            col["sdf_val"], col["sdf_grad"], col["res"] = val.data_ptr(), grad.data_ptr(), res
+            col["origin"] = origin
+            col["cell"] = cell
+        if c.occupancy is not None:
+            col["occupancy"] = c.occupancy.data_ptr()
+        if c.moving:
+            self._integrate(c, col, times, dt)
+        else:
+            col["point"] = c.point
+            col["velocity"] = c.velocity
+            col["omega"] = c.omega
+            col["rot"] = np.asarray(c.rot, dtype=f32).ravel()
+            col["quat"] = c.quat
+            box = self.aabb(c)
+            if box is None:
+                col["active"] = 0
+            else:
+                col["active"] = 1
+                col["lo"] = box[0]
+                col["hi"] = [box[0][i] + box[1][i] for i in range(3)]
+
+    def static_row(self, c: Collider):
+        """The (substep-independent) row of a non-moving collider, built once: plane,
+        wall and point-cloud colliders have no pose setters."""
+        row = getattr(c, "_static_row", None)
+        if row is None or c._static_restrict != self.restrict_bc:
+            row = np.zeros(1, dtype=native.COLLIDER_DTYPE)
+            self.fill(c, row, np.zeros(1), 0.0)
+            c._static_row, c._static_restrict = row, self.restrict_bc
+        return row[0]
 
     def _integrate(self, c: Collider, col, times, dt):
         S = len(times)
@@ -297,3 +318,106 @@ class ColliderSet:
         col["lo"] = lo
         col["hi"] = hi
         col["active"] = act.astype(np.int32)
+
+
+def _schedule_boxes(cols, sets, blk, times, dt):
+    """Vectorised pose integration of one kinematic box per scene (the same fp32
+    round trip as ColliderSet._integrate: p <- fp32(p + dt v) while live) into blk
+    [substeps, n_scenes]; advances every box."""
+    S, ns = len(times), len(cols)
+    P = np.array([c.point for c in cols], dtype=np.float64)
+    V = np.array([c.velocity for c in cols], dtype=np.float64)
+    st = np.array([c.start_time for c in cols], dtype=np.float64)
+    en = np.array([c.end_time for c in cols], dtype=np.float64)
+    pts = np.empty((S, ns, 3), dtype=np.float64)
+    for s in range(S):
+        pts[s] = P
+        live = (st <= times[s]) & (times[s] < en)
+        if live.any():
+            P = np.where(live[:, None], (P + dt * V).astype(f32).astype(np.float64), P)
+    for c, p in zip(cols, P):
+        c.point = [float(v) for v in p]
+    for fld in ("kind", "surface", "reset", "slot", "start_time", "end_time", "friction"):
+        blk[fld] = np.array([getattr(c, fld) for c in cols])[None, :]
+    for fld in ("normal", "size", "velocity", "omega", "quat"):
+        blk[fld] = np.array([getattr(c, fld) for c in cols], dtype=f32)[None, :, :]
+    blk["rot"] = np.array([np.asarray(c.rot, dtype=f32).ravel() for c in cols])[None, :, :]
+    blk["point"] = pts.astype(f32)
+    cs0 = sets[0]
+    if not cs0.restrict_bc:
+        blk["active"] = 1
+        blk["lo"] = (0, 0, 0)
+        blk["hi"] = cs0.n
+        return
+    ptsf = pts.astype(f32).astype(np.float64).reshape(-1, 3)
+    size = np.broadcast_to(np.array([c.size for c in cols], dtype=np.float64)[None], (S, ns, 3)).reshape(-1, 3)
+    lo, hi, act = _grid_box_vec(ptsf - size, ptsf + size, cs0.dx, cs0.n)
+    blk["lo"] = lo.reshape(S, ns, 3)
+    blk["hi"] = hi.reshape(S, ns, 3)
+    blk["active"] = act.reshape(S, ns).astype(np.int32)
+
+
+class SceneColliders:
+    """One ColliderSet per batched scene and the per-tick table builder: rows
+    [substeps, n_scenes * K], scene-major within a substep as the grid kernel reads them
+    (K = the most colliders of any scene; shorter scenes are padded with inactive rows).
+    Wrench slots become global: scene * slots_per_scene + the scene's own slot. The rows
+    of non-moving colliders are built once into a template; each tick copies it and
+    integrates the moving colliders (kinematic boxes vectorised over scenes)."""
+
+    def __init__(self, n_scenes: int, n_grid: int, dx: float, slots_per_scene: int):
+        self.sets = [ColliderSet(n_grid, dx) for _ in range(n_scenes)]
+        self.slots_per_scene = int(slots_per_scene)
+        self._template = None
+        self._sig = None
+
+    def _signature(self):
+        return tuple((len(cs.items), cs.restrict_bc) for cs in self.sets)
+
+    def _build_template(self, K):
+        ns = len(self.sets)
+        tpl = np.zeros((ns, K), dtype=native.COLLIDER_DTYPE)
+        tpl["slot"] = -1
+        for s, cs in enumerate(self.sets):
+            if cs.n_slots > self.slots_per_scene:
+                raise ValueError(f"scene {s} has {cs.n_slots} wrench colliders, more than the "
+                                 f"{self.slots_per_scene} slots reserved per scene")
+            for k, c in enumerate(cs.items):
+                if not c.moving:
+                    tpl[s, k] = cs.static_row(c)
+        slot = tpl["slot"]
+        scene = np.arange(ns, dtype=np.int32)[:, None]
+        tpl["slot"] = np.where(slot >= 0, slot + scene * self.slots_per_scene, -1)
+        return tpl
+
+    def schedule(self, time0: float, dt: float, substeps: int):
+        """(table, end time, K, total wrench slots) for one tick starting at time0."""
+        ns = len(self.sets)
+        K = max((len(cs.items) for cs in self.sets), default=0)
+        times, t_end = ColliderSet.times(time0, dt, substeps)
+        sig = self._signature()
+        if self._template is None or self._sig != sig:
+            self._template, self._sig = self._build_template(K), sig
+        tab = np.empty((substeps, ns, K), dtype=native.COLLIDER_DTYPE)
+        tab[:] = self._template[None]
+        if K == 0:
+            return tab.reshape(substeps, 0), t_end, 0, ns * self.slots_per_scene
+        same_restrict = len({cs.restrict_bc for cs in self.sets}) <= 1
+        scene_slot = np.arange(ns, dtype=np.int32) * self.slots_per_scene
+        for k in range(K):
+            cols = [cs.items[k] if k < len(cs.items) else None for cs in self.sets]
+            if not any(c is not None and c.moving for c in cols):
+                continue
+            blk = tab[:, :, k]
+            if same_restrict and all(c is not None and c.kind == native.COLLIDER_BOX and
+                                     c.aabb_kind == "box" for c in cols):
+                _schedule_boxes(cols, self.sets, blk, times, dt)
+            else:
+                for s, c in enumerate(cols):
+                    if c is not None and c.moving:
+                        self.sets[s].fill(c, blk[:, s], times, dt)
+            slot = blk["slot"]
+            blk["slot"] = np.where(slot >= 0, slot + scene_slot[None, :], -1)
+        tab["time"] = times.astype(f32)[:, None, None]
+        tab["dt"] = dt
+        return tab.reshape(substeps, ns * K), t_end, K, ns * self.slots_per_scene

@@ -79,7 +79,8 @@ def _resolve_device(device: str) -> str:
 class HipSimulator:
     """Device state and kernel orchestration (the role of MPM_Simulator_WARP)."""
 
-    def __init__(self, pos, vol, n_grid: int, grid_lim: float, device: str, cov=None):
+    def __init__(self, pos, vol, n_grid: int, grid_lim: float, device: str, cov=None,
+                 scene=None, n_scenes: int = 1):
         import torch
 
         self.torch = torch
@@ -111,6 +112,13 @@ class HipSimulator:
         self.Jp = t(n)
         self.material = t(n, dt=torch.int32)
         self.selection = t(n, dt=torch.int32)
+        # batched scenes: one grid per scene back to back, scene id per particle
+        self.n_scenes = int(n_scenes)
+        self.scene = None
+        if scene is not None:
+            self.scene = torch.from_numpy(np.asarray(scene, np.int32).copy()).to(dev)
+        elif self.n_scenes > 1:
+            raise ValueError("n_scenes > 1 needs a scene id per particle")
         self.init_cov = None
         self.cov = None
         if cov is not None:
@@ -119,7 +127,8 @@ class HipSimulator:
         # blocked dense grid
         nb = (self.n_grid + 3) // 4
         self.nb = (nb, nb, nb)
-        self.n_blocks = nb ** 3
+        self.blocks_per_scene = nb ** 3
+        self.n_blocks = self.blocks_per_scene * self.n_scenes
         self.node_in = t(self.n_blocks * 64 * 4)
         self.node_out = t(self.n_blocks * 64 * 4)
         self.error_flag = t(1, dt=torch.int32)
@@ -151,7 +160,10 @@ class HipSimulator:
         self._mat_ids = {0}  # material ids present (selects the pass specialisation)
         # colliders
         self.colliders = ColliderSet(self.n_grid, self.dx)
+        self.scene_colliders = None  # batched scenes: a colliders.SceneColliders
+        self.n_slots = 0
         self.wrench = t(6 * 64)
+        self.launch_bits = 0  # particle-pass launch shape (PASS_WG128 / PASS_PPT2 / PASS_PPT4)
         self.grid_ws = None
         self.sched_dev = None
         self._sched_host = None
@@ -175,12 +187,14 @@ class HipSimulator:
                    "yield_stress", "Jp", "material", "selection"):
             setattr(P, nm, native.ptr(getattr(self, nm)))
         P.cov = native.ptr(self.cov)
+        P.scene = native.ptr(self.scene)
         G = native.Grid()
         G.node_in, G.node_out = native.ptr(self.node_in), native.ptr(self.node_out)
         G.n[:] = [self.n_grid] * 3
         G.nb[:] = list(self.nb)
         G.dx, G.inv_dx = self.dx, self.inv_dx
         G.error_flag = native.ptr(self.error_flag)
+        G.n_scenes = self.n_scenes
         self.P, self.G = P, G
         self.pP, self.pG = C.byref(P), C.byref(G)
 
@@ -407,15 +421,16 @@ class HipSimulator:
         self._upload_model()
         L = self.lib
         s = self.stream()
-        tab, t_end = self.colliders.schedule(self.time, dt, S)
-        K = tab.shape[1]
-        if self.colliders.n_slots > self.wrench.numel() // 6:
-            raise ValueError("at most 64 wrench-reading colliders are supported")
+        tab, t_end, K, n_slots = self._schedule(dt, S)
+        self.n_slots = n_slots
+        if 6 * n_slots > self.wrench.numel():  # grow, keeping the accumulated values
+            w = torch.zeros(6 * n_slots, dtype=torch.float32, device=self.device)
+            w[:self.wrench.numel()] = self.wrench
+            self.wrench = w
+            self._graph = None
         nbytes = tab.nbytes
         self._ensure_sched(nbytes)
-        ws = int(L.mpm_grid_workspace_floats(self.n_blocks, self.colliders.n_slots))
-        if ws < 0:
-            raise ValueError("at most 64 wrench-reading colliders are supported")
+        ws = int(L.mpm_grid_workspace_floats(self.n_blocks, n_slots))
         if self.grid_ws is None or self.grid_ws.numel() < max(ws, 1):
             self.grid_ws = torch.zeros(max(ws, 1), dtype=torch.float32, device=self.device)
             self._graph = None
@@ -455,8 +470,8 @@ class HipSimulator:
             self._enqueue_tick_profiled(dt, S, K, fused, lists)
         elif self.use_graph and self._ticks > 0:
             key = (S, float(dt), bool(fused), K, lists, native.ptr(self.sched_dev),
-                   self.update_cov_with_F, self.colliders.n_slots, native.ptr(self.grid_ws),
-                   frozenset(self._mat_ids) <= native.VISCOUS_IDS)
+                   self.update_cov_with_F, n_slots, native.ptr(self.grid_ws), native.ptr(self.wrench),
+                   frozenset(self._mat_ids) <= native.VISCOUS_IDS, self.launch_bits)
             if self._graph is None or self._graph_key != key:
                 self._capture(dt, S, K, fused, lists)
                 self._graph_key = key
@@ -474,15 +489,24 @@ class HipSimulator:
             self._enqueue_tick(dt, S, K, fused, lists, self.stream())
         self._graph = g
 
+    def _schedule(self, dt, S):
+        """Collider rows of one tick: (table [S, n_scenes * K], end time, K colliders per
+        scene, wrench slots)."""
+        if self.scene_colliders is None:
+            tab, t_end = self.colliders.schedule(self.time, dt, S)
+            return tab, t_end, tab.shape[1], self.colliders.n_slots
+        return self.scene_colliders.schedule(self.time, dt, S)
+
     def _pass(self, mode, dt, s):
+        mode |= self.launch_bits
         if self._mat_ids <= native.VISCOUS_IDS:
             mode |= native.PASS_VISCOUS_ONLY
         native.check(self.lib.mpm_particle_pass(self.pP, self.pG, native.ptr(self.model_dev), dt, mode, s),
                      "particle_pass")
 
     def _grid(self, dt, k_row, K, lists, s):
-        sched = native.ptr(self.sched_dev) + k_row * K * native.COLLIDER_DTYPE.itemsize
-        slots = self.colliders.n_slots
+        sched = native.ptr(self.sched_dev) + k_row * self.n_scenes * K * native.COLLIDER_DTYPE.itemsize
+        slots = self.n_slots
         native.check(self.lib.mpm_grid_update(
             self.pG, native.ptr(self.model_dev), native.ptr(self.ctl), lists[0], lists[1],
             self.n_blocks, sched if K else None, K, native.ptr(self.wrench), slots,

@@ -105,3 +105,70 @@ class PourScene:
             R = quat_to_mat(q)
             self.center = self.pivot + R @ (self.center - self.pivot)
             self.quat = quat_mul(q, self.quat)
+
+
+def herschel_bulkley_dough() -> Material:
+    """Herschel-Bulkley dough (K = 50 Pa s^n, n = 0.5, rho 1000); the yield stress is set
+    per particle."""
+    return newtonian(eta=0.0, density=1000.0).with_yield(200.0).with_powerlaw(K=50.0, n=0.5)
+
+
+class DoughPressBatch:
+    """Independent dough-press scenes on a 64^3 grid over [0, 1]^3: a seeded
+    Herschel-Bulkley slab per scene (particle count drawn uniformly from
+    [n_min, n_max] at 8 particles per cell, seeded footprint aspect ratio and height),
+    per-particle yield stress log-uniform in [50, 500] Pa, a sticky floor 3 cells up,
+    domain walls, and a kinematic box plate over the slab moving down at a seeded speed
+    in [0.02, 0.1] m/s. dt = 5e-5."""
+
+    def __init__(self, n_scenes: int = 256, n_grid: int = 64, n_min: int = 5000, n_max: int = 60000,
+                 seed: int = 0, dt: float = 5e-5):
+        self.grid = GridConfig(n_grid=n_grid, grid_lim=1.0)
+        self.dt = dt
+        dx = self.grid.dx
+        h = dx / 2.0
+        rng = np.random.default_rng(seed)
+        self.floor = 3 * dx
+        self.pos, self.vol, self.yield_stress = [], [], []
+        self.box_center, self.box_half, self.box_speed = [], [], []
+        side_max = 0.6
+        for s in range(n_scenes):
+            n_target = int(rng.integers(n_min, n_max + 1))
+            aspect = float(np.exp(rng.uniform(np.log(0.5), np.log(2.0))))
+            volume = n_target * h ** 3
+            lz = max(float(rng.uniform(0.05, 0.12)), volume / (side_max * side_max / max(aspect, 1.0 / aspect)))
+            area = volume / lz
+            lx, ly = np.sqrt(area * aspect), np.sqrt(area / aspect)
+            nx, ny, nz = (max(2, int(round(v / h))) for v in (lx, ly, lz))
+            lx, ly, lz = nx * h, ny * h, nz * h
+            lo = (0.5 - 0.5 * lx, 0.5 - 0.5 * ly, self.floor)
+            pos, vol = lattice_box(lo, (lo[0] + lx, lo[1] + ly, lo[2] + lz), h, seed=seed * 100003 + s)
+            self.pos.append(pos)
+            self.vol.append(vol)
+            self.yield_stress.append(np.exp(rng.uniform(np.log(50.0), np.log(500.0), size=len(pos))).astype(np.float32))
+            half = (0.5 * lx + 2 * dx, 0.5 * ly + 2 * dx, 2 * dx)
+            self.box_half.append(half)
+            self.box_center.append((0.5, 0.5, self.floor + lz + 0.5 * dx + half[2]))
+            self.box_speed.append(float(rng.uniform(0.02, 0.1)))
+        self.counts = np.array([len(p) for p in self.pos])
+
+    def build_batch(self, solver):
+        """Load every scene into a BatchSolver and register its colliders."""
+        solver.load_scenes(self.pos, self.vol)
+        solver.set_material(herschel_bulkley_dough())
+        solver.set_yield_stress(self.yield_stress)
+        solver.add_plane((0, 0, self.floor), (0, 0, 1), "sticky")
+        solver.add_domain_walls()
+        self.tool = [solver.add_box(s, self.box_center[s], self.box_half[s], (0.0, 0.0, -self.box_speed[s]))
+                     for s in range(len(self.pos))]
+        return solver
+
+    def build_one(self, solver, s: int):
+        """Scene s alone in a plain Solver (the sequential baseline)."""
+        solver.load_particles(self.pos[s], self.vol[s])
+        solver.set_material(herschel_bulkley_dough())
+        solver._sim.yield_stress.copy_(solver._sim.torch.from_numpy(self.yield_stress[s]).to(solver._sim.device))
+        solver.add_plane((0, 0, self.floor), (0, 0, 1), "sticky")
+        solver.add_domain_walls()
+        self.tool_one = solver.add_box(self.box_center[s], self.box_half[s], (0.0, 0.0, -self.box_speed[s]))
+        return solver

@@ -29,7 +29,8 @@ def test_library_exports_every_declared_symbol():
 
 def test_struct_layouts_and_introspection():
     lib = native.load()
-    assert lib.mpm_abi_version() == 1
+    assert lib.mpm_abi_version() == 2
+    assert lib.mpm_struct_size(1) == ctypes.sizeof(native.Grid)
     assert lib.mpm_struct_size(2) == native.MODEL_DTYPE.itemsize
     assert lib.mpm_struct_size(3) == native.COLLIDER_DTYPE.itemsize
     assert lib.mpm_struct_size(0) == ctypes.sizeof(native.Particles)
@@ -41,5 +42,7 @@ def test_entry_points_reject_null_arguments_without_touching_the_gpu():
     lib = native.load()
     assert lib.mpm_particle_pass(None, None, None, 1e-4, 7, None) == 1
     assert lib.mpm_grid_update(None, None, None, None, None, 0, None, 0, None, 0, None, 1e-4, 1, None) == 1
-    assert lib.mpm_grid_workspace_floats(100, 65) == -1
+    assert lib.mpm_grid_workspace_floats(100, -1) == -1
+    assert lib.mpm_grid_workspace_floats(100, 2) == 25 * 2 * 6   # fixed-order partials
+    assert lib.mpm_grid_workspace_floats(100, 65) == 0           # atomic path, no workspace
     assert lib.mpm_permute(None, None, None, None) == 1
