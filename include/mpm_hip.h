@@ -242,6 +242,41 @@ int32_t mpm_export_cov(const mpm_particles_t *P, const float *init_cov, float *c
  * v_out [n^3][3], one such copy per scene back to back; any output may be NULL. */
 int32_t mpm_grid_to_dense(const mpm_grid_t *G, float *m, float *v_in, float *v_out, void *stream);
 
+/* ---- adjoint (reverse-mode gradient of one substep) ---------------------------------
+ * No counterpart in the reference (its solver has no gradient). One reverse substep k is
+ *   mpm_adjoint_g2p -> mpm_adjoint_grid -> mpm_adjoint_p2g_stress
+ * on the forward data of substep k: P holds x_k, v_k, C_k, F_trial_k (state before the
+ * substep) and F_k, stress_k (after its return map); G holds node_in = (mv, m) after
+ * the scatter and node_out = grid velocity after the colliders. On entry `A` holds the
+ * adjoint of S_{k+1} = (x, v, C, F_trial); on exit the adjoint of S_k, with the
+ * per-particle parameter adjoints (mu, lam, yield_stress) accumulated. Single scene;
+ * material 1 (von Mises + Hencky, no hardening) and the plane / box / walls /
+ * point-cloud colliders have adjoints; other ids set error bits 16 / 32. */
+typedef struct mpm_adjoint {
+    float *x, *v, *C, *F_trial;      /* adjoint of the substep state (3, 3, 9, 9 planes) */
+    float *F, *stress;               /* adjoint of the return-mapped F and of tau (9, 9) */
+    float *mu, *lam, *yield_stress;  /* accumulated per-particle parameter adjoints */
+    float *grid_v;                   /* float4 per node: adjoint of the grid velocity; zero
+                                        before mpm_adjoint_g2p, consumed (zeroed) by
+                                        mpm_adjoint_grid */
+    float *grid_mv;                  /* float4 per node: adjoint of (momentum, mass) */
+} mpm_adjoint_t;
+
+/* G2P adjoint: scatters the grid-velocity adjoint (float atomics), writes the adjoint
+ * of x_k (partial) into A->x and of F_k into A->F. */
+int32_t mpm_adjoint_g2p(const mpm_particles_t *P, const mpm_grid_t *G, const mpm_adjoint_t *A,
+                        const mpm_model_t *model, float dt, void *stream);
+/* Grid adjoint over every node: the colliders (rows of this substep, ncoll <= 16) and
+ * the normalisation transposed; impulse_adj (float[6 * wrench_slots]) is the adjoint of
+ * the accumulated wrench. */
+int32_t mpm_adjoint_grid(const mpm_grid_t *G, const mpm_model_t *model, const mpm_collider_t *colliders,
+                         int32_t ncoll, const float *impulse_adj, int32_t wrench_slots,
+                         const mpm_adjoint_t *A, float dt, void *stream);
+/* P2G adjoint (gather) fused with the return-map adjoint: completes the adjoint of x_k,
+ * writes those of v_k, C_k, F_trial_k and adds the parameter adjoints. */
+int32_t mpm_adjoint_p2g_stress(const mpm_particles_t *P, const mpm_grid_t *G, const mpm_adjoint_t *A,
+                               const mpm_model_t *model, float dt, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

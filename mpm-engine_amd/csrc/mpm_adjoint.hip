@@ -1,0 +1,516 @@
+// Hand-written adjoint of the explicit MLS-MPM/APIC substep (gfx950).
+//
+// The forward substep k maps the particle state S_k = (x, v, C, F_trial) to S_{k+1}:
+//   stress:  (F_k, tau_k) = return map of F_trial_k        (stress_update_particle,
+//                                                           mpm_utils.py:1056-1149)
+//   P2G:     m_i, mv_i += w m (v + C (x_i - x)) - dt V tau grad w   (p2g, :819-916)
+//   grid:    u_i = mv_i / m_i + dt g, then the colliders in order    (:927-941, BCs)
+//   G2P:     v' = sum w u, C' = 4/dx^2 sum w u (x_i - x)^T, L = sum u grad w^T,
+//            x' = x + dt v', F_trial' = (I + dt L) F_k               (g2p, :944-1043)
+// The reverse substep runs the transposes in the opposite order: the G2P adjoint is a
+// scatter into the grid adjoint (global float atomics), the grid adjoint is per node,
+// and the P2G adjoint is a gather, fused with the return-map adjoint. Derivatives through
+// the B-spline weights use the weight Hessian, so gradients with respect to positions
+// are exact for the discrete map (between cell crossings of the stencil base).
+//
+// Return-map adjoint (von Mises with Hencky elasticity, material 1): F_trial = U S V^T;
+// the stress tau = U diag(h) U^T and the corrected F = U diag(g) V^T are isotropic
+// functions of F_trial. Their derivatives are assembled in the SVD frame from the
+// diagonal Jacobians dh/dsigma, dg/dsigma and, per pair (a, b), the divided differences
+// (h_a - h_b)/(s_a - s_b) and (g_a - g_b)/(s_a - s_b) plus the sums over (s_a + s_b).
+// The divided differences are evaluated through log-ratios with expm1 and a series
+// below |log(s_a/s_b)| < 1e-3, so repeated or nearly repeated singular values (F near a
+// rotation, the undeformed state) give the finite limit instead of 1/(s_a^2 - s_b^2).
+#include <hip/hip_runtime.h>
+
+#include "mpm_device.h"
+
+namespace {
+
+constexpr int kThreads = 256;
+constexpr int kMaxAdjColliders = 16;
+
+// error bits of the adjoint (OR-ed into grid.error_flag)
+constexpr int kErrAdjMaterial = 16;  // a material without an adjoint
+constexpr int kErrAdjCollider = 32;  // a collider kind without an adjoint
+
+// dimensionless B-spline data of one stencil node: weight, d/dfx and the Hessian
+struct NodeW {
+    float w;
+    v3 dw;      // gradient of w with respect to fx (x = dx * fx)
+    m33 H;      // Hessian of w with respect to fx
+    v3 q;       // node offset (i - fx), (x_i - x_p) / dx
+};
+
+MPM_DEV float sel3f(int i, const float* a) { return i == 0 ? a[0] : (i == 1 ? a[1] : a[2]); }
+
+MPM_DEV NodeW node_w(const Stencil& st, int i, int j, int k) {
+    const float w0 = sel3f(i, st.w[0]), w1 = sel3f(j, st.w[1]), w2 = sel3f(k, st.w[2]);
+    const float d0 = sel3f(i, st.dw[0]), d1 = sel3f(j, st.dw[1]), d2 = sel3f(k, st.dw[2]);
+    const float dd0 = i == 1 ? -2.0f : 1.0f, dd1 = j == 1 ? -2.0f : 1.0f, dd2 = k == 1 ? -2.0f : 1.0f;
+    NodeW n;
+    n.w = w0 * w1 * w2;
+    n.dw = mk3(d0 * w1 * w2, w0 * d1 * w2, w0 * w1 * d2);
+    n.H.a[0] = dd0 * w1 * w2; n.H.a[4] = w0 * dd1 * w2; n.H.a[8] = w0 * w1 * dd2;
+    n.H.a[1] = n.H.a[3] = d0 * d1 * w2;
+    n.H.a[2] = n.H.a[6] = d0 * w1 * d2;
+    n.H.a[5] = n.H.a[7] = w0 * d1 * d2;
+    n.q = mk3((float)i - st.fx[0], (float)j - st.fx[1], (float)k - st.fx[2]);
+    return n;
+}
+
+MPM_DEV bool node_of(const mpm_grid_t& G, const Stencil& st, int n, int& i, int& j, int& k, size_t& o) {
+    i = n / 9;
+    j = (n / 3) % 3;
+    k = n % 3;
+    const int gi = st.base[0] + i, gj = st.base[1] + j, gk = st.base[2] + k;
+    if ((unsigned)gi >= (unsigned)G.n[0] || (unsigned)gj >= (unsigned)G.n[1] || (unsigned)gk >= (unsigned)G.n[2])
+        return false;
+    o = 4 * (size_t)node_index(G, gi, gj, gk);
+    return true;
+}
+
+MPM_DEV m33 outer(v3 a, v3 b) {
+    m33 r;
+    r.a[0] = a.x * b.x; r.a[1] = a.x * b.y; r.a[2] = a.x * b.z;
+    r.a[3] = a.y * b.x; r.a[4] = a.y * b.y; r.a[5] = a.y * b.z;
+    r.a[6] = a.z * b.x; r.a[7] = a.z * b.y; r.a[8] = a.z * b.z;
+    return r;
+}
+
+// u / expm1(u), its series below |u| < 1e-3
+MPM_DEV float u_over_expm1(float u) { return fabsf(u) < 1e-3f ? 1.0f - 0.5f * u + u * u * (1.0f / 12.0f) : u / expm1f(u); }
+// expm1(k u) / expm1(u)
+MPM_DEV float expm1_ratio(float k, float u) {
+    return fabsf(u) < 1e-3f ? k * (1.0f + 0.5f * (k - 1.0f) * u) : expm1f(k * u) / expm1f(u);
+}
+
+// Adjoint of the von Mises return map + Hencky stress (mat 1, no hardening) at F_trial.
+// In: Fb = adjoint of the corrected F, tb = adjoint of tau. Out: adjoint of F_trial and
+// the per-particle parameter adjoints (added).
+MPM_DEV m33 adj_von_mises(const m33& Ft, float mu, float lam, float ys, const m33& Fb, const m33& tb_in,
+                          float& mub, float& lamb, float& yb) {
+    m33 U, V;
+    v3 s3;
+    svd3(Ft, U, s3, V);
+    const float s[3] = {s3.x, s3.y, s3.z};
+    float e[3];
+    for (int a = 0; a < 3; ++a) e[a] = logf(fmaxf(s[a], 0.01f));
+    const float me = (e[0] + e[1] + e[2]) / 3.0f;
+    float eh[3] = {e[0] - me, e[1] - me, e[2] - me};
+    // forward's yield test, same expression and order
+    const float trl = lam * (e[0] + e[1] + e[2]);
+    const float t0 = 2.0f * mu * e[0] + trl, t1 = 2.0f * mu * e[1] + trl, t2 = 2.0f * mu * e[2] + trl;
+    const float stv = (t0 + t1 + t2) / 3.0f;
+    const float c0 = t0 - stv, c1 = t1 - stv, c2 = t2 - stv;
+    const bool yield = sqrtf(c0 * c0 + c1 * c1 + c2 * c2) > ys;
+    float k = 1.0f, hn = 0.0f;
+    if (yield) {
+        hn = sqrtf(eh[0] * eh[0] + eh[1] * eh[1] + eh[2] * eh[2]) + 1e-6f;
+        const float dg = hn - ys / (2.0f * mu);
+        k = 1.0f - dg / hn;
+    }
+    float en[3], h[3];
+    for (int a = 0; a < 3; ++a) en[a] = me + k * eh[a];
+    const float tre = en[0] + en[1] + en[2];
+    for (int a = 0; a < 3; ++a) h[a] = 2.0f * mu * en[a] + lam * tre;
+
+    // tau = U diag(h) U^T
+    const m33 tb = mscale(madd(tb_in, mT(tb_in)), 0.5f);
+    const m33 T = mmul(mT(U), mmul(tb, U));
+    const float trT = T.a[0] + T.a[4] + T.a[8];
+    float enb[3];
+    for (int a = 0; a < 3; ++a) enb[a] = 2.0f * mu * T.a[4 * a] + lam * trT;
+    mub += 2.0f * (en[0] * T.a[0] + en[1] * T.a[4] + en[2] * T.a[8]);
+    lamb += tre * trT;
+    m33 Al = mzero();
+    const int pa[3] = {0, 0, 1}, pb[3] = {1, 2, 2};
+    for (int q = 0; q < 3; ++q) {
+        const int a = pa[q], b = pb[q];
+        const float u = e[a] - e[b];
+        const float d1 = 2.0f * mu * k * u_over_expm1(u) / s[b];
+        const float ssum = s[a] + s[b];
+        const float d2 = fabsf(ssum) > 1e-12f ? (h[b] - h[a]) / ssum : 0.0f;
+        const float Tab = T.a[3 * a + b] + T.a[3 * b + a];
+        Al.a[3 * a + b] += 0.5f * Tab * (d1 + d2);
+        Al.a[3 * b + a] += 0.5f * Tab * (d1 - d2);
+    }
+    float eb[3];
+    if (yield) {
+        // F = U diag(g) V^T, g = exp(en)
+        const m33 M = mmul(mT(U), mmul(Fb, V));
+        float g[3];
+        for (int a = 0; a < 3; ++a) {
+            g[a] = expf(en[a]);
+            enb[a] += M.a[4 * a] * g[a];
+        }
+        for (int q = 0; q < 3; ++q) {
+            const int a = pa[q], b = pb[q];
+            const float u = e[a] - e[b];
+            const float cA = g[b] / s[b] * expm1_ratio(k, u);
+            const float ssum = s[a] + s[b];
+            const float cB = fabsf(ssum) > 1e-12f ? (g[a] + g[b]) / ssum : 0.0f;
+            const float S = M.a[3 * a + b] + M.a[3 * b + a], D = M.a[3 * a + b] - M.a[3 * b + a];
+            Al.a[3 * a + b] += 0.5f * (cA * S + cB * D);
+            Al.a[3 * b + a] += 0.5f * (cA * S - cB * D);
+        }
+        // en = me 1 + (ys / (2 mu hn)) eh with hn = |eh| + 1e-6
+        const float ehn = hn - 1e-6f;
+        const float kk = ys / (2.0f * mu * hn);
+        const float dotv = eh[0] * enb[0] + eh[1] * enb[1] + eh[2] * enb[2];
+        const float c = ehn > 0.0f ? dotv / (ehn * hn) : 0.0f;
+        float tmp[3];
+        for (int a = 0; a < 3; ++a) tmp[a] = enb[a] - eh[a] * c;
+        const float mt = (tmp[0] + tmp[1] + tmp[2]) / 3.0f;
+        const float mb = (enb[0] + enb[1] + enb[2]) / 3.0f;
+        for (int a = 0; a < 3; ++a) eb[a] = mb + kk * (tmp[a] - mt);
+        yb += dotv / (2.0f * mu * hn);
+        mub += -ys * dotv / (2.0f * mu * mu * hn);
+    } else {
+        for (int a = 0; a < 3; ++a) eb[a] = enb[a];
+    }
+    for (int a = 0; a < 3; ++a) Al.a[4 * a] += s[a] > 0.01f ? eb[a] / s[a] : 0.0f;
+    m33 Ftb = mmul(U, mmul(Al, mT(V)));
+    if (!yield) Ftb = madd(Ftb, Fb);  // F = F_trial passes through
+    return Ftb;
+}
+
+// ---------------------------------------------------------------- G2P adjoint
+__global__ __launch_bounds__(kThreads) void k_adj_g2p(mpm_particles_t P, mpm_grid_t G, mpm_adjoint_t A,
+                                                      const mpm_model_t* __restrict__ md, float dt) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= P.n || P.selection[p] != 0) return;
+    const int S = P.stride;
+    const int mat = P.material[p];
+    const v3 xb1 = ld3(A.x, S, p);
+    if (mat == 7 || mat == 8) {  // no gather for stationary / rigid ids: x, F pass through
+        st9(A.F, S, p, ld9(A.F_trial, S, p));
+        st3(A.v, S, p, mk3(0.f, 0.f, 0.f));
+        st9(A.C, S, p, mzero());
+        return;
+    }
+    const float inv_dx = G.inv_dx;
+    const v3 x = ld3(P.x, S, p);
+    const m33 F = ld9(P.F, S, p);
+    const Stencil st = make_stencil(x, inv_dx);
+    const float4* __restrict__ uo = reinterpret_cast<const float4*>(G.node_out);
+    // forward velocity gradient L = inv_dx sum u dw^T
+    m33 L = mzero();
+#pragma unroll 1
+    for (int n = 0; n < 27; ++n) {
+        int i, j, k;
+        size_t o;
+        if (!node_of(G, st, n, i, j, k, o)) continue;
+        const float4 u4 = uo[o / 4];
+        const NodeW nw = node_w(st, i, j, k);
+        L = madd(L, outer(mk3(u4.x, u4.y, u4.z), nw.dw));
+    }
+    L = mscale(L, inv_dx);
+    const v3 vb1 = ld3(A.v, S, p);
+    const m33 Cb1 = ld9(A.C, S, p);
+    const m33 Ftb1 = ld9(A.F_trial, S, p);
+    const v3 vt = vb1 + xb1 * dt;               // x' = x + dt v'
+    const m33 Lb = mscale(mmul(Ftb1, mT(F)), dt);  // F_trial' = (I + dt L) F
+    m33 IL = mscale(L, dt);
+    IL.a[0] += 1.f; IL.a[4] += 1.f; IL.a[8] += 1.f;
+    const m33 Fb = mmul(mT(IL), Ftb1);
+    v3 xb = xb1;
+    const float c4 = 4.0f * inv_dx;
+    float* __restrict__ gub = A.grid_v;
+#pragma unroll 1
+    for (int n = 0; n < 27; ++n) {
+        int i, j, k;
+        size_t o;
+        if (!node_of(G, st, n, i, j, k, o)) continue;
+        const float4 u4 = uo[o / 4];
+        const v3 u = mk3(u4.x, u4.y, u4.z);
+        const NodeW nw = node_w(st, i, j, k);
+        const v3 Cq = mvmul(Cb1, nw.q);
+        // position adjoint through the weights, offsets and weight gradients
+        xb = xb + nw.dw * (inv_dx * dot(u, vt) + c4 * inv_dx * dot(u, Cq)) - mTvmul(Cb1, u) * (c4 * inv_dx * nw.w) +
+             mvmul(nw.H, mTvmul(Lb, u)) * (inv_dx * inv_dx);
+        const v3 ub = vt * nw.w + Cq * (c4 * nw.w) + mvmul(Lb, nw.dw) * inv_dx;
+        atomicAdd(gub + o + 0, ub.x);
+        atomicAdd(gub + o + 1, ub.y);
+        atomicAdd(gub + o + 2, ub.z);
+    }
+    st3(A.x, S, p, xb);
+    st9(A.F, S, p, Fb);
+}
+
+// --------------------------------------------------------------- grid adjoint
+// One node: recompute the forward chain (normalise + gravity, damping, colliders in
+// order), then transpose it. Plate / box impulses add their adjoint through `impb`.
+MPM_DEV void adj_node(const mpm_grid_t& G, const mpm_model_t* md, const mpm_collider_t* col, int ncoll,
+                      const float* impb, int nslots, int gi, int gj, int gk, float4 in4, v3 ub, float dt, float4& out,
+                      int& err) {
+    const float m = in4.w;
+    v3 us[kMaxAdjColliders + 1];
+    v3 uo = mk3(0.f, 0.f, 0.f);
+    const v3 mv = mk3(in4.x, in4.y, in4.z);
+    if (m > 1e-15f) uo = mk3(mv.x / m + dt * md->gravity[0], mv.y / m + dt * md->gravity[1], mv.z / m + dt * md->gravity[2]);
+    const float damp = md->grid_v_damping_scale;
+    if (damp < 1.0f) uo = uo * damp;
+    bool act[kMaxAdjColliders];
+    for (int q = 0; q < ncoll; ++q) {
+        const mpm_collider_t& c = col[q];
+        us[q] = uo;
+        act[q] = c.active && m > 0.0f && gi >= c.lo[0] && gi < c.hi[0] && gj >= c.lo[1] && gj < c.hi[1] &&
+                 gk >= c.lo[2] && gk < c.hi[2];
+        if (act[q]) {
+            // forward collider rules of the grid update (mpm_kernels.hip apply_collider)
+            const float time = c.time;
+            const bool live = time >= c.start_time && time < c.end_time;
+            if (c.kind == MPM_COLLIDER_BOX) {
+                const v3 vb = mk3(c.velocity[0], c.velocity[1], c.velocity[2]);
+                if (live) {
+                    const float ox = (float)gi * G.dx - c.point[0], oy = (float)gj * G.dx - c.point[1],
+                                oz = (float)gk * G.dx - c.point[2];
+                    if (fabsf(ox) < c.size[0] && fabsf(oy) < c.size[1] && fabsf(oz) < c.size[2]) uo = vb;
+                } else if (c.reset == 1 && time < c.end_time + 15.0f * c.dt) {
+                    uo = mk3(0.f, 0.f, 0.f);
+                }
+            } else if (c.kind == MPM_COLLIDER_PLANE) {
+                if (live) {
+                    const v3 n = mk3(c.normal[0], c.normal[1], c.normal[2]);
+                    const v3 off = mk3((float)gi * G.dx - c.point[0], (float)gj * G.dx - c.point[1], (float)gk * G.dx - c.point[2]);
+                    if (dot(off, n) < 0.0f) {
+                        if (c.surface == 0) uo = mk3(0.f, 0.f, 0.f);
+                        else if (c.surface == 11) err |= kErrAdjCollider;
+                        else {
+                            const float nc = dot(uo, n);
+                            v3 w = (c.surface == 1) ? uo - n * nc : uo - n * fminf(nc, 0.0f);
+                            if (nc < 0.0f && len(w) > 1e-20f) w = normalize(w) * fmaxf(0.0f, len(w) + nc * c.friction);
+                            uo = w;
+                        }
+                    }
+                }
+            } else if (c.kind == MPM_COLLIDER_WALLS) {
+                if (live) {
+                    const int pad = 3;
+                    if (gi < pad && uo.x < 0) uo.x = 0.0f;
+                    if (gi >= G.n[0] - pad && uo.x > 0) uo.x = 0.0f;
+                    if (gj < pad && uo.y < 0) uo.y = 0.0f;
+                    if (gj >= G.n[1] - pad && uo.y > 0) uo.y = 0.0f;
+                    if (gk < pad && uo.z < 0) uo.z = 0.0f;
+                    if (gk >= G.n[2] - pad && uo.z > 0) uo.z = 0.0f;
+                }
+            } else if (c.kind == MPM_COLLIDER_POINTCLOUD) {
+                if (live && c.occupancy[((size_t)gi * G.n[1] + gj) * G.n[2] + gk] == 1) uo = mk3(0.f, 0.f, 0.f);
+            } else {
+                err |= kErrAdjCollider;
+            }
+        }
+    }
+    // reverse sweep
+    float mb = 0.0f;
+    for (int q = ncoll - 1; q >= 0; --q) {
+        if (!act[q]) continue;
+        const mpm_collider_t& c = col[q];
+        const v3 ui = us[q];
+        const float time = c.time;
+        const bool live = time >= c.start_time && time < c.end_time;
+        if (c.kind == MPM_COLLIDER_BOX) {
+            const v3 vb = mk3(c.velocity[0], c.velocity[1], c.velocity[2]);
+            if (live) {
+                const float ox = (float)gi * G.dx - c.point[0], oy = (float)gj * G.dx - c.point[1],
+                            oz = (float)gk * G.dx - c.point[2];
+                if (fabsf(ox) < c.size[0] && fabsf(oy) < c.size[1] && fabsf(oz) < c.size[2]) {
+                    v3 ib = mk3(0.f, 0.f, 0.f);
+                    if (c.slot >= 0 && c.slot < nslots)
+                        ib = mk3(impb[6 * c.slot], impb[6 * c.slot + 1], impb[6 * c.slot + 2]);
+                    mb += dot(ib, ui - vb);  // impulse = m (u_in - v_box)
+                    ub = ib * m;
+                }
+            } else if (c.reset == 1 && time < c.end_time + 15.0f * c.dt) {
+                ub = mk3(0.f, 0.f, 0.f);
+            }
+        } else if (c.kind == MPM_COLLIDER_PLANE) {
+            if (!live) continue;
+            const v3 n = mk3(c.normal[0], c.normal[1], c.normal[2]);
+            const v3 off = mk3((float)gi * G.dx - c.point[0], (float)gj * G.dx - c.point[1], (float)gk * G.dx - c.point[2]);
+            if (!(dot(off, n) < 0.0f)) continue;
+            if (c.surface == 0) {
+                ub = mk3(0.f, 0.f, 0.f);
+            } else {
+                // w = u - n (u.n) (slip) or u - n min(u.n, 0) (separable); for u.n < 0
+                // both are the tangential part, then Coulomb: w_hat max(0, |w| + f u.n)
+                const float nc = dot(ui, n);
+                const v3 Pub = ub - n * dot(n, ub);
+                if (nc >= 0.0f) {
+                    if (c.surface == 1) ub = Pub;
+                } else {
+                    const v3 w = ui - n * nc;
+                    const float wl = len(w);
+                    if (!(wl > 1e-20f)) {
+                        ub = Pub;
+                    } else if (wl + nc * c.friction <= 0.0f) {
+                        ub = mk3(0.f, 0.f, 0.f);
+                    } else {
+                        const float sc = 1.0f + c.friction * nc / wl;
+                        const float a = dot(w, ub);
+                        ub = Pub * sc + n * (c.friction * a / wl) - w * (c.friction * nc * a / (wl * wl * wl));
+                    }
+                }
+            }
+        } else if (c.kind == MPM_COLLIDER_WALLS) {
+            if (!live) continue;
+            const int pad = 3;
+            if ((gi < pad && ui.x < 0) || (gi >= G.n[0] - pad && ui.x > 0)) ub.x = 0.0f;
+            if ((gj < pad && ui.y < 0) || (gj >= G.n[1] - pad && ui.y > 0)) ub.y = 0.0f;
+            if ((gk < pad && ui.z < 0) || (gk >= G.n[2] - pad && ui.z > 0)) ub.z = 0.0f;
+        } else if (c.kind == MPM_COLLIDER_POINTCLOUD) {
+            if (live && c.occupancy[((size_t)gi * G.n[1] + gj) * G.n[2] + gk] == 1) ub = mk3(0.f, 0.f, 0.f);
+        }
+    }
+    if (damp < 1.0f) ub = ub * damp;
+    out = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (m > 1e-15f) {
+        const float inv = 1.0f / m;
+        out.x = ub.x * inv;
+        out.y = ub.y * inv;
+        out.z = ub.z * inv;
+        out.w = mb - dot(ub, mv) * inv * inv;
+    }
+}
+
+__global__ __launch_bounds__(kThreads) void k_adj_grid(mpm_grid_t G, const mpm_model_t* __restrict__ md,
+                                                       const mpm_collider_t* __restrict__ col, int ncoll,
+                                                       const float* __restrict__ impb, int nslots, mpm_adjoint_t A,
+                                                       float dt) {
+    const int nblk = G.nb[0] * G.nb[1] * G.nb[2];
+    const int lane = threadIdx.x & 63;
+    int err = 0;
+    for (int b = blockIdx.x * (kThreads / 64) + (threadIdx.x >> 6); b < nblk; b += gridDim.x * (kThreads / 64)) {
+        int bi, bj, bk;
+        decode_block(G, b, bi, bj, bk);
+        const int gi = bi * 4 + (lane >> 4), gj = bj * 4 + ((lane >> 2) & 3), gk = bk * 4 + (lane & 3);
+        const size_t o = 4 * ((size_t)b * 64 + lane);
+        if (gi >= G.n[0] || gj >= G.n[1] || gk >= G.n[2]) continue;
+        const float4 in4 = *reinterpret_cast<const float4*>(G.node_in + o);
+        float4 ub4 = *reinterpret_cast<const float4*>(A.grid_v + o);
+        float4 out;
+        adj_node(G, md, col, ncoll, impb, nslots, gi, gj, gk, in4, mk3(ub4.x, ub4.y, ub4.z), dt, out, err);
+        *reinterpret_cast<float4*>(A.grid_mv + o) = out;
+        *reinterpret_cast<float4*>(A.grid_v + o) = make_float4(0.f, 0.f, 0.f, 0.f);  // consumed
+    }
+    if (err && G.error_flag) atomicOr(G.error_flag, err);
+}
+
+// ----------------------------------------------------- P2G + return-map adjoint
+__global__ __launch_bounds__(kThreads) void k_adj_p2g_stress(mpm_particles_t P, mpm_grid_t G, mpm_adjoint_t A,
+                                                             const mpm_model_t* __restrict__ md, float dt) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= P.n || P.selection[p] != 0) return;
+    const int S = P.stride;
+    const int mat = P.material[p];
+    const float inv_dx = G.inv_dx, dx = G.dx;
+    const v3 x = ld3(P.x, S, p);
+    const v3 v = ld3(P.v, S, p);
+    m33 Cm = ld9(P.C, S, p);
+    const float r = md->rpic_damping;
+    if (r != 0.0f) {
+        Cm = madd(mscale(Cm, 1.0f - r), mscale(msub(Cm, mT(Cm)), r / 2.0f));
+        if (r < -0.001f) Cm = mzero();
+    }
+    const m33 tau = ld9(P.stress, S, p);
+    const float mp = P.mass[p], V0 = P.vol[p];
+    const m33 fs = mscale(tau, -V0 * dt);
+    const Stencil st = make_stencil(x, inv_dx);
+    const float4* __restrict__ gmv = reinterpret_cast<const float4*>(A.grid_mv);
+    v3 vb = mk3(0.f, 0.f, 0.f), xb = ld3(A.x, S, p), wa = vb;
+    m33 Cb = mzero(), fsb = mzero();
+#pragma unroll 1
+    for (int n = 0; n < 27; ++n) {
+        int i, j, k;
+        size_t o;
+        if (!node_of(G, st, n, i, j, k, o)) continue;
+        const float4 ab = gmv[o / 4];
+        const v3 a = mk3(ab.x, ab.y, ab.z);
+        const NodeW nw = node_w(st, i, j, k);
+        wa = wa + a * nw.w;
+        Cb = madd(Cb, outer(a * nw.w, nw.q));
+        fsb = madd(fsb, outer(a, nw.dw));
+        const v3 vi = v + mvmul(Cm, nw.q * dx);
+        const float s = mp * dot(a, vi) + ab.w * mp;
+        xb = xb + nw.dw * (inv_dx * s) + mvmul(nw.H, mTvmul(fs, a)) * (inv_dx * inv_dx);
+    }
+    vb = wa * mp;
+    xb = xb - mTvmul(Cm, wa) * mp;
+    Cb = mscale(Cb, mp * dx);
+    if (r != 0.0f) Cb = r < -0.001f ? mzero() : msub(mscale(Cb, 1.0f - 0.5f * r), mscale(mT(Cb), 0.5f * r));
+    const m33 tb = mscale(fsb, -V0 * dt * inv_dx);
+    st3(A.x, S, p, xb);
+    st3(A.v, S, p, vb);
+    st9(A.C, S, p, Cb);
+    // return map + stress adjoint
+    const m33 Fb = ld9(A.F, S, p);
+    m33 Ftb;
+    if (mat == 1 && md->hardening[mat] <= 0.5f) {
+        float mub = 0.f, lamb = 0.f, yb = 0.f;
+        Ftb = adj_von_mises(ld9(P.F_trial, S, p), P.mu[p], P.lam[p], P.yield_stress[p], Fb, tb, mub, lamb, yb);
+        A.mu[p] += mub;
+        A.lam[p] += lamb;
+        A.yield_stress[p] += yb;
+    } else {
+        Ftb = mzero();
+        if (G.error_flag) atomicOr(G.error_flag, kErrAdjMaterial);
+    }
+    st9(A.F_trial, S, p, Ftb);
+}
+
+inline unsigned blocks_for(long n) { return (unsigned)((n + kThreads - 1) / kThreads); }
+inline int32_t launch_status() { return hipGetLastError() == hipSuccess ? MPM_OK : MPM_ERR_HIP; }
+
+inline bool adj_ok(const mpm_adjoint_t* A) {
+    return A && A->x && A->v && A->C && A->F_trial && A->F && A->stress && A->mu && A->lam && A->yield_stress &&
+           A->grid_v && A->grid_mv;
+}
+
+inline bool grid_ok1(const mpm_grid_t* G) {
+    return G && G->node_in && G->node_out && G->n_scenes == 1 && G->n[0] > 0 && G->n[1] > 0 && G->n[2] > 0 &&
+           G->nb[0] == (G->n[0] + 3) / 4 && G->nb[1] == (G->n[1] + 3) / 4 && G->nb[2] == (G->n[2] + 3) / 4;
+}
+
+}  // namespace
+
+extern "C" {
+
+int32_t mpm_adjoint_g2p(const mpm_particles_t* P, const mpm_grid_t* G, const mpm_adjoint_t* A,
+                        const mpm_model_t* model, float dt, void* stream) {
+    if (!P || !grid_ok1(G) || !adj_ok(A) || !model || P->n < 0 || P->stride < P->n || !P->x || !P->F ||
+        !P->material || !P->selection)
+        return MPM_ERR_ARG;
+    if (P->n == 0) return MPM_OK;
+    hipLaunchKernelGGL(k_adj_g2p, dim3(blocks_for(P->n)), dim3(kThreads), 0, (hipStream_t)stream, *P, *G, *A, model,
+                       dt);
+    return launch_status();
+}
+
+int32_t mpm_adjoint_grid(const mpm_grid_t* G, const mpm_model_t* model, const mpm_collider_t* colliders,
+                         int32_t ncoll, const float* impulse_adj, int32_t wrench_slots, const mpm_adjoint_t* A,
+                         float dt, void* stream) {
+    if (!grid_ok1(G) || !model || !adj_ok(A) || ncoll < 0 || (ncoll > 0 && !colliders) || wrench_slots < 0 ||
+        (wrench_slots > 0 && !impulse_adj))
+        return MPM_ERR_ARG;
+    if (ncoll > kMaxAdjColliders) return MPM_ERR_SIZE;
+    const long nblk = (long)G->nb[0] * G->nb[1] * G->nb[2];
+    const unsigned nwg = (unsigned)min((nblk + 3) / 4, 2048L);
+    hipLaunchKernelGGL(k_adj_grid, dim3(nwg), dim3(kThreads), 0, (hipStream_t)stream, *G, model, colliders, ncoll,
+                       impulse_adj, wrench_slots, *A, dt);
+    return launch_status();
+}
+
+int32_t mpm_adjoint_p2g_stress(const mpm_particles_t* P, const mpm_grid_t* G, const mpm_adjoint_t* A,
+                               const mpm_model_t* model, float dt, void* stream) {
+    if (!P || !grid_ok1(G) || !adj_ok(A) || !model || P->n < 0 || P->stride < P->n || !P->x || !P->v || !P->C ||
+        !P->F_trial || !P->stress || !P->vol || !P->mass || !P->mu || !P->lam || !P->yield_stress || !P->material ||
+        !P->selection)
+        return MPM_ERR_ARG;
+    if (P->n == 0) return MPM_OK;
+    hipLaunchKernelGGL(k_adj_p2g_stress, dim3(blocks_for(P->n)), dim3(kThreads), 0, (hipStream_t)stream, *P, *G, *A,
+                       model, dt);
+    return launch_status();
+}
+
+}  // extern "C"

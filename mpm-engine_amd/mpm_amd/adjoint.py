@@ -1,0 +1,312 @@
+"""Differentiable rollouts: reverse-mode gradients of a Solver rollout on the GPU.
+
+``Rollout(solver, dt, substeps)`` takes a configured single-scene ``Solver`` (particles
+loaded, material set, colliders added) and exposes the rollout of `substeps` substeps as
+a ``torch.autograd.Function`` of the per-particle Lame parameters, yield stress, initial
+velocity and initial position. It returns the final positions and the accumulated
+collider wrench, so a loss on the final state and on the plate reaction forces
+back-propagates to Young's modulus, Poisson's ratio, the yield stress (through the
+per-particle parameters) and to the initial particle state.
+
+Forward: fused ticks of `checkpoint_every` substeps (the production substep), storing
+the particle state (x, v, C, F_trial) at every tick boundary. Backward, segment by
+segment from the last: restore the checkpoint, recompute the segment substep by
+substep with the split passes while keeping every intermediate particle state and grid,
+then run the hand-written adjoint kernels of libmpm_hip.so (mpm_adjoint_g2p,
+mpm_adjoint_grid, mpm_adjoint_p2g_stress) backwards through the segment. Memory is
+O(checkpoints + one segment), time about one forward for the recompute plus the
+adjoint kernels.
+
+The reference solver has no gradient path; this module is the differentiation form
+built on the same kernels and collider schedule as the forward Solver.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from .core import native
+
+_STATE_PLANES = 24   # x 3, v 3, C 9, F_trial 9
+_SLOT_PLANES = 42    # + F 9, stress 9
+
+
+def lame(E, nu):
+    """compute_mu_lam_from_E_nu (mpm_utils.py:583-588) on torch tensors."""
+    return E / (2.0 * (1.0 + nu)), E * nu / ((1.0 + nu) * (1.0 - 2.0 * nu))
+
+
+class Rollout:
+    """Differentiable rollout of `substeps` substeps of dt on a configured Solver."""
+
+    SUPPORTED_MATERIALS = frozenset((1,))
+    SUPPORTED_COLLIDERS = frozenset((native.COLLIDER_PLANE, native.COLLIDER_BOX, native.COLLIDER_WALLS,
+                                     native.COLLIDER_POINTCLOUD))
+
+    def __init__(self, solver, dt: float, substeps: int, checkpoint_every: int = 20):
+        sim = solver._sim
+        if sim is None:
+            raise ValueError("load particles and set a material before building a Rollout")
+        if sim.n_scenes != 1:
+            raise ValueError("a Rollout differentiates one scene")
+        if not sim._mat_ids <= self.SUPPORTED_MATERIALS:
+            raise NotImplementedError(f"adjoint kernels cover material ids {sorted(self.SUPPORTED_MATERIALS)}; "
+                                      f"the solver has {sorted(sim._mat_ids)}")
+        kinds = {c.kind for c in sim.colliders.items}
+        if not kinds <= self.SUPPORTED_COLLIDERS:
+            raise NotImplementedError("adjoint kernels cover plane, box, wall and point-cloud colliders")
+        if sim.model[0]["periodic_x"]:
+            raise NotImplementedError("periodic domains have no adjoint")
+        if len(sim.colliders.items) > 16:
+            raise ValueError("at most 16 colliders in a differentiable rollout")
+        self.solver = solver
+        self.sim = sim
+        self.torch = sim.torch
+        self.dt = float(dt)
+        self.substeps = int(substeps)
+        self.every = int(checkpoint_every)
+        torch = self.torch
+        dev = sim.device
+        n = sim.n_particles
+        self.n = n
+        # initial state the rollout starts from (positions may be overridden per call)
+        self.x0 = sim.x.clone()
+        self.C0 = sim.C.clone()
+        self.Ft0 = sim.F_trial.clone()
+        self.F0 = sim.F.clone()
+        # collider rows of every substep, integrated once from the current poses
+        saved = [(list(c.point), np.array(c.rot), list(c.quat)) for c in sim.colliders.items]
+        tab, _ = sim.colliders.schedule(sim.time, self.dt, self.substeps)
+        for c, (p, r, q) in zip(sim.colliders.items, saved):
+            c.point, c.rot, c.quat = p, r, q
+        self.K = tab.shape[1]
+        self.n_slots = sim.colliders.n_slots
+        self.sched = torch.from_numpy(tab.view(np.uint8).reshape(-1).copy()).to(dev)
+        self.row_bytes = self.K * native.COLLIDER_DTYPE.itemsize
+        # whole-grid launch box (single scene)
+        nb = sim.n_blocks
+        self.ctl = torch.zeros(native.TICK_CTL_DTYPE.itemsize // 4, dtype=torch.int32, device=dev)
+        ctl = np.zeros(1, dtype=native.TICK_CTL_DTYPE)
+        ctl["box_lo"], ctl["box_hi"] = (0, 0, 0), (sim.n_grid,) * 3
+        self.ctl.copy_(torch.from_numpy(ctl.view(np.int32).reshape(-1).copy()))
+        self.blocks = torch.arange(nb, dtype=torch.int32, device=dev)
+        self.nblocks = torch.tensor([nb], dtype=torch.int32, device=dev)
+        self.ws = torch.zeros(max(int(sim.lib.mpm_grid_workspace_floats(nb, self.n_slots)), 1),
+                              dtype=torch.float32, device=dev)
+        self.wrench = torch.zeros(max(6 * self.n_slots, 6), dtype=torch.float32, device=dev)
+        nseg = (self.substeps + self.every - 1) // self.every
+        self.ckpt = torch.empty(nseg, _STATE_PLANES, n, dtype=torch.float32, device=dev)
+        # one segment of recomputed substeps: particle slots and grids
+        L = min(self.every, self.substeps)
+        self.slots = torch.zeros(L + 1, _SLOT_PLANES, n, dtype=torch.float32, device=dev)
+        node_floats = nb * 64 * 4
+        self.g_in = torch.zeros(L, node_floats, dtype=torch.float32, device=dev)
+        self.g_out = torch.zeros(L, node_floats, dtype=torch.float32, device=dev)
+        # adjoint buffers
+        z = lambda *s: torch.zeros(*s, dtype=torch.float32, device=dev)  # noqa: E731
+        self.adj = {"x": z(3, n), "v": z(3, n), "C": z(9, n), "F_trial": z(9, n), "F": z(9, n),
+                    "stress": z(9, n), "mu": z(n), "lam": z(n), "yield_stress": z(n),
+                    "grid_v": z(node_floats), "grid_mv": z(node_floats)}
+        A = native.Adjoint()
+        for k, t in self.adj.items():
+            setattr(A, k, native.ptr(t))
+        self.A = A
+        self.timing = {}
+
+    # ------------------------------------------------------------------ structs
+    def _grid_struct(self, node_in, node_out):
+        G = native.Grid()
+        C.memmove(C.byref(G), C.byref(self.sim.G), C.sizeof(G))
+        G.node_in, G.node_out = native.ptr(node_in), native.ptr(node_out)
+        return G
+
+    def _particles(self, state_slot, post_slot):
+        """Particle struct: x, v, C, F_trial from state_slot's planes, F and stress from
+        post_slot's (shared per-particle arrays from the simulator)."""
+        P = native.Particles()
+        C.memmove(C.byref(P), C.byref(self.sim.P), C.sizeof(P))
+        n = self.n
+        base = native.ptr(state_slot)
+        post = native.ptr(post_slot)
+        P.x, P.v, P.C, P.F_trial = base, base + 3 * 4 * n, base + 6 * 4 * n, base + 15 * 4 * n
+        P.F, P.stress = post + 24 * 4 * n, post + 33 * 4 * n
+        P.L = native.ptr(self.sim.L)
+        P.cov = 0
+        return P
+
+    # ------------------------------------------------------------------ forward
+    def _set_state(self, mu, lam, ys, v0, x0):
+        sim = self.sim
+        sim.mu.copy_(mu)
+        sim.lam.copy_(lam)
+        sim.yield_stress.copy_(ys)
+        sim.x.copy_(x0.t())
+        sim.v.copy_(v0.t())
+        sim.C.copy_(self.C0)
+        sim.F_trial.copy_(self.Ft0)
+        sim.F.copy_(self.F0)
+        sim.stress.zero_()
+        sim._upload_model()
+
+    def _grid_update(self, G, row, dt, stream, wrench, slots, flags):
+        sim = self.sim
+        sched = native.ptr(self.sched) + row * self.row_bytes
+        native.check(sim.lib.mpm_grid_update(
+            C.byref(G), native.ptr(sim.model_dev), native.ptr(self.ctl), native.ptr(self.blocks),
+            native.ptr(self.nblocks), sim.n_blocks, sched if self.K else None, self.K,
+            native.ptr(wrench) if slots else None, slots, native.ptr(self.ws) if slots else None,
+            dt, flags, stream), "grid_update")
+
+    def forward(self, mu, lam, ys, v0, x0):
+        """Fused forward rollout from (x0, v0); returns (x_T (n, 3), wrench (slots, 6))."""
+        sim = self.sim
+        torch = self.torch
+        self._set_state(mu, lam, ys, v0, x0)
+        self.wrench.zero_()
+        sim.error_flag.zero_()
+        s = sim.stream()
+        lib = sim.lib
+        dt = self.dt
+        G2P, ST, P2G = native.PASS_G2P, native.PASS_STRESS, native.PASS_P2G
+        pG = sim.pG
+        zl = (C.c_int32 * 3)(0, 0, 0)
+        zh = (C.c_int32 * 3)(*([sim.n_grid] * 3))
+        for seg, k0 in enumerate(range(0, self.substeps, self.every)):
+            S = min(self.every, self.substeps - k0)
+            ck = self.ckpt[seg]
+            ck[0:3].copy_(sim.x)
+            ck[3:6].copy_(sim.v)
+            ck[6:15].copy_(sim.C)
+            ck[15:24].copy_(sim.F_trial)
+            native.check(lib.mpm_grid_zero_box(pG, zl, zh, 3, s), "zero_box")
+            G = sim.G
+            sim._pass(ST | P2G, dt, s)
+            self._grid_update(G, k0, dt, s, self.wrench, self.n_slots, native.GRID_ZERO_IN)
+            for k in range(1, S):
+                mode = G2P | ST | P2G | native.PASS_LEAN
+                if k == S - 1:
+                    mode |= native.PASS_KEEP_STRESS
+                sim._pass(mode, dt, s)
+                self._grid_update(G, k0 + k, dt, s, self.wrench, self.n_slots, native.GRID_ZERO_IN)
+            sim._pass(G2P, dt, s)
+        flag = int(sim.error_flag.item())
+        if flag & (native.ERR_STENCIL | native.ERR_NONFINITE):
+            raise RuntimeError(f"forward rollout left the grid or produced non-finite state (flag {flag})")
+        wrench = self.wrench[:6 * self.n_slots].clone().view(-1, 6)
+        return sim.x.t().clone(), wrench
+
+    # ------------------------------------------------------------------ backward
+    def _recompute(self, seg, k0, S):
+        """Split-pass recompute of substeps k0 .. k0+S-1 keeping every state and grid."""
+        sim = self.sim
+        lib = sim.lib
+        s = sim.stream()
+        dt = self.dt
+        self.slots[0, :_STATE_PLANES].copy_(self.ckpt[seg])
+        self.g_in[:S].zero_()
+        self.g_out[:S].zero_()
+        md = native.ptr(sim.model_dev)
+        for k in range(S):
+            self.slots[k + 1, :_STATE_PLANES].copy_(self.slots[k, :_STATE_PLANES])
+            P = self._particles(self.slots[k + 1], self.slots[k + 1])
+            G = self._grid_struct(self.g_in[k], self.g_out[k])
+            native.check(lib.mpm_particle_pass(C.byref(P), C.byref(G), md, dt,
+                                               native.PASS_STRESS | native.PASS_P2G, s), "recompute p2g")
+            self._grid_update(G, k0 + k, dt, s, None, 0, 0)
+            native.check(lib.mpm_particle_pass(C.byref(P), C.byref(G), md, dt, native.PASS_G2P, s),
+                         "recompute g2p")
+
+    def backward(self, x_bar, wrench_bar):
+        """Adjoint of the last forward: returns (mu_bar, lam_bar, ys_bar, v0_bar, x0_bar)."""
+        sim = self.sim
+        torch = self.torch
+        lib = sim.lib
+        s = sim.stream()
+        dt = self.dt
+        a = self.adj
+        for t in a.values():
+            t.zero_()
+        a["x"].copy_(x_bar.t())
+        impb = torch.zeros(max(6 * self.n_slots, 6), dtype=torch.float32, device=sim.device)
+        if wrench_bar is not None and self.n_slots:
+            impb[:6 * self.n_slots].copy_(wrench_bar.reshape(-1))
+        md = native.ptr(sim.model_dev)
+        pA = C.byref(self.A)
+        segs = list(enumerate(range(0, self.substeps, self.every)))
+        for seg, k0 in reversed(segs):
+            S = min(self.every, self.substeps - k0)
+            self._recompute(seg, k0, S)
+            for k in reversed(range(S)):
+                P = self._particles(self.slots[k], self.slots[k + 1])
+                G = self._grid_struct(self.g_in[k], self.g_out[k])
+                native.check(lib.mpm_adjoint_g2p(C.byref(P), C.byref(G), pA, md, dt, s), "adjoint g2p")
+                sched = native.ptr(self.sched) + (k0 + k) * self.row_bytes
+                native.check(lib.mpm_adjoint_grid(C.byref(G), md, sched if self.K else None, self.K,
+                                                  native.ptr(impb), self.n_slots, pA, dt, s), "adjoint grid")
+                native.check(lib.mpm_adjoint_p2g_stress(C.byref(P), C.byref(G), pA, md, dt, s),
+                             "adjoint p2g+stress")
+        flag = int(sim.error_flag.item())
+        if flag & (native.ERR_ADJ_MATERIAL | native.ERR_ADJ_COLLIDER):
+            raise NotImplementedError(f"a material or collider without an adjoint was met (flag {flag})")
+        return (a["mu"].clone(), a["lam"].clone(), a["yield_stress"].clone(), a["v"].t().clone(),
+                a["x"].t().clone())
+
+    # ------------------------------------------------------------------ autograd
+    def __call__(self, mu, lam, yield_stress, v0, x0=None):
+        """(x_T, wrench) as a differentiable function of per-particle mu, lam, yield
+        stress (n,), initial velocity (n, 3) and initial position (n, 3)."""
+        if x0 is None:
+            x0 = self.x0.t()
+        return _RolloutFn.apply(self, mu, lam, yield_stress, v0, x0)
+
+    def run(self, E, nu, yield_stress, v0, x0=None):
+        """Rollout as a function of the material's E, nu (scalars, broadcast to every
+        particle through compute_mu_lam_from_E_nu) and yield stress (scalar or (n,))."""
+        torch = self.torch
+        mu, lam = lame(E, nu)
+        ones = torch.ones(self.n, dtype=torch.float32, device=self.sim.device)
+        ys = yield_stress * ones if yield_stress.dim() == 0 else yield_stress
+        return self(mu * ones, lam * ones, ys, v0, x0)
+
+
+def _as_f32(t, torch):
+    return t.detach().to(torch.float32).contiguous()
+
+
+def _make_fn():
+    import torch
+
+    class RolloutFn(torch.autograd.Function):
+        @staticmethod
+        def forward(ctx, ro, mu, lam, ys, v0, x0):
+            ctx.ro = ro
+            ctx.dtypes = (mu.dtype, lam.dtype, ys.dtype, v0.dtype, x0.dtype)
+            xT, w = ro.forward(_as_f32(mu, torch), _as_f32(lam, torch), _as_f32(ys, torch),
+                               _as_f32(v0, torch), _as_f32(x0, torch))
+            return xT, w
+
+        @staticmethod
+        def backward(ctx, xT_bar, w_bar):
+            ro = ctx.ro
+            if xT_bar is None:
+                xT_bar = torch.zeros(ro.n, 3, dtype=torch.float32, device=ro.sim.device)
+            mub, lamb, yb, vb, xb = ro.backward(_as_f32(xT_bar, torch),
+                                                None if w_bar is None else _as_f32(w_bar, torch))
+            dts = ctx.dtypes
+            return (None, mub.to(dts[0]), lamb.to(dts[1]), yb.to(dts[2]), vb.to(dts[3]), xb.to(dts[4]))
+
+    return RolloutFn
+
+
+class _Lazy:
+    fn = None
+
+    @classmethod
+    def apply(cls, *args):
+        if cls.fn is None:
+            cls.fn = _make_fn()
+        return cls.fn.apply(*args)
+
+
+_RolloutFn = _Lazy

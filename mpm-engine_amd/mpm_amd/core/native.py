@@ -22,6 +22,7 @@ VISCOUS_IDS = frozenset((6, 7, 8, 10, 12))
 GRID_ZERO_IN = 1
 MAX_FIXED_ORDER_SLOTS = 64  # wrench slots reduced in fixed order; more use float atomics
 ERR_STENCIL, ERR_NONFINITE, ERR_SOLID_IN_VISCOUS, ERR_SCENE = 1, 2, 4, 8
+ERR_ADJ_MATERIAL, ERR_ADJ_COLLIDER = 16, 32
 COLLIDER_PLANE, COLLIDER_BOX, COLLIDER_CUP, COLLIDER_WALLS, COLLIDER_SDF, COLLIDER_POINTCLOUD = \
     1, 2, 3, 4, 5, 6
 
@@ -34,6 +35,11 @@ class Particles(C.Structure):
                 ("stress", _FP), ("L", _FP), ("vol", _FP), ("mass", _FP), ("mu", _FP),
                 ("lam", _FP), ("yield_stress", _FP), ("Jp", _FP), ("material", _FP),
                 ("selection", _FP), ("cov", _FP), ("scene", _FP)]
+
+
+class Adjoint(C.Structure):
+    _fields_ = [(nm, _FP) for nm in ("x", "v", "C", "F_trial", "F", "stress", "mu", "lam",
+                                     "yield_stress", "grid_v", "grid_mv")]
 
 
 class Grid(C.Structure):
@@ -94,6 +100,9 @@ SIGNATURES = {
     "mpm_export_R": (_I32, [_P, _P, _P]),
     "mpm_export_cov": (_I32, [_P, _P, _P, _P]),
     "mpm_grid_to_dense": (_I32, [_P, _P, _P, _P, _P]),
+    "mpm_adjoint_g2p": (_I32, [_P, _P, _P, _P, _F, _P]),
+    "mpm_adjoint_grid": (_I32, [_P, _P, _P, _I32, _P, _I32, _P, _F, _P]),
+    "mpm_adjoint_p2g_stress": (_I32, [_P, _P, _P, _P, _F, _P]),
 }
 
 _LIB = None

@@ -172,3 +172,36 @@ class DoughPressBatch:
         solver.add_domain_walls()
         self.tool_one = solver.add_box(self.box_center[s], self.box_half[s], (0.0, 0.0, -self.box_speed[s]))
         return solver
+
+
+class SqueezeScene:
+    """Squeeze identification: a seeded jittered 0.2^3 block of von Mises material
+    (E = 2e5 Pa, nu = 0.3, yield 1e3 Pa, rho 1000) resting on a sticky floor 3 cells up,
+    between two box plates that close along x at `plate_speed` each. `per_side`^3
+    particles (37^3 = 50,653 at about 24 per cell on 64^3); dt = 1e-4."""
+
+    def __init__(self, n_grid: int = 64, per_side: int = 37, side: float = 0.2, dt: float = 1e-4,
+                 plate_speed: float = 0.05, seed: int = 0):
+        from .materials import vonmises
+
+        self.grid = GridConfig(n_grid=n_grid, grid_lim=1.0)
+        dx = self.grid.dx
+        self.dt = dt
+        h = side / per_side
+        self.floor = 3 * dx
+        lo = (0.5 - 0.5 * side, 0.5 - 0.5 * side, self.floor + 0.5 * dx)
+        self.pos, self.vol = lattice_box(lo, (lo[0] + side, lo[1] + side, lo[2] + side), h, seed)
+        self.material = vonmises(E=2.0e5, nu=0.3, yield_stress=1.0e3, density=1000.0)
+        t = 2 * dx
+        self.plate_half = (t, 0.5 * side + 2 * dx, 0.5 * side + 2 * dx)
+        zc = lo[2] + 0.5 * side
+        self.plate_centers = ((lo[0] - t, 0.5, zc), (lo[0] + side + t, 0.5, zc))
+        self.plate_velocity = ((plate_speed, 0.0, 0.0), (-plate_speed, 0.0, 0.0))
+
+    def build(self, solver):
+        solver.load_particles(self.pos, self.vol)
+        solver.set_material(self.material)
+        solver.add_plane((0, 0, self.floor), (0, 0, 1), "sticky")
+        self.plates = [solver.add_box(c, self.plate_half, v)
+                       for c, v in zip(self.plate_centers, self.plate_velocity)]
+        return solver
