@@ -48,6 +48,7 @@ def parse():
     ap.add_argument("--batch-ticks", type=int, default=3, help="timed 100-substep graph chunks")
     ap.add_argument("--seq-scenes", type=int, default=0,
                     help="scenes timed one by one for the sequential baseline (0 = all)")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint leg")
     return ap.parse_args()
 
 
@@ -131,6 +132,70 @@ def batch_leg(args):
             "autotune": tune["chosen"], "autotune_shapes_s": tune["shapes"]}
 
 
+def adjoint_leg(args):
+    """Squeeze identification (64^3, 37^3 von Mises particles, two closing plates, 400
+    substeps of 1e-4, checkpoints every 20): gradient of |x_T - x*|^2 + |plate impulses -
+    target|^2 (target: yield x 1.1) with respect to E, nu, yield and v0, and the time of
+    the backward pass against the plain forward rollout on the same GPU."""
+    import numpy as np
+    import torch
+
+    from mpm_amd import Solver
+    from mpm_amd.adjoint import Rollout
+    from mpm_amd.scenes import SqueezeScene
+
+    sc = SqueezeScene()
+    S, every = 400, 20
+    dev = torch.device("cuda:0")
+    # plain forward: the production Solver, 20 ticks of 20 substeps (graph replays)
+    plain = sc.build(Solver(grid=sc.grid, device="cuda:0"))
+    plain.step(sc.dt, every)
+    plain.step(sc.dt, every)
+    plain = sc.build(Solver(grid=sc.grid, device="cuda:0"))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(S // every):
+        plain.step(sc.dt, every)
+    torch.cuda.synchronize()
+    t_plain = time.perf_counter() - t0
+    del plain
+    s = sc.build(Solver(grid=sc.grid, device="cuda:0"))
+    n = s.n_particles
+    ro = Rollout(s, sc.dt, S, checkpoint_every=every)
+    f64 = lambda v, g=False: torch.tensor(v, dtype=torch.float64, device=dev, requires_grad=g)  # noqa: E731
+    E0, nu0, y0 = 2.0e5, 0.3, 1.0e3
+    with torch.no_grad():
+        xt, wt = ro.run(f64(E0), f64(nu0), f64(1.1 * y0), torch.zeros(n, 3, device=dev))
+    x_star, w_star = xt.double(), wt[:, :3].double()
+    times = []
+    for rep in range(2):  # the first pass warms allocations up
+        E, nu, ys = f64(E0, True), f64(nu0, True), f64(y0, True)
+        v0 = torch.zeros(n, 3, device=dev, requires_grad=True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        xT, w = ro.run(E, nu, ys, v0)
+        loss = ((xT.double() - x_star) ** 2).sum() + ((w[:, :3].double() - w_star) ** 2).sum()
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        loss.backward()
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        times.append((t1 - t0, t2 - t1))
+    t_fwd, t_bwd = times[-1]
+    if not all(np.isfinite(t.grad.item()) for t in (E, nu, ys)):
+        raise RuntimeError("non-finite adjoint gradient")
+    return {"metric": "adjoint/forward time", "value": round(t_bwd / t_plain, 3), "unit": "ratio",
+            "higher_is_better": False,
+            "backward_s": round(t_bwd, 4), "checkpointed_forward_s": round(t_fwd, 4),
+            "plain_forward_s": round(t_plain, 4),
+            "forward_plus_backward_over_plain": round((t_fwd + t_bwd) / t_plain, 3),
+            "loss": float(loss), "grad": {"E": E.grad.item(), "nu": nu.grad.item(), "yield": ys.grad.item(),
+                                          "v0_norm": float(v0.grad.norm())},
+            "config": {"workload": "squeeze identification: 64^3, von Mises E=2e5 nu=0.3 yield=1e3, "
+                                   "two box plates closing at 0.05 m/s, target rollout at yield x 1.1",
+                       "particles": n, "substeps": S, "dt": sc.dt, "checkpoint_every": every}}
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -201,6 +266,12 @@ def main():
                 "avg_launch_us": round(avg_s * 1e6, 2), "launches": len(ms),
                 "algorithmic_bytes_per_launch": int(algo), "active_nodes": active}
     cpu = None if args.no_cpu else cpu_baseline(args.cpu_substeps)
+    adjoint = None
+    if not args.no_adjoint:
+        try:
+            adjoint = adjoint_leg(args)
+        except Exception as e:  # reported in the line; the headline value stands
+            adjoint = {"error": f"{type(e).__name__}: {e}"}
     batch = None
     if not args.no_batch:
         del s, sim
@@ -218,7 +289,7 @@ def main():
                                   "source tilting 1 rad/s", "particles": n,
                       "substeps_per_step": sc.substeps, "dt": sc.dt, "grid": 192,
                       "graph": bool(args.graph), "sort_interval": args.sort_interval},
-           "roofline": roof, "cpu_baseline": cpu, "batched": batch}
+           "roofline": roof, "cpu_baseline": cpu, "batched": batch, "adjoint": adjoint}
     print(json.dumps(out))
 
 

@@ -246,7 +246,8 @@ int32_t mpm_grid_to_dense(const mpm_grid_t *G, float *m, float *v_in, float *v_o
  * No counterpart in the reference (its solver has no gradient). One reverse substep k is
  *   mpm_adjoint_g2p -> mpm_adjoint_grid -> mpm_adjoint_p2g_stress
  * on the forward data of substep k: P holds x_k, v_k, C_k, F_trial_k (state before the
- * substep) and F_k, stress_k (after its return map); G holds node_in = (mv, m) after
+ * substep) and F_k, stress_k (after its return map) and L_k (from the G2P that ended the
+ * substep); G holds node_in = (mv, m) after
  * the scatter and node_out = grid velocity after the colliders. On entry `A` holds the
  * adjoint of S_{k+1} = (x, v, C, F_trial); on exit the adjoint of S_k, with the
  * per-particle parameter adjoints (mu, lam, yield_stress) accumulated. Single scene;

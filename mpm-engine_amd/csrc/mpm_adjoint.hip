@@ -24,6 +24,7 @@
 #include <hip/hip_runtime.h>
 
 #include "mpm_device.h"
+#include "mpm_tile.h"
 
 namespace {
 
@@ -176,66 +177,109 @@ MPM_DEV m33 adj_von_mises(const m33& Ft, float mu, float lam, float ys, const m3
 }
 
 // ---------------------------------------------------------------- G2P adjoint
-__global__ __launch_bounds__(kThreads) void k_adj_g2p(mpm_particles_t P, mpm_grid_t G, mpm_adjoint_t A,
-                                                      const mpm_model_t* __restrict__ md, float dt) {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= P.n || P.selection[p] != 0) return;
+// One workgroup of kAT particles stages the grid velocity over its stencil box in LDS
+// (read once per node instead of 27 times per particle) and accumulates the scattered
+// grid-velocity adjoint in an LDS arena, flushed with one global float atomic per node
+// and component; chunks whose box does not fit (unsorted particles) gather and scatter
+// through global memory. L_k comes from the forward's G2P (P.L).
+constexpr int kAT = 256;
+
+__global__ __launch_bounds__(kAT) void k_adj_g2p(mpm_particles_t P, mpm_grid_t G, mpm_adjoint_t A,
+                                                 const mpm_model_t* __restrict__ md, float dt) {
+    constexpr int kArena = arena_of<kAT>();
+    __shared__ float4 s_u[kArena];
+    __shared__ float s_ub[3][kArena];
+    __shared__ int s_red[kAT / 64][8];
+    (void)md;
+    const int p = blockIdx.x * kAT + threadIdx.x;
     const int S = P.stride;
-    const int mat = P.material[p];
-    const v3 xb1 = ld3(A.x, S, p);
-    if (mat == 7 || mat == 8) {  // no gather for stationary / rigid ids: x, F pass through
+    const bool sel = p < P.n && P.selection[p] == 0;
+    const int mat = sel ? P.material[p] : 0;
+    const bool moving = sel && mat != 7 && mat != 8;
+    if (sel && !moving) {  // no gather for stationary / rigid ids: x, F pass through
         st9(A.F, S, p, ld9(A.F_trial, S, p));
         st3(A.v, S, p, mk3(0.f, 0.f, 0.f));
         st9(A.C, S, p, mzero());
-        return;
     }
     const float inv_dx = G.inv_dx;
-    const v3 x = ld3(P.x, S, p);
-    const m33 F = ld9(P.F, S, p);
+    const v3 x = moving ? ld3(P.x, S, p) : mk3(0.f, 0.f, 0.f);
     const Stencil st = make_stencil(x, inv_dx);
+    int lo[3], hi[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        lo[a] = moving ? st.base[a] : INT_MAX;
+        hi[a] = moving ? st.base[a] + 3 : INT_MIN;
+    }
+    const Tile t = make_tile<kAT>(G, lo, hi, moving ? 0 : INT_MAX, moving ? 0 : INT_MIN, s_red, false);
     const float4* __restrict__ uo = reinterpret_cast<const float4*>(G.node_out);
-    // forward velocity gradient L = inv_dx sum u dw^T
-    m33 L = mzero();
+    tile_for<kAT>(t, [&](int a, int gi, int gj, int gk) {
+        s_u[a] = in_grid(G, gi, gj, gk) ? uo[node_index(G, gi, gj, gk)] : make_float4(0.f, 0.f, 0.f, 0.f);
+        s_ub[0][a] = 0.f;
+        s_ub[1][a] = 0.f;
+        s_ub[2][a] = 0.f;
+    });
+    __syncthreads();
+    if (moving) {
+        const m33 F = ld9(P.F, S, p);
+        const m33 L = ld9(P.L, S, p);
+        const v3 xb1 = ld3(A.x, S, p);
+        const v3 vb1 = ld3(A.v, S, p);
+        const m33 Cb1 = ld9(A.C, S, p);
+        const m33 Ftb1 = ld9(A.F_trial, S, p);
+        const v3 vt = vb1 + xb1 * dt;                   // x' = x + dt v'
+        const m33 Lb = mscale(mmul(Ftb1, mT(F)), dt);  // F_trial' = (I + dt L) F
+        m33 IL = mscale(L, dt);
+        IL.a[0] += 1.f; IL.a[4] += 1.f; IL.a[8] += 1.f;
+        const m33 Fb = mmul(mT(IL), Ftb1);
+        v3 xb = xb1;
+        const float c4 = 4.0f * inv_dx;
+        const int o0 = st.base[0] - t.lo[0], o1 = st.base[1] - t.lo[1], o2 = st.base[2] - t.lo[2];
+        float* __restrict__ gub = A.grid_v;
 #pragma unroll 1
-    for (int n = 0; n < 27; ++n) {
-        int i, j, k;
-        size_t o;
-        if (!node_of(G, st, n, i, j, k, o)) continue;
-        const float4 u4 = uo[o / 4];
-        const NodeW nw = node_w(st, i, j, k);
-        L = madd(L, outer(mk3(u4.x, u4.y, u4.z), nw.dw));
+        for (int n = 0; n < 27; ++n) {
+            const int i = n / 9, j = (n / 3) % 3, k = n % 3;
+            const int gi = st.base[0] + i, gj = st.base[1] + j, gk = st.base[2] + k;
+            const bool ing = in_grid(G, gi, gj, gk);
+            int a = 0;
+            float4 u4;
+            if (t.lds) {
+                a = t.base + ((o0 + i) * t.d1 + (o1 + j)) * t.d2 + (o2 + k);
+                u4 = s_u[a];
+            } else {
+                if (!ing) continue;
+                u4 = uo[node_index(G, gi, gj, gk)];
+            }
+            const v3 u = mk3(u4.x, u4.y, u4.z);
+            const NodeW nw = node_w(st, i, j, k);
+            const v3 Cq = mvmul(Cb1, nw.q);
+            // position adjoint through the weights, offsets and weight gradients
+            xb = xb + nw.dw * (inv_dx * dot(u, vt) + c4 * inv_dx * dot(u, Cq)) -
+                 mTvmul(Cb1, u) * (c4 * inv_dx * nw.w) + mvmul(nw.H, mTvmul(Lb, u)) * (inv_dx * inv_dx);
+            const v3 ub = vt * nw.w + Cq * (c4 * nw.w) + mvmul(Lb, nw.dw) * inv_dx;
+            if (t.lds) {
+                atomicAdd(&s_ub[0][a], ub.x);
+                atomicAdd(&s_ub[1][a], ub.y);
+                atomicAdd(&s_ub[2][a], ub.z);
+            } else {
+                const size_t o = 4 * (size_t)node_index(G, gi, gj, gk);
+                atomicAdd(gub + o + 0, ub.x);
+                atomicAdd(gub + o + 1, ub.y);
+                atomicAdd(gub + o + 2, ub.z);
+            }
+        }
+        st3(A.x, S, p, xb);
+        st9(A.F, S, p, Fb);
     }
-    L = mscale(L, inv_dx);
-    const v3 vb1 = ld3(A.v, S, p);
-    const m33 Cb1 = ld9(A.C, S, p);
-    const m33 Ftb1 = ld9(A.F_trial, S, p);
-    const v3 vt = vb1 + xb1 * dt;               // x' = x + dt v'
-    const m33 Lb = mscale(mmul(Ftb1, mT(F)), dt);  // F_trial' = (I + dt L) F
-    m33 IL = mscale(L, dt);
-    IL.a[0] += 1.f; IL.a[4] += 1.f; IL.a[8] += 1.f;
-    const m33 Fb = mmul(mT(IL), Ftb1);
-    v3 xb = xb1;
-    const float c4 = 4.0f * inv_dx;
+    __syncthreads();
     float* __restrict__ gub = A.grid_v;
-#pragma unroll 1
-    for (int n = 0; n < 27; ++n) {
-        int i, j, k;
-        size_t o;
-        if (!node_of(G, st, n, i, j, k, o)) continue;
-        const float4 u4 = uo[o / 4];
-        const v3 u = mk3(u4.x, u4.y, u4.z);
-        const NodeW nw = node_w(st, i, j, k);
-        const v3 Cq = mvmul(Cb1, nw.q);
-        // position adjoint through the weights, offsets and weight gradients
-        xb = xb + nw.dw * (inv_dx * dot(u, vt) + c4 * inv_dx * dot(u, Cq)) - mTvmul(Cb1, u) * (c4 * inv_dx * nw.w) +
-             mvmul(nw.H, mTvmul(Lb, u)) * (inv_dx * inv_dx);
-        const v3 ub = vt * nw.w + Cq * (c4 * nw.w) + mvmul(Lb, nw.dw) * inv_dx;
-        atomicAdd(gub + o + 0, ub.x);
-        atomicAdd(gub + o + 1, ub.y);
-        atomicAdd(gub + o + 2, ub.z);
-    }
-    st3(A.x, S, p, xb);
-    st9(A.F, S, p, Fb);
+    tile_for<kAT>(t, [&](int a, int gi, int gj, int gk) {
+        const float b0 = s_ub[0][a], b1 = s_ub[1][a], b2 = s_ub[2][a];
+        if ((b0 == 0.f && b1 == 0.f && b2 == 0.f) || !in_grid(G, gi, gj, gk)) return;
+        const size_t o = 4 * (size_t)node_index(G, gi, gj, gk);
+        atomicAdd(gub + o + 0, b0);
+        atomicAdd(gub + o + 1, b1);
+        atomicAdd(gub + o + 2, b2);
+    });
 }
 
 // --------------------------------------------------------------- grid adjoint
@@ -478,12 +522,12 @@ extern "C" {
 
 int32_t mpm_adjoint_g2p(const mpm_particles_t* P, const mpm_grid_t* G, const mpm_adjoint_t* A,
                         const mpm_model_t* model, float dt, void* stream) {
-    if (!P || !grid_ok1(G) || !adj_ok(A) || !model || P->n < 0 || P->stride < P->n || !P->x || !P->F ||
+    if (!P || !grid_ok1(G) || !adj_ok(A) || !model || P->n < 0 || P->stride < P->n || !P->x || !P->F || !P->L ||
         !P->material || !P->selection)
         return MPM_ERR_ARG;
     if (P->n == 0) return MPM_OK;
-    hipLaunchKernelGGL(k_adj_g2p, dim3(blocks_for(P->n)), dim3(kThreads), 0, (hipStream_t)stream, *P, *G, *A, model,
-                       dt);
+    hipLaunchKernelGGL(k_adj_g2p, dim3((unsigned)((P->n + kAT - 1) / kAT)), dim3(kAT), 0, (hipStream_t)stream, *P, *G,
+                       *A, model, dt);
     return launch_status();
 }
 

@@ -1,0 +1,125 @@
+// LDS node tiles shared by the particle kernels (forward pass and its adjoint): the
+// stencil bounding box of a workgroup's (or a wavefront's) particles staged in LDS so
+// that gathers read each node once and scatters accumulate with LDS atomics before one
+// global flush per node.
+#ifndef MPM_TILE_H
+#define MPM_TILE_H
+
+#include <limits.h>
+
+#include "mpm_device.h"
+
+namespace {
+
+// Workgroup size kT (256 or 128, a launch-shape choice) is a template parameter; the LDS
+// arena holds 3 nodes per thread: 768 nodes (12 KB gather + 12 KB scatter) at 256.
+template <int kT>
+constexpr int arena_of() { return 3 * kT; }
+
+// An LDS tile: the node box [lo, lo + (d0, d1, d2)) of one scene (node offset `soff` in
+// the batched grid) staged at arena offset `base`.
+// Tier 1: the stencil box of all particles of the workgroup fits kArena nodes and they
+// belong to one scene (the usual case for block-sorted particles). Tier 2 (`wg` false):
+// each wavefront stages the box of its own 64 particles in its quarter of the arena; a
+// wavefront whose box still does not fit, or that spans two scenes (`lds` false), gathers
+// and scatters through global memory.
+struct Tile {
+    int lo[3];
+    int d1, d2, vol, base;
+    size_t soff;
+    bool wg, lds;
+};
+
+MPM_DEV bool fits(const int lo[3], const int hi[3], int cap, int& d0, int& d1, int& d2) {
+    d0 = d1 = d2 = 0;
+    if (hi[0] <= lo[0] || hi[1] <= lo[1] || hi[2] <= lo[2]) return false;  // empty (sentinels)
+    d0 = hi[0] - lo[0];
+    d1 = hi[1] - lo[1];
+    d2 = hi[2] - lo[2];
+    if (d0 > cap || d1 > cap || d2 > cap) return false;
+    return d0 * d1 * d2 <= cap;
+}
+
+// per-thread stencil box [lo, hi) and scene range [slo, shi] (sentinels INT_MAX / INT_MIN
+// when inactive) -> tile; contains one __syncthreads, so every thread of the workgroup
+// must call it
+template <int kT>
+MPM_DEV Tile make_tile(const mpm_grid_t& G, int lo[3], int hi[3], int slo, int shi, int (*red)[8], bool periodic) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            lo[a] = min(lo[a], __shfl_xor(lo[a], off, 64));
+            hi[a] = max(hi[a], __shfl_xor(hi[a], off, 64));
+        }
+        slo = min(slo, __shfl_xor(slo, off, 64));
+        shi = max(shi, __shfl_xor(shi, off, 64));
+    }
+    const int wave = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        for (int a = 0; a < 3; ++a) {
+            red[wave][a] = lo[a];
+            red[wave][3 + a] = hi[a];
+        }
+        red[wave][6] = slo;
+        red[wave][7] = shi;
+    }
+    __syncthreads();
+    int wlo[3], whi[3];
+    for (int a = 0; a < 3; ++a) {
+        int l = red[0][a], h = red[0][3 + a];
+        for (int w = 1; w < kT / 64; ++w) {
+            l = min(l, red[w][a]);
+            h = max(h, red[w][3 + a]);
+        }
+        wlo[a] = l;
+        whi[a] = h;
+    }
+    int wslo = red[0][6], wshi = red[0][7];
+    for (int w = 1; w < kT / 64; ++w) {
+        wslo = min(wslo, red[w][6]);
+        wshi = max(wshi, red[w][7]);
+    }
+    constexpr int kArena = arena_of<kT>();
+    Tile t;
+    int d0, d1, d2;
+    if (!periodic && wslo == wshi && fits(wlo, whi, kArena, d0, d1, d2)) {
+        t.wg = true;
+        t.lds = true;
+        t.base = 0;
+        t.soff = scene_base(G, wslo);
+        for (int a = 0; a < 3; ++a) t.lo[a] = wlo[a];
+    } else {
+        t.wg = false;
+        t.lds = !periodic && slo == shi && fits(lo, hi, kArena / (kT / 64), d0, d1, d2);
+        t.base = wave * (kArena / (kT / 64));
+        t.soff = t.lds ? scene_base(G, slo) : 0;
+        for (int a = 0; a < 3; ++a) t.lo[a] = lo[a];
+    }
+    t.d1 = d1;
+    t.d2 = d2;
+    t.vol = t.lds ? d0 * d1 * d2 : 0;
+    return t;
+}
+
+// cooperative loop over the tile's nodes: the whole workgroup for a workgroup tile, the
+// owning wavefront for a wavefront tile
+template <int kT, class F>
+MPM_DEV void tile_for(const Tile& t, F f) {
+    if (!t.lds) return;
+    const int start = t.wg ? (int)threadIdx.x : (int)(threadIdx.x & 63);
+    const int step = t.wg ? kT : 64;
+    for (int n = start; n < t.vol; n += step) {
+        const int k = n % t.d2, r = n / t.d2;
+        const int j = r % t.d1, i = r / t.d1;
+        f(t.base + n, t.lo[0] + i, t.lo[1] + j, t.lo[2] + k);
+    }
+}
+
+MPM_DEV bool in_grid(const mpm_grid_t& G, int i, int j, int k) {
+    return (unsigned)i < (unsigned)G.n[0] && (unsigned)j < (unsigned)G.n[1] && (unsigned)k < (unsigned)G.n[2];
+}
+
+}  // namespace
+
+#endif

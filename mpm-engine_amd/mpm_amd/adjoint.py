@@ -29,7 +29,7 @@ import numpy as np
 from .core import native
 
 _STATE_PLANES = 24   # x 3, v 3, C 9, F_trial 9
-_SLOT_PLANES = 42    # + F 9, stress 9
+_SLOT_PLANES = 51    # + F 9, stress 9, L 9
 
 
 def lame(E, nu):
@@ -63,6 +63,13 @@ class Rollout:
         self.solver = solver
         self.sim = sim
         self.torch = sim.torch
+        # Block-sort the particles once (Solver._sort_particles, solver.py:454-467) so the
+        # adjoint scatter, like the forward pass, works on LDS tiles; inputs and outputs
+        # keep the particle order the solver had when the Rollout was built.
+        n0 = sim.n_particles
+        self.perm = sim.torch.arange(n0, device=sim.device)
+        if solver._sort_particles():
+            self.perm = sim._last_perm.clone()
         self.dt = float(dt)
         self.substeps = int(substeps)
         self.every = int(checkpoint_every)
@@ -71,7 +78,7 @@ class Rollout:
         n = sim.n_particles
         self.n = n
         # initial state the rollout starts from (positions may be overridden per call)
-        self.x0 = sim.x.clone()
+        self.x0 = self._to_user(sim.x.t()).contiguous()
         self.C0 = sim.C.clone()
         self.Ft0 = sim.F_trial.clone()
         self.F0 = sim.F.clone()
@@ -112,7 +119,16 @@ class Rollout:
         for k, t in self.adj.items():
             setattr(A, k, native.ptr(t))
         self.A = A
-        self.timing = {}
+        self.impb = z(max(6 * self.n_slots, 6))
+        # collider rows of the segment being reversed (a fixed buffer, so one captured
+        # graph serves every full segment)
+        self.seg_sched = torch.zeros(max(L * self.row_bytes, 1), dtype=torch.uint8, device=dev)
+        # ABI structs of every substep slot, built once
+        self._P_fwd = [self._particles(self.slots[k + 1], self.slots[k + 1]) for k in range(L)]
+        self._P_adj = [self._particles(self.slots[k], self.slots[k + 1]) for k in range(L)]
+        self._G = [self._grid_struct(self.g_in[k], self.g_out[k]) for k in range(L)]
+        self.use_graph = True
+        self._graph = None
 
     # ------------------------------------------------------------------ structs
     def _grid_struct(self, node_in, node_out):
@@ -130,19 +146,28 @@ class Rollout:
         base = native.ptr(state_slot)
         post = native.ptr(post_slot)
         P.x, P.v, P.C, P.F_trial = base, base + 3 * 4 * n, base + 6 * 4 * n, base + 15 * 4 * n
-        P.F, P.stress = post + 24 * 4 * n, post + 33 * 4 * n
-        P.L = native.ptr(self.sim.L)
+        P.F, P.stress, P.L = post + 24 * 4 * n, post + 33 * 4 * n, post + 42 * 4 * n
         P.cov = 0
         return P
+
+    # ------------------------------------------------------------------ ordering
+    def _to_slots(self, t):
+        """User particle order -> the rollout's internal (block-sorted) order."""
+        return t[self.perm]
+
+    def _to_user(self, t):
+        out = self.torch.empty_like(t)
+        out[self.perm] = t
+        return out
 
     # ------------------------------------------------------------------ forward
     def _set_state(self, mu, lam, ys, v0, x0):
         sim = self.sim
-        sim.mu.copy_(mu)
-        sim.lam.copy_(lam)
-        sim.yield_stress.copy_(ys)
-        sim.x.copy_(x0.t())
-        sim.v.copy_(v0.t())
+        sim.mu.copy_(self._to_slots(mu))
+        sim.lam.copy_(self._to_slots(lam))
+        sim.yield_stress.copy_(self._to_slots(ys))
+        sim.x.copy_(self._to_slots(x0).t())
+        sim.v.copy_(self._to_slots(v0).t())
         sim.C.copy_(self.C0)
         sim.F_trial.copy_(self.Ft0)
         sim.F.copy_(self.F0)
@@ -194,70 +219,84 @@ class Rollout:
         if flag & (native.ERR_STENCIL | native.ERR_NONFINITE):
             raise RuntimeError(f"forward rollout left the grid or produced non-finite state (flag {flag})")
         wrench = self.wrench[:6 * self.n_slots].clone().view(-1, 6)
-        return sim.x.t().clone(), wrench
+        return self._to_user(sim.x.t()), wrench
 
     # ------------------------------------------------------------------ backward
-    def _recompute(self, seg, k0, S):
-        """Split-pass recompute of substeps k0 .. k0+S-1 keeping every state and grid."""
+    def _load_segment(self, seg, k0, S):
+        """Checkpoint of segment `seg` into slot 0 and its collider rows into seg_sched."""
+        self.slots[0, :_STATE_PLANES].copy_(self.ckpt[seg])
+        if self.K:
+            nb = S * self.row_bytes
+            self.seg_sched[:nb].copy_(self.sched[k0 * self.row_bytes:k0 * self.row_bytes + nb])
+
+    def _segment_body(self, S):
+        """Recompute S substeps from slot 0 with the split passes, keeping every state and
+        grid, then run the adjoint kernels backwards through them."""
         sim = self.sim
         lib = sim.lib
         s = sim.stream()
         dt = self.dt
-        self.slots[0, :_STATE_PLANES].copy_(self.ckpt[seg])
+        md = native.ptr(sim.model_dev)
         self.g_in[:S].zero_()
         self.g_out[:S].zero_()
-        md = native.ptr(sim.model_dev)
+        sched0 = native.ptr(self.seg_sched)
         for k in range(S):
             self.slots[k + 1, :_STATE_PLANES].copy_(self.slots[k, :_STATE_PLANES])
-            P = self._particles(self.slots[k + 1], self.slots[k + 1])
-            G = self._grid_struct(self.g_in[k], self.g_out[k])
-            native.check(lib.mpm_particle_pass(C.byref(P), C.byref(G), md, dt,
-                                               native.PASS_STRESS | native.PASS_P2G, s), "recompute p2g")
-            self._grid_update(G, k0 + k, dt, s, None, 0, 0)
-            native.check(lib.mpm_particle_pass(C.byref(P), C.byref(G), md, dt, native.PASS_G2P, s),
-                         "recompute g2p")
+            P, G = C.byref(self._P_fwd[k]), C.byref(self._G[k])
+            native.check(lib.mpm_particle_pass(P, G, md, dt, native.PASS_STRESS | native.PASS_P2G, s),
+                         "recompute p2g")
+            native.check(lib.mpm_grid_update(
+                G, md, native.ptr(self.ctl), native.ptr(self.blocks), native.ptr(self.nblocks),
+                sim.n_blocks, sched0 + k * self.row_bytes if self.K else None, self.K, None, 0, None,
+                dt, 0, s), "recompute grid")
+            native.check(lib.mpm_particle_pass(P, G, md, dt, native.PASS_G2P, s), "recompute g2p")
+        pA = C.byref(self.A)
+        impb = native.ptr(self.impb)
+        for k in reversed(range(S)):
+            P, G = C.byref(self._P_adj[k]), C.byref(self._G[k])
+            native.check(lib.mpm_adjoint_g2p(P, G, pA, md, dt, s), "adjoint g2p")
+            native.check(lib.mpm_adjoint_grid(G, md, sched0 + k * self.row_bytes if self.K else None, self.K,
+                                              impb, self.n_slots, pA, dt, s), "adjoint grid")
+            native.check(lib.mpm_adjoint_p2g_stress(P, G, pA, md, dt, s), "adjoint p2g+stress")
 
     def backward(self, x_bar, wrench_bar):
         """Adjoint of the last forward: returns (mu_bar, lam_bar, ys_bar, v0_bar, x0_bar)."""
         sim = self.sim
         torch = self.torch
-        lib = sim.lib
-        s = sim.stream()
-        dt = self.dt
         a = self.adj
         for t in a.values():
             t.zero_()
-        a["x"].copy_(x_bar.t())
-        impb = torch.zeros(max(6 * self.n_slots, 6), dtype=torch.float32, device=sim.device)
+        a["x"].copy_(self._to_slots(x_bar).t())
+        self.impb.zero_()
         if wrench_bar is not None and self.n_slots:
-            impb[:6 * self.n_slots].copy_(wrench_bar.reshape(-1))
-        md = native.ptr(sim.model_dev)
-        pA = C.byref(self.A)
+            self.impb[:6 * self.n_slots].copy_(wrench_bar.reshape(-1))
+        sim._upload_model()
         segs = list(enumerate(range(0, self.substeps, self.every)))
         for seg, k0 in reversed(segs):
             S = min(self.every, self.substeps - k0)
-            self._recompute(seg, k0, S)
-            for k in reversed(range(S)):
-                P = self._particles(self.slots[k], self.slots[k + 1])
-                G = self._grid_struct(self.g_in[k], self.g_out[k])
-                native.check(lib.mpm_adjoint_g2p(C.byref(P), C.byref(G), pA, md, dt, s), "adjoint g2p")
-                sched = native.ptr(self.sched) + (k0 + k) * self.row_bytes
-                native.check(lib.mpm_adjoint_grid(C.byref(G), md, sched if self.K else None, self.K,
-                                                  native.ptr(impb), self.n_slots, pA, dt, s), "adjoint grid")
-                native.check(lib.mpm_adjoint_p2g_stress(C.byref(P), C.byref(G), pA, md, dt, s),
-                             "adjoint p2g+stress")
+            self._load_segment(seg, k0, S)
+            if self.use_graph and S == self.every:
+                if self._graph is None:  # one capture serves every full segment
+                    g = torch.cuda.CUDAGraph()
+                    torch.cuda.synchronize(sim.device)
+                    with torch.cuda.graph(g):
+                        self._segment_body(S)
+                    self._graph = g
+                self._graph.replay()
+            else:
+                self._segment_body(S)
         flag = int(sim.error_flag.item())
         if flag & (native.ERR_ADJ_MATERIAL | native.ERR_ADJ_COLLIDER):
             raise NotImplementedError(f"a material or collider without an adjoint was met (flag {flag})")
-        return (a["mu"].clone(), a["lam"].clone(), a["yield_stress"].clone(), a["v"].t().clone(),
-                a["x"].t().clone())
+        return (self._to_user(a["mu"]), self._to_user(a["lam"]), self._to_user(a["yield_stress"]),
+                self._to_user(a["v"].t()), self._to_user(a["x"].t()))
 
     # ------------------------------------------------------------------ autograd
     def __call__(self, mu, lam, yield_stress, v0, x0=None):
         """(x_T, wrench) as a differentiable function of per-particle mu, lam, yield
         stress (n,), initial velocity (n, 3) and initial position (n, 3)."""
         if x0 is None:
-            x0 = self.x0.t()
+            x0 = self.x0
         return _RolloutFn.apply(self, mu, lam, yield_stress, v0, x0)
 
     def run(self, E, nu, yield_stress, v0, x0=None):
