@@ -1,14 +1,20 @@
 """Throughput of the explicit MLS-MPM/APIC substep on one MI355X.
 
-Workload: the 192^3 Newtonian honey pour (340k particles at 8 per cell, two revolved-SDF
-glasses, the source tilting at 1 rad/s about its rim, 432 substeps per 1/60 s frame).
-A "step" is one frame (one control tick of 432 substeps). Prints one JSON line:
-value = particle-substeps/s over the timed frames (particles resident on the GPU),
-plus the roofline of the dominant kernel (the fused g2p+stress+p2g particle pass),
-timed with HIP events on its stream during the timed frames, and the CPU oracle's
-throughput on a bounded sample of the same workload.
+Headline workload: the 192^3 Newtonian honey pour (340k particles at 8 per cell, two
+revolved-SDF glasses, the source tilting at 1 rad/s about its rim, 432 substeps per
+1/60 s frame). A "step" is one frame (one control tick of 432 substeps). Prints one JSON
+line: value = particle-substeps/s over the timed frames (particles resident on the GPU),
+the roofline of the dominant kernel (the fused g2p+stress+p2g particle pass, timed with
+HIP events on its stream during the timed frames; HBM traffic from the rocprofv3 --pmc
+summary under profiles/), and the CPU oracle's throughput on a bounded sample of the same
+workload. Two more legs ride in the same line:
+  "batched"  256 dough-press scenes (64^3 each, ~8.2M particles) in one launch per phase,
+             100-substep hipGraph chunks, autotuned launch shape and sort interval, against
+             the same scenes solved one after another with the plain Solver;
+  "adjoint"  the squeeze identification (50k von Mises particles, 400 substeps): gradient
+             of a loss on final positions and plate forces, backward time / forward time.
 
-    python bench.py [--gpus 1] [--steps 10] [--warmup 2] [--no-cpu]
+    python bench.py [--gpus 1] [--steps 10] [--warmup 2] [--no-cpu] [--legs pour,adjoint,batch]
 """
 from __future__ import annotations
 
@@ -43,12 +49,17 @@ def parse():
     ap.add_argument("--cpu-substeps", type=int, default=6)
     ap.add_argument("--graph", action="store_true", help="replay ticks as hipGraphs")
     ap.add_argument("--sort-interval", type=int, default=1)
+    ap.add_argument("--sort-key", choices=("block", "cell"), default="cell")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-scenes leg")
     ap.add_argument("--batch-scenes", type=int, default=256)
     ap.add_argument("--batch-ticks", type=int, default=3, help="timed 100-substep graph chunks")
     ap.add_argument("--seq-scenes", type=int, default=0,
                     help="scenes timed one by one for the sequential baseline (0 = all)")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint leg")
+    ap.add_argument("--legs", default="pour,adjoint,batch",
+                    help="comma list of legs to run (pour, adjoint, batch); for profiling one leg")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a device copy of 256 MiB (the byte-counter calibration for --pmc)")
     return ap.parse_args()
 
 
@@ -99,36 +110,39 @@ def batch_leg(args):
     if not np.isfinite(b.x(0)).all():
         raise RuntimeError("non-finite batch state")
     batched = ns * chunk * args.batch_ticks / el
-    # sequential baseline: each scene alone in a plain Solver (same sort policy), one
-    # timed graph-replayed chunk after two warm-up chunks
+    # sequential baseline: each scene alone in a plain Solver, one timed graph-replayed
+    # chunk after two warm-up chunks, under two sort policies (the reference's default,
+    # no sort, and the batch's autotuned one); the speedup is against the faster
     picks = range(ns) if not args.seq_scenes else np.linspace(0, ns - 1, args.seq_scenes).astype(int)
     print(f"[batch] batched {batched:.1f} scene-steps/s; sequential baseline over {len(picks)} scenes",
           file=sys.stderr, flush=True)
-    seq_el, seq_steps, seq_particles = 0.0, 0, 0
-    for k, s_id in enumerate(picks):
-        s = sc.build_one(Solver(grid=sc.grid, device="cuda:0", sort_interval=b.sort_interval), int(s_id))
-        s.step(sc.dt, chunk)
-        s.step(sc.dt, chunk)
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        s.step(sc.dt, chunk)
-        torch.cuda.synchronize()
-        seq_el += time.perf_counter() - t0
-        seq_steps += chunk
-        seq_particles += len(sc.pos[int(s_id)])
-        del s
-        if k % 32 == 31:
-            print(f"[batch] sequential {k + 1}/{len(picks)}", file=sys.stderr, flush=True)
-    seq_rate = seq_steps / seq_el
+    seq = {}
+    for policy, kw in (("no_sort", {}), ("batch_sort", {"sort_interval": b.sort_interval, "sort_key": "cell"})):
+        seq_el, seq_steps = 0.0, 0
+        for k, s_id in enumerate(picks):
+            s = sc.build_one(Solver(grid=sc.grid, device="cuda:0", **kw), int(s_id))
+            s.step(sc.dt, chunk)
+            s.step(sc.dt, chunk)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            s.step(sc.dt, chunk)
+            torch.cuda.synchronize()
+            seq_el += time.perf_counter() - t0
+            seq_steps += chunk
+            del s
+            if k % 64 == 63:
+                print(f"[batch] sequential {policy} {k + 1}/{len(picks)}", file=sys.stderr, flush=True)
+        seq[policy] = seq_steps / seq_el
+    seq_rate = max(seq.values())
     return {"metric": "batched scene-steps/s", "value": round(batched, 1), "unit": "scene-substeps/s",
             "sequential_value": round(seq_rate, 1), "speedup_vs_sequential": round(batched / seq_rate, 2),
+            "sequential_by_sort_policy": {k: round(v, 1) for k, v in seq.items()},
             "particle_substeps_per_s": round(n * chunk * args.batch_ticks / el, 1),
             "ms_per_chunk": round(el / args.batch_ticks * 1e3, 2), "timed_chunks": args.batch_ticks,
             "config": {"workload": f"{ns} dough-press scenes, 64^3 each, Herschel-Bulkley K=50 n=0.5, "
                                    f"yield log-U[50,500] Pa per particle, box plate U[0.02,0.1] m/s",
                        "particles": n, "substeps_per_chunk": chunk, "dt": sc.dt,
-                       "sequential_scenes_timed": len(picks),
-                       "sequential_particles_timed": int(seq_particles)},
+                       "sequential_scenes_timed": len(picks)},
             "autotune": tune["chosen"], "autotune_shapes_s": tune["shapes"]}
 
 
@@ -196,12 +210,37 @@ def adjoint_leg(args):
                        "particles": n, "substeps": S, "dt": sc.dt, "checkpoint_every": every}}
 
 
-def main():
-    args = parse()
-    if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
-                                   "(multi-GPU sharding is a design only)"}))
-        sys.exit(2)
+TRAFFIC_FILE = os.path.join(REPO, "profiles", "pour192_fused_pass_traffic.md")
+
+
+def measured_traffic():
+    """HBM bytes per fused-pass launch measured with rocprofv3 --pmc (FETCH_SIZE +
+    WRITE_SIZE, calibrated on a copy of known size), recorded in the profile summary."""
+    try:
+        with open(TRAFFIC_FILE) as fh:
+            for line in fh:
+                if line.startswith("| traffic per launch (bytes)"):
+                    return int(float(line.split("|")[2]))
+    except OSError:
+        pass
+    return None
+
+
+def copy_calibration():
+    import torch
+
+    n = 64 * 1024 * 1024  # 256 MiB of float32
+    src = torch.ones(n, device="cuda:0")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        torch.mul(src, 1.0, out=dst)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_bytes_read": 4 * n, "bytes_written": 4 * n, "launches": 3}))
+
+
+def pour_leg(args):
+    """The headline: fused ticks of the 192^3 pour, frames timed end to end; the fused
+    particle pass timed with HIP events on its stream."""
     import numpy as np
     import torch
 
@@ -210,9 +249,10 @@ def main():
     from mpm_amd.scenes import PourScene
 
     sc = PourScene()
-    # block sort at the start of every frame keeps each 256-particle chunk's stencil box
-    # inside the LDS arena of the particle pass
-    s = sc.build(Solver(grid=sc.grid, device="cuda:0", sort_interval=args.sort_interval))
+    # a sort at the start of every frame keeps each chunk's stencil box inside the LDS
+    # arena of the particle pass; the cell key lets same-cell lanes sum before the atomics
+    s = sc.build(Solver(grid=sc.grid, device="cuda:0", sort_interval=args.sort_interval,
+                        sort_key=args.sort_key))
     sim = s._sim
     sim.use_graph = bool(args.graph)
     n = s.n_particles
@@ -261,35 +301,56 @@ def main():
         algo = PASS_B_PER_PARTICLE * n + PASS_B_PER_NODE * active
         ach = algo / avg_s / 1e9
         roof = {"bound": "hbm", "achieved": round(ach, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
+                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": measured_traffic(),
                 "kernel": "k_particle_pass (fused g2p+stress+p2g)",
                 "avg_launch_us": round(avg_s * 1e6, 2), "launches": len(ms),
                 "algorithmic_bytes_per_launch": int(algo), "active_nodes": active}
-    cpu = None if args.no_cpu else cpu_baseline(args.cpu_substeps)
-    adjoint = None
-    if not args.no_adjoint:
+    return {"value": round(value, 1), "ms_per_step": round(el / args.steps * 1e3, 3), "particles": n,
+            "substeps_per_step": sc.substeps, "dt": sc.dt, "roofline": roof}
+
+
+def main():
+    args = parse()
+    if args.copy_calibration:
+        copy_calibration()
+        return
+    legs = set(args.legs.split(","))
+    if args.no_batch:
+        legs.discard("batch")
+    if args.no_adjoint:
+        legs.discard("adjoint")
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
+                                   "(multi-GPU sharding is a design only)"}))
+        sys.exit(2)
+    import torch
+
+    pour = pour_leg(args) if "pour" in legs else {}
+    torch.cuda.empty_cache()
+    cpu = None if (args.no_cpu or "pour" not in legs) else cpu_baseline(args.cpu_substeps)
+    adjoint = batch = None
+    if "adjoint" in legs:
         try:
             adjoint = adjoint_leg(args)
         except Exception as e:  # reported in the line; the headline value stands
             adjoint = {"error": f"{type(e).__name__}: {e}"}
-    batch = None
-    if not args.no_batch:
-        del s, sim
         torch.cuda.empty_cache()
+    if "batch" in legs:
         try:
             batch = batch_leg(args)
         except Exception as e:  # reported in the line; the headline value stands
             batch = {"error": f"{type(e).__name__}: {e}"}
-    out = {"metric": "particle-substeps/s at 340k pour", "value": round(value, 1),
+    out = {"metric": "particle-substeps/s at 340k pour", "value": pour.get("value"),
            "unit": "particle-substeps/s", "n_gpus": 1, "steps": args.steps,
-           "warmup": args.warmup, "ms_per_step": round(el / args.steps * 1e3, 3),
+           "warmup": args.warmup, "ms_per_step": pour.get("ms_per_step"),
            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
            "data": "synthetic",
            "config": {"workload": "honey pour 192^3, eta 10 Pa s, 2 revolved-SDF glasses, "
-                                  "source tilting 1 rad/s", "particles": n,
-                      "substeps_per_step": sc.substeps, "dt": sc.dt, "grid": 192,
-                      "graph": bool(args.graph), "sort_interval": args.sort_interval},
-           "roofline": roof, "cpu_baseline": cpu, "batched": batch, "adjoint": adjoint}
+                                  "source tilting 1 rad/s", "particles": pour.get("particles"),
+                      "substeps_per_step": pour.get("substeps_per_step"), "dt": pour.get("dt"),
+                      "grid": 192, "graph": bool(args.graph), "sort_interval": args.sort_interval,
+                      "sort_key": args.sort_key},
+           "roofline": pour.get("roofline"), "cpu_baseline": cpu, "batched": batch, "adjoint": adjoint}
     print(json.dumps(out))
 
 

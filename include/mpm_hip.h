@@ -221,6 +221,13 @@ int32_t mpm_bounds(const mpm_particles_t *P, float *out, void *stream);
  * if keys are not non-decreasing. MPM_ERR_SIZE when nb^3 * n_scenes exceeds int32. */
 int32_t mpm_sort_keys(const mpm_particles_t *P, const mpm_grid_t *G, int32_t *keys,
                       int32_t *vals, int32_t *unsorted, void *stream);
+/* Same with the key refined by the stencil base cell inside its block: (key << 6) |
+ * ((cx & 3) << 4 | (cy & 3) << 2 | (cz & 3)), c = floor(x/dx - 0.5). Sorted by it, the
+ * particles keep the block order of mpm_sort_keys and those sharing a stencil base cell
+ * are consecutive, which the particle pass's segmented scatter reduction exploits. No
+ * counterpart in the reference (its sort is by block only). */
+int32_t mpm_sort_keys_cells(const mpm_particles_t *P, const mpm_grid_t *G, int32_t *keys,
+                            int32_t *vals, int32_t *unsorted, void *stream);
 /* temp storage needed by mpm_sort_pairs for n keys */
 int64_t mpm_sort_temp_bytes(int32_t n);
 /* stable radix sort of (keys, vals) pairs into (keys_out, vals_out) */

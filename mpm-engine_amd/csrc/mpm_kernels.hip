@@ -449,15 +449,22 @@ __global__ void k_bounds_finish(int* out) {
 // ------------------------------------------------------------------------ sorting
 // key of solver.py:458-461: base = floor(x / dx - 0.5) >> 2, nb = n/4 + 2; batched
 // scenes are scene-major (scene * nb^3 + key), so each scene stays one contiguous range
-__global__ void k_sort_keys(mpm_particles_t P, mpm_grid_t G, int32_t* keys, int32_t* vals, int32_t* unsorted) {
+// cells = 1 refines the key by the stencil base cell inside the block (6 low bits), so
+// particles sharing a base cell become consecutive
+__global__ void k_sort_keys(mpm_particles_t P, mpm_grid_t G, int32_t* keys, int32_t* vals, int32_t* unsorted,
+                            int cells) {
     int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= P.n) return;
     const int nb = (G.n[0] >> 2) + 2;
     auto key_of = [&](int q) {
-        int b[3];
-        for (int a = 0; a < 3; ++a) b[a] = ((int)floorf(P.x[a * P.stride + q] / G.dx - 0.5f)) >> 2;
+        int c[3], b[3];
+        for (int a = 0; a < 3; ++a) {
+            c[a] = (int)floorf(P.x[a * P.stride + q] / G.dx - 0.5f);
+            b[a] = c[a] >> 2;
+        }
         const int scene = P.scene ? min(max(P.scene[q], 0), G.n_scenes - 1) : 0;
-        return scene * (nb * nb * nb) + (b[0] * nb + b[1]) * nb + b[2];
+        const int k = scene * (nb * nb * nb) + (b[0] * nb + b[1]) * nb + b[2];
+        return cells ? (k << 6) | ((c[0] & 3) << 4) | ((c[1] & 3) << 2) | (c[2] & 3) : k;
     };
     int k = key_of(p);
     keys[p] = k;
@@ -650,15 +657,25 @@ int32_t mpm_bounds(const mpm_particles_t* P, float* out, void* stream) {
     return launch_status();
 }
 
-int32_t mpm_sort_keys(const mpm_particles_t* P, const mpm_grid_t* G, int32_t* keys, int32_t* vals, int32_t* unsorted,
-                      void* stream) {
+static int32_t sort_keys(const mpm_particles_t* P, const mpm_grid_t* G, int32_t* keys, int32_t* vals,
+                         int32_t* unsorted, int cells, void* stream) {
     if (!particles_ok(P) || !grid_ok(G) || !keys || !vals || !unsorted) return MPM_ERR_ARG;
     const long nbs = (G->n[0] >> 2) + 2;
-    if (nbs * nbs * nbs * G->n_scenes > 0x7fffffffL) return MPM_ERR_SIZE;
+    if (nbs * nbs * nbs * G->n_scenes * (cells ? 64 : 1) > 0x7fffffffL) return MPM_ERR_SIZE;
     if (P->n == 0) return MPM_OK;
     hipLaunchKernelGGL(k_sort_keys, dim3(blocks_for(P->n)), dim3(kThreads), 0, (hipStream_t)stream, *P, *G, keys, vals,
-                       unsorted);
+                       unsorted, cells);
     return launch_status();
+}
+
+int32_t mpm_sort_keys(const mpm_particles_t* P, const mpm_grid_t* G, int32_t* keys, int32_t* vals, int32_t* unsorted,
+                      void* stream) {
+    return sort_keys(P, G, keys, vals, unsorted, 0, stream);
+}
+
+int32_t mpm_sort_keys_cells(const mpm_particles_t* P, const mpm_grid_t* G, int32_t* keys, int32_t* vals,
+                            int32_t* unsorted, void* stream) {
+    return sort_keys(P, G, keys, vals, unsorted, 1, stream);
 }
 
 int32_t mpm_permute(const mpm_particles_t* P, const int32_t* perm, float* scratch, void* stream) {

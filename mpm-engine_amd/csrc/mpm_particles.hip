@@ -178,6 +178,23 @@ MPM_DEV void scatter_rotated(const Stencil& st, float dx, float inv_dx, float ma
     }
 }
 
+// Segmented sum within 16-lane rows: lanes [seg_start, lane] of the caller's segment
+// (consecutive lanes scattering to the same stencil) are summed into `x` of the last
+// lane, with four DPP row shifts; c1..c8 are the per-step "source lane is in my
+// segment" conditions.
+MPM_DEV float seg_row_sum(float x, bool c1, bool c2, bool c4, bool c8) {
+    float y;
+    y = __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x111, 0xf, 0xf, true));
+    x += c1 ? y : 0.0f;
+    y = __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x112, 0xf, 0xf, true));
+    x += c2 ? y : 0.0f;
+    y = __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x114, 0xf, 0xf, true));
+    x += c4 ? y : 0.0f;
+    y = __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x118, 0xf, 0xf, true));
+    x += c8 ? y : 0.0f;
+    return x;
+}
+
 MPM_DEV bool stencil_in_grid(const mpm_grid_t& G, const Stencil& st, bool periodic) {
     bool ok = true;
 #pragma unroll
@@ -355,6 +372,8 @@ __global__ __launch_bounds__(kT) __attribute__((amdgpu_waves_per_eu(4))) void k_
                 s_scatter[3][a] = 0.f;
             });
             __syncthreads();
+            m33 fs = mzero();
+            float mass = 0.0f;
             if (alive) {
                 const float r = md->rpic_damping;
                 if (r != 0.0f) {
@@ -362,12 +381,46 @@ __global__ __launch_bounds__(kT) __attribute__((amdgpu_waves_per_eu(4))) void k_
                     Cm = madd(mscale(Cm, 1.0f - r), mscale(msub(Cm, Ct), r / 2.0f));
                     if (r < -0.001f) Cm = mzero();
                 }
-                const m33 fs = mscale(stress, -P.vol[p] * dt);
+                fs = mscale(stress, -P.vol[p] * dt);
+                mass = P.mass[p];
                 if (!stencil_in_grid(G, st, periodic)) err |= 1;
+            }
+            // Lanes whose particles share a stencil base cell (consecutive after a cell
+            // sort) form segments; when a wavefront has enough of them, each segment is
+            // summed with DPP row shifts and only its last lane (per 16-lane row) adds
+            // to LDS. Otherwise every lane adds, in lane-rotated node order.
+            const int lane = (int)(threadIdx.x & 63);
+            const int o0 = st.base[0] - ts.lo[0], o1 = st.base[1] - ts.lo[1], o2 = st.base[2] - ts.lo[2];
+            const int cellkey = (ts.lds && alive) ? (o0 * ts.d1 + o1) * ts.d2 + o2 : -1 - lane;
+            const int prevkey = __shfl_up(cellkey, 1, 64);
+            const bool head = lane == 0 || prevkey != cellkey;
+            const unsigned long long heads = __ballot(head);
+            const bool segmented = ts.lds && __popcll(~heads) >= 16;  // wavefront-uniform
+            if (segmented) {
+                const unsigned long long upto = lane == 63 ? ~0ull : ((1ull << (lane + 1)) - 1ull);
+                const int seg0 = 63 - __clzll(heads & upto);
+                const int rl = lane & 15;
+                const bool c1 = rl >= 1 && lane - 1 >= seg0, c2 = rl >= 2 && lane - 2 >= seg0;
+                const bool c4 = rl >= 4 && lane - 4 >= seg0, c8 = rl >= 8 && lane - 8 >= seg0;
+                const bool tail = rl == 15 || lane == 63 || ((heads >> (lane + 1)) & 1ull);
+                const int d1 = ts.d1, d2 = ts.d2, b0 = ts.base;
+                scatter_rotated(st, G.dx, G.inv_dx, mass, v, Cm, fs, 0, [&](int i, int j, int k, v3 add, float wm) {
+                    const float s0 = seg_row_sum(add.x, c1, c2, c4, c8);
+                    const float s1 = seg_row_sum(add.y, c1, c2, c4, c8);
+                    const float s2 = seg_row_sum(add.z, c1, c2, c4, c8);
+                    const float s3 = seg_row_sum(wm, c1, c2, c4, c8);
+                    if (tail && alive) {
+                        const int a = b0 + ((o0 + i) * d1 + (o1 + j)) * d2 + (o2 + k);
+                        atomicAdd(&s_scatter[0][a], s0);
+                        atomicAdd(&s_scatter[1][a], s1);
+                        atomicAdd(&s_scatter[2][a], s2);
+                        atomicAdd(&s_scatter[3][a], s3);
+                    }
+                });
+            } else if (alive) {
                 if (ts.lds) {
-                    const int o0 = st.base[0] - ts.lo[0], o1 = st.base[1] - ts.lo[1], o2 = st.base[2] - ts.lo[2];
                     const int d1 = ts.d1, d2 = ts.d2, b0 = ts.base;
-                    scatter_rotated(st, G.dx, G.inv_dx, P.mass[p], v, Cm, fs, (int)(threadIdx.x % 27u),
+                    scatter_rotated(st, G.dx, G.inv_dx, mass, v, Cm, fs, (int)(threadIdx.x % 27u),
                                     [&](int i, int j, int k, v3 add, float wm) {
                                         const int a = b0 + ((o0 + i) * d1 + (o1 + j)) * d2 + (o2 + k);
                                         atomicAdd(&s_scatter[0][a], add.x);
@@ -376,7 +429,7 @@ __global__ __launch_bounds__(kT) __attribute__((amdgpu_waves_per_eu(4))) void k_
                                         atomicAdd(&s_scatter[3][a], wm);
                                     });
                 } else {
-                    scatter_global(G, soff, st, periodic, P.mass[p], v, Cm, fs);
+                    scatter_global(G, soff, st, periodic, mass, v, Cm, fs);
                 }
             }
             __syncthreads();

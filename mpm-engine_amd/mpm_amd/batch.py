@@ -249,7 +249,7 @@ class BatchSolver:
         return self
 
     def _sort(self) -> bool:
-        moved = self._sim.sort_by_block()
+        moved = self._sim.sort_by_block(cells=True)
         if moved:
             self._orig = self._orig[self._sim._last_perm]
         return moved

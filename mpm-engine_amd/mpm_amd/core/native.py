@@ -94,6 +94,7 @@ SIGNATURES = {
     "mpm_blocks_compact": (_I32, [_P, _P, _P, _P, _P, _P, _P]),
     "mpm_bounds": (_I32, [_P, _P, _P]),
     "mpm_sort_keys": (_I32, [_P, _P, _P, _P, _P, _P]),
+    "mpm_sort_keys_cells": (_I32, [_P, _P, _P, _P, _P, _P]),
     "mpm_sort_temp_bytes": (_I64, [_I32]),
     "mpm_sort_pairs": (_I32, [_P, _P, _P, _P, _I32, _P, _I64, _P]),
     "mpm_permute": (_I32, [_P, _P, _P, _P]),

@@ -373,9 +373,10 @@ class HipSimulator:
         return int(self._blk["counts"][0].item()) / float(self.n_blocks)
 
     # --------------------------------------------------------------- sort
-    def sort_by_block(self) -> bool:
+    def sort_by_block(self, cells: bool = False) -> bool:
         """Stable block-key sort on device (solver.py:454-467, mpm_solver_warp.py:
-        1548-1619); returns False (no permutation) when already ordered."""
+        1548-1619); returns False (no permutation) when already ordered. With `cells`
+        the key is refined by the stencil base cell inside the block."""
         torch = self.torch
         n = self.n_particles
         if n < 2:
@@ -385,8 +386,9 @@ class HipSimulator:
         vals = torch.empty(n, dtype=torch.int32, device=dev)
         flag = torch.zeros(1, dtype=torch.int32, device=dev)
         s = self.stream()
-        native.check(self.lib.mpm_sort_keys(self.pP, self.pG, native.ptr(keys), native.ptr(vals),
-                                            native.ptr(flag), s), "sort_keys")
+        keyfn = self.lib.mpm_sort_keys_cells if cells else self.lib.mpm_sort_keys
+        native.check(keyfn(self.pP, self.pG, native.ptr(keys), native.ptr(vals), native.ptr(flag), s),
+                     "sort_keys")
         if int(flag.item()) == 0:
             return False
         tb = int(self.lib.mpm_sort_temp_bytes(n))
@@ -601,6 +603,9 @@ class Solver:
     profile: bool = False
     inversion_policy: str = "warn"
     periodic_x: bool = False
+    # "block": the reference's sort key; "cell": the same order refined by stencil base
+    # cell, so that the particle pass can sum same-cell particles before its LDS atomics
+    sort_key: str = "block"
     _sim: Any = field(default=None, init=False, repr=False)
     _step: int = field(default=0, init=False, repr=False)
     _tick: int = field(default=0, init=False, repr=False)
@@ -609,6 +614,8 @@ class Solver:
     def __post_init__(self) -> None:
         if self.inversion_policy not in {"warn", "raise", "nan"}:
             raise ValueError("inversion_policy must be 'warn', 'raise', or 'nan'")
+        if self.sort_key not in {"block", "cell"}:
+            raise ValueError("sort_key must be 'block' or 'cell'")
 
     # ------------------------------------------------------------------ setup
     def load_particles(self, pos: np.ndarray, vol: np.ndarray, cov: np.ndarray | None = None,
@@ -899,7 +906,7 @@ class Solver:
         return self
 
     def _sort_particles(self) -> bool:
-        moved = self._sim.sort_by_block()
+        moved = self._sim.sort_by_block(cells=self.sort_key == "cell")
         if moved:
             self._vol0 = self._vol0[self._sim._last_perm.cpu().numpy()]
         return moved
