@@ -86,25 +86,29 @@ class PourScene:
         self.dst = solver.add_cup(self.profile, self.target_center)
         return solver
 
-    def frame(self, solver):
-        """Command the source glass for one frame (start-of-tick pose, rigid velocity of
-        the tilt about the pivot), then advance one frame."""
-        from .colliders.glass import quat_from_axis_angle, quat_mul
-
-        fdt = self.dt * self.substeps
+    def command(self, solver):
+        """Command the source glass for the coming frame: its start-of-frame pose and the
+        rigid velocity of the tilt about the pivot."""
         vel = np.cross(self.omega, self.center - self.pivot)
         solver.set_cup(self.src, center=self.center, quat=self.quat, velocity=vel, omega=self.omega)
-        solver.step(self.dt, self.substeps)
-        # exact rigid motion of the commanded pose over the frame
-        ang = float(np.linalg.norm(self.omega)) * fdt
+
+    def advance(self):
+        """Exact rigid motion of the commanded pose over one frame."""
+        from .colliders.glass import quat_from_axis_angle, quat_mul, quat_to_mat
+
+        ang = float(np.linalg.norm(self.omega)) * self.dt * self.substeps
         if ang > 0:
             ax = self.omega / np.linalg.norm(self.omega)
             q = quat_from_axis_angle(ax, ang)
-            from .colliders.glass import quat_to_mat
-
             R = quat_to_mat(q)
             self.center = self.pivot + R @ (self.center - self.pivot)
             self.quat = quat_mul(q, self.quat)
+
+    def frame(self, solver):
+        """Command the source glass, advance one frame, move the commanded pose."""
+        self.command(solver)
+        solver.step(self.dt, self.substeps)
+        self.advance()
 
 
 def herschel_bulkley_dough() -> Material:
