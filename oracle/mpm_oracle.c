@@ -548,10 +548,144 @@ static inline int wrap_x(const OModel *md, int ix) {
     return ix;
 }
 
-/* p2g_particle, mpm_utils.py:819-916 (CDF masking absent: n_cdf == 0) */
+/* ------------------------------------------------------- CPIC transfer functions */
+/* cdf_pair_incompatible, mpm_utils.py:656-662: a lane valid on both with differing sides */
+static inline int cdf_incompat(int pc, int nt) {
+    int both_valid = pc & nt & 0x55;
+    int sides_differ = (pc ^ nt) >> 1;
+    return (both_valid & sides_differ) != 0;
+}
+
+static inline long cdf_bidx(const OCdf *c, int bi, int bj, int bk) {
+    return ((long)bi * c->nb[1] + bj) * c->nb[2] + bk;
+}
+
+/* cdf_stencil_near, mpm_utils.py:665-683: any 4^3 block under the 3x3x3 stencil tagged */
+static int cdf_near(const OCdf *c, const Stencil *s) {
+    for (int bi = s->base[0] >> 2; bi <= (s->base[0] + 2) >> 2; ++bi)
+        for (int bj = s->base[1] >> 2; bj <= (s->base[1] + 2) >> 2; ++bj)
+            for (int bk = s->base[2] >> 2; bk <= (s->base[2] + 2) >> 2; ++bk)
+                if (c->block[cdf_bidx(c, bi, bj, bk)] != 0) return 1;
+    return 0;
+}
+
+/* cdf_particle_tag, mpm_utils.py:686-724: affinity = any valid node in the stencil, side =
+ * sign of the weighted vote sum w * d * side per lane (>= 0 -> side bit set) */
+static int cdf_vote(const OCdf *c, const OModel *md, const Stencil *s, int use_prev) {
+    float votes[O_MAX_CDF] = {0.f, 0.f, 0.f, 0.f};
+    int seen = 0;
+    const int32_t *tg = use_prev ? c->tag_prev : c->tag;
+    const float *dg = use_prev ? c->d_prev : c->d;
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j)
+            for (int k = 0; k < 3; ++k) {
+                long g = gidx(md, s->base[0] + i, s->base[1] + j, s->base[2] + k);
+                int t = tg[g];
+                float d = dg[g];
+                if (t == 0) continue;
+                float weight = s->w[0][i] * s->w[1][j] * s->w[2][k];
+                for (int l = 0; l < O_MAX_CDF; ++l)
+                    if ((t >> (2 * l)) & 1) {
+                        seen |= 1 << (2 * l);
+                        float sg = ((t >> (2 * l + 1)) & 1) ? 1.0f : -1.0f;
+                        votes[l] = votes[l] + weight * d * sg;
+                    }
+            }
+    int pc = 0;
+    for (int l = 0; l < O_MAX_CDF; ++l)
+        if ((seen >> (2 * l)) & 1) {
+            pc |= 1 << (2 * l);
+            if (votes[l] >= 0.0f) pc |= 1 << (2 * l + 1);
+        }
+    return pc;
+}
+
+/* cdf_persist_tag, mpm_utils.py:727-741: a lane kept in the kernel keeps its old side */
+static int cdf_persist(int old, int fresh) {
+    int pc = 0;
+    for (int l = 0; l < O_MAX_CDF; ++l)
+        if ((fresh >> (2 * l)) & 1) {
+            pc |= 1 << (2 * l);
+            if ((old >> (2 * l)) & 1) pc |= old & (1 << (2 * l + 1));
+            else pc |= fresh & (1 << (2 * l + 1));
+        }
+    return pc;
+}
+
+/* particle tag for one half of the pass (mpm_utils.py:851-860 scatter, :971-982 gather):
+ * block pretest (skipped with mask_off), vote, persistence against the stored tag */
+static int cdf_particle_pc(const OState *st, const OModel *md, const Stencil *s, int p, int use_prev) {
+    const OCdf *c = st->cdf;
+    if (!c || c->n_cdf <= 0) return 0;
+    if (c->mask_off != 0 || cdf_near(c, s)) return cdf_persist(c->ptag[p], cdf_vote(c, md, s, use_prev));
+    return 0;
+}
+
+/* cdf_particle_ghost, mpm_utils.py:744-816: the particle's free velocity projected against
+ * its primary surface (reconstructed from the prev tag grids), plus a separation push */
+static v3 cdf_ghost(const OState *st, const OModel *md, float dt, int p, int pc, const Stencil *s) {
+    const OCdf *c = st->cdf;
+    float d_hat = 0.0f, best_w = 0.0f;
+    v3 n_acc = V3(0.f, 0.f, 0.f);
+    int lane_p = 0;
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j)
+            for (int k = 0; k < 3; ++k) {
+                long g = gidx(md, s->base[0] + i, s->base[1] + j, s->base[2] + k);
+                int t = c->tag_prev[g];
+                if (t == 0) continue;
+                int own = (t >> O_CDF_OWNER_SHIFT) & 3;
+                if (((t >> (2 * own)) & 1) == 0) continue;
+                float weight = s->w[0][i] * s->w[1][j] * s->w[2][k];
+                float sg = ((t >> (2 * own + 1)) & 1) ? 1.0f : -1.0f;
+                float d = c->d_prev[g];
+                d_hat = d_hat + weight * sg * d;
+                v3 dwt = dweight(md, s, i, j, k);
+                n_acc = vadd(n_acc, vscale(vscale(dwt, sg), d));
+                if (weight > best_w) {
+                    best_w = weight;
+                    lane_p = own;
+                }
+            }
+    v3 v_free = vadd(vload(st->v + 3 * p), vscale(V3(md->gravity[0], md->gravity[1], md->gravity[2]), dt));
+    v3 ghost = v_free;
+    if (vlen(n_acc) > 1.0e-12f) {
+        v3 n_hat = vnormalize(n_acc);
+        float side = ((pc >> (2 * lane_p + 1)) & 1) ? 1.0f : -1.0f;
+        v3 n_par = vscale(n_hat, side);
+        v3 rel = vsub(vload(st->x + 3 * p), vload(c->lane_center_prev[lane_p]));
+        v3 v_surf = vadd(vload(c->lane_velocity_prev[lane_p]), vcross(vload(c->lane_omega_prev[lane_p]), rel));
+        v3 v_rel = vsub(v_free, v_surf);
+        float vn = vdot(v_rel, n_par);
+        v3 v_tan = vsub(v_rel, vscale(n_par, vn));
+        float cpush = 0.0f;
+        float d_side = d_hat * side;
+        float eps = 0.05f * md->dx;
+        if (d_side < eps) cpush = fminf((eps - d_side) / dt, 0.5f * md->dx / dt);
+        int stype = c->lane_type[lane_p];
+        if (stype == 0) {
+            ghost = vadd(v_surf, vscale(n_par, cpush));
+        } else if (stype == 1) {
+            ghost = vadd(vadd(v_surf, v_tan), vscale(n_par, cpush));
+        } else {
+            float dvn = fmaxf(-vn, 0.0f) + cpush;
+            float tlen = vlen(v_tan);
+            if (tlen > 1.0e-12f && dvn > 0.0f) {
+                float scale = fmaxf(0.0f, tlen - c->lane_friction[lane_p] * dvn) / tlen;
+                v_tan = vscale(v_tan, scale);
+            }
+            ghost = vadd(vadd(vadd(v_surf, v_tan), vscale(n_par, fmaxf(vn, 0.0f))), vscale(n_par, cpush));
+        }
+    }
+    return ghost;
+}
+
+/* p2g_particle, mpm_utils.py:819-916, with the CPIC blocked deposits of :851-912 */
 static void p2g_particle(OState *st, const OModel *md, float dt, int p) {
     m33 stress = mload(st->stress + 9 * p);
     Stencil s = stencil(md, vload(st->x + 3 * p));
+    const int pc = cdf_particle_pc(st, md, &s, p, 0);
+    OCdf *cd = st->cdf;
     m33 C = mload(st->C + 9 * p);
     C = madd(mscale(C, 1.0f - md->rpic_damping), mscale(msub(C, mT(C)), md->rpic_damping / 2.0f));
     if (md->rpic_damping < -0.001f) C = mzero();
@@ -567,6 +701,19 @@ static void p2g_particle(OState *st, const OModel *md, float dt, int p) {
                 v3 ef = mvmul(mscale(stress, -vol), dwt);
                 v3 add = vadd(vscale(vadd(vp, mvmul(C, dpos)), weight * mass), vscale(ef, dt));
                 long g = gidx(md, ix, iy, iz);
+                if (pc != 0) {
+                    int nt = cd->tag[g];
+                    if (cdf_incompat(pc, nt)) {
+                        int own = (nt >> O_CDF_OWNER_SHIFT) & 3;
+                        v3 xw = vscale(V3((float)ix, (float)iy, (float)iz), md->dx);
+                        v3 tq = vcross(vsub(xw, vload(cd->lane_center[own])), add);
+                        for (int c = 0; c < 3; ++c) {
+                            cd->reaction_force[own][c] += add.c[c];
+                            cd->reaction_torque[own][c] += tq.c[c];
+                        }
+                        continue;
+                    }
+                }
                 st->grid_v_in[3 * g + 0] += add.c[0];
                 st->grid_v_in[3 * g + 1] += add.c[1];
                 st->grid_v_in[3 * g + 2] += add.c[2];
@@ -586,10 +733,18 @@ static void update_cov(OState *st, int p, m33 gv, float dt) {
     c[3] = n1.a[4]; c[4] = n1.a[5]; c[5] = n1.a[8];
 }
 
-/* g2p_particle, mpm_utils.py:944-1043 (CDF ghost absent: n_cdf == 0) */
+/* g2p_particle, mpm_utils.py:944-1043, with the CPIC tag store and ghost fill of :971-1017 */
 static void g2p_particle(OState *st, const OModel *md, float dt, int p) {
     v3 xp = vload(st->x + 3 * p);
     Stencil s = stencil(md, xp);
+    int pc = 0;
+    v3 ghost = V3(0.f, 0.f, 0.f);
+    OCdf *cd = st->cdf;
+    if (cd && cd->n_cdf > 0) {
+        pc = cdf_particle_pc(st, md, &s, p, 1);
+        cd->ptag[p] = pc;
+        if (pc != 0) ghost = cdf_ghost(st, md, dt, p, pc, &s);
+    }
     v3 nv = V3(0, 0, 0);
     m33 nC = mzero(), nF = mzero();
     for (int i = 0; i < 3; ++i)
@@ -599,6 +754,7 @@ static void g2p_particle(OState *st, const OModel *md, float dt, int p) {
                 v3 dpos = vsub(V3((float)i, (float)j, (float)k), s.fx);
                 float weight = s.w[0][i] * s.w[1][j] * s.w[2][k];
                 v3 gv = vload(st->grid_v_out + 3 * gidx(md, ix, iy, iz));
+                if (pc != 0 && cdf_incompat(pc, cd->tag_prev[gidx(md, ix, iy, iz)])) gv = ghost;
                 nv = vadd(nv, vscale(gv, weight));
                 nC = madd(nC, mscale(mouter(gv, dpos), weight * md->inv_dx * 4.0f));
                 v3 dwt = dweight(md, &s, i, j, k);
@@ -876,6 +1032,80 @@ static v3 quat_rotate(const float *q, v3 v, int inverse) {
     float d = 2.0f * vdot(qv, v);
     v3 cr = vscale(vcross(qv, v), (inverse ? -w : w) * 2.0f);
     return vadd(vadd(vscale(v, c), vscale(qv, d)), cr);
+}
+
+/* ------------------------------------------------------------------ CPIC stamps */
+/* cdf_zero_tags, mpm_solver_warp.py:94-102 */
+EXPORT void o_cdf_zero_tags(OState *st, OModel *md, const int32_t *lo, const int32_t *dims) {
+    OCdf *c = st->cdf;
+    BOX_LOOP(lo, dims) {
+        long g = gidx(md, gi, gj, gk);
+        c->tag[g] = 0;
+        c->d[g] = 1.0e9f;
+    }
+}
+
+/* cdf_copy_prev, mpm_solver_warp.py:105-115 */
+EXPORT void o_cdf_copy_prev(OState *st, OModel *md, const int32_t *lo, const int32_t *dims) {
+    OCdf *c = st->cdf;
+    BOX_LOOP(lo, dims) {
+        long g = gidx(md, gi, gj, gk);
+        c->tag_prev[g] = c->tag[g];
+        c->d_prev[g] = c->d[g];
+    }
+}
+
+/* cdf_block_mask, mpm_solver_warp.py:118-139 (blo / bdims in 4^3 blocks) */
+EXPORT void o_cdf_block_mask(OState *st, OModel *md, const int32_t *blo, const int32_t *bdims) {
+    OCdf *c = st->cdf;
+    BOX_LOOP(blo, bdims) {
+        int occ = 0;
+        for (int i = 0; i < 4; ++i)
+            for (int j = 0; j < 4; ++j)
+                for (int k = 0; k < 4; ++k) {
+                    int ci = (gi << 2) + i, cj = (gj << 2) + j, ck = (gk << 2) + k;
+                    if (ci < md->nx && cj < md->ny && ck < md->nz) {
+                        long g = gidx(md, ci, cj, ck);
+                        occ = occ | c->tag[g] | c->tag_prev[g];
+                    }
+                }
+        c->block[cdf_bidx(c, gi, gj, gk)] = occ;
+    }
+}
+
+/* cdf_stamp, mpm_solver_warp.py:142-183: side / validity bits and owner distance of one
+ * collider under its live pose; a node is tagged only when all eight trilerp corners are
+ * valid (trilerp >= 0.999) and |d| <= band */
+EXPORT void o_cdf_stamp(float time, OState *st, OModel *md, const OCdfCollider *pm, const int32_t *lo,
+                        const int32_t *dims) {
+    OCdf *c = st->cdf;
+    BOX_LOOP(lo, dims) {
+        if (time < pm->start_time || time >= pm->end_time) return;
+        v3 xw = V3((float)gi * md->dx, (float)gj * md->dx, (float)gk * md->dx);
+        v3 rel = vsub(xw, vload(pm->center));
+        v3 body = quat_rotate(pm->quat, rel, 1);
+        v3 fidx = V3((body.c[0] - pm->origin[0]) / pm->cell, (body.c[1] - pm->origin[1]) / pm->cell,
+                     (body.c[2] - pm->origin[2]) / pm->cell);
+        float rf = (float)pm->res - 1.0f;
+        if (fidx.c[0] < 0.0f || fidx.c[1] < 0.0f || fidx.c[2] < 0.0f || fidx.c[0] > rf || fidx.c[1] > rf ||
+            fidx.c[2] > rf)
+            continue;
+        float ok, d;
+        trilerp(pm->cdf_valid, 1, pm->res, fidx, &ok);
+        if (ok < 0.999f) continue;
+        trilerp(pm->cdf_val, 1, pm->res, fidx, &d);
+        float ud = fabsf(d);
+        if (ud > pm->band) continue;
+        int side = d >= 0.0f ? 1 : 0;
+        long g = gidx(md, gi, gj, gk);
+        int t = c->tag[g];
+        t = t | (1 << (2 * pm->lane)) | (side << (2 * pm->lane + 1));
+        if (ud < c->d[g]) {
+            c->d[g] = ud;
+            t = (t ^ (t & (3 << O_CDF_OWNER_SHIFT))) | (pm->lane << O_CDF_OWNER_SHIFT);
+        }
+        c->tag[g] = t;
+    }
 }
 
 /* mesh-SDF collide kernel, mpm_solver_warp.py:2680-2736 */

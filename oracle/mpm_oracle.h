@@ -36,6 +36,38 @@ typedef struct OModel {
     float *mu, *lam, *yield_stress;
 } OModel;
 
+/* CPIC thin-boundary (CDF) state (warp_utils.py:9-20 bit layout; mpm_solver_warp.py:389-436
+ * allocation). Node tag: bit 2l = lane l valid, bit 2l+1 = lane l side, bits [8, 10) = owner
+ * lane. Tag / distance grids are dense z-fastest like grid_m; block = 4^3 occupancy mask. */
+#define O_MAX_CDF 4
+#define O_CDF_OWNER_SHIFT 8
+typedef struct OCdf {
+    int32_t n_cdf, mask_off;
+    int32_t nb[3];
+    int32_t *tag, *tag_prev;
+    float *d, *d_prev;
+    int32_t *block;
+    int32_t *ptag; /* particle_cdf_tag [N] */
+    float lane_center[O_MAX_CDF][3], lane_velocity[O_MAX_CDF][3], lane_omega[O_MAX_CDF][3];
+    float lane_center_prev[O_MAX_CDF][3], lane_velocity_prev[O_MAX_CDF][3], lane_omega_prev[O_MAX_CDF][3];
+    float lane_friction[O_MAX_CDF];
+    int32_t lane_type[O_MAX_CDF];
+    float reaction_force[O_MAX_CDF][3], reaction_torque[O_MAX_CDF][3];
+} OCdf;
+
+/* one CDF collider's stamp parameters (CDFCollider, mpm_solver_warp.py:2859-2880) */
+typedef struct OCdfCollider {
+    const float *cdf_val, *cdf_valid; /* [res^3] */
+    int32_t res;
+    float origin[3];
+    float cell;
+    float center[3];
+    float quat[4]; /* x y z w */
+    float band;
+    int32_t lane;
+    float start_time, end_time;
+} OCdfCollider;
+
 typedef struct OState {
     int32_t n;
     float *x, *v, *F, *F_trial, *C, *stress, *L, *R;
@@ -43,6 +75,7 @@ typedef struct OState {
     float *cov, *init_cov;
     int32_t *selection, *material;
     float *grid_m, *grid_v_in, *grid_v_out;
+    OCdf *cdf; /* NULL: no CDF colliders (n_cdf == 0) */
 } OState;
 
 /* collider parameter blocks (warp_utils.py collider structs; mpm_solver_warp.py:1880-2777) */

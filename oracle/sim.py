@@ -52,6 +52,28 @@ class OModel(C.Structure):
     ]
 
 
+MAX_CDF = 4
+_LANE3 = (C.c_float * 3) * MAX_CDF
+
+
+class OCdf(C.Structure):
+    _fields_ = [("n_cdf", C.c_int32), ("mask_off", C.c_int32), ("nb", C.c_int32 * 3),
+                ("tag", IP), ("tag_prev", IP), ("d", FP), ("d_prev", FP), ("block", IP),
+                ("ptag", IP),
+                ("lane_center", _LANE3), ("lane_velocity", _LANE3), ("lane_omega", _LANE3),
+                ("lane_center_prev", _LANE3), ("lane_velocity_prev", _LANE3),
+                ("lane_omega_prev", _LANE3), ("lane_friction", C.c_float * MAX_CDF),
+                ("lane_type", C.c_int32 * MAX_CDF), ("reaction_force", _LANE3),
+                ("reaction_torque", _LANE3)]
+
+
+class OCdfCollider(C.Structure):
+    _fields_ = [("cdf_val", FP), ("cdf_valid", FP), ("res", C.c_int32),
+                ("origin", C.c_float * 3), ("cell", C.c_float), ("center", C.c_float * 3),
+                ("quat", C.c_float * 4), ("band", C.c_float), ("lane", C.c_int32),
+                ("start_time", C.c_float), ("end_time", C.c_float)]
+
+
 class OState(C.Structure):
     _fields_ = [
         ("n", C.c_int32),
@@ -59,6 +81,7 @@ class OState(C.Structure):
         ("L", FP), ("R", FP), ("vol", FP), ("mass", FP), ("density", FP), ("Jp", FP),
         ("cov", FP), ("init_cov", FP), ("selection", IP), ("material", IP),
         ("grid_m", FP), ("grid_v_in", FP), ("grid_v_out", FP),
+        ("cdf", C.POINTER(OCdf)),
     ]
 
 
@@ -131,6 +154,9 @@ def lib():
         L.o_blocks_dilate.argtypes = [IP, IP, IP]
         L.o_blocks_compact.argtypes = [IP, IP, IP, IP, IP, IP]
         L.o_blocks_grid_op.argtypes = [S, M, C.c_int, C.c_float, C.c_float, IP, C.c_int, IP]
+        for nm in ("o_cdf_zero_tags", "o_cdf_copy_prev", "o_cdf_block_mask"):
+            getattr(L, nm).argtypes = [S, M, IP, IP]
+        L.o_cdf_stamp.argtypes = [C.c_float, S, M, C.POINTER(OCdfCollider), IP, IP]
         L.o_compute_R.argtypes = [S]
         L.o_compute_cov.argtypes = [S]
         _LIB = L
@@ -266,6 +292,20 @@ class OracleSim:
         self._blk = None
         self._revolved_rot = {}
         self._sdf_guard = {}
+        # CPIC (CDF) colliders, mpm_solver_warp.py:389-436: grids allocated on first add
+        self.particle_cdf_tag = np.zeros(n, np.int32)
+        self.cdf = None          # OCdf (tag grids, lane arrays, reaction accumulators)
+        self.cdf_params = []
+        self.cdf_aabbs = []
+        self._cdf_guard = {}
+        self._cdf_lane_dirty = []
+        self._cdf_lane_active = []
+        self._cdf_lane_prev_box = []
+        self._cdf_lane_prev_stale = False
+        self._cdf_tag_prev_stale_box = None
+        self._cdf_stamp_count = 0
+        self._cdf_force_stamp = False
+        self._cdf_lag_prev = False
 
     # ------------------------------------------------------------------ ctypes views
     def _state(self):
@@ -276,6 +316,9 @@ class OracleSim:
             setattr(s, nm, fptr(getattr(self, nm)))
         s.selection = iptr(self.selection)
         s.material = iptr(self.material)
+        if self.cdf is not None and self.cdf.n_cdf > 0:
+            self.cdf.ptag = iptr(self.particle_cdf_tag)
+            s.cdf = C.pointer(self.cdf)
         self._sref = s
         return C.byref(s)
 
@@ -666,6 +709,246 @@ class OracleSim:
                 tuple(int(a.max(0)[i] - a.min(0)[i] + 1) for i in range(3))) if len(a) else None)
         self._register("pointcloud", p, None, (lambda box=box: box), "pointcloud")
 
+    # ------------------------------------------------------------------ CPIC (CDF)
+    def add_cdf_collider(self, cdf_values, cdf_valid, origin, cell, built_band, center,
+                         quat=(0.0, 0.0, 0.0, 1.0), velocity=(0.0, 0.0, 0.0),
+                         omega=(0.0, 0.0, 0.0), band=None, surface="separable", friction=0.4,
+                         start_time=0.0, end_time=999.0):
+        """mpm_solver_warp.py:2819-2919 (guards, AABB with a 2-node halo, lazy grids)."""
+        lane = len(self.cdf_params)
+        if lane >= MAX_CDF:
+            raise ValueError(f"at most {MAX_CDF} CDF colliders per scene")
+        res = int(cdf_values.shape[0])
+        if band is None:
+            band = min(float(built_band), 2.0 * float(self.dx))
+        if band > float(built_band) + 1e-12:
+            raise ValueError("runtime band exceeds the band the field was built with")
+        if band < 1.5 * float(self.dx):
+            warnings.warn("CDF band is below 1.5 dx: straddling particle-node pairs can go "
+                          "untagged and the wall can leak", RuntimeWarning, stacklevel=2)
+        sid = {"sticky": 0, "slip": 1, "separable": 2}[surface]
+        ext = (res - 1) * float(cell)
+        corners = np.array([[float(origin[0]) + a * ext, float(origin[1]) + b * ext,
+                             float(origin[2]) + c * ext]
+                            for a in (0, 1) for b in (0, 1) for c in (0, 1)])
+        r_max = float(np.linalg.norm(corners, axis=1).max())
+        self._cdf_guard[lane] = {"r_max": r_max, "band": float(band), "warned": False}
+        p = OCdfCollider()
+        val = np.ascontiguousarray(cdf_values, dtype=f32)
+        ok = np.ascontiguousarray(cdf_valid, dtype=f32)
+        self._keep += [val, ok]
+        p.cdf_val, p.cdf_valid, p.res = fptr(val), fptr(ok), res
+        p.origin[:] = [float(v) for v in origin]
+        p.cell = float(cell)
+        p.center[:] = [float(v) for v in center]
+        p.quat[:] = [float(v) for v in quat]
+        p.band, p.lane = float(band), lane
+        p.start_time, p.end_time = start_time, end_time
+        p._velocity = [float(f32(v)) for v in velocity]
+        p._omega = [float(f32(v)) for v in omega]
+        p._type, p._friction = sid, float(friction)
+        self.cdf_params.append(p)
+
+        def aabb(param=p, corners=corners):
+            qx, qy, qz, qw = param.quat[0], param.quat[1], param.quat[2], param.quat[3]
+            R = np.array([
+                [1 - 2 * (qy * qy + qz * qz), 2 * (qx * qy - qz * qw), 2 * (qx * qz + qy * qw)],
+                [2 * (qx * qy + qz * qw), 1 - 2 * (qx * qx + qz * qz), 2 * (qy * qz - qx * qw)],
+                [2 * (qx * qz - qy * qw), 2 * (qy * qz + qx * qw), 1 - 2 * (qx * qx + qy * qy)]])
+            c = np.array([param.center[0], param.center[1], param.center[2]])
+            pts = c + corners @ R.T
+            return self._grid_box(pts.min(axis=0), pts.max(axis=0), halo=2)
+        self.cdf_aabbs.append(aabb)
+        self._cdf_lane_dirty.append(True)
+        self._cdf_lane_active.append(self.time >= start_time and self.time < end_time)
+        self._cdf_lane_prev_box.append(None)
+        if self.cdf is None:
+            g = self.n_grid
+            nb = (g + 3) // 4
+            self.cdf_tag = np.zeros((g, g, g), np.int32)
+            self.cdf_tag_prev = np.zeros((g, g, g), np.int32)
+            self.cdf_d = np.full((g, g, g), 1.0e9, f32)
+            self.cdf_d_prev = np.full((g, g, g), 1.0e9, f32)
+            self.cdf_block = np.zeros((nb, nb, nb), np.int32)
+            c = OCdf()
+            c.nb[:] = [nb] * 3
+            c.tag, c.tag_prev = iptr(self.cdf_tag), iptr(self.cdf_tag_prev)
+            c.d, c.d_prev = fptr(self.cdf_d), fptr(self.cdf_d_prev)
+            c.block = iptr(self.cdf_block)
+            self.cdf = c
+        self.cdf.n_cdf = len(self.cdf_params)
+        self._sync_cdf_lane(lane)
+        return lane
+
+    def _sync_cdf_lane(self, lane):
+        """mpm_solver_warp.py:2921-2943: pose / contact state into the current lane arrays."""
+        p, c = self.cdf_params[lane], self.cdf
+        c.lane_center[lane][:] = [p.center[0], p.center[1], p.center[2]]
+        c.lane_velocity[lane][:] = p._velocity
+        c.lane_omega[lane][:] = p._omega
+        c.lane_friction[lane] = p._friction
+        c.lane_type[lane] = p._type
+
+    def set_cdf_pose(self, handle, center=None, quat=None, velocity=None, omega=None):
+        """mpm_solver_warp.py:2945-2996."""
+        p = self.cdf_params[handle]
+        guard = self._cdf_guard.get(handle)
+        if guard is not None and not guard["warned"]:
+            jump = 0.0
+            if center is not None:
+                jump += float(np.linalg.norm(np.array([float(center[i]) - p.center[i]
+                                                       for i in range(3)])))
+            if quat is not None:
+                dot = abs(sum(float(quat[i]) * p.quat[i] for i in range(4)))
+                jump += 2.0 * float(np.arccos(min(1.0, dot))) * guard["r_max"]
+            if jump > guard["band"]:
+                guard["warned"] = True
+                warnings.warn("set_cdf_pose jumped the surface by more than the band in one "
+                              "tick", RuntimeWarning, stacklevel=2)
+        moved = synced = False
+        if center is not None:
+            new = [float(f32(v)) for v in center]
+            moved |= new != [p.center[0], p.center[1], p.center[2]]
+            p.center[:] = new
+        if quat is not None:
+            new = [float(f32(v)) for v in quat]
+            moved |= new != [p.quat[i] for i in range(4)]
+            p.quat[:] = new
+        if velocity is not None:
+            new = [float(f32(v)) for v in velocity]
+            synced |= new != p._velocity
+            p._velocity = new
+        if omega is not None:
+            new = [float(f32(v)) for v in omega]
+            synced |= new != p._omega
+            p._omega = new
+        if moved:
+            self._cdf_lane_dirty[handle] = True
+        if moved or synced:
+            self._sync_cdf_lane(handle)
+            self._cdf_lane_prev_stale = True
+
+    def _cdf_dirty_box(self):
+        """mpm_solver_warp.py:2998-3018."""
+        boxes = []
+        for lane, fn in enumerate(self.cdf_aabbs):
+            if not self._cdf_lane_dirty[lane]:
+                continue
+            b = fn()
+            prev = self._cdf_lane_prev_box[lane]
+            if b is not None:
+                boxes.append(b)
+            if prev is not None:
+                boxes.append(prev)
+            self._cdf_lane_prev_box[lane] = b
+        if not boxes:
+            return None
+        lo = tuple(min(b[0][i] for b in boxes) for i in range(3))
+        hi = tuple(max(b[0][i] + b[1][i] for b in boxes) for i in range(3))
+        return (lo, tuple(hi[i] - lo[i] for i in range(3)))
+
+    @staticmethod
+    def _cdf_box_intersect(a, b):
+        """mpm_solver_warp.py:3020-3028."""
+        if a is None or b is None:
+            return None
+        lo = tuple(max(a[0][i], b[0][i]) for i in range(3))
+        hi = tuple(min(a[0][i] + a[1][i], b[0][i] + b[1][i]) for i in range(3))
+        if any(hi[i] <= lo[i] for i in range(3)):
+            return None
+        return (lo, tuple(hi[i] - lo[i] for i in range(3)))
+
+    def _cdf_reconcile_prev(self):
+        """mpm_solver_warp.py:3030-3038."""
+        if self._cdf_tag_prev_stale_box is not None:
+            lo, dims = self._cdf_tag_prev_stale_box
+            lib().o_cdf_copy_prev(self._state(), self._model(), i3(lo), i3(dims))
+            self._cdf_tag_prev_stale_box = None
+
+    def _stamp_cdf(self, dt, lag_prev=False):
+        """mpm_solver_warp.py:3040-3098: per-substep restamp of the dirty lanes' boxes."""
+        if self.cdf is None or self.cdf.n_cdf == 0:
+            return
+        self._cdf_reconcile_prev()
+        for lane, p in enumerate(self.cdf_params):
+            active = self.time >= p.start_time and self.time < p.end_time
+            if active != self._cdf_lane_active[lane]:
+                self._cdf_lane_active[lane] = active
+                self._cdf_lane_dirty[lane] = True
+        if self._cdf_force_stamp:
+            self._cdf_lane_dirty = [True] * len(self.cdf_params)
+        if not any(self._cdf_lane_dirty):
+            return
+        box = self._cdf_dirty_box()
+        self._cdf_lane_dirty = [False] * len(self.cdf_params)
+        if box is None:
+            return
+        self._cdf_stamp_count += 1
+        L = lib()
+        st, md = self._state(), self._model()
+        lo, dims = box
+        L.o_cdf_zero_tags(st, md, i3(lo), i3(dims))
+        for lane, p in enumerate(self.cdf_params):
+            clip = self._cdf_box_intersect(box, self.cdf_aabbs[lane]())
+            if clip is None:
+                continue
+            L.o_cdf_stamp(float(f32(self.time)), st, md, C.byref(p), i3(clip[0]), i3(clip[1]))
+        if lag_prev:
+            self._cdf_tag_prev_stale_box = box
+        else:
+            L.o_cdf_copy_prev(st, md, i3(lo), i3(dims))
+        blo = tuple(lo[i] >> 2 for i in range(3))
+        bdims = tuple(((lo[i] + dims[i] - 1) >> 2) - blo[i] + 1 for i in range(3))
+        L.o_cdf_block_mask(st, md, i3(blo), i3(bdims))
+
+    def _cdf_copy_lane_prev(self):
+        """mpm_solver_warp.py:3100-3105."""
+        c = self.cdf
+        for lane in range(MAX_CDF):
+            c.lane_center_prev[lane][:] = list(c.lane_center[lane])
+            c.lane_velocity_prev[lane][:] = list(c.lane_velocity[lane])
+            c.lane_omega_prev[lane][:] = list(c.lane_omega[lane])
+        self._cdf_lane_prev_stale = False
+
+    def _integrate_cdf_poses(self, dt, lag_prev=False):
+        """mpm_solver_warp.py:3107-3148: advance moving lanes one substep."""
+        if not self.cdf_params:
+            return
+        if lag_prev and self._cdf_lane_prev_stale:
+            self._cdf_copy_lane_prev()
+        for lane, p in enumerate(self.cdf_params):
+            if not (self.time >= p.start_time and self.time < p.end_time):
+                continue
+            v = p._velocity
+            w = np.array(p._omega, dtype=float)
+            if v[0] == 0.0 and v[1] == 0.0 and v[2] == 0.0 and not w.any():
+                continue
+            guard = self._cdf_guard[lane]
+            speed = float(np.hypot(np.hypot(v[0], v[1]), v[2])) + \
+                float(np.linalg.norm(w)) * guard["r_max"]
+            if speed * dt > guard["band"] and not guard["warned"]:
+                guard["warned"] = True
+                warnings.warn("CDF collider surface sweeps more than the band per substep: "
+                              "material can end up on the wrong side", RuntimeWarning,
+                              stacklevel=2)
+            c = [p.center[0], p.center[1], p.center[2]]
+            p.center[:] = [c[i] + dt * v[i] for i in range(3)]
+            q = np.array([p.quat[0], p.quat[1], p.quat[2], p.quat[3]], dtype=float)
+            q = _omega_step_quat(q, w, dt)
+            p.quat[:] = [float(t) for t in q]
+            self._sync_cdf_lane(lane)
+            self._cdf_lane_dirty[lane] = True
+            self._cdf_lane_prev_stale = True
+        if not lag_prev and self._cdf_lane_prev_stale:
+            self._cdf_copy_lane_prev()
+
+    def reset_cdf_wrench(self):
+        """mpm_solver_warp.py:3153-3155."""
+        if self.cdf is not None:
+            for lane in range(MAX_CDF):
+                self.cdf.reaction_force[lane][:] = [0.0, 0.0, 0.0]
+                self.cdf.reaction_torque[lane][:] = [0.0, 0.0, 0.0]
+
     # ------------------------------------------------------------------ BC application
     _COLLIDE = {"plane": "o_collide_plane", "box": "o_collide_box", "cup": "o_collide_cup",
                 "walls": "o_collide_walls", "sdf": "o_collide_sdf",
@@ -696,6 +979,7 @@ class OracleSim:
                                                     C.byref(param), i3(lo), i3(dims))
             if self.modify_bc[k] is not None:
                 self.modify_bc[k](self.time, dt, param)
+        self._integrate_cdf_poses(dt, lag_prev=self._cdf_lag_prev)
 
     # ------------------------------------------------------------------ substeps
     def _boxes(self):
@@ -717,6 +1001,8 @@ class OracleSim:
     def p2g2p(self, dt):
         """Split substep, mpm_solver_warp.py:1092-1281 (CPU path: no graphs)."""
         L = lib()
+        self._cdf_lag_prev = False
+        self._stamp_cdf(dt)
         st, md = self._state(), self._model()
         sparse = self.sparse_ready and self.restrict_grid
         if sparse:
@@ -753,6 +1039,8 @@ class OracleSim:
         """Fused tick, mpm_solver_warp.py:1379-1546 (CPU path: no graphs)."""
         L = lib()
         for s in range(substeps):
+            self._cdf_lag_prev = True
+            self._stamp_cdf(dt, lag_prev=True)
             st, md = self._state(), self._model()
             g, z = self._boxes()
             if s == 0:
@@ -768,6 +1056,8 @@ class OracleSim:
                 L.o_grid_damping(st, md, self.grid_v_damping_scale, i3(g[0]), i3(g[1]))
             self._apply_grid_bc(dt)
             self.time = self.time + dt
+        if self.cdf is not None and self.cdf.n_cdf > 0:
+            self._cdf_reconcile_prev()
         L.o_g2p(self._state(), self._model(), dt)
 
     # ------------------------------------------------------------------ sparse blocks
@@ -802,7 +1092,8 @@ class OracleSim:
         """mpm_solver_warp.py:1548-1619 (every per-particle array, incl. mu/lam/yield)."""
         perm = np.asarray(perm)
         for nm in ("x", "v", "F", "F_trial", "R", "stress", "C", "L", "vol", "mass",
-                   "density", "Jp", "mu", "lam", "yield_stress", "selection", "material"):
+                   "density", "Jp", "mu", "lam", "yield_stress", "selection", "material",
+                   "particle_cdf_tag"):
             a = getattr(self, nm)
             a[...] = a[perm]
         for nm in ("init_cov", "cov"):
@@ -935,6 +1226,32 @@ class OracleSolver:
     def set_sdf_pose(self, handle, center=None, quat=None, velocity=None, omega=None):
         self._sim.set_sdf_pose(handle, center=center, quat=quat, velocity=velocity, omega=omega)
         return self
+
+    def add_cdf_collider(self, cdf, center, quat=(0.0, 0.0, 0.0, 1.0), velocity=(0.0, 0.0, 0.0),
+                         omega=(0.0, 0.0, 0.0), band=None, surface="separable", friction=0.4,
+                         start_time=0.0, end_time=1.0e9):
+        """solver.py:363-386."""
+        if self.periodic_x:
+            raise NotImplementedError("CDF colliders do not wrap the periodic x axis")
+        return self._sim.add_cdf_collider(cdf.values, cdf.valid, cdf.origin, cdf.cell, cdf.band,
+                                          center, quat=quat, velocity=velocity, omega=omega,
+                                          band=band, surface=surface, friction=friction,
+                                          start_time=start_time, end_time=end_time)
+
+    def set_cdf_pose(self, handle, center=None, quat=None, velocity=None, omega=None):
+        self._sim.set_cdf_pose(handle, center=center, quat=quat, velocity=velocity, omega=omega)
+        return self
+
+    def reset_cdf_wrench(self):
+        self._sim.reset_cdf_wrench()
+        return self
+
+    def cdf_wrench(self, handle, dt):
+        """solver.py:401-413: blocked-deposit reaction since the last reset, / dt."""
+        c = self._sim.cdf
+        f = np.array(list(c.reaction_force[handle]), dtype=float)
+        t = np.array(list(c.reaction_torque[handle]), dtype=float)
+        return {"force": f / dt, "torque": t / dt}
 
     def step(self, dt, substeps=1):
         """solver.py:429-452."""
