@@ -36,7 +36,7 @@ extern "C" {
 #endif
 
 #define MPM_MAX_MATERIALS 16
-#define MPM_ABI_VERSION 2
+#define MPM_ABI_VERSION 3
 
 typedef enum mpm_status {
     MPM_OK = 0,
@@ -80,7 +80,50 @@ typedef struct mpm_particles {
     int32_t *material, *selection;
     float *cov; /* 6 planes (xx, xy, xz, yy, yz, zz) or NULL when covariance is off */
     int32_t *scene; /* scene id per particle for batched scenes, or NULL (all scene 0) */
+    int32_t *cdf_tag; /* CPIC particle tag (particle_cdf_tag, warp_utils.py:87-157), required
+                         when the grid carries CDF colliders, else NULL */
 } mpm_particles_t;
+
+/* ---- CPIC thin-boundary (CDF) colliders ---------------------------------------------
+ * Open oriented surfaces that sever particle-node transfers across themselves (the
+ * reference's add_cdf_collider, mpm_solver_warp.py:2819-3155; transfer masking and ghost
+ * fill, mpm_utils.py:648-1017). Node tag (warp_utils.py:9-20): bit 2l = lane l valid,
+ * bit 2l+1 = lane l side, bits [8, 10) = owner lane (nearest surface). Tag and owner
+ * distance grids use the blocked node layout of node_in; the occupancy mask holds one int
+ * per 4x4x4 block (the grid block), OR of tag | tag_prev over its nodes. The gather half
+ * of a pass reads tag_prev / dist_prev and the lanes' *_prev pose, the scatter half tag /
+ * dist and the current pose: the fused pass's gather uses the grid state of the previous
+ * substep, its scatter builds the next one. Single scene only. */
+#define MPM_MAX_CDF 4
+#define MPM_CDF_OWNER_SHIFT 8
+
+typedef struct mpm_cdf_lanes {
+    float center[MPM_MAX_CDF][3], velocity[MPM_MAX_CDF][3], omega[MPM_MAX_CDF][3];
+    float center_prev[MPM_MAX_CDF][3], velocity_prev[MPM_MAX_CDF][3], omega_prev[MPM_MAX_CDF][3];
+    float friction[MPM_MAX_CDF];
+    int32_t type[MPM_MAX_CDF]; /* 0 sticky, 1 slip, 2 separable + Coulomb */
+} mpm_cdf_lanes_t;
+
+typedef struct mpm_cdf {
+    int32_t n_cdf;        /* lanes in use; 0 = no CDF colliders (other fields unused) */
+    int32_t mask_off;     /* 1 = vote for every particle (no block-occupancy pretest) */
+    int32_t *tag, *tag_prev;   /* int per node */
+    float *dist, *dist_prev;   /* owner distance per node (1e9 where untagged) */
+    int32_t *block;            /* int per grid block */
+    mpm_cdf_lanes_t *lanes;    /* device */
+    float *reaction;           /* device [MPM_MAX_CDF][6]: blocked-deposit force xyz, torque xyz */
+} mpm_cdf_t;
+
+/* One collider's stamp (the reference's CDFCollider at its live pose). */
+typedef struct mpm_cdf_collider {
+    const float *val, *valid;  /* device [res^3], z fastest: signed distance, validity */
+    int32_t res, lane;
+    float origin[3], cell;     /* body-frame voxel grid */
+    float center[3];
+    float quat[4];             /* x, y, z, w */
+    float band;
+    float time, start_time, end_time; /* fp32 simulation time of the stamp; active window */
+} mpm_cdf_collider_t;
 
 typedef struct mpm_grid {
     float *node_in;       /* float4 per node: momentum xyz, mass */
@@ -96,6 +139,7 @@ typedef struct mpm_grid {
                              blocks [s * B, (s + 1) * B), B = nb[0] * nb[1] * nb[2];
                              block codes in every block list are these global codes */
     int32_t pad_;
+    mpm_cdf_t cdf;        /* CPIC colliders (cdf.n_cdf == 0: none) */
 } mpm_grid_t;
 
 /* Model parameters (MPMModelStruct, warp_utils.py:23-84). Lives in device memory; the
@@ -154,7 +198,8 @@ typedef struct mpm_tick_ctl {
 int32_t mpm_abi_version(void);
 const char *mpm_status_string(int32_t status);
 /* sizeof of the ABI structs, so bindings can verify their mirrors */
-int32_t mpm_struct_size(int32_t which); /* 0 particles, 1 grid, 2 model, 3 collider, 4 tick_ctl */
+int32_t mpm_struct_size(int32_t which); /* 0 particles, 1 grid, 2 model, 3 collider, 4 tick_ctl,
+                                           5 cdf lanes, 6 cdf collider */
 
 /* ---- particle pass --------------------------------------------------------------- */
 /* One pass over all particles with selection == 0 doing the stages in `mode`:
@@ -162,9 +207,31 @@ int32_t mpm_struct_size(int32_t which); /* 0 particles, 1 grid, 2 model, 3 colli
  * STRESS|P2G is the tick prologue (compute_stress_from_F_trial + p2g_apic_with_stress,
  * mpm_utils.py:919-923, 1152-1158); G2P alone is the tick epilogue (g2p, :1046-1052).
  * Scatter accumulates into node_in with float atomics; a stencil node outside the grid
- * is skipped and sets error bit 1. `model` is a device pointer. */
+ * is skipped and sets error bit 1. `model` is a device pointer.
+ * With G->cdf.n_cdf > 0 (single scene, P->cdf_tag set, no MPM_PASS_LEAN: the ghost
+ * velocity needs the stored particle velocity) the gather reconstructs each particle's
+ * tag from tag_prev with persistence against cdf_tag, stores it, and substitutes the
+ * ghost velocity at incompatible nodes (mpm_utils.py:971-1017); the scatter
+ * reconstructs the tag from tag and sends deposits to incompatible nodes into
+ * cdf.reaction instead of node_in (mpm_utils.py:851-912). */
 int32_t mpm_particle_pass(const mpm_particles_t *P, const mpm_grid_t *G, const mpm_model_t *model,
                           float dt, int32_t mode, void *stream);
+
+/* ---- CDF stamps (mpm_solver_warp.py:94-183; boxes are node boxes [lo, hi)) -------- */
+/* tag = 0, dist = 1e9 over the box (cdf_zero_tags) */
+int32_t mpm_cdf_zero_tags(const mpm_grid_t *G, const int32_t *lo, const int32_t *hi, void *stream);
+/* tag_prev = tag, dist_prev = dist over the box (cdf_copy_prev) */
+int32_t mpm_cdf_copy_prev(const mpm_grid_t *G, const int32_t *lo, const int32_t *hi, void *stream);
+/* stamp one collider's bits over the box (cdf_stamp): a node is tagged when all eight
+ * trilinear corners of its body-frame voxel are valid (>= 0.999) and |d| <= band; the
+ * nearest lane owns it. Colliders stamp one after the other on one stream. */
+int32_t mpm_cdf_stamp(const mpm_grid_t *G, const mpm_cdf_collider_t *collider, const int32_t *lo,
+                      const int32_t *hi, void *stream);
+/* rebuild the occupancy mask over the block box [blo, bhi) (cdf_block_mask) */
+int32_t mpm_cdf_block_mask(const mpm_grid_t *G, const int32_t *blo, const int32_t *bhi, void *stream);
+/* write `lanes` (host struct, copied by value into the launch) to G->cdf.lanes in stream
+ * order (the reference's _sync_cdf_lane / _cdf_copy_lane_prev uploads) */
+int32_t mpm_cdf_set_lanes(const mpm_grid_t *G, const mpm_cdf_lanes_t *lanes, void *stream);
 
 /* ---- grid -------------------------------------------------------------------------- */
 /* Clear node_in (which & 1) and / or node_out (which & 2) over a node box [lo, hi).

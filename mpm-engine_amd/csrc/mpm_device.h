@@ -429,6 +429,32 @@ MPM_DEV void stress_update(const m33& Ft, const m33& L, int mat, const mpm_model
     S = mscale(madd(S, mT(S)), 0.5f);
 }
 
+// ------------------------------------------------------------- voxel fields
+// sdf_trilerp(_vec), mpm_solver_warp.py:60-91
+MPM_DEV void trilerp(const float* __restrict__ vals, int nc, int res, v3 f, float* out) {
+    float hi = (float)res - 1.0001f;
+    float fi = fminf(fmaxf(f.x, 0.0f), hi), fj = fminf(fmaxf(f.y, 0.0f), hi), fk = fminf(fmaxf(f.z, 0.0f), hi);
+    int i = (int)floorf(fi), j = (int)floorf(fj), k = (int)floorf(fk);
+    float tx = fi - (float)i, ty = fj - (float)j, tz = fk - (float)k;
+    auto V = [&](int a, int b, int c, int ch) { return vals[(((size_t)a * res + b) * res + c) * nc + ch]; };
+    for (int ch = 0; ch < nc; ++ch) {
+        float c00 = V(i, j, k, ch) * (1.0f - tx) + V(i + 1, j, k, ch) * tx;
+        float c01 = V(i, j, k + 1, ch) * (1.0f - tx) + V(i + 1, j, k + 1, ch) * tx;
+        float c10 = V(i, j + 1, k, ch) * (1.0f - tx) + V(i + 1, j + 1, k, ch) * tx;
+        float c11 = V(i, j + 1, k + 1, ch) * (1.0f - tx) + V(i + 1, j + 1, k + 1, ch) * tx;
+        float c0 = c00 * (1.0f - ty) + c10 * ty;
+        float c1 = c01 * (1.0f - ty) + c11 * ty;
+        out[ch] = c0 * (1.0f - tz) + c1 * tz;
+    }
+}
+
+// v' = v(2w^2 - 1) + 2(q.v) q +/- 2w (q x v): rotation by a unit quaternion (x, y, z, w)
+MPM_DEV v3 quat_rotate(const float* q, v3 v, bool inverse) {
+    v3 qv = mk3(q[0], q[1], q[2]);
+    float w = q[3];
+    return v * (2.0f * w * w - 1.0f) + qv * (2.0f * dot(qv, v)) + cross(qv, v) * ((inverse ? -w : w) * 2.0f);
+}
+
 // ------------------------------------------------------------- layout helpers
 MPM_DEV int node_index(const mpm_grid_t& G, int i, int j, int k) {
     return ((((i >> 2) * G.nb[1] + (j >> 2)) * G.nb[2] + (k >> 2)) << 6) + ((i & 3) << 4) +

@@ -29,31 +29,6 @@ MPM_DEV float glass_sdf(v3 l, const mpm_collider_t& c) {
     return fmaxf(d_outer, -d_cav);
 }
 
-// sdf_trilerp(_vec), mpm_solver_warp.py:60-91
-MPM_DEV void trilerp(const float* __restrict__ vals, int nc, int res, v3 f, float* out) {
-    float hi = (float)res - 1.0001f;
-    float fi = fminf(fmaxf(f.x, 0.0f), hi), fj = fminf(fmaxf(f.y, 0.0f), hi), fk = fminf(fmaxf(f.z, 0.0f), hi);
-    int i = (int)floorf(fi), j = (int)floorf(fj), k = (int)floorf(fk);
-    float tx = fi - (float)i, ty = fj - (float)j, tz = fk - (float)k;
-    auto V = [&](int a, int b, int c, int ch) { return vals[(((size_t)a * res + b) * res + c) * nc + ch]; };
-    for (int ch = 0; ch < nc; ++ch) {
-        float c00 = V(i, j, k, ch) * (1.0f - tx) + V(i + 1, j, k, ch) * tx;
-        float c01 = V(i, j, k + 1, ch) * (1.0f - tx) + V(i + 1, j, k + 1, ch) * tx;
-        float c10 = V(i, j + 1, k, ch) * (1.0f - tx) + V(i + 1, j + 1, k, ch) * tx;
-        float c11 = V(i, j + 1, k + 1, ch) * (1.0f - tx) + V(i + 1, j + 1, k + 1, ch) * tx;
-        float c0 = c00 * (1.0f - ty) + c10 * ty;
-        float c1 = c01 * (1.0f - ty) + c11 * ty;
-        out[ch] = c0 * (1.0f - tz) + c1 * tz;
-    }
-}
-
-// v' = v(2w^2 - 1) + 2(q.v) q +/- 2w (q x v): rotation by a unit quaternion (x, y, z, w)
-MPM_DEV v3 quat_rotate(const float* q, v3 v, bool inverse) {
-    v3 qv = mk3(q[0], q[1], q[2]);
-    float w = q[3];
-    return v * (2.0f * w * w - 1.0f) + qv * (2.0f * dot(qv, v)) + cross(qv, v) * ((inverse ? -w : w) * 2.0f);
-}
-
 // Applies collider c to one node; returns the reaction impulse in imp (zero if untouched).
 MPM_DEV void apply_collider(const mpm_collider_t& c, const mpm_grid_t& G, int gi, int gj, int gk,
                             float m, v3& vo, v3& imp, v3& torque) {
@@ -562,6 +537,8 @@ int32_t mpm_struct_size(int32_t which) {
         case 2: return (int32_t)sizeof(mpm_model_t);
         case 3: return (int32_t)sizeof(mpm_collider_t);
         case 4: return (int32_t)sizeof(mpm_tick_ctl_t);
+        case 5: return (int32_t)sizeof(mpm_cdf_lanes_t);
+        case 6: return (int32_t)sizeof(mpm_cdf_collider_t);
         default: return -1;
     }
 }
@@ -706,6 +683,7 @@ int32_t mpm_permute(const mpm_particles_t* P, const int32_t* perm, float* scratc
     if (P->cov) r |= one(P->cov, 6);
     r |= onei(P->material); r |= onei(P->selection);
     if (P->scene) r |= onei(P->scene);
+    if (P->cdf_tag) r |= onei(P->cdf_tag);
     if (r != MPM_OK) return MPM_ERR_HIP;
     return launch_status();
 }

@@ -16,6 +16,7 @@
 
 #include "mpm_device.h"
 #include "mpm_tile.h"
+#include "mpm_cdf.h"
 
 namespace {
 
@@ -99,8 +100,9 @@ MPM_DEV float sel3(int i, const float* a) { return i == 0 ? a[0] : (i == 1 ? a[1
 // (unsorted or spread-out particles): a rolled loop over the 27 nodes with the weights
 // picked by selects, so this rare path stays small next to the LDS fast path it shares a
 // kernel (and a register budget) with.
+template <class Subst>
 MPM_DEV void gather_global(const mpm_grid_t& G, size_t soff, const Stencil& st, bool periodic, v3& nv, m33& nC,
-                           m33& nF) {
+                           m33& nF, Subst subst) {
     const float4* __restrict__ out = reinterpret_cast<const float4*>(G.node_out) + soff;
     nv = mk3(0.f, 0.f, 0.f);
     nC = mzero();
@@ -113,7 +115,8 @@ MPM_DEV void gather_global(const mpm_grid_t& G, size_t soff, const Stencil& st, 
         if (periodic) ix = ((ix % G.n[0]) + G.n[0]) % G.n[0];
         const int iy = st.base[1] + j, iz = st.base[2] + k;
         if (!in_grid(G, ix, iy, iz)) continue;
-        const float4 g4 = out[node_index(G, ix, iy, iz)];
+        const float4 o4 = out[node_index(G, ix, iy, iz)];
+        const v3 g4 = subst(i, j, k, mk3(o4.x, o4.y, o4.z));
         const float w0 = sel3(i, st.w[0]), w1 = sel3(j, st.w[1]), w2 = sel3(k, st.w[2]);
         const float d0 = sel3(i, st.dw[0]), d1 = sel3(j, st.dw[1]), d2 = sel3(k, st.dw[2]);
         const float w = w0 * w1 * w2;
@@ -132,8 +135,9 @@ MPM_DEV void gather_global(const mpm_grid_t& G, size_t soff, const Stencil& st, 
     }
 }
 
+template <class Mask>
 MPM_DEV void scatter_global(const mpm_grid_t& G, size_t soff, const Stencil& st, bool periodic, float mass, v3 v,
-                            const m33& Cm, const m33& fs) {
+                            const m33& Cm, const m33& fs, Mask mask) {
     float* __restrict__ gin = G.node_in + 4 * soff;
 #pragma unroll 1
     for (int n = 0; n < 27; ++n) {
@@ -147,12 +151,14 @@ MPM_DEV void scatter_global(const mpm_grid_t& G, size_t soff, const Stencil& st,
         const float wm = w0 * w1 * w2 * mass;
         const v3 dpos = mk3(((float)i - st.fx[0]) * G.dx, ((float)j - st.fx[1]) * G.dx, ((float)k - st.fx[2]) * G.dx);
         const v3 dwt = mk3(d0 * w1 * w2, w0 * d1 * w2, w0 * w1 * d2) * G.inv_dx;
-        const v3 add = (v + mvmul(Cm, dpos)) * wm + mvmul(fs, dwt);
+        v3 add = (v + mvmul(Cm, dpos)) * wm + mvmul(fs, dwt);
+        float wmv = wm;
+        if (!mask(i, j, k, add, wmv)) continue;
         const size_t o = 4 * (size_t)node_index(G, ix, iy, iz);
         atomicAdd(gin + o + 0, add.x);
         atomicAdd(gin + o + 1, add.y);
         atomicAdd(gin + o + 2, add.z);
-        atomicAdd(gin + o + 3, wm);
+        atomicAdd(gin + o + 3, wmv);
     }
 }
 
@@ -212,7 +218,7 @@ constexpr int kXcds = 8;
 
 // amdgpu_waves_per_eu(4): keep the allocation within 128 VGPRs (4 wavefronts per SIMD);
 // left alone the scheduler lands between 110 and 200 depending on unrelated edits.
-template <int MODE, bool kSolid, int kT>
+template <int MODE, bool kSolid, int kT, bool kCdf>
 __global__ __launch_bounds__(kT) __attribute__((amdgpu_waves_per_eu(4))) void k_particle_pass(mpm_particles_t P, mpm_grid_t G,
                                                       const mpm_model_t* __restrict__ md, float dt,
                                                       int xcd_q, int ppt) {
@@ -226,6 +232,7 @@ __global__ __launch_bounds__(kT) __attribute__((amdgpu_waves_per_eu(4))) void k_
     __shared__ float s_scatter[kP2G ? 4 : 1][kP2G ? kArena : 1];
     __shared__ int s_red0[kT / 64][8];
     __shared__ int s_red1[kT / 64][8];
+    __shared__ float s_react[kCdf && kP2G ? 6 * MPM_MAX_CDF : 1];  // blocked deposits per lane
 
     const int S = P.stride;
     const int group = (int)(blockIdx.x % kXcds) * xcd_q + (int)(blockIdx.x / kXcds);
@@ -246,9 +253,29 @@ __global__ __launch_bounds__(kT) __attribute__((amdgpu_waves_per_eu(4))) void k_
         v3 v = mk3(0.f, 0.f, 0.f);
         m33 Cm = mzero(), L = mzero(), Ft = mzero();
 
+        int pc_g = 0;  // CPIC tag of the gather half (stored), 0 without CDF colliders
         if constexpr (kG2P) {
             const bool gather = alive && mat != 7 && mat != 8;
             const Stencil st = make_stencil(x, G.inv_dx);
+            v3 ghost = mk3(0.f, 0.f, 0.f);
+            if constexpr (kCdf) {
+                if (gather) {
+                    pc_g = cdf_particle_tag(G, st, P.cdf_tag[p], true);
+                    P.cdf_tag[p] = pc_g;
+                    if (pc_g != 0) ghost = cdf_ghost(G, md, dt, x, ld3(P.v, S, p), pc_g, st);
+                }
+            }
+            // ghost fill (mpm_utils.py:1005-1017): incompatible nodes read the ghost
+            auto subst = [&](int i, int j, int k, v3 g) {
+                if constexpr (kCdf) {
+                    if (pc_g != 0) {
+                        const int gi = st.base[0] + i, gj = st.base[1] + j, gk = st.base[2] + k;
+                        if (in_nodes(G, gi, gj, gk) && cdf_incompat(pc_g, G.cdf.tag_prev[node_index(G, gi, gj, gk)]))
+                            g = ghost;
+                    }
+                }
+                return g;
+            };
             int lo[3], hi[3];
     #pragma unroll
             for (int a = 0; a < 3; ++a) {
@@ -273,11 +300,11 @@ __global__ __launch_bounds__(kT) __attribute__((amdgpu_waves_per_eu(4))) void k_
                     gather_tp(st, G.inv_dx,
                               [&](int i, int j, int k) {
                                   const float4 g = s_gather[b0 + ((o0 + i) * d1 + (o1 + j)) * d2 + (o2 + k)];
-                                  return mk3(g.x, g.y, g.z);
+                                  return subst(i, j, k, mk3(g.x, g.y, g.z));
                               },
                               nv, nC, nF);
                 } else {
-                    gather_global(G, soff, st, periodic, nv, nC, nF);
+                    gather_global(G, soff, st, periodic, nv, nC, nF, subst);
                 }
                 v = nv;
                 Cm = nC;
@@ -371,7 +398,44 @@ __global__ __launch_bounds__(kT) __attribute__((amdgpu_waves_per_eu(4))) void k_
                 s_scatter[2][a] = 0.f;
                 s_scatter[3][a] = 0.f;
             });
+            int pc_s = 0;  // CPIC tag of the scatter half (mpm_utils.py:851-860)
+            if constexpr (kCdf) {
+                if (threadIdx.x < 6 * MPM_MAX_CDF) s_react[threadIdx.x] = 0.f;
+                if (alive) {
+                    const int old = (kG2P && mat != 7 && mat != 8) ? pc_g : P.cdf_tag[p];
+                    pc_s = cdf_particle_tag(G, st, old, false);
+                }
+            }
             __syncthreads();
+            // blocked deposits (mpm_utils.py:894-912): a contribution to an incompatible
+            // node goes to its owner lane's reaction wrench instead of the grid
+            auto mask = [&](int i, int j, int k, v3& add, float& wm) {
+                if constexpr (kCdf) {
+                    if (pc_s != 0) {
+                        const int gi = st.base[0] + i, gj = st.base[1] + j, gk = st.base[2] + k;
+                        if (in_nodes(G, gi, gj, gk)) {
+                            const int nt = G.cdf.tag[node_index(G, gi, gj, gk)];
+                            if (cdf_incompat(pc_s, nt)) {
+                                const int own = (nt >> MPM_CDF_OWNER_SHIFT) & 3;
+                                const float* c = G.cdf.lanes->center[own];
+                                const v3 xw = mk3((float)gi * G.dx, (float)gj * G.dx, (float)gk * G.dx);
+                                const v3 tq = cross(xw - mk3(c[0], c[1], c[2]), add);
+                                float* r = s_react + 6 * own;
+                                atomicAdd(r + 0, add.x);
+                                atomicAdd(r + 1, add.y);
+                                atomicAdd(r + 2, add.z);
+                                atomicAdd(r + 3, tq.x);
+                                atomicAdd(r + 4, tq.y);
+                                atomicAdd(r + 5, tq.z);
+                                add = mk3(0.f, 0.f, 0.f);
+                                wm = 0.f;
+                                return false;
+                            }
+                        }
+                    }
+                }
+                return true;
+            };
             m33 fs = mzero();
             float mass = 0.0f;
             if (alive) {
@@ -405,6 +469,9 @@ __global__ __launch_bounds__(kT) __attribute__((amdgpu_waves_per_eu(4))) void k_
                 const bool tail = rl == 15 || lane == 63 || ((heads >> (lane + 1)) & 1ull);
                 const int d1 = ts.d1, d2 = ts.d2, b0 = ts.base;
                 scatter_rotated(st, G.dx, G.inv_dx, mass, v, Cm, fs, 0, [&](int i, int j, int k, v3 add, float wm) {
+                    if constexpr (kCdf) {
+                        if (alive) mask(i, j, k, add, wm);
+                    }
                     const float s0 = seg_row_sum(add.x, c1, c2, c4, c8);
                     const float s1 = seg_row_sum(add.y, c1, c2, c4, c8);
                     const float s2 = seg_row_sum(add.z, c1, c2, c4, c8);
@@ -422,6 +489,7 @@ __global__ __launch_bounds__(kT) __attribute__((amdgpu_waves_per_eu(4))) void k_
                     const int d1 = ts.d1, d2 = ts.d2, b0 = ts.base;
                     scatter_rotated(st, G.dx, G.inv_dx, mass, v, Cm, fs, (int)(threadIdx.x % 27u),
                                     [&](int i, int j, int k, v3 add, float wm) {
+                                        if (!mask(i, j, k, add, wm)) return;
                                         const int a = b0 + ((o0 + i) * d1 + (o1 + j)) * d2 + (o2 + k);
                                         atomicAdd(&s_scatter[0][a], add.x);
                                         atomicAdd(&s_scatter[1][a], add.y);
@@ -429,7 +497,7 @@ __global__ __launch_bounds__(kT) __attribute__((amdgpu_waves_per_eu(4))) void k_
                                         atomicAdd(&s_scatter[3][a], wm);
                                     });
                 } else {
-                    scatter_global(G, soff, st, periodic, mass, v, Cm, fs);
+                    scatter_global(G, soff, st, periodic, mass, v, Cm, fs, mask);
                 }
             }
             __syncthreads();
@@ -445,6 +513,10 @@ __global__ __launch_bounds__(kT) __attribute__((amdgpu_waves_per_eu(4))) void k_
                     atomicAdd(gin + o + 2, a2);
                     atomicAdd(gin + o + 3, a3);
                 });
+            }
+            if constexpr (kCdf) {
+                if (threadIdx.x < 6 * MPM_MAX_CDF && s_react[threadIdx.x] != 0.f)
+                    atomicAdd(G.cdf.reaction + threadIdx.x, s_react[threadIdx.x]);
             }
         }
         if (alive && !(x.x == x.x && x.y == x.y && x.z == x.z)) err |= 2;
@@ -464,8 +536,12 @@ extern "C" int32_t mpm_particle_pass(const mpm_particles_t* P, const mpm_grid_t*
         G->n[2] <= 0 || G->n_scenes < 1 || (G->n_scenes > 1 && !P->scene) || G->nb[0] != (G->n[0] + 3) / 4 || G->nb[1] != (G->n[1] + 3) / 4 ||
         G->nb[2] != (G->n[2] + 3) / 4)
         return MPM_ERR_ARG;
+    const bool cdf = G->cdf.n_cdf > 0;
+    if (cdf && (G->n_scenes != 1 || !P->cdf_tag || !G->cdf.tag || !G->cdf.tag_prev || !G->cdf.dist ||
+                !G->cdf.dist_prev || !G->cdf.block || !G->cdf.lanes || !G->cdf.reaction || (mode & MPM_PASS_LEAN)))
+        return MPM_ERR_ARG;
     if (P->n == 0) return MPM_OK;
-    const int wg = (mode & MPM_PASS_WG128) ? 128 : 256;
+    const int wg = (!cdf && (mode & MPM_PASS_WG128)) ? 128 : 256;
     const int ppt = (mode & MPM_PASS_PPT4) ? 4 : ((mode & MPM_PASS_PPT2) ? 2 : 1);
     hipStream_t s = (hipStream_t)stream;
     const int nchunks = (P->n + wg - 1) / wg;
@@ -474,28 +550,40 @@ extern "C" int32_t mpm_particle_pass(const mpm_particles_t* P, const mpm_grid_t*
     const dim3 grid((unsigned)(q * kXcds)), block(wg);
     const bool solid = (mode & MPM_PASS_VISCOUS_ONLY) == 0;
     const int m = mode & (MPM_PASS_G2P | MPM_PASS_STRESS | MPM_PASS_P2G | MPM_PASS_LEAN | MPM_PASS_KEEP_STRESS);
-#define LAUNCH(M, T)                                                                                               \
-    if (solid) hipLaunchKernelGGL((k_particle_pass<(M), true, T>), grid, block, 0, s, *P, *G, model, dt, q, ppt); \
-    else hipLaunchKernelGGL((k_particle_pass<(M), false, T>), grid, block, 0, s, *P, *G, model, dt, q, ppt);
-#define CASE(M)            \
-    case (M):              \
-        if (wg == 128) {   \
-            LAUNCH(M, 128) \
-        } else {           \
-            LAUNCH(M, 256) \
-        }                  \
+#define LAUNCH(M, T, C)                                                                                               \
+    if (solid) hipLaunchKernelGGL((k_particle_pass<(M), true, T, C>), grid, block, 0, s, *P, *G, model, dt, q, ppt); \
+    else hipLaunchKernelGGL((k_particle_pass<(M), false, T, C>), grid, block, 0, s, *P, *G, model, dt, q, ppt);
+#define CASE(M)                   \
+    case (M):                     \
+        if (wg == 128) {          \
+            LAUNCH(M, 128, false) \
+        } else {                  \
+            LAUNCH(M, 256, false) \
+        }                         \
+        break;
+// CPIC colliders: non-lean modes, 256-thread workgroups
+#define CASE_CDF(M)               \
+    case (M):                     \
+        if (cdf) {                \
+            LAUNCH(M, 256, true)  \
+        } else if (wg == 128) {   \
+            LAUNCH(M, 128, false) \
+        } else {                  \
+            LAUNCH(M, 256, false) \
+        }                         \
         break;
     switch (m) {
-        CASE(MPM_PASS_G2P)
-        CASE(MPM_PASS_STRESS)
-        CASE(MPM_PASS_P2G)
-        CASE(MPM_PASS_STRESS | MPM_PASS_P2G)
-        CASE(MPM_PASS_G2P | MPM_PASS_STRESS | MPM_PASS_P2G)
+        CASE_CDF(MPM_PASS_G2P)
+        CASE_CDF(MPM_PASS_STRESS)
+        CASE_CDF(MPM_PASS_P2G)
+        CASE_CDF(MPM_PASS_STRESS | MPM_PASS_P2G)
+        CASE_CDF(MPM_PASS_G2P | MPM_PASS_STRESS | MPM_PASS_P2G)
         CASE(MPM_PASS_G2P | MPM_PASS_STRESS | MPM_PASS_P2G | MPM_PASS_LEAN)
         CASE(MPM_PASS_G2P | MPM_PASS_STRESS | MPM_PASS_P2G | MPM_PASS_LEAN | MPM_PASS_KEEP_STRESS)
         default: return MPM_ERR_ARG;
     }
 #undef LAUNCH
 #undef CASE
+#undef CASE_CDF
     return launch_status();
 }

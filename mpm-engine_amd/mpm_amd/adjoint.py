@@ -58,6 +58,8 @@ class Rollout:
             raise NotImplementedError("adjoint kernels cover plane, box, wall and point-cloud colliders")
         if sim.model[0]["periodic_x"]:
             raise NotImplementedError("periodic domains have no adjoint")
+        if sim.cdf is not None and sim.cdf.n > 0:
+            raise NotImplementedError("CPIC (CDF) colliders have no adjoint")
         if len(sim.colliders.items) > 16:
             raise ValueError("at most 16 colliders in a differentiable rollout")
         self.solver = solver

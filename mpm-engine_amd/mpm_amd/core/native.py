@@ -34,7 +34,7 @@ class Particles(C.Structure):
                 ("x", _FP), ("v", _FP), ("F", _FP), ("F_trial", _FP), ("C", _FP),
                 ("stress", _FP), ("L", _FP), ("vol", _FP), ("mass", _FP), ("mu", _FP),
                 ("lam", _FP), ("yield_stress", _FP), ("Jp", _FP), ("material", _FP),
-                ("selection", _FP), ("cov", _FP), ("scene", _FP)]
+                ("selection", _FP), ("cov", _FP), ("scene", _FP), ("cdf_tag", _FP)]
 
 
 class Adjoint(C.Structure):
@@ -42,10 +42,35 @@ class Adjoint(C.Structure):
                                      "yield_stress", "grid_v", "grid_mv")]
 
 
+MAX_CDF = 4
+CDF_OWNER_SHIFT = 8
+
+
+class Cdf(C.Structure):
+    _fields_ = [("n_cdf", C.c_int32), ("mask_off", C.c_int32), ("tag", _FP), ("tag_prev", _FP),
+                ("dist", _FP), ("dist_prev", _FP), ("block", _FP), ("lanes", _FP),
+                ("reaction", _FP)]
+
+
 class Grid(C.Structure):
     _fields_ = [("node_in", _FP), ("node_out", _FP), ("n", C.c_int32 * 3),
                 ("nb", C.c_int32 * 3), ("dx", C.c_float), ("inv_dx", C.c_float),
-                ("error_flag", _FP), ("n_scenes", C.c_int32), ("pad_", C.c_int32)]
+                ("error_flag", _FP), ("n_scenes", C.c_int32), ("pad_", C.c_int32),
+                ("cdf", Cdf)]
+
+
+_L3 = (np.float32, (MAX_CDF, 3))
+CDF_LANES_DTYPE = np.dtype([
+    ("center",) + _L3, ("velocity",) + _L3, ("omega",) + _L3,
+    ("center_prev",) + _L3, ("velocity_prev",) + _L3, ("omega_prev",) + _L3,
+    ("friction", np.float32, (MAX_CDF,)), ("type", np.int32, (MAX_CDF,))])
+
+
+class CdfCollider(C.Structure):
+    _fields_ = [("val", _FP), ("valid", _FP), ("res", C.c_int32), ("lane", C.c_int32),
+                ("origin", C.c_float * 3), ("cell", C.c_float), ("center", C.c_float * 3),
+                ("quat", C.c_float * 4), ("band", C.c_float), ("time", C.c_float),
+                ("start_time", C.c_float), ("end_time", C.c_float)]
 
 
 _M16 = (np.float32, (MAX_MATERIALS,))
@@ -101,6 +126,11 @@ SIGNATURES = {
     "mpm_export_R": (_I32, [_P, _P, _P]),
     "mpm_export_cov": (_I32, [_P, _P, _P, _P]),
     "mpm_grid_to_dense": (_I32, [_P, _P, _P, _P, _P]),
+    "mpm_cdf_zero_tags": (_I32, [_P, _P, _P, _P]),
+    "mpm_cdf_copy_prev": (_I32, [_P, _P, _P, _P]),
+    "mpm_cdf_stamp": (_I32, [_P, _P, _P, _P, _P]),
+    "mpm_cdf_block_mask": (_I32, [_P, _P, _P, _P]),
+    "mpm_cdf_set_lanes": (_I32, [_P, _P, _P]),
     "mpm_adjoint_g2p": (_I32, [_P, _P, _P, _P, _F, _P]),
     "mpm_adjoint_grid": (_I32, [_P, _P, _P, _I32, _P, _I32, _P, _F, _P]),
     "mpm_adjoint_p2g_stress": (_I32, [_P, _P, _P, _P, _F, _P]),
@@ -128,7 +158,8 @@ def load():
         fn.restype = res
         fn.argtypes = args
     sizes = {0: C.sizeof(Particles), 1: C.sizeof(Grid), 2: MODEL_DTYPE.itemsize,
-             3: COLLIDER_DTYPE.itemsize, 4: TICK_CTL_DTYPE.itemsize}
+             3: COLLIDER_DTYPE.itemsize, 4: TICK_CTL_DTYPE.itemsize,
+             5: CDF_LANES_DTYPE.itemsize, 6: C.sizeof(CdfCollider)}
     for which, size in sizes.items():
         got = lib.mpm_struct_size(which)
         if got != size:

@@ -177,6 +177,8 @@ class HipSimulator:
         self._graph = None
         self._graph_key = None
         self._ticks = 0
+        self.cdf = None      # CdfColliders, created by the first CPIC collider
+        self.cdf_tag = None  # per-particle CPIC tag (allocated with it)
         self._structs()
 
     # --------------------------------------------------------------- ABI structs
@@ -188,6 +190,7 @@ class HipSimulator:
             setattr(P, nm, native.ptr(getattr(self, nm)))
         P.cov = native.ptr(self.cov)
         P.scene = native.ptr(self.scene)
+        P.cdf_tag = native.ptr(self.cdf_tag)
         G = native.Grid()
         G.node_in, G.node_out = native.ptr(self.node_in), native.ptr(self.node_out)
         G.n[:] = [self.n_grid] * 3
@@ -195,6 +198,8 @@ class HipSimulator:
         G.dx, G.inv_dx = self.dx, self.inv_dx
         G.error_flag = native.ptr(self.error_flag)
         G.n_scenes = self.n_scenes
+        if self.cdf is not None:
+            self.cdf.fill(G)
         self.P, self.G = P, G
         self.pP, self.pG = C.byref(P), C.byref(G)
 
@@ -415,6 +420,16 @@ class HipSimulator:
             self._sched_host = torch.zeros(max(nbytes, 256), dtype=torch.uint8).pin_memory()
             self._graph = None
 
+    def cdf_colliders(self):
+        """The CPIC collider set, created (with the per-particle tags) on first use."""
+        if self.cdf is None:
+            from .cdf import CdfColliders
+            self.cdf = CdfColliders(self)
+            self.cdf_tag = self.torch.zeros(self.n_particles, dtype=self.torch.int32, device=self.device)
+            self._structs()
+            self._graph = None
+        return self.cdf
+
     def run_tick(self, dt: float, substeps: int, fused: bool, sparse: bool):
         torch = self.torch
         S = int(substeps)
@@ -468,12 +483,21 @@ class HipSimulator:
                          "blocks_from_box")
             glist, gcount = self.box_list, self.ctl[6:]
         lists = (native.ptr(glist), native.ptr(gcount))
-        if self.profile:
+        cdf = self.cdf if (self.cdf is not None and self.cdf.n > 0) else None
+        if cdf is not None:
+            self.G.cdf.n_cdf = cdf.n
+            self.G.cdf.mask_off = int(cdf.mask_off)
+        if cdf is not None and not cdf.quiet(ColliderSet.times(self.time, dt, S)[0]):
+            # a CPIC surface moves, restamps or opens / closes this tick: the stamps and
+            # lane uploads interleave with the substeps, so the tick runs eagerly
+            self._enqueue_tick_cdf(dt, S, K, fused, lists, s)
+        elif self.profile:
             self._enqueue_tick_profiled(dt, S, K, fused, lists)
         elif self.use_graph and self._ticks > 0:
             key = (S, float(dt), bool(fused), K, lists, native.ptr(self.sched_dev),
                    self.update_cov_with_F, n_slots, native.ptr(self.grid_ws), native.ptr(self.wrench),
-                   frozenset(self._mat_ids) <= native.VISCOUS_IDS, self.launch_bits)
+                   frozenset(self._mat_ids) <= native.VISCOUS_IDS, self.launch_bits,
+                   None if cdf is None else (cdf.n, int(cdf.mask_off), native.ptr(self.cdf_tag)))
             if self._graph is None or self._graph_key != key:
                 self._capture(dt, S, K, fused, lists)
                 self._graph_key = key
@@ -514,14 +538,35 @@ class HipSimulator:
             self.n_blocks, sched if K else None, K, native.ptr(self.wrench), slots,
             native.ptr(self.grid_ws) if slots else None, dt, native.GRID_ZERO_IN, s), "grid_update")
 
+    def _enqueue_tick_cdf(self, dt, S, K, fused, lists, s):
+        """One tick with the CPIC per-substep host work between the launches (the
+        reference's p2g2p / p2g2p_fused_tick with _stamp_cdf and _integrate_cdf_poses,
+        mpm_solver_warp.py:1130-1131, 1334-1336, 1410-1413, 1536-1539)."""
+        G2P, ST, P2G = native.PASS_G2P, native.PASS_STRESS, native.PASS_P2G
+        cdf = self.cdf
+        times, _ = ColliderSet.times(self.time, dt, S)
+        for k in range(S):
+            t = float(times[k])
+            cdf.stamp(t, lag_prev=fused)
+            self._pass(G2P | ST | P2G if (fused and k > 0) else ST | P2G, dt, s)
+            self._grid(dt, k, K, lists, s)
+            cdf.integrate(t, dt, lag_prev=fused)
+            if not fused:
+                self._pass(G2P, dt, s)
+        if fused:
+            cdf.reconcile_prev()
+            self._pass(G2P, dt, s)
+
     def _enqueue_tick(self, dt, S, K, fused, lists, s):
         G2P, ST, P2G = native.PASS_G2P, native.PASS_STRESS, native.PASS_P2G
+        # the CPIC ghost reads the stored particle velocity: no lean passes with CDF
+        lean = native.PASS_LEAN if (self.cdf is None or self.cdf.n == 0) else 0
         if fused:
             self._pass(ST | P2G, dt, s)
             self._grid(dt, 0, K, lists, s)
             for k in range(1, S):
-                mode = G2P | ST | P2G | native.PASS_LEAN
-                if k == S - 1:
+                mode = G2P | ST | P2G | lean
+                if k == S - 1 and lean:
                     mode |= native.PASS_KEEP_STRESS
                 self._pass(mode, dt, s)
                 self._grid(dt, k, K, lists, s)
@@ -884,9 +929,37 @@ class Solver:
         self._cs().add(c)
         return self
 
-    def add_cdf_collider(self, *args, **kwargs) -> int:
-        raise NotImplementedError("CPIC thin-boundary (CDF) colliders are not implemented on the "
-                                  "HIP substep yet")
+    def add_cdf_collider(self, cdf, center, quat=(0.0, 0.0, 0.0, 1.0), velocity=(0.0, 0.0, 0.0),
+                         omega=(0.0, 0.0, 0.0), band=None, surface: str = "separable",
+                         friction: float = 0.4, start_time: float = 0.0, end_time: float = 1.0e9) -> int:
+        """CPIC thin-boundary collider from an open oriented surface (geometry.CDFData),
+        solver.py:363-386: particle-node transfers are severed across the surface, so it
+        holds material at zero wall thickness. Default band = min(built band, 2 dx).
+        Handles are their own space (0..3)."""
+        if self.periodic_x:
+            raise NotImplementedError("CDF colliders do not wrap the periodic x axis")
+        sim = self._sim
+        return sim.cdf_colliders().add(
+            cdf.values, cdf.valid, cdf.origin, cdf.cell, cdf.band, center, quat, velocity, omega,
+            band, surface, friction, start_time, end_time, sim.time)
+
+    def set_cdf_pose(self, handle: int, center=None, quat=None, velocity=None, omega=None) -> "Solver":
+        """Start-of-tick pose of a CDF collider (solver.py:388-394); the pose advances
+        with velocity / omega every substep."""
+        self._sim.cdf.set_pose(handle, center=center, quat=quat, velocity=velocity, omega=omega)
+        return self
+
+    def reset_cdf_wrench(self) -> "Solver":
+        """Zero every CDF lane's reaction accumulators (solver.py:396-399)."""
+        if self._sim.cdf is not None:
+            self._sim.cdf.reset_wrench()
+        return self
+
+    def cdf_wrench(self, handle: int, dt: float) -> dict:
+        """Reaction the material exerts on a CDF collider since the last reset, from the
+        blocked scatter deposits, divided by dt (solver.py:401-413)."""
+        f, t = self._sim.cdf.wrench(handle)
+        return {"force": f / dt, "torque": t / dt}
 
     # ------------------------------------------------------------------ stepping
     def step(self, dt: float, substeps: int = 1) -> "Solver":
