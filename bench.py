@@ -11,10 +11,11 @@ workload. Two more legs ride in the same line:
   "batched"  256 dough-press scenes (64^3 each, ~8.2M particles) in one launch per phase,
              100-substep hipGraph chunks, autotuned launch shape and sort interval, against
              the same scenes solved one after another with the plain Solver;
+  "cdf_pour" the same pour with both glasses as CPIC thin-boundary sheets;
   "adjoint"  the squeeze identification (50k von Mises particles, 400 substeps): gradient
              of a loss on final positions and plate forces, backward time / forward time.
 
-    python bench.py [--gpus 1] [--steps 10] [--warmup 2] [--no-cpu] [--legs pour,adjoint,batch]
+    python bench.py [--gpus 1] [--steps 10] [--warmup 2] [--no-cpu] [--legs pour,adjoint,batch,cdf]
 """
 from __future__ import annotations
 
@@ -56,8 +57,9 @@ def parse():
     ap.add_argument("--seq-scenes", type=int, default=0,
                     help="scenes timed one by one for the sequential baseline (0 = all)")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint leg")
-    ap.add_argument("--legs", default="pour,adjoint,batch",
-                    help="comma list of legs to run (pour, adjoint, batch); for profiling one leg")
+    ap.add_argument("--legs", default="pour,adjoint,batch,cdf",
+                    help="comma list of legs to run (pour, adjoint, batch, cdf); for profiling one leg")
+    ap.add_argument("--cdf-frames", type=int, default=2, help="timed frames of the CDF-glass pour")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a device copy of 256 MiB (the byte-counter calibration for --pmc)")
     return ap.parse_args()
@@ -280,6 +282,90 @@ class FenceFreeEvent:
             self._hip.hipEventDestroy(self.h)
 
 
+def held_frame_pass_times(s, sc):
+    """Durations (ms) of every fused particle pass of one more frame of the workload.
+
+    The frame's guard and sort run first (they read back to the host), then the stream
+    is held by a GPU spin while the host enqueues the whole tick eagerly with a
+    fence-free HIP event pair around every fused pass on its stream, so the events time
+    the kernels back to back with no host launch gaps and no cache flushes of their own."""
+    import torch
+
+    from mpm_amd.core import native
+
+    sim = s._sim
+    a = torch.cuda.Event(enable_timing=True)
+    b = torch.cuda.Event(enable_timing=True)
+    a.record()
+    torch.cuda._sleep(1_000_000)
+    b.record()
+    torch.cuda.synchronize()
+    spin_ms = a.elapsed_time(b)
+    sc.command(s)
+    s._update_grid_box(sc.dt, sc.substeps)
+    s._sort_particles()
+    saved = (s.guard_interval, s.sort_interval, sim.use_graph)
+    s.guard_interval, s.sort_interval, sim.use_graph = 10 ** 9, 0, False
+    events = []
+    orig_pass = sim._pass
+    fused_mode = native.PASS_G2P | native.PASS_STRESS | native.PASS_P2G
+
+    def timed_pass(mode, dt, stream):
+        if (mode & fused_mode) == fused_mode:
+            e0, e1 = FenceFreeEvent(), FenceFreeEvent()
+            e0.record(stream)
+            orig_pass(mode, dt, stream)
+            e1.record(stream)
+            events.append((e0, e1))
+        else:
+            orig_pass(mode, dt, stream)
+
+    sim._pass = timed_pass
+    if spin_ms > 0:
+        torch.cuda._sleep(int(1_000_000 * 150.0 / spin_ms))
+    s.step(sc.dt, sc.substeps)
+    torch.cuda.synchronize()
+    sim._pass = orig_pass
+    s.guard_interval, s.sort_interval, sim.use_graph = saved
+    sc.advance()
+    return [e0.elapsed_ms(e1) for e0, e1 in events]
+
+
+def cdf_leg(args):
+    """The pour with both glasses as CPIC (CDF) sheets along their cavities (the
+    reference's --glass cdf pour): frames timed end to end (the moving source restamps
+    every substep, so these ticks run eagerly), and the fused particle pass, which now
+    carries the tag vote, ghost fill and blocked deposits, timed like the headline's."""
+    import numpy as np
+    import torch
+
+    from mpm_amd import Solver
+    from mpm_amd.scenes import PourScene
+
+    sc = PourScene(glass="cdf")
+    s = sc.build(Solver(grid=sc.grid, device="cuda:0", sort_interval=args.sort_interval,
+                        sort_key=args.sort_key))
+    n = s.n_particles
+    frames = max(1, args.cdf_frames)
+    for _ in range(args.warmup):
+        sc.frame(s)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(frames):
+        sc.frame(s)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    if not np.isfinite(s.x()).all():
+        raise RuntimeError("non-finite particle state after the timed CDF frames")
+    durations = held_frame_pass_times(s, sc)
+    return {"value": round(n * frames * sc.substeps / el, 1), "unit": "particle-substeps/s",
+            "ms_per_frame": round(el / frames * 1e3, 2), "frames": frames, "particles": n,
+            "fused_pass_avg_us": round(float(np.mean(durations)) * 1e3, 2),
+            "stamps": s._sim.cdf.stamp_count,
+            "reference": "GH200 CDF pour: fused pass 2.93 ms/substep, ~2 s/frame "
+                         "(docs/performance.md:99-105)"}
+
+
 def pour_leg(args):
     """The headline: fused ticks of the 192^3 pour, frames timed end to end; the fused
     particle pass timed with HIP events on its stream."""
@@ -315,48 +401,7 @@ def pour_leg(args):
     sim._pass(native.PASS_STRESS | native.PASS_P2G, sc.dt, sim.stream())
     active = int((sim.node_in.view(-1, 4)[:, 3] > 0).sum().item())
     sim.node_in.zero_()
-    # Dominant-kernel duration, live over one more frame of the workload: its guard and
-    # sort run first (they read back to the host), then the stream is held by a GPU spin
-    # while the host enqueues the whole tick eagerly with a fence-free HIP event pair
-    # around every fused pass on its stream, so the events time the kernels back to back
-    # with no host launch gaps and no cache flushes of their own.
-    hold_ms = 0.0
-    a = torch.cuda.Event(enable_timing=True)
-    b = torch.cuda.Event(enable_timing=True)
-    a.record()
-    torch.cuda._sleep(1_000_000)
-    b.record()
-    torch.cuda.synchronize()
-    spin_ms = a.elapsed_time(b)
-    sc.command(s)
-    s._update_grid_box(sc.dt, sc.substeps)
-    s._sort_particles()
-    saved = (s.guard_interval, s.sort_interval, sim.use_graph)
-    s.guard_interval, s.sort_interval, sim.use_graph = 10 ** 9, 0, False
-    events = []
-    orig_pass = sim._pass
-    fused_mode = native.PASS_G2P | native.PASS_STRESS | native.PASS_P2G
-
-    def timed_pass(mode, dt, stream):
-        if (mode & fused_mode) == fused_mode:
-            e0, e1 = FenceFreeEvent(), FenceFreeEvent()
-            e0.record(stream)
-            orig_pass(mode, dt, stream)
-            e1.record(stream)
-            events.append((e0, e1))
-        else:
-            orig_pass(mode, dt, stream)
-
-    sim._pass = timed_pass
-    if spin_ms > 0:
-        hold_ms = 150.0
-        torch.cuda._sleep(int(1_000_000 * hold_ms / spin_ms))
-    s.step(sc.dt, sc.substeps)
-    torch.cuda.synchronize()
-    sim._pass = orig_pass
-    s.guard_interval, s.sort_interval, sim.use_graph = saved
-    sc.advance()
-    durations = [e0.elapsed_ms(e1) for e0, e1 in events]
+    durations = held_frame_pass_times(s, sc)
     avg_s = float(np.mean(durations)) / 1e3
     algo = PASS_B_PER_PARTICLE * n + PASS_B_PER_NODE * active
     ach = algo / avg_s / 1e9
@@ -400,6 +445,13 @@ def main():
             batch = batch_leg(args)
         except Exception as e:  # reported in the line; the headline value stands
             batch = {"error": f"{type(e).__name__}: {e}"}
+        torch.cuda.empty_cache()
+    cdf = None
+    if "cdf" in legs:
+        try:
+            cdf = cdf_leg(args)
+        except Exception as e:  # reported in the line; the headline value stands
+            cdf = {"error": f"{type(e).__name__}: {e}"}
     out = {"metric": "particle-substeps/s at 340k pour", "value": pour.get("value"),
            "unit": "particle-substeps/s", "n_gpus": 1, "steps": args.steps,
            "warmup": args.warmup, "ms_per_step": pour.get("ms_per_step"),
@@ -410,7 +462,8 @@ def main():
                       "substeps_per_step": pour.get("substeps_per_step"), "dt": pour.get("dt"),
                       "grid": 192, "graph": not args.no_graph, "sort_interval": args.sort_interval,
                       "sort_key": args.sort_key},
-           "roofline": pour.get("roofline"), "cpu_baseline": cpu, "batched": batch, "adjoint": adjoint}
+           "roofline": pour.get("roofline"), "cpu_baseline": cpu, "batched": batch, "adjoint": adjoint,
+           "cdf_pour": cdf}
     print(json.dumps(out))
 
 

@@ -108,6 +108,27 @@ class GlassProfile:
         return r
 
 
+def glass_cavity_profile(profile: GlassProfile | None = None) -> np.ndarray:
+    """Open (r, z) polyline of the glass cavity (floor disk, then the inner wall up to the
+    rim), the CPIC contact sheet of the glass once revolved (colliders/glass.py:163-180 of
+    the reference): tracing the cavity rather than the wall's mid-surface keeps CDF and
+    SDF contact at the same place. The floor fillet is not represented."""
+    p = GlassProfile() if profile is None else profile
+    return np.array([(0.0, p.inner_floor_z), (p.inner_radius, p.inner_floor_z), (p.inner_radius, p.rim_z)])
+
+
+def glass_cdf(profile: GlassProfile, dx: float, res: int = 96, n_theta: int = 96):
+    """CDFData of the glass cavity sheet, built band wide enough for a 2 dx runtime band
+    (examples/pour_franka.py:249-263 of the reference)."""
+    from ..geometry import build_surface_cdf, revolve_profile_open
+
+    verts, faces = revolve_profile_open(glass_cavity_profile(profile), n_theta=n_theta)
+    span = float((verts.max(0) - verts.min(0)).max())
+    cell = span / (res - 1 - 8)
+    band_cells = max(3.0, float(np.ceil(2.2 * dx / cell)))
+    return build_surface_cdf(verts, faces, res=res, band_cells=band_cells)
+
+
 def _capped_cylinder(r, z, radius, z0, z1):
     zc, hh = 0.5 * (z0 + z1), 0.5 * (z1 - z0)
     dr = r - radius

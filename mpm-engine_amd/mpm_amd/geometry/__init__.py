@@ -72,15 +72,25 @@ class CDFData:
 
 
 def _triangle_samples(verts, faces, spacing):
-    """Points on every triangle on a barycentric lattice no coarser than `spacing`,
-    with the index of the face each point lies on."""
+    """Points on every triangle no farther apart than about `spacing`: a lattice along the
+    shortest edge and the other edge leaving one of its ends (steps sized per edge, so
+    long thin triangles stay cheap) plus the third edge, with the index of the face each
+    point lies on."""
     pts, owner = [], []
-    for fi, (a, b, c) in enumerate(verts[faces]):
-        longest = max(np.linalg.norm(b - a), np.linalg.norm(c - b), np.linalg.norm(a - c))
-        m = max(1, int(np.ceil(longest / spacing)))
-        i, j = np.meshgrid(np.arange(m + 1), np.arange(m + 1), indexing="ij")
-        keep = (i + j) <= m
-        u, v = i[keep] / m, j[keep] / m
+    for fi, tri in enumerate(verts[faces]):
+        e = [np.linalg.norm(tri[(k + 1) % 3] - tri[k]) for k in range(3)]
+        k = int(np.argmin(e))  # shortest edge tri[k] -> tri[k+1]
+        a, b, c = tri[k], tri[(k + 1) % 3], tri[(k + 2) % 3]
+        mu = max(1, int(np.ceil(np.linalg.norm(b - a) / spacing)))
+        mv = max(1, int(np.ceil(np.linalg.norm(c - a) / spacing)))
+        i, j = np.meshgrid(np.arange(mu + 1), np.arange(mv + 1), indexing="ij")
+        u, v = (i / mu).ravel(), (j / mv).ravel()
+        keep = u + v <= 1.0 + 1e-12
+        u, v = u[keep], v[keep]
+        mw = max(1, int(np.ceil(np.linalg.norm(c - b) / spacing)))
+        t = np.arange(mw + 1) / mw
+        u = np.concatenate([u, 1.0 - t])
+        v = np.concatenate([v, t])
         pts.append(a[None] * (1.0 - u - v)[:, None] + b[None] * u[:, None] + c[None] * v[:, None])
         owner.append(np.full(len(u), fi, dtype=np.int64))
     return np.concatenate(pts), np.concatenate(owner)
@@ -132,10 +142,10 @@ def build_surface_cdf(verts, faces, res: int = 64, band_cells: float = 3.0,
     # distances past `reach` only ever read as invalid, so the nearest-sample queries
     # stop there (far voxels get +reach)
     reach = 2.0 * band + 2.0 * cell
-    dist, idx = cKDTree(samples).query(grid, distance_upper_bound=reach)
+    dist, idx = cKDTree(samples).query(grid, distance_upper_bound=reach, workers=-1)
     near = np.isfinite(dist)
     rim = _edge_samples(verts, open_edges(faces), 0.4 * cell)
-    rim_dist = (cKDTree(rim).query(grid, distance_upper_bound=reach)[0] if len(rim)
+    rim_dist = (cKDTree(rim).query(grid, distance_upper_bound=reach, workers=-1)[0] if len(rim)
                 else np.full(len(grid), np.inf))
     side = np.ones(len(grid))
     j = idx[near]

@@ -50,12 +50,18 @@ class PourScene:
     honey (eta 10 Pa s, bulk 1e4 Pa, rho 1400) fills a cylinder of radius 0.08 at 8
     particles per cell inside a source glass whose wall is 2 cells thick; a static target
     glass stands beside it. The source tilts at omega_tilt about a pivot on its rim. One
-    frame is 1/60 s in `substeps` substeps (dt = 1 / (60 * substeps))."""
+    frame is 1/60 s in `substeps` substeps (dt = 1 / (60 * substeps)).
+
+    glass="cdf" replaces the two revolved-SDF glasses by CPIC sheets along their cavities
+    (the reference's --glass cdf pour, examples/pour_franka.py:249-290)."""
 
     def __init__(self, n_grid: int = 192, height: float = 0.30, radius: float = 0.08,
-                 substeps: int = 432, omega_tilt: float = 1.0, seed: int = 0):
+                 substeps: int = 432, omega_tilt: float = 1.0, seed: int = 0, glass: str = "cup"):
         from .colliders.glass import GlassProfile
 
+        if glass not in ("cup", "cdf"):
+            raise ValueError("glass must be 'cup' or 'cdf'")
+        self.glass = glass
         self.grid = GridConfig(n_grid=n_grid, grid_lim=1.0)
         dx = self.grid.dx
         self.substeps = substeps
@@ -82,15 +88,27 @@ class PourScene:
         solver.set_material(self.material())
         solver.add_plane((0, 0, 3 * self.grid.dx), (0, 0, 1), "separate", friction=0.3)
         solver.add_domain_walls()
-        self.src = solver.add_cup(self.profile, self.source_center)
-        self.dst = solver.add_cup(self.profile, self.target_center)
+        if self.glass == "cdf":
+            from .colliders.glass import glass_cdf
+
+            cdf = glass_cdf(self.profile, self.grid.dx)
+            self.src = solver.add_cdf_collider(cdf, self.source_center, friction=0.05)
+            self.dst = solver.add_cdf_collider(cdf, self.target_center, friction=0.05)
+        else:
+            self.src = solver.add_cup(self.profile, self.source_center)
+            self.dst = solver.add_cup(self.profile, self.target_center)
         return solver
 
     def command(self, solver):
         """Command the source glass for the coming frame: its start-of-frame pose and the
         rigid velocity of the tilt about the pivot."""
         vel = np.cross(self.omega, self.center - self.pivot)
-        solver.set_cup(self.src, center=self.center, quat=self.quat, velocity=vel, omega=self.omega)
+        if self.glass == "cdf":
+            w, x, y, z = (float(q) for q in self.quat)
+            solver.set_cdf_pose(self.src, center=self.center, quat=(x, y, z, w), velocity=vel,
+                                omega=self.omega)
+        else:
+            solver.set_cup(self.src, center=self.center, quat=self.quat, velocity=vel, omega=self.omega)
 
     def advance(self):
         """Exact rigid motion of the commanded pose over one frame."""
