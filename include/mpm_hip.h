@@ -57,9 +57,16 @@ typedef enum mpm_status {
 #define MPM_PASS_WG128 64       /* 128 threads per workgroup instead of 256 */
 #define MPM_PASS_PPT2 128       /* each thread takes 2 particles in turn */
 #define MPM_PASS_PPT4 256       /* each thread takes 4 particles in turn */
+/* deterministic scatter: every contribution is rounded to 64-bit fixed point (node_fix,
+   scales fix_scale) and accumulated with integer atomics, so the grid sums do not depend
+   on the order atomics land in: runs are bit-reproducible, and identical across launch
+   shapes and particle orders. Needs G->node_fix; no CDF colliders. */
+#define MPM_PASS_DETERMINISTIC 512
 
 /* grid-update flag bits (mpm_grid_update) */
 #define MPM_GRID_ZERO_IN 1      /* clear node_in after it is consumed */
+#define MPM_GRID_FIXED_IN 2     /* take (momentum, mass) from node_fix (deterministic scatter)
+                                   instead of node_in, and clear node_fix after use */
 
 /* collider kinds (the reference's grid_postprocess plugins) */
 #define MPM_COLLIDER_PLANE 1      /* mpm_solver_warp.py:1929-1992 */
@@ -134,12 +141,17 @@ typedef struct mpm_grid {
     int32_t *error_flag;  /* device int, OR-ed bits: 1 = a stencil left the grid,
                              2 = a non-finite particle position, 4 = a solid material
                              met a MPM_PASS_VISCOUS_ONLY pass, 8 = a scene id outside
-                             [0, n_scenes) (that particle is skipped) */
+                             [0, n_scenes) (that particle is skipped), 64 = a fixed-point
+                             contribution beyond 2^52 units (deterministic scatter: the
+                             scale is too fine for this state) */
     int32_t n_scenes;     /* independent scenes sharing the buffers (>= 1): scene s owns
                              blocks [s * B, (s + 1) * B), B = nb[0] * nb[1] * nb[2];
                              block codes in every block list are these global codes */
     int32_t pad_;
     mpm_cdf_t cdf;        /* CPIC colliders (cdf.n_cdf == 0: none) */
+    int64_t *node_fix;    /* deterministic scatter: 4 int64 per node (momentum xyz, mass) in
+                             fixed point, zero between uses; NULL otherwise */
+    float fix_scale[2];   /* units per kg (mass) and per kg m/s (momentum), powers of two */
 } mpm_grid_t;
 
 /* Model parameters (MPMModelStruct, warp_utils.py:23-84). Lives in device memory; the
@@ -258,7 +270,10 @@ int32_t mpm_blocks_from_box(const mpm_grid_t *G, mpm_tick_ctl_t *ctl, int32_t *b
  * wrench[6k .. 6k+5]. Up to 64 slots this goes through `workspace` (device floats, at
  * least mpm_grid_workspace_floats(capacity, wrench_slots)) with a fixed-order reduction,
  * so the readout is bit-reproducible; with more slots (batched scenes) per-wavefront sums
- * are added with float atomics, workspace_floats is 0 and `workspace` may be NULL.
+ * are added with float atomics and `workspace` may be NULL. With MPM_GRID_FIXED_IN the
+ * per-wavefront sums go to `workspace` as 64-bit fixed point (integer atomics, zero
+ * between calls, order-independent) and are added to `wrench` after the sweep, for any
+ * slot count.
  * A block of scene s applies colliders[s * ncoll .. s * ncoll + ncoll). Nodes without mass above 1e-15 read velocity 0 (the
  * split-zero contract, mpm_utils.py:1193-1202). With MPM_GRID_ZERO_IN, node_in is
  * cleared afterwards. */
@@ -273,7 +288,9 @@ int32_t mpm_grid_update(const mpm_grid_t *G, const mpm_model_t *model, const mpm
 /* raw must be zero on entry; marks the stencil-base block of every particle. */
 int32_t mpm_blocks_mark(const mpm_particles_t *P, const mpm_grid_t *G, int32_t *raw, void *stream);
 int32_t mpm_blocks_dilate(const mpm_grid_t *G, const int32_t *raw, int32_t *cur, void *stream);
-/* counts (device, 2 ints) must be zero on entry; cur_list / union_list get block codes. */
+/* counts (device, 2 ints) must be zero on entry; cur_list / union_list get block codes in
+ * ascending order (an order-preserving compaction, so the block lists, and the fixed-order
+ * wrench reduction that follows them, are the same from run to run). */
 int32_t mpm_blocks_compact(const mpm_grid_t *G, const int32_t *cur, const int32_t *prev,
                            int32_t *cur_list, int32_t *union_list, int32_t *counts, void *stream);
 

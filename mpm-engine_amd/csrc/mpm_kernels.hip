@@ -162,6 +162,25 @@ MPM_DEV float wave_sum(float v) {
 constexpr int kMaxSlots = 64;    // wrench slots per launch (documented in mpm_hip.h)
 constexpr int kGridWgs = 256;    // persistent workgroups of the grid update
 
+// Deterministic wrench: per-wavefront sums in fixed point. Force units are the momentum
+// scale / 2^8 (a wavefront sums 64 nodes' impulses, each a sum of many contributions),
+// torque units that over the next power of two of the domain diagonal (the lever arm
+// bound). A wavefront sum beyond 2^52 units is dropped and raises error bit 64.
+MPM_DEV void wrench_scales(const mpm_grid_t& G, float& sf, float& st) {
+    sf = ldexpf(G.fix_scale[1], -8);
+    const float diag = 1.7320508f * (float)max(G.n[0], max(G.n[1], G.n[2])) * G.dx;
+    st = ldexpf(sf, -max(0, (int)ceilf(log2f(diag))));
+}
+
+MPM_DEV unsigned long long wrench_q(float x, float S, int* error_flag) {
+    const float y = x * S;
+    if (!(fabsf(y) < 4503599627370496.0f)) {
+        if (error_flag) atomicOr(error_flag, 64);
+        return 0ull;
+    }
+    return (unsigned long long)__float2ll_rn(y);
+}
+
 // One substep's grid update over listed blocks (one 64-node block per wavefront per
 // iteration, persistent workgroups). Block codes are global over the batched scenes
 // (scene * blocks-per-scene + local code); a block of scene s applies that scene's row
@@ -193,6 +212,11 @@ __global__ __launch_bounds__(kThreads) void k_grid_update(mpm_grid_t G, const mp
     const float damp = md->grid_v_damping_scale;
     const int blo0 = ctl->box_lo[0], blo1 = ctl->box_lo[1], blo2 = ctl->box_lo[2];
     const int bhi0 = ctl->box_hi[0], bhi1 = ctl->box_hi[1], bhi2 = ctl->box_hi[2];
+    const bool fixed = (flags & MPM_GRID_FIXED_IN) != 0;
+    const double inv_m = fixed ? 1.0 / (double)G.fix_scale[0] : 0.0;
+    const double inv_p = fixed ? 1.0 / (double)G.fix_scale[1] : 0.0;
+    float sw_f = 0.f, sw_t = 0.f;  // fixed-point wrench scales (force, torque)
+    if (fixed) wrench_scales(G, sw_f, sw_t);
     for (int b = blockIdx.x * (kThreads / 64) + wave; b < nblk; b += waves_total) {
         const int code = list[b];
         if ((unsigned)code >= (unsigned)ncode) continue;  // wavefront-uniform
@@ -205,7 +229,16 @@ __global__ __launch_bounds__(kThreads) void k_grid_update(mpm_grid_t G, const mp
         const bool valid = gi < G.n[0] && gj < G.n[1] && gk < G.n[2];
         const bool inbox = valid && gi >= blo0 && gi < bhi0 && gj >= blo1 && gj < bhi1 && gk >= blo2 && gk < bhi2;
         float4 in4 = make_float4(0.f, 0.f, 0.f, 0.f);
-        if (valid) in4 = *reinterpret_cast<const float4*>(G.node_in + o);
+        if (valid && fixed) {  // deterministic scatter: fixed point -> fp32, then clear
+            longlong2* f = reinterpret_cast<longlong2*>(G.node_fix + o);
+            const longlong2 f01 = f[0], f23 = f[1];
+            in4 = make_float4((float)((double)f01.x * inv_p), (float)((double)f01.y * inv_p),
+                              (float)((double)f23.x * inv_p), (float)((double)f23.y * inv_m));
+            f[0] = make_longlong2(0, 0);
+            f[1] = make_longlong2(0, 0);
+        } else if (valid) {
+            in4 = *reinterpret_cast<const float4*>(G.node_in + o);
+        }
         const float m = in4.w;
         v3 vo = mk3(0.f, 0.f, 0.f);
         if (m > 1e-15f) {
@@ -222,7 +255,13 @@ __global__ __launch_bounds__(kThreads) void k_grid_update(mpm_grid_t G, const mp
             if (c.slot >= 0 && c.slot < nslots && __any(act)) {
                 const float s0 = wave_sum(imp.x), s1 = wave_sum(imp.y), s2 = wave_sum(imp.z);
                 const float s3 = wave_sum(tq.x), s4 = wave_sum(tq.y), s5 = wave_sum(tq.z);
-                if (lane == 0) {
+                if (lane == 0 && fixed) {  // order-independent: integer sums of fixed point
+                    unsigned long long* w = reinterpret_cast<unsigned long long*>(partials) + 6 * (size_t)c.slot;
+                    int* ef = G.error_flag;
+                    atomicAdd(w + 0, wrench_q(s0, sw_f, ef)); atomicAdd(w + 1, wrench_q(s1, sw_f, ef));
+                    atomicAdd(w + 2, wrench_q(s2, sw_f, ef)); atomicAdd(w + 3, wrench_q(s3, sw_t, ef));
+                    atomicAdd(w + 4, wrench_q(s4, sw_t, ef)); atomicAdd(w + 5, wrench_q(s5, sw_t, ef));
+                } else if (lane == 0) {
                     if (lds_w) {
                         float* w = &s_w[wave][6 * c.slot];
                         w[0] += s0; w[1] += s1; w[2] += s2; w[3] += s3; w[4] += s4; w[5] += s5;
@@ -235,9 +274,10 @@ __global__ __launch_bounds__(kThreads) void k_grid_update(mpm_grid_t G, const mp
             }
         }
         if (inbox) *reinterpret_cast<float4*>(G.node_out + o) = make_float4(vo.x, vo.y, vo.z, 0.0f);
-        if (valid && (flags & MPM_GRID_ZERO_IN)) *reinterpret_cast<float4*>(G.node_in + o) = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (valid && !fixed && (flags & MPM_GRID_ZERO_IN))
+            *reinterpret_cast<float4*>(G.node_in + o) = make_float4(0.f, 0.f, 0.f, 0.f);
     }
-    if (nslots > 0 && lds_w) {
+    if (nslots > 0 && lds_w && !fixed) {
         __syncthreads();
         for (int t = threadIdx.x; t < nslots * 6; t += kThreads) {
             float acc = s_w[0][t];
@@ -262,6 +302,19 @@ __global__ __launch_bounds__(kThreads) void k_wrench_reduce(const float* __restr
         __syncthreads();
     }
     if (threadIdx.x == 0) wrench[v] += sh[0];
+}
+
+// deterministic wrench: wrench[v] += fixed[v] / scale, then fixed[v] = 0; one thread per
+// value, so the float result does not depend on any ordering
+__global__ void k_wrench_fix_flush(mpm_grid_t G, unsigned long long* __restrict__ fixed, int nvals,
+                                   float* __restrict__ wrench) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= nvals) return;
+    float sf, st;
+    wrench_scales(G, sf, st);
+    const double inv = 1.0 / (double)((v % 6) < 3 ? sf : st);
+    wrench[v] += (float)((double)(long long)fixed[v] * inv);
+    fixed[v] = 0ull;
 }
 
 inline unsigned grid_update_wgs(int capacity) { return (unsigned)min((capacity + 3) / 4, kGridWgs); }
@@ -577,8 +630,8 @@ int32_t mpm_blocks_from_box(const mpm_grid_t* G, mpm_tick_ctl_t* ctl, int32_t* l
 
 int32_t mpm_grid_workspace_floats(int32_t capacity, int32_t wrench_slots) {
     if (capacity < 0 || wrench_slots < 0) return -1;
-    if (wrench_slots > kMaxSlots) return 0;  // atomic path, no workspace
-    return (int32_t)(grid_update_wgs(capacity) * wrench_slots * 6);
+    if (wrench_slots > kMaxSlots) return 12 * wrench_slots;  // atomic path; fixed-point wrench only
+    return (int32_t)max(grid_update_wgs(capacity) * wrench_slots * 6, 12u * wrench_slots);
 }
 
 int32_t mpm_grid_update(const mpm_grid_t* G, const mpm_model_t* model, const mpm_tick_ctl_t* ctl,
@@ -588,14 +641,19 @@ int32_t mpm_grid_update(const mpm_grid_t* G, const mpm_model_t* model, const mpm
     if (!grid_ok(G) || !model || !ctl || !list || !count || capacity < 0 || ncoll < 0) return MPM_ERR_ARG;
     if (ncoll > 0 && !colliders) return MPM_ERR_ARG;
     if (wrench_slots < 0) return MPM_ERR_ARG;
+    const bool fixed = (flags & MPM_GRID_FIXED_IN) != 0;
     const bool fixed_order = wrench_slots <= kMaxSlots;
-    if (wrench_slots > 0 && (!wrench || (fixed_order && !workspace))) return MPM_ERR_ARG;
+    if (wrench_slots > 0 && (!wrench || ((fixed_order || fixed) && !workspace))) return MPM_ERR_ARG;
+    if (fixed && (!G->node_fix || !(G->fix_scale[0] > 0.0f) || !(G->fix_scale[1] > 0.0f))) return MPM_ERR_ARG;
     if (capacity == 0) return MPM_OK;
     const unsigned nwg = grid_update_wgs(capacity);
     hipStream_t s = (hipStream_t)stream;
     hipLaunchKernelGGL(k_grid_update, dim3(nwg), dim3(kThreads), 0, s, *G, model, ctl, list, count, colliders, ncoll,
                        wrench_slots, workspace, wrench, dt, flags);
-    if (wrench_slots > 0 && fixed_order)
+    if (wrench_slots > 0 && fixed)
+        hipLaunchKernelGGL(k_wrench_fix_flush, dim3((wrench_slots * 6 + kThreads - 1) / kThreads), dim3(kThreads), 0,
+                           s, *G, reinterpret_cast<unsigned long long*>(workspace), wrench_slots * 6, wrench);
+    else if (wrench_slots > 0 && fixed_order)
         hipLaunchKernelGGL(k_wrench_reduce, dim3(wrench_slots * 6), dim3(kThreads), 0, s, workspace, (int)nwg,
                            wrench_slots * 6, wrench);
     return launch_status();

@@ -14,6 +14,8 @@
 #include <hip/hip_runtime.h>
 #include <limits.h>
 
+#include <type_traits>
+
 #include "mpm_device.h"
 #include "mpm_tile.h"
 #include "mpm_cdf.h"
@@ -135,10 +137,24 @@ MPM_DEV void gather_global(const mpm_grid_t& G, size_t soff, const Stencil& st, 
     }
 }
 
-template <class Mask>
+// Deterministic scatter: x in fixed point of scale S (a power of two, so x * S is exact
+// in fp32), rounded to the nearest unit. Integer sums do not depend on the order the
+// atomics land in. Beyond 2^52 units (or NaN) the contribution is dropped and error bit
+// 64 raised: 2^11 such contributions to one node still fit an int64.
+MPM_DEV unsigned long long fix_q(float x, float S, int& err) {
+    const float y = x * S;
+    if (!(fabsf(y) < 4503599627370496.0f)) {
+        err |= 64;
+        return 0ull;
+    }
+    return (unsigned long long)__float2ll_rn(y);
+}
+
+template <bool kDet, class Mask>
 MPM_DEV void scatter_global(const mpm_grid_t& G, size_t soff, const Stencil& st, bool periodic, float mass, v3 v,
-                            const m33& Cm, const m33& fs, Mask mask) {
+                            const m33& Cm, const m33& fs, Mask mask, int& err) {
     float* __restrict__ gin = G.node_in + 4 * soff;
+    unsigned long long* __restrict__ gfix = reinterpret_cast<unsigned long long*>(G.node_fix) + 4 * soff;
 #pragma unroll 1
     for (int n = 0; n < 27; ++n) {
         const int i = n / 9, j = (n / 3) % 3, k = n % 3;
@@ -155,10 +171,18 @@ MPM_DEV void scatter_global(const mpm_grid_t& G, size_t soff, const Stencil& st,
         float wmv = wm;
         if (!mask(i, j, k, add, wmv)) continue;
         const size_t o = 4 * (size_t)node_index(G, ix, iy, iz);
-        atomicAdd(gin + o + 0, add.x);
-        atomicAdd(gin + o + 1, add.y);
-        atomicAdd(gin + o + 2, add.z);
-        atomicAdd(gin + o + 3, wmv);
+        if constexpr (kDet) {
+            const float Sm = G.fix_scale[0], Sp = G.fix_scale[1];
+            atomicAdd(gfix + o + 0, fix_q(add.x, Sp, err));
+            atomicAdd(gfix + o + 1, fix_q(add.y, Sp, err));
+            atomicAdd(gfix + o + 2, fix_q(add.z, Sp, err));
+            atomicAdd(gfix + o + 3, fix_q(wmv, Sm, err));
+        } else {
+            atomicAdd(gin + o + 0, add.x);
+            atomicAdd(gin + o + 1, add.y);
+            atomicAdd(gin + o + 2, add.z);
+            atomicAdd(gin + o + 3, wmv);
+        }
     }
 }
 
@@ -218,7 +242,7 @@ constexpr int kXcds = 8;
 
 // amdgpu_waves_per_eu(4): keep the allocation within 128 VGPRs (4 wavefronts per SIMD);
 // left alone the scheduler lands between 110 and 200 depending on unrelated edits.
-template <int MODE, bool kSolid, int kT, bool kCdf>
+template <int MODE, bool kSolid, int kT, bool kCdf, bool kDet>
 __global__ __launch_bounds__(kT) __attribute__((amdgpu_waves_per_eu(4))) void k_particle_pass(mpm_particles_t P, mpm_grid_t G,
                                                       const mpm_model_t* __restrict__ md, float dt,
                                                       int xcd_q, int ppt) {
@@ -229,7 +253,9 @@ __global__ __launch_bounds__(kT) __attribute__((amdgpu_waves_per_eu(4))) void k_
     constexpr bool kLean = (MODE & MPM_PASS_LEAN) != 0;
     constexpr bool kKeepStress = (MODE & MPM_PASS_KEEP_STRESS) != 0;
     __shared__ float4 s_gather[kG2P ? kArena : 1];
-    __shared__ float s_scatter[kP2G ? 4 : 1][kP2G ? kArena : 1];
+    // scatter arena: float sums, or fixed-point integer sums in the deterministic mode
+    using Acc = typename std::conditional<kDet, unsigned long long, float>::type;
+    __shared__ Acc s_scatter[kP2G ? 4 : 1][kP2G ? kArena : 1];
     __shared__ int s_red0[kT / 64][8];
     __shared__ int s_red1[kT / 64][8];
     __shared__ float s_react[kCdf && kP2G ? 6 * MPM_MAX_CDF : 1];  // blocked deposits per lane
@@ -393,10 +419,10 @@ __global__ __launch_bounds__(kT) __attribute__((amdgpu_waves_per_eu(4))) void k_
             }
             const Tile ts = make_tile<kT>(G, lo, hi, alive ? scene : INT_MAX, alive ? scene : INT_MIN, s_red1, periodic);
             tile_for<kT>(ts, [&](int a, int, int, int) {
-                s_scatter[0][a] = 0.f;
-                s_scatter[1][a] = 0.f;
-                s_scatter[2][a] = 0.f;
-                s_scatter[3][a] = 0.f;
+                s_scatter[0][a] = Acc(0);
+                s_scatter[1][a] = Acc(0);
+                s_scatter[2][a] = Acc(0);
+                s_scatter[3][a] = Acc(0);
             });
             int pc_s = 0;  // CPIC tag of the scatter half (mpm_utils.py:851-860)
             if constexpr (kCdf) {
@@ -459,7 +485,8 @@ __global__ __launch_bounds__(kT) __attribute__((amdgpu_waves_per_eu(4))) void k_
             const int prevkey = __shfl_up(cellkey, 1, 64);
             const bool head = lane == 0 || prevkey != cellkey;
             const unsigned long long heads = __ballot(head);
-            const bool segmented = ts.lds && __popcll(~heads) >= 16;  // wavefront-uniform
+            // (never in the deterministic mode: a float pre-sum would depend on the grouping)
+            const bool segmented = !kDet && ts.lds && __popcll(~heads) >= 16;  // wavefront-uniform
             if (segmented) {
                 const unsigned long long upto = lane == 63 ? ~0ull : ((1ull << (lane + 1)) - 1ull);
                 const int seg0 = 63 - __clzll(heads & upto);
@@ -478,10 +505,12 @@ __global__ __launch_bounds__(kT) __attribute__((amdgpu_waves_per_eu(4))) void k_
                     const float s3 = seg_row_sum(wm, c1, c2, c4, c8);
                     if (tail && alive) {
                         const int a = b0 + ((o0 + i) * d1 + (o1 + j)) * d2 + (o2 + k);
-                        atomicAdd(&s_scatter[0][a], s0);
-                        atomicAdd(&s_scatter[1][a], s1);
-                        atomicAdd(&s_scatter[2][a], s2);
-                        atomicAdd(&s_scatter[3][a], s3);
+                        if constexpr (!kDet) {
+                            atomicAdd(&s_scatter[0][a], s0);
+                            atomicAdd(&s_scatter[1][a], s1);
+                            atomicAdd(&s_scatter[2][a], s2);
+                            atomicAdd(&s_scatter[3][a], s3);
+                        }
                     }
                 });
             } else if (alive) {
@@ -491,27 +520,43 @@ __global__ __launch_bounds__(kT) __attribute__((amdgpu_waves_per_eu(4))) void k_
                                     [&](int i, int j, int k, v3 add, float wm) {
                                         if (!mask(i, j, k, add, wm)) return;
                                         const int a = b0 + ((o0 + i) * d1 + (o1 + j)) * d2 + (o2 + k);
-                                        atomicAdd(&s_scatter[0][a], add.x);
-                                        atomicAdd(&s_scatter[1][a], add.y);
-                                        atomicAdd(&s_scatter[2][a], add.z);
-                                        atomicAdd(&s_scatter[3][a], wm);
+                                        if constexpr (kDet) {
+                                            const float Sm = G.fix_scale[0], Sp = G.fix_scale[1];
+                                            atomicAdd(&s_scatter[0][a], fix_q(add.x, Sp, err));
+                                            atomicAdd(&s_scatter[1][a], fix_q(add.y, Sp, err));
+                                            atomicAdd(&s_scatter[2][a], fix_q(add.z, Sp, err));
+                                            atomicAdd(&s_scatter[3][a], fix_q(wm, Sm, err));
+                                        } else {
+                                            atomicAdd(&s_scatter[0][a], add.x);
+                                            atomicAdd(&s_scatter[1][a], add.y);
+                                            atomicAdd(&s_scatter[2][a], add.z);
+                                            atomicAdd(&s_scatter[3][a], wm);
+                                        }
                                     });
                 } else {
-                    scatter_global(G, soff, st, periodic, mass, v, Cm, fs, mask);
+                    scatter_global<kDet>(G, soff, st, periodic, mass, v, Cm, fs, mask, err);
                 }
             }
             __syncthreads();
             {
                 float* __restrict__ gin = G.node_in + 4 * ts.soff;
+                unsigned long long* __restrict__ gfix = reinterpret_cast<unsigned long long*>(G.node_fix) + 4 * ts.soff;
                 tile_for<kT>(ts, [&](int a, int gi, int gj, int gk) {
-                    const float a0 = s_scatter[0][a], a1 = s_scatter[1][a], a2 = s_scatter[2][a], a3 = s_scatter[3][a];
-                    if (a0 == 0.f && a1 == 0.f && a2 == 0.f && a3 == 0.f) return;
+                    const Acc a0 = s_scatter[0][a], a1 = s_scatter[1][a], a2 = s_scatter[2][a], a3 = s_scatter[3][a];
+                    if (a0 == Acc(0) && a1 == Acc(0) && a2 == Acc(0) && a3 == Acc(0)) return;
                     if (!in_grid(G, gi, gj, gk)) return;
                     const size_t o = 4 * (size_t)node_index(G, gi, gj, gk);
-                    atomicAdd(gin + o + 0, a0);
-                    atomicAdd(gin + o + 1, a1);
-                    atomicAdd(gin + o + 2, a2);
-                    atomicAdd(gin + o + 3, a3);
+                    if constexpr (kDet) {
+                        atomicAdd(gfix + o + 0, a0);
+                        atomicAdd(gfix + o + 1, a1);
+                        atomicAdd(gfix + o + 2, a2);
+                        atomicAdd(gfix + o + 3, a3);
+                    } else {
+                        atomicAdd(gin + o + 0, a0);
+                        atomicAdd(gin + o + 1, a1);
+                        atomicAdd(gin + o + 2, a2);
+                        atomicAdd(gin + o + 3, a3);
+                    }
                 });
             }
             if constexpr (kCdf) {
@@ -537,11 +582,14 @@ extern "C" int32_t mpm_particle_pass(const mpm_particles_t* P, const mpm_grid_t*
         G->nb[2] != (G->n[2] + 3) / 4)
         return MPM_ERR_ARG;
     const bool cdf = G->cdf.n_cdf > 0;
+    const bool det = (mode & MPM_PASS_DETERMINISTIC) != 0;
+    if (det && (cdf || !G->node_fix || !(G->fix_scale[0] > 0.0f) || !(G->fix_scale[1] > 0.0f)))
+        return MPM_ERR_ARG;
     if (cdf && (G->n_scenes != 1 || !P->cdf_tag || !G->cdf.tag || !G->cdf.tag_prev || !G->cdf.dist ||
                 !G->cdf.dist_prev || !G->cdf.block || !G->cdf.lanes || !G->cdf.reaction || (mode & MPM_PASS_LEAN)))
         return MPM_ERR_ARG;
     if (P->n == 0) return MPM_OK;
-    const int wg = (!cdf && (mode & MPM_PASS_WG128)) ? 128 : 256;
+    const int wg = (!cdf && !det && (mode & MPM_PASS_WG128)) ? 128 : 256;
     const int ppt = (mode & MPM_PASS_PPT4) ? 4 : ((mode & MPM_PASS_PPT2) ? 2 : 1);
     hipStream_t s = (hipStream_t)stream;
     const int nchunks = (P->n + wg - 1) / wg;
@@ -550,40 +598,40 @@ extern "C" int32_t mpm_particle_pass(const mpm_particles_t* P, const mpm_grid_t*
     const dim3 grid((unsigned)(q * kXcds)), block(wg);
     const bool solid = (mode & MPM_PASS_VISCOUS_ONLY) == 0;
     const int m = mode & (MPM_PASS_G2P | MPM_PASS_STRESS | MPM_PASS_P2G | MPM_PASS_LEAN | MPM_PASS_KEEP_STRESS);
-#define LAUNCH(M, T, C)                                                                                               \
-    if (solid) hipLaunchKernelGGL((k_particle_pass<(M), true, T, C>), grid, block, 0, s, *P, *G, model, dt, q, ppt); \
-    else hipLaunchKernelGGL((k_particle_pass<(M), false, T, C>), grid, block, 0, s, *P, *G, model, dt, q, ppt);
-#define CASE(M)                   \
-    case (M):                     \
-        if (wg == 128) {          \
-            LAUNCH(M, 128, false) \
-        } else {                  \
-            LAUNCH(M, 256, false) \
-        }                         \
-        break;
-// CPIC colliders: non-lean modes, 256-thread workgroups
-#define CASE_CDF(M)               \
-    case (M):                     \
-        if (cdf) {                \
-            LAUNCH(M, 256, true)  \
-        } else if (wg == 128) {   \
-            LAUNCH(M, 128, false) \
-        } else {                  \
-            LAUNCH(M, 256, false) \
-        }                         \
-        break;
+#define LAUNCH(M, T, C, D)                                                                                     \
+    if (solid)                                                                                                 \
+        hipLaunchKernelGGL((k_particle_pass<(M), true, T, C, D>), grid, block, 0, s, *P, *G, model, dt, q, ppt); \
+    else                                                                                                       \
+        hipLaunchKernelGGL((k_particle_pass<(M), false, T, C, D>), grid, block, 0, s, *P, *G, model, dt, q, ppt);
+#define PLAIN(M)                         \
+    if (wg == 128) {                     \
+        LAUNCH(M, 128, false, false)     \
+    } else {                             \
+        LAUNCH(M, 256, false, false)     \
+    }
+// CPIC colliders: non-lean modes, 256-thread workgroups; deterministic scatter: the
+// modes that scatter, 256-thread workgroups (each variant instantiated only where used)
+#define CDF_(M) if (cdf) { LAUNCH(M, 256, true, false) break; }
+#define DET_(M) if (det) { LAUNCH(M, 256, false, true) break; }
+#define CASE_CD(M) case (M): CDF_(M) DET_(M) PLAIN(M) break;
+#define CASE_C(M) case (M): CDF_(M) PLAIN(M) break;  /* no scatter: the deterministic bit is moot */
+#define CASE_D(M) case (M): if (cdf) return MPM_ERR_ARG; DET_(M) PLAIN(M) break;
     switch (m) {
-        CASE_CDF(MPM_PASS_G2P)
-        CASE_CDF(MPM_PASS_STRESS)
-        CASE_CDF(MPM_PASS_P2G)
-        CASE_CDF(MPM_PASS_STRESS | MPM_PASS_P2G)
-        CASE_CDF(MPM_PASS_G2P | MPM_PASS_STRESS | MPM_PASS_P2G)
-        CASE(MPM_PASS_G2P | MPM_PASS_STRESS | MPM_PASS_P2G | MPM_PASS_LEAN)
-        CASE(MPM_PASS_G2P | MPM_PASS_STRESS | MPM_PASS_P2G | MPM_PASS_LEAN | MPM_PASS_KEEP_STRESS)
+        CASE_C(MPM_PASS_G2P)
+        CASE_C(MPM_PASS_STRESS)
+        CASE_CD(MPM_PASS_P2G)
+        CASE_CD(MPM_PASS_STRESS | MPM_PASS_P2G)
+        CASE_CD(MPM_PASS_G2P | MPM_PASS_STRESS | MPM_PASS_P2G)
+        CASE_D(MPM_PASS_G2P | MPM_PASS_STRESS | MPM_PASS_P2G | MPM_PASS_LEAN)
+        CASE_D(MPM_PASS_G2P | MPM_PASS_STRESS | MPM_PASS_P2G | MPM_PASS_LEAN | MPM_PASS_KEEP_STRESS)
         default: return MPM_ERR_ARG;
     }
+#undef CDF_
+#undef DET_
+#undef CASE_CD
+#undef CASE_C
+#undef CASE_D
 #undef LAUNCH
-#undef CASE
-#undef CASE_CDF
+#undef PLAIN
     return launch_status();
 }

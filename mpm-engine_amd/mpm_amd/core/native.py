@@ -18,11 +18,12 @@ MAX_MATERIALS = 16
 # mode / flag bits, mirrored from include/mpm_hip.h
 PASS_G2P, PASS_STRESS, PASS_P2G, PASS_LEAN, PASS_KEEP_STRESS, PASS_VISCOUS_ONLY = 1, 2, 4, 8, 16, 32
 PASS_WG128, PASS_PPT2, PASS_PPT4 = 64, 128, 256  # launch shape (results do not depend on it)
+PASS_DETERMINISTIC = 512
 VISCOUS_IDS = frozenset((6, 7, 8, 10, 12))
-GRID_ZERO_IN = 1
+GRID_ZERO_IN, GRID_FIXED_IN = 1, 2
 MAX_FIXED_ORDER_SLOTS = 64  # wrench slots reduced in fixed order; more use float atomics
 ERR_STENCIL, ERR_NONFINITE, ERR_SOLID_IN_VISCOUS, ERR_SCENE = 1, 2, 4, 8
-ERR_ADJ_MATERIAL, ERR_ADJ_COLLIDER = 16, 32
+ERR_ADJ_MATERIAL, ERR_ADJ_COLLIDER, ERR_FIXED_RANGE = 16, 32, 64
 COLLIDER_PLANE, COLLIDER_BOX, COLLIDER_CUP, COLLIDER_WALLS, COLLIDER_SDF, COLLIDER_POINTCLOUD = \
     1, 2, 3, 4, 5, 6
 
@@ -56,7 +57,7 @@ class Grid(C.Structure):
     _fields_ = [("node_in", _FP), ("node_out", _FP), ("n", C.c_int32 * 3),
                 ("nb", C.c_int32 * 3), ("dx", C.c_float), ("inv_dx", C.c_float),
                 ("error_flag", _FP), ("n_scenes", C.c_int32), ("pad_", C.c_int32),
-                ("cdf", Cdf)]
+                ("cdf", Cdf), ("node_fix", _FP), ("fix_scale", C.c_float * 2)]
 
 
 _L3 = (np.float32, (MAX_CDF, 3))

@@ -158,6 +158,7 @@ class HipSimulator:
         self._model_dirty = True
         self.material_id = 0
         self._mat_ids = {0}  # material ids present (selects the pass specialisation)
+        self._m_max = None  # heaviest particle (deterministic-scatter scale), cached
         # colliders
         self.colliders = ColliderSet(self.n_grid, self.dx)
         self.scene_colliders = None  # batched scenes: a colliders.SceneColliders
@@ -179,6 +180,11 @@ class HipSimulator:
         self._ticks = 0
         self.cdf = None      # CdfColliders, created by the first CPIC collider
         self.cdf_tag = None  # per-particle CPIC tag (allocated with it)
+        # deterministic scatter (fixed-point integer accumulation), off by default
+        self.deterministic = False
+        self.node_fix = None
+        self._vmax = 0.0
+        self._fix_exp = None
         self._structs()
 
     # --------------------------------------------------------------- ABI structs
@@ -200,6 +206,10 @@ class HipSimulator:
         G.n_scenes = self.n_scenes
         if self.cdf is not None:
             self.cdf.fill(G)
+        G.node_fix = native.ptr(self.node_fix)
+        if self._fix_exp is not None:
+            G.fix_scale[0] = math.ldexp(1.0, self._fix_exp[0])
+            G.fix_scale[1] = math.ldexp(1.0, self._fix_exp[1])
         self.P, self.G = P, G
         self.pP, self.pG = C.byref(P), C.byref(G)
 
@@ -275,6 +285,7 @@ class HipSimulator:
         self._model_dirty = True
 
     def finalize_mu_lam(self):
+        self._m_max = None
         """compute_mu_lam_from_E_nu (mpm_utils.py:583-588), fp32 on device."""
         torch = self.torch
         md = self.model[0]
@@ -284,6 +295,7 @@ class HipSimulator:
         self.lam.copy_(E * nu / ((1.0 + nu) * (1.0 - 2.0 * nu)))
 
     def set_parameters_for_particles(self, a: int, b: int, kw: dict):
+        self._m_max = None
         """set_parameters_for_particles (mpm_solver_warp.py:1700-1783)."""
         m = None
         if "material" in kw:
@@ -345,7 +357,34 @@ class HipSimulator:
         native.check(self.lib.mpm_bounds(self.pP, native.ptr(self.bounds), self.stream()), "bounds")
         b = self.bounds.cpu().numpy()
         flag = int(self.error_flag.item())
+        self._vmax = float(b[6])
         return b[0:3].copy(), b[3:6].copy(), float(b[6]), int(b[7]), flag
+
+    # --------------------------------------------------------------- deterministic mode
+    def enable_deterministic(self):
+        """Switch the scatter to 64-bit fixed-point integer accumulation (bit-reproducible
+        grid sums, independent of atomic order, launch shape and particle order)."""
+        if self.cdf is not None and self.cdf.n > 0:
+            raise NotImplementedError("the deterministic scatter does not cover CDF colliders")
+        self.deterministic = True
+        if self.node_fix is None:
+            self.node_fix = self.torch.zeros(self.n_blocks * 64 * 4, dtype=self.torch.int64, device=self.device)
+        self.grid_ws = None  # the fixed-point wrench needs a zeroed workspace
+        self._graph = None
+        self._structs()
+
+    def _fix_scales(self, dt: float):
+        """Powers of two such that the largest expected contribution is about 2^40 units
+        (2^12 below the per-contribution limit): mass by the heaviest particle, momentum by
+        it times (max particle speed + dx/dt), the speed a stable substep cannot exceed."""
+        if self._m_max is None:
+            self._m_max = float(self.mass.max().item()) if self.n_particles else 1.0
+        m = max(self._m_max, 1e-30)
+        p = m * (self._vmax + self.dx / dt)
+        exp = (40 - math.ceil(math.log2(m)), 40 - math.ceil(math.log2(p)))
+        if exp != self._fix_exp:
+            self._fix_exp = exp
+            self._structs()
 
     # --------------------------------------------------------------- sparse blocks
     def rebuild_active_blocks(self):
@@ -438,6 +477,8 @@ class HipSimulator:
         self._upload_model()
         L = self.lib
         s = self.stream()
+        if self.deterministic:
+            self._fix_scales(dt)
         tab, t_end, K, n_slots = self._schedule(dt, S)
         self.n_slots = n_slots
         if 6 * n_slots > self.wrench.numel():  # grow, keeping the accumulated values
@@ -497,7 +538,8 @@ class HipSimulator:
             key = (S, float(dt), bool(fused), K, lists, native.ptr(self.sched_dev),
                    self.update_cov_with_F, n_slots, native.ptr(self.grid_ws), native.ptr(self.wrench),
                    frozenset(self._mat_ids) <= native.VISCOUS_IDS, self.launch_bits,
-                   None if cdf is None else (cdf.n, int(cdf.mask_off), native.ptr(self.cdf_tag)))
+                   None if cdf is None else (cdf.n, int(cdf.mask_off), native.ptr(self.cdf_tag)),
+                   self._fix_exp if self.deterministic else None)
             if self._graph is None or self._graph_key != key:
                 self._capture(dt, S, K, fused, lists)
                 self._graph_key = key
@@ -525,6 +567,8 @@ class HipSimulator:
 
     def _pass(self, mode, dt, s):
         mode |= self.launch_bits
+        if self.deterministic:
+            mode |= native.PASS_DETERMINISTIC
         if self._mat_ids <= native.VISCOUS_IDS:
             mode |= native.PASS_VISCOUS_ONLY
         native.check(self.lib.mpm_particle_pass(self.pP, self.pG, native.ptr(self.model_dev), dt, mode, s),
@@ -536,7 +580,8 @@ class HipSimulator:
         native.check(self.lib.mpm_grid_update(
             self.pG, native.ptr(self.model_dev), native.ptr(self.ctl), lists[0], lists[1],
             self.n_blocks, sched if K else None, K, native.ptr(self.wrench), slots,
-            native.ptr(self.grid_ws) if slots else None, dt, native.GRID_ZERO_IN, s), "grid_update")
+            native.ptr(self.grid_ws) if slots else None, dt,
+            native.GRID_FIXED_IN if self.deterministic else native.GRID_ZERO_IN, s), "grid_update")
 
     def _enqueue_tick_cdf(self, dt, S, K, fused, lists, s):
         """One tick with the CPIC per-substep host work between the launches (the
@@ -651,6 +696,9 @@ class Solver:
     # "block": the reference's sort key; "cell": the same order refined by stencil base
     # cell, so that the particle pass can sum same-cell particles before its LDS atomics
     sort_key: str = "block"
+    # no counterpart in the reference: 64-bit fixed-point integer accumulation of the
+    # scatter and the wrench, bit-reproducible from run to run
+    deterministic: bool = False
     _sim: Any = field(default=None, init=False, repr=False)
     _step: int = field(default=0, init=False, repr=False)
     _tick: int = field(default=0, init=False, repr=False)
@@ -682,6 +730,8 @@ class Solver:
             self._sim._model_dirty = True
         if cov6 is not None and cov_mode == "step":
             self._sim.enable_cov_step()
+        if self.deterministic:
+            self._sim.enable_deterministic()
         return self
 
     def set_material(self, material, **overrides: float) -> "Solver":
@@ -938,6 +988,8 @@ class Solver:
         Handles are their own space (0..3)."""
         if self.periodic_x:
             raise NotImplementedError("CDF colliders do not wrap the periodic x axis")
+        if self.deterministic:
+            raise NotImplementedError("the deterministic scatter does not cover CDF colliders")
         sim = self._sim
         return sim.cdf_colliders().add(
             cdf.values, cdf.valid, cdf.origin, cdf.cell, cdf.band, center, quat, velocity, omega,
@@ -991,6 +1043,9 @@ class Solver:
         lim = self.grid.grid_lim
         if flag & 4:
             raise RuntimeError("internal: a solid material ran in the viscous-only particle pass")
+        if flag & native.ERR_FIXED_RANGE:
+            raise RuntimeError("deterministic scatter: a contribution exceeded the fixed-point range "
+                               "(the state is far outside the expected momentum scale)")
         if flag & 1:
             raise RuntimeError("a particle stencil left the grid during the last tick (out of "
                                "bounds P2G was skipped); enlarge grid_lim or add a wall collider")
