@@ -12,10 +12,11 @@ workload. Two more legs ride in the same line:
              100-substep hipGraph chunks, autotuned launch shape and sort interval, against
              the same scenes solved one after another with the plain Solver;
   "cdf_pour" the same pour with both glasses as CPIC thin-boundary sheets;
+  "deterministic" the pour with the fixed-point (bit-reproducible) scatter;
   "adjoint"  the squeeze identification (50k von Mises particles, 400 substeps): gradient
              of a loss on final positions and plate forces, backward time / forward time.
 
-    python bench.py [--gpus 1] [--steps 10] [--warmup 2] [--no-cpu] [--legs pour,adjoint,batch,cdf]
+    python bench.py [--gpus 1] [--steps 10] [--warmup 2] [--no-cpu] [--legs pour,adjoint,batch,cdf,det]
 """
 from __future__ import annotations
 
@@ -57,8 +58,9 @@ def parse():
     ap.add_argument("--seq-scenes", type=int, default=0,
                     help="scenes timed one by one for the sequential baseline (0 = all)")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint leg")
-    ap.add_argument("--legs", default="pour,adjoint,batch,cdf",
-                    help="comma list of legs to run (pour, adjoint, batch, cdf); for profiling one leg")
+    ap.add_argument("--legs", default="pour,adjoint,batch,cdf,det",
+                    help="comma list of legs to run (pour, adjoint, batch, cdf, det); for profiling one leg")
+    ap.add_argument("--det-frames", type=int, default=2, help="timed frames of the deterministic pour")
     ap.add_argument("--cdf-frames", type=int, default=2, help="timed frames of the CDF-glass pour")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a device copy of 256 MiB (the byte-counter calibration for --pmc)")
@@ -366,6 +368,43 @@ def cdf_leg(args):
                          "(docs/performance.md:99-105)"}
 
 
+def det_leg(args):
+    """The headline pour with Solver(deterministic=True): fixed-point integer scatter and
+    wrench sums, bit-reproducible. Reports its frame time against the float-atomic
+    headline and a checksum of the final positions (identical from run to run)."""
+    import hashlib
+
+    import numpy as np
+    import torch
+
+    from mpm_amd import Solver
+    from mpm_amd.scenes import PourScene
+
+    sc = PourScene()
+    s = sc.build(Solver(grid=sc.grid, device="cuda:0", sort_interval=args.sort_interval,
+                        sort_key=args.sort_key, deterministic=True))
+    n = s.n_particles
+    frames = max(1, args.det_frames)
+    for _ in range(args.warmup):
+        sc.frame(s)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(frames):
+        sc.frame(s)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    x = s.x()
+    if not np.isfinite(x).all():
+        raise RuntimeError("non-finite particle state after the deterministic frames")
+    # particle order depends only on the (deterministic) sort, so the raw bytes are comparable
+    digest = hashlib.sha256(np.ascontiguousarray(x).tobytes()).hexdigest()[:16]
+    durations = held_frame_pass_times(s, sc)
+    return {"value": round(n * frames * sc.substeps / el, 1), "unit": "particle-substeps/s",
+            "ms_per_frame": round(el / frames * 1e3, 2), "frames": frames + args.warmup,
+            "fused_pass_avg_us": round(float(np.mean(durations)) * 1e3, 2),
+            "x_sha256_16": digest}
+
+
 def pour_leg(args):
     """The headline: fused ticks of the 192^3 pour, frames timed end to end; the fused
     particle pass timed with HIP events on its stream."""
@@ -446,6 +485,13 @@ def main():
         except Exception as e:  # reported in the line; the headline value stands
             batch = {"error": f"{type(e).__name__}: {e}"}
         torch.cuda.empty_cache()
+    det = None
+    if "det" in legs:
+        try:
+            det = det_leg(args)
+        except Exception as e:  # reported in the line; the headline value stands
+            det = {"error": f"{type(e).__name__}: {e}"}
+        torch.cuda.empty_cache()
     cdf = None
     if "cdf" in legs:
         try:
@@ -463,7 +509,7 @@ def main():
                       "grid": 192, "graph": not args.no_graph, "sort_interval": args.sort_interval,
                       "sort_key": args.sort_key},
            "roofline": pour.get("roofline"), "cpu_baseline": cpu, "batched": batch, "adjoint": adjoint,
-           "cdf_pour": cdf}
+           "cdf_pour": cdf, "deterministic": det}
     print(json.dumps(out))
 
 

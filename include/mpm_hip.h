@@ -321,6 +321,19 @@ int32_t mpm_sort_pairs(const int32_t *keys, int32_t *keys_out, const int32_t *va
  * (>= 9 * stride floats) and copy back in place, so device pointers stay stable. */
 int32_t mpm_permute(const mpm_particles_t *P, const int32_t *perm, float *scratch, void *stream);
 
+/* ---- coupling readout (box_contact_wrench, coupling/wrench.py:15-46) -------------- */
+/* Stress-integral reaction on a box end-effector from the particle state on the device.
+ * box = (centre xyz, half extents xyz) on the host; layer = contact-layer thickness T (m),
+ * or <= 0 for 3 * mean(vol0 |det F|)^(1/3). A particle counts when it lies under the box
+ * footprint with z in (bottom - T, bottom + T/2) and, with compressive_only, sigma_zz < 0;
+ * Cauchy = tau / max(|det F|, 1e-9). out (device, 8 floats): force xyz, torque xyz about
+ * the centre, contact count (force and torque zero when it is below 3), count of particles
+ * with det F <= 0 or non-finite. workspace: mpm_box_contact_workspace_floats() floats.
+ * Fixed-order reductions: bit-reproducible. */
+int32_t mpm_box_contact_workspace_floats(void);
+int32_t mpm_box_contact_wrench(const mpm_particles_t *P, const float *box, float layer,
+                               int32_t compressive_only, float *out, float *workspace, void *stream);
+
 /* ---- exports ------------------------------------------------------------------------ */
 /* compute_R_from_F (mpm_utils.py:1261-1285): R_out[9*p + ...] = (U V^T)^T from F_trial,
  * row-major AoS [n][3][3], as the reference stores it. */

@@ -1133,6 +1133,37 @@ class Solver:
     def vol(self) -> np.ndarray:
         return self._vol0 * self._checked_jacobian()
 
+    def box_contact_wrench(self, box_center, box_half, layer_cells: float | None = None,
+                           compressive_only: bool = True) -> dict:
+        """The reference's box_contact_wrench (coupling/wrench.py:15-46) evaluated on the
+        device state: 8 floats come back instead of x, F and stress of every particle.
+        Volume and Cauchy stress follow inversion_policy "warn" (|det F|, clipped at
+        1e-9); with "raise" an inverted particle raises, with "nan" the host formula on
+        the exported arrays is used (NaN volumes poison the layer thickness there)."""
+        sim = self._sim
+        if self.inversion_policy == "nan":
+            from ..coupling.wrench import box_contact_wrench
+            return box_contact_wrench(self.x(), self.cauchy(), self.vol(), box_center, box_half,
+                                      layer_cells=layer_cells, dx=self.grid.dx,
+                                      compressive_only=compressive_only)
+        torch = sim.torch
+        if getattr(sim, "_bcw_ws", None) is None:
+            sim._bcw_ws = torch.zeros(int(sim.lib.mpm_box_contact_workspace_floats()), dtype=torch.float32,
+                                      device=sim.device)
+            sim._bcw_out = torch.zeros(8, dtype=torch.float32, device=sim.device)
+        box = (C.c_float * 6)(*[float(v) for v in box_center], *[float(v) for v in box_half])
+        layer = float(layer_cells) * self.grid.dx if layer_cells else 0.0
+        native.check(sim.lib.mpm_box_contact_wrench(sim.pP, box, layer, int(bool(compressive_only)),
+                                                    native.ptr(sim._bcw_out), native.ptr(sim._bcw_ws),
+                                                    sim.stream()), "box_contact_wrench")
+        out = sim._bcw_out.cpu().numpy().astype(np.float64)
+        if out[7] > 0 and self.inversion_policy == "raise":
+            raise RuntimeError(f"{int(out[7])} particles have non-finite or non-positive det(F); current "
+                               "volume and Cauchy stress are undefined for an inverted configuration")
+        n = int(out[6])
+        return {"force": out[0:3].copy(), "torque": out[3:6].copy(), "n_contact": n if n >= 3 else 0,
+                "Fz": float(out[2])}
+
     def cauchy(self) -> np.ndarray:
         J = self._checked_jacobian()
         if self.inversion_policy == "warn":

@@ -227,3 +227,77 @@ class SqueezeScene:
         self.plates = [solver.add_box(c, self.plate_half, v)
                        for c, v in zip(self.plate_centers, self.plate_velocity)]
         return solver
+
+
+class DoughSandPress:
+    """Dough press beside a sand heap on a 128^3 grid over [0, 1]^3 (sparse blocks, block
+    sort every `sort_interval` ticks): a Herschel-Bulkley dough slab 0.3 x 0.3 x 0.1
+    (yield 200 Pa, K 50, n 0.5, rho 1000) and a mu(I) sand heap of seeded random height on
+    a 0.3 x 0.3 footprint (mean height 0.085, mu_s 0.38, mu_2 = mu_s + 0.26, I0 0.28, d 1 mm, rho 1550), both
+    at `ppc` particles per cell (16 gives ~300k each), a sticky floor 3 cells up, domain
+    walls, and a kinematic 0.12 x 0.12 box touching the dough top pressing down at 0.05 m/s until the
+    stress-integral reaction under it reaches `f_stop` (30 N), then holding. dt = 5e-5;
+    a tick is `substeps` substeps (force checked once per tick)."""
+
+    def __init__(self, n_grid: int = 128, ppc: int = 16, seed: int = 0, substeps: int = 20,
+                 f_stop: float = 30.0, speed: float = 0.05, gap_cells: float = 0.0):
+        self.grid = GridConfig(n_grid=n_grid, grid_lim=1.0)
+        dx = self.grid.dx
+        self.dt = 5e-5
+        self.substeps = substeps
+        self.f_stop, self.speed = f_stop, speed
+        h = dx / ppc ** (1.0 / 3.0)
+        self.floor = 3 * dx
+        fl = self.floor
+        rng = np.random.default_rng(seed)
+        dpos, dvol = lattice_box((0.10, 0.35, fl), (0.40, 0.65, fl + 0.10), h, seed)
+        # heap: columns of the footprint lattice, height = smooth seeded bumps in [0.05, 0.15]
+        cols = np.stack(np.meshgrid(np.arange(0.55 + 0.5 * h, 0.85, h), np.arange(0.35 + 0.5 * h, 0.65, h),
+                                    indexing="ij"), axis=-1).reshape(-1, 2)
+        k = rng.uniform(4.0, 12.0, size=(3, 2))
+        ph = rng.uniform(0.0, 2.0 * np.pi, size=3)
+        bump = sum(np.sin(k[i, 0] * cols[:, 0] + ph[i]) * np.sin(k[i, 1] * cols[:, 1]) for i in range(3)) / 3.0
+        height = 0.085 + 0.04 * bump
+        spos = []
+        for j, (x, y) in enumerate(cols):
+            zs = np.arange(fl + 0.5 * h, fl + height[j], h)
+            spos.append(np.stack([np.full(len(zs), x), np.full(len(zs), y), zs], axis=-1))
+        spos = np.concatenate(spos)
+        spos = spos + rng.uniform(-0.25 * h, 0.25 * h, size=spos.shape)
+        self.n_dough = len(dpos)
+        self.pos = np.concatenate([dpos, spos.astype(np.float32)])
+        self.vol = np.concatenate([dvol, np.full(len(spos), h ** 3, np.float32)])
+        self.box_half = np.array([0.06, 0.06, 0.02])
+        # the plate starts `gap_cells` above the slab top (0: touching)
+        self.box_center = np.array([0.25, 0.50, fl + 0.10 + 0.02 + gap_cells * dx])
+        self.stopped = False
+
+    def materials(self):
+        from .materials import granular
+        dough_m = newtonian(eta=0.0, density=1000.0).with_yield(200.0).with_powerlaw(K=50.0, n=0.5)
+        sand = granular(mu_s=0.38, delta_mu=0.26, I0=0.28, grain_diameter=1.0e-3, density=1550.0)
+        return dough_m, sand
+
+    def build(self, solver):
+        dough_m, sand = self.materials()
+        solver.load_particles(self.pos, self.vol)
+        solver.set_material(dough_m)
+        solver.set_material_range(self.n_dough, len(self.pos), sand)
+        solver.add_plane((0, 0, self.floor), (0, 0, 1), "sticky")
+        solver.add_domain_walls()
+        self.box = solver.add_box(self.box_center, self.box_half, velocity=(0.0, 0.0, -self.speed))
+        return solver
+
+    def tick(self, solver, wrench=None):
+        """One tick, then the force check: the stress-integral Fz under the box (from
+        `wrench(center, half)`, default the solver's device readout) stops the box once it
+        reaches f_stop. Returns Fz."""
+        solver.step(self.dt, self.substeps)
+        if not self.stopped:
+            self.box_center = self.box_center + np.array([0.0, 0.0, -self.speed * self.dt * self.substeps])
+        fn = wrench or solver.box_contact_wrench
+        fz = float(fn(self.box_center, self.box_half)["Fz"])
+        if not self.stopped and fz >= self.f_stop:
+            self.stopped = True
+            solver.set_box(self.box, velocity=(0.0, 0.0, 0.0))
+        return fz
