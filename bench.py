@@ -12,11 +12,12 @@ workload. Two more legs ride in the same line:
              100-substep hipGraph chunks, autotuned launch shape and sort interval, against
              the same scenes solved one after another with the plain Solver;
   "cdf_pour" the same pour with both glasses as CPIC thin-boundary sheets;
-  "deterministic" the pour with the fixed-point (bit-reproducible) scatter;
+  "float_atomics" the pour with the reference's float-atomic scatter instead of the
+             default fixed-point (bit-reproducible) one;
   "adjoint"  the squeeze identification (50k von Mises particles, 400 substeps): gradient
              of a loss on final positions and plate forces, backward time / forward time.
 
-    python bench.py [--gpus 1] [--steps 10] [--warmup 2] [--no-cpu] [--legs pour,adjoint,batch,cdf,det]
+    python bench.py [--gpus 1] [--steps 10] [--warmup 2] [--no-cpu] [--legs pour,adjoint,batch,cdf,float]
 """
 from __future__ import annotations
 
@@ -58,9 +59,9 @@ def parse():
     ap.add_argument("--seq-scenes", type=int, default=0,
                     help="scenes timed one by one for the sequential baseline (0 = all)")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint leg")
-    ap.add_argument("--legs", default="pour,adjoint,batch,cdf,det",
-                    help="comma list of legs to run (pour, adjoint, batch, cdf, det); for profiling one leg")
-    ap.add_argument("--det-frames", type=int, default=2, help="timed frames of the deterministic pour")
+    ap.add_argument("--legs", default="pour,adjoint,batch,cdf,float",
+                    help="comma list of legs to run (pour, adjoint, batch, cdf, float); for profiling one leg")
+    ap.add_argument("--float-frames", type=int, default=2, help="timed frames of the float-atomic pour")
     ap.add_argument("--cdf-frames", type=int, default=2, help="timed frames of the CDF-glass pour")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a device copy of 256 MiB (the byte-counter calibration for --pmc)")
@@ -368,12 +369,10 @@ def cdf_leg(args):
                          "(docs/performance.md:99-105)"}
 
 
-def det_leg(args):
-    """The headline pour with Solver(deterministic=True): fixed-point integer scatter and
-    wrench sums, bit-reproducible. Reports its frame time against the float-atomic
-    headline and a checksum of the final positions (identical from run to run)."""
-    import hashlib
-
+def float_leg(args):
+    """The headline pour with Solver(deterministic=False): the float-atomic scatter (the
+    reference's accumulation) instead of the default 64-bit fixed-point integer one.
+    Frame time and fused-pass time for comparison with the headline."""
     import numpy as np
     import torch
 
@@ -382,9 +381,9 @@ def det_leg(args):
 
     sc = PourScene()
     s = sc.build(Solver(grid=sc.grid, device="cuda:0", sort_interval=args.sort_interval,
-                        sort_key=args.sort_key, deterministic=True))
+                        sort_key=args.sort_key, deterministic=False))
     n = s.n_particles
-    frames = max(1, args.det_frames)
+    frames = max(1, args.float_frames)
     for _ in range(args.warmup):
         sc.frame(s)
     torch.cuda.synchronize()
@@ -393,16 +392,12 @@ def det_leg(args):
         sc.frame(s)
     torch.cuda.synchronize()
     el = time.perf_counter() - t0
-    x = s.x()
-    if not np.isfinite(x).all():
-        raise RuntimeError("non-finite particle state after the deterministic frames")
-    # particle order depends only on the (deterministic) sort, so the raw bytes are comparable
-    digest = hashlib.sha256(np.ascontiguousarray(x).tobytes()).hexdigest()[:16]
+    if not np.isfinite(s.x()).all():
+        raise RuntimeError("non-finite particle state after the float-atomic frames")
     durations = held_frame_pass_times(s, sc)
     return {"value": round(n * frames * sc.substeps / el, 1), "unit": "particle-substeps/s",
             "ms_per_frame": round(el / frames * 1e3, 2), "frames": frames + args.warmup,
-            "fused_pass_avg_us": round(float(np.mean(durations)) * 1e3, 2),
-            "x_sha256_16": digest}
+            "fused_pass_avg_us": round(float(np.mean(durations)) * 1e3, 2)}
 
 
 def pour_leg(args):
@@ -431,26 +426,32 @@ def pour_leg(args):
         sc.frame(s)
     torch.cuda.synchronize()
     el = time.perf_counter() - t0
-    if not np.isfinite(s.x()).all():
+    x_end = s.x()
+    if not np.isfinite(x_end).all():
         raise RuntimeError("non-finite particle state after the timed frames")
+    # the default scatter is bit-reproducible: this digest repeats from run to run
+    import hashlib
+    digest = hashlib.sha256(np.ascontiguousarray(x_end).tobytes()).hexdigest()[:16]
     substeps = args.steps * sc.substeps
     value = n * substeps / el
-    # active nodes: nodes receiving mass in one scatter of the current state
-    sim.node_in.zero_()
+    # active nodes: nodes receiving mass in one scatter of the current state (the default
+    # scatter accumulates into the fixed-point buffer, the float one into node_in)
+    acc = sim.node_fix if sim.deterministic else sim.node_in
+    acc.zero_()
     sim._pass(native.PASS_STRESS | native.PASS_P2G, sc.dt, sim.stream())
-    active = int((sim.node_in.view(-1, 4)[:, 3] > 0).sum().item())
-    sim.node_in.zero_()
+    active = int((acc.view(-1, 4)[:, 3] > 0).sum().item())
+    acc.zero_()
     durations = held_frame_pass_times(s, sc)
     avg_s = float(np.mean(durations)) / 1e3
     algo = PASS_B_PER_PARTICLE * n + PASS_B_PER_NODE * active
     ach = algo / avg_s / 1e9
     roof = {"bound": "hbm", "achieved": round(ach, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": measured_traffic(),
-            "kernel": "k_particle_pass (fused g2p+stress+p2g)",
+            "kernel": "k_particle_pass (fused g2p+stress+p2g, fixed-point scatter)",
             "avg_launch_us": round(avg_s * 1e6, 2), "launches": len(durations),
             "algorithmic_bytes_per_launch": int(algo), "active_nodes": active}
     return {"value": round(value, 1), "ms_per_step": round(el / args.steps * 1e3, 3), "particles": n,
-            "substeps_per_step": sc.substeps, "dt": sc.dt, "roofline": roof}
+            "substeps_per_step": sc.substeps, "dt": sc.dt, "roofline": roof, "x_sha256_16": digest}
 
 
 def main():
@@ -485,12 +486,12 @@ def main():
         except Exception as e:  # reported in the line; the headline value stands
             batch = {"error": f"{type(e).__name__}: {e}"}
         torch.cuda.empty_cache()
-    det = None
-    if "det" in legs:
+    flt = None
+    if "float" in legs:
         try:
-            det = det_leg(args)
+            flt = float_leg(args)
         except Exception as e:  # reported in the line; the headline value stands
-            det = {"error": f"{type(e).__name__}: {e}"}
+            flt = {"error": f"{type(e).__name__}: {e}"}
         torch.cuda.empty_cache()
     cdf = None
     if "cdf" in legs:
@@ -507,9 +508,10 @@ def main():
                                   "source tilting 1 rad/s", "particles": pour.get("particles"),
                       "substeps_per_step": pour.get("substeps_per_step"), "dt": pour.get("dt"),
                       "grid": 192, "graph": not args.no_graph, "sort_interval": args.sort_interval,
-                      "sort_key": args.sort_key},
+                      "sort_key": args.sort_key, "scatter": "fixed-point (deterministic)",
+                      "x_sha256_16": pour.get("x_sha256_16")},
            "roofline": pour.get("roofline"), "cpu_baseline": cpu, "batched": batch, "adjoint": adjoint,
-           "cdf_pour": cdf, "deterministic": det}
+           "cdf_pour": cdf, "float_atomics": flt}
     print(json.dumps(out))
 
 

@@ -198,12 +198,18 @@ __global__ __launch_bounds__(kThreads) void k_grid_update(mpm_grid_t G, const mp
                                                           const mpm_collider_t* __restrict__ col, int ncoll,
                                                           int nslots, float* __restrict__ partials,
                                                           float* __restrict__ wrench, float dt, int flags) {
-    __shared__ float s_w[kThreads / 64][kMaxSlots * 6];
+    // per-wavefront wrench rows: float sums, or fixed-point integer sums (deterministic)
+    __shared__ unsigned long long s_wraw[kThreads / 64][kMaxSlots * 6];
+    float(*s_w)[kMaxSlots * 6] = reinterpret_cast<float(*)[kMaxSlots * 6]>(&s_wraw[0][0]);
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
     const bool lds_w = nslots <= kMaxSlots;
+    const bool fixed = (flags & MPM_GRID_FIXED_IN) != 0;
     if (lds_w)
-        for (int t = lane; t < nslots * 6; t += 64) s_w[wave][t] = 0.0f;
+        for (int t = lane; t < nslots * 6; t += 64) {
+            if (fixed) s_wraw[wave][t] = 0ull;
+            else s_w[wave][t] = 0.0f;
+        }
     const int nblk = *count;
     const int bps = G.nb[0] * G.nb[1] * G.nb[2];
     const int ncode = bps * G.n_scenes;
@@ -212,7 +218,6 @@ __global__ __launch_bounds__(kThreads) void k_grid_update(mpm_grid_t G, const mp
     const float damp = md->grid_v_damping_scale;
     const int blo0 = ctl->box_lo[0], blo1 = ctl->box_lo[1], blo2 = ctl->box_lo[2];
     const int bhi0 = ctl->box_hi[0], bhi1 = ctl->box_hi[1], bhi2 = ctl->box_hi[2];
-    const bool fixed = (flags & MPM_GRID_FIXED_IN) != 0;
     const double inv_m = fixed ? 1.0 / (double)G.fix_scale[0] : 0.0;
     const double inv_p = fixed ? 1.0 / (double)G.fix_scale[1] : 0.0;
     float sw_f = 0.f, sw_t = 0.f;  // fixed-point wrench scales (force, torque)
@@ -256,11 +261,17 @@ __global__ __launch_bounds__(kThreads) void k_grid_update(mpm_grid_t G, const mp
                 const float s0 = wave_sum(imp.x), s1 = wave_sum(imp.y), s2 = wave_sum(imp.z);
                 const float s3 = wave_sum(tq.x), s4 = wave_sum(tq.y), s5 = wave_sum(tq.z);
                 if (lane == 0 && fixed) {  // order-independent: integer sums of fixed point
-                    unsigned long long* w = reinterpret_cast<unsigned long long*>(partials) + 6 * (size_t)c.slot;
                     int* ef = G.error_flag;
-                    atomicAdd(w + 0, wrench_q(s0, sw_f, ef)); atomicAdd(w + 1, wrench_q(s1, sw_f, ef));
-                    atomicAdd(w + 2, wrench_q(s2, sw_f, ef)); atomicAdd(w + 3, wrench_q(s3, sw_t, ef));
-                    atomicAdd(w + 4, wrench_q(s4, sw_t, ef)); atomicAdd(w + 5, wrench_q(s5, sw_t, ef));
+                    const unsigned long long q[6] = {wrench_q(s0, sw_f, ef), wrench_q(s1, sw_f, ef),
+                                                     wrench_q(s2, sw_f, ef), wrench_q(s3, sw_t, ef),
+                                                     wrench_q(s4, sw_t, ef), wrench_q(s5, sw_t, ef)};
+                    if (lds_w) {
+                        unsigned long long* w = &s_wraw[wave][6 * c.slot];
+                        for (int e = 0; e < 6; ++e) w[e] += q[e];
+                    } else {
+                        unsigned long long* w = reinterpret_cast<unsigned long long*>(partials) + 6 * (size_t)c.slot;
+                        for (int e = 0; e < 6; ++e) atomicAdd(w + e, q[e]);
+                    }
                 } else if (lane == 0) {
                     if (lds_w) {
                         float* w = &s_w[wave][6 * c.slot];
@@ -277,12 +288,18 @@ __global__ __launch_bounds__(kThreads) void k_grid_update(mpm_grid_t G, const mp
         if (valid && !fixed && (flags & MPM_GRID_ZERO_IN))
             *reinterpret_cast<float4*>(G.node_in + o) = make_float4(0.f, 0.f, 0.f, 0.f);
     }
-    if (nslots > 0 && lds_w && !fixed) {
+    if (nslots > 0 && lds_w) {
         __syncthreads();
         for (int t = threadIdx.x; t < nslots * 6; t += kThreads) {
-            float acc = s_w[0][t];
-            for (int w = 1; w < kThreads / 64; ++w) acc += s_w[w][t];
-            partials[(size_t)blockIdx.x * nslots * 6 + t] = acc;
+            if (fixed) {
+                unsigned long long acc = 0ull;
+                for (int w = 0; w < kThreads / 64; ++w) acc += s_wraw[w][t];
+                reinterpret_cast<unsigned long long*>(partials)[(size_t)blockIdx.x * nslots * 6 + t] = acc;
+            } else {
+                float acc = s_w[0][t];
+                for (int w = 1; w < kThreads / 64; ++w) acc += s_w[w][t];
+                partials[(size_t)blockIdx.x * nslots * 6 + t] = acc;
+            }
         }
     }
 }
@@ -304,17 +321,28 @@ __global__ __launch_bounds__(kThreads) void k_wrench_reduce(const float* __restr
     if (threadIdx.x == 0) wrench[v] += sh[0];
 }
 
-// deterministic wrench: wrench[v] += fixed[v] / scale, then fixed[v] = 0; one thread per
-// value, so the float result does not depend on any ordering
-__global__ void k_wrench_fix_flush(mpm_grid_t G, unsigned long long* __restrict__ fixed, int nvals,
-                                   float* __restrict__ wrench) {
-    const int v = blockIdx.x * blockDim.x + threadIdx.x;
-    if (v >= nvals) return;
-    float sf, st;
-    wrench_scales(G, sf, st);
-    const double inv = 1.0 / (double)((v % 6) < 3 ? sf : st);
-    wrench[v] += (float)((double)(long long)fixed[v] * inv);
-    fixed[v] = 0ull;
+// deterministic wrench: wrench[v] += (sum over the nparts integer partial rows of value v)
+// / scale; with nparts == 1 the single row (global-atomic path) is cleared after use.
+// Integer sums: the float result does not depend on any ordering.
+__global__ __launch_bounds__(kThreads) void k_wrench_fix_flush(mpm_grid_t G, unsigned long long* __restrict__ fixed,
+                                                                int nparts, int nvals, float* __restrict__ wrench) {
+    __shared__ unsigned long long sh[kThreads];
+    const int v = blockIdx.x;  // one workgroup per value: strided sums, then an LDS tree
+    unsigned long long acc = 0ull;
+    for (int b = threadIdx.x; b < nparts; b += kThreads) acc += fixed[(size_t)b * nvals + v];
+    sh[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = kThreads / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        float sf, st;
+        wrench_scales(G, sf, st);
+        const double inv = 1.0 / (double)((v % 6) < 3 ? sf : st);
+        wrench[v] += (float)((double)(long long)sh[0] * inv);
+        if (nparts == 1) fixed[v] = 0ull;
+    }
 }
 
 inline unsigned grid_update_wgs(int capacity) { return (unsigned)min((capacity + 3) / 4, kGridWgs); }
@@ -631,7 +659,7 @@ int32_t mpm_blocks_from_box(const mpm_grid_t* G, mpm_tick_ctl_t* ctl, int32_t* l
 int32_t mpm_grid_workspace_floats(int32_t capacity, int32_t wrench_slots) {
     if (capacity < 0 || wrench_slots < 0) return -1;
     if (wrench_slots > kMaxSlots) return 12 * wrench_slots;  // atomic path; fixed-point wrench only
-    return (int32_t)max(grid_update_wgs(capacity) * wrench_slots * 6, 12u * wrench_slots);
+    return (int32_t)(grid_update_wgs(capacity) * wrench_slots * 12);  // float or int64 partial rows
 }
 
 int32_t mpm_grid_update(const mpm_grid_t* G, const mpm_model_t* model, const mpm_tick_ctl_t* ctl,
@@ -651,8 +679,9 @@ int32_t mpm_grid_update(const mpm_grid_t* G, const mpm_model_t* model, const mpm
     hipLaunchKernelGGL(k_grid_update, dim3(nwg), dim3(kThreads), 0, s, *G, model, ctl, list, count, colliders, ncoll,
                        wrench_slots, workspace, wrench, dt, flags);
     if (wrench_slots > 0 && fixed)
-        hipLaunchKernelGGL(k_wrench_fix_flush, dim3((wrench_slots * 6 + kThreads - 1) / kThreads), dim3(kThreads), 0,
-                           s, *G, reinterpret_cast<unsigned long long*>(workspace), wrench_slots * 6, wrench);
+        hipLaunchKernelGGL(k_wrench_fix_flush, dim3(wrench_slots * 6), dim3(kThreads), 0, s, *G,
+                           reinterpret_cast<unsigned long long*>(workspace), fixed_order ? (int)nwg : 1,
+                           wrench_slots * 6, wrench);
     else if (wrench_slots > 0 && fixed_order)
         hipLaunchKernelGGL(k_wrench_reduce, dim3(wrench_slots * 6), dim3(kThreads), 0, s, workspace, (int)nwg,
                            wrench_slots * 6, wrench);

@@ -60,6 +60,8 @@ class Rollout:
             raise NotImplementedError("periodic domains have no adjoint")
         if sim.cdf is not None and sim.cdf.n > 0:
             raise NotImplementedError("CPIC (CDF) colliders have no adjoint")
+        if sim.deterministic:  # the adjoint reads the float scatter result (node_in)
+            sim.disable_deterministic()
         if len(sim.colliders.items) > 16:
             raise ValueError("at most 16 colliders in a differentiable rollout")
         self.solver = solver

@@ -373,14 +373,22 @@ class HipSimulator:
         self._graph = None
         self._structs()
 
+    def disable_deterministic(self):
+        """Back to float-atomic accumulation of the scatter and the wrench."""
+        self.deterministic = False
+        self.grid_ws = None
+        self._graph = None
+        self._structs()
+
     def _fix_scales(self, dt: float):
         """Powers of two such that the largest expected contribution is about 2^40 units
         (2^12 below the per-contribution limit): mass by the heaviest particle, momentum by
-        it times (max particle speed + dx/dt), the speed a stable substep cannot exceed."""
+        it times 2 dx/dt (a stable explicit substep moves nothing a cell; the scale then
+        depends on dt and the masses only, so captured graphs stay valid tick to tick)."""
         if self._m_max is None:
             self._m_max = float(self.mass.max().item()) if self.n_particles else 1.0
         m = max(self._m_max, 1e-30)
-        p = m * (self._vmax + self.dx / dt)
+        p = m * 2.0 * self.dx / dt
         exp = (40 - math.ceil(math.log2(m)), 40 - math.ceil(math.log2(p)))
         if exp != self._fix_exp:
             self._fix_exp = exp
@@ -697,8 +705,10 @@ class Solver:
     # cell, so that the particle pass can sum same-cell particles before its LDS atomics
     sort_key: str = "block"
     # no counterpart in the reference: 64-bit fixed-point integer accumulation of the
-    # scatter and the wrench, bit-reproducible from run to run
-    deterministic: bool = False
+    # scatter and the wrench, bit-reproducible from run to run (and faster than float
+    # atomics on gfx950). None = on unless a CDF collider is added (its blocked-deposit
+    # wrench needs the float path); True = on, CDF colliders refused; False = float atomics.
+    deterministic: bool | None = None
     _sim: Any = field(default=None, init=False, repr=False)
     _step: int = field(default=0, init=False, repr=False)
     _tick: int = field(default=0, init=False, repr=False)
@@ -730,7 +740,7 @@ class Solver:
             self._sim._model_dirty = True
         if cov6 is not None and cov_mode == "step":
             self._sim.enable_cov_step()
-        if self.deterministic:
+        if self.deterministic is not False:
             self._sim.enable_deterministic()
         return self
 
@@ -991,6 +1001,8 @@ class Solver:
         if self.deterministic:
             raise NotImplementedError("the deterministic scatter does not cover CDF colliders")
         sim = self._sim
+        if sim.deterministic:  # deterministic=None: CDF scenes take the float-atomic scatter
+            sim.disable_deterministic()
         return sim.cdf_colliders().add(
             cdf.values, cdf.valid, cdf.origin, cdf.cell, cdf.band, center, quat, velocity, omega,
             band, surface, friction, start_time, end_time, sim.time)

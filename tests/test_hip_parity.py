@@ -50,13 +50,15 @@ def _c1(solver_cls, **kw):
     return s
 
 
-def test_c1_grid_after_one_substep():
+@pytest.mark.parametrize("deterministic", [False, True])
+def test_c1_grid_after_one_substep(deterministic):
     """Grid mass and momentum after one substep's scatter, within 1e-5 relative, from a
     non-trivial seeded state of the C1 cube (random velocity, affine C and deformation F,
-    so the momentum, APIC and stress terms all contribute)."""
+    so the momentum, APIC and stress terms all contribute), for the float-atomic scatter
+    and for the fixed-point one (read back from its integer accumulators)."""
     from oracle import sim as O
 
-    hip = _c1(Solver, device="hip")
+    hip = _c1(Solver, device="hip", deterministic=deterministic)
     ora = _c1(OracleSolver)
     n = hip.n_particles
     rng = np.random.default_rng(11)
@@ -75,7 +77,15 @@ def test_c1_grid_after_one_substep():
     dt = 1e-4
     sim.node_in.zero_()
     sim._upload_model()
+    if deterministic:
+        sim._fix_scales(dt)
     sim._pass(6, dt, sim.stream())  # stress + P2G, the tick prologue
+    if deterministic:  # integer accumulators -> the float layout of node_in, for reading
+        q = sim.node_fix.view(-1, 4).double()
+        sm, sp = (2.0 ** e for e in sim._fix_exp)
+        sim.node_in.view(-1, 4)[:, :3] = (q[:, :3] / sp).float()
+        sim.node_in.view(-1, 4)[:, 3] = (q[:, 3] / sm).float()
+        sim.node_fix.zero_()
     m, vi, _ = sim.grid_dense()
     O.lib().o_compute_stress(o._state(), o._model(), dt)
     O.lib().o_p2g(o._state(), o._model(), dt)
@@ -89,8 +99,9 @@ def test_c1_grid_after_one_substep():
                                atol=1e-6 * np.abs(mom).max())
 
 
-def test_c1_positions_after_100_substeps():
-    hip = _c1(Solver, device="hip")
+@pytest.mark.parametrize("deterministic", [False, True])
+def test_c1_positions_after_100_substeps(deterministic):
+    hip = _c1(Solver, device="hip", deterministic=deterministic)
     ora = _c1(OracleSolver)
     hip.step(1e-4, 100)
     ora.step(1e-4, 100)
