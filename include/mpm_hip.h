@@ -271,9 +271,9 @@ int32_t mpm_blocks_from_box(const mpm_grid_t *G, mpm_tick_ctl_t *ctl, int32_t *b
  * least mpm_grid_workspace_floats(capacity, wrench_slots)) with a fixed-order reduction,
  * so the readout is bit-reproducible; with more slots (batched scenes) per-wavefront sums
  * are added with float atomics and `workspace` may be NULL. With MPM_GRID_FIXED_IN the
- * per-wavefront sums go to `workspace` as 64-bit fixed point (integer atomics, zero
- * between calls, order-independent) and are added to `wrench` after the sweep, for any
- * slot count.
+ * per-wavefront sums are rounded to 64-bit fixed point and summed as integers (per
+ * workgroup rows in `workspace`, or above 64 slots integer atomics into a row that is zero
+ * between calls), then added to `wrench`: order-independent, for any slot count.
  * A block of scene s applies colliders[s * ncoll .. s * ncoll + ncoll). Nodes without mass above 1e-15 read velocity 0 (the
  * split-zero contract, mpm_utils.py:1193-1202). With MPM_GRID_ZERO_IN, node_in is
  * cleared afterwards. */
@@ -333,6 +333,20 @@ int32_t mpm_permute(const mpm_particles_t *P, const int32_t *perm, float *scratc
 int32_t mpm_box_contact_workspace_floats(void);
 int32_t mpm_box_contact_wrench(const mpm_particles_t *P, const float *box, float layer,
                                int32_t compressive_only, float *out, float *workspace, void *stream);
+
+/* ---- mesh voxelisation (geometry/mesh_sdf.py:328-366, 412-481) ----------------------- */
+/* Fields of a triangle mesh (verts device float[3 nv], faces device int32[3 nf]) on the
+ * cubic voxel grid origin + cell * (i, j, k), i, j, k < res (origin: host float[3]), z
+ * fastest (dist and aux device float[res^3]). dist = exact unsigned distance to the nearest triangle. mode 0: aux =
+ * winding number (sum of solid angles / 4 pi: ~1 inside a closed outward-oriented mesh, ~0
+ * outside); mode 1: aux = +1 / -1, the side of the nearest triangle's normal (for open
+ * sheets). A face with a vertex index outside [0, nv) is ignored. */
+int32_t mpm_mesh_field(const float *verts, int32_t nv, const int32_t *faces, int32_t nf,
+                       const float *origin, float cell, int32_t res, int32_t mode, float *dist,
+                       float *aux, void *stream);
+/* dist = distance from each voxel to the nearest of ns segments (device float[6 ns]: a, b) */
+int32_t mpm_segment_distance(const float *segments, int32_t ns, const float *origin, float cell,
+                             int32_t res, float *dist, void *stream);
 
 /* ---- exports ------------------------------------------------------------------------ */
 /* compute_R_from_F (mpm_utils.py:1261-1285): R_out[9*p + ...] = (U V^T)^T from F_trial,

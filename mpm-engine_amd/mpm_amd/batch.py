@@ -55,8 +55,11 @@ class BatchSolver:
 
     def __init__(self, grid: GridConfig | None = None, device: str = "auto", sparse: bool = True,
                  fused: bool = True, sort_interval: int = 1, guard_interval: int = 1,
-                 wrench_slots_per_scene: int = 2):
+                 wrench_slots_per_scene: int = 2, deterministic: bool = True):
         self.grid = grid or GridConfig()
+        # 64-bit fixed-point scatter and wrench sums (bit-reproducible, and faster than
+        # float atomics here); False = float atomics
+        self.deterministic = bool(deterministic)
         self.device = device
         self.sparse = sparse
         self.fused = fused
@@ -88,6 +91,8 @@ class BatchSolver:
         sim = self._sim
         sim.scene_colliders = SceneColliders(len(counts), self.grid.n_grid, sim.dx,
                                              self.slots_per_scene)
+        if self.deterministic:
+            sim.enable_deterministic()
         torch = sim.torch
         # load index of the particle in each slot (the block sort permutes slots)
         self._orig = torch.arange(len(pos), dtype=torch.int64, device=sim.device)
@@ -380,6 +385,8 @@ class BatchSolver:
         saved_si, saved_tick = self.sort_interval, self._tick
         snap = self._snapshot()
         result = {"shapes": {}, "sort": {}}
+        if sim.deterministic:  # the fixed-point scatter runs 256-lane workgroups only
+            shapes = tuple(sh for sh in shapes if sh[0] == 256) or ((256, 1),)
         try:
             best = None
             for wg, ppt in shapes:

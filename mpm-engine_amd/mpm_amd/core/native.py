@@ -134,6 +134,8 @@ SIGNATURES = {
     "mpm_cdf_set_lanes": (_I32, [_P, _P, _P]),
     "mpm_box_contact_workspace_floats": (_I32, []),
     "mpm_box_contact_wrench": (_I32, [_P, _P, _F, _I32, _P, _P, _P]),
+    "mpm_mesh_field": (_I32, [_P, _I32, _P, _I32, _P, _F, _I32, _I32, _P, _P, _P]),
+    "mpm_segment_distance": (_I32, [_P, _I32, _P, _F, _I32, _P, _P]),
     "mpm_adjoint_g2p": (_I32, [_P, _P, _P, _P, _F, _P]),
     "mpm_adjoint_grid": (_I32, [_P, _P, _P, _I32, _P, _I32, _P, _F, _P]),
     "mpm_adjoint_p2g_stress": (_I32, [_P, _P, _P, _P, _F, _P]),

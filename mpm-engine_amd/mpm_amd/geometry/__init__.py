@@ -2,9 +2,12 @@
 
 ``SDFData`` has the reference's field layout (src/warpmpm/geometry/mesh_sdf.py:26-45):
 values[res, res, res] (negative inside), grads[res, res, res, 3], origin (3,), cell, sdf_max,
-all in the collider body frame, z fastest. The mesh -> SDF winding-number builder of the
-reference is not part of this package; ``sdf_from_function`` samples any analytic SDF
-(and its central-difference gradient) onto that layout.
+all in the collider body frame, z fastest. ``build_sdf`` voxelises a closed triangle mesh
+(unsigned distance + winding-number sign, the reference's builder,
+src/warpmpm/geometry/mesh_sdf.py:328-366), ``build_surface_cdf`` an open sheet
+(:412-481); both run on the host (numpy, device="cpu") or in the HIP kernels
+mpm_mesh_field / mpm_segment_distance (device="hip", exact distances).
+``sdf_from_function`` samples any analytic SDF onto the same layout.
 """
 from __future__ import annotations
 
@@ -41,6 +44,133 @@ def sdf_from_function(fn, lo, hi, res: int) -> SDFData:
                      axis=-1)
     return SDFData(values=vals.astype(np.float32), grads=grads.astype(np.float32),
                    origin=lo.astype(np.float32), cell=cell, sdf_max=float(np.abs(vals).max()))
+
+
+def _cubic_grid(verts, res, margin_cells):
+    lo, hi = verts.min(axis=0), verts.max(axis=0)
+    cell = float((hi - lo).max()) / (res - 1 - 2 * margin_cells)
+    origin = 0.5 * (lo + hi) - 0.5 * cell * (res - 1)
+    ax = [origin[d] + cell * np.arange(res) for d in range(3)]
+    return origin, cell, np.stack(np.meshgrid(*ax, indexing="ij"), axis=-1).reshape(-1, 3)
+
+
+def winding_number(points, verts, faces, chunk: int = 4096) -> np.ndarray:
+    """Generalised winding number of a triangle mesh at each point: the sum of the solid
+    angles of its triangles (Van Oosterom & Strackee 1983) over 4 pi; ~1 inside a closed
+    outward-oriented mesh, ~0 outside."""
+    pts = np.asarray(points, dtype=np.float64).reshape(-1, 3)
+    tri = np.asarray(verts, dtype=np.float64)[np.asarray(faces, dtype=np.int64)]
+    out = np.empty(len(pts))
+    for s in range(0, len(pts), chunk):
+        p = pts[s:s + chunk, None, None, :]
+        x, y, z = tri[None, :, 0] - p[:, :, 0], tri[None, :, 1] - p[:, :, 0], tri[None, :, 2] - p[:, :, 0]
+        lx, ly, lz = (np.linalg.norm(v, axis=-1) for v in (x, y, z))
+        num = np.einsum("pfi,pfi->pf", x, np.cross(y, z))
+        den = (lx * ly * lz + np.einsum("pfi,pfi->pf", x, y) * lz + np.einsum("pfi,pfi->pf", x, z) * ly
+               + np.einsum("pfi,pfi->pf", y, z) * lx)
+        out[s:s + chunk] = (2.0 * np.arctan2(num, den)).sum(axis=1) / (4.0 * np.pi)
+    return out
+
+
+def _hip_field(verts, faces, origin, cell, res, mode):
+    """Run mpm_mesh_field on the GPU; returns (dist, aux) as float64 res^3 arrays."""
+    import ctypes as C
+
+    import torch
+
+    from ..core import native
+
+    lib = native.load()
+    dev = torch.device("cuda")
+    v = torch.from_numpy(np.ascontiguousarray(verts, dtype=np.float32).reshape(-1)).to(dev)
+    f = torch.from_numpy(np.ascontiguousarray(faces, dtype=np.int32).reshape(-1)).to(dev)
+    dist = torch.empty(res ** 3, dtype=torch.float32, device=dev)
+    aux = torch.empty(res ** 3, dtype=torch.float32, device=dev)
+    o = (C.c_float * 3)(*[float(t) for t in origin])
+    stream = torch.cuda.current_stream(dev).cuda_stream
+    native.check(lib.mpm_mesh_field(native.ptr(v), len(verts), native.ptr(f), len(faces), o, float(cell), int(res),
+                                    int(mode), native.ptr(dist), native.ptr(aux), stream), "mesh_field")
+    return (dist.cpu().numpy().astype(np.float64).reshape(res, res, res),
+            aux.cpu().numpy().astype(np.float64).reshape(res, res, res))
+
+
+def _hip_segments(segments, origin, cell, res):
+    import ctypes as C
+
+    import torch
+
+    from ..core import native
+
+    lib = native.load()
+    dev = torch.device("cuda")
+    sg = torch.from_numpy(np.ascontiguousarray(segments, dtype=np.float32).reshape(-1)).to(dev)
+    dist = torch.empty(res ** 3, dtype=torch.float32, device=dev)
+    o = (C.c_float * 3)(*[float(t) for t in origin])
+    stream = torch.cuda.current_stream(dev).cuda_stream
+    native.check(lib.mpm_segment_distance(native.ptr(sg), len(segments), o, float(cell), int(res), native.ptr(dist),
+                                          stream), "segment_distance")
+    return dist.cpu().numpy().astype(np.float64).reshape(res, res, res)
+
+
+def build_sdf(verts, faces, res: int = 64, margin_cells: float = 4.0, interior_probe=None,
+              device: str = "cpu") -> SDFData:
+    """Voxelise a closed triangle mesh into an SDFData (negative inside) on a cubic grid
+    spanning its bounding box plus `margin_cells` cells (mesh_sdf.py:328-366): unsigned
+    distance to the surface, sign from the winding number, the global orientation fixed by
+    the winding number at `interior_probe` (default the bounding-box centre). device="cpu":
+    distances to a dense surface sample (0.4-cell spacing), numpy winding numbers;
+    device="hip": exact point-triangle distances and winding numbers in mpm_mesh_field."""
+    verts = np.asarray(verts, dtype=np.float64)
+    faces = np.asarray(faces, dtype=np.int64)
+    if faces.size == 0 or faces.min() < 0 or faces.max() >= len(verts):
+        raise ValueError("faces must index the vertex array")
+    origin, cell, grid = _cubic_grid(verts, res, margin_cells)
+    if device == "hip":
+        dist, wn = _hip_field(verts, faces, origin, cell, res, mode=0)
+    elif device == "cpu":
+        from scipy.spatial import cKDTree
+
+        samples, _ = _triangle_samples(verts, faces, 0.4 * cell)
+        dist = cKDTree(samples).query(grid, workers=-1)[0].reshape(res, res, res)
+        wn = winding_number(grid, verts, faces).reshape(res, res, res)
+    else:
+        raise ValueError("device must be 'cpu' or 'hip'")
+    lo, hi = verts.min(axis=0), verts.max(axis=0)
+    probe = 0.5 * (lo + hi) if interior_probe is None else np.asarray(interior_probe, dtype=np.float64)
+    w_probe = float(winding_number(probe[None], verts, faces)[0])
+    inside = (wn * np.sign(w_probe)) > 0.5 if abs(w_probe) > 0.1 else np.abs(wn) > 0.5
+    signed = np.where(inside, -dist, dist)
+    grads = np.stack(np.gradient(signed, cell, edge_order=2), axis=-1)
+    return SDFData(values=signed, grads=grads, origin=origin, cell=float(cell),
+                   sdf_max=float(np.abs(signed).max()))
+
+
+def icosphere(radius: float = 1.0, subdivisions: int = 2):
+    """Triangulated sphere (outward faces): (verts, faces)."""
+    t = (1.0 + 5.0 ** 0.5) / 2.0
+    v = [(-1, t, 0), (1, t, 0), (-1, -t, 0), (1, -t, 0), (0, -1, t), (0, 1, t), (0, -1, -t), (0, 1, -t),
+         (t, 0, -1), (t, 0, 1), (-t, 0, -1), (-t, 0, 1)]
+    f = [(0, 11, 5), (0, 5, 1), (0, 1, 7), (0, 7, 10), (0, 10, 11), (1, 5, 9), (5, 11, 4), (11, 10, 2),
+         (10, 7, 6), (7, 1, 8), (3, 9, 4), (3, 4, 2), (3, 2, 6), (3, 6, 8), (3, 8, 9), (4, 9, 5), (2, 4, 11),
+         (6, 2, 10), (8, 6, 7), (9, 8, 1)]
+    verts = [np.array(p, dtype=np.float64) / np.linalg.norm(p) for p in v]
+    faces = list(f)
+    for _ in range(subdivisions):
+        cache, nf = {}, []
+
+        def mid(a, b):
+            key = (min(a, b), max(a, b))
+            if key not in cache:
+                m = verts[a] + verts[b]
+                verts.append(m / np.linalg.norm(m))
+                cache[key] = len(verts) - 1
+            return cache[key]
+
+        for a, b, c in faces:
+            ab, bc, ca = mid(a, b), mid(b, c), mid(c, a)
+            nf += [(a, ab, ca), (b, bc, ab), (c, ca, bc), (ab, bc, ca)]
+        faces = nf
+    return radius * np.array(verts), np.array(faces, dtype=np.int64)
 
 
 def sphere_sdf(radius: float, res: int = 32, margin: float = 4.0) -> SDFData:
@@ -115,7 +245,7 @@ def _edge_samples(verts, edges, spacing):
 
 
 def build_surface_cdf(verts, faces, res: int = 64, band_cells: float = 3.0,
-                      margin_cells: float = 4.0) -> CDFData:
+                      margin_cells: float = 4.0, device: str = "cpu") -> CDFData:
     """Voxelise an open oriented triangle surface into a CDFData.
 
     The cubic voxel grid spans the surface's bounding box plus `margin_cells` cells each
@@ -124,11 +254,26 @@ def build_surface_cdf(verts, faces, res: int = 64, band_cells: float = 3.0,
     that point's face normal; it is valid when that distance is within band =
     band_cells * cell and the distance to the open rim (edges used by one face) exceeds
     the band, since no consistent side exists past the rim. Voxels farther than
-    2 band + 2 cells from the surface are invalid and hold that distance, positive."""
+    2 band + 2 cells from the surface are invalid and hold that distance, positive.
+    device="hip" computes exact point-triangle distances, the nearest triangle's side and
+    the rim distance in mpm_mesh_field / mpm_segment_distance instead (no reach cut-off)."""
     from scipy.spatial import cKDTree
 
     verts = np.asarray(verts, dtype=np.float64)
     faces = np.asarray(faces, dtype=np.int64)
+    if device == "hip":
+        if faces.size == 0 or faces.min() < 0 or faces.max() >= len(verts):
+            raise ValueError("faces must index the vertex array")
+        origin, cell, _ = _cubic_grid(verts, res, margin_cells)
+        band = band_cells * cell
+        dist, side = _hip_field(verts, faces, origin, cell, res, mode=1)
+        edges = open_edges(faces)
+        rim = (_hip_segments(verts[edges].reshape(-1, 6), origin, cell, res) if len(edges)
+               else np.full((res, res, res), np.inf))
+        valid = ((dist <= band) & (rim > band)).astype(np.float64)
+        return CDFData(values=side * dist, valid=valid, origin=origin, cell=cell, band=band)
+    if device != "cpu":
+        raise ValueError("device must be 'cpu' or 'hip'")
     lo, hi = verts.min(axis=0), verts.max(axis=0)
     cell = float((hi - lo).max()) / (res - 1 - 2 * margin_cells)
     band = band_cells * cell
@@ -197,4 +342,5 @@ def load_cdf(path) -> CDFData:
 
 
 __all__ = ["SDFData", "sdf_from_function", "sphere_sdf", "CDFData", "build_surface_cdf",
-           "square_sheet", "revolve_profile_open", "open_edges", "save_cdf", "load_cdf"]
+           "square_sheet", "revolve_profile_open", "open_edges", "save_cdf", "load_cdf",
+           "build_sdf", "winding_number", "icosphere"]
