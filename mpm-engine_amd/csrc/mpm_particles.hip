@@ -240,10 +240,15 @@ MPM_DEV bool stencil_in_grid(const mpm_grid_t& G, const Stencil& st, bool period
 // other (one particle per thread per chunk).
 constexpr int kXcds = 8;
 
-// amdgpu_waves_per_eu(4): keep the allocation within 128 VGPRs (4 wavefronts per SIMD);
-// left alone the scheduler lands between 110 and 200 depending on unrelated edits.
+// amdgpu_waves_per_eu(3): keep the allocation within 168 VGPRs (3 wavefronts per SIMD; left
+// alone the scheduler lands anywhere between 110 and 200 depending on unrelated edits). At
+// 128 VGPRs (4 per SIMD) the fixed-point scatter spills ~21 dwords and runs 5-6 % slower on
+// the pour; with 3 nothing spills and the extra wavefront is not missed.
+#ifndef MPM_PASS_WAVES_PER_EU
+#define MPM_PASS_WAVES_PER_EU 3
+#endif
 template <int MODE, bool kSolid, int kT, bool kCdf, bool kDet>
-__global__ __launch_bounds__(kT) __attribute__((amdgpu_waves_per_eu(4))) void k_particle_pass(mpm_particles_t P, mpm_grid_t G,
+__global__ __launch_bounds__(kT) __attribute__((amdgpu_waves_per_eu(MPM_PASS_WAVES_PER_EU))) void k_particle_pass(mpm_particles_t P, mpm_grid_t G,
                                                       const mpm_model_t* __restrict__ md, float dt,
                                                       int xcd_q, int ppt) {
     constexpr int kArena = arena_of<kT>();
