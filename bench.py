@@ -82,9 +82,44 @@ def cpu_baseline(substeps: int):
     s.step(sc.dt, substeps)
     el = time.perf_counter() - t0
     n = len(sc.pos)
-    return {"value": n * substeps / el, "unit": "particle-substeps/s", "cores": 1, "kind": "port",
-            "sample": f"honey pour 192^3, {n} particles, {substeps} substeps of one frame "
-                      f"({el:.1f} s, single-threaded C oracle)"}
+    out = {"value": n * substeps / el, "unit": "particle-substeps/s", "cores": 1, "kind": "port",
+           "sample": f"honey pour 192^3, {n} particles, {substeps} substeps of one frame "
+                     f"({el:.1f} s, single-threaded C oracle)"}
+    try:  # the same oracle built with OpenMP, on every host core this process may use
+        out["threaded"] = cpu_baseline_threaded(4 * substeps)
+    except Exception as e:  # reported beside the single-threaded figure
+        out["threaded"] = {"error": f"{type(e).__name__}: {e}"}
+    return out
+
+
+def cpu_baseline_threaded(substeps: int):
+    """The oracle's OpenMP build (particle loops over all threads, atomic scatter) on the
+    same pour; threads = OMP_NUM_THREADS (or the CPU count)."""
+    from mpm_amd.scenes import PourScene
+    from oracle import sim as osim
+
+    threads = int(os.environ.get("OMP_NUM_THREADS") or os.cpu_count() or 1)
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as fh:
+            model = next((ln.split(":", 1)[1].strip() for ln in fh if ln.startswith("model name")), "")
+    except OSError:
+        pass
+    osim.use_threaded_build(True)
+    try:
+        sc = PourScene()
+        s = sc.build(osim.OracleSolver(sc.grid))
+        t0 = time.perf_counter()
+        s.set_cup(sc.src, center=sc.center, quat=sc.quat, velocity=[0, 0, 0], omega=sc.omega)
+        s.step(sc.dt, substeps)
+        el = time.perf_counter() - t0
+    finally:
+        osim.use_threaded_build(False)
+    n = len(sc.pos)
+    return {"value": n * substeps / el, "unit": "particle-substeps/s", "cores": threads,
+            "cpu_model": model, "kind": "port",
+            "sample": f"honey pour 192^3, {n} particles, {substeps} substeps of one frame ({el:.1f} s, "
+                      f"OpenMP build of the C oracle)"}
 
 
 def batch_leg(args):

@@ -18,6 +18,18 @@
 
 #define EXPORT __attribute__((visibility("default")))
 
+/* Built twice: libmpm_oracle.so (the checker: single-threaded, particle order) and, with
+ * -fopenmp -DO_OPENMP, libmpm_oracle_omp.so, used only to time the CPU baseline on every
+ * host core: particle loops split over threads, scatter adds atomic (summation order then
+ * varies, so that build is never the checker). */
+#ifdef O_OPENMP
+#define O_PARALLEL_FOR _Pragma("omp parallel for schedule(static)")
+#define O_ADD(lhs, val) _Pragma("omp atomic") lhs += (val)
+#else
+#define O_PARALLEL_FOR
+#define O_ADD(lhs, val) lhs += (val)
+#endif
+
 /* ---------------------------------------------------------------- small linear algebra */
 typedef struct { float a[9]; } m33;
 typedef struct { float c[3]; } v3;
@@ -708,16 +720,16 @@ static void p2g_particle(OState *st, const OModel *md, float dt, int p) {
                         v3 xw = vscale(V3((float)ix, (float)iy, (float)iz), md->dx);
                         v3 tq = vcross(vsub(xw, vload(cd->lane_center[own])), add);
                         for (int c = 0; c < 3; ++c) {
-                            cd->reaction_force[own][c] += add.c[c];
-                            cd->reaction_torque[own][c] += tq.c[c];
+                            O_ADD(cd->reaction_force[own][c], add.c[c]);
+                            O_ADD(cd->reaction_torque[own][c], tq.c[c]);
                         }
                         continue;
                     }
                 }
-                st->grid_v_in[3 * g + 0] += add.c[0];
-                st->grid_v_in[3 * g + 1] += add.c[1];
-                st->grid_v_in[3 * g + 2] += add.c[2];
-                st->grid_m[g] += weight * mass;
+                O_ADD(st->grid_v_in[3 * g + 0], add.c[0]);
+                O_ADD(st->grid_v_in[3 * g + 1], add.c[1]);
+                O_ADD(st->grid_v_in[3 * g + 2], add.c[2]);
+                O_ADD(st->grid_m[g], weight * mass);
             }
 }
 
@@ -787,18 +799,21 @@ EXPORT void o_compute_mu_lam(OState *st, OModel *md) {
 
 /* compute_stress_from_F_trial, mpm_utils.py:1152-1158 */
 EXPORT void o_compute_stress(OState *st, OModel *md, float dt) {
+    O_PARALLEL_FOR
     for (int p = 0; p < st->n; ++p)
         if (st->selection[p] == 0) stress_update_particle(st, md, dt, p);
 }
 
 /* p2g_apic_with_stress, mpm_utils.py:919-923 */
 EXPORT void o_p2g(OState *st, OModel *md, float dt) {
+    O_PARALLEL_FOR
     for (int p = 0; p < st->n; ++p)
         if (st->selection[p] == 0) p2g_particle(st, md, dt, p);
 }
 
 /* g2p, mpm_utils.py:1046-1052 (stationary/rigid 7, 8 skipped) */
 EXPORT void o_g2p(OState *st, OModel *md, float dt) {
+    O_PARALLEL_FOR
     for (int p = 0; p < st->n; ++p) {
         if (st->selection[p] != 0) continue;
         int mat = st->material[p];
@@ -809,6 +824,7 @@ EXPORT void o_g2p(OState *st, OModel *md, float dt) {
 
 /* g2p_stress_p2g, mpm_utils.py:1161-1178 */
 EXPORT void o_g2p_stress_p2g(OState *st, OModel *md, float dt) {
+    O_PARALLEL_FOR
     for (int p = 0; p < st->n; ++p) {
         if (st->selection[p] != 0) continue;
         int mat = st->material[p];

@@ -126,13 +126,23 @@ class OPointCloud(C.Structure):
 
 
 _LIB = None
+_LIB_NAME = "libmpm_oracle.so"
+
+
+def use_threaded_build(on: bool) -> None:
+    """Switch to the OpenMP build (every host core, atomic scatter) for CPU-baseline timing,
+    or back to the single-threaded checker build."""
+    global _LIB, _LIB_NAME
+    name = "libmpm_oracle_omp.so" if on else "libmpm_oracle.so"
+    if name != _LIB_NAME:
+        _LIB_NAME, _LIB = name, None
 
 
 def lib():
     """Load (building on first use if needed) the oracle shared library."""
     global _LIB
     if _LIB is None:
-        path = os.path.join(_HERE, "libmpm_oracle.so")
+        path = os.path.join(_HERE, _LIB_NAME)
         if not os.path.exists(path):
             import subprocess
             subprocess.run(["make", "-C", _HERE], check=True, capture_output=True)
