@@ -160,7 +160,7 @@ MPM_DEV float wave_sum(float v) {
 }
 
 constexpr int kMaxSlots = 64;    // wrench slots per launch (documented in mpm_hip.h)
-constexpr int kGridWgs = 256;    // persistent workgroups of the grid update
+constexpr int kGridWgs = 1024;   // persistent workgroups of the grid update (4 wavefronts per SIMD)
 
 // Deterministic wrench: per-wavefront sums in fixed point. Force units are the momentum
 // scale / 2^8 (a wavefront sums 64 nodes' impulses, each a sum of many contributions),
