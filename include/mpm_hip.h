@@ -36,7 +36,7 @@ extern "C" {
 #endif
 
 #define MPM_MAX_MATERIALS 16
-#define MPM_ABI_VERSION 3
+#define MPM_ABI_VERSION 4
 
 typedef enum mpm_status {
     MPM_OK = 0,
@@ -211,7 +211,7 @@ int32_t mpm_abi_version(void);
 const char *mpm_status_string(int32_t status);
 /* sizeof of the ABI structs, so bindings can verify their mirrors */
 int32_t mpm_struct_size(int32_t which); /* 0 particles, 1 grid, 2 model, 3 collider, 4 tick_ctl,
-                                           5 cdf lanes, 6 cdf collider */
+                                           5 cdf lanes, 6 cdf collider, 7 adjoint */
 
 /* ---- particle pass --------------------------------------------------------------- */
 /* One pass over all particles with selection == 0 doing the stages in `mode`:
@@ -368,9 +368,16 @@ int32_t mpm_grid_to_dense(const mpm_grid_t *G, float *m, float *v_in, float *v_o
  * substep); G holds node_in = (mv, m) after
  * the scatter and node_out = grid velocity after the colliders. On entry `A` holds the
  * adjoint of S_{k+1} = (x, v, C, F_trial); on exit the adjoint of S_k, with the
- * per-particle parameter adjoints (mu, lam, yield_stress) accumulated. Single scene;
- * material 1 (von Mises + Hencky, no hardening) and the plane / box / walls /
- * point-cloud colliders have adjoints; other ids set error bits 16 / 32. */
+ * per-particle parameter adjoints accumulated. Single scene; material 1 (von Mises +
+ * Hencky, no hardening: mu, lam, yield_stress) and material 10 (Newtonian / Bingham /
+ * Herschel-Bulkley fluid: viscosity, bulk modulus, K, n as per-particle partials of the
+ * per-type parameters, and yield_stress) and the plane / box / walls / point-cloud
+ * colliders have adjoints; other ids set error bits 16 / 32.
+ * The viscous stress of substep k reads the velocity gradient L_k of the gather that
+ * produced F_trial_k, so for mpm_adjoint_p2g_stress P->L must point at L_k (for
+ * mpm_adjoint_g2p at the L its gather produced); A->L carries the adjoint of L_k from the
+ * stress adjoint of substep k to the gather adjoint of substep k - 1 (zero for the
+ * last state). */
 typedef struct mpm_adjoint {
     float *x, *v, *C, *F_trial;      /* adjoint of the substep state (3, 3, 9, 9 planes) */
     float *F, *stress;               /* adjoint of the return-mapped F and of tau (9, 9) */
@@ -379,6 +386,9 @@ typedef struct mpm_adjoint {
                                         before mpm_adjoint_g2p, consumed (zeroed) by
                                         mpm_adjoint_grid */
     float *grid_mv;                  /* float4 per node: adjoint of (momentum, mass) */
+    float *L;                        /* adjoint of the velocity gradient L_k (9 planes) */
+    float *visc, *bulk, *hb_K, *hb_n; /* accumulated per-particle partials of the per-type
+                                        viscous parameters (material 10) */
 } mpm_adjoint_t;
 
 /* G2P adjoint: scatters the grid-velocity adjoint (float atomics), writes the adjoint

@@ -176,6 +176,72 @@ MPM_DEV m33 adj_von_mises(const m33& Ft, float mu, float lam, float ys, const m3
     return Ftb;
 }
 
+// cofactor matrix of m: the derivative of det(m) with respect to m
+MPM_DEV m33 mcof(const m33& m) {
+    const float* a = m.a;
+    m33 c;
+    c.a[0] = a[4] * a[8] - a[5] * a[7];
+    c.a[1] = a[5] * a[6] - a[3] * a[8];
+    c.a[2] = a[3] * a[7] - a[4] * a[6];
+    c.a[3] = a[2] * a[7] - a[1] * a[8];
+    c.a[4] = a[0] * a[8] - a[2] * a[6];
+    c.a[5] = a[1] * a[6] - a[0] * a[7];
+    c.a[6] = a[1] * a[5] - a[2] * a[4];
+    c.a[7] = a[2] * a[3] - a[0] * a[5];
+    c.a[8] = a[0] * a[4] - a[1] * a[3];
+    return c;
+}
+
+// Adjoint of the generalized-Newtonian fluid (mat 10, stress_update_particle's viscous
+// branch, mpm_utils.py:1056-1149 with the stress of stress_viscous): F = Jc I with
+// Jc = det(F_trial)^(1/3), J = Jc^3, and
+//   tau = J (2 eta_app D + p I),  D = dev sym L,  p = -bulk (J^-1.1 - 1),
+//   eta_app = eta + tau_y / gd + K gd^(n-1),  gd = sqrt(2 D:D + eps^2), eps = 0.02.
+// In: Fb = adjoint of F, tb = adjoint of tau. Returns the adjoint of F_trial; Lb is the
+// adjoint of L; the partials of eta, bulk, tau_y, K and n are added.
+MPM_DEV m33 adj_viscous(const m33& Ft, const m33& L, float bulk, float eta, float ty, float K, float n,
+                        const m33& Fb, const m33& tb_in, m33& Lb, float& etab, float& bulkb, float& tyb,
+                        float& Kb, float& nb) {
+    const float eps = 0.02f;
+    const float d = mdet(Ft);
+    const float Jc = powf(d, 1.0f / 3.0f);
+    const float J = Jc * Jc * Jc;
+    const float pJ = powf(J, -1.1f);
+    const float pressure = -bulk * (pJ - 1.0f);
+    m33 D = mscale(madd(L, mT(L)), 0.5f);
+    const float trd = (D.a[0] + D.a[4] + D.a[8]) / 3.0f;
+    D.a[0] -= trd; D.a[4] -= trd; D.a[8] -= trd;
+    float dd = 0.f;
+    for (int i = 0; i < 9; ++i) dd += D.a[i] * D.a[i];
+    const float gd = sqrtf(2.0f * dd + eps * eps);
+    const float gp = powf(gd, n - 1.0f);
+    const float eta_app = eta + ty / gd + K * gp;
+    // the forward symmetrises tau
+    const m33 T = mscale(madd(tb_in, mT(tb_in)), 0.5f);
+    float TD = 0.f;
+    for (int i = 0; i < 9; ++i) TD += T.a[i] * D.a[i];
+    const float trT = T.a[0] + T.a[4] + T.a[8];
+    // tau = J c: Jb = T : c, cb = J T; c = 2 eta_app D + p I
+    float Jb = 2.0f * eta_app * TD + pressure * trT;
+    const float eab = 2.0f * J * TD;
+    const float pb = J * trT;
+    etab += eab;
+    tyb += eab / gd;
+    Kb += eab * gp;
+    nb += eab * K * gp * logf(gd);
+    const float gdb = eab * (-ty / (gd * gd) + K * (n - 1.0f) * gp / gd);
+    // Db = 2 eta_app J T + 2 D (gdb / gd); L enters through the projector dev sym
+    m33 Db = madd(mscale(T, 2.0f * eta_app * J), mscale(D, 2.0f * gdb / gd));
+    const float trb = (Db.a[0] + Db.a[4] + Db.a[8]) / 3.0f;
+    Db.a[0] -= trb; Db.a[4] -= trb; Db.a[8] -= trb;
+    Lb = Db;
+    bulkb += -pb * (pJ - 1.0f);
+    Jb += pb * 1.1f * bulk * pJ / J;
+    const float Jcb = 3.0f * Jc * Jc * Jb + (Fb.a[0] + Fb.a[4] + Fb.a[8]);
+    const float db = Jc != 0.0f ? Jcb / (3.0f * Jc * Jc) : 0.0f;
+    return mscale(mcof(Ft), db);
+}
+
 // ---------------------------------------------------------------- G2P adjoint
 // One workgroup of kAT particles stages the grid velocity over its stencil box in LDS
 // (read once per node instead of 27 times per particle) and accumulates the scattered
@@ -227,7 +293,8 @@ __global__ __launch_bounds__(kAT) void k_adj_g2p(mpm_particles_t P, mpm_grid_t G
         const m33 Cb1 = ld9(A.C, S, p);
         const m33 Ftb1 = ld9(A.F_trial, S, p);
         const v3 vt = vb1 + xb1 * dt;                   // x' = x + dt v'
-        const m33 Lb = mscale(mmul(Ftb1, mT(F)), dt);  // F_trial' = (I + dt L) F
+        // F_trial' = (I + dt L) F, plus the adjoint of L from the next substep's stress
+        const m33 Lb = madd(mscale(mmul(Ftb1, mT(F)), dt), ld9(A.L, S, p));
         m33 IL = mscale(L, dt);
         IL.a[0] += 1.f; IL.a[4] += 1.f; IL.a[8] += 1.f;
         const m33 Fb = mmul(mT(IL), Ftb1);
@@ -490,17 +557,28 @@ __global__ __launch_bounds__(kThreads) void k_adj_p2g_stress(mpm_particles_t P, 
     // return map + stress adjoint
     const m33 Fb = ld9(A.F, S, p);
     m33 Ftb;
+    m33 Lb = mzero();
     if (mat == 1 && md->hardening[mat] <= 0.5f) {
         float mub = 0.f, lamb = 0.f, yb = 0.f;
         Ftb = adj_von_mises(ld9(P.F_trial, S, p), P.mu[p], P.lam[p], P.yield_stress[p], Fb, tb, mub, lamb, yb);
         A.mu[p] += mub;
         A.lam[p] += lamb;
         A.yield_stress[p] += yb;
+    } else if (mat == 10) {
+        float eb = 0.f, bb = 0.f, yb = 0.f, kb = 0.f, nb = 0.f;
+        Ftb = adj_viscous(ld9(P.F_trial, S, p), ld9(P.L, S, p), md->bulk[mat], md->plastic_viscosity[mat],
+                          P.yield_stress[p], md->hardening[mat], md->softening[mat], Fb, tb, Lb, eb, bb, yb, kb, nb);
+        A.visc[p] += eb;
+        A.bulk[p] += bb;
+        A.yield_stress[p] += yb;
+        A.hb_K[p] += kb;
+        A.hb_n[p] += nb;
     } else {
         Ftb = mzero();
         if (G.error_flag) atomicOr(G.error_flag, kErrAdjMaterial);
     }
     st9(A.F_trial, S, p, Ftb);
+    st9(A.L, S, p, Lb);
 }
 
 inline unsigned blocks_for(long n) { return (unsigned)((n + kThreads - 1) / kThreads); }
@@ -508,7 +586,7 @@ inline int32_t launch_status() { return hipGetLastError() == hipSuccess ? MPM_OK
 
 inline bool adj_ok(const mpm_adjoint_t* A) {
     return A && A->x && A->v && A->C && A->F_trial && A->F && A->stress && A->mu && A->lam && A->yield_stress &&
-           A->grid_v && A->grid_mv;
+           A->grid_v && A->grid_mv && A->L && A->visc && A->bulk && A->hb_K && A->hb_n;
 }
 
 inline bool grid_ok1(const mpm_grid_t* G) {
@@ -548,8 +626,8 @@ int32_t mpm_adjoint_grid(const mpm_grid_t* G, const mpm_model_t* model, const mp
 int32_t mpm_adjoint_p2g_stress(const mpm_particles_t* P, const mpm_grid_t* G, const mpm_adjoint_t* A,
                                const mpm_model_t* model, float dt, void* stream) {
     if (!P || !grid_ok1(G) || !adj_ok(A) || !model || P->n < 0 || P->stride < P->n || !P->x || !P->v || !P->C ||
-        !P->F_trial || !P->stress || !P->vol || !P->mass || !P->mu || !P->lam || !P->yield_stress || !P->material ||
-        !P->selection)
+        !P->F_trial || !P->stress || !P->L || !P->vol || !P->mass || !P->mu || !P->lam || !P->yield_stress ||
+        !P->material || !P->selection)
         return MPM_ERR_ARG;
     if (P->n == 0) return MPM_OK;
     hipLaunchKernelGGL(k_adj_p2g_stress, dim3(blocks_for(P->n)), dim3(kThreads), 0, (hipStream_t)stream, *P, *G, *A,

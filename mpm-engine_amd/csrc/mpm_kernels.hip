@@ -620,6 +620,7 @@ int32_t mpm_struct_size(int32_t which) {
         case 4: return (int32_t)sizeof(mpm_tick_ctl_t);
         case 5: return (int32_t)sizeof(mpm_cdf_lanes_t);
         case 6: return (int32_t)sizeof(mpm_cdf_collider_t);
+        case 7: return (int32_t)sizeof(mpm_adjoint_t);
         default: return -1;
     }
 }

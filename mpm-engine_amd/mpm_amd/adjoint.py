@@ -3,13 +3,16 @@
 ``Rollout(solver, dt, substeps)`` takes a configured single-scene ``Solver`` (particles
 loaded, material set, colliders added) and exposes the rollout of `substeps` substeps as
 a ``torch.autograd.Function`` of the per-particle Lame parameters, yield stress, initial
-velocity and initial position. It returns the final positions and the accumulated
-collider wrench, so a loss on the final state and on the plate reaction forces
-back-propagates to Young's modulus, Poisson's ratio, the yield stress (through the
-per-particle parameters) and to the initial particle state.
+velocity and initial position, and of the generalized-Newtonian fluid's parameters
+(viscosity, bulk modulus, Herschel-Bulkley K and n of material 10). It returns the final
+positions and the accumulated collider wrench, so a loss on the final state and on the
+plate reaction forces back-propagates to Young's modulus, Poisson's ratio, the yield
+stress (through the per-particle parameters), the fluid's rheology and to the initial
+particle state.
 
 Forward: fused ticks of `checkpoint_every` substeps (the production substep), storing
-the particle state (x, v, C, F_trial) at every tick boundary. Backward, segment by
+the particle state (x, v, C, F_trial and the velocity gradient L, which the viscous
+stress of the next substep reads) at every tick boundary. Backward, segment by
 segment from the last: restore the checkpoint, recompute the segment substep by
 substep with the split passes while keeping every intermediate particle state and grid,
 then run the hand-written adjoint kernels of libmpm_hip.so (mpm_adjoint_g2p,
@@ -30,6 +33,9 @@ from .core import native
 
 _STATE_PLANES = 24   # x 3, v 3, C 9, F_trial 9
 _SLOT_PLANES = 51    # + F 9, stress 9, L 9
+_L0 = 42             # first L plane of a slot
+_CKPT_PLANES = 33    # state + L
+FLUID_ID = 10        # the material whose parameters form the `fluid` input
 
 
 def lame(E, nu):
@@ -40,7 +46,7 @@ def lame(E, nu):
 class Rollout:
     """Differentiable rollout of `substeps` substeps of dt on a configured Solver."""
 
-    SUPPORTED_MATERIALS = frozenset((1,))
+    SUPPORTED_MATERIALS = frozenset((1, FLUID_ID))
     SUPPORTED_COLLIDERS = frozenset((native.COLLIDER_PLANE, native.COLLIDER_BOX, native.COLLIDER_WALLS,
                                      native.COLLIDER_POINTCLOUD))
 
@@ -86,6 +92,12 @@ class Rollout:
         self.C0 = sim.C.clone()
         self.Ft0 = sim.F_trial.clone()
         self.F0 = sim.F.clone()
+        self.L0 = sim.L.clone()
+        # the viscous stress reads L: slots carry it forward and checkpoints keep it
+        self.viscous = FLUID_ID in sim._mat_ids
+        md = sim.model[0]
+        self.fluid0 = [float(md["plastic_viscosity"][FLUID_ID]), float(md["bulk"][FLUID_ID]),
+                       float(md["hardening"][FLUID_ID]), float(md["softening"][FLUID_ID])]
         # collider rows of every substep, integrated once from the current poses
         saved = [(list(c.point), np.array(c.rot), list(c.quat)) for c in sim.colliders.items]
         tab, _ = sim.colliders.schedule(sim.time, self.dt, self.substeps)
@@ -107,7 +119,7 @@ class Rollout:
                               dtype=torch.float32, device=dev)
         self.wrench = torch.zeros(max(6 * self.n_slots, 6), dtype=torch.float32, device=dev)
         nseg = (self.substeps + self.every - 1) // self.every
-        self.ckpt = torch.empty(nseg, _STATE_PLANES, n, dtype=torch.float32, device=dev)
+        self.ckpt = torch.empty(nseg, _CKPT_PLANES, n, dtype=torch.float32, device=dev)
         # one segment of recomputed substeps: particle slots and grids
         L = min(self.every, self.substeps)
         self.slots = torch.zeros(L + 1, _SLOT_PLANES, n, dtype=torch.float32, device=dev)
@@ -118,7 +130,8 @@ class Rollout:
         z = lambda *s: torch.zeros(*s, dtype=torch.float32, device=dev)  # noqa: E731
         self.adj = {"x": z(3, n), "v": z(3, n), "C": z(9, n), "F_trial": z(9, n), "F": z(9, n),
                     "stress": z(9, n), "mu": z(n), "lam": z(n), "yield_stress": z(n),
-                    "grid_v": z(node_floats), "grid_mv": z(node_floats)}
+                    "grid_v": z(node_floats), "grid_mv": z(node_floats), "L": z(9, n),
+                    "visc": z(n), "bulk": z(n), "hb_K": z(n), "hb_n": z(n)}
         A = native.Adjoint()
         for k, t in self.adj.items():
             setattr(A, k, native.ptr(t))
@@ -130,6 +143,8 @@ class Rollout:
         # ABI structs of every substep slot, built once
         self._P_fwd = [self._particles(self.slots[k + 1], self.slots[k + 1]) for k in range(L)]
         self._P_adj = [self._particles(self.slots[k], self.slots[k + 1]) for k in range(L)]
+        # the stress adjoint of substep k reads L_k, which slot k holds
+        self._P_st = [self._particles(self.slots[k], self.slots[k + 1], self.slots[k]) for k in range(L)]
         self._G = [self._grid_struct(self.g_in[k], self.g_out[k]) for k in range(L)]
         self.use_graph = True
         self._graph = None
@@ -141,16 +156,18 @@ class Rollout:
         G.node_in, G.node_out = native.ptr(node_in), native.ptr(node_out)
         return G
 
-    def _particles(self, state_slot, post_slot):
+    def _particles(self, state_slot, post_slot, l_slot=None):
         """Particle struct: x, v, C, F_trial from state_slot's planes, F and stress from
-        post_slot's (shared per-particle arrays from the simulator)."""
+        post_slot's, L from l_slot's (default post_slot; shared per-particle arrays from
+        the simulator)."""
         P = native.Particles()
         C.memmove(C.byref(P), C.byref(self.sim.P), C.sizeof(P))
         n = self.n
         base = native.ptr(state_slot)
         post = native.ptr(post_slot)
+        lp = native.ptr(post_slot if l_slot is None else l_slot)
         P.x, P.v, P.C, P.F_trial = base, base + 3 * 4 * n, base + 6 * 4 * n, base + 15 * 4 * n
-        P.F, P.stress, P.L = post + 24 * 4 * n, post + 33 * 4 * n, post + 42 * 4 * n
+        P.F, P.stress, P.L = post + 24 * 4 * n, post + 33 * 4 * n, lp + _L0 * 4 * n
         P.cov = 0
         return P
 
@@ -165,8 +182,13 @@ class Rollout:
         return out
 
     # ------------------------------------------------------------------ forward
-    def _set_state(self, mu, lam, ys, v0, x0):
+    def _set_state(self, mu, lam, ys, v0, x0, fluid):
         sim = self.sim
+        md = sim.model[0]
+        vals = [float(v) for v in fluid.cpu()]
+        for fld, val in zip(("plastic_viscosity", "bulk", "hardening", "softening"), vals):
+            md[fld][FLUID_ID] = val
+        sim._model_dirty = True
         sim.mu.copy_(self._to_slots(mu))
         sim.lam.copy_(self._to_slots(lam))
         sim.yield_stress.copy_(self._to_slots(ys))
@@ -175,6 +197,7 @@ class Rollout:
         sim.C.copy_(self.C0)
         sim.F_trial.copy_(self.Ft0)
         sim.F.copy_(self.F0)
+        sim.L.copy_(self.L0)
         sim.stress.zero_()
         sim._upload_model()
 
@@ -187,11 +210,11 @@ class Rollout:
             native.ptr(wrench) if slots else None, slots, native.ptr(self.ws) if slots else None,
             dt, flags, stream), "grid_update")
 
-    def forward(self, mu, lam, ys, v0, x0):
+    def forward(self, mu, lam, ys, v0, x0, fluid):
         """Fused forward rollout from (x0, v0); returns (x_T (n, 3), wrench (slots, 6))."""
         sim = self.sim
         torch = self.torch
-        self._set_state(mu, lam, ys, v0, x0)
+        self._set_state(mu, lam, ys, v0, x0, fluid)
         self.wrench.zero_()
         sim.error_flag.zero_()
         s = sim.stream()
@@ -208,6 +231,8 @@ class Rollout:
             ck[3:6].copy_(sim.v)
             ck[6:15].copy_(sim.C)
             ck[15:24].copy_(sim.F_trial)
+            if self.viscous:
+                ck[24:33].copy_(sim.L)
             native.check(lib.mpm_grid_zero_box(pG, zl, zh, 3, s), "zero_box")
             G = sim.G
             sim._pass(ST | P2G, dt, s)
@@ -228,7 +253,9 @@ class Rollout:
     # ------------------------------------------------------------------ backward
     def _load_segment(self, seg, k0, S):
         """Checkpoint of segment `seg` into slot 0 and its collider rows into seg_sched."""
-        self.slots[0, :_STATE_PLANES].copy_(self.ckpt[seg])
+        self.slots[0, :_STATE_PLANES].copy_(self.ckpt[seg, :_STATE_PLANES])
+        if self.viscous:
+            self.slots[0, _L0:].copy_(self.ckpt[seg, _STATE_PLANES:])
         if self.K:
             nb = S * self.row_bytes
             self.seg_sched[:nb].copy_(self.sched[k0 * self.row_bytes:k0 * self.row_bytes + nb])
@@ -246,6 +273,8 @@ class Rollout:
         sched0 = native.ptr(self.seg_sched)
         for k in range(S):
             self.slots[k + 1, :_STATE_PLANES].copy_(self.slots[k, :_STATE_PLANES])
+            if self.viscous:  # the recomputed stress of substep k reads L_k
+                self.slots[k + 1, _L0:].copy_(self.slots[k, _L0:])
             P, G = C.byref(self._P_fwd[k]), C.byref(self._G[k])
             native.check(lib.mpm_particle_pass(P, G, md, dt, native.PASS_STRESS | native.PASS_P2G, s),
                          "recompute p2g")
@@ -261,10 +290,12 @@ class Rollout:
             native.check(lib.mpm_adjoint_g2p(P, G, pA, md, dt, s), "adjoint g2p")
             native.check(lib.mpm_adjoint_grid(G, md, sched0 + k * self.row_bytes if self.K else None, self.K,
                                               impb, self.n_slots, pA, dt, s), "adjoint grid")
-            native.check(lib.mpm_adjoint_p2g_stress(P, G, pA, md, dt, s), "adjoint p2g+stress")
+            native.check(lib.mpm_adjoint_p2g_stress(C.byref(self._P_st[k]), G, pA, md, dt, s),
+                         "adjoint p2g+stress")
 
     def backward(self, x_bar, wrench_bar):
-        """Adjoint of the last forward: returns (mu_bar, lam_bar, ys_bar, v0_bar, x0_bar)."""
+        """Adjoint of the last forward: returns (mu_bar, lam_bar, ys_bar, v0_bar, x0_bar,
+        fluid_bar)."""
         sim = self.sim
         torch = self.torch
         a = self.adj
@@ -292,16 +323,21 @@ class Rollout:
         flag = int(sim.error_flag.item())
         if flag & (native.ERR_ADJ_MATERIAL | native.ERR_ADJ_COLLIDER):
             raise NotImplementedError(f"a material or collider without an adjoint was met (flag {flag})")
+        fluid_bar = torch.stack([a[k].double().sum() for k in ("visc", "bulk", "hb_K", "hb_n")])
         return (self._to_user(a["mu"]), self._to_user(a["lam"]), self._to_user(a["yield_stress"]),
-                self._to_user(a["v"].t()), self._to_user(a["x"].t()))
+                self._to_user(a["v"].t()), self._to_user(a["x"].t()), fluid_bar)
 
     # ------------------------------------------------------------------ autograd
-    def __call__(self, mu, lam, yield_stress, v0, x0=None):
+    def __call__(self, mu, lam, yield_stress, v0, x0=None, fluid=None):
         """(x_T, wrench) as a differentiable function of per-particle mu, lam, yield
-        stress (n,), initial velocity (n, 3) and initial position (n, 3)."""
+        stress (n,), initial velocity (n, 3), initial position (n, 3) and the fluid
+        parameters (4,) = (viscosity, bulk modulus, K, n) of material 10 (default: the
+        solver's values)."""
         if x0 is None:
             x0 = self.x0
-        return _RolloutFn.apply(self, mu, lam, yield_stress, v0, x0)
+        if fluid is None:
+            fluid = self.torch.tensor(self.fluid0, dtype=self.torch.float32, device=self.sim.device)
+        return _RolloutFn.apply(self, mu, lam, yield_stress, v0, x0, fluid)
 
     def run(self, E, nu, yield_stress, v0, x0=None):
         """Rollout as a function of the material's E, nu (scalars, broadcast to every
@@ -311,6 +347,19 @@ class Rollout:
         ones = torch.ones(self.n, dtype=torch.float32, device=self.sim.device)
         ys = yield_stress * ones if yield_stress.dim() == 0 else yield_stress
         return self(mu * ones, lam * ones, ys, v0, x0)
+
+    def run_fluid(self, yield_stress, v0, viscosity=None, bulk=None, K=None, n=None, x0=None):
+        """Rollout as a function of the generalized-Newtonian fluid's viscosity, bulk
+        modulus, Herschel-Bulkley K and n (scalars; None keeps the solver's value) and
+        yield stress (scalar or (n,)); the Lame parameters keep the solver's values."""
+        torch = self.torch
+        dev = self.sim.device
+        vals = [torch.as_tensor(self.fluid0[i], dtype=torch.float32, device=dev) if v is None else v
+                for i, v in enumerate((viscosity, bulk, K, n))]
+        fluid = torch.stack([v.to(torch.float32).reshape(()) for v in vals])
+        ones = torch.ones(self.n, dtype=torch.float32, device=dev)
+        ys = yield_stress * ones if yield_stress.dim() == 0 else yield_stress
+        return self(self._to_user(self.sim.mu).clone(), self._to_user(self.sim.lam).clone(), ys, v0, x0, fluid)
 
 
 def _as_f32(t, torch):
@@ -322,11 +371,11 @@ def _make_fn():
 
     class RolloutFn(torch.autograd.Function):
         @staticmethod
-        def forward(ctx, ro, mu, lam, ys, v0, x0):
+        def forward(ctx, ro, mu, lam, ys, v0, x0, fluid):
             ctx.ro = ro
-            ctx.dtypes = (mu.dtype, lam.dtype, ys.dtype, v0.dtype, x0.dtype)
+            ctx.dtypes = (mu.dtype, lam.dtype, ys.dtype, v0.dtype, x0.dtype, fluid.dtype)
             xT, w = ro.forward(_as_f32(mu, torch), _as_f32(lam, torch), _as_f32(ys, torch),
-                               _as_f32(v0, torch), _as_f32(x0, torch))
+                               _as_f32(v0, torch), _as_f32(x0, torch), _as_f32(fluid, torch))
             return xT, w
 
         @staticmethod
@@ -334,10 +383,11 @@ def _make_fn():
             ro = ctx.ro
             if xT_bar is None:
                 xT_bar = torch.zeros(ro.n, 3, dtype=torch.float32, device=ro.sim.device)
-            mub, lamb, yb, vb, xb = ro.backward(_as_f32(xT_bar, torch),
-                                                None if w_bar is None else _as_f32(w_bar, torch))
+            mub, lamb, yb, vb, xb, fb = ro.backward(_as_f32(xT_bar, torch),
+                                                    None if w_bar is None else _as_f32(w_bar, torch))
             dts = ctx.dtypes
-            return (None, mub.to(dts[0]), lamb.to(dts[1]), yb.to(dts[2]), vb.to(dts[3]), xb.to(dts[4]))
+            return (None, mub.to(dts[0]), lamb.to(dts[1]), yb.to(dts[2]), vb.to(dts[3]), xb.to(dts[4]),
+                    fb.to(dts[5]))
 
     return RolloutFn
 

@@ -40,7 +40,8 @@ class Particles(C.Structure):
 
 class Adjoint(C.Structure):
     _fields_ = [(nm, _FP) for nm in ("x", "v", "C", "F_trial", "F", "stress", "mu", "lam",
-                                     "yield_stress", "grid_v", "grid_mv")]
+                                     "yield_stress", "grid_v", "grid_mv", "L", "visc", "bulk",
+                                     "hb_K", "hb_n")]
 
 
 MAX_CDF = 4
@@ -164,7 +165,7 @@ def load():
         fn.argtypes = args
     sizes = {0: C.sizeof(Particles), 1: C.sizeof(Grid), 2: MODEL_DTYPE.itemsize,
              3: COLLIDER_DTYPE.itemsize, 4: TICK_CTL_DTYPE.itemsize,
-             5: CDF_LANES_DTYPE.itemsize, 6: C.sizeof(CdfCollider)}
+             5: CDF_LANES_DTYPE.itemsize, 6: C.sizeof(CdfCollider), 7: C.sizeof(Adjoint)}
     for which, size in sizes.items():
         got = lib.mpm_struct_size(which)
         if got != size:

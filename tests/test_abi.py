@@ -29,13 +29,14 @@ def test_library_exports_every_declared_symbol():
 
 def test_struct_layouts_and_introspection():
     lib = native.load()
-    assert lib.mpm_abi_version() == 3
+    assert lib.mpm_abi_version() == 4
     assert lib.mpm_struct_size(1) == ctypes.sizeof(native.Grid)
     assert lib.mpm_struct_size(2) == native.MODEL_DTYPE.itemsize
     assert lib.mpm_struct_size(3) == native.COLLIDER_DTYPE.itemsize
     assert lib.mpm_struct_size(0) == ctypes.sizeof(native.Particles)
     assert lib.mpm_struct_size(5) == native.CDF_LANES_DTYPE.itemsize
     assert lib.mpm_struct_size(6) == ctypes.sizeof(native.CdfCollider)
+    assert lib.mpm_struct_size(7) == ctypes.sizeof(native.Adjoint)
     assert lib.mpm_struct_size(9) == -1
     assert lib.mpm_status_string(1) == b"invalid argument"
 

@@ -241,14 +241,148 @@ def test_one_substep_material_directions():
 
 
 def test_unsupported_material_is_refused():
-    from mpm_amd.materials import newtonian
+    from mpm_amd.materials import elastic
 
     sc = _scene()
     s = Solver(grid=sc.grid, device="hip")
     s.load_particles(sc.pos, sc.vol)
-    s.set_material(newtonian(eta=1.0))
+    s.set_material(elastic(E=1.0e5))
     with pytest.raises(NotImplementedError):
         Rollout(s, DT, 10)
+
+
+def _fluid(eta, bulk, ty, K, n):
+    from mpm_amd.materials import newtonian
+
+    return newtonian(eta=eta, bulk_modulus=bulk, density=1000.0).with_yield(ty).with_powerlaw(K=K, n=n)
+
+
+def _fluid_scene(mat, plate_speed):
+    sc = _scene(plate_speed=plate_speed)
+    s = sc.build(Solver(grid=sc.grid, device="hip"))
+    s.set_material(mat)
+    return sc, s
+
+
+def test_fluid_one_substep_parameter_directions():
+    """Herschel-Bulkley fluid (material 10; eta 2, bulk 1e4, yield 50, K 20, n 0.5), one
+    substep from a random state: F_trial with +-2 % volume change and a random velocity
+    gradient L (entries up to 10 /s), so pressure, viscous, yield and power-law terms all
+    carry the stress. d (plate x impulses) / d (viscosity, bulk, K, n, per-particle yield)
+    against central differences of the GPU forward (relative steps 5e-2, 2e-2 for n,
+    whose third derivative carries log^3 of the shear rate). The fluid stress is smooth in
+    every parameter (no yield branch), so every particle is perturbed. The plates stand
+    still: a moving plate adds m v_plate of every node it covers to the impulse, a term
+    hundreds of times the viscous part that would leave the difference at the fp32
+    rounding of the sum."""
+    import torch
+
+    par = {"viscosity": 2.0, "bulk": 1.0e4, "K": 20.0, "n": 0.5}
+    sc, s = _fluid_scene(_fluid(par["viscosity"], par["bulk"], 50.0, par["K"], par["n"]), 0.0)
+    n = s.n_particles
+    rng = np.random.default_rng(9)
+    sim = s._sim
+    Ft0 = ((1.0 + 0.02 * rng.uniform(-1, 1, (n, 1, 1))) ** (1.0 / 3.0) * np.eye(3)
+           + 0.01 * rng.uniform(-1, 1, (n, 3, 3))).astype(np.float32)
+    L0 = (10.0 * rng.uniform(-1, 1, (n, 3, 3))).astype(np.float32)
+    sim.F_trial.copy_(torch.from_numpy(Ft0.reshape(n, 9).T.copy()))
+    sim.L.copy_(torch.from_numpy(L0.reshape(n, 9).T.copy()))
+    ro = Rollout(s, DT, 1, checkpoint_every=1)
+    dev = sim.device
+    cw = np.zeros((2, 3))
+    cw[:, 0] = rng.standard_normal(2)
+    tens = {k: torch.tensor(v, dtype=torch.float64, device=dev, requires_grad=True) for k, v in par.items()}
+    ys = torch.full((n,), 50.0, device=dev, requires_grad=True)
+    v = torch.zeros(n, 3, device=dev)
+    _, w = ro.run_fluid(ys, v, viscosity=tens["viscosity"], bulk=tens["bulk"], K=tens["K"], n=tens["n"])
+    lw = (w[:, :3].double() * torch.from_numpy(cw).to(dev)).sum()
+    grads = dict(zip(list(tens) + ["yield"], torch.autograd.grad(lw, list(tens.values()) + [ys])))
+
+    def impulse(**kw):
+        args = dict(par)
+        yv = kw.pop("yield", ys.detach())
+        args.update(kw)
+        with torch.no_grad():
+            _, wv = ro.run_fluid(yv, v, **{k: torch.tensor(a, dtype=torch.float64, device=dev)
+                                           for k, a in args.items()})
+        return float((wv[:, :3].double().cpu().numpy() * cw).sum())
+
+    rel = {"viscosity": 5e-2, "bulk": 5e-2, "K": 5e-2, "n": 2e-2}
+    for name, r in rel.items():
+        h = r * par[name]
+        fd = (impulse(**{name: par[name] + h}) - impulse(**{name: par[name] - h})) / (2 * h)
+        ad = grads[name].item()
+        print("w", name, ad, fd)
+        assert abs(ad - fd) <= 2e-3 * abs(fd), f"w {name}: adjoint {ad:.6e} vs central difference {fd:.6e}"
+    d = torch.from_numpy((rng.uniform(0.5, 1.5, n) * 2.5).astype(np.float32)).to(dev)
+    ad = float((grads["yield"].double() * d.double()).sum())
+    fd = (impulse(**{"yield": ys.detach() + d}) - impulse(**{"yield": ys.detach() - d})) / 2.0
+    print("w yield", ad, fd)
+    assert abs(ad - fd) <= 2e-3 * abs(fd), f"w yield: adjoint {ad:.6e} vs central difference {fd:.6e}"
+
+
+def test_fluid_rollout_gradients_match_oracle_differences():
+    """100 substeps (checkpoints every 20) of the fluid (eta 2, bulk 1e4, yield 10, K 5,
+    n 0.5) sheared between the two plates standing still, gravity off: initial velocity
+    v_z = sin(pi (x - x_lo) / 0.2) m/s across the gap plus a small v_y, so the plates
+    take the viscous drag, and the velocity gradient that the viscous stress reads crosses
+    every substep and four checkpoints. Loss g . (plate impulses), random g. Checked:
+    d / d (viscosity, yield, K, n) and the directional derivative along a smooth
+    initial-velocity field against central differences of the oracle rollout (relative
+    step 3e-2; the initial-velocity direction pins the adjoint of L through G2P: without
+    it the viscous stress's dependence on the velocity field drops out of the gradient).
+
+    The set-up keeps the differences well above the fp32 rounding of the impulse sums: no
+    plate velocity (whose m v_plate term would dominate the impulse), no gravity (whose
+    m g dt per plate node would), and a shear profile that vanishes at the plates, so the
+    impulse is the stress carried to them. The bulk modulus barely acts in this shear
+    flow; the one-substep test pins its derivative."""
+    import torch
+
+    par = {"viscosity": 2.0, "bulk": 1.0e4, "yield": 10.0, "K": 5.0, "n": 0.5}
+    no_g = {"g": [0.0, 0.0, 0.0]}
+    mat = lambda p: _fluid(p["viscosity"], p["bulk"], p["yield"], p["K"], p["n"])  # noqa: E731
+    sc, s = _fluid_scene(mat(par), 0.0)
+    s.set_material(mat(par), **no_g)
+    n = s.n_particles
+    S = 100
+    ro = Rollout(s, DT, S, checkpoint_every=20)
+    dev = s._sim.device
+    xs = sc.pos.astype(np.float64)
+    v_base = np.zeros((n, 3), np.float32)
+    v_base[:, 2] = np.sin(np.pi * (xs[:, 0] - xs[:, 0].min()) / 0.2)
+    v_base[:, 1] = 0.3 * np.sin(np.pi * (xs[:, 2] - 0.2) / 0.2)
+    tens = {k: torch.tensor(v, dtype=torch.float64, device=dev, requires_grad=True) for k, v in par.items()}
+    v0 = torch.from_numpy(v_base).to(dev).requires_grad_(True)
+    _, w = ro.run_fluid(tens["yield"], v0, viscosity=tens["viscosity"], bulk=tens["bulk"], K=tens["K"],
+                        n=tens["n"])
+    rng = np.random.default_rng(4)
+    gw = rng.standard_normal((2, 3))
+    (w[:, :3].double() * torch.from_numpy(gw).to(dev)).sum().backward()
+    g = {k: t.grad.item() for k, t in tens.items()}
+
+    def linearised(p, v_):
+        o, plates = _oracle(sc, 0.0, 0.0, np.float32(p["yield"]), v_)
+        o.set_material(mat(p), **no_g)
+        o._sim.v[:] = v_
+        o.step(DT, S)
+        imp = np.stack([o.tool_force(hnd, 1.0) for hnd in plates]).astype(np.float64)
+        return (imp * gw).sum()
+
+    report = {}
+    for name in ("viscosity", "yield", "K", "n"):
+        h = 3e-2 * par[name]
+        pp, pm = dict(par), dict(par)
+        pp[name] += h
+        pm[name] -= h
+        report[name] = (g[name], (linearised(pp, v_base) - linearised(pm, v_base)) / (2 * h))
+    d = np.stack([np.sin(7 * xs[:, 1] + 1), np.cos(5 * xs[:, 2]), np.sin(6 * xs[:, 0] + 2)], 1).astype(np.float32)
+    hv = 1e-2
+    report["v0.d"] = (float((v0.grad.double().cpu().numpy() * d).sum()),
+                      (linearised(par, v_base + hv * d) - linearised(par, v_base - hv * d)) / (2 * hv))
+    print(report)
+    for k, (a, f) in report.items():
+        assert abs(a - f) <= 2e-3 * abs(f), f"{k}: adjoint {a:.6e} vs central difference {f:.6e}"
 
 
 def test_graph_replayed_backward_matches_eager():
