@@ -369,10 +369,12 @@ int32_t mpm_grid_to_dense(const mpm_grid_t *G, float *m, float *v_in, float *v_o
  * the scatter and node_out = grid velocity after the colliders. On entry `A` holds the
  * adjoint of S_{k+1} = (x, v, C, F_trial); on exit the adjoint of S_k, with the
  * per-particle parameter adjoints accumulated. Single scene; material 1 (von Mises +
- * Hencky, no hardening: mu, lam, yield_stress) and material 10 (Newtonian / Bingham /
- * Herschel-Bulkley fluid: viscosity, bulk modulus, K, n as per-particle partials of the
- * per-type parameters, and yield_stress) and the plane / box / walls / point-cloud
- * colliders have adjoints; other ids set error bits 16 / 32.
+ * Hencky, no hardening: mu, lam, yield_stress), material 9 (local mu(I): mu, lam and
+ * mu_s, delta_mu, I0, grain diameter as per-particle partials of the per-type
+ * coefficients) and material 10 (Newtonian / Bingham / Herschel-Bulkley fluid:
+ * viscosity, bulk modulus, K, n as per-particle partials of the per-type parameters, and
+ * yield_stress) and the plane / box / walls / point-cloud colliders have adjoints; other
+ * ids set error bits 16 / 32.
  * The viscous stress of substep k reads the velocity gradient L_k of the gather that
  * produced F_trial_k, so for mpm_adjoint_p2g_stress P->L must point at L_k (for
  * mpm_adjoint_g2p at the L its gather produced); A->L carries the adjoint of L_k from the
@@ -389,6 +391,9 @@ typedef struct mpm_adjoint {
     float *L;                        /* adjoint of the velocity gradient L_k (9 planes) */
     float *visc, *bulk, *hb_K, *hb_n; /* accumulated per-particle partials of the per-type
                                         viscous parameters (material 10) */
+    float *mui_mu_s, *mui_dmu, *mui_I0, *mui_d; /* accumulated per-particle partials of the
+                                        per-type mu(I) coefficients (material 9): mu_s,
+                                        delta_mu, I0, grain diameter */
 } mpm_adjoint_t;
 
 /* G2P adjoint: scatters the grid-velocity adjoint (float atomics), writes the adjoint

@@ -13,9 +13,10 @@
 // the B-spline weights use the weight Hessian, so gradients with respect to positions
 // are exact for the discrete map (between cell crossings of the stencil base).
 //
-// Return-map adjoint (von Mises with Hencky elasticity, material 1): F_trial = U S V^T;
-// the stress tau = U diag(h) U^T and the corrected F = U diag(g) V^T are isotropic
-// functions of F_trial. Their derivatives are assembled in the SVD frame from the
+// Return-map adjoints (von Mises, material 1, and local mu(I), material 9, both with
+// Hencky elasticity): F_trial = U S V^T; the stress tau = U diag(h) U^T and the
+// corrected F = U diag(g) V^T are isotropic functions of F_trial, and both maps scale
+// the deviatoric log-strain by one factor k, so they share the assembly below. Their derivatives are assembled in the SVD frame from the
 // diagonal Jacobians dh/dsigma, dg/dsigma and, per pair (a, b), the divided differences
 // (h_a - h_b)/(s_a - s_b) and (g_a - g_b)/(s_a - s_b) plus the sums over (s_a + s_b).
 // The divided differences are evaluated through log-ratios with expm1 and a series
@@ -25,6 +26,7 @@
 
 #include "mpm_device.h"
 #include "mpm_tile.h"
+#include "mpm_adjoint_math.h"
 
 namespace {
 
@@ -77,169 +79,6 @@ MPM_DEV m33 outer(v3 a, v3 b) {
     r.a[3] = a.y * b.x; r.a[4] = a.y * b.y; r.a[5] = a.y * b.z;
     r.a[6] = a.z * b.x; r.a[7] = a.z * b.y; r.a[8] = a.z * b.z;
     return r;
-}
-
-// u / expm1(u), its series below |u| < 1e-3
-MPM_DEV float u_over_expm1(float u) { return fabsf(u) < 1e-3f ? 1.0f - 0.5f * u + u * u * (1.0f / 12.0f) : u / expm1f(u); }
-// expm1(k u) / expm1(u)
-MPM_DEV float expm1_ratio(float k, float u) {
-    return fabsf(u) < 1e-3f ? k * (1.0f + 0.5f * (k - 1.0f) * u) : expm1f(k * u) / expm1f(u);
-}
-
-// Adjoint of the von Mises return map + Hencky stress (mat 1, no hardening) at F_trial.
-// In: Fb = adjoint of the corrected F, tb = adjoint of tau. Out: adjoint of F_trial and
-// the per-particle parameter adjoints (added).
-MPM_DEV m33 adj_von_mises(const m33& Ft, float mu, float lam, float ys, const m33& Fb, const m33& tb_in,
-                          float& mub, float& lamb, float& yb) {
-    m33 U, V;
-    v3 s3;
-    svd3(Ft, U, s3, V);
-    const float s[3] = {s3.x, s3.y, s3.z};
-    float e[3];
-    for (int a = 0; a < 3; ++a) e[a] = logf(fmaxf(s[a], 0.01f));
-    const float me = (e[0] + e[1] + e[2]) / 3.0f;
-    float eh[3] = {e[0] - me, e[1] - me, e[2] - me};
-    // forward's yield test, same expression and order
-    const float trl = lam * (e[0] + e[1] + e[2]);
-    const float t0 = 2.0f * mu * e[0] + trl, t1 = 2.0f * mu * e[1] + trl, t2 = 2.0f * mu * e[2] + trl;
-    const float stv = (t0 + t1 + t2) / 3.0f;
-    const float c0 = t0 - stv, c1 = t1 - stv, c2 = t2 - stv;
-    const bool yield = sqrtf(c0 * c0 + c1 * c1 + c2 * c2) > ys;
-    float k = 1.0f, hn = 0.0f;
-    if (yield) {
-        hn = sqrtf(eh[0] * eh[0] + eh[1] * eh[1] + eh[2] * eh[2]) + 1e-6f;
-        const float dg = hn - ys / (2.0f * mu);
-        k = 1.0f - dg / hn;
-    }
-    float en[3], h[3];
-    for (int a = 0; a < 3; ++a) en[a] = me + k * eh[a];
-    const float tre = en[0] + en[1] + en[2];
-    for (int a = 0; a < 3; ++a) h[a] = 2.0f * mu * en[a] + lam * tre;
-
-    // tau = U diag(h) U^T
-    const m33 tb = mscale(madd(tb_in, mT(tb_in)), 0.5f);
-    const m33 T = mmul(mT(U), mmul(tb, U));
-    const float trT = T.a[0] + T.a[4] + T.a[8];
-    float enb[3];
-    for (int a = 0; a < 3; ++a) enb[a] = 2.0f * mu * T.a[4 * a] + lam * trT;
-    mub += 2.0f * (en[0] * T.a[0] + en[1] * T.a[4] + en[2] * T.a[8]);
-    lamb += tre * trT;
-    m33 Al = mzero();
-    const int pa[3] = {0, 0, 1}, pb[3] = {1, 2, 2};
-    for (int q = 0; q < 3; ++q) {
-        const int a = pa[q], b = pb[q];
-        const float u = e[a] - e[b];
-        const float d1 = 2.0f * mu * k * u_over_expm1(u) / s[b];
-        const float ssum = s[a] + s[b];
-        const float d2 = fabsf(ssum) > 1e-12f ? (h[b] - h[a]) / ssum : 0.0f;
-        const float Tab = T.a[3 * a + b] + T.a[3 * b + a];
-        Al.a[3 * a + b] += 0.5f * Tab * (d1 + d2);
-        Al.a[3 * b + a] += 0.5f * Tab * (d1 - d2);
-    }
-    float eb[3];
-    if (yield) {
-        // F = U diag(g) V^T, g = exp(en)
-        const m33 M = mmul(mT(U), mmul(Fb, V));
-        float g[3];
-        for (int a = 0; a < 3; ++a) {
-            g[a] = expf(en[a]);
-            enb[a] += M.a[4 * a] * g[a];
-        }
-        for (int q = 0; q < 3; ++q) {
-            const int a = pa[q], b = pb[q];
-            const float u = e[a] - e[b];
-            const float cA = g[b] / s[b] * expm1_ratio(k, u);
-            const float ssum = s[a] + s[b];
-            const float cB = fabsf(ssum) > 1e-12f ? (g[a] + g[b]) / ssum : 0.0f;
-            const float S = M.a[3 * a + b] + M.a[3 * b + a], D = M.a[3 * a + b] - M.a[3 * b + a];
-            Al.a[3 * a + b] += 0.5f * (cA * S + cB * D);
-            Al.a[3 * b + a] += 0.5f * (cA * S - cB * D);
-        }
-        // en = me 1 + (ys / (2 mu hn)) eh with hn = |eh| + 1e-6
-        const float ehn = hn - 1e-6f;
-        const float kk = ys / (2.0f * mu * hn);
-        const float dotv = eh[0] * enb[0] + eh[1] * enb[1] + eh[2] * enb[2];
-        const float c = ehn > 0.0f ? dotv / (ehn * hn) : 0.0f;
-        float tmp[3];
-        for (int a = 0; a < 3; ++a) tmp[a] = enb[a] - eh[a] * c;
-        const float mt = (tmp[0] + tmp[1] + tmp[2]) / 3.0f;
-        const float mb = (enb[0] + enb[1] + enb[2]) / 3.0f;
-        for (int a = 0; a < 3; ++a) eb[a] = mb + kk * (tmp[a] - mt);
-        yb += dotv / (2.0f * mu * hn);
-        mub += -ys * dotv / (2.0f * mu * mu * hn);
-    } else {
-        for (int a = 0; a < 3; ++a) eb[a] = enb[a];
-    }
-    for (int a = 0; a < 3; ++a) Al.a[4 * a] += s[a] > 0.01f ? eb[a] / s[a] : 0.0f;
-    m33 Ftb = mmul(U, mmul(Al, mT(V)));
-    if (!yield) Ftb = madd(Ftb, Fb);  // F = F_trial passes through
-    return Ftb;
-}
-
-// cofactor matrix of m: the derivative of det(m) with respect to m
-MPM_DEV m33 mcof(const m33& m) {
-    const float* a = m.a;
-    m33 c;
-    c.a[0] = a[4] * a[8] - a[5] * a[7];
-    c.a[1] = a[5] * a[6] - a[3] * a[8];
-    c.a[2] = a[3] * a[7] - a[4] * a[6];
-    c.a[3] = a[2] * a[7] - a[1] * a[8];
-    c.a[4] = a[0] * a[8] - a[2] * a[6];
-    c.a[5] = a[1] * a[6] - a[0] * a[7];
-    c.a[6] = a[1] * a[5] - a[2] * a[4];
-    c.a[7] = a[2] * a[3] - a[0] * a[5];
-    c.a[8] = a[0] * a[4] - a[1] * a[3];
-    return c;
-}
-
-// Adjoint of the generalized-Newtonian fluid (mat 10, stress_update_particle's viscous
-// branch, mpm_utils.py:1056-1149 with the stress of stress_viscous): F = Jc I with
-// Jc = det(F_trial)^(1/3), J = Jc^3, and
-//   tau = J (2 eta_app D + p I),  D = dev sym L,  p = -bulk (J^-1.1 - 1),
-//   eta_app = eta + tau_y / gd + K gd^(n-1),  gd = sqrt(2 D:D + eps^2), eps = 0.02.
-// In: Fb = adjoint of F, tb = adjoint of tau. Returns the adjoint of F_trial; Lb is the
-// adjoint of L; the partials of eta, bulk, tau_y, K and n are added.
-MPM_DEV m33 adj_viscous(const m33& Ft, const m33& L, float bulk, float eta, float ty, float K, float n,
-                        const m33& Fb, const m33& tb_in, m33& Lb, float& etab, float& bulkb, float& tyb,
-                        float& Kb, float& nb) {
-    const float eps = 0.02f;
-    const float d = mdet(Ft);
-    const float Jc = powf(d, 1.0f / 3.0f);
-    const float J = Jc * Jc * Jc;
-    const float pJ = powf(J, -1.1f);
-    const float pressure = -bulk * (pJ - 1.0f);
-    m33 D = mscale(madd(L, mT(L)), 0.5f);
-    const float trd = (D.a[0] + D.a[4] + D.a[8]) / 3.0f;
-    D.a[0] -= trd; D.a[4] -= trd; D.a[8] -= trd;
-    float dd = 0.f;
-    for (int i = 0; i < 9; ++i) dd += D.a[i] * D.a[i];
-    const float gd = sqrtf(2.0f * dd + eps * eps);
-    const float gp = powf(gd, n - 1.0f);
-    const float eta_app = eta + ty / gd + K * gp;
-    // the forward symmetrises tau
-    const m33 T = mscale(madd(tb_in, mT(tb_in)), 0.5f);
-    float TD = 0.f;
-    for (int i = 0; i < 9; ++i) TD += T.a[i] * D.a[i];
-    const float trT = T.a[0] + T.a[4] + T.a[8];
-    // tau = J c: Jb = T : c, cb = J T; c = 2 eta_app D + p I
-    float Jb = 2.0f * eta_app * TD + pressure * trT;
-    const float eab = 2.0f * J * TD;
-    const float pb = J * trT;
-    etab += eab;
-    tyb += eab / gd;
-    Kb += eab * gp;
-    nb += eab * K * gp * logf(gd);
-    const float gdb = eab * (-ty / (gd * gd) + K * (n - 1.0f) * gp / gd);
-    // Db = 2 eta_app J T + 2 D (gdb / gd); L enters through the projector dev sym
-    m33 Db = madd(mscale(T, 2.0f * eta_app * J), mscale(D, 2.0f * gdb / gd));
-    const float trb = (Db.a[0] + Db.a[4] + Db.a[8]) / 3.0f;
-    Db.a[0] -= trb; Db.a[4] -= trb; Db.a[8] -= trb;
-    Lb = Db;
-    bulkb += -pb * (pJ - 1.0f);
-    Jb += pb * 1.1f * bulk * pJ / J;
-    const float Jcb = 3.0f * Jc * Jc * Jb + (Fb.a[0] + Fb.a[4] + Fb.a[8]);
-    const float db = Jc != 0.0f ? Jcb / (3.0f * Jc * Jc) : 0.0f;
-    return mscale(mcof(Ft), db);
 }
 
 // ---------------------------------------------------------------- G2P adjoint
@@ -564,6 +403,15 @@ __global__ __launch_bounds__(kThreads) void k_adj_p2g_stress(mpm_particles_t P, 
         A.mu[p] += mub;
         A.lam[p] += lamb;
         A.yield_stress[p] += yb;
+    } else if (mat == 9) {
+        float mub = 0.f, lamb = 0.f, pb[4] = {0.f, 0.f, 0.f, 0.f};
+        Ftb = adj_mu_i(ld9(P.F_trial, S, p), md, mat, dt, P.mu[p], P.lam[p], Fb, tb, mub, lamb, pb);
+        A.mu[p] += mub;
+        A.lam[p] += lamb;
+        A.mui_mu_s[p] += pb[0];
+        A.mui_dmu[p] += pb[1];
+        A.mui_I0[p] += pb[2];
+        A.mui_d[p] += pb[3];
     } else if (mat == 10) {
         float eb = 0.f, bb = 0.f, yb = 0.f, kb = 0.f, nb = 0.f;
         Ftb = adj_viscous(ld9(P.F_trial, S, p), ld9(P.L, S, p), md->bulk[mat], md->plastic_viscosity[mat],
@@ -586,7 +434,8 @@ inline int32_t launch_status() { return hipGetLastError() == hipSuccess ? MPM_OK
 
 inline bool adj_ok(const mpm_adjoint_t* A) {
     return A && A->x && A->v && A->C && A->F_trial && A->F && A->stress && A->mu && A->lam && A->yield_stress &&
-           A->grid_v && A->grid_mv && A->L && A->visc && A->bulk && A->hb_K && A->hb_n;
+           A->grid_v && A->grid_mv && A->L && A->visc && A->bulk && A->hb_K && A->hb_n && A->mui_mu_s &&
+           A->mui_dmu && A->mui_I0 && A->mui_d;
 }
 
 inline bool grid_ok1(const mpm_grid_t* G) {

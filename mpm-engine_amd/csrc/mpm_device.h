@@ -9,7 +9,11 @@
 
 #include "../../include/mpm_hip.h"
 
+// Device functions; a host-side check of the constitutive math defines MPM_DEV as
+// __host__ __device__ before including this header.
+#ifndef MPM_DEV
 #define MPM_DEV __device__ __forceinline__
+#endif
 
 struct v3 { float x, y, z; };
 struct m33 { float a[9]; };  // row-major: a[3*r + c]

@@ -3,12 +3,13 @@
 ``Rollout(solver, dt, substeps)`` takes a configured single-scene ``Solver`` (particles
 loaded, material set, colliders added) and exposes the rollout of `substeps` substeps as
 a ``torch.autograd.Function`` of the per-particle Lame parameters, yield stress, initial
-velocity and initial position, and of the generalized-Newtonian fluid's parameters
-(viscosity, bulk modulus, Herschel-Bulkley K and n of material 10). It returns the final
-positions and the accumulated collider wrench, so a loss on the final state and on the
-plate reaction forces back-propagates to Young's modulus, Poisson's ratio, the yield
-stress (through the per-particle parameters), the fluid's rheology and to the initial
-particle state.
+velocity and initial position, of the generalized-Newtonian fluid's parameters
+(viscosity, bulk modulus, Herschel-Bulkley K and n of material 10) and of the local
+mu(I) coefficients (mu_s, delta_mu, I0, grain diameter of material 9). It returns the
+final positions and the accumulated collider wrench, so a loss on the final state and on
+the plate reaction forces back-propagates to Young's modulus, Poisson's ratio, the yield
+stress (through the per-particle parameters), the fluid's rheology, the granular
+friction law and to the initial particle state.
 
 Forward: fused ticks of `checkpoint_every` substeps (the production substep), storing
 the particle state (x, v, C, F_trial and the velocity gradient L, which the viscous
@@ -36,6 +37,9 @@ _SLOT_PLANES = 51    # + F 9, stress 9, L 9
 _L0 = 42             # first L plane of a slot
 _CKPT_PLANES = 33    # state + L
 FLUID_ID = 10        # the material whose parameters form the `fluid` input
+GRANULAR_ID = 9      # the material whose mu(I) coefficients form the `granular` input
+_FLUID_FIELDS = ("plastic_viscosity", "bulk", "hardening", "softening")
+_GRANULAR_FIELDS = ("muI_mu_s", "muI_delta_mu", "muI_I0", "muI_d")
 
 
 def lame(E, nu):
@@ -46,7 +50,7 @@ def lame(E, nu):
 class Rollout:
     """Differentiable rollout of `substeps` substeps of dt on a configured Solver."""
 
-    SUPPORTED_MATERIALS = frozenset((1, FLUID_ID))
+    SUPPORTED_MATERIALS = frozenset((1, GRANULAR_ID, FLUID_ID))
     SUPPORTED_COLLIDERS = frozenset((native.COLLIDER_PLANE, native.COLLIDER_BOX, native.COLLIDER_WALLS,
                                      native.COLLIDER_POINTCLOUD))
 
@@ -96,8 +100,8 @@ class Rollout:
         # the viscous stress reads L: slots carry it forward and checkpoints keep it
         self.viscous = FLUID_ID in sim._mat_ids
         md = sim.model[0]
-        self.fluid0 = [float(md["plastic_viscosity"][FLUID_ID]), float(md["bulk"][FLUID_ID]),
-                       float(md["hardening"][FLUID_ID]), float(md["softening"][FLUID_ID])]
+        self.fluid0 = [float(md[fld][FLUID_ID]) for fld in _FLUID_FIELDS]
+        self.granular0 = [float(md[fld][GRANULAR_ID]) for fld in _GRANULAR_FIELDS]
         # collider rows of every substep, integrated once from the current poses
         saved = [(list(c.point), np.array(c.rot), list(c.quat)) for c in sim.colliders.items]
         tab, _ = sim.colliders.schedule(sim.time, self.dt, self.substeps)
@@ -131,7 +135,8 @@ class Rollout:
         self.adj = {"x": z(3, n), "v": z(3, n), "C": z(9, n), "F_trial": z(9, n), "F": z(9, n),
                     "stress": z(9, n), "mu": z(n), "lam": z(n), "yield_stress": z(n),
                     "grid_v": z(node_floats), "grid_mv": z(node_floats), "L": z(9, n),
-                    "visc": z(n), "bulk": z(n), "hb_K": z(n), "hb_n": z(n)}
+                    "visc": z(n), "bulk": z(n), "hb_K": z(n), "hb_n": z(n),
+                    "mui_mu_s": z(n), "mui_dmu": z(n), "mui_I0": z(n), "mui_d": z(n)}
         A = native.Adjoint()
         for k, t in self.adj.items():
             setattr(A, k, native.ptr(t))
@@ -182,12 +187,12 @@ class Rollout:
         return out
 
     # ------------------------------------------------------------------ forward
-    def _set_state(self, mu, lam, ys, v0, x0, fluid):
+    def _set_state(self, mu, lam, ys, v0, x0, fluid, granular):
         sim = self.sim
         md = sim.model[0]
-        vals = [float(v) for v in fluid.cpu()]
-        for fld, val in zip(("plastic_viscosity", "bulk", "hardening", "softening"), vals):
-            md[fld][FLUID_ID] = val
+        for fields, mat, vals in ((_FLUID_FIELDS, FLUID_ID, fluid), (_GRANULAR_FIELDS, GRANULAR_ID, granular)):
+            for fld, val in zip(fields, vals.cpu().tolist()):
+                md[fld][mat] = val
         sim._model_dirty = True
         sim.mu.copy_(self._to_slots(mu))
         sim.lam.copy_(self._to_slots(lam))
@@ -210,11 +215,11 @@ class Rollout:
             native.ptr(wrench) if slots else None, slots, native.ptr(self.ws) if slots else None,
             dt, flags, stream), "grid_update")
 
-    def forward(self, mu, lam, ys, v0, x0, fluid):
+    def forward(self, mu, lam, ys, v0, x0, fluid, granular):
         """Fused forward rollout from (x0, v0); returns (x_T (n, 3), wrench (slots, 6))."""
         sim = self.sim
         torch = self.torch
-        self._set_state(mu, lam, ys, v0, x0, fluid)
+        self._set_state(mu, lam, ys, v0, x0, fluid, granular)
         self.wrench.zero_()
         sim.error_flag.zero_()
         s = sim.stream()
@@ -295,7 +300,7 @@ class Rollout:
 
     def backward(self, x_bar, wrench_bar):
         """Adjoint of the last forward: returns (mu_bar, lam_bar, ys_bar, v0_bar, x0_bar,
-        fluid_bar)."""
+        fluid_bar, granular_bar)."""
         sim = self.sim
         torch = self.torch
         a = self.adj
@@ -324,20 +329,32 @@ class Rollout:
         if flag & (native.ERR_ADJ_MATERIAL | native.ERR_ADJ_COLLIDER):
             raise NotImplementedError(f"a material or collider without an adjoint was met (flag {flag})")
         fluid_bar = torch.stack([a[k].double().sum() for k in ("visc", "bulk", "hb_K", "hb_n")])
+        granular_bar = torch.stack([a[k].double().sum() for k in ("mui_mu_s", "mui_dmu", "mui_I0", "mui_d")])
         return (self._to_user(a["mu"]), self._to_user(a["lam"]), self._to_user(a["yield_stress"]),
-                self._to_user(a["v"].t()), self._to_user(a["x"].t()), fluid_bar)
+                self._to_user(a["v"].t()), self._to_user(a["x"].t()), fluid_bar, granular_bar)
 
     # ------------------------------------------------------------------ autograd
-    def __call__(self, mu, lam, yield_stress, v0, x0=None, fluid=None):
+    def __call__(self, mu, lam, yield_stress, v0, x0=None, fluid=None, granular=None):
         """(x_T, wrench) as a differentiable function of per-particle mu, lam, yield
-        stress (n,), initial velocity (n, 3), initial position (n, 3) and the fluid
-        parameters (4,) = (viscosity, bulk modulus, K, n) of material 10 (default: the
-        solver's values)."""
+        stress (n,), initial velocity (n, 3), initial position (n, 3), the fluid
+        parameters (4,) = (viscosity, bulk modulus, K, n) of material 10 and the mu(I)
+        coefficients (4,) = (mu_s, delta_mu, I0, grain diameter) of material 9 (default:
+        the solver's values)."""
+        torch = self.torch
         if x0 is None:
             x0 = self.x0
         if fluid is None:
-            fluid = self.torch.tensor(self.fluid0, dtype=self.torch.float32, device=self.sim.device)
-        return _RolloutFn.apply(self, mu, lam, yield_stress, v0, x0, fluid)
+            fluid = torch.tensor(self.fluid0, dtype=torch.float32, device=self.sim.device)
+        if granular is None:
+            granular = torch.tensor(self.granular0, dtype=torch.float32, device=self.sim.device)
+        return _RolloutFn.apply(self, mu, lam, yield_stress, v0, x0, fluid, granular)
+
+    def _scalars(self, given, defaults):
+        torch = self.torch
+        dev = self.sim.device
+        vals = [torch.as_tensor(defaults[i], dtype=torch.float32, device=dev) if v is None else v
+                for i, v in enumerate(given)]
+        return torch.stack([v.to(torch.float32).reshape(()) for v in vals])
 
     def run(self, E, nu, yield_stress, v0, x0=None):
         """Rollout as a function of the material's E, nu (scalars, broadcast to every
@@ -353,13 +370,29 @@ class Rollout:
         modulus, Herschel-Bulkley K and n (scalars; None keeps the solver's value) and
         yield stress (scalar or (n,)); the Lame parameters keep the solver's values."""
         torch = self.torch
-        dev = self.sim.device
-        vals = [torch.as_tensor(self.fluid0[i], dtype=torch.float32, device=dev) if v is None else v
-                for i, v in enumerate((viscosity, bulk, K, n))]
-        fluid = torch.stack([v.to(torch.float32).reshape(()) for v in vals])
-        ones = torch.ones(self.n, dtype=torch.float32, device=dev)
+        fluid = self._scalars((viscosity, bulk, K, n), self.fluid0)
+        ones = torch.ones(self.n, dtype=torch.float32, device=self.sim.device)
         ys = yield_stress * ones if yield_stress.dim() == 0 else yield_stress
         return self(self._to_user(self.sim.mu).clone(), self._to_user(self.sim.lam).clone(), ys, v0, x0, fluid)
+
+    def run_granular(self, v0, mu_s=None, delta_mu=None, I0=None, grain_diameter=None, E=None, nu=None,
+                     x0=None):
+        """Rollout as a function of the local mu(I) coefficients mu_s, delta_mu, I0 and the
+        grain diameter (scalars; None keeps the solver's value) and of E, nu (scalars
+        broadcast through compute_mu_lam_from_E_nu; None keeps the solver's mu, lam)."""
+        torch = self.torch
+        granular = self._scalars((mu_s, delta_mu, I0, grain_diameter), self.granular0)
+        ones = torch.ones(self.n, dtype=torch.float32, device=self.sim.device)
+        if E is None and nu is None:
+            mu, lam = self._to_user(self.sim.mu).clone(), self._to_user(self.sim.lam).clone()
+        else:
+            md = self.sim.model[0]
+            E = torch.as_tensor(float(md["E"][GRANULAR_ID])) if E is None else E
+            nu = torch.as_tensor(float(md["nu"][GRANULAR_ID])) if nu is None else nu
+            m, l_ = lame(E, nu)
+            mu, lam = m * ones, l_ * ones
+        ys = self._to_user(self.sim.yield_stress).clone()
+        return self(mu, lam, ys, v0, x0, None, granular)
 
 
 def _as_f32(t, torch):
@@ -371,11 +404,12 @@ def _make_fn():
 
     class RolloutFn(torch.autograd.Function):
         @staticmethod
-        def forward(ctx, ro, mu, lam, ys, v0, x0, fluid):
+        def forward(ctx, ro, mu, lam, ys, v0, x0, fluid, granular):
             ctx.ro = ro
-            ctx.dtypes = (mu.dtype, lam.dtype, ys.dtype, v0.dtype, x0.dtype, fluid.dtype)
+            ctx.dtypes = (mu.dtype, lam.dtype, ys.dtype, v0.dtype, x0.dtype, fluid.dtype, granular.dtype)
             xT, w = ro.forward(_as_f32(mu, torch), _as_f32(lam, torch), _as_f32(ys, torch),
-                               _as_f32(v0, torch), _as_f32(x0, torch), _as_f32(fluid, torch))
+                               _as_f32(v0, torch), _as_f32(x0, torch), _as_f32(fluid, torch),
+                               _as_f32(granular, torch))
             return xT, w
 
         @staticmethod
@@ -383,11 +417,11 @@ def _make_fn():
             ro = ctx.ro
             if xT_bar is None:
                 xT_bar = torch.zeros(ro.n, 3, dtype=torch.float32, device=ro.sim.device)
-            mub, lamb, yb, vb, xb, fb = ro.backward(_as_f32(xT_bar, torch),
-                                                    None if w_bar is None else _as_f32(w_bar, torch))
+            mub, lamb, yb, vb, xb, fb, gb = ro.backward(_as_f32(xT_bar, torch),
+                                                        None if w_bar is None else _as_f32(w_bar, torch))
             dts = ctx.dtypes
             return (None, mub.to(dts[0]), lamb.to(dts[1]), yb.to(dts[2]), vb.to(dts[3]), xb.to(dts[4]),
-                    fb.to(dts[5]))
+                    fb.to(dts[5]), gb.to(dts[6]))
 
     return RolloutFn
 

@@ -41,7 +41,7 @@ class Particles(C.Structure):
 class Adjoint(C.Structure):
     _fields_ = [(nm, _FP) for nm in ("x", "v", "C", "F_trial", "F", "stress", "mu", "lam",
                                      "yield_stress", "grid_v", "grid_mv", "L", "visc", "bulk",
-                                     "hb_K", "hb_n")]
+                                     "hb_K", "hb_n", "mui_mu_s", "mui_dmu", "mui_I0", "mui_d")]
 
 
 MAX_CDF = 4

@@ -416,3 +416,144 @@ def test_graph_replayed_backward_matches_eager():
         g_graph, v_graph = grads(True)
         assert np.allclose(g_graph, g_eager, rtol=2e-3, atol=0), (g_graph, g_eager)
         assert np.abs(v_graph - v_eager).max() <= 2e-3 * np.abs(v_eager).max()
+
+
+def _granular():
+    from mpm_amd.materials import granular
+
+    return granular(mu_s=0.38, delta_mu=0.26, I0=0.28, density=1550.0, grain_diameter=1e-3, grain_density=2650.0,
+                    E=1.0e6, nu=0.3)
+
+
+def _granular_states(n, rng, G, lam, mu_s):
+    """F_trial = diag(exp(tr / 3 + dn (2, -1, -1) / sqrt 6)): 85 % compressed (tr e in
+    [-0.03, -0.005]) with a deviatoric part whose yield ratio r = tau_K / (mu_s p_K) lies
+    in [0.2, 0.8] or [1.25, 3] (no state within 20 % of the surface), 15 % in tension
+    (tr e > 0). One principal frame and one deviatoric direction for every particle, so
+    the yielding particles' shear adds up in the plate impulse instead of cancelling
+    between random orientations (which would leave the coefficient derivatives at the
+    fp32 rounding of the pressure-dominated impulse sum)."""
+    d = np.array([2.0, -1.0, -1.0]) / np.sqrt(6.0)
+    Ft = np.zeros((n, 3, 3))
+    for p in range(n):
+        if rng.uniform() < 0.15:
+            tr, dn = rng.uniform(0.002, 0.01), rng.uniform(0.0, 0.02)
+        else:
+            tr = -rng.uniform(0.005, 0.03)
+            r = rng.uniform(0.2, 0.8) if rng.uniform() < 0.5 else rng.uniform(1.25, 3.0)
+            dn = r * mu_s * (2.0 * G / 3.0 + lam) * -tr / (np.sqrt(2.0) * G)
+        Ft[p] = np.diag(np.exp(tr / 3.0 + dn * d))
+    return Ft.astype(np.float32)
+
+
+def test_granular_one_substep_coefficient_directions():
+    """Local mu(I) (material 9; mu_s 0.38, delta_mu 0.26, I0 0.28, d 1 mm, rho_s 2650,
+    E 1e6, nu 0.3), one substep from random F_trial states on both sides of the yield
+    surface and in tension, plates standing still: d (plate x impulses) / d (mu_s,
+    delta_mu, I0, grain diameter, E, nu) against central differences of the GPU forward
+    at relative steps 2e-2. The states keep 20 % from the yield surface, so no step
+    moves a particle across it (where the return map's derivative jumps); the yielding
+    ones carry the root of the mu(I) return map, whose derivative the adjoint takes from
+    the implicit function theorem."""
+    import torch
+
+    sc = _scene(plate_speed=0.0)
+    s = sc.build(Solver(grid=sc.grid, device="hip"))
+    s.set_material(_granular())
+    n = s.n_particles
+    sim = s._sim
+    E0, nu0 = 1.0e6, 0.3
+    G, lam = _lame(E0, nu0)
+    rng = np.random.default_rng(21)
+    Ft0 = _granular_states(n, rng, G, lam, 0.38)
+    sim.F_trial.copy_(torch.from_numpy(Ft0.reshape(n, 9).T.copy()))
+    ro = Rollout(s, DT, 1, checkpoint_every=1)
+    dev = sim.device
+    cw = np.zeros((2, 3))
+    cw[:, 0] = rng.standard_normal(2)
+    par = {"mu_s": 0.38, "delta_mu": 0.26, "I0": 0.28, "grain_diameter": 1e-3, "E": E0, "nu": nu0}
+    tens = {k: torch.tensor(v, dtype=torch.float64, device=dev, requires_grad=True) for k, v in par.items()}
+    v = torch.zeros(n, 3, device=dev)
+    _, w = ro.run_granular(v, **tens)
+    lw = (w[:, :3].double() * torch.from_numpy(cw).to(dev)).sum()
+    grads = dict(zip(tens, torch.autograd.grad(lw, list(tens.values()))))
+
+    def impulse(**kw):
+        args = dict(par)
+        args.update(kw)
+        with torch.no_grad():
+            _, wv = ro.run_granular(v, **{k: torch.tensor(a, dtype=torch.float64, device=dev)
+                                          for k, a in args.items()})
+        return float((wv[:, :3].double().cpu().numpy() * cw).sum())
+
+    for name in par:
+        h = 2e-2 * par[name]
+        fd = (impulse(**{name: par[name] + h}) - impulse(**{name: par[name] - h})) / (2 * h)
+        ad = grads[name].item()
+        print("w", name, ad, fd)
+        assert abs(ad - fd) <= 2e-3 * abs(fd), f"w {name}: adjoint {ad:.6e} vs central difference {fd:.6e}"
+
+
+def test_granular_rollout_gradients_match_oracle_differences():
+    """60 substeps (checkpoints every 20) of the mu(I) block relaxing between the plates
+    standing still, gravity off, from the random compressed / sheared / tensile F_trial
+    states of the one-substep test: the plates take the granular pressure and the
+    yielding particles' shear. Loss g . (plate impulses), random g; d / d (mu_s, delta_mu,
+    I0, grain diameter) and a smooth initial-velocity direction against central
+    differences of the oracle rollout (relative steps 2e-2). Particles cross the yield
+    surface during the rollout, so the loss has kinks in the coefficients: over steps
+    from 5e-3 to 4e-2 the differences for I0 and the grain diameter (which act only
+    through the weak I-dependence of mu) scatter by +-1 % around the adjoint with no
+    trend, so they are held to 2e-2; mu_s, delta_mu and the velocity direction to 5e-3.
+    The one-substep test pins every coefficient to 2e-3."""
+    import torch
+
+    no_g = {"g": [0.0, 0.0, 0.0]}
+    sc = _scene(plate_speed=0.0)
+    s = sc.build(Solver(grid=sc.grid, device="hip"))
+    s.set_material(_granular(), **no_g)
+    n = s.n_particles
+    sim = s._sim
+    G, lam = _lame(1.0e6, 0.3)
+    rng = np.random.default_rng(21)
+    Ft0 = _granular_states(n, rng, G, lam, 0.38)
+    sim.F_trial.copy_(torch.from_numpy(Ft0.reshape(n, 9).T.copy()))
+    S = 60
+    ro = Rollout(s, DT, S, checkpoint_every=20)
+    dev = sim.device
+    par = {"mu_s": 0.38, "delta_mu": 0.26, "I0": 0.28, "grain_diameter": 1e-3}
+    tens = {k: torch.tensor(v, dtype=torch.float64, device=dev, requires_grad=True) for k, v in par.items()}
+    xs = sc.pos.astype(np.float64)
+    v_base = (0.05 * np.stack([np.sin(7 * xs[:, 1]), np.cos(5 * xs[:, 2]), np.sin(6 * xs[:, 0])], 1)).astype(np.float32)
+    v0 = torch.from_numpy(v_base).to(dev).requires_grad_(True)
+    _, w = ro.run_granular(v0, **tens)
+    gw = rng.standard_normal((2, 3))
+    (w[:, :3].double() * torch.from_numpy(gw).to(dev)).sum().backward()
+    g = {k: t.grad.item() for k, t in tens.items()}
+
+    def linearised(p, v_):
+        from mpm_amd.materials import granular
+
+        o, plates = _oracle(sc, 0.0, 0.0, 0.0, v_, Ft0=Ft0)
+        o.set_material(granular(mu_s=p["mu_s"], delta_mu=p["delta_mu"], I0=p["I0"], density=1550.0,
+                                grain_diameter=p["grain_diameter"], grain_density=2650.0, E=1.0e6, nu=0.3), **no_g)
+        o._sim.v[:] = v_
+        o.step(DT, S)
+        imp = np.stack([o.tool_force(hnd, 1.0) for hnd in plates]).astype(np.float64)
+        return (imp * gw).sum()
+
+    report = {}
+    for name in par:
+        h = 2e-2 * par[name]
+        pp, pm = dict(par), dict(par)
+        pp[name] += h
+        pm[name] -= h
+        report[name] = (g[name], (linearised(pp, v_base) - linearised(pm, v_base)) / (2 * h))
+    d = np.stack([np.cos(4 * xs[:, 2] + 1), np.sin(5 * xs[:, 0]), np.cos(6 * xs[:, 1] + 2)], 1).astype(np.float32)
+    hv = 1e-3
+    report["v0.d"] = (float((v0.grad.double().cpu().numpy() * d).sum()),
+                      (linearised(par, v_base + hv * d) - linearised(par, v_base - hv * d)) / (2 * hv))
+    print(report)
+    tol = {"I0": 2e-2, "grain_diameter": 2e-2}
+    for k, (a, f) in report.items():
+        assert abs(a - f) <= tol.get(k, 5e-3) * abs(f), f"{k}: adjoint {a:.6e} vs central difference {f:.6e}"
