@@ -323,3 +323,221 @@ def test_cup_sdf_matches_host_twin():
     assert touched[deep].all()
     # nodes clearly outside the band are never touched
     assert not touched[sdf > band + 1e-6].any()
+
+
+# ------------------------------------------- sand, plasticine, mu(I)-Phi(I), tabulated mu(I)
+def _rot(rng):
+    q = np.linalg.qr(rng.normal(size=(3, 3)))[0]
+    return q * np.sign(np.linalg.det(q))
+
+
+def _frame(rng, e):
+    """F = R1 diag(exp(e)) R2^T in float32 with random proper rotations."""
+    R1, R2 = _rot(rng), _rot(rng)
+    return (R1 @ np.diag(np.exp(e)) @ R2.T).astype(f32), R1, R2
+
+
+def _run(F, mat, Jp=None, dt=1e-4, **kw):
+    s = _single([[0.5, 0.5, 0.5]], mat=mat, **kw)
+    s.F_trial[0] = np.asarray(F, f32)
+    if Jp is not None:
+        s.Jp[0] = Jp
+    O.lib().o_stress_particle(s._state(), s._model(), dt, 0)
+    return s
+
+
+def _logsv(F):
+    return np.log(np.linalg.svd(np.asarray(F, np.float64), compute_uv=False))
+
+
+def test_sand_cone_projection_closed_form():
+    """Drucker-Prager return (mat 2, sand_return_mapping mpm_utils.py:543-580) with
+    alpha = sqrt(2/3) 2 sin(phi) / (3 - sin(phi)) from the friction angle: inside the cone
+    (|dev e| + c tr e <= 0, c = (3 lam + 2 mu) / (2 mu) alpha) F is untouched; outside it
+    in tension F -> U V^T (scipy's polar rotation); outside it in compression the
+    log-strain is projected onto the cone along its deviatoric part: trace kept,
+    |dev e_new| = -c tr e, deviatoric direction and principal frame kept."""
+    E, nu, phi = 1e5, 0.3, 30.0
+    mu, lam = E / 2.6, E * 0.3 / (1.3 * 0.4)
+    sp = np.sin(np.radians(phi))
+    c = (3 * lam + 2 * mu) / (2 * mu) * np.sqrt(2.0 / 3.0) * 2 * sp / (3 - sp)
+    rng = np.random.default_rng(31)
+    for kind in ("inside", "outside", "tension"):
+        for _ in range(5):
+            d = rng.normal(size=3)
+            d -= d.mean()
+            d /= np.linalg.norm(d)
+            tr = 0.01 if kind == "tension" else -0.03
+            mag = {"inside": 0.5, "outside": 2.0, "tension": 1.0}[kind] * c * 0.03
+            e = tr / 3 + mag * d
+            F, R1, R2 = _frame(rng, e)
+            s = _run(F, "sand", E=E, nu=nu, friction_angle=phi)
+            Fn = s.F[0].astype(np.float64)
+            if kind == "inside":
+                np.testing.assert_array_equal(s.F[0], F)
+            elif kind == "tension":
+                R, _ = scipy.linalg.polar(F.astype(np.float64))
+                np.testing.assert_allclose(Fn, R, atol=3e-6)
+            else:
+                e_new = tr / 3 + (-c * tr) * d
+                np.testing.assert_allclose(Fn, R1 @ np.diag(np.exp(e_new)) @ R2.T, atol=3e-6)
+                ln = np.sort(_logsv(Fn))
+                np.testing.assert_allclose(ln.sum(), tr, atol=2e-6)
+                np.testing.assert_allclose(np.linalg.norm(ln - ln.mean()), -c * tr, rtol=2e-4)
+
+
+def test_drucker_prager_stress_equals_hencky():
+    """kirchoff_stress_drucker_prager (mpm_utils.py:109-124), U diag((2 mu log s + lam
+    sum log s) / s) V^T F^T, is the Hencky stress U diag(2 mu log s + lam sum log s) U^T
+    (V^T F^T = diag(s) U^T); checked on the stress of the returned F for states inside the
+    cone, where F is the trial F."""
+    E, nu = 1e5, 0.3
+    mu, lam = E / 2.6, E * 0.3 / (1.3 * 0.4)
+    rng = np.random.default_rng(32)
+    for _ in range(10):
+        e = -0.02 + 0.002 * rng.normal(size=3)
+        F, _, _ = _frame(rng, e)
+        s = _run(F, "sand", E=E, nu=nu, friction_angle=40.0)
+        U, sv, _ = np.linalg.svd(s.F[0].astype(np.float64))
+        ls = np.log(sv)
+        tau = U @ np.diag(2 * mu * ls + lam * ls.sum()) @ U.T
+        np.testing.assert_allclose(s.stress[0], tau, rtol=1e-4, atol=1e-4 * np.abs(tau).max())
+
+
+def test_plasticine_softening_and_damage():
+    """von_mises_return_mapping_with_damage (mat 5, mpm_utils.py:173-225) with FCR stress:
+    a yielding state returns like von Mises and the yield stress drops by softening x
+    |plastic log-strain increment| = softening x dg |h| / (|h| + 1e-6); a large softening
+    drives it to <= 0, which zeroes mu and lam (so the FCR stress vanishes) while F still
+    takes the return computed with the old yield stress; a particle already at yield
+    stress <= 0 keeps its trial F."""
+    E, nu, ys, soft = 2e5, 0.3, 1e3, 5e3
+    mu, lam = E / 2.6, E * 0.3 / (1.3 * 0.4)
+    F = np.diag([1.05, 0.97, 0.99]).astype(f32)
+    eps = np.log(np.diag(F).astype(np.float64))
+    h = eps - eps.mean()
+    hn = np.linalg.norm(h) + 1e-6
+    dg = hn - ys / (2 * mu)
+    e_new = eps - dg / hn * h
+    s = _run(F, "plasticine", E=E, nu=nu, yield_stress=ys, softening=soft)
+    np.testing.assert_allclose(np.diag(s.F[0]), np.exp(e_new), rtol=2e-5)
+    np.testing.assert_allclose(s.yield_stress[0], ys - soft * dg * np.linalg.norm(h) / hn, rtol=1e-4)
+    Fe = np.diag(np.exp(e_new))
+    J = np.prod(np.exp(e_new))
+    tau = 2 * mu * (Fe - np.eye(3)) @ Fe.T + lam * J * (J - 1) * np.eye(3)  # FCR, R = I
+    np.testing.assert_allclose(s.stress[0], tau, rtol=1e-3, atol=1e-2)
+    # damage to zero
+    d = _run(F, "plasticine", E=E, nu=nu, yield_stress=ys, softening=1e9)
+    assert d.yield_stress[0] <= 0 and d.mu[0] == 0 and d.lam[0] == 0
+    np.testing.assert_allclose(np.diag(d.F[0]), np.exp(e_new), rtol=2e-5)
+    np.testing.assert_array_equal(d.stress[0], np.zeros((3, 3), f32))
+    # fully damaged already: the trial F is kept
+    z = _run(F, "plasticine", E=E, nu=nu, yield_stress=0.0, softening=soft)
+    np.testing.assert_array_equal(z.F[0], F)
+
+
+def _phi_setup(E=1e6, nu=0.3):
+    return dict(E=E, nu=nu, mu_s=0.38, delta_mu=0.26, I0=0.28, grain_diameter=1e-3, grain_density=2650.0,
+                phi_chi=0.2)
+
+
+def test_mu_i_phi_pressure_branches_and_reference_volume():
+    """mu_i_phi_return_mapping (mat 11, mpm_utils.py:434-522) with the stress of
+    :525-540: compaction pressure p = max(K (Jp_ref / J - 1), 0), K = E / (3 (1 - 2 nu)),
+    Jp_ref = Jp or 1 when Jp < 0.5. No pressure (J >= Jp_ref): F -> exp(mean e) U V^T and
+    zero stress. Pressure below the mu_s cone: F kept, Jp = Jp_ref, tau = U diag(2 G dev e
+    - p J) U^T. A stored Jp of 0.8 (>= 0.5) is the reference volume, so a 1 % compression
+    is still dilated (no pressure)."""
+    kw = _phi_setup()
+    E, nu = kw["E"], kw["nu"]
+    G = E / (2 * (1 + nu))
+    K = E / (3 * (1 - 2 * nu))
+    rng = np.random.default_rng(33)
+    # expansion: isotropic return, zero stress
+    F, R1, R2 = _frame(rng, np.array([0.012, -0.002, 0.0]))
+    s = _run(F, "mu_i_phi", **kw)
+    e = _logsv(F)
+    np.testing.assert_allclose(s.F[0], np.exp(e.mean()) * (R1 @ R2.T), atol=3e-6)
+    assert s.Jp[0] == 1.0
+    np.testing.assert_allclose(s.stress[0], 0.0, atol=1e-2)
+    # compression below the cone
+    e = np.array([-0.004, -0.006, -0.005])
+    F, R1, _ = _frame(rng, e)
+    s = _run(F, "mu_i_phi", **kw)
+    np.testing.assert_array_equal(s.F[0], F)
+    J = np.exp(e.sum())
+    p = K * (1.0 / J - 1.0)
+    assert np.sqrt(2) * G * np.linalg.norm(e - e.mean()) < 0.38 * p * J  # below the cone
+    tau = R1 @ np.diag(2 * G * (e - e.mean()) - p * J) @ R1.T
+    np.testing.assert_allclose(s.stress[0], tau, rtol=2e-4, atol=2e-4 * np.abs(tau).max())
+    np.testing.assert_allclose(s.Jp[0], 1.0)
+    # stored reference volume 0.8: a 1 % compression has no pressure
+    F, R1, R2 = _frame(rng, np.array([-0.004, -0.003, -0.003]))
+    s = _run(F, "mu_i_phi", Jp=0.8, **kw)
+    np.testing.assert_allclose(s.F[0], np.exp(_logsv(F).mean()) * (R1 @ R2.T), atol=3e-6)
+    np.testing.assert_allclose(s.Jp[0], 0.8)
+
+
+def test_mu_i_phi_yield_root_and_dilatancy():
+    """Yielding mu(I)-Phi(I) state: gdp solves tau_bar - G dt gdp - mu(I) p_K = 0 with
+    p_K = p J, I = gdp d sqrt(rho_s / p) (scipy brentq on the same g), the deviatoric
+    log-strain is scaled by (tau_bar - G dt gdp) / tau_bar, the reference volume relaxes
+    as Jp = Jp_ref + min(8 gdp dt, 1) (j_cs - Jp_ref) with j_cs = 1 / max(1 - chi I /
+    (I + I0), 1e-3), and the stress uses the returned F and the updated Jp."""
+    kw = _phi_setup()
+    E, nu, dt = kw["E"], kw["nu"], 1e-4
+    G = E / (2 * (1 + nu))
+    K = E / (3 * (1 - 2 * nu))
+    rng = np.random.default_rng(34)
+    e = np.array([0.02, -0.035, -0.015])
+    F, R1, R2 = _frame(rng, e)
+    s = _run(F, "mu_i_phi", dt=dt, **kw)
+    e = _logsv(F)[[0, 1, 2]]
+    # frame-free checks on singular values (sorted)
+    J = np.exp(e.sum())
+    p = K * (1.0 / J - 1.0)
+    pK = p * J
+    tb = np.sqrt(2) * G * np.linalg.norm(e - e.mean())
+    assert p > 0 and tb > 0.38 * pK
+    Ic = 1e-3 * np.sqrt(2650.0 / p)
+    g = lambda x: tb - G * dt * x - (0.38 + 0.26 * x * Ic / (x * Ic + 0.28)) * pK  # noqa: E731
+    gd = scipy.optimize.brentq(g, 0.0, tb / (G * dt), xtol=1e-14)
+    scale = (tb - G * dt * gd) / tb
+    e_new = np.sort(e.mean() + scale * (e - e.mean()))
+    np.testing.assert_allclose(np.sort(_logsv(s.F[0])), e_new, atol=3e-6)
+    I = gd * Ic
+    jcs = 1.0 / max(1.0 - 0.2 * I / (I + 0.28), 1e-3)
+    Jp = 1.0 + min(8.0 * gd * dt, 1.0) * (jcs - 1.0)
+    np.testing.assert_allclose(s.Jp[0], Jp, rtol=1e-5)
+    # stress of the returned F with the new reference volume
+    U, sv, _ = np.linalg.svd(s.F[0].astype(np.float64))
+    ln = np.log(sv)
+    Jn = np.prod(sv)
+    pn = max(K * (Jp / Jn - 1.0), 0.0)
+    tau = U @ np.diag(2 * G * (ln - ln.mean()) - pn * Jn) @ U.T
+    np.testing.assert_allclose(s.stress[0], tau, rtol=2e-4, atol=2e-4 * np.abs(tau).max())
+
+
+def test_tabulated_mu_i_equals_parametric_and_uses_table_head_as_threshold():
+    """mu_i_tabulated_return_mapping (mat 13, mpm_utils.py:328-407): with a table sampling
+    the parametric mu(I) on log10 I in [-6, 1] the return equals the parametric mu(I)
+    return (mat 9) to interpolation error; the yield threshold is the table's first entry
+    (a table starting at 5 keeps a state the parametric law yields)."""
+    E, nu, dt = 1e6, 0.3, 1e-4
+    smin, smax, n = -6.0, 1.0, 4001
+    sg = np.linspace(smin, smax, n)
+    I = 10.0 ** sg
+    tab = 0.38 + 0.26 * I / (I + 0.28)
+    rng = np.random.default_rng(35)
+    e = np.array([0.02, -0.035, -0.015])
+    F, _, _ = _frame(rng, e)
+    common = dict(E=E, nu=nu, grain_diameter=1e-3, grain_density=2650.0, dt=dt)
+    a = _run(F, "mu_i_sand", mu_s=0.38, delta_mu=0.26, I0=0.28, **common)
+    b = _run(F, "tabulated_mu_i", eta_table=tab, eta_table_smin=smin, eta_table_smax=smax, **common)
+    assert np.abs(a.F[0] - F).max() > 1e-3  # the parametric law yields
+    np.testing.assert_allclose(b.F[0], a.F[0], atol=2e-6)
+    np.testing.assert_allclose(b.stress[0], a.stress[0], rtol=1e-3, atol=1e-3 * np.abs(a.stress[0]).max())
+    high = tab.copy()
+    high[0] = 5.0
+    c = _run(F, "tabulated_mu_i", eta_table=high, eta_table_smin=smin, eta_table_smax=smax, **common)
+    np.testing.assert_array_equal(c.F[0], F)
