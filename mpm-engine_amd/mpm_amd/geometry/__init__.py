@@ -11,7 +11,9 @@ mpm_mesh_field / mpm_segment_distance (device="hip", exact distances).
 """
 from __future__ import annotations
 
+import hashlib
 from dataclasses import dataclass
+from pathlib import Path
 
 import numpy as np
 
@@ -331,6 +333,8 @@ def revolve_profile_open(profile, n_theta: int = 64):
 
 
 def save_cdf(cdf: CDFData, path) -> None:
+    path = Path(path)
+    path.parent.mkdir(parents=True, exist_ok=True)
     np.savez_compressed(path, values=cdf.values, valid=cdf.valid, origin=cdf.origin,
                         cell=cdf.cell, band=cdf.band)
 
@@ -341,6 +345,79 @@ def load_cdf(path) -> CDFData:
                    cell=float(d["cell"]), band=float(d["band"]))
 
 
+def save_sdf(sdf: SDFData, path) -> None:
+    """npz of an SDFData (mesh_sdf.py:538-542 layout: values, grads, origin, cell, sdf_max)."""
+    path = Path(path)
+    path.parent.mkdir(parents=True, exist_ok=True)
+    np.savez_compressed(path, values=sdf.values, grads=sdf.grads, origin=sdf.origin,
+                        cell=sdf.cell, sdf_max=sdf.sdf_max)
+
+
+def load_sdf(path) -> SDFData:
+    d = np.load(path)
+    return SDFData(values=d["values"], grads=d["grads"], origin=d["origin"],
+                   cell=float(d["cell"]), sdf_max=float(d["sdf_max"]))
+
+
+# Versions of the cached field formats; bump when a builder's output changes.
+_SDF_CACHE_VERSION = 1
+_CDF_CACHE_VERSION = 1
+
+
+def _field_key(version, verts, faces, params, device, probe=None) -> str:
+    """First 16 hex digits of a SHA-1 over the format version, the geometry (float64
+    vertices, int64 faces), the grid parameters, the builder and, for SDFs, the sign
+    probe (the keying of mesh_sdf.py:496-530, plus the builder, since the host and HIP
+    voxelisers produce different fields)."""
+    h = hashlib.sha1()
+    for arr in (np.array([version], dtype=np.int64),
+                np.ascontiguousarray(verts, dtype=np.float64),
+                np.ascontiguousarray(faces, dtype=np.int64),
+                np.asarray(params, dtype=np.float64)):
+        h.update(arr.tobytes())
+    h.update(device.encode())
+    if probe is not None:
+        h.update(np.ascontiguousarray(probe, dtype=np.float64).tobytes())
+    return h.hexdigest()[:16]
+
+
+def build_sdf_cached(verts, faces, res: int = 64, margin_cells: float = 4.0, cache_dir=None,
+                     interior_probe=None, device: str = "cpu") -> SDFData:
+    """build_sdf with a versioned on-disk cache keyed by geometry, grid, builder and sign
+    probe (build_sdf_cached, mesh_sdf.py:553-563); cache_dir=None builds without caching."""
+    if cache_dir is None:
+        return build_sdf(verts, faces, res, margin_cells, interior_probe, device=device)
+    v64 = np.asarray(verts, dtype=np.float64)
+    probe = 0.5 * (v64.min(axis=0) + v64.max(axis=0)) if interior_probe is None else \
+        np.asarray(interior_probe, dtype=np.float64)
+    if probe.shape != (3,) or not np.isfinite(probe).all():
+        raise ValueError(f"interior_probe must be a finite 3-vector, got {probe}")
+    key = _field_key(_SDF_CACHE_VERSION, v64, faces, [res, margin_cells], device, probe)
+    path = Path(cache_dir) / f"sdf_{key}.npz"
+    if path.exists():
+        return load_sdf(path)
+    sdf = build_sdf(verts, faces, res, margin_cells, interior_probe, device=device)
+    save_sdf(sdf, path)
+    return sdf
+
+
+def build_surface_cdf_cached(verts, faces, res: int = 64, band_cells: float = 3.0,
+                             margin_cells: float = 4.0, cache_dir=None, device: str = "cpu") -> CDFData:
+    """build_surface_cdf with a versioned on-disk cache keyed by geometry, grid and builder
+    (build_surface_cdf_cached, mesh_sdf.py:496-514); cache_dir=None builds without
+    caching."""
+    if cache_dir is None:
+        return build_surface_cdf(verts, faces, res, band_cells, margin_cells, device=device)
+    key = _field_key(_CDF_CACHE_VERSION, verts, faces, [res, band_cells, margin_cells], device)
+    path = Path(cache_dir) / f"cdf_{key}.npz"
+    if path.exists():
+        return load_cdf(path)
+    cdf = build_surface_cdf(verts, faces, res, band_cells, margin_cells, device=device)
+    save_cdf(cdf, path)
+    return cdf
+
+
 __all__ = ["SDFData", "sdf_from_function", "sphere_sdf", "CDFData", "build_surface_cdf",
            "square_sheet", "revolve_profile_open", "open_edges", "save_cdf", "load_cdf",
-           "build_sdf", "winding_number", "icosphere"]
+           "build_sdf", "winding_number", "icosphere", "save_sdf", "load_sdf", "build_sdf_cached",
+           "build_surface_cdf_cached"]

@@ -46,3 +46,46 @@ def test_bad_faces_rejected():
     v, f = icosphere(0.1, 0)
     with pytest.raises(ValueError, match="index"):
         build_sdf(v, f + 100, res=12)
+
+
+def test_cached_builders_hit_miss_and_keys(tmp_path, monkeypatch):
+    """The cached builders (mesh_sdf.py:496-563): a second call with the same geometry and
+    grid loads the npz instead of building (bitwise-equal fields); another resolution,
+    builder or sign probe is another file; a bad probe raises ValueError; no cache_dir
+    builds without touching the disk."""
+    import mpm_amd.geometry as geo
+
+    v, f = geo.icosphere(0.2, 1)
+    calls = {"sdf": 0, "cdf": 0}
+    real_sdf, real_cdf = geo.build_sdf, geo.build_surface_cdf
+
+    def count_sdf(*a, **k):
+        calls["sdf"] += 1
+        return real_sdf(*a, **k)
+
+    def count_cdf(*a, **k):
+        calls["cdf"] += 1
+        return real_cdf(*a, **k)
+
+    monkeypatch.setattr(geo, "build_sdf", count_sdf)
+    monkeypatch.setattr(geo, "build_surface_cdf", count_cdf)
+    a = geo.build_sdf_cached(v, f, res=16, cache_dir=tmp_path)
+    b = geo.build_sdf_cached(v, f, res=16, cache_dir=tmp_path)
+    assert calls["sdf"] == 1
+    for fld in ("values", "grads", "origin"):
+        assert np.array_equal(getattr(a, fld), getattr(b, fld))
+    assert a.cell == b.cell and a.sdf_max == b.sdf_max
+    geo.build_sdf_cached(v, f, res=20, cache_dir=tmp_path)
+    geo.build_sdf_cached(v, f, res=16, cache_dir=tmp_path, interior_probe=(0.0, 0.0, 0.01))
+    assert calls["sdf"] == 3
+    assert len(list(tmp_path.glob("sdf_*.npz"))) == 3
+    with pytest.raises(ValueError, match="interior_probe"):
+        geo.build_sdf_cached(v, f, res=16, cache_dir=tmp_path, interior_probe=(0.0, np.nan, 0.0))
+    sv, sf = geo.square_sheet(0.3)
+    c = geo.build_surface_cdf_cached(sv, sf, res=16, cache_dir=tmp_path / "cdf")
+    d = geo.build_surface_cdf_cached(sv, sf, res=16, cache_dir=tmp_path / "cdf")
+    assert calls["cdf"] == 1
+    assert np.array_equal(c.values, d.values) and np.array_equal(c.valid, d.valid)
+    assert geo._field_key(1, sv, sf, [16, 3.0, 4.0], "cpu") != geo._field_key(1, sv, sf, [16, 3.0, 4.0], "hip")
+    geo.build_surface_cdf_cached(sv, sf, res=16)
+    assert calls["cdf"] == 2 and len(list((tmp_path / "cdf").glob("cdf_*.npz"))) == 1
