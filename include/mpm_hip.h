@@ -36,7 +36,7 @@ extern "C" {
 #endif
 
 #define MPM_MAX_MATERIALS 16
-#define MPM_ABI_VERSION 4
+#define MPM_ABI_VERSION 5
 
 typedef enum mpm_status {
     MPM_OK = 0,
@@ -89,6 +89,10 @@ typedef struct mpm_particles {
     int32_t *scene; /* scene id per particle for batched scenes, or NULL (all scene 0) */
     int32_t *cdf_tag; /* CPIC particle tag (particle_cdf_tag, warp_utils.py:87-157), required
                          when the grid carries CDF colliders, else NULL */
+    int32_t *rigid_id; /* rigid body per particle (particle_rigid_id; meaningful for material
+                          8), or NULL when the scene has no rigid bodies */
+    float *x_ref;      /* 3 planes: body-frame reference position of rigid particles
+                          (particle_x_ref), or NULL */
 } mpm_particles_t;
 
 /* ---- CPIC thin-boundary (CDF) colliders ---------------------------------------------
@@ -206,12 +210,51 @@ typedef struct mpm_tick_ctl {
     int32_t pad;
 } mpm_tick_ctl_t;
 
+/* ---- rigid bodies (material 8) -------------------------------------------------------
+ * Per-body state as finalize_rigid_bodies leaves it (mpm_solver_warp.py:834-873):
+ * centre of mass, velocity, angular velocity, orientation (row-major), mass and the
+ * inverse body-frame inertia. A restitution plane is a surface collider registered with
+ * restitution > 0 (mpm_solver_warp.py:1915-1925): point, unit normal, restitution,
+ * Coulomb friction, active time window [start, end). */
+typedef struct mpm_rigid_body {
+    float x_cm[3], v_cm[3], omega[3];
+    float R[9];
+    float mass;
+    float inv_inertia[9];
+    float pad_[4];
+} mpm_rigid_body_t;
+
+typedef struct mpm_rigid_plane {
+    float point[3], normal[3];
+    float restitution, friction, start_time, end_time;
+} mpm_rigid_plane_t;
+
+/* Device bytes of the workspace mpm_rigid_step needs for n_bodies and n_planes. */
+int64_t mpm_rigid_workspace_bytes(int32_t n_bodies, int32_t n_planes);
+/* The rigid step that follows the G2P pass of every split substep (_p2g2p_tail,
+ * mpm_solver_warp.py:1338-1367): gather the grid velocity (node_out) at every rigid
+ * particle into per-body linear and angular momentum (rigid_g2p_accumulate,
+ * mpm_utils.py:1369-1418), integrate each body (rigid_body_integrate, :1421-1466: v = P/M,
+ * omega = R I^-1 R^T L, x += v dt, R += skew(omega) R dt re-orthonormalised by its polar
+ * factor), apply the restitution planes in order (_apply_rigid_restitution,
+ * mpm_solver_warp.py:874-962, there on the host: push the body out by the deepest
+ * penetration, then the normal impulse -(1 + e) v_n / (1/M + ((I^-1 (r x n)) x r) . n) at
+ * the deepest particle, the lowest index on ties, and the Coulomb-capped friction impulse),
+ * then set x, v, C = skew(omega) and F = I of every rigid particle (rigid_particle_update,
+ * :1469-1493). `time` is a device float (the substep's start time, read when the kernels
+ * run, so a captured graph replays with the live value); bodies and planes are device
+ * arrays; single scene. Body momentum sums are float atomics (as in the reference). */
+int32_t mpm_rigid_step(const mpm_particles_t *P, const mpm_grid_t *G, mpm_rigid_body_t *bodies, int32_t n_bodies,
+                       const mpm_rigid_plane_t *planes, int32_t n_planes, const float *time, float dt,
+                       void *workspace, void *stream);
+
 /* ---- introspection (no GPU work) ------------------------------------------------- */
 int32_t mpm_abi_version(void);
 const char *mpm_status_string(int32_t status);
 /* sizeof of the ABI structs, so bindings can verify their mirrors */
 int32_t mpm_struct_size(int32_t which); /* 0 particles, 1 grid, 2 model, 3 collider, 4 tick_ctl,
-                                           5 cdf lanes, 6 cdf collider, 7 adjoint */
+                                           5 cdf lanes, 6 cdf collider, 7 adjoint,
+                                           8 rigid body, 9 rigid plane */
 
 /* ---- particle pass --------------------------------------------------------------- */
 /* One pass over all particles with selection == 0 doing the stages in `mode`:

@@ -621,6 +621,8 @@ int32_t mpm_struct_size(int32_t which) {
         case 5: return (int32_t)sizeof(mpm_cdf_lanes_t);
         case 6: return (int32_t)sizeof(mpm_cdf_collider_t);
         case 7: return (int32_t)sizeof(mpm_adjoint_t);
+        case 8: return (int32_t)sizeof(mpm_rigid_body_t);
+        case 9: return (int32_t)sizeof(mpm_rigid_plane_t);
         default: return -1;
     }
 }
@@ -772,6 +774,8 @@ int32_t mpm_permute(const mpm_particles_t* P, const int32_t* perm, float* scratc
     r |= onei(P->material); r |= onei(P->selection);
     if (P->scene) r |= onei(P->scene);
     if (P->cdf_tag) r |= onei(P->cdf_tag);
+    if (P->rigid_id) r |= onei(P->rigid_id);
+    if (P->x_ref) r |= one(P->x_ref, 3);
     if (r != MPM_OK) return MPM_ERR_HIP;
     return launch_status();
 }

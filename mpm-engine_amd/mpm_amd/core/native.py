@@ -35,7 +35,16 @@ class Particles(C.Structure):
                 ("x", _FP), ("v", _FP), ("F", _FP), ("F_trial", _FP), ("C", _FP),
                 ("stress", _FP), ("L", _FP), ("vol", _FP), ("mass", _FP), ("mu", _FP),
                 ("lam", _FP), ("yield_stress", _FP), ("Jp", _FP), ("material", _FP),
-                ("selection", _FP), ("cov", _FP), ("scene", _FP), ("cdf_tag", _FP)]
+                ("selection", _FP), ("cov", _FP), ("scene", _FP), ("cdf_tag", _FP),
+                ("rigid_id", _FP), ("x_ref", _FP)]
+
+
+# mpm_rigid_body_t / mpm_rigid_plane_t as numpy record types (device arrays of them)
+RIGID_BODY_DTYPE = np.dtype([("x_cm", "<f4", 3), ("v_cm", "<f4", 3), ("omega", "<f4", 3),
+                             ("R", "<f4", 9), ("mass", "<f4"), ("inv_inertia", "<f4", 9),
+                             ("pad_", "<f4", 4)])
+RIGID_PLANE_DTYPE = np.dtype([("point", "<f4", 3), ("normal", "<f4", 3), ("restitution", "<f4"),
+                              ("friction", "<f4"), ("start_time", "<f4"), ("end_time", "<f4")])
 
 
 class Adjoint(C.Structure):
@@ -135,6 +144,8 @@ SIGNATURES = {
     "mpm_cdf_set_lanes": (_I32, [_P, _P, _P]),
     "mpm_box_contact_workspace_floats": (_I32, []),
     "mpm_box_contact_wrench": (_I32, [_P, _P, _F, _I32, _P, _P, _P]),
+    "mpm_rigid_workspace_bytes": (_I64, [_I32, _I32]),
+    "mpm_rigid_step": (_I32, [_P, _P, _P, _I32, _P, _I32, _P, _F, _P, _P]),
     "mpm_mesh_field": (_I32, [_P, _I32, _P, _I32, _P, _F, _I32, _I32, _P, _P, _P]),
     "mpm_segment_distance": (_I32, [_P, _I32, _P, _F, _I32, _P, _P]),
     "mpm_adjoint_g2p": (_I32, [_P, _P, _P, _P, _F, _P]),
@@ -165,7 +176,8 @@ def load():
         fn.argtypes = args
     sizes = {0: C.sizeof(Particles), 1: C.sizeof(Grid), 2: MODEL_DTYPE.itemsize,
              3: COLLIDER_DTYPE.itemsize, 4: TICK_CTL_DTYPE.itemsize,
-             5: CDF_LANES_DTYPE.itemsize, 6: C.sizeof(CdfCollider), 7: C.sizeof(Adjoint)}
+             5: CDF_LANES_DTYPE.itemsize, 6: C.sizeof(CdfCollider), 7: C.sizeof(Adjoint),
+             8: RIGID_BODY_DTYPE.itemsize, 9: RIGID_PLANE_DTYPE.itemsize}
     for which, size in sizes.items():
         got = lib.mpm_struct_size(which)
         if got != size:

@@ -180,6 +180,17 @@ class HipSimulator:
         self._ticks = 0
         self.cdf = None      # CdfColliders, created by the first CPIC collider
         self.cdf_tag = None  # per-particle CPIC tag (allocated with it)
+        # rigid bodies (material 8): per-particle body id and body-frame reference
+        # position, per-body state, restitution planes (mpm_solver_warp.py:366-371, 486-504)
+        self.rigid_id = None
+        self.x_ref = None
+        self.n_rigid_bodies = 0
+        self.rigid_bodies = None    # device bytes of mpm_rigid_body_t[n_rigid_bodies]
+        self.rigid_planes = []      # host rows of mpm_rigid_plane_t
+        self._rigid_planes_dev = None
+        self._rigid_planes_key = None
+        self.rigid_ws = None
+        self.rigid_times = None     # device float per substep of the current tick
         # deterministic scatter (fixed-point integer accumulation), off by default
         self.deterministic = False
         self.node_fix = None
@@ -197,6 +208,8 @@ class HipSimulator:
         P.cov = native.ptr(self.cov)
         P.scene = native.ptr(self.scene)
         P.cdf_tag = native.ptr(self.cdf_tag)
+        P.rigid_id = native.ptr(self.rigid_id)
+        P.x_ref = native.ptr(self.x_ref)
         G = native.Grid()
         G.node_in, G.node_out = native.ptr(self.node_in), native.ptr(self.node_out)
         G.n[:] = [self.n_grid] * 3
@@ -315,6 +328,104 @@ class HipSimulator:
             E, nu = float(md["E"][m]), float(md["nu"][m])
             self.mu[a:b] = E / (2.0 * (1.0 + nu))
             self.lam[a:b] = E * nu / ((1.0 + nu) * (1.0 - 2.0 * nu))
+        if "obj_id" in kw:  # rigid body id (mpm_solver_warp.py:1755-1758)
+            if self.rigid_id is None:
+                self.rigid_id = self.torch.zeros(self.n_particles, dtype=self.torch.int32, device=self.device)
+                self._structs()
+            self.rigid_id[a:b] = int(kw["obj_id"])
+
+    # --------------------------------------------------------------- rigid bodies
+    def finalize_rigid_bodies(self):
+        """finalize_rigid_bodies (mpm_solver_warp.py:834-873): per body (material 8,
+        grouped by obj_id) mass, centre of mass (float32), inverse body-frame inertia
+        (summed in float64) and each particle's body-frame reference position;
+        orientation I, at rest."""
+        torch = self.torch
+        mat = self.material.cpu().numpy()
+        rigid = mat == 8
+        if not np.any(rigid):
+            return
+        if self.rigid_id is None:
+            self.rigid_id = torch.zeros(self.n_particles, dtype=torch.int32, device=self.device)
+        rid = self.rigid_id.cpu().numpy()
+        x = np.ascontiguousarray(self.x.t().cpu().numpy())
+        m = self.mass.cpu().numpy()
+        nb = int(rid[rigid].max()) + 1
+        body = np.zeros(nb, dtype=native.RIGID_BODY_DTYPE)
+        x_ref = np.zeros_like(x)
+        for b in range(nb):
+            idx = np.where(rigid & (rid == b))[0]
+            if len(idx) == 0:
+                raise ValueError(f"rigid body {b} has no particles (obj_id values must be 0..{nb - 1})")
+            m_b = m[idx]
+            mass = f32(m_b.sum())
+            x_cm = ((x[idx] * m_b[:, None]).sum(axis=0) / mass).astype(f32)
+            r = x[idx] - x_cm
+            x_ref[idx] = r
+            # m (r.r I - r r^T): float32 dot and outer, float64 eye and sum, as the
+            # reference's loop evaluates it
+            rr = np.einsum("ij,ij->i", r, r).astype(np.float64)
+            outer = (r[:, :, None] * r[:, None, :]).astype(np.float64)
+            inertia = (m_b.astype(np.float64)[:, None, None] * (rr[:, None, None] * np.eye(3) - outer)).sum(axis=0)
+            body[b]["x_cm"] = x_cm
+            body[b]["mass"] = mass
+            body[b]["R"] = np.eye(3, dtype=f32).reshape(-1)
+            body[b]["inv_inertia"] = np.linalg.inv(inertia).astype(f32).reshape(-1)
+        self.rigid_bodies = torch.from_numpy(body.view(np.uint8).copy()).to(self.device)
+        self.x_ref = torch.from_numpy(np.ascontiguousarray(x_ref.T)).to(self.device)
+        self.n_rigid_bodies = nb
+        self._rigid_ws_ensure()
+        self._structs()
+        self._graph = None
+
+    def _rigid_ws_ensure(self):
+        nbytes = int(self.lib.mpm_rigid_workspace_bytes(self.n_rigid_bodies, len(self.rigid_planes)))
+        if self.rigid_ws is None or self.rigid_ws.numel() < max(nbytes, 8):
+            self.rigid_ws = self.torch.zeros(max(nbytes, 8), dtype=self.torch.uint8, device=self.device)
+            self._graph = None
+
+    def add_rigid_plane(self, point, normal, restitution, friction, start_time, end_time):
+        """A restitution surface (mpm_solver_warp.py:1915-1925); `normal` already unit."""
+        row = np.zeros(1, dtype=native.RIGID_PLANE_DTYPE)
+        row["point"], row["normal"] = point, normal
+        row["restitution"], row["friction"] = restitution, friction
+        row["start_time"], row["end_time"] = start_time, min(end_time, 3.0e38)
+        self.rigid_planes.append(row)
+        if self.n_rigid_bodies:
+            self._rigid_ws_ensure()
+
+    def _rigid_planes_device(self):
+        if not self.rigid_planes:
+            return None
+        key = len(self.rigid_planes)
+        if self._rigid_planes_key != key:
+            rows = np.concatenate(self.rigid_planes)
+            self._rigid_planes_dev = self.torch.from_numpy(rows.view(np.uint8).copy()).to(self.device)
+            self._rigid_planes_key = key
+            self._graph = None
+        return self._rigid_planes_dev
+
+    def rigid_state(self, body: int = 0) -> dict:
+        if not 0 <= body < self.n_rigid_bodies:
+            raise ValueError(f"no rigid body {body} (finalize_rigid_bodies found {self.n_rigid_bodies})")
+        b = self.rigid_bodies.cpu().numpy().view(native.RIGID_BODY_DTYPE)[body]
+        return {"com": b["x_cm"].copy(), "v": b["v_cm"].copy(), "omega": b["omega"].copy(),
+                "R": b["R"].reshape(3, 3).copy()}
+
+    def set_rigid_body_velocity(self, body_id, v_cm=(0.0, 0.0, 0.0), omega=(0.0, 0.0, 0.0)):
+        """set_rigid_body_velocity (mpm_solver_warp.py:874-880)."""
+        host = self.rigid_bodies.cpu().numpy().copy()
+        rows = host.view(native.RIGID_BODY_DTYPE)
+        rows[body_id]["v_cm"] = v_cm
+        rows[body_id]["omega"] = omega
+        self.rigid_bodies.copy_(self.torch.from_numpy(host))
+
+    def _rigid(self, k, dt, s):
+        planes = self._rigid_planes_device()
+        native.check(self.lib.mpm_rigid_step(
+            self.pP, self.pG, native.ptr(self.rigid_bodies), self.n_rigid_bodies, native.ptr(planes),
+            len(self.rigid_planes), native.ptr(self.rigid_times) + 4 * k, dt, native.ptr(self.rigid_ws), s),
+            "rigid_step")
 
     def set_gravity(self, g):
         self.model[0]["gravity"] = np.asarray(g, dtype=np.float64).astype(f32)
@@ -487,6 +598,17 @@ class HipSimulator:
         s = self.stream()
         if self.deterministic:
             self._fix_scales(dt)
+        if self.n_rigid_bodies:
+            # rigid scenes run the split pipeline (solver.py:441-444); the substep start
+            # times feed the restitution planes' activity windows
+            fused = False
+            self._rigid_planes_device()
+            self._rigid_ws_ensure()
+            if self.rigid_times is None or self.rigid_times.numel() < S:
+                self.rigid_times = torch.zeros(max(S, 1), dtype=torch.float32, device=self.device)
+                self._graph = None
+            times = ColliderSet.times(self.time, dt, S)[0].astype(np.float32)
+            self.rigid_times[:S].copy_(torch.from_numpy(times))
         tab, t_end, K, n_slots = self._schedule(dt, S)
         self.n_slots = n_slots
         if 6 * n_slots > self.wrench.numel():  # grow, keeping the accumulated values
@@ -547,7 +669,10 @@ class HipSimulator:
                    self.update_cov_with_F, n_slots, native.ptr(self.grid_ws), native.ptr(self.wrench),
                    frozenset(self._mat_ids) <= native.VISCOUS_IDS, self.launch_bits,
                    None if cdf is None else (cdf.n, int(cdf.mask_off), native.ptr(self.cdf_tag)),
-                   self._fix_exp if self.deterministic else None)
+                   self._fix_exp if self.deterministic else None,
+                   (self.n_rigid_bodies, native.ptr(self.rigid_bodies), native.ptr(self._rigid_planes_dev),
+                    len(self.rigid_planes), native.ptr(self.rigid_times), native.ptr(self.rigid_ws),
+                    native.ptr(self.x_ref)) if self.n_rigid_bodies else None)
             if self._graph is None or self._graph_key != key:
                 self._capture(dt, S, K, fused, lists)
                 self._graph_key = key
@@ -606,6 +731,8 @@ class HipSimulator:
             cdf.integrate(t, dt, lag_prev=fused)
             if not fused:
                 self._pass(G2P, dt, s)
+                if self.n_rigid_bodies:
+                    self._rigid(k, dt, s)
         if fused:
             cdf.reconcile_prev()
             self._pass(G2P, dt, s)
@@ -629,6 +756,8 @@ class HipSimulator:
                 self._pass(ST | P2G, dt, s)
                 self._grid(dt, k, K, lists, s)
                 self._pass(G2P, dt, s)
+                if self.n_rigid_bodies:
+                    self._rigid(k, dt, s)
 
     def _enqueue_tick_profiled(self, dt, S, K, fused, lists):
         torch = self.torch
@@ -652,6 +781,8 @@ class HipSimulator:
             timed("grid_update+BC", lambda k=k: self._grid(dt, k, K, lists, s))
             if not fused:
                 timed("g2p", lambda: self._pass(G2P, dt, s))
+                if self.n_rigid_bodies:
+                    timed("rigid_body", lambda k=k: self._rigid(k, dt, s))
         if fused:
             timed("g2p", lambda: self._pass(G2P, dt, s))
         torch.cuda.synchronize(self.device)
@@ -766,8 +897,22 @@ class Solver:
         return self
 
     def finalize_rigid_bodies(self) -> "Solver":
-        raise NotImplementedError("rigid bodies run the reference's split pipeline only; "
-                                  "they are not part of the HIP substep")
+        """Compute each rigid body's centre of mass, mass and inertia from the particles
+        assigned material "rigid" (grouped by obj_id in set_material_range); solver.py:
+        193-198. Rigid scenes run the split pipeline; the body momentum sums are float
+        atomics, so deterministic=None switches to the float scatter and
+        deterministic=True refuses them."""
+        if self.deterministic:
+            raise NotImplementedError("the deterministic scatter does not cover rigid bodies")
+        sim = self._sim
+        if sim.deterministic:
+            sim.disable_deterministic()
+        sim.finalize_rigid_bodies()
+        return self
+
+    def rigid_state(self, body: int = 0) -> dict:
+        """State of one rigid body: com (3,), v (3,), omega (3,), R (3, 3) (solver.py:200-210)."""
+        return self._sim.rigid_state(body)
 
     # ------------------------------------------------------------------ colliders
     def _cs(self) -> ColliderSet:
@@ -775,15 +920,17 @@ class Solver:
 
     def add_plane(self, point, normal, surface: str = "sticky", friction: float = 0.0,
                   restitution: float = 0.0) -> "Solver":
-        """Plane collider (mpm_solver_warp.py:1880-2018)."""
-        if restitution != 0.0:
-            raise NotImplementedError("plane restitution acts on rigid bodies, which the HIP "
-                                      "substep does not carry")
+        """Plane collider (mpm_solver_warp.py:1880-2018). A restitution in (0, 1] also
+        registers the plane as a rigid-body contact surface (:1915-1925)."""
+        if restitution != 0.0 and not (0.0 < restitution <= 1.0):
+            raise ValueError("restitution must be between 0 (exclusive) and 1 (inclusive).")
         if surface == "sticky" and friction != 0:
             raise ValueError("friction must be 0 on sticky surfaces.")
         point = [float(v) for v in point]
         scale = 1.0 / float(f32(math.sqrt(float(sum(float(x) ** 2 for x in normal)))))
         normal = [scale * float(x) for x in normal]
+        if restitution != 0.0:
+            self._sim.add_rigid_plane(point, normal, restitution, friction, 0.0, 999.0)
         c = Collider(native.COLLIDER_PLANE, "plane", 0.0, 999.0)
         c.point = [float(f32(v)) for v in point]
         c.normal = [float(f32(v)) for v in normal]

@@ -1261,6 +1261,83 @@ EXPORT void o_blocks_grid_op(OState *st, OModel *md, int op, float dt, float sca
     }
 }
 
+/* ------------------------------------------------------------------- rigid bodies */
+/* rigid_g2p_accumulate, mpm_utils.py:1369-1418: per rigid particle (material 8, selected)
+ * the B-spline-weighted grid_v_out, summed as m v and (x - x_cm) x m v into its body;
+ * sequential in particle order (the checker's sum order) */
+EXPORT void o_rigid_gather(OState *st, OModel *md, const float *x_cm, int nb, float *lin, float *ang) {
+    for (int b = 0; b < 3 * nb; ++b) lin[b] = ang[b] = 0.0f;
+    for (int p = 0; p < st->n; ++p) {
+        if (st->selection[p] != 0 || st->material[p] != 8) continue;
+        int bid = st->rigid_id[p];
+        if (bid < 0 || bid >= nb) continue;
+        v3 xp = vload(st->x + 3 * p);
+        Stencil s = stencil(md, xp);
+        v3 vi = V3(0, 0, 0);
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j)
+                for (int k = 0; k < 3; ++k) {
+                    float w = s.w[0][i] * s.w[1][j] * s.w[2][k];
+                    vi = vadd(vi, vscale(vload(st->grid_v_out + 3 * gidx(md, wrap_x(md, s.base[0] + i),
+                                                                          s.base[1] + j, s.base[2] + k)), w));
+                }
+        v3 mv = vscale(vi, st->mass[p]);
+        v3 r = vsub(xp, vload(x_cm + 3 * bid));
+        v3 am = vcross(r, mv);
+        for (int c = 0; c < 3; ++c) {
+            lin[3 * bid + c] += mv.c[c];
+            ang[3 * bid + c] += am.c[c];
+        }
+    }
+}
+
+/* rigid_body_integrate, mpm_utils.py:1421-1466: v = P / M, omega = R I^-1 R^T L,
+ * x += v dt, R <- polar factor of R + skew(omega) R dt */
+EXPORT void o_rigid_integrate(float *x_cm, float *v_cm, float *omega, float *R, const float *mass,
+                              const float *inv_I, const float *lin, const float *ang, int nb, float dt) {
+    for (int b = 0; b < nb; ++b) {
+        m33 Rb = mload(R + 9 * b);
+        float M = mass[b];
+        v3 v = vdiv(vload(lin + 3 * b), M);
+        m33 Iw = mmul(mmul(Rb, mload(inv_I + 9 * b)), mT(Rb));
+        v3 om = mvmul(Iw, vload(ang + 3 * b));
+        v3 x = vadd(vload(x_cm + 3 * b), vscale(v, dt));
+        m33 sk = mzero();
+        sk.a[1] = -om.c[2]; sk.a[2] = om.c[1];
+        sk.a[3] = om.c[2];  sk.a[5] = -om.c[0];
+        sk.a[6] = -om.c[1]; sk.a[7] = om.c[0];
+        m33 Rn = madd(Rb, mscale(mmul(sk, Rb), dt));
+        m33 U, V; v3 sg;
+        svd3(Rn, &U, &sg, &V);
+        vstore(v_cm + 3 * b, v);
+        vstore(omega + 3 * b, om);
+        vstore(x_cm + 3 * b, x);
+        mstore(R + 9 * b, mmul(U, mT(V)));
+    }
+}
+
+/* rigid_particle_update, mpm_utils.py:1469-1493: x = x_cm + R x_ref, v = v_cm + omega x r,
+ * C = skew(omega), F = I */
+EXPORT void o_rigid_update(OState *st, const float *x_cm, const float *v_cm, const float *omega,
+                           const float *R, int nb) {
+    for (int p = 0; p < st->n; ++p) {
+        if (st->selection[p] != 0 || st->material[p] != 8) continue;
+        int bid = st->rigid_id[p];
+        if (bid < 0 || bid >= nb) continue;
+        v3 xc = vload(x_cm + 3 * bid), om = vload(omega + 3 * bid);
+        v3 xp = vadd(xc, mvmul(mload(R + 9 * bid), vload(st->x_ref + 3 * p)));
+        v3 r = vsub(xp, xc);
+        vstore(st->v + 3 * p, vadd(vload(v_cm + 3 * bid), vcross(om, r)));
+        vstore(st->x + 3 * p, xp);
+        m33 sk = mzero();
+        sk.a[1] = -om.c[2]; sk.a[2] = om.c[1];
+        sk.a[3] = om.c[2];  sk.a[5] = -om.c[0];
+        sk.a[6] = -om.c[1]; sk.a[7] = om.c[0];
+        mstore(st->C + 9 * p, sk);
+        mstore(st->F + 9 * p, meye());
+    }
+}
+
 /* ------------------------------------------------------------------------- exports */
 /* compute_R_from_F, mpm_utils.py:1261-1285 (stores R^T, as the reference does) */
 EXPORT void o_compute_R(OState *st) {

@@ -76,6 +76,8 @@ typedef struct OState {
     int32_t *selection, *material;
     float *grid_m, *grid_v_in, *grid_v_out;
     OCdf *cdf; /* NULL: no CDF colliders (n_cdf == 0) */
+    int32_t *rigid_id; /* particle_rigid_id, or NULL without rigid bodies */
+    float *x_ref;      /* particle_x_ref (n x 3), or NULL */
 } OState;
 
 /* collider parameter blocks (warp_utils.py collider structs; mpm_solver_warp.py:1880-2777) */

@@ -81,7 +81,7 @@ class OState(C.Structure):
         ("L", FP), ("R", FP), ("vol", FP), ("mass", FP), ("density", FP), ("Jp", FP),
         ("cov", FP), ("init_cov", FP), ("selection", IP), ("material", IP),
         ("grid_m", FP), ("grid_v_in", FP), ("grid_v_out", FP),
-        ("cdf", C.POINTER(OCdf)),
+        ("cdf", C.POINTER(OCdf)), ("rigid_id", IP), ("x_ref", FP),
     ]
 
 
@@ -169,6 +169,9 @@ def lib():
         L.o_cdf_stamp.argtypes = [C.c_float, S, M, C.POINTER(OCdfCollider), IP, IP]
         L.o_compute_R.argtypes = [S]
         L.o_compute_cov.argtypes = [S]
+        L.o_rigid_gather.argtypes = [S, M, FP, C.c_int, FP, FP]
+        L.o_rigid_integrate.argtypes = [FP] * 8 + [C.c_int, C.c_float]
+        L.o_rigid_update.argtypes = [S, FP, FP, FP, FP, C.c_int]
         _LIB = L
     return _LIB
 
@@ -257,6 +260,11 @@ class OracleSim:
             self.init_cov[:] = np.asarray(cov, f32).reshape(-1)
         self.selection = np.zeros(n, np.int32)
         self.material = np.zeros(n, np.int32)
+        # rigid bodies (mpm_solver_warp.py:366-371, 486-504)
+        self.rigid_id = np.zeros(n, np.int32)
+        self.x_ref = np.zeros((n, 3), f32)
+        self.n_rigid_bodies = 0
+        self.rigid_surface_colliders = []
         g = self.n_grid
         self.grid_m = np.zeros((g, g, g), f32)
         self.grid_v_in = np.zeros((g, g, g, 3), f32)
@@ -326,6 +334,8 @@ class OracleSim:
             setattr(s, nm, fptr(getattr(self, nm)))
         s.selection = iptr(self.selection)
         s.material = iptr(self.material)
+        s.rigid_id = iptr(self.rigid_id)
+        s.x_ref = fptr(self.x_ref)
         if self.cdf is not None and self.cdf.n_cdf > 0:
             self.cdf.ptag = iptr(self.particle_cdf_tag)
             s.cdf = C.pointer(self.cdf)
@@ -450,6 +460,104 @@ class OracleSim:
             E, nu = float(self.E[m]), float(self.nu[m])
             self.mu[a:b] = E / (2.0 * (1.0 + nu))
             self.lam[a:b] = E * nu / ((1.0 + nu) * (1.0 - 2.0 * nu))
+        if "obj_id" in kw:  # rigid body id (mpm_solver_warp.py:1755-1758)
+            self.rigid_id[a:b] = int(kw["obj_id"])
+
+    # ------------------------------------------------------------------ rigid bodies
+    def finalize_rigid_bodies(self):
+        """mpm_solver_warp.py:834-873: per body (material 8, grouped by obj_id) the mass,
+        the centre of mass, the inverse body-frame inertia (float64 sum, float32 store)
+        and the body-frame reference position of each particle; orientation I."""
+        rigid = self.material == 8
+        if not np.any(rigid):
+            return
+        nb = int(self.rigid_id[rigid].max()) + 1
+        self.n_rigid_bodies = nb
+        self.rigid_x_cm = np.zeros((nb, 3), f32)
+        self.rigid_v_cm = np.zeros((nb, 3), f32)
+        self.rigid_omega = np.zeros((nb, 3), f32)
+        self.rigid_orientation = np.tile(np.eye(3, dtype=f32), (nb, 1, 1))
+        self.rigid_mass = np.zeros(nb, f32)
+        self.rigid_inv_inertia_body = np.zeros((nb, 3, 3), f32)
+        for b in range(nb):
+            idx = np.where(rigid & (self.rigid_id == b))[0]
+            m_b = self.mass[idx]
+            self.rigid_mass[b] = float(m_b.sum())
+            self.rigid_x_cm[b] = (self.x[idx] * m_b[:, None]).sum(axis=0) / self.rigid_mass[b]
+        for b in range(nb):
+            idx = np.where(rigid & (self.rigid_id == b))[0]
+            r = self.x[idx] - self.rigid_x_cm[b]
+            self.x_ref[idx] = r
+            inertia = np.zeros((3, 3), np.float64)
+            for ri, mi in zip(r, self.mass[idx]):
+                inertia += mi * (np.dot(ri, ri) * np.eye(3) - np.outer(ri, ri))
+            self.rigid_inv_inertia_body[b] = np.linalg.inv(inertia).astype(f32)
+
+    def _apply_rigid_restitution(self):
+        """mpm_solver_warp.py:874-962 (host numpy, float64): per body and surface, push the
+        body out by the deepest penetration, then the normal impulse -(1 + e) v_n / denom at
+        the deepest particle and a Coulomb-capped friction impulse."""
+        if not self.rigid_surface_colliders or self.n_rigid_bodies == 0:
+            return
+        for b in range(self.n_rigid_bodies):
+            idx = np.where((self.material == 8) & (self.rigid_id == b))[0]
+            if len(idx) == 0:
+                continue
+            R = self.rigid_orientation[b]
+            x_cm = self.rigid_x_cm[b]  # a view: pushes below show through it
+            M = float(self.rigid_mass[b])
+            I_world_inv = R @ self.rigid_inv_inertia_body[b] @ R.T
+            x_world = x_cm + (R @ self.x_ref[idx].T).T
+            for surf in self.rigid_surface_colliders:
+                if self.time < surf["start_time"] or self.time >= surf["end_time"]:
+                    continue
+                pt = np.array(surf["point"], dtype=np.float64)
+                n = np.array(surf["normal"], dtype=np.float64)
+                e, mu = float(surf["restitution"]), float(surf["friction"])
+                dists = (x_world - pt) @ n
+                min_i = int(np.argmin(dists))
+                if dists[min_i] >= 0.0:
+                    continue
+                push = (-float(dists[min_i])) * n
+                self.rigid_x_cm[b] = self.rigid_x_cm[b] + push
+                x_world = x_world + push
+                r = x_world[min_i].astype(np.float64) - x_cm.astype(np.float64)
+                v_contact = self.rigid_v_cm[b].astype(np.float64) + np.cross(
+                    self.rigid_omega[b].astype(np.float64), r)
+                v_n = float(np.dot(v_contact, n))
+                if v_n >= 0.0:
+                    continue
+                r_cross_n = np.cross(r, n)
+                denom = 1.0 / M + np.dot(np.cross(I_world_inv @ r_cross_n, r), n)
+                J_n = -(1.0 + e) * v_n / denom
+                self.rigid_v_cm[b] = self.rigid_v_cm[b].astype(np.float64) + (J_n / M) * n
+                self.rigid_omega[b] = self.rigid_omega[b].astype(np.float64) + I_world_inv @ (J_n * r_cross_n)
+                if mu > 0.0:
+                    v_t_vec = v_contact - v_n * n
+                    v_t_mag = float(np.linalg.norm(v_t_vec))
+                    if v_t_mag > 1e-10:
+                        t_hat = v_t_vec / v_t_mag
+                        r_cross_t = np.cross(r, t_hat)
+                        denom_t = 1.0 / M + np.dot(np.cross(I_world_inv @ r_cross_t, r), t_hat)
+                        J_t = min(v_t_mag / denom_t, mu * J_n)
+                        self.rigid_v_cm[b] = self.rigid_v_cm[b] - (J_t / M) * t_hat
+                        self.rigid_omega[b] = self.rigid_omega[b] - I_world_inv @ (J_t * r_cross_t)
+
+    def _p2g2p_tail(self, dt):
+        """mpm_solver_warp.py:1338-1377: the rigid step, then time += dt."""
+        if self.n_rigid_bodies > 0:
+            L = lib()
+            nb = self.n_rigid_bodies
+            lin = np.zeros((nb, 3), f32)
+            ang = np.zeros((nb, 3), f32)
+            L.o_rigid_gather(self._state(), self._model(), fptr(self.rigid_x_cm), nb, fptr(lin), fptr(ang))
+            L.o_rigid_integrate(fptr(self.rigid_x_cm), fptr(self.rigid_v_cm), fptr(self.rigid_omega),
+                                fptr(self.rigid_orientation), fptr(self.rigid_mass),
+                                fptr(self.rigid_inv_inertia_body), fptr(lin), fptr(ang), nb, dt)
+            self._apply_rigid_restitution()
+            L.o_rigid_update(self._state(), fptr(self.rigid_x_cm), fptr(self.rigid_v_cm),
+                             fptr(self.rigid_omega), fptr(self.rigid_orientation), nb)
+        self.time = self.time + dt
 
     # ------------------------------------------------------------------ grid boxes
     def _grid_box(self, lo_w, hi_w, halo=1):
@@ -471,11 +579,18 @@ class OracleSim:
         self.collider_labels.append(label)
 
     def add_surface_collider(self, point, normal, surface="sticky", friction=0.0,
-                             start_time=0.0, end_time=999.0):
-        """mpm_solver_warp.py:1880-2018."""
+                             start_time=0.0, end_time=999.0, restitution=0.0):
+        """mpm_solver_warp.py:1880-2018 (restitution > 0 also registers a rigid-body
+        contact surface, :1915-1925)."""
         point = [float(v) for v in point]
         scale = 1.0 / float(f32(np.sqrt(float(sum(x ** 2 for x in normal)))))
         normal = [scale * x for x in normal]
+        if restitution != 0.0:
+            if not (0.0 < restitution <= 1.0):
+                raise ValueError("restitution must be between 0 (exclusive) and 1 (inclusive).")
+            self.rigid_surface_colliders.append({
+                "point": list(point), "normal": list(normal), "friction": friction,
+                "restitution": restitution, "start_time": start_time, "end_time": end_time})
         p = OPlane()
         p.start_time, p.end_time = start_time, end_time
         p.point[:] = point
@@ -1043,7 +1158,7 @@ class OracleSim:
                 L.o_grid_damping(st, md, self.grid_v_damping_scale, i3(g[0]), i3(g[1]))
         self._apply_grid_bc(dt)
         L.o_g2p(self._state(), self._model(), dt)
-        self.time = self.time + dt
+        self._p2g2p_tail(dt)
 
     def p2g2p_fused_tick(self, dt, substeps):
         """Fused tick, mpm_solver_warp.py:1379-1546 (CPU path: no graphs)."""
@@ -1103,7 +1218,7 @@ class OracleSim:
         perm = np.asarray(perm)
         for nm in ("x", "v", "F", "F_trial", "R", "stress", "C", "L", "vol", "mass",
                    "density", "Jp", "mu", "lam", "yield_stress", "selection", "material",
-                   "particle_cdf_tag"):
+                   "particle_cdf_tag", "rigid_id", "x_ref"):
             a = getattr(self, nm)
             a[...] = a[perm]
         for nm in ("init_cov", "cov"):
@@ -1164,9 +1279,19 @@ class OracleSolver:
                                                {"material": name, **params, **overrides})
         return self
 
-    def add_plane(self, point, normal, surface="sticky", friction=0.0):
-        self._sim.add_surface_collider(tuple(point), tuple(normal), surface, friction=friction)
+    def add_plane(self, point, normal, surface="sticky", friction=0.0, restitution=0.0):
+        self._sim.add_surface_collider(tuple(point), tuple(normal), surface, friction=friction,
+                                       restitution=restitution)
         return self
+
+    def finalize_rigid_bodies(self):
+        self._sim.finalize_rigid_bodies()
+        return self
+
+    def rigid_state(self, body=0):
+        s = self._sim
+        return {"com": s.rigid_x_cm[body].copy(), "v": s.rigid_v_cm[body].copy(),
+                "omega": s.rigid_omega[body].copy(), "R": s.rigid_orientation[body].copy()}
 
     def add_box(self, center, half_size, velocity=(0.0, 0.0, 0.0), start_time=0.0,
                 end_time=1.0e9):
@@ -1272,7 +1397,7 @@ class OracleSolver:
         if self.sparse:
             self._sim.rebuild_active_blocks()
         self._tick += 1
-        if self.fused and not self.sparse:
+        if self.fused and not self.sparse and self._sim.n_rigid_bodies == 0:
             self._sim.p2g2p_fused_tick(dt, substeps)
             self._step += substeps
         else:

@@ -29,7 +29,7 @@ def test_library_exports_every_declared_symbol():
 
 def test_struct_layouts_and_introspection():
     lib = native.load()
-    assert lib.mpm_abi_version() == 4
+    assert lib.mpm_abi_version() == 5
     assert lib.mpm_struct_size(1) == ctypes.sizeof(native.Grid)
     assert lib.mpm_struct_size(2) == native.MODEL_DTYPE.itemsize
     assert lib.mpm_struct_size(3) == native.COLLIDER_DTYPE.itemsize
@@ -37,7 +37,9 @@ def test_struct_layouts_and_introspection():
     assert lib.mpm_struct_size(5) == native.CDF_LANES_DTYPE.itemsize
     assert lib.mpm_struct_size(6) == ctypes.sizeof(native.CdfCollider)
     assert lib.mpm_struct_size(7) == ctypes.sizeof(native.Adjoint)
-    assert lib.mpm_struct_size(9) == -1
+    assert lib.mpm_struct_size(8) == native.RIGID_BODY_DTYPE.itemsize
+    assert lib.mpm_struct_size(9) == native.RIGID_PLANE_DTYPE.itemsize
+    assert lib.mpm_struct_size(10) == -1
     assert lib.mpm_status_string(1) == b"invalid argument"
 
 

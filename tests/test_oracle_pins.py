@@ -541,3 +541,90 @@ def test_tabulated_mu_i_equals_parametric_and_uses_table_head_as_threshold():
     high[0] = 5.0
     c = _run(F, "tabulated_mu_i", eta_table=high, eta_table_smin=smin, eta_table_smax=smax, **common)
     np.testing.assert_array_equal(c.F[0], F)
+
+
+# ------------------------------------------------------------------------ rigid bodies
+def _rigid_cube(n_grid=32, z0=0.5, fused=False, planes=()):
+    from mpm_amd.core.solver import GridConfig
+
+    g = GridConfig(n_grid=n_grid, grid_lim=1.0)
+    h = g.dx / 2
+    ax = np.arange(0.4 + h / 2, 0.6, h)
+    pts = np.stack(np.meshgrid(ax, ax, ax - 0.4 + z0, indexing="ij"), -1).reshape(-1, 3).astype(f32)
+    o = O.OracleSolver(g, fused=fused)
+    o.load_particles(pts, np.full(len(pts), h ** 3, f32))
+    o.set_material("rigid", density=1000.0)
+    o.set_material_range(0, len(pts), "rigid", obj_id=0)
+    for pl in planes:
+        o.add_plane(*pl)
+    o.finalize_rigid_bodies()
+    return o, pts, h
+
+
+def test_rigid_finalize_mass_com_and_lattice_inertia():
+    """finalize_rigid_bodies (mpm_solver_warp.py:834-873): mass = sum m, COM = mass-weighted
+    mean, inertia of an n^3 lattice of spacing h = M h^2 (n^2 - 1) / 6 on the diagonal
+    (2 x the variance of n equally spaced points) and zero off it; x_ref = x - COM."""
+    o, pts, h = _rigid_cube()
+    s = o._sim
+    M = float(s.mass.sum(dtype=np.float64))
+    np.testing.assert_allclose(s.rigid_mass[0], M, rtol=1e-6)
+    # (float32 sums over the body, as the reference's numpy finalize)
+    np.testing.assert_allclose(s.rigid_x_cm[0], pts.astype(np.float64).mean(0), atol=1e-5)
+    n = round(len(pts) ** (1 / 3))
+    inertia = np.linalg.inv(s.rigid_inv_inertia_body[0].astype(np.float64))
+    np.testing.assert_allclose(inertia, M * h * h * (n * n - 1) / 6 * np.eye(3), rtol=1e-4, atol=1e-9)
+    np.testing.assert_allclose(s.x_ref, pts - s.rigid_x_cm[0], atol=1e-7)
+
+
+def test_rigid_free_fall_and_orientation():
+    """A rigid cube in free fall (no colliders): v_cm = g t, COM falls g t (t + dt) / 2
+    (the symplectic update, x += v dt after v), the orientation stays orthonormal and
+    every particle moves with the body."""
+    o, pts, _ = _rigid_cube(z0=0.6)
+    dt, n = 1e-4, 60
+    z0 = float(o._sim.rigid_x_cm[0][2])
+    o.step(dt, n)
+    st = o.rigid_state(0)
+    t = n * dt
+    np.testing.assert_allclose(st["v"], [0.0, 0.0, -9.81 * t], atol=5e-6)
+    np.testing.assert_allclose(st["com"][2], z0 - 9.81 * dt * dt * n * (n + 1) / 2, atol=2e-6)
+    np.testing.assert_allclose(st["R"] @ st["R"].T, np.eye(3), atol=1e-6)
+    dz = o.x()[:, 2] - pts[:, 2]
+    assert np.ptp(dz) < 1e-5 and abs(dz.mean() - (st["com"][2] - z0)) < 2e-6
+
+
+def test_rigid_restitution_reverses_contact_normal_velocity():
+    """The restitution impulse (mpm_solver_warp.py:874-962): at the deepest particle the
+    normal contact velocity after the impulse is -e times the one before (the impulse is
+    -(1 + e) v_n over the effective inverse mass 1/M + ((I^-1 (r x n)) x r) . n), the body
+    is pushed out of the plane, and without friction the impulse is J n at the contact:
+    the COM velocity changes along n only and the angular impulse is r x (M dv_cm), i.e.
+    I_world d(omega) = r x M d(v_cm). Checked on the restitution step alone, from a
+    tilted, spinning body just through the plane."""
+    o, pts, _ = _rigid_cube(z0=0.5, planes=[((0, 0, 0.3), (0, 0, 1), "separate", 0.0, 0.5)])
+    s = o._sim
+    th = 0.3
+    Rz = np.array([[1, 0, 0], [0, np.cos(th), -np.sin(th)], [0, np.sin(th), np.cos(th)]], f32)
+    s.rigid_orientation[0] = Rz
+    s.rigid_x_cm[0] = [0.5, 0.5, 0.3 + 0.08]
+    s.rigid_v_cm[0] = [0.2, -0.1, -1.5]
+    s.rigid_omega[0] = [0.5, 2.0, -1.0]
+    x_world = s.rigid_x_cm[0] + (Rz @ s.x_ref.T).T
+    d = (x_world - np.array([0, 0, 0.3])) @ np.array([0, 0, 1.0])
+    i = int(np.argmin(d))
+    assert d[i] < 0
+    r = (Rz @ s.x_ref[i]).astype(np.float64)
+    v0, w0 = s.rigid_v_cm[0].astype(np.float64), s.rigid_omega[0].astype(np.float64)
+    v_before = v0 + np.cross(w0, r)
+    s._apply_rigid_restitution()
+    v1, w1 = s.rigid_v_cm[0].astype(np.float64), s.rigid_omega[0].astype(np.float64)
+    v_after = v1 + np.cross(w1, r)
+    np.testing.assert_allclose(v_after[2], -0.5 * v_before[2], rtol=1e-5)
+    np.testing.assert_allclose(v1[:2], v0[:2], atol=1e-7)
+    M = float(s.rigid_mass[0])
+    I_world = Rz.astype(np.float64) @ np.linalg.inv(s.rigid_inv_inertia_body[0].astype(np.float64)) @ Rz.T
+    np.testing.assert_allclose(I_world @ (w1 - w0), np.cross(r, M * (v1 - v0)), rtol=1e-4,
+                               atol=1e-6 * np.abs(M * (v1 - v0)).max())
+    x_new = s.rigid_x_cm[0] + (Rz @ s.x_ref.T).T
+    assert ((x_new - [0, 0, 0.3]) @ [0, 0, 1.0]).min() > -1e-6
