@@ -93,6 +93,7 @@ typedef struct mpm_particles {
                           8), or NULL when the scene has no rigid bodies */
     float *x_ref;      /* 3 planes: body-frame reference position of rigid particles
                           (particle_x_ref), or NULL */
+    int32_t *modifier_mask; /* bit k: particle selected by particle modifier k, or NULL */
 } mpm_particles_t;
 
 /* ---- CPIC thin-boundary (CDF) colliders ---------------------------------------------
@@ -215,7 +216,8 @@ typedef struct mpm_tick_ctl {
  * centre of mass, velocity, angular velocity, orientation (row-major), mass and the
  * inverse body-frame inertia. A restitution plane is a surface collider registered with
  * restitution > 0 (mpm_solver_warp.py:1915-1925): point, unit normal, restitution,
- * Coulomb friction, active time window [start, end). */
+ * Coulomb friction; its time window is evaluated by the caller (the reference compares
+ * float64 host times) and passed per substep as a bit mask. At most 32 planes. */
 typedef struct mpm_rigid_body {
     float x_cm[3], v_cm[3], omega[3];
     float R[9];
@@ -226,7 +228,7 @@ typedef struct mpm_rigid_body {
 
 typedef struct mpm_rigid_plane {
     float point[3], normal[3];
-    float restitution, friction, start_time, end_time;
+    float restitution, friction, start_time, end_time; /* window kept for the record */
 } mpm_rigid_plane_t;
 
 /* Device bytes of the workspace mpm_rigid_step needs for n_bodies and n_planes. */
@@ -241,12 +243,50 @@ int64_t mpm_rigid_workspace_bytes(int32_t n_bodies, int32_t n_planes);
  * penetration, then the normal impulse -(1 + e) v_n / (1/M + ((I^-1 (r x n)) x r) . n) at
  * the deepest particle, the lowest index on ties, and the Coulomb-capped friction impulse),
  * then set x, v, C = skew(omega) and F = I of every rigid particle (rigid_particle_update,
- * :1469-1493). `time` is a device float (the substep's start time, read when the kernels
- * run, so a captured graph replays with the live value); bodies and planes are device
- * arrays; single scene. Body momentum sums are float atomics (as in the reference). */
+ * :1469-1493). `active` is a device int32: bit k set = plane k active in this substep
+ * (read when the kernels run, so a captured graph replays with the live value); bodies
+ * and planes are device arrays; single scene. Body momentum sums are float atomics (as
+ * in the reference). */
 int32_t mpm_rigid_step(const mpm_particles_t *P, const mpm_grid_t *G, mpm_rigid_body_t *bodies, int32_t n_bodies,
-                       const mpm_rigid_plane_t *planes, int32_t n_planes, const float *time, float dt,
+                       const mpm_rigid_plane_t *planes, int32_t n_planes, const int32_t *active, float dt,
                        void *workspace, void *stream);
+
+/* ---- particle modifiers ---------------------------------------------------------------
+ * Pre-P2G impulses and particle velocity modifiers (add_impulse_on_particles,
+ * enforce_particle_velocity_translation / _rotation, mpm_solver_warp.py:2385-2539). Each
+ * selects its particles once, when it is added (a box |x - point| < size per axis, or a
+ * cylinder about `vec` through `point`: |axial offset| < half_height and radial distance
+ * < radius; selection kernels mpm_utils.py:1320-1363), into bit `bit` of
+ * P->modifier_mask, and acts in the substeps whose start time t has
+ * start_time <= t < end_time (evaluated by the caller in float64, as the reference's host
+ * loop does, and passed per substep as a bit mask over the modifiers):
+ *   kind 0 impulse:      v += vec / mass * dt          (vec = force)
+ *   kind 1 translation:  v = vec                       (vec = velocity)
+ *   kind 2 rotation:     v = -d sin(theta) s_r axis1 + d cos(theta) s_r axis2 + s_t vec
+ *                        (vec = unit axis, d = radial distance, theta the angle from axis1,
+ *                        signed by axis2; s_r, s_t the rotation / translation scales) */
+#define MPM_MAX_MODIFIERS 32
+typedef struct mpm_particle_modifier {
+    int32_t kind, bit;
+    float start_time, end_time;
+    float point[3], vec[3];
+    float axis1[3], axis2[3];
+    float size[3];
+    float half_height, radius;
+    float rotation_scale, translation_scale;
+    float pad_[3];
+} mpm_particle_modifier_t;
+
+/* Set (selected) or clear bit mod->bit of P->modifier_mask for every particle; `mod` is a
+ * host struct. */
+int32_t mpm_modifier_select(const mpm_particles_t *P, const mpm_particle_modifier_t *mod, void *stream);
+/* Apply n_mods modifiers (device array, in the order added) to the selected particles:
+ * every impulse first, then the velocity modifiers, as the reference's split substep
+ * launches its pre_p2g_operations and then its particle_velocity_modifiers
+ * (mpm_solver_warp.py:1181-1196); `active` is a device int32, bit j set = modifier j
+ * active in this substep. */
+int32_t mpm_modifiers_apply(const mpm_particles_t *P, const mpm_particle_modifier_t *mods, int32_t n_mods,
+                            const int32_t *active, float dt, void *stream);
 
 /* ---- introspection (no GPU work) ------------------------------------------------- */
 int32_t mpm_abi_version(void);
@@ -254,7 +294,8 @@ const char *mpm_status_string(int32_t status);
 /* sizeof of the ABI structs, so bindings can verify their mirrors */
 int32_t mpm_struct_size(int32_t which); /* 0 particles, 1 grid, 2 model, 3 collider, 4 tick_ctl,
                                            5 cdf lanes, 6 cdf collider, 7 adjoint,
-                                           8 rigid body, 9 rigid plane */
+                                           8 rigid body, 9 rigid plane,
+                                           10 particle modifier */
 
 /* ---- particle pass --------------------------------------------------------------- */
 /* One pass over all particles with selection == 0 doing the stages in `mode`:

@@ -19,7 +19,8 @@ CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "lib", "libmpm_hip.so")
 SOURCES = ["mpm_kernels.hip", "mpm_particles.hip", "mpm_sort.hip", "mpm_adjoint.hip", "mpm_cdf.hip",
-           "mpm_coupling.hip", "mpm_geometry.hip", "mpm_rigid.hip"]
+           "mpm_coupling.hip", "mpm_geometry.hip", "mpm_rigid.hip",
+           "mpm_modifiers.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-munsafe-fp-atomics",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

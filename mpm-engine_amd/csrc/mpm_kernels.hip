@@ -623,6 +623,7 @@ int32_t mpm_struct_size(int32_t which) {
         case 7: return (int32_t)sizeof(mpm_adjoint_t);
         case 8: return (int32_t)sizeof(mpm_rigid_body_t);
         case 9: return (int32_t)sizeof(mpm_rigid_plane_t);
+        case 10: return (int32_t)sizeof(mpm_particle_modifier_t);
         default: return -1;
     }
 }
@@ -776,6 +777,7 @@ int32_t mpm_permute(const mpm_particles_t* P, const int32_t* perm, float* scratc
     if (P->cdf_tag) r |= onei(P->cdf_tag);
     if (P->rigid_id) r |= onei(P->rigid_id);
     if (P->x_ref) r |= one(P->x_ref, 3);
+    if (P->modifier_mask) r |= onei(P->modifier_mask);
     if (r != MPM_OK) return MPM_ERR_HIP;
     return launch_status();
 }

@@ -59,7 +59,7 @@ MPM_DEV v3 world_pos(const mpm_rigid_body_t& b, v3 xr) {
     return mk3(b.x_cm[0] + rx.x, b.x_cm[1] + rx.y, b.x_cm[2] + rx.z);
 }
 
-MPM_DEV bool plane_active(const mpm_rigid_plane_t& pl, float t) { return t >= pl.start_time && t < pl.end_time; }
+MPM_DEV bool plane_active(unsigned active, int k) { return (active >> k) & 1u; }
 
 MPM_DEV double plane_dist(const mpm_rigid_plane_t& pl, v3 xw) {
     return ((double)xw.x - (double)pl.point[0]) * (double)pl.normal[0] +
@@ -149,15 +149,15 @@ __global__ void k_rigid_integrate(mpm_rigid_body_t* bodies, int nb, int np, void
 __global__ __launch_bounds__(kThreads) void k_rigid_contact_min(mpm_particles_t P,
                                                                 const mpm_rigid_body_t* __restrict__ bodies, int nb,
                                                                 const mpm_rigid_plane_t* __restrict__ planes, int np,
-                                                                const float* __restrict__ time, void* ws) {
+                                                                const int32_t* __restrict__ active, void* ws) {
     const int p = blockIdx.x * kThreads + threadIdx.x;
     int bid;
     if (!rigid_particle(P, p, nb, bid)) return;
     const RigidWs w = rigid_ws(ws, nb, np);
-    const float t = *time;
+    const unsigned act = (unsigned)*active;
     const v3 xw = world_pos(bodies[bid], ld3(P.x_ref, P.stride, p));
     for (int k = 0; k < np; ++k) {
-        if (!plane_active(planes[k], t)) continue;
+        if (!plane_active(act, k)) continue;
         atomicMin(w.dmin + (size_t)bid * np + k, ordered_bits(plane_dist(planes[k], xw)));
     }
 }
@@ -166,15 +166,15 @@ __global__ __launch_bounds__(kThreads) void k_rigid_contact_min(mpm_particles_t 
 __global__ __launch_bounds__(kThreads) void k_rigid_contact_index(mpm_particles_t P,
                                                                   const mpm_rigid_body_t* __restrict__ bodies, int nb,
                                                                   const mpm_rigid_plane_t* __restrict__ planes, int np,
-                                                                  const float* __restrict__ time, void* ws) {
+                                                                  const int32_t* __restrict__ active, void* ws) {
     const int p = blockIdx.x * kThreads + threadIdx.x;
     int bid;
     if (!rigid_particle(P, p, nb, bid)) return;
     const RigidWs w = rigid_ws(ws, nb, np);
-    const float t = *time;
+    const unsigned act = (unsigned)*active;
     const v3 xw = world_pos(bodies[bid], ld3(P.x_ref, P.stride, p));
     for (int k = 0; k < np; ++k) {
-        if (!plane_active(planes[k], t)) continue;
+        if (!plane_active(act, k)) continue;
         const size_t o = (size_t)bid * np + k;
         if (ordered_bits(plane_dist(planes[k], xw)) == w.dmin[o]) atomicMin(w.imin + o, (unsigned int)p);
     }
@@ -195,11 +195,11 @@ MPM_DEV void dmatvec(const float* M, const double* v, double* out) {
 // reference's numpy round trip does
 __global__ void k_rigid_restitution(mpm_particles_t P, mpm_rigid_body_t* bodies, int nb,
                                     const mpm_rigid_plane_t* __restrict__ planes, int np,
-                                    const float* __restrict__ time, void* ws) {
+                                    const int32_t* __restrict__ active, void* ws) {
     const int b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= nb) return;
     const RigidWs w = rigid_ws(ws, nb, np);
-    const float t = *time;
+    const unsigned act = (unsigned)*active;
     const mpm_rigid_body_t body0 = bodies[b];  // world positions start from this pose
     mpm_rigid_body_t body = body0;
     const m33 R = body_R(body);
@@ -212,7 +212,7 @@ __global__ void k_rigid_restitution(mpm_particles_t P, mpm_rigid_body_t* bodies,
     for (int k = 0; k < np; ++k) {
         const mpm_rigid_plane_t& pl = planes[k];
         const size_t o = (size_t)b * np + k;
-        if (!plane_active(pl, t) || w.imin[o] == 0xffffffffu) continue;
+        if (!plane_active(act, k) || w.imin[o] == 0xffffffffu) continue;
         const int pc = (int)w.imin[o];
         const double n[3] = {(double)pl.normal[0], (double)pl.normal[1], (double)pl.normal[2]};
         const v3 xw0 = world_pos(body0, ld3(P.x_ref, P.stride, pc));  // before any push
@@ -305,9 +305,10 @@ int64_t mpm_rigid_workspace_bytes(int32_t n_bodies, int32_t n_planes) {
 }
 
 int32_t mpm_rigid_step(const mpm_particles_t* P, const mpm_grid_t* G, mpm_rigid_body_t* bodies, int32_t n_bodies,
-                       const mpm_rigid_plane_t* planes, int32_t n_planes, const float* time, float dt, void* workspace,
-                       void* stream) {
-    if (!P || !G || !bodies || n_bodies < 1 || n_planes < 0 || (n_planes > 0 && !planes) || !time || !workspace ||
+                       const mpm_rigid_plane_t* planes, int32_t n_planes, const int32_t* active, float dt,
+                       void* workspace, void* stream) {
+    if (!P || !G || !bodies || n_bodies < 1 || n_planes < 0 || n_planes > 32 || (n_planes > 0 && !planes) ||
+        !active || !workspace ||
         P->n < 0 || P->stride < P->n || !P->x || !P->v || !P->C || !P->F || !P->mass || !P->material ||
         !P->selection || !P->rigid_id || !P->x_ref || !G->node_out || G->n_scenes != 1)
         return MPM_ERR_ARG;
@@ -318,12 +319,12 @@ int32_t mpm_rigid_step(const mpm_particles_t* P, const mpm_grid_t* G, mpm_rigid_
     if (pb) hipLaunchKernelGGL(k_rigid_gather, dim3(pb), dim3(kThreads), 0, s, *P, *G, bodies, nb, np, workspace);
     hipLaunchKernelGGL(k_rigid_integrate, dim3((nb + 63) / 64), dim3(64), 0, s, bodies, nb, np, workspace, dt);
     if (np > 0 && pb) {
-        hipLaunchKernelGGL(k_rigid_contact_min, dim3(pb), dim3(kThreads), 0, s, *P, bodies, nb, planes, np, time,
+        hipLaunchKernelGGL(k_rigid_contact_min, dim3(pb), dim3(kThreads), 0, s, *P, bodies, nb, planes, np, active,
                            workspace);
-        hipLaunchKernelGGL(k_rigid_contact_index, dim3(pb), dim3(kThreads), 0, s, *P, bodies, nb, planes, np, time,
+        hipLaunchKernelGGL(k_rigid_contact_index, dim3(pb), dim3(kThreads), 0, s, *P, bodies, nb, planes, np, active,
                            workspace);
         hipLaunchKernelGGL(k_rigid_restitution, dim3((nb + 63) / 64), dim3(64), 0, s, *P, bodies, nb, planes, np,
-                           time, workspace);
+                           active, workspace);
     }
     if (pb) hipLaunchKernelGGL(k_rigid_update, dim3(pb), dim3(kThreads), 0, s, *P, bodies, nb);
     return hipGetLastError() == hipSuccess ? MPM_OK : MPM_ERR_HIP;

@@ -36,7 +36,7 @@ class Particles(C.Structure):
                 ("stress", _FP), ("L", _FP), ("vol", _FP), ("mass", _FP), ("mu", _FP),
                 ("lam", _FP), ("yield_stress", _FP), ("Jp", _FP), ("material", _FP),
                 ("selection", _FP), ("cov", _FP), ("scene", _FP), ("cdf_tag", _FP),
-                ("rigid_id", _FP), ("x_ref", _FP)]
+                ("rigid_id", _FP), ("x_ref", _FP), ("modifier_mask", _FP)]
 
 
 # mpm_rigid_body_t / mpm_rigid_plane_t as numpy record types (device arrays of them)
@@ -45,6 +45,12 @@ RIGID_BODY_DTYPE = np.dtype([("x_cm", "<f4", 3), ("v_cm", "<f4", 3), ("omega", "
                              ("pad_", "<f4", 4)])
 RIGID_PLANE_DTYPE = np.dtype([("point", "<f4", 3), ("normal", "<f4", 3), ("restitution", "<f4"),
                               ("friction", "<f4"), ("start_time", "<f4"), ("end_time", "<f4")])
+# mpm_particle_modifier_t
+MODIFIER_DTYPE = np.dtype([("kind", "<i4"), ("bit", "<i4"), ("start_time", "<f4"), ("end_time", "<f4"),
+                           ("point", "<f4", 3), ("vec", "<f4", 3), ("axis1", "<f4", 3), ("axis2", "<f4", 3),
+                           ("size", "<f4", 3), ("half_height", "<f4"), ("radius", "<f4"),
+                           ("rotation_scale", "<f4"), ("translation_scale", "<f4"), ("pad_", "<f4", 3)])
+MAX_MODIFIERS = 32
 
 
 class Adjoint(C.Structure):
@@ -146,6 +152,8 @@ SIGNATURES = {
     "mpm_box_contact_wrench": (_I32, [_P, _P, _F, _I32, _P, _P, _P]),
     "mpm_rigid_workspace_bytes": (_I64, [_I32, _I32]),
     "mpm_rigid_step": (_I32, [_P, _P, _P, _I32, _P, _I32, _P, _F, _P, _P]),
+    "mpm_modifier_select": (_I32, [_P, _P, _P]),
+    "mpm_modifiers_apply": (_I32, [_P, _P, _I32, _P, _F, _P]),
     "mpm_mesh_field": (_I32, [_P, _I32, _P, _I32, _P, _F, _I32, _I32, _P, _P, _P]),
     "mpm_segment_distance": (_I32, [_P, _I32, _P, _F, _I32, _P, _P]),
     "mpm_adjoint_g2p": (_I32, [_P, _P, _P, _P, _F, _P]),
@@ -177,7 +185,7 @@ def load():
     sizes = {0: C.sizeof(Particles), 1: C.sizeof(Grid), 2: MODEL_DTYPE.itemsize,
              3: COLLIDER_DTYPE.itemsize, 4: TICK_CTL_DTYPE.itemsize,
              5: CDF_LANES_DTYPE.itemsize, 6: C.sizeof(CdfCollider), 7: C.sizeof(Adjoint),
-             8: RIGID_BODY_DTYPE.itemsize, 9: RIGID_PLANE_DTYPE.itemsize}
+             8: RIGID_BODY_DTYPE.itemsize, 9: RIGID_PLANE_DTYPE.itemsize, 10: MODIFIER_DTYPE.itemsize}
     for which, size in sizes.items():
         got = lib.mpm_struct_size(which)
         if got != size:

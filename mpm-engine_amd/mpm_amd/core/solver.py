@@ -190,7 +190,16 @@ class HipSimulator:
         self._rigid_planes_dev = None
         self._rigid_planes_key = None
         self.rigid_ws = None
-        self.rigid_times = None     # device float per substep of the current tick
+        # particle modifiers (pre-P2G impulses, velocity modifiers), one mask bit each
+        self.modifier_mask = None
+        self.modifiers = []
+        self._modifiers_dev = None
+        self._modifiers_key = None
+        # device int32 [S, 2] per tick: bit j of column 0 = modifier j active in substep k,
+        # of column 1 = restitution plane j active (windows evaluated in float64 on the
+        # host, as the reference's host loop compares its times)
+        self.substep_active = None
+        self._windows = ([], [])  # (modifier, plane) (start, end) in float64
         # deterministic scatter (fixed-point integer accumulation), off by default
         self.deterministic = False
         self.node_fix = None
@@ -210,6 +219,7 @@ class HipSimulator:
         P.cdf_tag = native.ptr(self.cdf_tag)
         P.rigid_id = native.ptr(self.rigid_id)
         P.x_ref = native.ptr(self.x_ref)
+        P.modifier_mask = native.ptr(self.modifier_mask)
         G = native.Grid()
         G.node_in, G.node_out = native.ptr(self.node_in), native.ptr(self.node_out)
         G.n[:] = [self.n_grid] * 3
@@ -390,7 +400,10 @@ class HipSimulator:
         row["point"], row["normal"] = point, normal
         row["restitution"], row["friction"] = restitution, friction
         row["start_time"], row["end_time"] = start_time, min(end_time, 3.0e38)
+        if len(self.rigid_planes) >= 32:
+            raise ValueError("at most 32 restitution planes")
         self.rigid_planes.append(row)
+        self._windows[1].append((float(start_time), float(end_time)))
         if self.n_rigid_bodies:
             self._rigid_ws_ensure()
 
@@ -424,8 +437,77 @@ class HipSimulator:
         planes = self._rigid_planes_device()
         native.check(self.lib.mpm_rigid_step(
             self.pP, self.pG, native.ptr(self.rigid_bodies), self.n_rigid_bodies, native.ptr(planes),
-            len(self.rigid_planes), native.ptr(self.rigid_times) + 4 * k, dt, native.ptr(self.rigid_ws), s),
+            len(self.rigid_planes), native.ptr(self.substep_active) + 8 * k + 4, dt, native.ptr(self.rigid_ws), s),
             "rigid_step")
+
+    # --------------------------------------------------------------- particle modifiers
+    def _add_modifier(self, row, t0: float, t1: float):
+        if len(self.modifiers) >= native.MAX_MODIFIERS:
+            raise ValueError(f"at most {native.MAX_MODIFIERS} particle modifiers")
+        if self.modifier_mask is None:
+            self.modifier_mask = self.torch.zeros(self.n_particles, dtype=self.torch.int32, device=self.device)
+            self._structs()
+        row["bit"] = len(self.modifiers)
+        native.check(self.lib.mpm_modifier_select(self.pP, row.ctypes.data, self.stream()), "modifier_select")
+        self.modifiers.append(row)
+        self._windows[0].append((float(t0), float(t1)))
+        self._graph = None
+
+    def add_impulse_on_particles(self, force, dt, point=(1, 1, 1), size=(1, 1, 1), num_dt=1, start_time=0.0,
+                                 device=None):
+        """add_impulse_on_particles (mpm_solver_warp.py:2385-2427): particles in the box
+        |x - point| < size (selected now) get v += force / mass * dt while
+        start_time <= t < start_time + dt * num_dt."""
+        row = np.zeros(1, dtype=native.MODIFIER_DTYPE)
+        row["kind"] = 0
+        row["start_time"], row["end_time"] = start_time, start_time + dt * num_dt
+        row["point"], row["size"], row["vec"] = point, size, force
+        self._add_modifier(row, start_time, start_time + dt * num_dt)
+
+    def enforce_particle_velocity_translation(self, point, size, velocity, start_time, end_time, device=None):
+        """enforce_particle_velocity_translation (mpm_solver_warp.py:2431-2469): particles in
+        the box (selected now) move at `velocity` while start_time <= t < end_time."""
+        row = np.zeros(1, dtype=native.MODIFIER_DTYPE)
+        row["kind"] = 1
+        row["start_time"], row["end_time"] = start_time, end_time
+        row["point"], row["size"], row["vec"] = point, size, velocity
+        self._add_modifier(row, start_time, end_time)
+
+    def enforce_particle_velocity_rotation(self, point, normal, half_height_and_radius, rotation_scale,
+                                           translation_scale, start_time, end_time, device=None):
+        """enforce_particle_velocity_rotation (mpm_solver_warp.py:2475-2539): particles in
+        the cylinder about `normal` through `point` (selected now) rotate about the axis at
+        rotation_scale rad/s and translate along it at translation_scale."""
+        n = np.asarray(normal, np.float64)
+        n = (1.0 / math.sqrt(float((n * n).sum()))) * n
+        nf = n.astype(f32)
+        h1 = np.ones(3, f32)
+        if abs(float(np.dot(nf, h1))) < 0.01:
+            h1 = np.array([0.72, 0.37, -0.67], f32)
+        h1 = (h1 - np.dot(h1, nf) * nf).astype(f32)
+        h1 = (h1 * (f32(1.0) / np.sqrt(np.dot(h1, h1)))).astype(f32)
+        h2 = np.cross(h1, nf).astype(f32)
+        row = np.zeros(1, dtype=native.MODIFIER_DTYPE)
+        row["kind"] = 2
+        row["start_time"], row["end_time"] = start_time, end_time
+        row["point"], row["vec"], row["axis1"], row["axis2"] = point, nf, h1, h2
+        row["half_height"], row["radius"] = half_height_and_radius[0], half_height_and_radius[1]
+        row["rotation_scale"], row["translation_scale"] = rotation_scale, translation_scale
+        self._add_modifier(row, start_time, end_time)
+
+    def _modifiers_device(self):
+        key = len(self.modifiers)
+        if self._modifiers_key != key:
+            rows = np.concatenate(self.modifiers)
+            self._modifiers_dev = self.torch.from_numpy(rows.view(np.uint8).copy()).to(self.device)
+            self._modifiers_key = key
+            self._graph = None
+        return self._modifiers_dev
+
+    def _apply_modifiers(self, k, s):
+        native.check(self.lib.mpm_modifiers_apply(
+            self.pP, native.ptr(self._modifiers_device()), len(self.modifiers),
+            native.ptr(self.substep_active) + 8 * k, self._dt, s), "modifiers_apply")
 
     def set_gravity(self, g):
         self.model[0]["gravity"] = np.asarray(g, dtype=np.float64).astype(f32)
@@ -598,17 +680,25 @@ class HipSimulator:
         s = self.stream()
         if self.deterministic:
             self._fix_scales(dt)
+        self._dt = float(dt)
+        if self.modifiers:
+            self._modifiers_device()
         if self.n_rigid_bodies:
-            # rigid scenes run the split pipeline (solver.py:441-444); the substep start
-            # times feed the restitution planes' activity windows
-            fused = False
             self._rigid_planes_device()
             self._rigid_ws_ensure()
-            if self.rigid_times is None or self.rigid_times.numel() < S:
-                self.rigid_times = torch.zeros(max(S, 1), dtype=torch.float32, device=self.device)
+        if self.n_rigid_bodies or self.modifiers:
+            # rigid bodies and particle modifiers run the split pipeline (solver.py:
+            # 441-444); their activity windows against each substep's start time
+            fused = False
+            if self.substep_active is None or self.substep_active.shape[0] < S:
+                self.substep_active = torch.zeros(max(S, 1), 2, dtype=torch.int32, device=self.device)
                 self._graph = None
-            times = ColliderSet.times(self.time, dt, S)[0].astype(np.float32)
-            self.rigid_times[:S].copy_(torch.from_numpy(times))
+            times = ColliderSet.times(self.time, dt, S)[0]
+            act = np.zeros((S, 2), dtype=np.uint32)
+            for col, wins in enumerate(self._windows):
+                for j, (t0, t1) in enumerate(wins):
+                    act[:, col] |= (((times >= t0) & (times < t1)).astype(np.uint32) << np.uint32(j))
+            self.substep_active[:S].copy_(torch.from_numpy(act.view(np.int32)))
         tab, t_end, K, n_slots = self._schedule(dt, S)
         self.n_slots = n_slots
         if 6 * n_slots > self.wrench.numel():  # grow, keeping the accumulated values
@@ -671,8 +761,10 @@ class HipSimulator:
                    None if cdf is None else (cdf.n, int(cdf.mask_off), native.ptr(self.cdf_tag)),
                    self._fix_exp if self.deterministic else None,
                    (self.n_rigid_bodies, native.ptr(self.rigid_bodies), native.ptr(self._rigid_planes_dev),
-                    len(self.rigid_planes), native.ptr(self.rigid_times), native.ptr(self.rigid_ws),
-                    native.ptr(self.x_ref)) if self.n_rigid_bodies else None)
+                    len(self.rigid_planes), native.ptr(self.substep_active), native.ptr(self.rigid_ws),
+                    native.ptr(self.x_ref)) if self.n_rigid_bodies else None,
+                   (len(self.modifiers), native.ptr(self._modifiers_dev), native.ptr(self.modifier_mask),
+                    native.ptr(self.substep_active)) if self.modifiers else None)
             if self._graph is None or self._graph_key != key:
                 self._capture(dt, S, K, fused, lists)
                 self._graph_key = key
@@ -726,6 +818,8 @@ class HipSimulator:
         for k in range(S):
             t = float(times[k])
             cdf.stamp(t, lag_prev=fused)
+            if self.modifiers and not fused:
+                self._apply_modifiers(k, s)
             self._pass(G2P | ST | P2G if (fused and k > 0) else ST | P2G, dt, s)
             self._grid(dt, k, K, lists, s)
             cdf.integrate(t, dt, lag_prev=fused)
@@ -753,6 +847,8 @@ class HipSimulator:
             self._pass(G2P, dt, s)
         else:
             for k in range(S):
+                if self.modifiers:
+                    self._apply_modifiers(k, s)
                 self._pass(ST | P2G, dt, s)
                 self._grid(dt, k, K, lists, s)
                 self._pass(G2P, dt, s)
@@ -774,6 +870,8 @@ class HipSimulator:
 
         G2P, ST, P2G = native.PASS_G2P, native.PASS_STRESS, native.PASS_P2G
         for k in range(S):
+            if self.modifiers and not fused:
+                timed("particle_modifiers", lambda k=k: self._apply_modifiers(k, s))
             if fused and k > 0:
                 timed("g2p2g_fused", lambda: self._pass(G2P | ST | P2G, dt, s))
             else:

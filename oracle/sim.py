@@ -265,6 +265,9 @@ class OracleSim:
         self.x_ref = np.zeros((n, 3), f32)
         self.n_rigid_bodies = 0
         self.rigid_surface_colliders = []
+        # particle modifiers (mpm_solver_warp.py:486-490): parameters and selection masks
+        self.impulse_params = []
+        self.particle_velocity_modifier_params = []
         g = self.n_grid
         self.grid_m = np.zeros((g, g, g), f32)
         self.grid_v_in = np.zeros((g, g, g, 3), f32)
@@ -542,6 +545,72 @@ class OracleSim:
                         J_t = min(v_t_mag / denom_t, mu * J_n)
                         self.rigid_v_cm[b] = self.rigid_v_cm[b] - (J_t / M) * t_hat
                         self.rigid_omega[b] = self.rigid_omega[b] - I_world_inv @ (J_t * r_cross_t)
+
+    # ------------------------------------------------------------ particle modifiers
+    def _box_mask(self, point, size):
+        off = self.x - np.asarray(point, f32)
+        return np.all(np.abs(off) < np.asarray(size, f32), axis=1).astype(np.int32)
+
+    def add_impulse_on_particles(self, force, dt, point=(1, 1, 1), size=(1, 1, 1), num_dt=1, start_time=0.0):
+        """mpm_solver_warp.py:2385-2427 with selection_add_impulse_on_particles
+        (mpm_utils.py:1320-1331)."""
+        self.impulse_params.append({
+            "start_time": start_time, "end_time": start_time + dt * num_dt,
+            "force": np.asarray(force, f32), "mask": self._box_mask(point, size)})
+
+    def enforce_particle_velocity_translation(self, point, size, velocity, start_time, end_time):
+        """mpm_solver_warp.py:2431-2469 with selection_enforce_particle_velocity_translation
+        (mpm_utils.py:1334-1345)."""
+        self.particle_velocity_modifier_params.append({
+            "kind": "translation", "start_time": start_time, "end_time": end_time,
+            "velocity": np.asarray(velocity, f32), "mask": self._box_mask(point, size)})
+
+    def enforce_particle_velocity_rotation(self, point, normal, half_height_and_radius, rotation_scale,
+                                           translation_scale, start_time, end_time):
+        """mpm_solver_warp.py:2475-2539 with selection_enforce_particle_velocity_cylinder
+        (mpm_utils.py:1348-1363)."""
+        n = np.asarray(normal, np.float64)
+        n = ((1.0 / np.sqrt((n * n).sum())) * n).astype(f32)
+        h1 = np.ones(3, f32)
+        if abs(float(np.dot(n, h1))) < 0.01:
+            h1 = np.array([0.72, 0.37, -0.67], f32)
+        h1 = (h1 - np.dot(h1, n) * n).astype(f32)
+        h1 = (h1 * (f32(1.0) / np.sqrt(np.dot(h1, h1)))).astype(f32)
+        h2 = np.cross(h1, n).astype(f32)
+        pt = np.asarray(point, f32)
+        off = self.x - pt
+        ax = off @ n
+        hd = np.linalg.norm(off - ax[:, None] * n, axis=1)
+        mask = ((np.abs(ax) < f32(half_height_and_radius[0])) & (hd < f32(half_height_and_radius[1]))).astype(np.int32)
+        self.particle_velocity_modifier_params.append({
+            "kind": "rotation", "start_time": start_time, "end_time": end_time, "point": pt, "normal": n,
+            "axis1": h1, "axis2": h2, "rotation_scale": f32(rotation_scale),
+            "translation_scale": f32(translation_scale), "mask": mask})
+
+    def _apply_particle_modifiers(self, dt):
+        """The split substep's pre_p2g_operations, then particle_velocity_modifiers
+        (mpm_solver_warp.py:1181-1196), each on its selected particles while active."""
+        t = self.time
+        for p in self.impulse_params:
+            if p["start_time"] <= t < p["end_time"]:
+                sel = p["mask"] == 1
+                m = self.mass[sel][:, None]
+                self.v[sel] = self.v[sel] + (p["force"][None, :] / m) * f32(dt)
+        for p in self.particle_velocity_modifier_params:
+            if not (p["start_time"] <= t < p["end_time"]):
+                continue
+            sel = p["mask"] == 1
+            if p["kind"] == "translation":
+                self.v[sel] = p["velocity"]
+                continue
+            n, a1, a2 = p["normal"], p["axis1"], p["axis2"]
+            off = self.x[sel] - p["point"]
+            hd = np.linalg.norm(off - (off @ n)[:, None] * n, axis=1).astype(f32)
+            th = np.arccos(np.clip((off @ a1) / hd, -1.0, 1.0)).astype(f32)
+            th = np.where(off @ a2 > 0, th, -th)
+            s1 = (-hd * np.sin(th) * p["rotation_scale"]).astype(f32)
+            s2 = (hd * np.cos(th) * p["rotation_scale"]).astype(f32)
+            self.v[sel] = (s1[:, None] * a1 + s2[:, None] * a2 + p["translation_scale"] * n).astype(f32)
 
     def _p2g2p_tail(self, dt):
         """mpm_solver_warp.py:1338-1377: the rigid step, then time += dt."""
@@ -1139,6 +1208,7 @@ class OracleSim:
         else:
             g, z = self._boxes()
             L.o_zero_grid(st, md, i3(z[0]), i3(z[1]), 7)
+        self._apply_particle_modifiers(dt)
         L.o_compute_stress(st, md, dt)
         L.o_p2g(st, md, dt)
         if sparse:
@@ -1224,6 +1294,8 @@ class OracleSim:
         for nm in ("init_cov", "cov"):
             a = getattr(self, nm).reshape(-1, 6)
             a[...] = a[perm]
+        for p in self.impulse_params + self.particle_velocity_modifier_params:
+            p["mask"] = p["mask"][perm]
 
     def compute_R(self):
         lib().o_compute_R(self._state())
@@ -1397,7 +1469,9 @@ class OracleSolver:
         if self.sparse:
             self._sim.rebuild_active_blocks()
         self._tick += 1
-        if self.fused and not self.sparse and self._sim.n_rigid_bodies == 0:
+        sim = self._sim
+        if (self.fused and not self.sparse and sim.n_rigid_bodies == 0 and not sim.impulse_params
+                and not sim.particle_velocity_modifier_params):
             self._sim.p2g2p_fused_tick(dt, substeps)
             self._step += substeps
         else:

@@ -39,7 +39,8 @@ def test_struct_layouts_and_introspection():
     assert lib.mpm_struct_size(7) == ctypes.sizeof(native.Adjoint)
     assert lib.mpm_struct_size(8) == native.RIGID_BODY_DTYPE.itemsize
     assert lib.mpm_struct_size(9) == native.RIGID_PLANE_DTYPE.itemsize
-    assert lib.mpm_struct_size(10) == -1
+    assert lib.mpm_struct_size(10) == native.MODIFIER_DTYPE.itemsize
+    assert lib.mpm_struct_size(11) == -1
     assert lib.mpm_status_string(1) == b"invalid argument"
 
 

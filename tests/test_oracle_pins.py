@@ -628,3 +628,49 @@ def test_rigid_restitution_reverses_contact_normal_velocity():
                                atol=1e-6 * np.abs(M * (v1 - v0)).max())
     x_new = s.rigid_x_cm[0] + (Rz @ s.x_ref.T).T
     assert ((x_new - [0, 0, 0.3]) @ [0, 0, 1.0]).min() > -1e-6
+
+
+# ------------------------------------------------------------------ particle modifiers
+def _modifier_block(fused=False):
+    from mpm_amd.core.solver import GridConfig
+
+    g = GridConfig(n_grid=32, grid_lim=1.0)
+    h = g.dx / 2
+    ax = np.arange(0.3 + h / 2, 0.7, h)
+    pts = np.stack(np.meshgrid(ax, ax, np.arange(0.3 + h / 2, 0.5, h), indexing="ij"), -1).reshape(-1, 3).astype(f32)
+    o = O.OracleSolver(g, fused=fused)
+    o.load_particles(pts, np.full(len(pts), h ** 3, f32))
+    o.set_material("jelly", E=1e5, nu=0.3, density=1000.0, g=[0.0, 0.0, 0.0])
+    return o, pts
+
+
+def test_modifier_selection_and_closed_forms():
+    """Particle modifiers (mpm_solver_warp.py:2385-2539): the box selection is strict
+    |x - point| < size per axis; the cylinder selection |axial| < half height and radial
+    < radius; on the first substep, an active translation sets v = velocity exactly; a
+    rotation sets v = -s_r n x r + s_t n for the radial offset r (speed s_r |r|, normal to
+    r; the reference's second axis is axis1 x n, so a positive rotation_scale turns about
+    -n); an impulse adds force / mass * dt, only inside its window; unselected particles
+    are untouched."""
+    o, pts = _modifier_block()
+    s = o._sim
+    s.enforce_particle_velocity_translation((0.35, 0.5, 0.4), (0.04, 0.3, 0.3), (0.2, 0.0, 0.0), 0.0, 1.0)
+    s.enforce_particle_velocity_rotation((0.6, 0.5, 0.4), (0, 0, 1), (0.3, 0.08), 3.0, 0.1, 0.0, 1.0)
+    s.add_impulse_on_particles((0.0, 0.0, 5e-4), 1e-4, point=(0.5, 0.35, 0.4), size=(0.05, 0.04, 0.3), num_dt=3)
+    box = np.all(np.abs(pts - [0.35, 0.5, 0.4]) < [0.04, 0.3, 0.3], axis=1)
+    off = pts - [0.6, 0.5, 0.4]
+    cyl = (np.abs(off[:, 2]) < 0.3) & (np.linalg.norm(off[:, :2], axis=1) < 0.08)
+    imp = np.all(np.abs(pts - [0.5, 0.35, 0.4]) < [0.05, 0.04, 0.3], axis=1) & ~box & ~cyl
+    np.testing.assert_array_equal(s.particle_velocity_modifier_params[0]["mask"], box)
+    np.testing.assert_array_equal(s.particle_velocity_modifier_params[1]["mask"], cyl)
+    s._apply_particle_modifiers(1e-4)
+    np.testing.assert_array_equal(s.v[box], np.tile([0.2, 0.0, 0.0], (box.sum(), 1)).astype(f32))
+    w_expected = -3.0 * np.stack([-off[cyl, 1], off[cyl, 0], np.zeros(cyl.sum())], 1) + [0, 0, 0.1]
+    np.testing.assert_allclose(s.v[cyl], w_expected, atol=2e-6)
+    dv = 5e-4 / s.mass[imp] * 1e-4
+    np.testing.assert_allclose(s.v[imp][:, 2], dv, rtol=1e-6)
+    assert np.all(s.v[~(box | cyl | imp)] == 0)
+    s.time = 3.5e-4  # past the impulse window (3 dt), the overrides still active
+    before = s.v.copy()
+    s._apply_particle_modifiers(1e-4)
+    np.testing.assert_array_equal(s.v[imp], before[imp])
