@@ -67,6 +67,11 @@ typedef enum mpm_status {
 #define MPM_GRID_ZERO_IN 1      /* clear node_in after it is consumed */
 #define MPM_GRID_FIXED_IN 2     /* take (momentum, mass) from node_fix (deterministic scatter)
                                    instead of node_in, and clear node_fix after use */
+#define MPM_GRID_WRENCH_DEFER 4 /* with MPM_GRID_FIXED_IN: add this call's integer wrench into
+                                   the workspace rows and leave them there; a later
+                                   mpm_grid_wrench_flush converts and clears them (one flush
+                                   per tick instead of one per substep; integer sums, so the
+                                   result does not depend on where the flush falls) */
 
 /* collider kinds (the reference's grid_postprocess plugins) */
 #define MPM_COLLIDER_PLANE 1      /* mpm_solver_warp.py:1929-1992 */
@@ -357,7 +362,9 @@ int32_t mpm_blocks_from_box(const mpm_grid_t *G, mpm_tick_ctl_t *ctl, int32_t *b
  * are added with float atomics and `workspace` may be NULL. With MPM_GRID_FIXED_IN the
  * per-wavefront sums are rounded to 64-bit fixed point and summed as integers (per
  * workgroup rows in `workspace`, or above 64 slots integer atomics into a row that is zero
- * between calls), then added to `wrench`: order-independent, for any slot count.
+ * between calls), then added to `wrench`: order-independent, for any slot count. The
+ * integer rows are zero between flushes; with MPM_GRID_WRENCH_DEFER they keep
+ * accumulating until mpm_grid_wrench_flush (same capacity, slots and workspace).
  * A block of scene s applies colliders[s * ncoll .. s * ncoll + ncoll). Nodes without mass above 1e-15 read velocity 0 (the
  * split-zero contract, mpm_utils.py:1193-1202). With MPM_GRID_ZERO_IN, node_in is
  * cleared afterwards. */
@@ -367,6 +374,10 @@ int32_t mpm_grid_update(const mpm_grid_t *G, const mpm_model_t *model, const mpm
                         const mpm_collider_t *colliders, int32_t ncoll, float *wrench,
                         int32_t wrench_slots, float *workspace, float dt, int32_t flags,
                         void *stream);
+/* Convert the integer wrench rows that MPM_GRID_FIXED_IN | MPM_GRID_WRENCH_DEFER calls left
+ * in `workspace` into `wrench` (added) and clear them. */
+int32_t mpm_grid_wrench_flush(const mpm_grid_t *G, int32_t capacity, float *wrench, int32_t wrench_slots,
+                              float *workspace, void *stream);
 
 /* ---- sparse active blocks (mpm_utils.py:1506-1544, mpm_solver_warp.py:989-1026) ----- */
 /* raw must be zero on entry; marks the stencil-base block of every particle. */

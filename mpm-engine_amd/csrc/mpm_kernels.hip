@@ -294,7 +294,8 @@ __global__ __launch_bounds__(kThreads) void k_grid_update(mpm_grid_t G, const mp
             if (fixed) {
                 unsigned long long acc = 0ull;
                 for (int w = 0; w < kThreads / 64; ++w) acc += s_wraw[w][t];
-                reinterpret_cast<unsigned long long*>(partials)[(size_t)blockIdx.x * nslots * 6 + t] = acc;
+                unsigned long long* row = reinterpret_cast<unsigned long long*>(partials) + (size_t)blockIdx.x * nslots * 6;
+                row[t] = (flags & MPM_GRID_WRENCH_DEFER) ? row[t] + acc : acc;
             } else {
                 float acc = s_w[0][t];
                 for (int w = 1; w < kThreads / 64; ++w) acc += s_w[w][t];
@@ -329,7 +330,10 @@ __global__ __launch_bounds__(kThreads) void k_wrench_fix_flush(mpm_grid_t G, uns
     __shared__ unsigned long long sh[kThreads];
     const int v = blockIdx.x;  // one workgroup per value: strided sums, then an LDS tree
     unsigned long long acc = 0ull;
-    for (int b = threadIdx.x; b < nparts; b += kThreads) acc += fixed[(size_t)b * nvals + v];
+    for (int b = threadIdx.x; b < nparts; b += kThreads) {
+        acc += fixed[(size_t)b * nvals + v];
+        fixed[(size_t)b * nvals + v] = 0ull;  // consumed: rows are zero between flushes
+    }
     sh[threadIdx.x] = acc;
     __syncthreads();
     for (int w = kThreads / 2; w > 0; w >>= 1) {
@@ -341,7 +345,6 @@ __global__ __launch_bounds__(kThreads) void k_wrench_fix_flush(mpm_grid_t G, uns
         wrench_scales(G, sf, st);
         const double inv = 1.0 / (double)((v % 6) < 3 ? sf : st);
         wrench[v] += (float)((double)(long long)sh[0] * inv);
-        if (nparts == 1) fixed[v] = 0ull;
     }
 }
 
@@ -682,13 +685,26 @@ int32_t mpm_grid_update(const mpm_grid_t* G, const mpm_model_t* model, const mpm
     hipStream_t s = (hipStream_t)stream;
     hipLaunchKernelGGL(k_grid_update, dim3(nwg), dim3(kThreads), 0, s, *G, model, ctl, list, count, colliders, ncoll,
                        wrench_slots, workspace, wrench, dt, flags);
-    if (wrench_slots > 0 && fixed)
+    if (wrench_slots > 0 && fixed && !(flags & MPM_GRID_WRENCH_DEFER))
         hipLaunchKernelGGL(k_wrench_fix_flush, dim3(wrench_slots * 6), dim3(kThreads), 0, s, *G,
                            reinterpret_cast<unsigned long long*>(workspace), fixed_order ? (int)nwg : 1,
                            wrench_slots * 6, wrench);
-    else if (wrench_slots > 0 && fixed_order)
+    else if (wrench_slots > 0 && fixed_order && !fixed)
         hipLaunchKernelGGL(k_wrench_reduce, dim3(wrench_slots * 6), dim3(kThreads), 0, s, workspace, (int)nwg,
                            wrench_slots * 6, wrench);
+    return launch_status();
+}
+
+int32_t mpm_grid_wrench_flush(const mpm_grid_t* G, int32_t capacity, float* wrench, int32_t wrench_slots,
+                              float* workspace, void* stream) {
+    if (!grid_ok(G) || capacity < 0 || wrench_slots < 0 || (wrench_slots > 0 && (!wrench || !workspace)) ||
+        !(G->fix_scale[0] > 0.0f) || !(G->fix_scale[1] > 0.0f))
+        return MPM_ERR_ARG;
+    if (wrench_slots == 0 || capacity == 0) return MPM_OK;
+    const unsigned nwg = grid_update_wgs(capacity);
+    hipLaunchKernelGGL(k_wrench_fix_flush, dim3(wrench_slots * 6), dim3(kThreads), 0, (hipStream_t)stream, *G,
+                       reinterpret_cast<unsigned long long*>(workspace), wrench_slots <= kMaxSlots ? (int)nwg : 1,
+                       wrench_slots * 6, wrench);
     return launch_status();
 }
 

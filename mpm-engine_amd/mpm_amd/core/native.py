@@ -20,7 +20,7 @@ PASS_G2P, PASS_STRESS, PASS_P2G, PASS_LEAN, PASS_KEEP_STRESS, PASS_VISCOUS_ONLY 
 PASS_WG128, PASS_PPT2, PASS_PPT4 = 64, 128, 256  # launch shape (results do not depend on it)
 PASS_DETERMINISTIC = 512
 VISCOUS_IDS = frozenset((6, 7, 8, 10, 12))
-GRID_ZERO_IN, GRID_FIXED_IN = 1, 2
+GRID_ZERO_IN, GRID_FIXED_IN, GRID_WRENCH_DEFER = 1, 2, 4
 MAX_FIXED_ORDER_SLOTS = 64  # wrench slots reduced in fixed order; more use float atomics
 ERR_STENCIL, ERR_NONFINITE, ERR_SOLID_IN_VISCOUS, ERR_SCENE = 1, 2, 4, 8
 ERR_ADJ_MATERIAL, ERR_ADJ_COLLIDER, ERR_FIXED_RANGE = 16, 32, 64
@@ -153,6 +153,7 @@ SIGNATURES = {
     "mpm_rigid_workspace_bytes": (_I64, [_I32, _I32]),
     "mpm_rigid_step": (_I32, [_P, _P, _P, _I32, _P, _I32, _P, _F, _P, _P]),
     "mpm_modifier_select": (_I32, [_P, _P, _P]),
+    "mpm_grid_wrench_flush": (_I32, [_P, _I32, _P, _I32, _P, _P]),
     "mpm_modifiers_apply": (_I32, [_P, _P, _I32, _P, _F, _P]),
     "mpm_mesh_field": (_I32, [_P, _I32, _P, _I32, _P, _F, _I32, _I32, _P, _P, _P]),
     "mpm_segment_distance": (_I32, [_P, _I32, _P, _F, _I32, _P, _P]),

@@ -802,11 +802,19 @@ class HipSimulator:
     def _grid(self, dt, k_row, K, lists, s):
         sched = native.ptr(self.sched_dev) + k_row * self.n_scenes * K * native.COLLIDER_DTYPE.itemsize
         slots = self.n_slots
+        # deterministic: the integer wrench rows accumulate over the tick and are flushed
+        # once at its end (_wrench_flush)
+        flags = native.GRID_FIXED_IN | native.GRID_WRENCH_DEFER if self.deterministic else native.GRID_ZERO_IN
         native.check(self.lib.mpm_grid_update(
             self.pG, native.ptr(self.model_dev), native.ptr(self.ctl), lists[0], lists[1],
             self.n_blocks, sched if K else None, K, native.ptr(self.wrench), slots,
-            native.ptr(self.grid_ws) if slots else None, dt,
-            native.GRID_FIXED_IN if self.deterministic else native.GRID_ZERO_IN, s), "grid_update")
+            native.ptr(self.grid_ws) if slots else None, dt, flags, s), "grid_update")
+
+    def _wrench_flush(self, s):
+        if self.deterministic and self.n_slots:
+            native.check(self.lib.mpm_grid_wrench_flush(
+                self.pG, self.n_blocks, native.ptr(self.wrench), self.n_slots, native.ptr(self.grid_ws), s),
+                "wrench_flush")
 
     def _enqueue_tick_cdf(self, dt, S, K, fused, lists, s):
         """One tick with the CPIC per-substep host work between the launches (the
@@ -830,6 +838,7 @@ class HipSimulator:
         if fused:
             cdf.reconcile_prev()
             self._pass(G2P, dt, s)
+        self._wrench_flush(s)
 
     def _enqueue_tick(self, dt, S, K, fused, lists, s):
         G2P, ST, P2G = native.PASS_G2P, native.PASS_STRESS, native.PASS_P2G
@@ -845,6 +854,7 @@ class HipSimulator:
                 self._pass(mode, dt, s)
                 self._grid(dt, k, K, lists, s)
             self._pass(G2P, dt, s)
+            self._wrench_flush(s)
         else:
             for k in range(S):
                 if self.modifiers:
@@ -854,6 +864,7 @@ class HipSimulator:
                 self._pass(G2P, dt, s)
                 if self.n_rigid_bodies:
                     self._rigid(k, dt, s)
+            self._wrench_flush(s)
 
     def _enqueue_tick_profiled(self, dt, S, K, fused, lists):
         torch = self.torch
@@ -883,6 +894,7 @@ class HipSimulator:
                     timed("rigid_body", lambda k=k: self._rigid(k, dt, s))
         if fused:
             timed("g2p", lambda: self._pass(G2P, dt, s))
+        timed("wrench_flush", lambda: self._wrench_flush(s))
         torch.cuda.synchronize(self.device)
         for name, a, b in ev:
             self.time_profile.setdefault(name, []).append(a.elapsed_time(b))
