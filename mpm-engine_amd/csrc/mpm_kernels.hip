@@ -239,8 +239,10 @@ __global__ __launch_bounds__(kThreads) void k_grid_update(mpm_grid_t G, const mp
             const longlong2 f01 = f[0], f23 = f[1];
             in4 = make_float4((float)((double)f01.x * inv_p), (float)((double)f01.y * inv_p),
                               (float)((double)f23.x * inv_p), (float)((double)f23.y * inv_m));
-            f[0] = make_longlong2(0, 0);
-            f[1] = make_longlong2(0, 0);
+            if ((f01.x | f01.y | f23.x | f23.y) != 0) {  // empty nodes are already clear
+                f[0] = make_longlong2(0, 0);
+                f[1] = make_longlong2(0, 0);
+            }
         } else if (valid) {
             in4 = *reinterpret_cast<const float4*>(G.node_in + o);
         }
@@ -285,7 +287,8 @@ __global__ __launch_bounds__(kThreads) void k_grid_update(mpm_grid_t G, const mp
             }
         }
         if (inbox) *reinterpret_cast<float4*>(G.node_out + o) = make_float4(vo.x, vo.y, vo.z, 0.0f);
-        if (valid && !fixed && (flags & MPM_GRID_ZERO_IN))
+        if (valid && !fixed && (flags & MPM_GRID_ZERO_IN) &&
+            (in4.x != 0.0f || in4.y != 0.0f || in4.z != 0.0f || in4.w != 0.0f))  // empty: already clear
             *reinterpret_cast<float4*>(G.node_in + o) = make_float4(0.f, 0.f, 0.f, 0.f);
     }
     if (nslots > 0 && lds_w) {
