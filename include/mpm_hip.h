@@ -162,6 +162,13 @@ typedef struct mpm_grid {
     int64_t *node_fix;    /* deterministic scatter: 4 int64 per node (momentum xyz, mass) in
                              fixed point, zero between uses; NULL otherwise */
     float fix_scale[2];   /* units per kg (mass) and per kg m/s (momentum), powers of two */
+    int32_t *block_touch; /* per grid block (global block codes), or NULL: bit 0 = a particle
+                             pass scattered into the block since the last grid update (the
+                             pass stores 3), bit 1 = the last grid update wrote velocities
+                             into it. The grid update skips blocks with neither bit (their
+                             accumulators and node_out are zero), only zeroes node_out of
+                             blocks at 2, and leaves processed blocks at 2. NULL: every
+                             listed block is processed. Zero-initialised with node_out. */
 } mpm_grid_t;
 
 /* Model parameters (MPMModelStruct, warp_utils.py:23-84). Lives in device memory; the

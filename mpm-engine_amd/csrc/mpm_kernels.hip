@@ -222,9 +222,21 @@ __global__ __launch_bounds__(kThreads) void k_grid_update(mpm_grid_t G, const mp
     const double inv_p = fixed ? 1.0 / (double)G.fix_scale[1] : 0.0;
     float sw_f = 0.f, sw_t = 0.f;  // fixed-point wrench scales (force, torque)
     if (fixed) wrench_scales(G, sw_f, sw_t);
-    for (int b = blockIdx.x * (kThreads / 64) + wave; b < nblk; b += waves_total) {
-        const int code = list[b];
-        if ((unsigned)code >= (unsigned)ncode) continue;  // wavefront-uniform
+    // Wavefront w owns list entries w, w + W, w + 2W, ... (W wavefronts in the launch) and
+    // takes 64 of them at a time (one per lane) with their block states, ballots the
+    // blocks that need a visit and walks only those: skipped blocks cost one parallel load
+    // instead of a chain of dependent ones.
+    const int wg_wave = blockIdx.x * (kThreads / 64) + wave;
+    for (long it = 0; wg_wave + (long)waves_total * 64 * it < nblk; ++it) {
+        const long bl = wg_wave + (long)waves_total * (lane + 64 * it);
+        int my_code = bl < nblk ? list[bl] : -1;
+        if ((unsigned)my_code >= (unsigned)ncode) my_code = -1;
+        const int my_touch = (my_code >= 0 && G.block_touch) ? G.block_touch[my_code] : 1;
+        unsigned long long todo = __ballot(my_code >= 0 && (my_touch & 3) != 0);
+    while (todo) {
+        const int l = __ffsll((long long)todo) - 1;
+        todo &= todo - 1;
+        const int code = __shfl(my_code, l, 64);
         const int scene = code / bps;
         const mpm_collider_t* __restrict__ cs = col + (size_t)scene * ncoll;
         int bi, bj, bk;
@@ -233,6 +245,16 @@ __global__ __launch_bounds__(kThreads) void k_grid_update(mpm_grid_t G, const mp
         const size_t o = 4 * ((size_t)code * 64 + lane);
         const bool valid = gi < G.n[0] && gj < G.n[1] && gk < G.n[2];
         const bool inbox = valid && gi >= blo0 && gi < bhi0 && gj >= blo1 && gj < bhi1 && gk >= blo2 && gk < bhi2;
+        if (G.block_touch) {  // wavefront-uniform: one block per wavefront
+            const int touch = __shfl(my_touch, l, 64);
+            if (!(touch & 1)) {  // nothing scattered here: accumulators are zero
+                if (touch & 2) {  // velocities written last time go back to zero
+                    if (inbox) *reinterpret_cast<float4*>(G.node_out + o) = make_float4(0.f, 0.f, 0.f, 0.f);
+                    if (lane == 0) G.block_touch[code] = 0;
+                }
+                continue;
+            }
+        }
         float4 in4 = make_float4(0.f, 0.f, 0.f, 0.f);
         if (valid && fixed) {  // deterministic scatter: fixed point -> fp32, then clear
             longlong2* f = reinterpret_cast<longlong2*>(G.node_fix + o);
@@ -290,6 +312,8 @@ __global__ __launch_bounds__(kThreads) void k_grid_update(mpm_grid_t G, const mp
         if (valid && !fixed && (flags & MPM_GRID_ZERO_IN) &&
             (in4.x != 0.0f || in4.y != 0.0f || in4.z != 0.0f || in4.w != 0.0f))  // empty: already clear
             *reinterpret_cast<float4*>(G.node_in + o) = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (G.block_touch && lane == 0) G.block_touch[code] = 2;
+    }
     }
     if (nslots > 0 && lds_w) {
         __syncthreads();

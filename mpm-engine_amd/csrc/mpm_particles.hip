@@ -155,6 +155,7 @@ MPM_DEV void scatter_global(const mpm_grid_t& G, size_t soff, const Stencil& st,
                             const m33& Cm, const m33& fs, Mask mask, int& err) {
     float* __restrict__ gin = G.node_in + 4 * soff;
     unsigned long long* __restrict__ gfix = reinterpret_cast<unsigned long long*>(G.node_fix) + 4 * soff;
+    size_t last_blk = ~(size_t)0;  // block this particle marked last
 #pragma unroll 1
     for (int n = 0; n < 27; ++n) {
         const int i = n / 9, j = (n / 3) % 3, k = n % 3;
@@ -171,6 +172,13 @@ MPM_DEV void scatter_global(const mpm_grid_t& G, size_t soff, const Stencil& st,
         float wmv = wm;
         if (!mask(i, j, k, add, wmv)) continue;
         const size_t o = 4 * (size_t)node_index(G, ix, iy, iz);
+        if (G.block_touch) {
+            const size_t blk = (soff + o / 4) >> 6;
+            if (blk != last_blk) {  // mark as in mark_tile_blocks: a plain store
+                last_blk = blk;
+                G.block_touch[blk] = 3;
+            }
+        }
         if constexpr (kDet) {
             const float Sm = G.fix_scale[0], Sp = G.fix_scale[1];
             atomicAdd(gfix + o + 0, fix_q(add.x, Sp, err));
@@ -563,6 +571,7 @@ __global__ __launch_bounds__(kT) __attribute__((amdgpu_waves_per_eu(MPM_PASS_WAV
                         atomicAdd(gin + o + 3, a3);
                     }
                 });
+                mark_tile_blocks<kT>(G, ts);
             }
             if constexpr (kCdf) {
                 if (threadIdx.x < 6 * MPM_MAX_CDF && s_react[threadIdx.x] != 0.f)

@@ -116,6 +116,32 @@ MPM_DEV void tile_for(const Tile& t, F f) {
     }
 }
 
+// Mark every grid block the tile's box overlaps (clamped to the grid) as scattered into,
+// by the tile's participants: a plain store of 3 (bit 0 = scattered; bit 1 does not matter
+// once bit 0 is set, the grid update rewrites the block and leaves it at 2), so concurrent
+// markers need no atomics and nothing waits on them; over-marking only costs the grid
+// update some work
+template <int kT>
+MPM_DEV void mark_tile_blocks(const mpm_grid_t& G, const Tile& t) {
+    if (!G.block_touch || !t.lds || t.vol == 0) return;
+    const int dims[3] = {t.vol / (t.d1 * t.d2), t.d1, t.d2};
+    int b0[3], nbk[3];
+    for (int a = 0; a < 3; ++a) {
+        const int lo = max(t.lo[a], 0), hi = min(t.lo[a] + dims[a], G.n[a]) - 1;
+        b0[a] = lo >> 2;
+        nbk[a] = hi >= lo ? (hi >> 2) - b0[a] + 1 : 0;
+    }
+    const int total = nbk[0] * nbk[1] * nbk[2];
+    const size_t boff = t.soff >> 6;
+    const int start = t.wg ? (int)threadIdx.x : (int)(threadIdx.x & 63);
+    const int step = t.wg ? kT : 64;
+    for (int n = start; n < total; n += step) {
+        const int k = n % nbk[2], r = n / nbk[2];
+        const int j = r % nbk[1], i = r / nbk[1];
+        G.block_touch[boff + ((size_t)(b0[0] + i) * G.nb[1] + (b0[1] + j)) * G.nb[2] + (b0[2] + k)] = 3;
+    }
+}
+
 MPM_DEV bool in_grid(const mpm_grid_t& G, int i, int j, int k) {
     return (unsigned)i < (unsigned)G.n[0] && (unsigned)j < (unsigned)G.n[1] && (unsigned)k < (unsigned)G.n[2];
 }

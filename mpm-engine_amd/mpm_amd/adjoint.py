@@ -159,6 +159,7 @@ class Rollout:
         G = native.Grid()
         C.memmove(C.byref(G), C.byref(self.sim.G), C.sizeof(G))
         G.node_in, G.node_out = native.ptr(node_in), native.ptr(node_out)
+        G.block_touch = None  # the simulator's block state belongs to its own grid
         return G
 
     def _particles(self, state_slot, post_slot, l_slot=None):

@@ -73,7 +73,7 @@ class Grid(C.Structure):
     _fields_ = [("node_in", _FP), ("node_out", _FP), ("n", C.c_int32 * 3),
                 ("nb", C.c_int32 * 3), ("dx", C.c_float), ("inv_dx", C.c_float),
                 ("error_flag", _FP), ("n_scenes", C.c_int32), ("pad_", C.c_int32),
-                ("cdf", Cdf), ("node_fix", _FP), ("fix_scale", C.c_float * 2)]
+                ("cdf", Cdf), ("node_fix", _FP), ("fix_scale", C.c_float * 2), ("block_touch", _FP)]
 
 
 _L3 = (np.float32, (MAX_CDF, 3))

@@ -131,6 +131,8 @@ class HipSimulator:
         self.n_blocks = self.blocks_per_scene * self.n_scenes
         self.node_in = t(self.n_blocks * 64 * 4)
         self.node_out = t(self.n_blocks * 64 * 4)
+        # per-block scatter / write state: the grid update skips blocks nothing reached
+        self.block_touch = t(self.n_blocks, dt=torch.int32)
         self.error_flag = t(1, dt=torch.int32)
         self.bounds = t(8)
         # block lists and device tick control
@@ -230,6 +232,7 @@ class HipSimulator:
         if self.cdf is not None:
             self.cdf.fill(G)
         G.node_fix = native.ptr(self.node_fix)
+        G.block_touch = native.ptr(self.block_touch)
         if self._fix_exp is not None:
             G.fix_scale[0] = math.ldexp(1.0, self._fix_exp[0])
             G.fix_scale[1] = math.ldexp(1.0, self._fix_exp[1])
