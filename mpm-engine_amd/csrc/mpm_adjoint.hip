@@ -16,12 +16,16 @@
 // Return-map adjoints (von Mises, material 1, and local mu(I), material 9, both with
 // Hencky elasticity): F_trial = U S V^T; the stress tau = U diag(h) U^T and the
 // corrected F = U diag(g) V^T are isotropic functions of F_trial, and both maps scale
-// the deviatoric log-strain by one factor k, so they share the assembly below. Their derivatives are assembled in the SVD frame from the
-// diagonal Jacobians dh/dsigma, dg/dsigma and, per pair (a, b), the divided differences
+// the deviatoric log-strain by one factor k, so they share the assembly below. Their
+// derivatives are assembled in the SVD frame from the diagonal Jacobians dh/dsigma, dg/dsigma and, per pair (a, b), the divided differences
 // (h_a - h_b)/(s_a - s_b) and (g_a - g_b)/(s_a - s_b) plus the sums over (s_a + s_b).
 // The divided differences are evaluated through log-ratios with expm1 and a series
 // below |log(s_a/s_b)| < 1e-3, so repeated or nearly repeated singular values (F near a
 // rotation, the undeformed state) give the finite limit instead of 1/(s_a^2 - s_b^2).
+// Fixed-corotated elasticity (material 0) has no return map; its stress adjoint goes
+// through the left stretch U S U^T, whose SVD-frame derivative has only the sums
+// (s_a + s_b) in its denominators. The generalized-Newtonian fluid (material 10) adds
+// the adjoint of the velocity gradient L, which flows to the previous substep's G2P.
 #include <hip/hip_runtime.h>
 
 #include "mpm_device.h"
@@ -397,7 +401,12 @@ __global__ __launch_bounds__(kThreads) void k_adj_p2g_stress(mpm_particles_t P, 
     const m33 Fb = ld9(A.F, S, p);
     m33 Ftb;
     m33 Lb = mzero();
-    if (mat == 1 && md->hardening[mat] <= 0.5f) {
+    if (mat == 0) {
+        float mub = 0.f, lamb = 0.f;
+        Ftb = adj_fcr(ld9(P.F_trial, S, p), P.mu[p], P.lam[p], Fb, tb, mub, lamb);
+        A.mu[p] += mub;
+        A.lam[p] += lamb;
+    } else if (mat == 1 && md->hardening[mat] <= 0.5f) {
         float mub = 0.f, lamb = 0.f, yb = 0.f;
         Ftb = adj_von_mises(ld9(P.F_trial, S, p), P.mu[p], P.lam[p], P.yield_stress[p], Fb, tb, mub, lamb, yb);
         A.mu[p] += mub;

@@ -1,8 +1,9 @@
 // Adjoints of the constitutive updates (return maps + Kirchhoff stress) used by the
-// reverse substep (mpm_adjoint.hip): von Mises (material 1), local mu(I) (material 9)
-// and the generalized-Newtonian fluid (material 10). Pure register math on the types of
-// mpm_device.h; a host-side check compiles it with MPM_DEV = __host__ __device__ and
-// compares it with float64 finite differences of the forward maps.
+// reverse substep (mpm_adjoint.hip): fixed-corotated elasticity (material 0), von Mises
+// (material 1), local mu(I) (material 9) and the generalized-Newtonian fluid (material
+// 10). Pure register math on the types of mpm_device.h; a host-side check compiles it
+// with MPM_DEV = __host__ __device__ and compares it with float64 finite differences of
+// the forward maps.
 #pragma once
 
 #include "mpm_device.h"
@@ -226,6 +227,39 @@ MPM_DEV m33 mcof(const m33& m) {
     c.a[7] = a[2] * a[3] - a[0] * a[5];
     c.a[8] = a[0] * a[4] - a[1] * a[3];
     return c;
+}
+
+// Adjoint of the fixed-corotated stress (mat 0, kirchoff_stress_FCR, mpm_utils.py:8-15, with
+// F = F_trial, mpm_utils.py:1094-1095). With F = U diag(s) V^T and R = U V^T,
+//   tau = 2 mu (F - R) F^T + lam J (J - 1) I = 2 mu (F F^T - P) + lam J (J - 1) I,
+// where P = R F^T = U diag(s) U^T is the left stretch. In the SVD frame (dF^ = U^T dF V)
+// dP^_ab = (s_b dF^_ab + s_a dF^_ba) / (s_a + s_b), which holds for the signed
+// decomposition too. In: Fb = adjoint of F, tb = adjoint of tau. Returns the adjoint of
+// F_trial; the mu and lam partials are added.
+MPM_DEV m33 adj_fcr(const m33& F, float mu, float lam, const m33& Fb, const m33& tb_in, float& mub, float& lamb) {
+    m33 U, V;
+    v3 s3;
+    svd3(F, U, s3, V);
+    const float s[3] = {s3.x, s3.y, s3.z};
+    const m33 T = mscale(madd(tb_in, mT(tb_in)), 0.5f);  // the forward symmetrises tau
+    const float J = mdet(F);
+    const float trT = T.a[0] + T.a[4] + T.a[8];
+    const m33 Th = mmul(mT(U), mmul(T, U));
+    m33 Ph;  // adjoint of dF^ through P
+    for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b) {
+            const float ssum = s[a] + s[b];
+            Ph.a[3 * a + b] = a == b ? Th.a[4 * a] : (fabsf(ssum) > 1e-12f ? 2.0f * Th.a[3 * a + b] * s[b] / ssum : 0.0f);
+        }
+    const m33 FFt = mmul(F, mT(F));
+    const m33 P = mmul(U, mmul(mdiag(s[0], s[1], s[2]), mT(U)));
+    float tp = 0.0f;
+    for (int i = 0; i < 9; ++i) tp += T.a[i] * (FFt.a[i] - P.a[i]);
+    mub += 2.0f * tp;
+    lamb += trT * J * (J - 1.0f);
+    m33 Ftb = madd(Fb, mscale(mmul(T, F), 4.0f * mu));
+    Ftb = msub(Ftb, mscale(mmul(U, mmul(Ph, mT(V))), 2.0f * mu));
+    return madd(Ftb, mscale(mcof(F), lam * (2.0f * J - 1.0f) * trT));
 }
 
 // Adjoint of the generalized-Newtonian fluid (mat 10, stress_update_particle's viscous

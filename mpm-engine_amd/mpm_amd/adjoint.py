@@ -9,7 +9,8 @@ mu(I) coefficients (mu_s, delta_mu, I0, grain diameter of material 9). It return
 final positions and the accumulated collider wrench, so a loss on the final state and on
 the plate reaction forces back-propagates to Young's modulus, Poisson's ratio, the yield
 stress (through the per-particle parameters), the fluid's rheology, the granular
-friction law and to the initial particle state.
+friction law and to the initial particle state. Materials: fixed-corotated jelly (0),
+von Mises (1), local mu(I) (9) and the generalized-Newtonian fluid (10).
 
 Forward: fused ticks of `checkpoint_every` substeps (the production substep), storing
 the particle state (x, v, C, F_trial and the velocity gradient L, which the viscous
@@ -50,7 +51,7 @@ def lame(E, nu):
 class Rollout:
     """Differentiable rollout of `substeps` substeps of dt on a configured Solver."""
 
-    SUPPORTED_MATERIALS = frozenset((1, GRANULAR_ID, FLUID_ID))
+    SUPPORTED_MATERIALS = frozenset((0, 1, GRANULAR_ID, FLUID_ID))
     SUPPORTED_COLLIDERS = frozenset((native.COLLIDER_PLANE, native.COLLIDER_BOX, native.COLLIDER_WALLS,
                                      native.COLLIDER_POINTCLOUD))
 

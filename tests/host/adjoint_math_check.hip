@@ -37,7 +37,9 @@ int main() {
         float mub = 0.f, lamb = 0.f, yb = 0.f, eb = 0.f, bb = 0.f, kb = 0.f, nb = 0.f;
         float pb[4] = {0.f, 0.f, 0.f, 0.f};
         m33 Lb = mzero(), Ftb = mzero();
-        if (mat == 1) {
+        if (mat == 0) {
+            Ftb = adj_fcr(Ft, mu, lam, Fb, tb, mub, lamb);
+        } else if (mat == 1) {
             md.hardening[mat] = 0.0f;
             Ftb = adj_von_mises(Ft, mu, lam, ys, Fb, tb, mub, lamb, yb);
         } else if (mat == 9) {

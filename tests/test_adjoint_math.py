@@ -1,15 +1,16 @@
-"""The constitutive adjoints of the reverse substep (csrc/mpm_adjoint_math.h: von Mises,
-local mu(I), generalized-Newtonian fluid) against float64 central differences of the
-forward maps, on the CPU.
+"""The constitutive adjoints of the reverse substep (csrc/mpm_adjoint_math.h: fixed-corotated
+elasticity, von Mises, local mu(I), generalized-Newtonian fluid) against float64 central
+differences of the forward maps, on the CPU.
 
 The header is compiled for the host (tests/host/adjoint_math_check.hip; no HIP call, no
 GPU) and fed random states; each forward map is restated here in float64 from the
-reference (mpm_utils.py: von_mises_return_mapping :128-170, mu_i_return_mapping
+reference (mpm_utils.py: kirchoff_stress_FCR :8-15, von_mises_return_mapping :128-170, mu_i_return_mapping
 :247-324, kirchoff_stress_newtonian :28-55 with stress_update_particle's fluid branch
 :1057-1149, Hencky stress :228-243). The checked quantity is the adjoint of
 J = <F, Fb> + <tau, tb> with respect to F_trial, L and every parameter, for states on both
-sides of each yield surface (kept 2 % away from it), in tension, and with repeated or
-nearly repeated singular values (the divided-difference guard). The GPU tests check the
+sides of each yield surface (kept 2 % away from it), in tension, with repeated or nearly
+repeated singular values (the divided-difference guard), and, for fixed-corotated
+elasticity, large stretches and an inverted F (a negative singular value). The GPU tests check the
 same kernels end to end against differences of whole rollouts; this test pins the
 formulas to float32 rounding (F_trial and L adjoints within 2e-4 of the largest entry,
 parameter derivatives within 1e-3 of the stress term they scale)."""
@@ -38,6 +39,17 @@ def _hencky(F, mu, lam):
     e = np.log(np.maximum(s, 1e-6))
     t = U @ np.diag(2 * mu * e + lam * e.sum()) @ U.T
     return 0.5 * (t + t.T)
+
+
+def _fcr(F, mu, lam):
+    # wp.svd3's decomposition has rotations U and V and gives the smallest singular value
+    # the sign of det F, so R = U V^T is a rotation; numpy's keeps s >= 0 and puts the sign
+    # into U V^T, which the last column's flip takes back out
+    U, s, V = _svd(F)
+    R = U @ np.diag([1.0, 1.0, np.linalg.det(U @ V.T)]) @ V.T
+    J = np.linalg.det(F)
+    t = 2 * mu * (F - R) @ F.T + lam * J * (J - 1) * np.eye(3)
+    return F, 0.5 * (t + t.T)
 
 
 def _von_mises(Ft, mu, lam, ys):
@@ -95,7 +107,7 @@ def _viscous(Ft, L, eta, bulk, ty, K, n):
 
 # ---------------------------------------------------------------- cases
 _NAMES = ("mu", "lam", "ys", "eta", "bulk", "K", "n", "mu_s", "dmu", "I0", "d")
-_PARAMS = {1: ("mu", "lam", "ys"), 9: ("mu", "lam", "mu_s", "dmu", "I0", "d"),
+_PARAMS = {0: ("mu", "lam"), 1: ("mu", "lam", "ys"), 9: ("mu", "lam", "mu_s", "dmu", "I0", "d"),
            10: ("eta", "bulk", "ys", "K", "n")}
 
 
@@ -108,6 +120,19 @@ def _cases(rng):
     out = []
     base = dict(dt=1e-4, mu=0.0, lam=0.0, ys=0.0, eta=0.0, bulk=0.0, K=0.0, n=1.0, mu_s=0.0, dmu=0.0, I0=1.0,
                 d=0.0, rho_s=0.0)
+    # fixed-corotated: E 1e5, nu 0.3; small and large stretches, nearly repeated singular
+    # values, an inverted F (det F < 0)
+    mu, lam = 1e5 / 2.6, 1e5 * 0.3 / (1.3 * 0.4)
+    for kind in ("small", "large", "large", "repeated", "inverted"):
+        if kind == "small":
+            s = np.exp(rng.uniform(-0.02, 0.02, 3))
+        elif kind == "large":
+            s = np.exp(rng.uniform(-0.4, 0.4, 3))
+        elif kind == "repeated":
+            s = np.array([1.1, 1.1 + 1e-6, 0.9])
+        else:
+            s = np.array([1.2, 0.9, -0.3])
+        out.append(dict(base, mat=0, mu=mu, lam=lam, Ft=_rot(rng) @ np.diag(s) @ _rot(rng).T))
     # von Mises: E 2e5, nu 0.3, yield 1e3; elastic, plastic, near-repeated singular values
     mu, lam = 2e5 / 2.6, 2e5 * 0.3 / (1.3 * 0.4)
     for kind in ("elastic", "plastic", "plastic", "repeated", "repeated_plastic", "rotation"):
@@ -157,7 +182,9 @@ def _cases(rng):
 
 def _objective(c, stress_only=False, **over):
     p = dict(c, **over)
-    if p["mat"] == 1:
+    if p["mat"] == 0:
+        F, t = _fcr(p["Ft"], p["mu"], p["lam"])
+    elif p["mat"] == 1:
         F, t = _von_mises(p["Ft"], p["mu"], p["lam"], p["ys"])
     elif p["mat"] == 9:
         F, t = _mu_i(p["Ft"], p["mu"], p["lam"], p["mu_s"], p["dmu"], p["I0"], p["d"], p["rho_s"], p["dt"])

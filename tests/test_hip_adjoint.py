@@ -26,10 +26,10 @@ def _scene(plate_speed=0.05):
     return SqueezeScene(n_grid=32, per_side=16, dt=DT, plate_speed=plate_speed)
 
 
-def _oracle(sc, mu, lam, ys, v0, x0=None, C0=None, Ft0=None):
+def _oracle(sc, mu, lam, ys, v0, x0=None, C0=None, Ft0=None, material=None):
     o = OracleSolver(sc.grid)
     o.load_particles(sc.pos if x0 is None else x0, sc.vol)
-    o.set_material(sc.material)
+    o.set_material(sc.material if material is None else material)
     o.add_plane((0, 0, sc.floor), (0, 0, 1), "sticky")
     plates = [o.add_box(c, sc.plate_half, v) for c, v in zip(sc.plate_centers, sc.plate_velocity)]
     o._sim.mu[:] = mu
@@ -43,7 +43,7 @@ def _oracle(sc, mu, lam, ys, v0, x0=None, C0=None, Ft0=None):
     return o, plates
 
 
-def _oracle_rollout(sc, substeps, mu, lam, ys, v0, **kw):
+def _oracle_rollout(sc, substeps, mu, lam, ys, v0, **kw):  # kw: x0, C0, Ft0, material
     o, plates = _oracle(sc, mu, lam, ys, v0, **kw)
     o.step(DT, substeps)
     imp = np.stack([o.tool_force(h, 1.0) for h in plates])
@@ -241,14 +241,144 @@ def test_one_substep_material_directions():
 
 
 def test_unsupported_material_is_refused():
-    from mpm_amd.materials import elastic
-
     sc = _scene()
     s = Solver(grid=sc.grid, device="hip")
     s.load_particles(sc.pos, sc.vol)
-    s.set_material(elastic(E=1.0e5))
+    s.set_material("plasticine", E=1.0e5, nu=0.3, density=1000.0)
     with pytest.raises(NotImplementedError):
         Rollout(s, DT, 10)
+
+
+def _jelly_solver(sc):
+    from mpm_amd.materials import elastic
+
+    s = sc.build(Solver(grid=sc.grid, device="hip"))
+    s.set_material(elastic(E=2.0e5, nu=0.3, density=1000.0))
+    return s, elastic(E=2.0e5, nu=0.3, density=1000.0)
+
+
+def test_jelly_rollout_gradients_match_central_differences():
+    """Fixed-corotated jelly (material 0, the reference's default material) in the same
+    squeeze: 30 substeps with checkpoints every 10, plates closing at 0.5 m/s, loss
+    linearised at the GPU trajectory as in the von Mises rollout test, against a target
+    rollout at E x 1.1. Checked: d loss / d (E, nu) against central differences
+    (eps = 3e-3 relative) of the oracle rollout, within 1e-3.
+
+    An elastic block keeps its waves and its contact switches against the plates (plastic
+    flow damps both in the von Mises scene), which makes the fp32 rollout only piecewise
+    smooth: over 30 substeps the oracle's own differences at eps = 1e-3, 3e-3 and 1e-2
+    agree within 1.2e-4 (E) and 4.5e-4 (nu); over 60 substeps they scatter by 0.4 %, and
+    along a velocity direction by ~1 % at any length, so velocity directions are checked
+    over a few substeps in the next test."""
+    import torch
+
+    sc = _scene(plate_speed=0.5)
+    s, jelly = _jelly_solver(sc)
+    n = s.n_particles
+    S = 30
+    ro = Rollout(s, DT, S, checkpoint_every=10)
+    dev = s._sim.device
+    E0, nu0, y0 = 2.0e5, 0.3, 1.0e3
+    f64 = dict(dtype=torch.float64, device=dev)
+    with torch.no_grad():
+        xt, wt = ro.run(torch.tensor(1.1 * E0, **f64), torch.tensor(nu0, **f64), torch.tensor(y0, **f64),
+                        torch.zeros(n, 3, device=dev))
+    x_star, w_star = xt.double(), wt[:, :3].double()
+    E = torch.tensor(E0, requires_grad=True, **f64)
+    nu = torch.tensor(nu0, requires_grad=True, **f64)
+    xT, w = ro.run(E, nu, torch.tensor(y0, **f64), torch.zeros(n, 3, device=dev))
+    loss = ((xT.double() - x_star) ** 2).sum() + ((w[:, :3].double() - w_star) ** 2).sum()
+    loss.backward()
+    gx = (2.0 * (xT.double() - x_star)).detach().cpu().numpy()
+    gw = (2.0 * (w[:, :3].double() - w_star)).detach().cpu().numpy()
+
+    def linearised(E_, nu_):
+        mu, lam = _lame(E_, nu_)
+        x, imp = _oracle_rollout(sc, S, np.float32(mu), np.float32(lam), np.float32(y0), np.zeros((n, 3), np.float32),
+                                 material=jelly)
+        return (x * gx).sum() + (imp * gw).sum()
+
+    report = {}
+    for i, (name, grad) in enumerate((("E", E), ("nu", nu))):
+        base = [E0, nu0]
+        h = 3e-3 * base[i]
+        p, m = list(base), list(base)
+        p[i] += h
+        m[i] -= h
+        report[name] = (grad.grad.item(), (linearised(*p) - linearised(*m)) / (2 * h))
+    print("loss", loss.item(), report)
+    for k, (a, f) in report.items():
+        assert abs(a - f) <= 1e-3 * abs(f) + 1e-12, f"{k}: adjoint {a:.6e} vs central difference {f:.6e}"
+
+
+def test_jelly_few_substep_directions():
+    """Three substeps of jelly (checkpoints every 2) from a strongly deformed state
+    (stretches up to 30 %, 2 % of the particles inverted, random v and affine C):
+    directional derivatives of a random functional of x_T along x0, and of the normal
+    plate impulses along x0, v0 and per-particle mu and lam, against central differences
+    of the oracle, within 2e-3. From the second substep on the stress reads an F_trial
+    that the velocity field has moved, so the v0, mu and lam derivatives pass through
+    the fixed-corotated F_trial adjoint at large and inverted strains.
+
+    Steps, where the oracle's own differences level off: x0 directions 1e-3 dx per
+    component (from 3e-4 to 1e-3 dx they agree within 6e-4; below 1e-4 dx rounding takes
+    over, 3 % at 1e-5 dx); v0 1e-2 m/s (1e-2 and 3e-2 agree within 2e-4); mu and lam 5e-2
+    relative (within 5e-5 from 1e-2 to 1e-1). A velocity direction moves positions by less than fp32 resolves within three
+    substeps, so it is checked through the impulses."""
+    import torch
+
+    sc = _scene()
+    s, jelly = _jelly_solver(sc)
+    n = s.n_particles
+    rng = np.random.default_rng(11)
+    v0 = rng.uniform(-0.2, 0.2, (n, 3)).astype(np.float32)
+    C0 = rng.uniform(-2.0, 2.0, (n, 3, 3)).astype(np.float32)
+    Ft0 = (np.eye(3) + 0.3 * rng.uniform(0, 1, (n, 1, 1)) * rng.uniform(-1, 1, (n, 3, 3))).astype(np.float32)
+    flip = rng.uniform(0, 1, n) < 0.02
+    Ft0[flip, :, 2] *= -0.5  # det F < 0: the signed decomposition's branch
+    sim = s._sim
+    sim.C.copy_(torch.from_numpy(C0.reshape(n, 9).T.copy()))
+    sim.F_trial.copy_(torch.from_numpy(Ft0.reshape(n, 9).T.copy()))
+    S = 3
+    ro = Rollout(s, DT, S, checkpoint_every=2)
+    dev = sim.device
+    mu0, lam0 = (np.float32(v) for v in _lame(2.0e5, 0.3))
+    y0 = np.float32(1.0e3)
+    cx = rng.standard_normal((n, 3))
+    cw = np.zeros((2, 3))
+    cw[:, 0] = rng.standard_normal(2)
+    mu = torch.full((n,), float(mu0), device=dev, requires_grad=True)
+    lam = torch.full((n,), float(lam0), device=dev, requires_grad=True)
+    v = torch.from_numpy(v0).to(dev).requires_grad_(True)
+    x0 = torch.from_numpy(sc.pos.copy()).to(dev).requires_grad_(True)
+    xT, w = ro(mu, lam, torch.full((n,), float(y0), device=dev), v, x0)
+    lx = (xT.double() * torch.from_numpy(cx).to(dev)).sum()
+    lw = (w[:, :3].double() * torch.from_numpy(cw).to(dev)).sum()
+    gx = dict(zip(("x0", "v0"), torch.autograd.grad(lx, [x0, v], retain_graph=True)))
+    gw = dict(zip(("mu", "lam", "x0", "v0"), torch.autograd.grad(lw, [mu, lam, x0, v])))
+
+    def oracle_loss(args, which):
+        x, imp = _oracle_rollout(sc, S, args["mu"], args["lam"], y0, args["v0"], x0=args["x0"], C0=C0, Ft0=Ft0,
+                                 material=jelly)
+        return (x * cx).sum() if which == "x" else (imp * cw).sum()
+
+    dx = sc.grid.dx
+    base = {"mu": np.full(n, mu0, np.float32), "lam": np.full(n, lam0, np.float32), "v0": v0,
+            "x0": sc.pos.astype(np.float32)}
+    dirs = {"x0": (rng.choice([-1.0, 1.0], (n, 3)) * 1e-3 * dx).astype(np.float32),
+            "v0": (rng.choice([-1.0, 1.0], (n, 3)) * 1e-2).astype(np.float32),
+            "mu": (rng.uniform(0.5, 1.5, n) * 5e-2 * mu0).astype(np.float32),
+            "lam": (rng.uniform(0.5, 1.5, n) * 5e-2 * lam0).astype(np.float32)}
+    for which, name, grads in (("x", "x0", gx), ("w", "x0", gw), ("w", "v0", gw), ("w", "mu", gw),
+                               ("w", "lam", gw)):
+        d = dirs[name]
+        ad = float((grads[name].double().cpu().numpy().reshape(d.shape) * d).sum())
+        args_p, args_m = dict(base), dict(base)
+        args_p[name] = base[name] + d
+        args_m[name] = base[name] - d
+        fd = (oracle_loss(args_p, which) - oracle_loss(args_m, which)) / 2.0
+        print(which, name, ad, fd)
+        assert abs(ad - fd) <= 2e-3 * abs(fd), f"{which} {name}: adjoint {ad:.6e} vs central difference {fd:.6e}"
 
 
 def _fluid(eta, bulk, ty, K, n):
