@@ -152,7 +152,7 @@ typedef struct mpm_grid {
                              2 = a non-finite particle position, 4 = a solid material
                              met a MPM_PASS_VISCOUS_ONLY pass, 8 = a scene id outside
                              [0, n_scenes) (that particle is skipped), 64 = a fixed-point
-                             contribution beyond 2^52 units (deterministic scatter: the
+                             contribution beyond 2^51 units (deterministic scatter: the
                              scale is too fine for this state) */
     int32_t n_scenes;     /* independent scenes sharing the buffers (>= 1): scene s owns
                              blocks [s * B, (s + 1) * B), B = nb[0] * nb[1] * nb[2];

@@ -138,16 +138,21 @@ MPM_DEV void gather_global(const mpm_grid_t& G, size_t soff, const Stencil& st, 
 }
 
 // Deterministic scatter: x in fixed point of scale S (a power of two, so x * S is exact
-// in fp32), rounded to the nearest unit. Integer sums do not depend on the order the
-// atomics land in. Beyond 2^52 units (or NaN) the contribution is dropped and error bit
-// 64 raised: 2^11 such contributions to one node still fit an int64.
+// in fp32), rounded to the nearest unit (ties to even). Integer sums do not depend on the
+// order the atomics land in. Beyond 2^51 units (or NaN) the contribution is dropped and
+// error bit 64 raised: 2^12 such contributions to one node still fit an int64.
+// The rounding goes through float64: y is exact there, and y + 1.5 * 2^52 lies in the
+// binade [2^52, 2^53), whose spacing is 1, so the add rounds y to an integer and the
+// difference of the two bit patterns is that integer (for |y| < 2^51). Four instructions
+// instead of the generic float -> int64 conversion, 108 times per particle.
 MPM_DEV unsigned long long fix_q(float x, float S, int& err) {
     const float y = x * S;
-    if (!(fabsf(y) < 4503599627370496.0f)) {
+    if (!(fabsf(y) < 2251799813685248.0f)) {
         err |= 64;
         return 0ull;
     }
-    return (unsigned long long)__float2ll_rn(y);
+    const double magic = 6755399441055744.0;  // 1.5 * 2^52
+    return (unsigned long long)(__double_as_longlong((double)y + magic) - __double_as_longlong(magic));
 }
 
 template <bool kDet, class Mask>

@@ -578,7 +578,7 @@ class HipSimulator:
 
     def _fix_scales(self, dt: float):
         """Powers of two such that the largest expected contribution is about 2^40 units
-        (2^12 below the per-contribution limit): mass by the heaviest particle, momentum by
+        (2^11 below the per-contribution limit): mass by the heaviest particle, momentum by
         it times 2 dx/dt (a stable explicit substep moves nothing a cell; the scale then
         depends on dt and the masses only, so captured graphs stay valid tick to tick)."""
         if self._m_max is None:

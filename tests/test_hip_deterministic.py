@@ -102,7 +102,7 @@ def test_sparse_deterministic_is_bit_identical_and_equals_dense():
 
 def test_range_overflow_is_reported():
     """A scale far too fine for the state (forced from the host) drops contributions
-    beyond 2^52 units and raises at the next guard instead of wrapping silently."""
+    beyond 2^51 units and raises at the next guard instead of wrapping silently."""
     s = _c1(Solver, device="hip", deterministic=True)
     s.step(1e-4, 2)
     sim = s._sim
