@@ -255,8 +255,8 @@ constexpr int kXcds = 8;
 
 // amdgpu_waves_per_eu(3): keep the allocation within 168 VGPRs (3 wavefronts per SIMD; left
 // alone the scheduler lands anywhere between 110 and 200 depending on unrelated edits). At
-// 128 VGPRs (4 per SIMD) the fixed-point scatter spills ~21 dwords and runs 5-6 % slower on
-// the pour; with 3 nothing spills and the extra wavefront is not missed.
+// 128 VGPRs (4 per SIMD) the fixed-point lean pass fits without spilling but runs ~4 %
+// slower on the pour: the extra wavefront is not missed.
 #ifndef MPM_PASS_WAVES_PER_EU
 #define MPM_PASS_WAVES_PER_EU 3
 #endif
