@@ -931,6 +931,92 @@ class HipSimulator:
                                                 self.stream()), "grid_to_dense")
         return m, vi, vo
 
+    # ------------------------------------------- torch imports / exports (the simulator's API)
+    # MPM_Simulator_WARP's import_*_from_torch / export_*_to_torch (mpm_solver_warp.py:
+    # 1638-1700, 1785-1860), same names, shapes and in-place copy semantics; `clone` and
+    # `device` are accepted for signature compatibility (imports always copy).
+    def _import_rows(self, dst, tensor, width, name):
+        t = tensor.detach().to(device=self.device, dtype=dst.dtype).reshape(-1, width)
+        if t.shape[0] != self.n_particles:
+            raise ValueError(f"import_particle_{name}: {t.shape[0]} rows for {self.n_particles} particles")
+        dst.copy_(t.t() if dst.dim() == 2 else t.reshape(-1))
+
+    def import_particle_x_from_torch(self, tensor_x, clone=True, device="cuda:0"):
+        if tensor_x is not None:
+            self._import_rows(self.x, tensor_x, 3, "x")
+
+    def import_particle_v_from_torch(self, tensor_v, clone=True, device="cuda:0"):
+        if tensor_v is not None:
+            self._import_rows(self.v, tensor_v, 3, "v")
+
+    def import_particle_F_from_torch(self, tensor_F, clone=True, device="cuda:0"):
+        if tensor_F is not None:  # (n, 3, 3) or (n, 9), row-major
+            self._import_rows(self.F, tensor_F, 9, "F")
+
+    def import_particle_C_from_torch(self, tensor_C, clone=True, device="cuda:0"):
+        if tensor_C is not None:
+            self._import_rows(self.C, tensor_C, 9, "C")
+
+    def import_particle_selection_from_torch(self, tensor_selection, clone=True, device="cuda:0"):
+        """Nonzero entries leave the particle out of every transfer (as the reference)."""
+        if tensor_selection is not None:
+            self._import_rows(self.selection, tensor_selection, 1, "selection")
+
+    def import_particle_material_from_torch(self, tensor_material, clone=True, device="cuda:0"):
+        if tensor_material is not None:
+            self._import_rows(self.material, tensor_material, 1, "material")
+            # the ids present select the particle-pass specialisation
+            self._mat_ids = {int(m) for m in self.torch.unique(self.material).cpu().tolist()}
+
+    def export_particle_material_to_torch(self):
+        return self.material
+
+    def export_particle_selection_to_torch(self):
+        return self.selection
+
+    def export_particle_x_to_torch(self):
+        return self.x.t().contiguous()
+
+    def export_particle_v_to_torch(self):
+        return self.v.t().contiguous()
+
+    def export_particle_vol_to_torch(self):
+        return self.vol
+
+    def export_particle_stress_to_torch(self):
+        return self.stress.t().contiguous()
+
+    def export_particle_F_to_torch(self):
+        return self.F.t().contiguous()
+
+    def export_particle_L_to_torch(self):
+        return self.L.t().contiguous()
+
+    def export_particle_C_to_torch(self):
+        return self.C.t().contiguous()
+
+    def export_particle_R_to_torch(self, device="cuda:0"):
+        """compute_R_from_F's R^T per particle, (n, 9) row-major (as the reference stores it)."""
+        return self.export_R()
+
+    def export_particle_cov_to_torch(self, device="cuda:0"):
+        """Flat (6 n,) upper triangles (xx, xy, xz, yy, yz, zz), as the reference's array."""
+        return self.export_cov().reshape(-1)
+
+    def export_grid_v_out_to_torch(self):
+        return self.grid_dense()[2]
+
+    def export_grid_v_in_to_torch(self):
+        """The last scatter's momentum sums. The grid update consumes (clears) them, so after
+        a completed substep this reads zero, where the reference keeps them until the next
+        substep's zeroing; the deterministic scatter keeps its sums in the fixed-point
+        buffer, so there it is zero throughout."""
+        return self.grid_dense()[1]
+
+    def export_grid_m_to_torch(self):
+        """The last scatter's node masses, with the lifetime of export_grid_v_in_to_torch."""
+        return self.grid_dense()[0]
+
 
 @dataclass
 class Solver:
@@ -967,8 +1053,13 @@ class Solver:
     # ------------------------------------------------------------------ setup
     def load_particles(self, pos: np.ndarray, vol: np.ndarray, cov: np.ndarray | None = None,
                        cov_mode: str = "step") -> "Solver":
-        self.device = _resolve_device(self.device)
         pos = np.asarray(pos)
+        if pos.ndim != 2 or pos.shape[1] != 3 or len(pos) == 0:
+            # (the reference fails on an empty set at its first guard, reducing an empty array)
+            raise ValueError(f"load_particles: positions must be a non-empty (n, 3) array, got shape {pos.shape}")
+        if np.asarray(vol).reshape(-1).shape[0] != len(pos):
+            raise ValueError(f"load_particles: {np.asarray(vol).size} volumes for {len(pos)} particles")
+        self.device = _resolve_device(self.device)
         self._vol0 = np.asarray(vol, np.float32).copy()
         cov6 = None
         if cov is not None:

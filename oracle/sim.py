@@ -1509,6 +1509,16 @@ class OracleSolver:
             lo[0], hi[0] = 0.0, lim
         self._sim.grid_launch_box = self._sim._grid_box(lo, hi, halo=0)
 
+    def set_x(self, pos):
+        """solver.py:520-526"""
+        self._sim.x[:] = np.asarray(pos, np.float32)
+        return self
+
+    def set_v(self, vel):
+        """solver.py:528-534"""
+        self._sim.v[:] = np.asarray(vel, np.float32)
+        return self
+
     def x(self):
         return self._sim.x.copy()
 
