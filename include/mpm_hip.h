@@ -471,7 +471,8 @@ int32_t mpm_grid_to_dense(const mpm_grid_t *G, float *m, float *v_in, float *v_o
  * the scatter and node_out = grid velocity after the colliders. On entry `A` holds the
  * adjoint of S_{k+1} = (x, v, C, F_trial); on exit the adjoint of S_k, with the
  * per-particle parameter adjoints accumulated. Single scene; material 0 (fixed-corotated
- * jelly: mu, lam), material 1 (von Mises + Hencky, no hardening: mu, lam, yield_stress), material 9 (local mu(I): mu, lam and
+ * jelly: mu, lam), material 1 (von Mises + Hencky, no hardening: mu, lam, yield_stress),
+ * material 2 (Drucker-Prager sand: mu, lam), material 9 (local mu(I): mu, lam and
  * mu_s, delta_mu, I0, grain diameter as per-particle partials of the per-type
  * coefficients) and material 10 (Newtonian / Bingham / Herschel-Bulkley fluid:
  * viscosity, bulk modulus, K, n as per-particle partials of the per-type parameters, and

@@ -22,7 +22,8 @@
 // The divided differences are evaluated through log-ratios with expm1 and a series
 // below |log(s_a/s_b)| < 1e-3, so repeated or nearly repeated singular values (F near a
 // rotation, the undeformed state) give the finite limit instead of 1/(s_a^2 - s_b^2).
-// Fixed-corotated elasticity (material 0) has no return map; its stress adjoint goes
+// Drucker-Prager sand (material 2) shares it too (k = -c tr / |e - me| on the cone, 0 in
+// tension). Fixed-corotated elasticity (material 0) has no return map; its stress adjoint goes
 // through the left stretch U S U^T, whose SVD-frame derivative has only the sums
 // (s_a + s_b) in its denominators. The generalized-Newtonian fluid (material 10) adds
 // the adjoint of the velocity gradient L, which flows to the previous substep's G2P.
@@ -404,6 +405,11 @@ __global__ __launch_bounds__(kThreads) void k_adj_p2g_stress(mpm_particles_t P, 
     if (mat == 0) {
         float mub = 0.f, lamb = 0.f;
         Ftb = adj_fcr(ld9(P.F_trial, S, p), P.mu[p], P.lam[p], Fb, tb, mub, lamb);
+        A.mu[p] += mub;
+        A.lam[p] += lamb;
+    } else if (mat == 2) {
+        float mub = 0.f, lamb = 0.f;
+        Ftb = adj_sand(ld9(P.F_trial, S, p), P.mu[p], P.lam[p], md->alpha[mat], Fb, tb, mub, lamb);
         A.mu[p] += mub;
         A.lam[p] += lamb;
     } else if (mat == 1 && md->hardening[mat] <= 0.5f) {

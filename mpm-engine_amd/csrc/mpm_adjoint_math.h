@@ -1,7 +1,7 @@
 // Adjoints of the constitutive updates (return maps + Kirchhoff stress) used by the
 // reverse substep (mpm_adjoint.hip): fixed-corotated elasticity (material 0), von Mises
-// (material 1), local mu(I) (material 9) and the generalized-Newtonian fluid (material
-// 10). Pure register math on the types of mpm_device.h; a host-side check compiles it
+// (material 1), Drucker-Prager sand (material 2), local mu(I) (material 9) and the
+// generalized-Newtonian fluid (material 10). Pure register math on the types of mpm_device.h; a host-side check compiles it
 // with MPM_DEV = __host__ __device__ and compares it with float64 finite differences of
 // the forward maps.
 #pragma once
@@ -210,6 +210,62 @@ MPM_DEV m33 adj_mu_i(const m33& Ft, const mpm_model_t* md, int mat, float dt, fl
     for (int a = 0; a < 3; ++a) Al.a[4 * a] += s[a] > 1e-6f ? eb[a] / s[a] : 0.0f;
     m33 Ftb = mmul(U, mmul(Al, mT(V)));
     if (!tension && !yield) Ftb = madd(Ftb, Fb);  // F = F_trial passes through
+    return Ftb;
+}
+
+// Adjoint of the Drucker-Prager sand return map (mat 2, rm_sand; reference
+// sand_return_mapping, mpm_utils.py:544-580) and its stress, the Hencky stress of the
+// returned F (kirchoff_stress_drucker_prager, mpm_utils.py:109-124). With e = log of the
+// clamped singular values, tr = sum e, h = e - tr / 3 and dg = |h| + c tr, c = alpha (3 lam
+// + 2 mu) / (2 mu): dg <= 0 keeps F = F_trial; dg > 0 in tension (tr > 0) gives F = U V^T
+// (en = 0, k = 0); dg > 0 in compression puts the log-strain on the cone, en = tr / 3 + k h
+// with k = 1 - dg / |h| = -c tr / |h|. Out: adjoint of F_trial; mu and lam partials added
+// (through the stress and through c).
+MPM_DEV m33 adj_sand(const m33& Ft, float mu, float lam, float alpha, const m33& Fb, const m33& tb_in, float& mub,
+                     float& lamb) {
+    m33 U, V;
+    v3 s3;
+    svd3(Ft, U, s3, V);
+    const float s[3] = {s3.x, s3.y, s3.z};
+    float e[3];
+    for (int a = 0; a < 3; ++a) e[a] = logf(fmaxf(fabsf(s[a]), 1e-14f));
+    const float tr = e[0] + e[1] + e[2];
+    const float me = tr / 3.0f;
+    const float h[3] = {e[0] - me, e[1] - me, e[2] - me};
+    const float hn = sqrtf(h[0] * h[0] + h[1] * h[1] + h[2] * h[2]);
+    const float c = (3.0f * lam + 2.0f * mu) / (2.0f * mu) * alpha;
+    const float dg = hn + c * tr;  // forward's test, same expression
+    const bool tension = dg > 0.0f && tr > 0.0f;
+    const bool cone = dg > 0.0f && tr <= 0.0f;  // then |h| > -c tr >= 0
+    float k = 1.0f;
+    if (tension) k = 0.0f;
+    if (cone) k = -c * tr / hn;
+    float en[3];
+    for (int a = 0; a < 3; ++a) en[a] = tension ? 0.0f : (cone ? me + k * h[a] : e[a]);
+    m33 Al;
+    float enb[3];
+    iso_adjoint(U, V, s, e, en, k, mu, lam, tension || cone, Fb, tb_in, Al, enb, mub, lamb);
+    float eb[3] = {0.f, 0.f, 0.f};
+    if (cone) {
+        const float meb = enb[0] + enb[1] + enb[2];
+        const float kb = enb[0] * h[0] + enb[1] * h[1] + enb[2] * h[2];
+        // k = -c tr / |h|
+        float trb = -kb * c / hn + meb / 3.0f;
+        const float hnb = kb * c * tr / (hn * hn);
+        const float cb = -kb * tr / hn;
+        float hb[3];
+        for (int a = 0; a < 3; ++a) hb[a] = k * enb[a] + hnb * h[a] / hn;
+        const float mhb = (hb[0] + hb[1] + hb[2]) / 3.0f;
+        for (int a = 0; a < 3; ++a) eb[a] = hb[a] - mhb + trb;
+        // c = alpha (3 lam / (2 mu) + 1)
+        lamb += cb * 3.0f * alpha / (2.0f * mu);
+        mub += -cb * 3.0f * alpha * lam / (2.0f * mu * mu);
+    } else if (!tension) {
+        for (int a = 0; a < 3; ++a) eb[a] = enb[a];
+    }
+    for (int a = 0; a < 3; ++a) Al.a[4 * a] += fabsf(s[a]) > 1e-14f ? eb[a] / s[a] : 0.0f;
+    m33 Ftb = mmul(U, mmul(Al, mT(V)));
+    if (!tension && !cone) Ftb = madd(Ftb, Fb);  // F = F_trial passes through
     return Ftb;
 }
 

@@ -10,7 +10,8 @@ final positions and the accumulated collider wrench, so a loss on the final stat
 the plate reaction forces back-propagates to Young's modulus, Poisson's ratio, the yield
 stress (through the per-particle parameters), the fluid's rheology, the granular
 friction law and to the initial particle state. Materials: fixed-corotated jelly (0),
-von Mises (1), local mu(I) (9) and the generalized-Newtonian fluid (10).
+von Mises (1), Drucker-Prager sand (2), local mu(I) (9) and the generalized-Newtonian
+fluid (10).
 
 Forward: fused ticks of `checkpoint_every` substeps (the production substep), storing
 the particle state (x, v, C, F_trial and the velocity gradient L, which the viscous
@@ -51,7 +52,7 @@ def lame(E, nu):
 class Rollout:
     """Differentiable rollout of `substeps` substeps of dt on a configured Solver."""
 
-    SUPPORTED_MATERIALS = frozenset((0, 1, GRANULAR_ID, FLUID_ID))
+    SUPPORTED_MATERIALS = frozenset((0, 1, 2, GRANULAR_ID, FLUID_ID))
     SUPPORTED_COLLIDERS = frozenset((native.COLLIDER_PLANE, native.COLLIDER_BOX, native.COLLIDER_WALLS,
                                      native.COLLIDER_POINTCLOUD))
 

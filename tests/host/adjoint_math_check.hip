@@ -2,6 +2,7 @@
 // tests/test_adjoint_math.py with MPM_DEV = __host__ __device__; it makes no HIP call and
 // runs without a GPU). One case per input line:
 //   mat dt mu lam ys eta bulk K n mu_s dmu I0 d rho_s Ft[9] L[9] Fb[9] tb[9]
+// (for material 2 the ys slot carries the Drucker-Prager alpha)
 // One output line per case:
 //   Ftb[9] Lb[9] mu_b lam_b ys_b eta_b bulk_b K_b n_b mu_s_b dmu_b I0_b d_b
 #define MPM_DEV __host__ __device__ inline
@@ -39,6 +40,8 @@ int main() {
         m33 Lb = mzero(), Ftb = mzero();
         if (mat == 0) {
             Ftb = adj_fcr(Ft, mu, lam, Fb, tb, mub, lamb);
+        } else if (mat == 2) {
+            Ftb = adj_sand(Ft, mu, lam, ys, Fb, tb, mub, lamb);
         } else if (mat == 1) {
             md.hardening[mat] = 0.0f;
             Ftb = adj_von_mises(Ft, mu, lam, ys, Fb, tb, mub, lamb, yb);

@@ -52,6 +52,22 @@ def _fcr(F, mu, lam):
     return F, 0.5 * (t + t.T)
 
 
+def _sand(Ft, mu, lam, alpha):
+    U, s, V = _svd(Ft)
+    e = np.log(np.maximum(np.abs(s), 1e-14))
+    tr = e.sum()
+    h = e - tr / 3
+    hn = np.linalg.norm(h)
+    dg = hn + (3 * lam + 2 * mu) / (2 * mu) * tr * alpha
+    if dg <= 0:
+        F = Ft
+    elif tr > 0:
+        F = U @ V.T
+    else:
+        F = U @ np.diag(np.exp(e - h * dg / hn)) @ V.T
+    return F, _hencky(F, mu, lam)
+
+
 def _von_mises(Ft, mu, lam, ys):
     U, s, V = _svd(Ft)
     e = np.log(np.maximum(s, 0.01))
@@ -107,7 +123,7 @@ def _viscous(Ft, L, eta, bulk, ty, K, n):
 
 # ---------------------------------------------------------------- cases
 _NAMES = ("mu", "lam", "ys", "eta", "bulk", "K", "n", "mu_s", "dmu", "I0", "d")
-_PARAMS = {0: ("mu", "lam"), 1: ("mu", "lam", "ys"), 9: ("mu", "lam", "mu_s", "dmu", "I0", "d"),
+_PARAMS = {0: ("mu", "lam"), 1: ("mu", "lam", "ys"), 2: ("mu", "lam"), 9: ("mu", "lam", "mu_s", "dmu", "I0", "d"),
            10: ("eta", "bulk", "ys", "K", "n")}
 
 
@@ -133,6 +149,23 @@ def _cases(rng):
         else:
             s = np.array([1.2, 0.9, -0.3])
         out.append(dict(base, mat=0, mu=mu, lam=lam, Ft=_rot(rng) @ np.diag(s) @ _rot(rng).T))
+    # Drucker-Prager sand: E 1e6, nu 0.3, friction angle 30 deg; inside the cone, on it
+    # (compression), and in tension beyond it, each kept 2 % from the switch
+    mu, lam = 1e6 / 2.6, 1e6 * 0.3 / (1.3 * 0.4)
+    sphi = np.sin(np.radians(30.0))
+    alpha = np.sqrt(2.0 / 3.0) * 2.0 * sphi / (3.0 - sphi)
+    cc = (3 * lam + 2 * mu) / (2 * mu) * alpha
+    for kind in ("elastic", "cone", "cone", "cone", "tension"):
+        while True:
+            tr = rng.uniform(0.002, 0.01) if kind == "tension" else -rng.uniform(0.005, 0.03)
+            dv = rng.standard_normal(3)
+            dv -= dv.mean()
+            dv *= rng.uniform(0.001, 0.05) / np.linalg.norm(dv)
+            r = np.linalg.norm(dv) / (-cc * tr) if tr < 0 else 2.0
+            if kind == "tension" or (abs(r - 1) > 0.02 and (r > 1) == (kind == "cone")):
+                break
+        out.append(dict(base, mat=2, mu=mu, lam=lam, ys=alpha,
+                        Ft=_rot(rng) @ np.diag(np.exp(tr / 3 + dv)) @ _rot(rng).T))
     # von Mises: E 2e5, nu 0.3, yield 1e3; elastic, plastic, near-repeated singular values
     mu, lam = 2e5 / 2.6, 2e5 * 0.3 / (1.3 * 0.4)
     for kind in ("elastic", "plastic", "plastic", "repeated", "repeated_plastic", "rotation"):
@@ -184,6 +217,8 @@ def _objective(c, stress_only=False, **over):
     p = dict(c, **over)
     if p["mat"] == 0:
         F, t = _fcr(p["Ft"], p["mu"], p["lam"])
+    elif p["mat"] == 2:
+        F, t = _sand(p["Ft"], p["mu"], p["lam"], p["ys"])
     elif p["mat"] == 1:
         F, t = _von_mises(p["Ft"], p["mu"], p["lam"], p["ys"])
     elif p["mat"] == 9:

@@ -687,3 +687,120 @@ def test_granular_rollout_gradients_match_oracle_differences():
     tol = {"I0": 2e-2, "grain_diameter": 2e-2}
     for k, (a, f) in report.items():
         assert abs(a - f) <= tol.get(k, 5e-3) * abs(f), f"{k}: adjoint {a:.6e} vs central difference {f:.6e}"
+
+
+def _sand_states(n, rng, c):
+    """F_trial = diag(exp(tr / 3 + dn (2, -1, -1) / sqrt 6)): 85 % compressed (tr e in
+    [-0.03, -0.005]) with |dev e| / (-c tr e) in [0.2, 0.8] (inside the Drucker-Prager cone)
+    or [1.25, 3] (returned onto it), 15 % in tension (tr e > 0: F = U V^T, no stress); one
+    principal frame, as for the mu(I) states, so the shear adds up in the plate impulse."""
+    d = np.array([2.0, -1.0, -1.0]) / np.sqrt(6.0)
+    Ft = np.zeros((n, 3, 3))
+    for p in range(n):
+        if rng.uniform() < 0.15:
+            tr, dn = rng.uniform(0.002, 0.01), rng.uniform(0.0, 0.02)
+        else:
+            tr = -rng.uniform(0.005, 0.03)
+            r = rng.uniform(0.2, 0.8) if rng.uniform() < 0.5 else rng.uniform(1.25, 3.0)
+            dn = r * c * -tr
+        Ft[p] = np.diag(np.exp(tr / 3.0 + dn * d))
+    return Ft.astype(np.float32)
+
+
+def test_sand_parameter_and_state_directions():
+    """Drucker-Prager sand (material 2; E 1e6, nu 0.3, friction angle 30 deg) between the
+    plates standing still, from compressed states inside and beyond the cone and tensile
+    ones, kept 20 % from either switch. One substep: d (plate x impulses) / d (E, nu)
+    against central differences of the GPU forward at relative steps 2e-2. Three
+    substeps (checkpoints every 2; from the second on the stress reads an F_trial the
+    velocity field has moved, through the sand F_trial adjoint): per-particle mu and lam
+    (relative 2e-2) and a smooth velocity field (1e-2 m/s; its gradient is what moves
+    F_trial) against central differences of the oracle, whose values at steps 3e-3 and
+    1e-2 agree within 4e-4. All within 2e-3. (A random +-1 velocity direction largely
+    cancels in the plate impulse: for some draws what is left is near fp32 rounding.)"""
+    import torch
+
+    E0, nu0 = 1.0e6, 0.3
+    sand = dict(E=E0, nu=nu0, density=1550.0, friction_angle=30.0)
+    sc = _scene(plate_speed=0.0)
+    s = sc.build(Solver(grid=sc.grid, device="hip"))
+    s.set_material("sand", **sand)
+    n = s.n_particles
+    sim = s._sim
+    mu0, lam0 = _lame(E0, nu0)
+    sphi = np.sin(30.0 / 180.0 * 3.14159265)
+    c = (3.0 * lam0 + 2.0 * mu0) / (2.0 * mu0) * np.sqrt(2.0 / 3.0) * 2.0 * sphi / (3.0 - sphi)
+    rng = np.random.default_rng(31)
+    Ft0 = _sand_states(n, rng, c)
+    sim.F_trial.copy_(torch.from_numpy(Ft0.reshape(n, 9).T.copy()))
+    dev = sim.device
+    cw = np.zeros((2, 3))
+    cw[:, 0] = rng.standard_normal(2)
+    f64 = dict(dtype=torch.float64, device=dev)
+    zero_v = torch.zeros(n, 3, device=dev)
+    # built first: a Rollout keeps the particle order it was built with (the scene's here),
+    # and the one-substep rollout's inputs and outputs below do not depend on the order
+    S = 3
+    ro3 = Rollout(s, DT, S, checkpoint_every=2)
+
+    # one substep: E, nu against the GPU forward
+    ro1 = Rollout(s, DT, 1, checkpoint_every=1)
+    par = {"E": E0, "nu": nu0}
+    tens = {k: torch.tensor(v, requires_grad=True, **f64) for k, v in par.items()}
+    _, w = ro1.run(tens["E"], tens["nu"], torch.tensor(1.0, **f64), zero_v)
+    grads = dict(zip(tens, torch.autograd.grad((w[:, :3].double() * torch.from_numpy(cw).to(dev)).sum(),
+                                              list(tens.values()))))
+
+    def impulse1(E_, nu_):
+        with torch.no_grad():
+            _, wv = ro1.run(torch.tensor(E_, **f64), torch.tensor(nu_, **f64), torch.tensor(1.0, **f64), zero_v)
+        return float((wv[:, :3].double().cpu().numpy() * cw).sum())
+
+    for i, name in enumerate(par):
+        base = [E0, nu0]
+        h = 2e-2 * base[i]
+        p, m = list(base), list(base)
+        p[i] += h
+        m[i] -= h
+        fd = (impulse1(*p) - impulse1(*m)) / (2 * h)
+        print("w", name, grads[name].item(), fd)
+        assert abs(grads[name].item() - fd) <= 2e-3 * abs(fd), f"w {name}: {grads[name].item():.6e} vs {fd:.6e}"
+
+    # three substeps: per-particle mu, lam and a velocity direction against the oracle
+    mu = torch.full((n,), float(np.float32(mu0)), device=dev, requires_grad=True)
+    lam = torch.full((n,), float(np.float32(lam0)), device=dev, requires_grad=True)
+    v = torch.zeros(n, 3, device=dev, requires_grad=True)
+    x0 = torch.from_numpy(sc.pos.copy()).to(dev)
+    _, w = ro3(mu, lam, torch.ones(n, device=dev), v, x0)
+    lw = (w[:, :3].double() * torch.from_numpy(cw).to(dev)).sum()
+    gw = dict(zip(("mu", "lam", "v0"), torch.autograd.grad(lw, [mu, lam, v])))
+    base = {"mu": np.full(n, np.float32(mu0), np.float32), "lam": np.full(n, np.float32(lam0), np.float32),
+            "v0": np.zeros((n, 3), np.float32)}
+    xs = sc.pos.astype(np.float64)
+    smooth = np.stack([np.sin(np.pi * (xs[:, 0] - xs[:, 0].min()) / 0.2),
+                       0.5 * np.cos(np.pi * (xs[:, 1] - xs[:, 1].min()) / 0.2), np.zeros(n)], 1)
+    dirs = {"mu": (rng.uniform(0.5, 1.5, n) * 2e-2 * mu0).astype(np.float32),
+            "lam": (rng.uniform(0.5, 1.5, n) * 2e-2 * lam0).astype(np.float32),
+            "v0": (smooth * 1e-2).astype(np.float32)}
+
+    def oracle_impulse(args):
+        o, plates = _oracle(sc, args["mu"], args["lam"], np.float32(1.0), args["v0"], Ft0=Ft0,
+                            material="sand")
+        o.set_material("sand", **sand)
+        o._sim.mu[:] = args["mu"]
+        o._sim.lam[:] = args["lam"]
+        o._sim.v[:] = args["v0"]
+        o._sim.F_trial[:] = Ft0
+        o.step(DT, S)
+        imp = np.stack([o.tool_force(hnd, 1.0) for hnd in plates]).astype(np.float64)
+        return (imp * cw).sum()
+
+    for name in ("mu", "lam", "v0"):
+        d = dirs[name]
+        ad = float((gw[name].double().cpu().numpy().reshape(d.shape) * d).sum())
+        args_p, args_m = dict(base), dict(base)
+        args_p[name] = base[name] + d
+        args_m[name] = base[name] - d
+        fd = (oracle_impulse(args_p) - oracle_impulse(args_m)) / 2.0
+        print("w3", name, ad, fd)
+        assert abs(ad - fd) <= 2e-3 * abs(fd), f"w3 {name}: adjoint {ad:.6e} vs central difference {fd:.6e}"
