@@ -65,6 +65,9 @@ class Rollout:
         if not sim._mat_ids <= self.SUPPORTED_MATERIALS:
             raise NotImplementedError(f"adjoint kernels cover material ids {sorted(self.SUPPORTED_MATERIALS)}; "
                                       f"the solver has {sorted(sim._mat_ids)}")
+        if 1 in sim._mat_ids and float(sim.model[0]["hardening"][1]) > 0.5:
+            # the hardened yield stress is particle state; its adjoint is not carried
+            raise NotImplementedError("von Mises with hardening (hardening_xi > 0) has no adjoint")
         kinds = {c.kind for c in sim.colliders.items}
         if not kinds <= self.SUPPORTED_COLLIDERS:
             raise NotImplementedError("adjoint kernels cover plane, box, wall and point-cloud colliders")

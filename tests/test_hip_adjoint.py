@@ -247,6 +247,11 @@ def test_unsupported_material_is_refused():
     s.set_material("plasticine", E=1.0e5, nu=0.3, density=1000.0)
     with pytest.raises(NotImplementedError):
         Rollout(s, DT, 10)
+    from mpm_amd.materials import vonmises
+
+    s.set_material(vonmises(E=2.0e5, nu=0.3, yield_stress=1.0e3, hardening_xi=0.1))
+    with pytest.raises(NotImplementedError, match="hardening"):
+        Rollout(s, DT, 10)
 
 
 def _jelly_solver(sc):
