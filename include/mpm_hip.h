@@ -243,7 +243,8 @@ typedef struct mpm_rigid_plane {
     float restitution, friction, start_time, end_time; /* window kept for the record */
 } mpm_rigid_plane_t;
 
-/* Device bytes of the workspace mpm_rigid_step needs for n_bodies and n_planes. */
+/* Device bytes of the workspace mpm_rigid_step needs for n_bodies and n_planes (no
+ * counterpart in the reference, which allocates its body sums itself). */
 int64_t mpm_rigid_workspace_bytes(int32_t n_bodies, int32_t n_planes);
 /* The rigid step that follows the G2P pass of every split substep (_p2g2p_tail,
  * mpm_solver_warp.py:1338-1367): gather the grid velocity (node_out) at every rigid
@@ -290,7 +291,8 @@ typedef struct mpm_particle_modifier {
 } mpm_particle_modifier_t;
 
 /* Set (selected) or clear bit mod->bit of P->modifier_mask for every particle; `mod` is a
- * host struct. */
+ * host struct. Replaces the selection kernels of add_impulse_on_particles and
+ * enforce_particle_velocity_* (mpm_utils.py:1320-1363, mpm_solver_warp.py:2385-2539). */
 int32_t mpm_modifier_select(const mpm_particles_t *P, const mpm_particle_modifier_t *mod, void *stream);
 /* Apply n_mods modifiers (device array, in the order added) to the selected particles:
  * every impulse first, then the velocity modifiers, as the reference's split substep
@@ -353,7 +355,8 @@ int32_t mpm_grid_zero_blocks(const mpm_grid_t *G, const int32_t *block_list,
                              void *stream);
 
 /* Build the list of blocks intersecting ctl->box (ctl is device memory) in every scene
- * into block_list (capacity nb0*nb1*nb2*n_scenes) and set ctl->nblocks. */
+ * into block_list (capacity nb0*nb1*nb2*n_scenes) and set ctl->nblocks: the dense
+ * path's launch box (Solver._update_grid_box, solver.py:485-518) as a block list. */
 int32_t mpm_blocks_from_box(const mpm_grid_t *G, mpm_tick_ctl_t *ctl, int32_t *block_list,
                             void *stream);
 
@@ -382,7 +385,8 @@ int32_t mpm_grid_update(const mpm_grid_t *G, const mpm_model_t *model, const mpm
                         int32_t wrench_slots, float *workspace, float dt, int32_t flags,
                         void *stream);
 /* Convert the integer wrench rows that MPM_GRID_FIXED_IN | MPM_GRID_WRENCH_DEFER calls left
- * in `workspace` into `wrench` (added) and clear them. */
+ * in `workspace` into `wrench` (added) and clear them. No counterpart in the reference
+ * (its wrench accumulators are float atomics, mpm_solver_warp.py:2121-2269). */
 int32_t mpm_grid_wrench_flush(const mpm_grid_t *G, int32_t capacity, float *wrench, int32_t wrench_slots,
                               float *workspace, void *stream);
 
@@ -414,7 +418,8 @@ int32_t mpm_sort_keys(const mpm_particles_t *P, const mpm_grid_t *G, int32_t *ke
  * counterpart in the reference (its sort is by block only). */
 int32_t mpm_sort_keys_cells(const mpm_particles_t *P, const mpm_grid_t *G, int32_t *keys,
                             int32_t *vals, int32_t *unsorted, void *stream);
-/* temp storage needed by mpm_sort_pairs for n keys */
+/* temp storage needed by mpm_sort_pairs for n keys (no counterpart: the reference sorts
+ * with a host argsort, solver.py:454-467) */
 int64_t mpm_sort_temp_bytes(int32_t n);
 /* stable radix sort of (keys, vals) pairs into (keys_out, vals_out) */
 int32_t mpm_sort_pairs(const int32_t *keys, int32_t *keys_out, const int32_t *vals,
