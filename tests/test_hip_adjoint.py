@@ -809,3 +809,59 @@ def test_sand_parameter_and_state_directions():
         fd = (oracle_impulse(args_p) - oracle_impulse(args_m)) / 2.0
         print("w3", name, ad, fd)
         assert abs(ad - fd) <= 2e-3 * abs(fd), f"w3 {name}: adjoint {ad:.6e} vs central difference {fd:.6e}"
+
+
+def test_sand_rollout_gradients_match_central_differences():
+    """Drucker-Prager sand (material 2; E 1e6, nu 0.3, friction angle 30 deg) in the
+    squeeze: 30 substeps with checkpoints every 10, plates closing at 0.5 m/s, loss
+    linearised at the GPU trajectory (as the von Mises and jelly rollout tests) against a
+    target rollout at E x 1.1. Checked: d loss / d (E, nu) against central differences
+    (eps = 3e-3 relative) of the oracle rollout, within 2e-3: the oracle's own differences
+    at eps = 1e-3, 3e-3 and 1e-2 agree within 6.3e-4 (E) and 4.9e-4 (nu) over these 30
+    substeps (within 3.8e-3 over 60, where the cone and contact switches accumulate)."""
+    import torch
+
+    sand = dict(density=1550.0, friction_angle=30.0)
+    sc = _scene(plate_speed=0.5)
+    s = sc.build(Solver(grid=sc.grid, device="hip"))
+    s.set_material("sand", E=1.0e6, nu=0.3, **sand)
+    n = s.n_particles
+    S = 30
+    ro = Rollout(s, DT, S, checkpoint_every=10)
+    dev = s._sim.device
+    E0, nu0 = 1.0e6, 0.3
+    f64 = dict(dtype=torch.float64, device=dev)
+    one = torch.tensor(1.0, **f64)
+    with torch.no_grad():
+        xt, wt = ro.run(torch.tensor(1.1 * E0, **f64), torch.tensor(nu0, **f64), one, torch.zeros(n, 3, device=dev))
+    x_star, w_star = xt.double(), wt[:, :3].double()
+    E = torch.tensor(E0, requires_grad=True, **f64)
+    nu = torch.tensor(nu0, requires_grad=True, **f64)
+    xT, w = ro.run(E, nu, one, torch.zeros(n, 3, device=dev))
+    loss = ((xT.double() - x_star) ** 2).sum() + ((w[:, :3].double() - w_star) ** 2).sum()
+    loss.backward()
+    gx = (2.0 * (xT.double() - x_star)).detach().cpu().numpy()
+    gw = (2.0 * (w[:, :3].double() - w_star)).detach().cpu().numpy()
+
+    def linearised(E_, nu_):
+        mu, lam = _lame(E_, nu_)
+        o, plates = _oracle(sc, np.float32(mu), np.float32(lam), np.float32(1.0), np.zeros((n, 3), np.float32),
+                            material="sand")
+        o.set_material("sand", E=E_, nu=nu_, **sand)
+        o._sim.mu[:] = np.float32(mu)
+        o._sim.lam[:] = np.float32(lam)
+        o.step(DT, S)
+        imp = np.stack([o.tool_force(h, 1.0) for h in plates]).astype(np.float64)
+        return (o.x().astype(np.float64) * gx).sum() + (imp * gw).sum()
+
+    report = {}
+    for i, (name, grad) in enumerate((("E", E), ("nu", nu))):
+        base = [E0, nu0]
+        h = 3e-3 * base[i]
+        p, m = list(base), list(base)
+        p[i] += h
+        m[i] -= h
+        report[name] = (grad.grad.item(), (linearised(*p) - linearised(*m)) / (2 * h))
+    print("loss", loss.item(), report)
+    for k, (a, f) in report.items():
+        assert abs(a - f) <= 2e-3 * abs(f), f"{k}: adjoint {a:.6e} vs central difference {f:.6e}"
