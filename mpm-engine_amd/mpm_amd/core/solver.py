@@ -311,8 +311,8 @@ class HipSimulator:
         self._model_dirty = True
 
     def finalize_mu_lam(self):
-        self._m_max = None
         """compute_mu_lam_from_E_nu (mpm_utils.py:583-588), fp32 on device."""
+        self._m_max = None
         torch = self.torch
         md = self.model[0]
         E = torch.from_numpy(md["E"].copy()).to(self.device)[self.material.long()]
@@ -930,6 +930,18 @@ class HipSimulator:
         native.check(self.lib.mpm_grid_to_dense(self.pG, native.ptr(m), native.ptr(vi), native.ptr(vo),
                                                 self.stream()), "grid_to_dense")
         return m, vi, vo
+
+    def finalize_mu_lam_bulk(self):
+        """mpm_solver_warp.py:808-809 (an alias of finalize_mu_lam)."""
+        self.finalize_mu_lam()
+
+    def reset_densities_and_update_masses(self, all_particle_densities, device="cuda:0"):
+        """mpm_solver_warp.py:1622-1635: mass = density * vol per particle."""
+        d = all_particle_densities.detach().to(device=self.device, dtype=self.mass.dtype).reshape(-1)
+        if d.shape[0] != self.n_particles:
+            raise ValueError(f"{d.shape[0]} densities for {self.n_particles} particles")
+        self.mass.copy_(d * self.vol)
+        self._m_max = None  # the fixed-point mass scale follows the heaviest particle
 
     # ------------------------------------------- torch imports / exports (the simulator's API)
     # MPM_Simulator_WARP's import_*_from_torch / export_*_to_torch (mpm_solver_warp.py:

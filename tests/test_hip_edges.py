@@ -104,6 +104,15 @@ def test_simulator_torch_import_export_round_trip():
     assert tuple(sim.export_grid_v_out_to_torch().shape) == (32, 32, 32, 3)
     with pytest.raises(ValueError, match="rows"):
         sim.import_particle_x_from_torch(torch.zeros(n + 1, 3))
+    # reset_densities_and_update_masses: mass = density * vol, and the deterministic
+    # scatter's mass scale follows the new heaviest particle
+    dens = torch.from_numpy(np.random.default_rng(6).uniform(500.0, 3000.0, n).astype(np.float32))
+    sim.reset_densities_and_update_masses(dens)
+    np.testing.assert_allclose(sim.mass.cpu().numpy(), dens.numpy() * sim.vol.cpu().numpy(), rtol=1e-6)
+    assert sim._m_max is None
+    hip.step(1e-4, 2)
+    assert sim._m_max == pytest.approx(float((dens * sim.vol.cpu()).max()), rel=1e-6)
+    assert np.isfinite(hip.x()).all()
     # material import re-selects the pass specialisation (here: all viscous)
     sim.import_particle_material_from_torch(torch.full((n,), 6, dtype=torch.int32))
     assert sim._mat_ids == {6}
