@@ -163,6 +163,7 @@ SIGNATURES = {
 }
 
 _LIB = None
+ABI_VERSION = 5  # MPM_ABI_VERSION of include/mpm_hip.h this binding is written for
 
 
 class NativeError(RuntimeError):
@@ -183,6 +184,9 @@ def load():
         fn = getattr(lib, name)
         fn.restype = res
         fn.argtypes = args
+    if lib.mpm_abi_version() != ABI_VERSION:
+        raise ImportError(f"libmpm_hip ABI version {lib.mpm_abi_version()}, this binding needs {ABI_VERSION}: "
+                          f"rebuild with `python mpm-engine_amd/build.py`")
     sizes = {0: C.sizeof(Particles), 1: C.sizeof(Grid), 2: MODEL_DTYPE.itemsize,
              3: COLLIDER_DTYPE.itemsize, 4: TICK_CTL_DTYPE.itemsize,
              5: CDF_LANES_DTYPE.itemsize, 6: C.sizeof(CdfCollider), 7: C.sizeof(Adjoint),

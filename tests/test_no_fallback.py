@@ -21,6 +21,16 @@ def test_missing_library_raises_import_error(monkeypatch, tmp_path):
         native.load()
 
 
+def test_binding_matches_the_header_abi_version():
+    import os
+    import re
+
+    hdr = open(os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "include",
+                            "mpm_hip.h")).read()
+    assert int(re.search(r"#define MPM_ABI_VERSION (\d+)", hdr).group(1)) == native.ABI_VERSION
+    assert native.load().mpm_abi_version() == native.ABI_VERSION
+
+
 def test_cpu_device_is_refused():
     with pytest.raises(NotImplementedError, match="only on the GPU"):
         Solver(grid=GridConfig(n_grid=32, grid_lim=1.0), device="cpu").load_particles(PTS, VOL)
