@@ -75,8 +75,6 @@ class Rollout:
             raise NotImplementedError("periodic domains have no adjoint")
         if sim.cdf is not None and sim.cdf.n > 0:
             raise NotImplementedError("CPIC (CDF) colliders have no adjoint")
-        if sim.deterministic:  # the adjoint reads the float scatter result (node_in)
-            sim.disable_deterministic()
         if len(sim.colliders.items) > 16:
             raise ValueError("at most 16 colliders in a differentiable rollout")
         self.solver = solver
@@ -158,6 +156,9 @@ class Rollout:
         self._G = [self._grid_struct(self.g_in[k], self.g_out[k]) for k in range(L)]
         self.use_graph = True
         self._graph = None
+        # checkpoints, slots and model fields hold the most recent forward only: each
+        # forward gets a serial number and a backward refuses to run on another's data
+        self.forward_count = 0
 
     # ------------------------------------------------------------------ structs
     def _grid_struct(self, node_in, node_out):
@@ -222,9 +223,24 @@ class Rollout:
             dt, flags, stream), "grid_update")
 
     def forward(self, mu, lam, ys, v0, x0, fluid, granular):
-        """Fused forward rollout from (x0, v0); returns (x_T (n, 3), wrench (slots, 6))."""
+        """Fused forward rollout from (x0, v0); returns (x_T (n, 3), wrench (slots, 6)).
+
+        The adjoint reads the float scatter result (node_in), so the rollout runs the
+        float-atomic scatter; a solver in the (default) fixed-point mode is switched back
+        to it when the rollout returns."""
         sim = self.sim
-        torch = self.torch
+        was_det = sim.deterministic
+        if was_det:
+            sim.disable_deterministic()
+        try:
+            return self._forward(mu, lam, ys, v0, x0, fluid, granular)
+        finally:
+            self.forward_count += 1
+            if was_det:
+                sim.enable_deterministic()
+
+    def _forward(self, mu, lam, ys, v0, x0, fluid, granular):
+        sim = self.sim
         self._set_state(mu, lam, ys, v0, x0, fluid, granular)
         self.wrench.zero_()
         sim.error_flag.zero_()
@@ -304,9 +320,25 @@ class Rollout:
             native.check(lib.mpm_adjoint_p2g_stress(C.byref(self._P_st[k]), G, pA, md, dt, s),
                          "adjoint p2g+stress")
 
-    def backward(self, x_bar, wrench_bar):
+    def backward(self, x_bar, wrench_bar, forward_id=None):
         """Adjoint of the last forward: returns (mu_bar, lam_bar, ys_bar, v0_bar, x0_bar,
-        fluid_bar, granular_bar)."""
+        fluid_bar, granular_bar). forward_id (the forward's serial number, forward_count - 1
+        when it returned) must name the last forward: the checkpoints belong to it."""
+        if forward_id is not None and forward_id != self.forward_count - 1:
+            raise RuntimeError("this Rollout ran another forward since the one being differentiated; "
+                               "its checkpoints are gone (backward each result before the next forward, "
+                               "or use one Rollout per differentiable call)")
+        sim = self.sim
+        was_det = sim.deterministic
+        if was_det:
+            sim.disable_deterministic()
+        try:
+            return self._backward(x_bar, wrench_bar)
+        finally:
+            if was_det:
+                sim.enable_deterministic()
+
+    def _backward(self, x_bar, wrench_bar):
         sim = self.sim
         torch = self.torch
         a = self.adj
@@ -416,6 +448,7 @@ def _make_fn():
             xT, w = ro.forward(_as_f32(mu, torch), _as_f32(lam, torch), _as_f32(ys, torch),
                                _as_f32(v0, torch), _as_f32(x0, torch), _as_f32(fluid, torch),
                                _as_f32(granular, torch))
+            ctx.forward_id = ro.forward_count - 1
             return xT, w
 
         @staticmethod
@@ -424,7 +457,8 @@ def _make_fn():
             if xT_bar is None:
                 xT_bar = torch.zeros(ro.n, 3, dtype=torch.float32, device=ro.sim.device)
             mub, lamb, yb, vb, xb, fb, gb = ro.backward(_as_f32(xT_bar, torch),
-                                                        None if w_bar is None else _as_f32(w_bar, torch))
+                                                        None if w_bar is None else _as_f32(w_bar, torch),
+                                                        ctx.forward_id)
             dts = ctx.dtypes
             return (None, mub.to(dts[0]), lamb.to(dts[1]), yb.to(dts[2]), vb.to(dts[3]), xb.to(dts[4]),
                     fb.to(dts[5]), gb.to(dts[6]))

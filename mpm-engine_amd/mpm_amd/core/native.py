@@ -180,13 +180,16 @@ def load():
             f"libmpm_hip.so not found at {LIB_PATH}: build it with "
             f"`python mpm-engine_amd/build.py` (there is no non-HIP fallback)")
     lib = C.CDLL(LIB_PATH)
+    # the version first: a stale library may lack entry points bound below
+    lib.mpm_abi_version.restype = _I32
+    lib.mpm_abi_version.argtypes = []
+    if lib.mpm_abi_version() != ABI_VERSION:
+        raise ImportError(f"libmpm_hip ABI version {lib.mpm_abi_version()}, this binding needs {ABI_VERSION}: "
+                          f"rebuild with `python mpm-engine_amd/build.py`")
     for name, (res, args) in SIGNATURES.items():
         fn = getattr(lib, name)
         fn.restype = res
         fn.argtypes = args
-    if lib.mpm_abi_version() != ABI_VERSION:
-        raise ImportError(f"libmpm_hip ABI version {lib.mpm_abi_version()}, this binding needs {ABI_VERSION}: "
-                          f"rebuild with `python mpm-engine_amd/build.py`")
     sizes = {0: C.sizeof(Particles), 1: C.sizeof(Grid), 2: MODEL_DTYPE.itemsize,
              3: COLLIDER_DTYPE.itemsize, 4: TICK_CTL_DTYPE.itemsize,
              5: CDF_LANES_DTYPE.itemsize, 6: C.sizeof(CdfCollider), 7: C.sizeof(Adjoint),

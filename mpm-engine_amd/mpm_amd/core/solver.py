@@ -302,6 +302,7 @@ class HipSimulator:
         if "density" in kw:
             self.density.fill_(float(kw["density"]))
             torch.mul(self.density, self.vol, out=self.mass)
+            self._m_max = None  # the fixed-point mass scale follows the heaviest particle
         if "E" in kw or "nu" in kw:
             self.finalize_mu_lam()
         if "rpic_damping" in kw:
@@ -321,8 +322,8 @@ class HipSimulator:
         self.lam.copy_(E * nu / ((1.0 + nu) * (1.0 - 2.0 * nu)))
 
     def set_parameters_for_particles(self, a: int, b: int, kw: dict):
-        self._m_max = None
         """set_parameters_for_particles (mpm_solver_warp.py:1700-1783)."""
+        self._m_max = None
         m = None
         if "material" in kw:
             m = self._material_name_to_id(kw["material"])
@@ -940,7 +941,8 @@ class HipSimulator:
         d = all_particle_densities.detach().to(device=self.device, dtype=self.mass.dtype).reshape(-1)
         if d.shape[0] != self.n_particles:
             raise ValueError(f"{d.shape[0]} densities for {self.n_particles} particles")
-        self.mass.copy_(d * self.vol)
+        self.density.copy_(d)  # particle_density first, as the reference (mpm_solver_warp.py:1624)
+        self.torch.mul(self.density, self.vol, out=self.mass)
         self._m_max = None  # the fixed-point mass scale follows the heaviest particle
 
     # ------------------------------------------- torch imports / exports (the simulator's API)

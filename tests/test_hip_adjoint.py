@@ -254,6 +254,37 @@ def test_unsupported_material_is_refused():
         Rollout(s, DT, 10)
 
 
+def test_backward_after_another_forward_is_refused_and_the_scatter_mode_restored():
+    """A Rollout keeps the checkpoints of its most recent forward only: differentiating an
+    earlier result after a second forward raises instead of returning the second
+    trajectory's gradient. The rollout runs the float scatter and gives the solver its
+    (default, fixed-point) mode back afterwards."""
+    import torch
+
+    sc = _scene()
+    s = sc.build(Solver(grid=sc.grid, device="hip"))
+    assert s._sim.deterministic
+    n = s.n_particles
+    ro = Rollout(s, DT, 10, checkpoint_every=5)
+    assert s._sim.deterministic
+    dev = s._sim.device
+    E = torch.tensor(2.0e5, dtype=torch.float64, device=dev, requires_grad=True)
+    nu = torch.tensor(0.3, dtype=torch.float64, device=dev)
+    ys = torch.tensor(1.0e3, dtype=torch.float64, device=dev)
+    xa, _ = ro.run(E, nu, ys, torch.zeros(n, 3, device=dev))
+    assert s._sim.deterministic
+    with torch.no_grad():
+        ro.run(E, nu, 1.1 * ys, torch.zeros(n, 3, device=dev))
+    with pytest.raises(RuntimeError, match="another forward"):
+        xa.sum().backward()
+    xb, _ = ro.run(E, nu, ys, torch.zeros(n, 3, device=dev))
+    xb.sum().backward()  # the latest forward differentiates
+    assert np.isfinite(E.grad.item())
+    assert s._sim.deterministic
+    s.step(DT, 5)  # the solver steps on in its own mode
+    assert np.isfinite(s.x()).all()
+
+
 def _jelly_solver(sc):
     from mpm_amd.materials import elastic
 

@@ -109,6 +109,7 @@ def test_simulator_torch_import_export_round_trip():
     dens = torch.from_numpy(np.random.default_rng(6).uniform(500.0, 3000.0, n).astype(np.float32))
     sim.reset_densities_and_update_masses(dens)
     np.testing.assert_allclose(sim.mass.cpu().numpy(), dens.numpy() * sim.vol.cpu().numpy(), rtol=1e-6)
+    np.testing.assert_array_equal(sim.density.cpu().numpy(), dens.numpy())  # particle_density kept
     assert sim._m_max is None
     hip.step(1e-4, 2)
     assert sim._m_max == pytest.approx(float((dens * sim.vol.cpu()).max()), rel=1e-6)
