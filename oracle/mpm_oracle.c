@@ -18,6 +18,15 @@
 
 #define EXPORT __attribute__((visibility("default")))
 
+/* Planted-mistake builds for the tests that show a pin has teeth: compiled with
+ * -DO_MUTANT=k, the oracle carries mistake k (a flipped sign, a transposed operand, a
+ * dropped factor) at one place marked MUT(k). The checker build has O_MUTANT 0, where
+ * every MUT(k) is false and the code is the restatement. */
+#ifndef O_MUTANT
+#define O_MUTANT 0
+#endif
+#define MUT(k) (O_MUTANT == (k))
+
 /* Built twice: libmpm_oracle.so (the checker: single-threaded, particle order) and, with
  * -fopenmp -DO_OPENMP, libmpm_oracle_omp.so, used only to time the CPU baseline on every
  * host core: particle loops split over threads, scatter adds atomic (summation order then
@@ -740,6 +749,7 @@ static void update_cov(OState *st, int p, m33 gv, float dt) {
     cn.a[0] = c[0]; cn.a[1] = c[1]; cn.a[2] = c[2];
     cn.a[3] = c[1]; cn.a[4] = c[3]; cn.a[5] = c[4];
     cn.a[6] = c[2]; cn.a[7] = c[4]; cn.a[8] = c[5];
+    if (MUT(10)) gv = mT(gv);
     m33 n1 = madd(cn, mscale(madd(mmul(gv, cn), mmul(cn, mT(gv))), dt));
     c[0] = n1.a[0]; c[1] = n1.a[1]; c[2] = n1.a[2];
     c[3] = n1.a[4]; c[4] = n1.a[5]; c[5] = n1.a[8];
@@ -869,7 +879,8 @@ EXPORT void o_grid_normalize(OState *st, OModel *md, float dt, const int32_t *lo
 EXPORT void o_grid_damping(OState *st, OModel *md, float scale, const int32_t *lo, const int32_t *dims) {
     BOX_LOOP(lo, dims) {
         long g = gidx(md, gi, gj, gk);
-        for (int c = 0; c < 3; ++c) st->grid_v_out[3 * g + c] = st->grid_v_out[3 * g + c] * scale;
+        for (int c = 0; c < 3; ++c)
+            st->grid_v_out[3 * g + c] = MUT(9) ? st->grid_v_out[3 * g + c] / scale : st->grid_v_out[3 * g + c] * scale;
     }
 }
 
@@ -886,6 +897,7 @@ EXPORT void o_collide_plane(float time, float dt, OState *st, OModel *md, const 
                     (float)gk * md->dx - pm->point[2]);
         v3 n = V3(pm->normal[0], pm->normal[1], pm->normal[2]);
         float dp = vdot(off, n);
+        if (MUT(2)) dp = -dp;
         if (!(dp < 0.0f)) continue;
         float *vo = st->grid_v_out + 3 * g;
         if (pm->surface_type == 0) {
@@ -904,7 +916,7 @@ EXPORT void o_collide_plane(float time, float dt, OState *st, OModel *md, const 
             if (pm->surface_type == 1) v = vsub(v, vscale(n, nc));
             else v = vsub(v, vscale(n, fminf(nc, 0.0f)));
             if (nc < 0.0f && vlen(v) > 1e-20f)
-                v = vscale(vnormalize(v), fmaxf(0.0f, vlen(v) + nc * pm->friction));
+                v = vscale(vnormalize(v), fmaxf(0.0f, vlen(v) + (MUT(1) ? -nc : nc) * pm->friction));
             vstore(vo, v);
         }
     }
@@ -974,7 +986,7 @@ EXPORT void o_collide_cup(float time, float dt, OState *st, OModel *md, const OC
         float *vo = st->grid_v_out + 3 * g;
         v3 vin = vload(vo);
         v3 vnew = vs;
-        if (sdf > -pm->sticky_depth) {
+        if (MUT(7) ? sdf <= -pm->sticky_depth : sdf > -pm->sticky_depth) {
             vnew = vin;
             float h = 0.25f * md->dx;
             v3 nl = V3(glass_sdf(vadd(loc, V3(h, 0, 0)), pm) - glass_sdf(vsub(loc, V3(h, 0, 0)), pm),
@@ -988,13 +1000,14 @@ EXPORT void o_collide_cup(float time, float dt, OState *st, OModel *md, const OC
                 if (vn < 0.0f) {
                     v3 vt = vsub(vr, vscale(n, vn));
                     float vtl = vlen(vt);
-                    if (vtl > 1.0e-20f) vt = vscale(vdiv(vt, vtl), fmaxf(vtl + pm->friction * vn, 0.0f));
+                    if (vtl > 1.0e-20f)
+                        vt = vscale(vdiv(vt, vtl), fmaxf(vtl + (MUT(6) ? -pm->friction : pm->friction) * vn, 0.0f));
                     vnew = vadd(vs, vt);
                 }
             }
         }
         v3 imp = vscale(vsub(vin, vnew), st->grid_m[g]);
-        v3 tq = vcross(rel, imp);
+        v3 tq = MUT(11) ? vcross(imp, rel) : vcross(rel, imp);
         for (int c = 0; c < 3; ++c) { pm->force[c] += imp.c[c]; pm->torque[c] += tq.c[c]; }
         vstore(vo, vnew);
     }
@@ -1010,7 +1023,7 @@ EXPORT void o_collide_walls(float time, float dt, OState *st, OModel *md, const 
         if (st->grid_m[g] <= 0.0f) continue;
         if (!(time >= pm->start_time && time < pm->end_time)) continue;
         float *vo = st->grid_v_out + 3 * g;
-        if (gi < pad && vo[0] < 0) vo[0] = 0.0f;
+        if (gi < pad && (MUT(3) ? vo[0] > 0 : vo[0] < 0)) vo[0] = 0.0f;
         if (gi >= md->nx - pad && vo[0] > 0) vo[0] = 0.0f;
         if (gj < pad && vo[1] < 0) vo[1] = 0.0f;
         if (gj >= md->ny - pad && vo[1] > 0) vo[1] = 0.0f;
@@ -1148,6 +1161,7 @@ EXPORT void o_collide_sdf(float time, float dt, OState *st, OModel *md, const OS
         trilerp(pm->sdf_grad, 3, pm->res, fidx, gr.c);
         if (!(vlen(gr) > 1.0e-12f)) continue;
         v3 nw = quat_rotate(pm->quat, vnormalize(gr), 0);
+        if (MUT(4)) nw = vscale(nw, -1.0f);
         v3 vsurf = vadd(vload(pm->velocity), vcross(vload(pm->omega), rel));
         float *vo = st->grid_v_out + 3 * g;
         v3 vf = vload(vo);
@@ -1155,11 +1169,11 @@ EXPORT void o_collide_sdf(float time, float dt, OState *st, OModel *md, const OS
         float vn = vdot(vr, nw);
         v3 vnew = vf;
         if (pm->surface_type == 0) vnew = vsurf;
-        else if (pm->surface_type == 1) vnew = vadd(vsurf, vsub(vr, vscale(nw, vn)));
+        else if (pm->surface_type == 1) vnew = vadd(vsurf, MUT(12) ? vr : vsub(vr, vscale(nw, vn)));
         else if (vn < 0.0f) {
             v3 vt = vsub(vr, vscale(nw, vn));
             float tl = vlen(vt);
-            if (tl > 1.0e-12f) vt = vscale(vt, fmaxf(0.0f, tl + pm->friction * vn) / tl);
+            if (tl > 1.0e-12f) vt = vscale(vt, fmaxf(0.0f, tl + (MUT(5) ? -pm->friction : pm->friction) * vn) / tl);
             vnew = vadd(vsurf, vt);
         }
         v3 imp = vscale(vsub(vf, vnew), st->grid_m[g]);
@@ -1177,7 +1191,7 @@ EXPORT void o_collide_pointcloud(float time, float dt, OState *st, OModel *md, c
         long g = gidx(md, gi, gj, gk);
         if (st->grid_m[g] <= 0.0f) continue;
         if (!(time >= pm->start_time && time < pm->end_time)) continue;
-        if (pm->occupancy[g] == 1) {
+        if (pm->occupancy[g] == (MUT(8) ? 0 : 1)) {
             float *vo = st->grid_v_out + 3 * g;
             vo[0] = vo[1] = vo[2] = 0.0f;
         }

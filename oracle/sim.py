@@ -138,6 +138,43 @@ def use_threaded_build(on: bool) -> None:
         _LIB_NAME, _LIB = name, None
 
 
+def mutant_path(k: int) -> str:
+    """Path of the oracle built with planted mistake k (-DO_MUTANT=k, see MUT(k) in
+    mpm_oracle.c), compiled on first use into oracle/_mut/. Tests load it to show that a
+    pin fails on the mistake; it is never the checker."""
+    import subprocess
+
+    d = os.path.join(_HERE, "_mut")
+    os.makedirs(d, exist_ok=True)
+    path = os.path.join(d, f"libmpm_oracle_mut{int(k)}.so")
+    src = os.path.join(_HERE, "mpm_oracle.c")
+    if not os.path.exists(path) or os.path.getmtime(path) < os.path.getmtime(src):
+        tmp = f"{path}.{os.getpid()}.tmp"
+        subprocess.run(["gcc", "-O1", "-fPIC", "-std=c11", "-ffp-contract=off", "-fno-fast-math",
+                        f"-DO_MUTANT={int(k)}", "-shared", "-o", tmp, src, "-lm"], check=True,
+                       capture_output=True)
+        os.replace(tmp, path)
+    return path
+
+
+class mutant:
+    """Context manager: the oracle functions run the build with planted mistake k."""
+
+    def __init__(self, k: int):
+        self.path = mutant_path(k)
+
+    def __enter__(self):
+        global _LIB, _LIB_NAME
+        self.saved = (_LIB, _LIB_NAME)
+        _LIB, _LIB_NAME = None, self.path
+        return self
+
+    def __exit__(self, *exc):
+        global _LIB, _LIB_NAME
+        _LIB, _LIB_NAME = self.saved
+        return False
+
+
 def lib():
     """Load (building on first use if needed) the oracle shared library."""
     global _LIB

@@ -396,3 +396,101 @@ def test_grid_edge_raises_on_gpu():
     s.set_material(newtonian(eta=5.0))
     with pytest.raises(RuntimeError, match="out of bounds"):
         s.step(2e-4, 1)
+
+
+def _shelf_points(rng, n=600):
+    """A point-cloud shelf right under the blob: a 0.14 x 0.14 slab 1.5 cells thick."""
+    p = rng.uniform(0.0, 1.0, (n, 3))
+    return (np.array([0.13, 0.13, 0.13]) + p * np.array([0.14, 0.14, 0.015])).astype(np.float64)
+
+
+def _pointcloud_run(cls, **kw):
+    grid, pts, vol = _blob(n=2000, seed=12)
+    s = cls(grid=grid, **kw) if cls is Solver else cls(grid)
+    s.load_particles(pts, vol)
+    s.set_material(newtonian(eta=2.0, density=1000.0, bulk_modulus=2.0e5))
+    cloud = _shelf_points(np.random.default_rng(5))
+    if cls is Solver:
+        s.add_point_cloud_collider(cloud, padding=1)
+    else:
+        s._sim.add_point_cloud_collider(cloud, padding=1)
+    s.add_domain_walls()
+    v = np.zeros((len(pts), 3), np.float32)
+    v[:, 2] = -1.0  # onto the shelf
+    s.set_v(v)
+    for _ in range(4):
+        s.step(2.0e-4, 8)
+    return s
+
+
+def test_pointcloud_collider_matches_oracle():
+    """The point-cloud collider (mpm_solver_warp.py:2545-2616): a blob falling onto a
+    dilated point-cloud shelf, 4 ticks of 8 substeps, against the oracle (whose
+    occupancy rule is pinned by brute force on the CPU)."""
+    hip = _pointcloud_run(Solver, device="hip")
+    ora = _pointcloud_run(OracleSolver)
+    dx = 0.4 / 40
+    assert np.abs(hip.x() - ora.x()).max() <= 1e-4 * dx
+    # the shelf stopped the material: without it the front falls 6.4 mm
+    assert hip.x()[:, 2].min() > 0.15 - 0.003
+
+
+def test_grid_damping_matches_oracle():
+    """add_damping_via_grid (mpm_utils.py:1287-1295) at scale 0.999, applied only below
+    1: the damped run against the oracle's, and visibly apart from the undamped one."""
+    def run(cls, scale, **kw):
+        return _mat_run(cls, ("newtonian", dict(plastic_viscosity=2.0, density=1000.0, bulk_modulus=2.0e5,
+                                                grid_v_damping_scale=scale)), **kw)
+
+    hip = run(Solver, 0.999, device="hip")
+    ora = run(OracleSolver, 0.999)
+    undamped = run(OracleSolver, 1.1)
+    dx = 0.4 / 40
+    assert np.abs(hip.x() - ora.x()).max() <= 1e-4 * dx
+    assert np.abs(undamped.x() - ora.x()).max() > 1e-3 * dx
+
+
+def test_stationary_material_matches_oracle():
+    """Material 7 (stationary, mpm_utils.py:1090-1091, skipped by the gather): the lower
+    half of the blob is stationary and the upper half jelly falling onto it; the
+    stationary particles do not move, and everything matches the oracle."""
+    def run(cls, **kw):
+        grid, pts, vol = _blob(n=3000, seed=3)
+        order = np.argsort(pts[:, 2])
+        pts = pts[order]
+        s = cls(grid=grid, **kw) if cls is Solver else cls(grid)
+        s.load_particles(pts, vol)
+        s.set_material(elastic(E=5.0e4, nu=0.3, density=800.0))
+        s.set_material_range(0, 1500, "stationary", density=800.0)
+        s.add_plane((0, 0, 3 * grid.dx), (0, 0, 1), "sticky")
+        for _ in range(3):
+            s.step(2.0e-4, 8)
+        return s, pts
+
+    hip, pts = run(Solver, device="hip")
+    ora, _ = run(OracleSolver)
+    np.testing.assert_array_equal(hip.x()[:1500], pts[:1500])
+    assert np.abs(hip.x() - ora.x()).max() <= 1e-4 * 0.01
+    assert np.abs(hip.x()[1500:] - pts[1500:]).max() > 5e-5  # the jelly half fell (~1e-4 m)
+
+
+def test_covariance_transport_step_mode_matches_oracle():
+    """cov_mode="step" (the Solver's default with a covariance): update_cov
+    (mpm_utils.py:614-633) transports each particle's covariance with the gathered
+    velocity gradient every substep; after 3 ticks against the oracle."""
+    grid, pts, vol = _blob(n=800, seed=8)
+    rng = np.random.default_rng(4)
+    A = rng.normal(size=(800, 3, 3)) * 1e-3
+    S = A @ A.transpose(0, 2, 1)
+    cov = S[:, [0, 0, 0, 1, 1, 2], [0, 1, 2, 1, 2, 2]].astype(np.float32)
+    out = []
+    for s in (Solver(grid=grid, device="hip"), OracleSolver(grid)):
+        s.load_particles(pts, vol, cov=cov, cov_mode="step")
+        s.set_material(newtonian(eta=2.0, density=1000.0, bulk_modulus=2.0e5))
+        s.add_plane((0, 0, 0.15), (0, 0, 1), "separate", friction=0.3)
+        for _ in range(3):
+            s.step(2e-4, 8)
+        out.append(s.cov() if isinstance(s, Solver) else s._sim.cov.reshape(-1, 6).copy())
+    hip_cov, ora_cov = out
+    assert np.abs(ora_cov - cov).max() > 1e-3 * np.abs(cov).max()  # it was transported
+    np.testing.assert_allclose(hip_cov, ora_cov, rtol=1e-4, atol=1e-5 * np.abs(cov).max())
