@@ -35,14 +35,18 @@ def _run(cmd):
         raise RuntimeError(f"build failed: {' '.join(cmd)}\n{err}")
 
 
-def build_hip():
+def build_hip(jobs: int | None = None):
+    """Compile every source (in parallel: each hipcc is one process) and link."""
+    from concurrent.futures import ThreadPoolExecutor
+
     os.makedirs(BUILD, exist_ok=True)
     os.makedirs(os.path.dirname(LIB), exist_ok=True)
-    objs = []
-    for src in SOURCES:
-        obj = os.path.join(BUILD, src.replace(".hip", ".o"))
-        _run([HIPCC, *FLAGS, "-c", os.path.join(CSRC, src), "-o", obj])
-        objs.append(obj)
+    objs = [os.path.join(BUILD, src.replace(".hip", ".o")) for src in SOURCES]
+    jobs = jobs or min(len(SOURCES), max(1, (os.cpu_count() or 1)), 16)
+    with ThreadPoolExecutor(jobs) as ex:
+        for f in [ex.submit(_run, [HIPCC, *FLAGS, "-c", os.path.join(CSRC, src), "-o", obj])
+                  for src, obj in zip(SOURCES, objs)]:
+            f.result()
     tmp = LIB + ".tmp"
     _run([HIPCC, "--offload-arch=gfx950", "-shared", "-o", tmp, *objs])
     os.replace(tmp, LIB)
