@@ -36,7 +36,7 @@ extern "C" {
 #endif
 
 #define MPM_MAX_MATERIALS 16
-#define MPM_ABI_VERSION 5
+#define MPM_ABI_VERSION 6
 
 typedef enum mpm_status {
     MPM_OK = 0,
@@ -384,6 +384,22 @@ int32_t mpm_grid_update(const mpm_grid_t *G, const mpm_model_t *model, const mpm
                         const mpm_collider_t *colliders, int32_t ncoll, float *wrench,
                         int32_t wrench_slots, float *workspace, float dt, int32_t flags,
                         void *stream);
+/* Allocation-free variant of mpm_grid_update. No counterpart in the reference: it is for
+ * callers that cannot allocate device memory or pass a workspace (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc). The wrench partial rows
+ * live in a file-scope device array of MPM_NOALLOC_SCRATCH_FLOATS floats (3 MiB) that each
+ * GPU the library runs on holds until the process ends; its address is looked up on the
+ * first call on a device (the device current at the call; `stream` must belong to it).
+ * Same arguments and results as mpm_grid_update without `workspace`; MPM_ERR_SIZE when
+ * mpm_grid_workspace_floats(capacity, wrench_slots) exceeds the array, MPM_ERR_ARG with
+ * MPM_GRID_WRENCH_DEFER (deferred rows cannot outlive the call in a shared array). It
+ * allocates nothing. Every call on one GPU shares that GPU's array: calls that overlap
+ * on one GPU (other streams, other host threads) race. */
+#define MPM_NOALLOC_SCRATCH_FLOATS (1024 * 64 * 12)
+int32_t mpm_grid_update_noalloc(const mpm_grid_t *G, const mpm_model_t *model, const mpm_tick_ctl_t *ctl,
+                                const int32_t *block_list, const int32_t *block_count, int32_t capacity,
+                                const mpm_collider_t *colliders, int32_t ncoll, float *wrench,
+                                int32_t wrench_slots, float dt, int32_t flags, void *stream);
 /* Convert the integer wrench rows that MPM_GRID_FIXED_IN | MPM_GRID_WRENCH_DEFER calls left
  * in `workspace` into `wrench` (added) and clear them. No counterpart in the reference
  * (its wrench accumulators are float atomics, mpm_solver_warp.py:2121-2269). */

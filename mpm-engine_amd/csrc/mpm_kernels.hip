@@ -5,6 +5,8 @@
 // stream; nothing here allocates.
 #include <hip/hip_runtime.h>
 
+#include <mutex>
+
 #include "mpm_device.h"
 
 namespace {
@@ -627,6 +629,40 @@ inline int particles_ok(const mpm_particles_t* P) {
 }  // namespace
 
 // ======================================================================== C ABI
+// Scratch of mpm_grid_update_noalloc: the wrench partial rows that mpm_grid_update takes
+// as its `workspace`, held in a file-scope device array instead (each GPU the library runs
+// on holds its own copy until the process ends). In host code `scratch` names a host-side
+// stand-in, not device memory: kernels get the per-device address from scratch_ptr().
+static __device__ __align__(256) float scratch[MPM_NOALLOC_SCRATCH_FLOATS];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+float* g_scratch[kMaxDevices] = {};
+
+// Device address of `scratch` on the current device, looked up once per device with that
+// device current (hipGetSymbolAddress) and kept.
+int32_t scratch_ptr(float** out) {
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) return MPM_ERR_HIP;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch[dev]) {
+        void* p = nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return MPM_ERR_HIP;
+        g_scratch[dev] = static_cast<float*>(p);
+    }
+    *out = g_scratch[dev];
+    return MPM_OK;
+}
+
+__global__ void k_zero_u64(unsigned long long* __restrict__ p, int n) {
+    const int t = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (t < n) p[t] = 0ull;
+}
+
+}  // namespace
+
 extern "C" {
 
 int32_t mpm_abi_version(void) { return MPM_ABI_VERSION; }
@@ -720,6 +756,34 @@ int32_t mpm_grid_update(const mpm_grid_t* G, const mpm_model_t* model, const mpm
         hipLaunchKernelGGL(k_wrench_reduce, dim3(wrench_slots * 6), dim3(kThreads), 0, s, workspace, (int)nwg,
                            wrench_slots * 6, wrench);
     return launch_status();
+}
+
+int32_t mpm_grid_update_noalloc(const mpm_grid_t* G, const mpm_model_t* model, const mpm_tick_ctl_t* ctl,
+                                const int32_t* list, const int32_t* count, int32_t capacity,
+                                const mpm_collider_t* colliders, int32_t ncoll, float* wrench, int32_t wrench_slots,
+                                float dt, int32_t flags, void* stream) {
+    if (capacity < 0 || wrench_slots < 0) return MPM_ERR_ARG;
+    // the deferred integer rows would have to survive between calls in the shared array
+    if (flags & MPM_GRID_WRENCH_DEFER) return MPM_ERR_ARG;
+    const int64_t need = wrench_slots > kMaxSlots ? 12 * (int64_t)wrench_slots
+                                                  : (int64_t)grid_update_wgs(capacity) * wrench_slots * 12;
+    if (need > MPM_NOALLOC_SCRATCH_FLOATS) return MPM_ERR_SIZE;
+    float* ws = nullptr;
+    if (wrench_slots > 0) {
+        const int32_t st = scratch_ptr(&ws);
+        if (st != MPM_OK) return st;
+        // above kMaxSlots the fixed-point wrench is summed with integer atomics into one
+        // row, which must start at zero (the array keeps the last call's contents; a fill
+        // kernel, not a memset, so the call can be captured in a graph). Up to kMaxSlots
+        // every workgroup overwrites its own row.
+        if ((flags & MPM_GRID_FIXED_IN) && wrench_slots > kMaxSlots) {
+            const int n = 6 * wrench_slots;
+            hipLaunchKernelGGL(k_zero_u64, dim3((unsigned)((n + kThreads - 1) / kThreads)), dim3(kThreads), 0,
+                               (hipStream_t)stream, reinterpret_cast<unsigned long long*>(ws), n);
+        }
+    }
+    return mpm_grid_update(G, model, ctl, list, count, capacity, colliders, ncoll, wrench, wrench_slots, ws, dt,
+                           flags, stream);
 }
 
 int32_t mpm_grid_wrench_flush(const mpm_grid_t* G, int32_t capacity, float* wrench, int32_t wrench_slots,

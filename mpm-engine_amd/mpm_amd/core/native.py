@@ -157,13 +157,14 @@ SIGNATURES = {
     "mpm_modifiers_apply": (_I32, [_P, _P, _I32, _P, _F, _P]),
     "mpm_mesh_field": (_I32, [_P, _I32, _P, _I32, _P, _F, _I32, _I32, _P, _P, _P]),
     "mpm_segment_distance": (_I32, [_P, _I32, _P, _F, _I32, _P, _P]),
+    "mpm_grid_update_noalloc": (_I32, [_P, _P, _P, _P, _P, _I32, _P, _I32, _P, _I32, _F, _I32, _P]),
     "mpm_adjoint_g2p": (_I32, [_P, _P, _P, _P, _F, _P]),
     "mpm_adjoint_grid": (_I32, [_P, _P, _P, _I32, _P, _I32, _P, _F, _P]),
     "mpm_adjoint_p2g_stress": (_I32, [_P, _P, _P, _P, _F, _P]),
 }
 
 _LIB = None
-ABI_VERSION = 5  # MPM_ABI_VERSION of include/mpm_hip.h this binding is written for
+ABI_VERSION = 6  # MPM_ABI_VERSION of include/mpm_hip.h this binding is written for
 
 
 class NativeError(RuntimeError):

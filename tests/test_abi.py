@@ -29,7 +29,7 @@ def test_library_exports_every_declared_symbol():
 
 def test_struct_layouts_and_introspection():
     lib = native.load()
-    assert lib.mpm_abi_version() == 5
+    assert lib.mpm_abi_version() == native.ABI_VERSION
     assert lib.mpm_struct_size(1) == ctypes.sizeof(native.Grid)
     assert lib.mpm_struct_size(2) == native.MODEL_DTYPE.itemsize
     assert lib.mpm_struct_size(3) == native.COLLIDER_DTYPE.itemsize
