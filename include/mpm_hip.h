@@ -36,7 +36,7 @@ extern "C" {
 #endif
 
 #define MPM_MAX_MATERIALS 16
-#define MPM_ABI_VERSION 6
+#define MPM_ABI_VERSION 7
 
 typedef enum mpm_status {
     MPM_OK = 0,
@@ -99,6 +99,13 @@ typedef struct mpm_particles {
     float *x_ref;      /* 3 planes: body-frame reference position of rigid particles
                           (particle_x_ref), or NULL */
     int32_t *modifier_mask; /* bit k: particle selected by particle modifier k, or NULL */
+    const int32_t *chunk_start; /* device, *chunk_count + 1 entries (mpm_sort_chunks): chunk c of
+                                   the particle pass is particles [chunk_start[c], chunk_start[c + 1]),
+                                   at most 256 of them; NULL: chunks of 256 consecutive particles
+                                   (also with MPM_PASS_WG128, which uses 128) */
+    const int32_t *chunk_count; /* device int: chunks in chunk_start */
+    int32_t chunk_capacity;     /* chunks the pass launches, >= *chunk_count (with chunk_start) */
+    int32_t pad_;
 } mpm_particles_t;
 
 /* ---- CPIC thin-boundary (CDF) colliders ---------------------------------------------
@@ -153,7 +160,8 @@ typedef struct mpm_grid {
                              met a MPM_PASS_VISCOUS_ONLY pass, 8 = a scene id outside
                              [0, n_scenes) (that particle is skipped), 64 = a fixed-point
                              contribution beyond 2^51 units (deterministic scatter: the
-                             scale is too fine for this state) */
+                             scale is too fine for this state), 256 = more chunks than
+                             P->chunk_capacity (the chunks beyond were not processed) */
     int32_t n_scenes;     /* independent scenes sharing the buffers (>= 1): scene s owns
                              blocks [s * B, (s + 1) * B), B = nb[0] * nb[1] * nb[2];
                              block codes in every block list are these global codes */
@@ -440,6 +448,18 @@ int64_t mpm_sort_temp_bytes(int32_t n);
 /* stable radix sort of (keys, vals) pairs into (keys_out, vals_out) */
 int32_t mpm_sort_pairs(const int32_t *keys, int32_t *keys_out, const int32_t *vals,
                        int32_t *vals_out, int32_t n, void *temp, int64_t temp_bytes, void *stream);
+/* Chunk boundaries for the particle pass over sorted keys (mpm_sort_keys: key_shift 0;
+ * mpm_sort_keys_cells: key_shift 6; nb = n0 / 4 + 2 as in the keys): the particles of each
+ * group of gz consecutive blocks of one block column (scene, bx, by, bz / gz) form a run in
+ * key order, and each run is cut into chunks of at most `chunk` particles. chunk_start gets
+ * *count + 1 entries (capacity n + 1; the last is n), *count (device int) the number of
+ * chunks. Fixed chunks of 256 consecutive particles can straddle a jump in key order, whose
+ * stencil box then spans two distant clusters and overflows the pass's LDS tile; these keep
+ * each box within gz blocks of one column. temp: mpm_sort_chunks_temp_bytes(n) bytes. No
+ * counterpart in the reference (its particle kernels launch one thread per particle). */
+int64_t mpm_sort_chunks_temp_bytes(int32_t n);
+int32_t mpm_sort_chunks(const int32_t *keys, int32_t n, int32_t key_shift, int32_t nb, int32_t gz, int32_t chunk,
+                        int32_t *chunk_start, int32_t *count, void *temp, int64_t temp_bytes, void *stream);
 /* gather every particle array by perm (dst[i] = src[perm[i]]) through `scratch`
  * (>= 9 * stride floats) and copy back in place, so device pointers stay stable. */
 int32_t mpm_permute(const mpm_particles_t *P, const int32_t *perm, float *scratch, void *stream);

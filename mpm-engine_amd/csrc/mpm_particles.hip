@@ -49,16 +49,21 @@ MPM_DEV bool stress_update_viscous(const m33& Ft, const m33& L, int mat, const m
     return false;
 }
 
+MPM_DEV float sel3(int i, const float* a) { return i == 0 ? a[0] : (i == 1 ? a[1] : a[2]); }
+
 // Tensor-product form of the 27-node gather: per (i, j) row sum over k, then over j,
 // then over i. Same sums as the reference's triple loop (v = sum w v_i, C = sum w v_i
 // (x_i - x_p)^T 4/dx^2, L = sum v_i grad w^T) with fewer products and live values.
-// fetch(i, j, k) returns the node velocity for stencil offsets (i, j, k).
-template <class Fetch>
+// fetch(i, j, k) returns the node velocity for stencil offsets (i, j, k). kRolled keeps
+// the loop over i rolled (the weights of row i picked by selects, which are exact): the
+// global-memory variant then holds 9 node loads in flight instead of 27 and fits the
+// register budget, and computes bit for bit what the unrolled LDS variant computes.
+template <bool kRolled = false, class Fetch>
 MPM_DEV void gather_tp(const Stencil& st, float inv_dx, Fetch fetch, v3& nv, m33& nC, m33& nF) {
     nv = mk3(0.f, 0.f, 0.f);
     v3 cX = nv, cY = nv, cZ = nv;  // columns of C (by dpos component)
     v3 fX = nv, fY = nv, fZ = nv;  // columns of L (by gradient component)
-#pragma unroll
+#pragma unroll(kRolled ? 1 : 3)
     for (int i = 0; i < 3; ++i) {
         v3 Ai = mk3(0.f, 0.f, 0.f), Ay = Ai, Bi = Ai, Cz = Ai, Dy = Ai;
 #pragma unroll
@@ -78,12 +83,13 @@ MPM_DEV void gather_tp(const Stencil& st, float inv_dx, Fetch fetch, v3& nv, m33
             Cz = Cz + Z * wj;
             Dy = Dy + A * st.dw[1][j];
         }
-        const float wi = st.w[0][i];
+        const float wi = kRolled ? sel3(i, st.w[0]) : st.w[0][i];
+        const float dwi = kRolled ? sel3(i, st.dw[0]) : st.dw[0][i];
         nv = nv + Ai * wi;
         cX = cX + Ai * (wi * ((float)i - st.fx[0]));
         cY = cY + Ay * wi;
         cZ = cZ + Cz * wi;
-        fX = fX + Ai * st.dw[0][i];
+        fX = fX + Ai * dwi;
         fY = fY + Dy * wi;
         fZ = fZ + Bi * wi;
     }
@@ -96,71 +102,126 @@ MPM_DEV void gather_tp(const Stencil& st, float inv_dx, Fetch fetch, v3& nv, m33
     nF.a[6] = fX.z * inv_dx; nF.a[7] = fY.z * inv_dx; nF.a[8] = fZ.z * inv_dx;
 }
 
-MPM_DEV float sel3(int i, const float* a) { return i == 0 ? a[0] : (i == 1 ? a[1] : a[2]); }
-
-// Global-memory gather / scatter for chunks whose stencil box does not fit the LDS arena
-// (unsorted or spread-out particles): a rolled loop over the 27 nodes with the weights
-// picked by selects, so this rare path stays small next to the LDS fast path it shares a
-// kernel (and a register budget) with.
-template <class Subst>
-MPM_DEV void gather_global(const mpm_grid_t& G, size_t soff, const Stencil& st, bool periodic, v3& nv, m33& nC,
-                           m33& nF, Subst subst) {
-    const float4* __restrict__ out = reinterpret_cast<const float4*>(G.node_out) + soff;
-    nv = mk3(0.f, 0.f, 0.f);
-    nC = mzero();
-    nF = mzero();
-    const float c4 = G.inv_dx * 4.0f;
-#pragma unroll 1
-    for (int n = 0; n < 27; ++n) {
-        const int i = n / 9, j = (n / 3) % 3, k = n % 3;
-        int ix = st.base[0] + i;
-        if (periodic) ix = ((ix % G.n[0]) + G.n[0]) % G.n[0];
-        const int iy = st.base[1] + j, iz = st.base[2] + k;
-        if (!in_grid(G, ix, iy, iz)) continue;
-        const float4 o4 = out[node_index(G, ix, iy, iz)];
-        const v3 g4 = subst(i, j, k, mk3(o4.x, o4.y, o4.z));
-        const float w0 = sel3(i, st.w[0]), w1 = sel3(j, st.w[1]), w2 = sel3(k, st.w[2]);
-        const float d0 = sel3(i, st.dw[0]), d1 = sel3(j, st.dw[1]), d2 = sel3(k, st.dw[2]);
-        const float w = w0 * w1 * w2;
-        const float g[3] = {g4.x, g4.y, g4.z};
-        const float dp[3] = {((float)i - st.fx[0]) * w * c4, ((float)j - st.fx[1]) * w * c4,
-                             ((float)k - st.fx[2]) * w * c4};
-        const float dw[3] = {d0 * w1 * w2 * G.inv_dx, w0 * d1 * w2 * G.inv_dx, w0 * w1 * d2 * G.inv_dx};
-        nv = nv + mk3(g4.x, g4.y, g4.z) * w;
-#pragma unroll
-        for (int r = 0; r < 3; ++r)
-#pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                nC.a[3 * r + c] += g[r] * dp[c];
-                nF.a[3 * r + c] += g[r] * dw[c];
-            }
-    }
-}
-
-// Deterministic scatter: x in fixed point of scale S (a power of two, so x * S is exact
-// in fp32), rounded to the nearest unit (ties to even). Integer sums do not depend on the
-// order the atomics land in. Beyond 2^51 units (or NaN) the contribution is dropped and
-// error bit 64 raised: 2^12 such contributions to one node still fit an int64.
-// The rounding goes through float64: y is exact there, and y + 1.5 * 2^52 lies in the
-// binade [2^52, 2^53), whose spacing is 1, so the add rounds y to an integer and the
-// difference of the two bit patterns is that integer (for |y| < 2^51). Four instructions
-// instead of the generic float -> int64 conversion, 108 times per particle.
-MPM_DEV unsigned long long fix_q(float x, float S, int& err) {
-    const float y = x * S;
-    if (!(fabsf(y) < 2251799813685248.0f)) {
-        err |= 64;
-        return 0ull;
-    }
+// Deterministic scatter: every contribution in fixed point of scale S (a power of two),
+// rounded to the nearest unit (ties to even). Integer sums do not depend on the order the
+// atomics land in. The rounding goes through float64: y is exact there, and y + 1.5 *
+// 2^52 lies in the binade [2^52, 2^53), whose spacing is 1, so the add rounds y to an
+// integer and the difference of the two bit patterns is that integer (for |y| < 2^51;
+// 2^12 such contributions to one node still fit an int64). Three instructions instead of
+// the generic float -> int64 conversion, 108 times per particle.
+MPM_DEV unsigned long long fix_round(float y) {
     const double magic = 6755399441055744.0;  // 1.5 * 2^52
     return (unsigned long long)(__double_as_longlong((double)y + magic) - __double_as_longlong(magic));
 }
 
+// Fixed-point scatter of one particle in tensor-product form. The reference's momentum
+// contribution to stencil node (i, j, k) (p2g_particle, mpm_utils.py:880-892),
+//   w m (v + C dpos) + fs grad w,   w = w_i w_j w_k,  dpos = ((i, j, k) - f) dx,
+//   grad w = (dw_i w_j w_k, w_i dw_j w_k, w_i w_j dw_k) / dx,  fs = -vol dt tau,
+// regroups as
+//   w (a + b0 i + b1 j + b2 k) + g0 dw_i w_j w_k + g1 w_i dw_j w_k + g2 w_i w_j dw_k
+// with a = m (v - C f dx), b_c = m dx C e_c, g_c = fs e_c / dx, evaluated row by row:
+// about ten operations per node instead of the ~60 of the node-by-node form. The
+// coefficients carry the fixed-point unit S (a power of two, so scaling them is exact),
+// and one bound per particle replaces the per-contribution range test: with |w| < 1 and
+// |dw| <= 1 every contribution is below |a| + 2 sum_c |b_c| + sum_c |g_c| per component.
+// Beyond 2^51 units (or NaN) the particle adds nothing and false is returned (error bit
+// 64). add(i, j, k, q_momentum[3], q_mass) receives the rounded integers.
+template <class Add>
+MPM_DEV bool scatter_tp_fix(const Stencil& st, float dx, float inv_dx, float mass, v3 v, const m33& Cm,
+                            const m33& fs, float Sm, float Sp, int rot, Add add) {
+    const float cb = mass * dx * Sp, cg = inv_dx * Sp;
+    v3 b[3], g[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        b[c] = mk3(Cm.a[c], Cm.a[3 + c], Cm.a[6 + c]) * cb;
+        g[c] = mk3(fs.a[c], fs.a[3 + c], fs.a[6 + c]) * cg;
+    }
+    const v3 a = v * (mass * Sp) - (b[0] * st.fx[0] + b[1] * st.fx[1] + b[2] * st.fx[2]);
+    const v3 bnd = mk3(fabsf(a.x), fabsf(a.y), fabsf(a.z)) +
+                   2.0f * (mk3(fabsf(b[0].x) + fabsf(b[1].x) + fabsf(b[2].x), fabsf(b[0].y) + fabsf(b[1].y) + fabsf(b[2].y),
+                               fabsf(b[0].z) + fabsf(b[1].z) + fabsf(b[2].z))) +
+                   mk3(fabsf(g[0].x) + fabsf(g[1].x) + fabsf(g[2].x), fabsf(g[0].y) + fabsf(g[1].y) + fabsf(g[2].y),
+                       fabsf(g[0].z) + fabsf(g[1].z) + fabsf(g[2].z));
+    const float ms = mass * Sm;
+    const float lim = 2251799813685248.0f;  // 2^51
+    if (!(fmaxf(bnd.x, fmaxf(bnd.y, bnd.z)) < lim) || !(fabsf(ms) < lim)) return false;
+    // Lanes whose particles share a stencil cell (about 8 consecutive lanes after a cell
+    // sort) target the same 27 nodes; each lane starts its j and k loops at its own
+    // offset (rot % 3, rot / 3 % 3), so the same instruction of such lanes hits up to 9
+    // different nodes instead of one LDS address. The rotated weights are picked once per
+    // particle.
+    const int j0 = rot % 3, k0 = (rot / 3) % 3;
+    int jr[3], kr[3];
+    float wjr[3], djr[3], wkr[3], dkr[3];
+#pragma unroll
+    for (int t = 0; t < 3; ++t) {
+        const int j = j0 + t >= 3 ? j0 + t - 3 : j0 + t, k = k0 + t >= 3 ? k0 + t - 3 : k0 + t;
+        jr[t] = j;
+        kr[t] = k;
+        wjr[t] = sel3(j, st.w[1]);
+        djr[t] = sel3(j, st.dw[1]);
+        wkr[t] = sel3(k, st.w[2]);
+        dkr[t] = sel3(k, st.dw[2]);
+    }
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        const float wi = st.w[0][i];
+        const v3 Pi = (a + b[0] * (float)i) * wi + g[0] * st.dw[0][i];
+        const v3 Qi = b[1] * wi, Ri = g[1] * wi, Ti = b[2] * wi, Ui = g[2] * wi;
+        const float mi = ms * wi;
+#pragma unroll
+        for (int tj = 0; tj < 3; ++tj) {
+            const float wj = wjr[tj];
+            const v3 Pij = (Pi + Qi * (float)jr[tj]) * wj + Ri * djr[tj];
+            const v3 Tij = Ti * wj, Uij = Ui * wj;
+            const float mij = mi * wj;
+#pragma unroll
+            for (int tk = 0; tk < 3; ++tk) {
+                const v3 mom = (Pij + Tij * (float)kr[tk]) * wkr[tk] + Uij * dkr[tk];
+                add(i, jr[tj], kr[tk], fix_round(mom.x), fix_round(mom.y), fix_round(mom.z),
+                    fix_round(mij * wkr[tk]));
+            }
+        }
+    }
+    return true;
+}
+
+// Global-memory scatter for chunks whose stencil box does not fit the LDS arena
+// (unsorted or spread-out particles): a rolled loop over the 27 nodes with the weights
+// picked by selects (the fixed-point contributions come from scatter_tp_fix, as in the
+// LDS path), so this rare path stays small next to the LDS fast path it shares a kernel
+// (and a register budget) with.
 template <bool kDet, class Mask>
 MPM_DEV void scatter_global(const mpm_grid_t& G, size_t soff, const Stencil& st, bool periodic, float mass, v3 v,
                             const m33& Cm, const m33& fs, Mask mask, int& err) {
     float* __restrict__ gin = G.node_in + 4 * soff;
-    unsigned long long* __restrict__ gfix = reinterpret_cast<unsigned long long*>(G.node_fix) + 4 * soff;
     size_t last_blk = ~(size_t)0;  // block this particle marked last
+    if constexpr (kDet) {  // the same contributions as the LDS path, bit for bit
+        unsigned long long* __restrict__ gfix = reinterpret_cast<unsigned long long*>(G.node_fix) + 4 * soff;
+        const bool ok = scatter_tp_fix(st, G.dx, G.inv_dx, mass, v, Cm, fs, G.fix_scale[0], G.fix_scale[1], 0,
+                                       [&](int i, int j, int k, unsigned long long q0, unsigned long long q1,
+                                           unsigned long long q2, unsigned long long q3) {
+            int ix = st.base[0] + i;
+            if (periodic) ix = ((ix % G.n[0]) + G.n[0]) % G.n[0];
+            const int iy = st.base[1] + j, iz = st.base[2] + k;
+            if (!in_grid(G, ix, iy, iz)) return;
+            const size_t o = 4 * (size_t)node_index(G, ix, iy, iz);
+            if (G.block_touch) {
+                const size_t blk = (soff + o / 4) >> 6;
+                if (blk != last_blk) {
+                    last_blk = blk;
+                    G.block_touch[blk] = 3;
+                }
+            }
+            atomicAdd(gfix + o + 0, q0);
+            atomicAdd(gfix + o + 1, q1);
+            atomicAdd(gfix + o + 2, q2);
+            atomicAdd(gfix + o + 3, q3);
+        });
+        if (!ok) err |= 64;
+        return;
+    }
 #pragma unroll 1
     for (int n = 0; n < 27; ++n) {
         const int i = n / 9, j = (n / 3) % 3, k = n % 3;
@@ -184,18 +245,10 @@ MPM_DEV void scatter_global(const mpm_grid_t& G, size_t soff, const Stencil& st,
                 G.block_touch[blk] = 3;
             }
         }
-        if constexpr (kDet) {
-            const float Sm = G.fix_scale[0], Sp = G.fix_scale[1];
-            atomicAdd(gfix + o + 0, fix_q(add.x, Sp, err));
-            atomicAdd(gfix + o + 1, fix_q(add.y, Sp, err));
-            atomicAdd(gfix + o + 2, fix_q(add.z, Sp, err));
-            atomicAdd(gfix + o + 3, fix_q(wmv, Sm, err));
-        } else {
-            atomicAdd(gin + o + 0, add.x);
-            atomicAdd(gin + o + 1, add.y);
-            atomicAdd(gin + o + 2, add.z);
-            atomicAdd(gin + o + 3, wmv);
-        }
+        atomicAdd(gin + o + 0, add.x);
+        atomicAdd(gin + o + 1, add.y);
+        atomicAdd(gin + o + 2, add.z);
+        atomicAdd(gin + o + 3, wmv);
     }
 }
 
@@ -280,12 +333,19 @@ __global__ __launch_bounds__(kT) __attribute__((amdgpu_waves_per_eu(MPM_PASS_WAV
 
     const int S = P.stride;
     const int group = (int)(blockIdx.x % kXcds) * xcd_q + (int)(blockIdx.x / kXcds);
+    // chunks: group-aligned runs of sorted particles (chunk_start), or kT consecutive ones
+    const bool chunked = P.chunk_start != nullptr;
+    const int nchunks = chunked ? *P.chunk_count : (P.n + kT - 1) / kT;
+    if (chunked && blockIdx.x == 0 && threadIdx.x == 0 && nchunks > P.chunk_capacity && G.error_flag)
+        atomicOr(G.error_flag, 256);
     for (int it = 0; it < ppt; ++it) {
         const int chunk = group * ppt + it;
-        if (chunk * kT >= P.n) return;  // past the last chunk (uniform exit before any barrier)
+        if (chunk >= nchunks) return;  // past the last chunk (uniform exit before any barrier)
         if (it > 0) __syncthreads();    // the previous chunk's LDS reads are done
-        const int p = chunk * kT + threadIdx.x;
-        const bool sel = p < P.n && P.selection[p] == 0;
+        const int p0 = chunked ? P.chunk_start[chunk] : chunk * kT;
+        const int p1 = chunked ? P.chunk_start[chunk + 1] : min(p0 + kT, P.n);
+        const int p = p0 + threadIdx.x;
+        const bool sel = p < p1 && P.selection[p] == 0;
         const int scene = (sel && P.scene) ? P.scene[p] : 0;
         const bool alive = sel && (unsigned)scene < (unsigned)G.n_scenes;  // bad scene ids: error bit 8
         const int mat = alive ? P.material[p] : 0;
@@ -348,7 +408,22 @@ __global__ __launch_bounds__(kT) __attribute__((amdgpu_waves_per_eu(MPM_PASS_WAV
                               },
                               nv, nC, nF);
                 } else {
-                    gather_global(G, soff, st, periodic, nv, nC, nF, subst);
+                    // the same tensor-product sums, fed from global memory: a particle's
+                    // gather is bitwise the same whichever path its chunk takes
+                    const float4* __restrict__ gout = reinterpret_cast<const float4*>(G.node_out) + soff;
+                    gather_tp<true>(st, G.inv_dx,
+                              [&](int i, int j, int k) {
+                                  int ix = st.base[0] + i;
+                                  if (periodic) ix = ((ix % G.n[0]) + G.n[0]) % G.n[0];
+                                  const int iy = st.base[1] + j, iz = st.base[2] + k;
+                                  v3 g = mk3(0.f, 0.f, 0.f);
+                                  if (in_grid(G, ix, iy, iz)) {
+                                      const float4 o4 = gout[node_index(G, ix, iy, iz)];
+                                      g = mk3(o4.x, o4.y, o4.z);
+                                  }
+                                  return subst(i, j, k, g);
+                              },
+                              nv, nC, nF);
                 }
                 v = nv;
                 Cm = nC;
@@ -531,6 +606,21 @@ __global__ __launch_bounds__(kT) __attribute__((amdgpu_waves_per_eu(MPM_PASS_WAV
                         }
                     }
                 });
+            } else if (alive && kDet && ts.lds) {
+                // tile address of node (i, j, k): row base per (i, j), k in the offset
+                const int d2 = ts.d2, d12 = ts.d1 * ts.d2;
+                const int r0 = ts.base + (o0 * ts.d1 + o1) * d2 + o2;
+                const bool ok = scatter_tp_fix(st, G.dx, G.inv_dx, mass, v, Cm, fs, G.fix_scale[0], G.fix_scale[1],
+                                               lane, [&](int i, int j, int k, unsigned long long q0, unsigned long long q1,
+                                                   unsigned long long q2, unsigned long long q3) {
+                    const int a = r0 + i * d12 + j * d2 + k;
+                    unsigned long long* sc = reinterpret_cast<unsigned long long*>(&s_scatter[0][0]);
+                    atomicAdd(sc + a, q0);
+                    atomicAdd(sc + kArena + a, q1);
+                    atomicAdd(sc + 2 * kArena + a, q2);
+                    atomicAdd(sc + 3 * kArena + a, q3);
+                });
+                if (!ok) err |= 64;
             } else if (alive) {
                 if (ts.lds) {
                     const int d1 = ts.d1, d2 = ts.d2, b0 = ts.base;
@@ -538,18 +628,10 @@ __global__ __launch_bounds__(kT) __attribute__((amdgpu_waves_per_eu(MPM_PASS_WAV
                                     [&](int i, int j, int k, v3 add, float wm) {
                                         if (!mask(i, j, k, add, wm)) return;
                                         const int a = b0 + ((o0 + i) * d1 + (o1 + j)) * d2 + (o2 + k);
-                                        if constexpr (kDet) {
-                                            const float Sm = G.fix_scale[0], Sp = G.fix_scale[1];
-                                            atomicAdd(&s_scatter[0][a], fix_q(add.x, Sp, err));
-                                            atomicAdd(&s_scatter[1][a], fix_q(add.y, Sp, err));
-                                            atomicAdd(&s_scatter[2][a], fix_q(add.z, Sp, err));
-                                            atomicAdd(&s_scatter[3][a], fix_q(wm, Sm, err));
-                                        } else {
-                                            atomicAdd(&s_scatter[0][a], add.x);
-                                            atomicAdd(&s_scatter[1][a], add.y);
-                                            atomicAdd(&s_scatter[2][a], add.z);
-                                            atomicAdd(&s_scatter[3][a], wm);
-                                        }
+                                        atomicAdd(&s_scatter[0][a], add.x);
+                                        atomicAdd(&s_scatter[1][a], add.y);
+                                        atomicAdd(&s_scatter[2][a], add.z);
+                                        atomicAdd(&s_scatter[3][a], wm);
                                     });
                 } else {
                     scatter_global<kDet>(G, soff, st, periodic, mass, v, Cm, fs, mask, err);
@@ -611,7 +693,13 @@ extern "C" int32_t mpm_particle_pass(const mpm_particles_t* P, const mpm_grid_t*
     const int wg = (!cdf && !det && (mode & MPM_PASS_WG128)) ? 128 : 256;
     const int ppt = (mode & MPM_PASS_PPT4) ? 4 : ((mode & MPM_PASS_PPT2) ? 2 : 1);
     hipStream_t s = (hipStream_t)stream;
-    const int nchunks = (P->n + wg - 1) / wg;
+    // chunk_start holds chunks of up to 256 particles: used by 256-lane workgroups only
+    mpm_particles_t Pl = *P;
+    if (wg != 256 || (Pl.chunk_start && (!Pl.chunk_count || Pl.chunk_capacity < 1))) {
+        if (Pl.chunk_start && wg == 256) return MPM_ERR_ARG;
+        Pl.chunk_start = nullptr;
+    }
+    const int nchunks = Pl.chunk_start ? Pl.chunk_capacity : (P->n + wg - 1) / wg;
     const int ngroups = (nchunks + ppt - 1) / ppt;
     const int q = (ngroups + kXcds - 1) / kXcds;
     const dim3 grid((unsigned)(q * kXcds)), block(wg);
@@ -619,9 +707,9 @@ extern "C" int32_t mpm_particle_pass(const mpm_particles_t* P, const mpm_grid_t*
     const int m = mode & (MPM_PASS_G2P | MPM_PASS_STRESS | MPM_PASS_P2G | MPM_PASS_LEAN | MPM_PASS_KEEP_STRESS);
 #define LAUNCH(M, T, C, D)                                                                                     \
     if (solid)                                                                                                 \
-        hipLaunchKernelGGL((k_particle_pass<(M), true, T, C, D>), grid, block, 0, s, *P, *G, model, dt, q, ppt); \
+        hipLaunchKernelGGL((k_particle_pass<(M), true, T, C, D>), grid, block, 0, s, Pl, *G, model, dt, q, ppt); \
     else                                                                                                       \
-        hipLaunchKernelGGL((k_particle_pass<(M), false, T, C, D>), grid, block, 0, s, *P, *G, model, dt, q, ppt);
+        hipLaunchKernelGGL((k_particle_pass<(M), false, T, C, D>), grid, block, 0, s, Pl, *G, model, dt, q, ppt);
 #define PLAIN(M)                         \
     if (wg == 128) {                     \
         LAUNCH(M, 128, false, false)     \

@@ -29,3 +29,111 @@ int32_t mpm_sort_pairs(const int32_t* keys, int32_t* keys_out, const int32_t* va
 }
 
 }  // extern "C"
+
+// ------------------------------------------------------------------ particle-pass chunks
+// Chunk boundaries over block-sorted particles: runs of particles that share a group of gz
+// consecutive blocks of one block column (scene, bx, by, bz / gz), each cut into chunks of
+// at most `chunk` particles. A fixed split into 256 consecutive particles lets a chunk
+// straddle a jump in key order (the end of one block column and the start of the next):
+// its stencil box then spans two distant clusters, overflows the particle pass's LDS tile
+// and sends the whole chunk through global memory; group-aligned chunks keep every box
+// within a column segment of gz blocks.
+namespace {
+
+constexpr int kSortThreads = 256;
+
+__device__ inline int group_of(int key, int shift, int nb, int gz) {
+    const int bk = key >> shift;                  // scene * nb^3 + (bx * nb + by) * nb + bz
+    const int col = bk / nb, bz = bk - col * nb;  // col = scene * nb^2 + bx * nb + by
+    return col * ((nb + gz - 1) / gz) + bz / gz;
+}
+
+__global__ void k_group_flags(const int32_t* __restrict__ keys, int n, int shift, int nb, int gz,
+                              int32_t* __restrict__ flags) {
+    const int p = blockIdx.x * kSortThreads + threadIdx.x;
+    if (p >= n) return;
+    flags[p] = (p == 0 || group_of(keys[p], shift, nb, gz) != group_of(keys[p - 1], shift, nb, gz)) ? 1 : 0;
+}
+
+// gs[g] = first particle of group g; gs[G] = n and *ng = G (written by the last thread)
+__global__ void k_group_starts(const int32_t* __restrict__ flags, const int32_t* __restrict__ gidx, int n,
+                               int32_t* __restrict__ gs, int32_t* __restrict__ ng) {
+    const int p = blockIdx.x * kSortThreads + threadIdx.x;
+    if (p >= n) return;
+    if (flags[p]) gs[gidx[p]] = p;
+    if (p == n - 1) {
+        const int G = gidx[p] + flags[p];
+        gs[G] = n;
+        *ng = G;
+    }
+}
+
+__global__ void k_group_chunks(const int32_t* __restrict__ gs, const int32_t* __restrict__ ng, int n, int chunk,
+                               int32_t* __restrict__ nck) {
+    const int g = blockIdx.x * kSortThreads + threadIdx.x;
+    if (g >= n) return;
+    nck[g] = g < *ng ? (gs[g + 1] - gs[g] + chunk - 1) / chunk : 0;
+}
+
+__global__ void k_chunk_fill(const int32_t* __restrict__ gs, const int32_t* __restrict__ nck,
+                             const int32_t* __restrict__ coff, const int32_t* __restrict__ ng, int n, int chunk,
+                             int32_t* __restrict__ chunk_start, int32_t* __restrict__ count) {
+    const int g = blockIdx.x * kSortThreads + threadIdx.x;
+    const int G = *ng;
+    if (g >= G) return;
+    for (int k = 0; k < nck[g]; ++k) chunk_start[coff[g] + k] = gs[g] + k * chunk;
+    if (g == G - 1) {
+        const int total = coff[g] + nck[g];
+        chunk_start[total] = n;
+        *count = total;
+    }
+}
+
+size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+}  // namespace
+
+extern "C" {
+
+int64_t mpm_sort_chunks_temp_bytes(int32_t n) {
+    if (n <= 0) return 0;
+    size_t scan = 0;
+    if (hipcub::DeviceScan::ExclusiveSum(nullptr, scan, (const int32_t*)nullptr, (int32_t*)nullptr, n) != hipSuccess)
+        return -1;
+    // scan temp, then flags, gidx, gs (n + 1), nck, coff, ng
+    return (int64_t)(align256(scan) + 5 * align256(sizeof(int32_t) * ((size_t)n + 1)) + 256);
+}
+
+int32_t mpm_sort_chunks(const int32_t* keys, int32_t n, int32_t key_shift, int32_t nb, int32_t gz, int32_t chunk,
+                        int32_t* chunk_start, int32_t* count, void* temp, int64_t temp_bytes, void* stream) {
+    if (n < 0 || (n > 0 && (!keys || !chunk_start || !count || !temp)) || nb <= 0 || gz <= 0 || chunk <= 0 ||
+        (key_shift != 0 && key_shift != 6))
+        return MPM_ERR_ARG;
+    if (n == 0) return MPM_OK;
+    if (temp_bytes < mpm_sort_chunks_temp_bytes(n)) return MPM_ERR_SIZE;
+    hipStream_t s = (hipStream_t)stream;
+    size_t scan = 0;
+    if (hipcub::DeviceScan::ExclusiveSum(nullptr, scan, (const int32_t*)nullptr, (int32_t*)nullptr, n) != hipSuccess)
+        return MPM_ERR_HIP;
+    char* base = static_cast<char*>(temp);
+    const size_t arr = align256(sizeof(int32_t) * ((size_t)n + 1));
+    void* cub_tmp = base;
+    int32_t* flags = reinterpret_cast<int32_t*>(base + align256(scan));
+    int32_t* gidx = reinterpret_cast<int32_t*>(base + align256(scan) + arr);
+    int32_t* gs = reinterpret_cast<int32_t*>(base + align256(scan) + 2 * arr);
+    int32_t* nck = reinterpret_cast<int32_t*>(base + align256(scan) + 3 * arr);
+    int32_t* coff = reinterpret_cast<int32_t*>(base + align256(scan) + 4 * arr);
+    int32_t* ng = reinterpret_cast<int32_t*>(base + align256(scan) + 5 * arr);
+    const dim3 grid((unsigned)((n + kSortThreads - 1) / kSortThreads)), block(kSortThreads);
+    hipLaunchKernelGGL(k_group_flags, grid, block, 0, s, keys, n, key_shift, nb, gz, flags);
+    size_t b = scan;
+    if (hipcub::DeviceScan::ExclusiveSum(cub_tmp, b, flags, gidx, n, s) != hipSuccess) return MPM_ERR_HIP;
+    hipLaunchKernelGGL(k_group_starts, grid, block, 0, s, flags, gidx, n, gs, ng);
+    hipLaunchKernelGGL(k_group_chunks, grid, block, 0, s, gs, ng, n, chunk, nck);
+    b = scan;
+    if (hipcub::DeviceScan::ExclusiveSum(cub_tmp, b, nck, coff, n, s) != hipSuccess) return MPM_ERR_HIP;
+    hipLaunchKernelGGL(k_chunk_fill, grid, block, 0, s, gs, nck, coff, ng, n, chunk, chunk_start, count);
+    return hipGetLastError() == hipSuccess ? MPM_OK : MPM_ERR_HIP;
+}
+
+}  // extern "C"

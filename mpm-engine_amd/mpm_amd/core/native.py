@@ -23,7 +23,7 @@ VISCOUS_IDS = frozenset((6, 7, 8, 10, 12))
 GRID_ZERO_IN, GRID_FIXED_IN, GRID_WRENCH_DEFER = 1, 2, 4
 MAX_FIXED_ORDER_SLOTS = 64  # wrench slots reduced in fixed order; more use float atomics
 ERR_STENCIL, ERR_NONFINITE, ERR_SOLID_IN_VISCOUS, ERR_SCENE = 1, 2, 4, 8
-ERR_ADJ_MATERIAL, ERR_ADJ_COLLIDER, ERR_FIXED_RANGE = 16, 32, 64
+ERR_ADJ_MATERIAL, ERR_ADJ_COLLIDER, ERR_FIXED_RANGE, ERR_CHUNKS = 16, 32, 64, 256
 COLLIDER_PLANE, COLLIDER_BOX, COLLIDER_CUP, COLLIDER_WALLS, COLLIDER_SDF, COLLIDER_POINTCLOUD = \
     1, 2, 3, 4, 5, 6
 
@@ -36,7 +36,8 @@ class Particles(C.Structure):
                 ("stress", _FP), ("L", _FP), ("vol", _FP), ("mass", _FP), ("mu", _FP),
                 ("lam", _FP), ("yield_stress", _FP), ("Jp", _FP), ("material", _FP),
                 ("selection", _FP), ("cov", _FP), ("scene", _FP), ("cdf_tag", _FP),
-                ("rigid_id", _FP), ("x_ref", _FP), ("modifier_mask", _FP)]
+                ("rigid_id", _FP), ("x_ref", _FP), ("modifier_mask", _FP),
+                ("chunk_start", _FP), ("chunk_count", _FP), ("chunk_capacity", C.c_int32), ("pad_", C.c_int32)]
 
 
 # mpm_rigid_body_t / mpm_rigid_plane_t as numpy record types (device arrays of them)
@@ -140,6 +141,8 @@ SIGNATURES = {
     "mpm_sort_temp_bytes": (_I64, [_I32]),
     "mpm_sort_pairs": (_I32, [_P, _P, _P, _P, _I32, _P, _I64, _P]),
     "mpm_permute": (_I32, [_P, _P, _P, _P]),
+    "mpm_sort_chunks_temp_bytes": (_I64, [_I32]),
+    "mpm_sort_chunks": (_I32, [_P, _I32, _I32, _I32, _I32, _I32, _P, _P, _P, _I64, _P]),
     "mpm_export_R": (_I32, [_P, _P, _P]),
     "mpm_export_cov": (_I32, [_P, _P, _P, _P]),
     "mpm_grid_to_dense": (_I32, [_P, _P, _P, _P, _P]),
@@ -164,7 +167,7 @@ SIGNATURES = {
 }
 
 _LIB = None
-ABI_VERSION = 6  # MPM_ABI_VERSION of include/mpm_hip.h this binding is written for
+ABI_VERSION = 7  # MPM_ABI_VERSION of include/mpm_hip.h this binding is written for
 
 
 class NativeError(RuntimeError):

@@ -205,6 +205,10 @@ class HipSimulator:
         # deterministic scatter (fixed-point integer accumulation), off by default
         self.deterministic = False
         self.node_fix = None
+        # particle-pass chunks (mpm_sort_chunks), computed at every sort; None until then
+        self.chunk_start = None
+        self.chunk_count = None
+        self.chunk_capacity = 0
         self._vmax = 0.0
         self._fix_exp = None
         self._structs()
@@ -222,6 +226,9 @@ class HipSimulator:
         P.rigid_id = native.ptr(self.rigid_id)
         P.x_ref = native.ptr(self.x_ref)
         P.modifier_mask = native.ptr(self.modifier_mask)
+        P.chunk_start = native.ptr(self.chunk_start)
+        P.chunk_count = native.ptr(self.chunk_count)
+        P.chunk_capacity = self.chunk_capacity
         G = native.Grid()
         G.node_in, G.node_out = native.ptr(self.node_in), native.ptr(self.node_out)
         G.n[:] = [self.n_grid] * 3
@@ -639,6 +646,7 @@ class HipSimulator:
         native.check(keyfn(self.pP, self.pG, native.ptr(keys), native.ptr(vals), native.ptr(flag), s),
                      "sort_keys")
         if int(flag.item()) == 0:
+            self._chunks(keys, cells)
             return False
         tb = int(self.lib.mpm_sort_temp_bytes(n))
         temp = torch.empty(max(tb, 1), dtype=torch.uint8, device=dev)
@@ -654,7 +662,34 @@ class HipSimulator:
         if self.init_cov is not None:
             self.init_cov.copy_(self.init_cov[pl])
         self._last_perm = pl
+        self._chunks(keys_o, cells)
         return True
+
+    def _chunks(self, sorted_keys, cells: bool):
+        """Particle-pass chunks over the sorted particles (mpm_sort_chunks): runs of at most
+        256 particles within 4 consecutive blocks of one block column. The chunk count is
+        read back (the sort already waits for its unsorted flag) to size the launch."""
+        torch = self.torch
+        n = self.n_particles
+        if self.chunk_start is None:
+            self.chunk_start = torch.zeros(n + 1, dtype=torch.int32, device=self.device)
+            self.chunk_count = torch.zeros(1, dtype=torch.int32, device=self.device)
+        tb = int(self.lib.mpm_sort_chunks_temp_bytes(n))
+        temp = torch.empty(max(tb, 1), dtype=torch.uint8, device=self.device)
+        nb = self.n_grid // 4 + 2
+        native.check(self.lib.mpm_sort_chunks(native.ptr(sorted_keys), n, 6 if cells else 0, nb, 4, 256,
+                                              native.ptr(self.chunk_start), native.ptr(self.chunk_count),
+                                              native.ptr(temp), tb, self.stream()), "sort_chunks")
+        count = int(self.chunk_count.item())
+        # launch capacity in steps of 64 chunks, kept while it fits (a new capacity means a
+        # new captured graph)
+        cap = self.chunk_capacity
+        if count > cap or cap - count > 256:
+            cap = (count + 63) // 64 * 64
+        if cap != self.chunk_capacity:
+            self.chunk_capacity = cap
+            self._graph = None
+            self._structs()
 
     # --------------------------------------------------------------- ticks
     def _ensure_sched(self, nbytes: int):
@@ -761,6 +796,7 @@ class HipSimulator:
         elif self.use_graph and self._ticks > 0:
             key = (S, float(dt), bool(fused), K, lists, native.ptr(self.sched_dev),
                    self.update_cov_with_F, n_slots, native.ptr(self.grid_ws), native.ptr(self.wrench),
+                   self.chunk_capacity,
                    frozenset(self._mat_ids) <= native.VISCOUS_IDS, self.launch_bits,
                    None if cdf is None else (cdf.n, int(cdf.mask_off), native.ptr(self.cdf_tag)),
                    self._fix_exp if self.deterministic else None,
@@ -1423,6 +1459,8 @@ class Solver:
         if flag & native.ERR_FIXED_RANGE:
             raise RuntimeError("deterministic scatter: a contribution exceeded the fixed-point range "
                                "(the state is far outside the expected momentum scale)")
+        if flag & native.ERR_CHUNKS:
+            raise RuntimeError("internal: the particle pass launched fewer chunks than the sort produced")
         if flag & 1:
             raise RuntimeError("a particle stencil left the grid during the last tick (out of "
                                "bounds P2G was skipped); enlarge grid_lim or add a wall collider")

@@ -494,3 +494,74 @@ def test_covariance_transport_step_mode_matches_oracle():
     hip_cov, ora_cov = out
     assert np.abs(ora_cov - cov).max() > 1e-3 * np.abs(cov).max()  # it was transported
     np.testing.assert_allclose(hip_cov, ora_cov, rtol=1e-4, atol=1e-5 * np.abs(cov).max())
+
+
+def test_sort_chunks_match_group_runs():
+    """mpm_sort_chunks against a numpy restatement on sorted cell keys of a clustered,
+    gappy particle set: chunks partition the particles, none holds more than 256, and
+    every boundary falls on a group change (4 consecutive blocks of one block column) or
+    every 256 particles within a group."""
+    import torch
+
+    from mpm_amd.core import native
+
+    rng = np.random.default_rng(21)
+    grid = GridConfig(n_grid=64, grid_lim=1.0)
+    pts = np.concatenate([rng.uniform(0.1, 0.3, (6000, 3)), rng.uniform(0.6, 0.9, (500, 3)),
+                          np.column_stack([np.full(800, 0.5), np.full(800, 0.5), rng.uniform(0.1, 0.9, 800)])])
+    pts = pts.astype(np.float32)
+    s = Solver(grid=grid, device="hip").load_particles(pts, np.full(len(pts), 1e-7, np.float32))
+    s.set_material(newtonian(eta=2.0))
+    sim = s._sim
+    lib = sim.lib
+    n = sim.n_particles
+    dev = sim.device
+    keys, vals = torch.empty(n, dtype=torch.int32, device=dev), torch.empty(n, dtype=torch.int32, device=dev)
+    flag = torch.zeros(1, dtype=torch.int32, device=dev)
+    st = sim.stream()
+    native.check(lib.mpm_sort_keys_cells(sim.pP, sim.pG, native.ptr(keys), native.ptr(vals), native.ptr(flag), st))
+    tb = int(lib.mpm_sort_temp_bytes(n))
+    temp = torch.empty(tb, dtype=torch.uint8, device=dev)
+    ko, vo = torch.empty_like(keys), torch.empty_like(vals)
+    native.check(lib.mpm_sort_pairs(native.ptr(keys), native.ptr(ko), native.ptr(vals), native.ptr(vo), n,
+                                    native.ptr(temp), tb, st))
+    cs = torch.full((n + 1,), -7, dtype=torch.int32, device=dev)
+    cnt = torch.zeros(1, dtype=torch.int32, device=dev)
+    tb2 = int(lib.mpm_sort_chunks_temp_bytes(n))
+    temp2 = torch.empty(tb2, dtype=torch.uint8, device=dev)
+    nb = 64 // 4 + 2
+    native.check(lib.mpm_sort_chunks(native.ptr(ko), n, 6, nb, 4, 256, native.ptr(cs), native.ptr(cnt),
+                                     native.ptr(temp2), tb2, st))
+    torch.cuda.synchronize()
+    k = ko.cpu().numpy().astype(np.int64) >> 6
+    grp = (k // nb) * ((nb + 3) // 4) + (k % nb) // 4
+    gstart = np.flatnonzero(np.r_[True, grp[1:] != grp[:-1]])
+    want = np.concatenate([np.arange(a, b, 256) for a, b in zip(gstart, np.r_[gstart[1:], n])] + [[n]])
+    c = int(cnt.item())
+    got = cs.cpu().numpy()[:c + 1]
+    np.testing.assert_array_equal(got, want)
+    assert c > (n + 255) // 256  # the gaps add chunks
+    assert np.diff(got).max() <= 256 and np.diff(got).min() >= 1
+
+
+def test_chunked_pass_matches_fixed_chunks_bitwise():
+    """The particle pass over group-aligned chunks against fixed 256-particle chunks on the
+    same sorted state (fixed-point scatter: bit-identical)."""
+    def run(chunks):
+        grid, pts, vol = _blob(n=6000, seed=4)
+        pts = np.concatenate([pts, pts[:300] + np.float32(0.1)])  # a second, detached cluster
+        vol = np.full(len(pts), 1.0e-7, np.float32)
+        s = Solver(grid=grid, device="hip", sort_interval=1, sort_key="cell")
+        s.load_particles(pts, vol)
+        s.set_material(newtonian(eta=2.0, density=1000.0, bulk_modulus=2.0e5))
+        s.add_plane((0, 0, 0.12), (0, 0, 1), "separate", friction=0.3)
+        if not chunks:
+            s._sim._chunks = lambda keys, cells: None
+        for _ in range(4):
+            s.step(2e-4, 6)
+        return s
+
+    a, b = run(True), run(False)
+    assert a._sim.chunk_capacity > 0 and b._sim.chunk_start is None
+    for u, v in zip((a.x(), a.v(), a.F()), (b.x(), b.v(), b.F())):
+        assert np.array_equal(u, v)
