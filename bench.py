@@ -27,6 +27,8 @@ import os
 import sys
 import time
 
+import numpy as np
+
 REPO = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(REPO, "mpm-engine_amd"))
 sys.path.insert(0, REPO)
@@ -41,6 +43,17 @@ HBM_PEAK_GBS = 8000.0
 PASS_B_PER_PARTICLE = 280.0
 PASS_B_PER_NODE = 28.0
 GRID_B_PER_NODE = 56.0
+# The three byte counts of the fused pass, per particle and per active node:
+#   A (compulsory): only x and F carry state between fused substeps: the 76 B of reads
+#     above plus the x and F writes (48 B);
+#   B (reference-state semantics, the primary count): as above;
+#   C (stencil touch, uncached): tier B's particle bytes plus, per particle, 27 node reads
+#     of the gather (12 B) and 27 read-modify-writes of the scatter (m + mv, 2 x 16 B),
+#     the traffic of a pass with no tile reuse; the grid is inside those touches.
+TIERS = {"A": (124.0, 28.0), "B": (PASS_B_PER_PARTICLE, PASS_B_PER_NODE), "C": (280.0 + 27 * (12 + 32), 0.0)}
+# measured ceiling of 64-bit global atomic adds on MI355X (a tile-flush pattern: each
+# workgroup adds 4 channels into 256 consecutive nodes), for the atomic-throughput figure
+ATOMIC_CEILING_GPS = 45.0
 
 
 def parse():
@@ -65,6 +78,10 @@ def parse():
     ap.add_argument("--cdf-frames", type=int, default=2, help="timed frames of the CDF-glass pour")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a device copy of 256 MiB (the byte-counter calibration for --pmc)")
+    ap.add_argument("--no-traffic", action="store_true",
+                    help="skip the rocprofv3 counter runs that measure the fused pass's HBM bytes and atomics")
+    ap.add_argument("--traffic-child", action="store_true",
+                    help="internal: the copy calibration, then the pour up to its held frame (run under rocprofv3)")
     return ap.parse_args()
 
 
@@ -250,20 +267,75 @@ def adjoint_leg(args):
                        "particles": n, "substeps": S, "dt": sc.dt, "checkpoint_every": every}}
 
 
-TRAFFIC_FILE = os.path.join(REPO, "profiles", "pour192_fused_pass_traffic.md")
+COUNTER_SETS = (("FETCH_SIZE", "TCC_EA0_ATOMIC_sum"), ("WRITE_SIZE", "TCC_ATOMIC_sum"))
+FUSED_KERNEL = "k_particle_pass<15,"  # G2P | STRESS | P2G | LEAN, the interior substep
 
 
-def measured_traffic():
-    """HBM bytes per fused-pass launch measured with rocprofv3 --pmc (FETCH_SIZE +
-    WRITE_SIZE, calibrated on a copy of known size), recorded in the profile summary."""
-    try:
-        with open(TRAFFIC_FILE) as fh:
-            for line in fh:
-                if line.startswith("| traffic per launch (bytes)"):
-                    return int(float(line.split("|")[2]))
-    except OSError:
-        pass
-    return None
+def _counter_run(args, counters, outdir):
+    """One rocprofv3 --pmc run of this script in --traffic-child mode; returns (per-dispatch
+    rows as {name, grid, counters}, child's JSON line)."""
+    import csv
+    import subprocess
+
+    cmd = ["rocprofv3", "--pmc", *counters, "-f", "csv", "-d", outdir, "-o", "t", "--",
+           sys.executable, os.path.abspath(__file__), "--traffic-child", "--steps", str(args.steps),
+           "--warmup", str(args.warmup), "--sort-interval", str(args.sort_interval), "--sort-key", args.sort_key]
+    if args.no_graph:
+        cmd.append("--no-graph")
+    env = dict(os.environ, TMPDIR="/tmp")
+    r = subprocess.run(["timeout", "-s", "KILL", "600", *cmd], cwd="/tmp", env=env, capture_output=True, text=True)
+    line = next((ln for ln in reversed(r.stdout.splitlines()) if ln.startswith("{")), None)
+    if r.returncode != 0 or line is None:
+        raise RuntimeError(f"rocprofv3 run failed (rc {r.returncode}): {r.stderr.strip()[-300:]}")
+    rows = {}
+    with open(os.path.join(outdir, "t_counter_collection.csv")) as fh:
+        for row in csv.DictReader(fh):
+            d = rows.setdefault(int(row["Dispatch_Id"]), {"name": row["Kernel_Name"], "grid": int(row["Grid_Size"]),
+                                                           "c": {}})
+            d["c"][row["Counter_Name"]] = float(row["Counter_Value"])
+    return [rows[k] for k in sorted(rows)], json.loads(line)
+
+
+def measured_traffic(args, launches: int, avg_s: float, digest: str):
+    """HBM bytes and global atomics per fused-pass launch, from two rocprofv3 --pmc runs
+    (FETCH_SIZE + TCC_EA0_ATOMIC_sum, then WRITE_SIZE + TCC_ATOMIC_sum) of a child that
+    replays this workload to the same state (the fixed-point scatter is bit-reproducible:
+    the child's final-position digest must equal this run's) and counts the fused passes
+    of the same held frame. Byte counters are scaled by what they read for a device copy
+    of 256 MiB run first in the same child."""
+    import shutil
+    import tempfile
+
+    if shutil.which("rocprofv3") is None:
+        return None, "rocprofv3 not found"
+    out = {}
+    child = None
+    for counters in COUNTER_SETS:
+        d = tempfile.mkdtemp(prefix="mpm_pmc_", dir="/tmp")
+        try:
+            rows, child = _counter_run(args, counters, d)
+        finally:
+            shutil.rmtree(d, ignore_errors=True)
+        copies = [r for r in rows if "elementwise" in r["name"] and r["grid"] >= (1 << 22)][:3]
+        # the held frame's interior passes (its last one keeps the stress: another mode)
+        fused = [r for r in rows if FUSED_KERNEL in r["name"].replace(" ", "")][-(launches - 1):]
+        if len(copies) < 3 or not fused:
+            return None, f"counter rows not found ({len(copies)} copies, {len(fused)} fused)"
+        for cn in counters:
+            out[cn] = float(np.mean([r["c"][cn] for r in fused]))
+            if cn in ("FETCH_SIZE", "WRITE_SIZE"):
+                out[cn + "_copy"] = float(np.mean([r["c"][cn] for r in copies]))
+    copy_kib = 256 * 1024.0  # bytes read (and written) by one copy, in KiB
+    f_scale, w_scale = copy_kib / out["FETCH_SIZE_copy"], copy_kib / out["WRITE_SIZE_copy"]
+    read_b = out["FETCH_SIZE"] * 1024.0 * f_scale
+    write_b = out["WRITE_SIZE"] * 1024.0 * w_scale
+    atom = out["TCC_EA0_ATOMIC_sum"]
+    return {"bytes": int(read_b + write_b), "read_bytes": int(read_b), "write_bytes": int(write_b),
+            "fetch_scale": round(f_scale, 3), "write_scale": round(w_scale, 3),
+            "atomics_per_launch": int(atom), "atomics_at_l2_per_launch": int(out["TCC_ATOMIC_sum"]),
+            "atomic_G_per_s": round(atom / avg_s / 1e9, 2),
+            "atomic_ceiling_G_per_s": ATOMIC_CEILING_GPS,
+            "same_state": child.get("digest") == digest, "launches_counted": launches}, None
 
 
 def copy_calibration():
@@ -435,14 +507,15 @@ def float_leg(args):
             "fused_pass_avg_us": round(float(np.mean(durations)) * 1e3, 2)}
 
 
-def pour_leg(args):
-    """The headline: fused ticks of the 192^3 pour, frames timed end to end; the fused
-    particle pass timed with HIP events on its stream."""
+def _pour_frames(args):
+    """The headline workload up to the end of the timed frames: (solver, scene, elapsed
+    seconds of the timed frames, final-position digest)."""
+    import hashlib
+
     import numpy as np
     import torch
 
     from mpm_amd import Solver
-    from mpm_amd.core import native
     from mpm_amd.scenes import PourScene
 
     sc = PourScene()
@@ -450,9 +523,7 @@ def pour_leg(args):
     # arena of the particle pass; the cell key lets same-cell lanes sum before the atomics
     s = sc.build(Solver(grid=sc.grid, device="cuda:0", sort_interval=args.sort_interval,
                         sort_key=args.sort_key))
-    sim = s._sim
-    sim.use_graph = not args.no_graph
-    n = s.n_particles
+    s._sim.use_graph = not args.no_graph
     for _ in range(args.warmup):
         sc.frame(s)
     torch.cuda.synchronize()
@@ -465,26 +536,63 @@ def pour_leg(args):
     if not np.isfinite(x_end).all():
         raise RuntimeError("non-finite particle state after the timed frames")
     # the default scatter is bit-reproducible: this digest repeats from run to run
-    import hashlib
     digest = hashlib.sha256(np.ascontiguousarray(x_end).tobytes()).hexdigest()[:16]
-    substeps = args.steps * sc.substeps
-    value = n * substeps / el
-    # active nodes: nodes receiving mass in one scatter of the current state (the default
-    # scatter accumulates into the fixed-point buffer, the float one into node_in)
+    return s, sc, el, digest
+
+
+def _active_nodes(s, sc):
+    """Nodes receiving mass in one scatter of the current state (the default scatter
+    accumulates into the fixed-point buffer, the float one into node_in)."""
+    from mpm_amd.core import native
+
+    sim = s._sim
     acc = sim.node_fix if sim.deterministic else sim.node_in
     acc.zero_()
     sim._pass(native.PASS_STRESS | native.PASS_P2G, sc.dt, sim.stream())
     active = int((acc.view(-1, 4)[:, 3] > 0).sum().item())
     acc.zero_()
+    return active
+
+
+def traffic_child(args):
+    """Run under rocprofv3 by measured_traffic: the copy calibration, then the same pour
+    as pour_leg up to and through its held frame; prints the final-position digest."""
+    copy_calibration()
+    s, sc, _, digest = _pour_frames(args)
+    _active_nodes(s, sc)
+    held_frame_pass_times(s, sc)
+    print(json.dumps({"digest": digest}))
+
+
+def pour_leg(args):
+    """The headline: fused ticks of the 192^3 pour, frames timed end to end; the fused
+    particle pass timed with HIP events on its stream; its HBM bytes and atomics counted
+    by rocprofv3 on the same state."""
+    import numpy as np
+    import torch
+
+    s, sc, el, digest = _pour_frames(args)
+    n = s.n_particles
+    substeps = args.steps * sc.substeps
+    value = n * substeps / el
+    active = _active_nodes(s, sc)
     durations = held_frame_pass_times(s, sc)
     avg_s = float(np.mean(durations)) / 1e3
-    algo = PASS_B_PER_PARTICLE * n + PASS_B_PER_NODE * active
-    ach = algo / avg_s / 1e9
-    roof = {"bound": "hbm", "achieved": round(ach, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": measured_traffic(),
+    tiers = {}
+    for name, (bp, bn) in TIERS.items():
+        b = bp * n + bn * active
+        tiers[name] = {"bytes_per_particle": bp, "bytes_per_active_node": bn, "algorithmic_bytes": int(b),
+                       "achieved_GBs": round(b / avg_s / 1e9, 2), "frac": round(b / avg_s / 1e9 / HBM_PEAK_GBS, 4)}
+    del s
+    torch.cuda.empty_cache()
+    traffic, why = (None, "--no-traffic") if args.no_traffic else measured_traffic(args, len(durations), avg_s, digest)
+    tb = tiers["B"]
+    roof = {"bound": "hbm", "achieved": tb["achieved_GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": tb["frac"], "traffic": None if traffic is None else traffic["bytes"],
             "kernel": "k_particle_pass (fused g2p+stress+p2g, fixed-point scatter)",
             "avg_launch_us": round(avg_s * 1e6, 2), "launches": len(durations),
-            "algorithmic_bytes_per_launch": int(algo), "active_nodes": active}
+            "algorithmic_bytes_per_launch": tb["algorithmic_bytes"], "active_nodes": active, "tier": "B",
+            "tiers": tiers, "measured": traffic if traffic is not None else {"error": why}}
     return {"value": round(value, 1), "ms_per_step": round(el / args.steps * 1e3, 3), "particles": n,
             "substeps_per_step": sc.substeps, "dt": sc.dt, "roofline": roof, "x_sha256_16": digest}
 
@@ -493,6 +601,9 @@ def main():
     args = parse()
     if args.copy_calibration:
         copy_calibration()
+        return
+    if args.traffic_child:
+        traffic_child(args)
         return
     legs = set(args.legs.split(","))
     if args.no_batch:
