@@ -106,7 +106,72 @@ def cpu_baseline(substeps: int):
         out["threaded"] = cpu_baseline_threaded(4 * substeps)
     except Exception as e:  # reported beside the single-threaded figure
         out["threaded"] = {"error": f"{type(e).__name__}: {e}"}
+    for name, fn in (("press_C3", cpu_press), ("batch_sequential_C4", cpu_batch_sequential),
+                     ("squeeze_forward_C5", cpu_squeeze_forward)):
+        try:
+            out[name] = fn()
+        except Exception as e:  # reported beside the headline figure
+            out[name] = {"error": f"{type(e).__name__}: {e}"}
     return out
+
+
+def cpu_press(substeps: int = 2):
+    """The oracle (one thread) on the dough + sand press: 128^3, sparse blocks, ~597k
+    particles of Herschel-Bulkley dough and mu(I) sand, a pressing box."""
+    from mpm_amd.scenes import DoughSandPress
+    from oracle.sim import OracleSolver
+
+    sc = DoughSandPress()
+    s = sc.build(OracleSolver(sc.grid, sparse=True, sort_interval=20))
+    t0 = time.perf_counter()
+    s.step(sc.dt, substeps)
+    el = time.perf_counter() - t0
+    n = len(sc.pos)
+    return {"value": n * substeps / el, "unit": "particle-substeps/s", "cores": 1, "kind": "port",
+            "sample": f"dough + sand press 128^3, {n} particles, {substeps} substeps ({el:.1f} s)"}
+
+
+def cpu_batch_sequential(scenes=(0, 64, 128, 192), substeps: int = 5):
+    """The oracle (one thread) solving dough-press scenes of the 256-scene batch one after
+    another, as scene-substeps/s (the batched leg's sequential baseline on the host)."""
+    from mpm_amd.scenes import DoughPressBatch, herschel_bulkley_dough
+    from oracle.sim import OracleSolver
+
+    sc = DoughPressBatch(n_scenes=256, seed=0)
+    el, parts = 0.0, 0
+    for k in scenes:
+        s = OracleSolver(sc.grid)
+        s.load_particles(sc.pos[k], sc.vol[k])
+        s.set_material(herschel_bulkley_dough())
+        s._sim.yield_stress[:] = sc.yield_stress[k]
+        s.add_plane((0, 0, sc.floor), (0, 0, 1), "sticky")
+        s.add_domain_walls()
+        s.add_box(sc.box_center[k], sc.box_half[k], (0.0, 0.0, -sc.box_speed[k]))
+        t0 = time.perf_counter()
+        s.step(sc.dt, substeps)
+        el += time.perf_counter() - t0
+        parts += len(sc.pos[k])
+    return {"value": len(scenes) * substeps / el, "unit": "scene-substeps/s", "cores": 1, "kind": "port",
+            "sample": f"scenes {list(scenes)} of the 256-scene dough-press batch ({parts} particles), "
+                      f"{substeps} substeps each ({el:.1f} s)"}
+
+
+def cpu_squeeze_forward(substeps: int = 10):
+    """The oracle (one thread) on the squeeze workload's forward rollout (50,653 von
+    Mises particles between closing plates); a central-difference gradient costs two
+    such rollouts per parameter direction."""
+    from mpm_amd.scenes import SqueezeScene
+    from oracle.sim import OracleSolver
+
+    sc = SqueezeScene()
+    s = sc.build(OracleSolver(sc.grid))
+    t0 = time.perf_counter()
+    s.step(sc.dt, substeps)
+    el = time.perf_counter() - t0
+    n = len(sc.pos)
+    return {"value": n * substeps / el, "unit": "particle-substeps/s", "cores": 1, "kind": "port",
+            "forward_s_per_400_substeps": round(el / substeps * 400, 1),
+            "sample": f"squeeze 64^3, {n} particles, {substeps} substeps ({el:.1f} s)"}
 
 
 def cpu_baseline_threaded(substeps: int):
