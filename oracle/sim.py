@@ -138,30 +138,42 @@ def use_threaded_build(on: bool) -> None:
         _LIB_NAME, _LIB = name, None
 
 
-def mutant_path(k: int) -> str:
-    """Path of the oracle built with planted mistake k (-DO_MUTANT=k, see MUT(k) in
-    mpm_oracle.c), compiled on first use into oracle/_mut/. Tests load it to show that a
-    pin fails on the mistake; it is never the checker."""
+def _variant_path(name: str, flags) -> str:
+    """The oracle compiled with extra flags into oracle/_mut/libmpm_oracle_<name>.so on
+    first use (never the checker)."""
     import subprocess
 
     d = os.path.join(_HERE, "_mut")
     os.makedirs(d, exist_ok=True)
-    path = os.path.join(d, f"libmpm_oracle_mut{int(k)}.so")
+    path = os.path.join(d, f"libmpm_oracle_{name}.so")
     src = os.path.join(_HERE, "mpm_oracle.c")
     if not os.path.exists(path) or os.path.getmtime(path) < os.path.getmtime(src):
         tmp = f"{path}.{os.getpid()}.tmp"
-        subprocess.run(["gcc", "-O1", "-fPIC", "-std=c11", "-ffp-contract=off", "-fno-fast-math",
-                        f"-DO_MUTANT={int(k)}", "-shared", "-o", tmp, src, "-lm"], check=True,
-                       capture_output=True)
+        subprocess.run(["gcc", "-O2", "-fPIC", "-std=c11", "-fno-fast-math", *flags, "-shared", "-o", tmp, src,
+                        "-lm"], check=True, capture_output=True)
         os.replace(tmp, path)
     return path
 
 
-class mutant:
-    """Context manager: the oracle functions run the build with planted mistake k."""
+def mutant_path(k: int) -> str:
+    """Path of the oracle built with planted mistake k (-DO_MUTANT=k, see MUT(k) in
+    mpm_oracle.c). Tests load it to show that a pin fails on the mistake."""
+    return _variant_path(f"mut{int(k)}", ["-ffp-contract=off", f"-DO_MUTANT={int(k)}"])
 
-    def __init__(self, k: int):
-        self.path = mutant_path(k)
+
+def contracted_path() -> str:
+    """The oracle with fused multiply-adds (-mfma -ffp-contract=fast): the same arithmetic
+    with other roundings in every substep, the kind of difference a GPU build has from the
+    checker. Tests use it to measure a workload's conditioning; it is never the checker."""
+    return _variant_path("fma2", ["-mfma", "-ffp-contract=fast"])
+
+
+class mutant:
+    """Context manager: the oracle functions run the build with planted mistake k (or,
+    with k = "fma", the contracted build)."""
+
+    def __init__(self, k):
+        self.path = contracted_path() if k == "fma" else mutant_path(k)
 
     def __enter__(self):
         global _LIB, _LIB_NAME
