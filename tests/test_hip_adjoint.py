@@ -896,3 +896,110 @@ def test_sand_rollout_gradients_match_central_differences():
     print("loss", loss.item(), report)
     for k, (a, f) in report.items():
         assert abs(a - f) <= 2e-3 * abs(f), f"{k}: adjoint {a:.6e} vs central difference {f:.6e}"
+
+
+def _squeeze_fd_job(job):
+    """One oracle rollout of the full-size squeeze (a worker thread): the loss
+    linearised at the GPU trajectory, g . (x_T, plate impulses), at the given E, nu, yield
+    and initial velocity hv * d (d the seeded +-1 direction)."""
+    E_, nu_, y_, hv, seed, S, gx, gw = job
+    sc = SqueezeScene()
+    n = len(sc.pos)
+    d = np.random.default_rng(seed).choice([-1.0, 1.0], size=(n, 3)).astype(np.float32)
+    mu, lam = _lame(E_, nu_)
+    x, imp = _oracle_rollout(sc, S, np.float32(mu), np.float32(lam), np.float32(y_), (hv * d).astype(np.float32))
+    return float((x * gx).sum() + (imp * gw).sum())
+
+
+def _full_squeeze_adjoint(S=400, seed=3):
+    """GPU forward + adjoint of the squeeze identification at its stated size (64^3,
+    50,653 particles, plates closing at 0.05 m/s, 400 substeps, checkpoints every 20;
+    loss |x_T - x*|^2 + |plate impulses - target|^2 against the rollout at yield x 1.1).
+    Returns the gradients, the random +-1 initial-velocity direction's derivative and
+    the linearisation seed (gx, gw)."""
+    import torch
+
+    sc = SqueezeScene()
+    s = sc.build(Solver(grid=sc.grid, device="hip"))
+    n = s.n_particles
+    ro = Rollout(s, DT, S, checkpoint_every=20)
+    dev = s._sim.device
+    f64 = dict(dtype=torch.float64, device=dev)
+    E0, nu0, y0 = 2.0e5, 0.3, 1.0e3
+    with torch.no_grad():
+        xt, wt = ro.run(torch.tensor(E0, **f64), torch.tensor(nu0, **f64), torch.tensor(1.1 * y0, **f64),
+                        torch.zeros(n, 3, device=dev))
+    x_star, w_star = xt.double(), wt[:, :3].double()
+    E = torch.tensor(E0, requires_grad=True, **f64)
+    nu = torch.tensor(nu0, requires_grad=True, **f64)
+    ys = torch.tensor(y0, requires_grad=True, **f64)
+    v0 = torch.zeros(n, 3, device=dev, requires_grad=True)
+    xT, w = ro.run(E, nu, ys, v0)
+    loss = ((xT.double() - x_star) ** 2).sum() + ((w[:, :3].double() - w_star) ** 2).sum()
+    loss.backward()
+    d = np.random.default_rng(seed).choice([-1.0, 1.0], size=(n, 3)).astype(np.float32)
+    grads = {"E": E.grad.item(), "nu": nu.grad.item(), "yield": ys.grad.item(),
+             "v0.d": float((v0.grad.double().cpu().numpy() * d).sum())}
+    gx = (2.0 * (xT.double() - x_star)).detach().cpu().numpy()
+    gw = (2.0 * (w[:, :3].double() - w_star)).detach().cpu().numpy()
+    return grads, gx, gw, loss.item()
+
+
+def _squeeze_central_differences(gx, gw, steps, S=400, seed=3, workers=8):
+    """Central differences of the linearised loss along E, nu, yield (relative steps) and
+    the velocity direction (absolute step, m/s): two oracle rollouts per entry of `steps`,
+    on `workers` threads of this process (the oracle keeps all its state in the arrays it
+    is handed, and ctypes lets go of the GIL for the C rollout, so they run in parallel;
+    no process is started from one that holds the GPU)."""
+    from concurrent.futures import ThreadPoolExecutor, as_completed
+
+    from oracle import sim as osim
+
+    osim.lib()  # load (and if need be build) the oracle before the threads start
+
+    base = {"E": 2.0e5, "nu": 0.3, "yield": 1.0e3, "v0.d": 0.0}
+    jobs, keys = [], []
+    for name, rel in steps:
+        h = rel if name == "v0.d" else rel * base[name]
+        for sgn in (1.0, -1.0):
+            p = dict(base)
+            p[name] += sgn * h
+            jobs.append((p["E"], p["nu"], p["yield"], p["v0.d"], seed, S, gx, gw))
+        keys.append((name, rel, h))
+    vals = [None] * len(jobs)
+    with ThreadPoolExecutor(max_workers=workers) as ex:
+        futs = {ex.submit(_squeeze_fd_job, j): i for i, j in enumerate(jobs)}
+        for f in as_completed(futs):
+            i = futs[f]
+            vals[i] = f.result()
+            name, rel, _ = keys[i // 2]
+            print(f"oracle rollout {name} {rel:g} {'+-'[i % 2]} done", flush=True)
+    return {(name, rel): (vals[2 * i] - vals[2 * i + 1]) / (2 * h) for i, (name, rel, h) in enumerate(keys)}
+
+
+@pytest.mark.slow
+@pytest.mark.timeout(900)
+def test_full_squeeze_gradients_match_central_differences():
+    """The squeeze identification workload at its stated size: 64^3, 50,653 von Mises
+    particles, plates closing at 0.05 m/s, 400 substeps of 1e-4 s (checkpoints every 20),
+    loss against the rollout at yield x 1.1. d loss / d (E, nu, yield) and the derivative
+    along a random +-1 initial-velocity direction against central differences of the
+    oracle rollout (loss linearised at the GPU trajectory, as in the reduced test), within
+    3e-3.
+
+    Steps: 1e-2 relative for E, nu and yield, 1e-2 m/s along the velocity direction. Over
+    400 fp32 substeps the rollout's rounding sets the difference's floor: at 1e-4 the
+    oracle's differences miss by 7 % (nu) and 19 % (velocity), at 1e-3 by 0.8 % and 14 %,
+    and the GPU's own deterministic forward scatters the same way (4 % and 45 % at 1e-4),
+    so the noise is the fp32 map's, not either implementation's. At 1e-2 both settle onto
+    the adjoint (measured: E 5.8e-4, nu 1.9e-3, yield 1.4e-3, velocity 2.3e-4 against the
+    oracle; the GPU's own differences 3.8e-4, 1.2e-3, 8.7e-4, 3.1e-3), and the oracle and
+    the GPU differ from each other by ~1e-3 there: the 3e-3 bar. The eight oracle
+    rollouts (~30 s each) run on threads."""
+    grads, gx, gw, loss = _full_squeeze_adjoint()
+    steps = [("E", 1e-2), ("nu", 1e-2), ("yield", 1e-2), ("v0.d", 1e-2)]
+    fd = _squeeze_central_differences(gx, gw, steps, workers=8)
+    report = {name: (grads[name], fd[(name, rel)]) for name, rel in steps}
+    print("full squeeze: loss", loss, report)
+    for k, (a, f) in report.items():
+        assert abs(a - f) <= 3e-3 * abs(f), f"{k}: adjoint {a:.6e} vs central difference {f:.6e}"
