@@ -148,7 +148,7 @@ EXPORT void o_svd3(const float *Af, float *Uf, float *sf, float *Vf) {
         for (int j = 0; j < 3; ++j) {
             double acc = 0.0;
             for (int k = 0; k < 3; ++k) acc += A[k][i] * A[k][j];
-            S[i][j] = acc;
+            S[i][j] = MUT(13) ? (double)(float)acc : acc;
         }
     jacobi_eig3(S, V);
     double ev[3] = {S[0][0], S[1][1], S[2][2]};
@@ -209,6 +209,19 @@ EXPORT void o_svd3(const float *Af, float *Uf, float *sf, float *Vf) {
     sf[0] = (float)n0;
     sf[1] = (float)n1;
     sf[2] = (float)s2;
+    if (MUT(13)) /* singular values read off the eigenvalues of a float32 A^T A */
+        for (int a = 0; a < 3; ++a) {
+            double e = ev[ord[a]] > 0.0 ? sqrt(ev[ord[a]]) : 0.0;
+            sf[a] = (float)(a == 2 && s2 < 0.0 ? -e : e);
+        }
+    if (MUT(14) && s2 < 0.0) { /* the reflection carried by the largest axis, not the smallest */
+        sf[0] = -sf[0];
+        sf[2] = -sf[2];
+        for (int i = 0; i < 3; ++i) {
+            Uf[3 * i + 0] = -Uf[3 * i + 0];
+            Uf[3 * i + 2] = -Uf[3 * i + 2];
+        }
+    }
 }
 
 static inline void svd3(m33 A, m33 *U, v3 *s, m33 *V) { o_svd3(A.a, U->a, s->c, V->a); }

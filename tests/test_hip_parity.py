@@ -165,7 +165,7 @@ def _deformed_start(n, seed=1, vol=(-0.03, 0.01)):
     return F.astype(np.float32), v
 
 
-def _mat_run(cls, material, fused=True, vol_range=(-0.03, 0.01), ticks=(8, 8, 8), F0=None, **kw):
+def _mat_run(cls, material, fused=True, vol_range=(-0.03, 0.01), ticks=(8, 8, 8), F0=None, dt=2.0e-4, **kw):
     import torch
 
     grid, pts, vol = _blob()
@@ -190,7 +190,7 @@ def _mat_run(cls, material, fused=True, vol_range=(-0.03, 0.01), ticks=(8, 8, 8)
     s.add_plane((0, 0, 3 * grid.dx), (0, 0, 1), "separate", friction=0.3)
     s.add_domain_walls()
     for k in ticks:
-        s.step(2.0e-4, k)
+        s.step(dt, k)
     return s
 
 
@@ -220,6 +220,93 @@ def test_materials_match_oracle(label, material):
     ys0 = float(np.abs(_mat_run(OracleSolver, material, ticks=())._sim.yield_stress).max())
     np.testing.assert_allclose(hip._sim.yield_stress.cpu().numpy(), ora._sim.yield_stress,
                                rtol=1e-4, atol=1e-4 * ys0)
+
+
+def _fragile_start(n, seed=7):
+    """F_trial = Q1 diag(s) Q2^T (random rotations) with the spectra where a fp32 SVD is
+    fragile, n/4 particles each: (a) repeated singular values (s, s, s), (s, s, t) and
+    (t, s, s) with s, t in [0.97, 1.03]; (b) sigma_min log-uniform down to 1e-3 with the
+    other two in [0.95, 1.05]; (c) det F < 0, the negative singular value in [-0.9, -1e-3]
+    and the others in [0.95, 1.05]; (d) both: (s, s, sigma_min) and (s, s, -sigma_min)
+    with sigma_min in [1e-3, 0.5]. A reflected singular value is kept below the others in
+    magnitude, where the signed decomposition (and R = U V^T) is unique."""
+    rng = np.random.default_rng(seed)
+    k = n // 4
+    S = np.empty((n, 3))
+    a = rng.uniform(0.97, 1.03, n)
+    b = rng.uniform(0.97, 1.03, n)
+    pattern = rng.integers(0, 3, k)
+    S[:k, 0] = a[:k]
+    S[:k, 1] = np.where(pattern == 2, b[:k], a[:k])
+    S[:k, 2] = np.where(pattern == 0, a[:k], b[:k])
+    S[k:2 * k, :2] = rng.uniform(0.95, 1.05, (k, 2))
+    S[k:2 * k, 2] = 10.0 ** rng.uniform(-3, -1, k)
+    S[2 * k:3 * k, :2] = rng.uniform(0.95, 1.05, (k, 2))
+    S[2 * k:3 * k, 2] = -10.0 ** rng.uniform(-3, np.log10(0.9), k)
+    m = n - 3 * k
+    S[3 * k:, 0] = S[3 * k:, 1] = rng.uniform(0.97, 1.03, m)
+    S[3 * k:, 2] = 10.0 ** rng.uniform(-3, np.log10(0.5), m) * rng.choice([-1.0, 1.0], m)
+    F = np.empty((n, 3, 3))
+    for p in range(n):
+        q1 = np.linalg.qr(rng.normal(size=(3, 3)))[0]
+        q2 = np.linalg.qr(rng.normal(size=(3, 3)))[0]
+        q1 *= np.sign(np.linalg.det(q1))
+        q2 *= np.sign(np.linalg.det(q2))
+        F[p] = q1 @ np.diag(S[p]) @ q2.T
+    groups = {"repeated": slice(0, k), "sigma_min": slice(k, 2 * k), "det<0": slice(2 * k, 3 * k),
+              "repeated+small": slice(3 * k, n)}
+    return F.astype(np.float32), groups
+
+
+FRAGILE = ["jelly", "metal", "sand", "mu_i", "mu_i_phi"]
+
+
+@pytest.mark.parametrize("label", FRAGILE)
+def test_svd_fragile_spectra_match_oracle(label):
+    """Materials 0, 1, 2, 9 and 11 (every return map and stress that reads the SVD) from
+    F_trial with repeated singular values, sigma_min down to 1e-3 and det F < 0, one
+    substep: the return-mapped F and the stress against the oracle, whose SVD runs in
+    float64. The bar is the fp32 problem's own conditioning: per spectrum group, the GPU
+    within 10x the spread of the oracle's result under a 1-ulp perturbation of F_trial
+    (a fp32 SVD that reads sigma_min off A^T A's eigenvalue, error eps sigma_max^2 /
+    sigma_min^2 = 6 % at sigma_min = 1e-3, or that puts the reflection on the wrong axis,
+    misses it by orders of magnitude). Where the reference's own stress is NaN (sand keeps
+    an F_trial with det F < 0 below its yield cone and its Drucker-Prager stress takes the
+    log of the negative singular value, mpm_utils.py:109-124, :567-568), the GPU's must be
+    NaN at exactly the same particles."""
+    import torch
+
+    material = dict(MATERIALS)[label]
+    F0, groups = _fragile_start(3000)
+    rng = np.random.default_rng(99)
+    F1 = (F0 * (1 + rng.choice([-1, 1], F0.shape) * 2.0 ** -23)).astype(np.float32)
+    # dt = 1e-7: at sigma_min = 1e-3 the stresses reach ~1e8 Pa, which over a 2e-4 s
+    # substep would throw particles out of the domain; F and the stress compared here are
+    # read at the substep's start
+    kw = dict(ticks=(1,), dt=1.0e-7)
+    hip = _mat_run(Solver, material, device="hip", F0=F0, **kw)
+    ora = _mat_run(OracleSolver, material, F0=F0, **kw)
+    ulp = _mat_run(OracleSolver, material, F0=F1, **kw)
+    get = {"F": lambda s: s.F().reshape(-1, 9).astype(np.float64),
+           "stress": lambda s: s.stress().reshape(-1, 9).astype(np.float64)}
+    if label == "mu_i_phi":
+        get["Jp"] = lambda s: np.asarray(s._sim.Jp.cpu().numpy() if hasattr(s._sim.Jp, "cpu")
+                                         else s._sim.Jp, np.float64).reshape(-1, 1)
+    for name, g in get.items():
+        h, o, u = g(hip), g(ora), g(ulp)
+        fin = np.isfinite(o).all(1)
+        assert np.array_equal(np.isfinite(h).all(1), fin), f"{name}: NaN at other particles than the oracle's"
+        assert np.array_equal(np.isfinite(u).all(1), fin)
+        scale = np.abs(o[fin]).max()
+        for gname, sl in groups.items():
+            keep = fin[sl]
+            if not keep.any():
+                continue
+            err = np.abs(h[sl][keep] - o[sl][keep]).max()
+            spread = np.abs(u[sl][keep] - o[sl][keep]).max()
+            print(f"{label} {name} {gname} ({keep.sum()} finite): GPU {err:.3e}, 1-ulp spread {spread:.3e},"
+                  f" scale {scale:.3e}")
+            assert err <= 10.0 * spread + 1e-6 * scale, f"{name} {gname}: {err:.3e} vs 1-ulp spread {spread:.3e}"
 
 
 def test_mu_i_phi_ticks_within_own_conditioning():

@@ -54,6 +54,34 @@ def test_svd_degenerate_cases():
 
 
 # ------------------------------------------------------------------- B-spline transfers
+@pytest.mark.parametrize("k,label,group", [(13, "mu_i", "sigma_min"), (13, "sand", "sigma_min"),
+                                             (14, "jelly", "det<0"), (14, "mu_i", "det<0")])
+def test_fragile_spectra_bar_has_teeth(k, label, group):
+    """The bar of the GPU's fragile-spectrum SVD case (per spectrum group, within 10x the
+    oracle's spread under a 1-ulp perturbation of F_trial) fails on the two mistakes a
+    fp32 SVD plausibly makes: singular values read off the eigenvalues of a float32
+    A^T A (planted mistake 13: the conditioning squared, ~3 % at sigma_min = 1e-3), and
+    the reflection of a det F < 0 state carried by the largest axis (planted mistake 14:
+    a valid decomposition, but another R = U V^T and other clamped logs)."""
+    from test_hip_parity import MATERIALS, _fragile_start, _mat_run
+
+    material = dict(MATERIALS)[label]
+    F0, groups = _fragile_start(3000)
+    rng = np.random.default_rng(99)
+    F1 = (F0 * (1 + rng.choice([-1, 1], F0.shape) * 2.0 ** -23)).astype(np.float32)
+    kw = dict(ticks=(1,), dt=1.0e-7)
+    ora = _mat_run(O.OracleSolver, material, F0=F0, **kw).stress().reshape(-1, 9).astype(np.float64)
+    ulp = _mat_run(O.OracleSolver, material, F0=F1, **kw).stress().reshape(-1, 9).astype(np.float64)
+    with O.mutant(k):
+        mut = _mat_run(O.OracleSolver, material, F0=F0, **kw).stress().reshape(-1, 9).astype(np.float64)
+    sl = groups[group]
+    spread = np.abs(ulp[sl] - ora[sl]).max()
+    err = np.abs(mut[sl] - ora[sl]).max()
+    bar = 10.0 * spread + 1e-6 * np.nanmax(np.abs(ora))
+    print(f"mutant {k} {label} {group}: {err:.3e} vs bar {bar:.3e}")
+    assert err > 10.0 * bar, f"mutant {k}: {err:.3e} vs bar {bar:.3e}"
+
+
 @pytest.mark.parametrize("seed", range(4))
 def test_p2g_conserves_mass_and_momentum(seed):
     """Sum of grid mass = particle mass; grid momentum = m v: the affine (C) and stress
