@@ -232,6 +232,16 @@ def batch_leg(args):
     if not np.isfinite(b.x(0)).all():
         raise RuntimeError("non-finite batch state")
     batched = ns * chunk * args.batch_ticks / el
+    sim = b._sim
+    # grid storage: the shared block pool against dense per-scene node arrays (node_in and
+    # node_out float4, node_fix 4 x int64 per node, for every block of every scene)
+    node_bytes = 16 + 16 + (32 if sim.node_fix is not None else 0)
+    grid = {"grid_bytes": sim.grid_bytes(), "pool_blocks": sim.pool_blocks,
+            "active_blocks": int(sim._blk["counts"][0].item()) if sim._blk is not None else None,
+            "blocks_dense": sim.n_blocks,
+            "grid_bytes_if_dense": sim.grid_bytes() + sim.n_blocks * 64 * node_bytes
+            - (sim.node_in.numel() * 4 + sim.node_out.numel() * 4
+               + (sim.node_fix.numel() * 8 if sim.node_fix is not None else 0))}
     # sequential baseline: each scene alone in a plain Solver, one timed graph-replayed
     # chunk after two warm-up chunks, under two sort policies (the reference's default,
     # no sort, and the batch's autotuned one); the speedup is against the faster
@@ -265,6 +275,7 @@ def batch_leg(args):
                                    f"yield log-U[50,500] Pa per particle, box plate U[0.02,0.1] m/s",
                        "particles": n, "substeps_per_chunk": chunk, "dt": sc.dt,
                        "sequential_scenes_timed": len(picks)},
+            "grid_storage": grid,
             "autotune": tune["chosen"], "autotune_shapes_s": tune["shapes"]}
 
 

@@ -36,7 +36,7 @@ extern "C" {
 #endif
 
 #define MPM_MAX_MATERIALS 16
-#define MPM_ABI_VERSION 7
+#define MPM_ABI_VERSION 8
 
 typedef enum mpm_status {
     MPM_OK = 0,
@@ -161,7 +161,9 @@ typedef struct mpm_grid {
                              [0, n_scenes) (that particle is skipped), 64 = a fixed-point
                              contribution beyond 2^51 units (deterministic scatter: the
                              scale is too fine for this state), 256 = more chunks than
-                             P->chunk_capacity (the chunks beyond were not processed) */
+                             P->chunk_capacity (the chunks beyond were not processed),
+                             512 = a scatter into a block without a pool slot (that
+                             contribution was dropped) */
     int32_t n_scenes;     /* independent scenes sharing the buffers (>= 1): scene s owns
                              blocks [s * B, (s + 1) * B), B = nb[0] * nb[1] * nb[2];
                              block codes in every block list are these global codes */
@@ -177,6 +179,20 @@ typedef struct mpm_grid {
                              accumulators and node_out are zero), only zeroes node_out of
                              blocks at 2, and leaves processed blocks at 2. NULL: every
                              listed block is processed. Zero-initialised with node_out. */
+    const int32_t *block_slot; /* block pool, or NULL (dense storage: block code c holds nodes
+                             [64 c, 64 c + 64)). Per global block code, the pool slot whose
+                             64 nodes hold that block in node_in / node_out / node_fix
+                             (node = 64 slot + cell), or -1: no storage this tick, gathers
+                             read zero velocity there and a scatter there sets error bit 512.
+                             Built by mpm_pool_assign from the active-block flags. Of the
+                             entry points that address node storage, the particle pass
+                             (without CPIC colliders), the grid update, mpm_grid_zero_blocks,
+                             mpm_grid_wrench_flush and mpm_grid_to_dense take a pool;
+                             mpm_grid_zero_box, mpm_blocks_from_box and the CPIC, rigid-body
+                             and adjoint entry points return MPM_ERR_ARG for one. */
+    int32_t pool_blocks;  /* slots in the node arrays when block_slot is set (slots >= it
+                             are treated as -1) */
+    int32_t pad2_;
 } mpm_grid_t;
 
 /* Model parameters (MPMModelStruct, warp_utils.py:23-84). Lives in device memory; the
@@ -460,6 +476,22 @@ int32_t mpm_sort_pairs(const int32_t *keys, int32_t *keys_out, const int32_t *va
 int64_t mpm_sort_chunks_temp_bytes(int32_t n);
 int32_t mpm_sort_chunks(const int32_t *keys, int32_t n, int32_t key_shift, int32_t nb, int32_t gz, int32_t chunk,
                         int32_t *chunk_start, int32_t *count, void *temp, int64_t temp_bytes, void *stream);
+
+/* ---- block pool ------------------------------------------------------------------------
+ * Sparse storage for the active blocks only (the batched scenes' grids: 256 scenes of
+ * 64^3 would need 95M nodes densely, the dough scenes touch ~10 % of them). Replaces the
+ * reference's dense grid under its sparse block lists (mpm_solver_warp.py:989-1026,
+ * blocks_compact mpm_utils.py:1532-1544): from the dilated active flags `cur` (1 =
+ * active, per global block code, n_codes = nb[0] nb[1] nb[2] n_scenes entries, as
+ * mpm_blocks_dilate writes them) it writes slot[code] = the number of active codes below
+ * `code` for an active code and -1 otherwise, list[slot] = code (the active codes in
+ * increasing order: the grid update's block list) and *count = the number of active
+ * codes. The caller sizes the node arrays to at least *count blocks, sets
+ * G->block_slot = slot and G->pool_blocks, and clears node_in / node_out of the slots in
+ * use (a slot may have held another block before). temp: mpm_pool_temp_bytes(n_codes). */
+int64_t mpm_pool_temp_bytes(int32_t n_codes);
+int32_t mpm_pool_assign(const mpm_grid_t *G, const int32_t *cur, int32_t *slot, int32_t *list, int32_t *count,
+                        void *temp, int64_t temp_bytes, void *stream);
 /* gather every particle array by perm (dst[i] = src[perm[i]]) through `scratch`
  * (>= 9 * stride floats) and copy back in place, so device pointers stay stable. */
 int32_t mpm_permute(const mpm_particles_t *P, const int32_t *perm, float *scratch, void *stream);

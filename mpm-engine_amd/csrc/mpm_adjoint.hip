@@ -454,8 +454,9 @@ inline bool adj_ok(const mpm_adjoint_t* A) {
 }
 
 inline bool grid_ok1(const mpm_grid_t* G) {
-    return G && G->node_in && G->node_out && G->n_scenes == 1 && G->n[0] > 0 && G->n[1] > 0 && G->n[2] > 0 &&
-           G->nb[0] == (G->n[0] + 3) / 4 && G->nb[1] == (G->n[1] + 3) / 4 && G->nb[2] == (G->n[2] + 3) / 4;
+    return G && G->node_in && G->node_out && G->n_scenes == 1 && !G->block_slot && G->n[0] > 0 && G->n[1] > 0 &&
+           G->n[2] > 0 && G->nb[0] == (G->n[0] + 3) / 4 && G->nb[1] == (G->n[1] + 3) / 4 &&
+           G->nb[2] == (G->n[2] + 3) / 4;
 }
 
 }  // namespace

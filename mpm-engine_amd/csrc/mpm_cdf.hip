@@ -108,7 +108,8 @@ __global__ void k_cdf_set_lanes(mpm_cdf_lanes_t* dst, mpm_cdf_lanes_t src) {
 }
 
 inline int cdf_ok(const mpm_grid_t* G) {
-    return G && G->n[0] > 0 && G->n[1] > 0 && G->n[2] > 0 && G->n_scenes == 1 && G->nb[0] == (G->n[0] + 3) / 4 &&
+    return G && !G->block_slot && G->n[0] > 0 && G->n[1] > 0 && G->n[2] > 0 && G->n_scenes == 1 &&
+           G->nb[0] == (G->n[0] + 3) / 4 &&
            G->nb[1] == (G->n[1] + 3) / 4 && G->nb[2] == (G->n[2] + 3) / 4 && G->cdf.tag && G->cdf.tag_prev &&
            G->cdf.dist && G->cdf.dist_prev && G->cdf.block && G->cdf.lanes && G->cdf.reaction;
 }

@@ -469,6 +469,23 @@ MPM_DEV size_t scene_base(const mpm_grid_t& G, int scene) {
     return (size_t)scene * (size_t)G.nb[0] * G.nb[1] * G.nb[2] * 64;
 }
 
+// Storage address (in nodes) of node (i, j, k) of the scene whose dense node offset is
+// soff (scene_base): dense storage keeps the block code's own 64 nodes; with a block pool
+// the block's slot holds them. -1: the block has no storage this tick.
+MPM_DEV long node_addr(const mpm_grid_t& G, size_t soff, int i, int j, int k) {
+    const int nidx = node_index(G, i, j, k);
+    if (!G.block_slot) return (long)soff + nidx;
+    const int s = G.block_slot[(soff >> 6) + (size_t)(nidx >> 6)];
+    return (s >= 0 && s < G.pool_blocks) ? ((long)s << 6) + (nidx & 63) : -1L;
+}
+
+// storage block of global block code `code` (dense: the code itself), -1 without storage
+MPM_DEV long block_storage(const mpm_grid_t& G, long code) {
+    if (!G.block_slot) return code;
+    const int s = G.block_slot[code];
+    return (s >= 0 && s < G.pool_blocks) ? (long)s : -1L;
+}
+
 MPM_DEV void decode_block(const mpm_grid_t& G, int code, int& bi, int& bj, int& bk) {
     bk = code % G.nb[2];
     int r = code / G.nb[2];

@@ -244,7 +244,9 @@ __global__ __launch_bounds__(kThreads) void k_grid_update(mpm_grid_t G, const mp
         int bi, bj, bk;
         decode_block(G, code - scene * bps, bi, bj, bk);
         const int gi = bi * 4 + (lane >> 4), gj = bj * 4 + ((lane >> 2) & 3), gk = bk * 4 + (lane & 3);
-        const size_t o = 4 * ((size_t)code * 64 + lane);
+        const long sb = block_storage(G, code);  // wavefront-uniform
+        if (sb < 0) continue;                    // a listed block without a pool slot: nothing stored
+        const size_t o = 4 * ((size_t)sb * 64 + lane);
         const bool valid = gi < G.n[0] && gj < G.n[1] && gk < G.n[2];
         const bool inbox = valid && gi >= blo0 && gi < bhi0 && gj >= blo1 && gj < bhi1 && gk >= blo2 && gk < bhi2;
         if (G.block_touch) {  // wavefront-uniform: one block per wavefront
@@ -398,7 +400,9 @@ __global__ void k_grid_zero_box(mpm_grid_t G, int lo0, int lo1, int lo2, int d0,
 __global__ void k_grid_zero_blocks(mpm_grid_t G, const int32_t* list, const int32_t* count, int which) {
     const int nblk = *count;
     for (int b = blockIdx.x * (kThreads / 64) + (threadIdx.x >> 6); b < nblk; b += gridDim.x * (kThreads / 64)) {
-        size_t o = 4 * ((size_t)list[b] * 64 + (threadIdx.x & 63));
+        const long sb = block_storage(G, list[b]);
+        if (sb < 0) continue;
+        size_t o = 4 * ((size_t)sb * 64 + (threadIdx.x & 63));
         const float4 z = make_float4(0.f, 0.f, 0.f, 0.f);
         if (which & 1) *reinterpret_cast<float4*>(G.node_in + o) = z;
         if (which & 2) *reinterpret_cast<float4*>(G.node_out + o) = z;
@@ -606,7 +610,14 @@ __global__ void k_grid_to_dense(mpm_grid_t G, float* m, float* vin, float* vout)
     long r = u / G.n[2];
     int j = (int)(r % G.n[1]);
     int i = (int)(r / G.n[1]);
-    size_t o = 4 * (scene_base(G, scene) + (size_t)node_index(G, i, j, k));
+    const long na = node_addr(G, scene_base(G, scene), i, j, k);
+    if (na < 0) {  // a block without pool storage reads as empty
+        if (m) m[t] = 0.0f;
+        if (vin) vin[3 * t] = vin[3 * t + 1] = vin[3 * t + 2] = 0.0f;
+        if (vout) vout[3 * t] = vout[3 * t + 1] = vout[3 * t + 2] = 0.0f;
+        return;
+    }
+    size_t o = 4 * (size_t)na;
     if (m) m[t] = G.node_in[o + 3];
     if (vin) { vin[3 * t] = G.node_in[o]; vin[3 * t + 1] = G.node_in[o + 1]; vin[3 * t + 2] = G.node_in[o + 2]; }
     if (vout) { vout[3 * t] = G.node_out[o]; vout[3 * t + 1] = G.node_out[o + 1]; vout[3 * t + 2] = G.node_out[o + 2]; }
@@ -618,8 +629,12 @@ inline int32_t launch_status() { return hipGetLastError() == hipSuccess ? MPM_OK
 
 inline int grid_ok(const mpm_grid_t* G) {
     return G && G->node_in && G->node_out && G->n[0] > 0 && G->n[1] > 0 && G->n[2] > 0 && G->n_scenes >= 1 &&
-           G->nb[0] == (G->n[0] + 3) / 4 && G->nb[1] == (G->n[1] + 3) / 4 && G->nb[2] == (G->n[2] + 3) / 4;
+           G->nb[0] == (G->n[0] + 3) / 4 && G->nb[1] == (G->n[1] + 3) / 4 && G->nb[2] == (G->n[2] + 3) / 4 &&
+           (!G->block_slot || G->pool_blocks > 0);
 }
+
+// entry points that address the node arrays densely
+inline int dense_grid_ok(const mpm_grid_t* G) { return grid_ok(G) && !G->block_slot; }
 
 inline int particles_ok(const mpm_particles_t* P) {
     return P && P->n >= 0 && P->stride >= P->n && P->x && P->v && P->F && P->F_trial && P->C && P->stress &&
@@ -695,7 +710,7 @@ int32_t mpm_struct_size(int32_t which) {
 }
 
 int32_t mpm_grid_zero_box(const mpm_grid_t* G, const int32_t* lo, const int32_t* hi, int32_t which, void* stream) {
-    if (!grid_ok(G) || !lo || !hi) return MPM_ERR_ARG;
+    if (!dense_grid_ok(G) || !lo || !hi) return MPM_ERR_ARG;
     int l[3], d[3];
     for (int a = 0; a < 3; ++a) {
         l[a] = lo[a] < 0 ? 0 : lo[a];
@@ -719,7 +734,7 @@ int32_t mpm_grid_zero_blocks(const mpm_grid_t* G, const int32_t* list, const int
 }
 
 int32_t mpm_blocks_from_box(const mpm_grid_t* G, mpm_tick_ctl_t* ctl, int32_t* list, void* stream) {
-    if (!grid_ok(G) || !ctl || !list) return MPM_ERR_ARG;
+    if (!dense_grid_ok(G) || !ctl || !list) return MPM_ERR_ARG;
     long total = (long)G->nb[0] * G->nb[1] * G->nb[2] * G->n_scenes;
     hipLaunchKernelGGL(k_blocks_from_box, dim3(blocks_for(total)), dim3(kThreads), 0, (hipStream_t)stream, *G, ctl,
                        list);

@@ -195,10 +195,25 @@ MPM_DEV bool scatter_tp_fix(const Stencil& st, float dx, float inv_dx, float mas
 template <bool kDet, class Mask>
 MPM_DEV void scatter_global(const mpm_grid_t& G, size_t soff, const Stencil& st, bool periodic, float mass, v3 v,
                             const m33& Cm, const m33& fs, Mask mask, int& err) {
-    float* __restrict__ gin = G.node_in + 4 * soff;
-    size_t last_blk = ~(size_t)0;  // block this particle marked last
+    float* __restrict__ gin = G.node_in;
+    size_t last_blk = ~(size_t)0;  // block code this particle marked last
+    // storage offset (x4) of node (ix, iy, iz), -1 without storage (error bit 512); marks
+    // the node's block as scattered into (a plain store, as in mark_tile_blocks)
+    auto target = [&](int ix, int iy, int iz) -> long {
+        const int nidx = node_index(G, ix, iy, iz);
+        if (G.block_touch) {
+            const size_t blk = (soff >> 6) + (size_t)(nidx >> 6);
+            if (blk != last_blk) {
+                last_blk = blk;
+                G.block_touch[blk] = 3;
+            }
+        }
+        const long na = node_addr(G, soff, ix, iy, iz);
+        if (na < 0) err |= 512;
+        return na < 0 ? -1L : 4 * na;
+    };
     if constexpr (kDet) {  // the same contributions as the LDS path, bit for bit
-        unsigned long long* __restrict__ gfix = reinterpret_cast<unsigned long long*>(G.node_fix) + 4 * soff;
+        unsigned long long* __restrict__ gfix = reinterpret_cast<unsigned long long*>(G.node_fix);
         const bool ok = scatter_tp_fix(st, G.dx, G.inv_dx, mass, v, Cm, fs, G.fix_scale[0], G.fix_scale[1], 0,
                                        [&](int i, int j, int k, unsigned long long q0, unsigned long long q1,
                                            unsigned long long q2, unsigned long long q3) {
@@ -206,14 +221,8 @@ MPM_DEV void scatter_global(const mpm_grid_t& G, size_t soff, const Stencil& st,
             if (periodic) ix = ((ix % G.n[0]) + G.n[0]) % G.n[0];
             const int iy = st.base[1] + j, iz = st.base[2] + k;
             if (!in_grid(G, ix, iy, iz)) return;
-            const size_t o = 4 * (size_t)node_index(G, ix, iy, iz);
-            if (G.block_touch) {
-                const size_t blk = (soff + o / 4) >> 6;
-                if (blk != last_blk) {
-                    last_blk = blk;
-                    G.block_touch[blk] = 3;
-                }
-            }
+            const long o = target(ix, iy, iz);
+            if (o < 0) return;
             atomicAdd(gfix + o + 0, q0);
             atomicAdd(gfix + o + 1, q1);
             atomicAdd(gfix + o + 2, q2);
@@ -237,14 +246,8 @@ MPM_DEV void scatter_global(const mpm_grid_t& G, size_t soff, const Stencil& st,
         v3 add = (v + mvmul(Cm, dpos)) * wm + mvmul(fs, dwt);
         float wmv = wm;
         if (!mask(i, j, k, add, wmv)) continue;
-        const size_t o = 4 * (size_t)node_index(G, ix, iy, iz);
-        if (G.block_touch) {
-            const size_t blk = (soff + o / 4) >> 6;
-            if (blk != last_blk) {  // mark as in mark_tile_blocks: a plain store
-                last_blk = blk;
-                G.block_touch[blk] = 3;
-            }
-        }
+        const long o = target(ix, iy, iz);
+        if (o < 0) continue;
         atomicAdd(gin + o + 0, add.x);
         atomicAdd(gin + o + 1, add.y);
         atomicAdd(gin + o + 2, add.z);
@@ -388,9 +391,10 @@ __global__ __launch_bounds__(kT) __attribute__((amdgpu_waves_per_eu(MPM_PASS_WAV
             }
             const Tile tg = make_tile<kT>(G, lo, hi, gather ? scene : INT_MAX, gather ? scene : INT_MIN, s_red0, periodic);
             {
-                const float4* __restrict__ out = reinterpret_cast<const float4*>(G.node_out) + tg.soff;
+                const float4* __restrict__ out = reinterpret_cast<const float4*>(G.node_out);
                 tile_for<kT>(tg, [&](int a, int gi, int gj, int gk) {
-                    s_gather[a] = in_grid(G, gi, gj, gk) ? out[node_index(G, gi, gj, gk)] : make_float4(0.f, 0.f, 0.f, 0.f);
+                    const long o = in_grid(G, gi, gj, gk) ? node_addr(G, tg.soff, gi, gj, gk) : -1L;
+                    s_gather[a] = o >= 0 ? out[o] : make_float4(0.f, 0.f, 0.f, 0.f);
                 });
             }
             __syncthreads();
@@ -410,15 +414,16 @@ __global__ __launch_bounds__(kT) __attribute__((amdgpu_waves_per_eu(MPM_PASS_WAV
                 } else {
                     // the same tensor-product sums, fed from global memory: a particle's
                     // gather is bitwise the same whichever path its chunk takes
-                    const float4* __restrict__ gout = reinterpret_cast<const float4*>(G.node_out) + soff;
+                    const float4* __restrict__ gout = reinterpret_cast<const float4*>(G.node_out);
                     gather_tp<true>(st, G.inv_dx,
                               [&](int i, int j, int k) {
                                   int ix = st.base[0] + i;
                                   if (periodic) ix = ((ix % G.n[0]) + G.n[0]) % G.n[0];
                                   const int iy = st.base[1] + j, iz = st.base[2] + k;
                                   v3 g = mk3(0.f, 0.f, 0.f);
-                                  if (in_grid(G, ix, iy, iz)) {
-                                      const float4 o4 = gout[node_index(G, ix, iy, iz)];
+                                  const long na = in_grid(G, ix, iy, iz) ? node_addr(G, soff, ix, iy, iz) : -1L;
+                                  if (na >= 0) {
+                                      const float4 o4 = gout[na];
                                       g = mk3(o4.x, o4.y, o4.z);
                                   }
                                   return subst(i, j, k, g);
@@ -639,13 +644,18 @@ __global__ __launch_bounds__(kT) __attribute__((amdgpu_waves_per_eu(MPM_PASS_WAV
             }
             __syncthreads();
             {
-                float* __restrict__ gin = G.node_in + 4 * ts.soff;
-                unsigned long long* __restrict__ gfix = reinterpret_cast<unsigned long long*>(G.node_fix) + 4 * ts.soff;
+                float* __restrict__ gin = G.node_in;
+                unsigned long long* __restrict__ gfix = reinterpret_cast<unsigned long long*>(G.node_fix);
                 tile_for<kT>(ts, [&](int a, int gi, int gj, int gk) {
                     const Acc a0 = s_scatter[0][a], a1 = s_scatter[1][a], a2 = s_scatter[2][a], a3 = s_scatter[3][a];
                     if (a0 == Acc(0) && a1 == Acc(0) && a2 == Acc(0) && a3 == Acc(0)) return;
                     if (!in_grid(G, gi, gj, gk)) return;
-                    const size_t o = 4 * (size_t)node_index(G, gi, gj, gk);
+                    const long na = node_addr(G, ts.soff, gi, gj, gk);
+                    if (na < 0) {  // a block without a pool slot: the contribution is dropped
+                        err |= 512;
+                        return;
+                    }
+                    const size_t o = 4 * (size_t)na;
                     if constexpr (kDet) {
                         atomicAdd(gfix + o + 0, a0);
                         atomicAdd(gfix + o + 1, a1);
@@ -686,6 +696,7 @@ extern "C" int32_t mpm_particle_pass(const mpm_particles_t* P, const mpm_grid_t*
     const bool det = (mode & MPM_PASS_DETERMINISTIC) != 0;
     if (det && (cdf || !G->node_fix || !(G->fix_scale[0] > 0.0f) || !(G->fix_scale[1] > 0.0f)))
         return MPM_ERR_ARG;
+    if (G->block_slot && (cdf || G->pool_blocks < 1)) return MPM_ERR_ARG;
     if (cdf && (G->n_scenes != 1 || !P->cdf_tag || !G->cdf.tag || !G->cdf.tag_prev || !G->cdf.dist ||
                 !G->cdf.dist_prev || !G->cdf.block || !G->cdf.lanes || !G->cdf.reaction || (mode & MPM_PASS_LEAN)))
         return MPM_ERR_ARG;

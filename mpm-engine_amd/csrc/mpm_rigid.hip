@@ -310,7 +310,7 @@ int32_t mpm_rigid_step(const mpm_particles_t* P, const mpm_grid_t* G, mpm_rigid_
     if (!P || !G || !bodies || n_bodies < 1 || n_planes < 0 || n_planes > 32 || (n_planes > 0 && !planes) ||
         !active || !workspace ||
         P->n < 0 || P->stride < P->n || !P->x || !P->v || !P->C || !P->F || !P->mass || !P->material ||
-        !P->selection || !P->rigid_id || !P->x_ref || !G->node_out || G->n_scenes != 1)
+        !P->selection || !P->rigid_id || !P->x_ref || !G->node_out || G->n_scenes != 1 || G->block_slot)
         return MPM_ERR_ARG;
     hipStream_t s = (hipStream_t)stream;
     const int nb = n_bodies, np = n_planes;

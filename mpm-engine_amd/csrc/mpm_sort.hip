@@ -137,3 +137,53 @@ int32_t mpm_sort_chunks(const int32_t* keys, int32_t n, int32_t key_shift, int32
 }
 
 }  // extern "C"
+
+// ------------------------------------------------------------------------ block pool
+// Slots of the active blocks: an exclusive scan of the 0/1 active flags gives each
+// active code its rank among the active codes (its slot, so slots follow code order and
+// spatial neighbours within a block column sit next to each other in the node arrays);
+// inactive codes get -1.
+namespace {
+
+__global__ void k_pool_fill(const int32_t* __restrict__ cur, int n, int32_t* __restrict__ slot,
+                            int32_t* __restrict__ list, int32_t* __restrict__ count) {
+    const int t = blockIdx.x * kSortThreads + threadIdx.x;
+    if (t >= n) return;
+    const int s = slot[t];  // rank among the active codes below t
+    const int on = cur[t] == 1;
+    if (on) list[s] = t;
+    slot[t] = on ? s : -1;
+    if (t == n - 1) *count = s + on;
+}
+
+}  // namespace
+
+extern "C" {
+
+int64_t mpm_pool_temp_bytes(int32_t n_codes) {
+    if (n_codes <= 0) return 0;
+    size_t scan = 0;
+    if (hipcub::DeviceScan::ExclusiveSum(nullptr, scan, (const int32_t*)nullptr, (int32_t*)nullptr, n_codes) !=
+        hipSuccess)
+        return -1;
+    return (int64_t)align256(scan);
+}
+
+int32_t mpm_pool_assign(const mpm_grid_t* G, const int32_t* cur, int32_t* slot, int32_t* list, int32_t* count,
+                        void* temp, int64_t temp_bytes, void* stream) {
+    if (!G || !cur || !slot || !list || !count || !temp || G->n_scenes < 1 || G->nb[0] <= 0 || G->nb[1] <= 0 ||
+        G->nb[2] <= 0)
+        return MPM_ERR_ARG;
+    const long total = (long)G->nb[0] * G->nb[1] * G->nb[2] * G->n_scenes;
+    if (total > 0x7fffffffL) return MPM_ERR_SIZE;
+    const int n = (int)total;
+    if (temp_bytes < mpm_pool_temp_bytes(n)) return MPM_ERR_SIZE;
+    hipStream_t s = (hipStream_t)stream;
+    size_t b = (size_t)temp_bytes;
+    if (hipcub::DeviceScan::ExclusiveSum(temp, b, cur, slot, n, s) != hipSuccess) return MPM_ERR_HIP;
+    hipLaunchKernelGGL(k_pool_fill, dim3((unsigned)((n + kSortThreads - 1) / kSortThreads)), dim3(kSortThreads), 0, s,
+                       cur, n, slot, list, count);
+    return hipGetLastError() == hipSuccess ? MPM_OK : MPM_ERR_HIP;
+}
+
+}  // extern "C"

@@ -3,11 +3,15 @@
 The reference runs one scene per ``Solver`` (src/warpmpm/core/solver.py:47-642); a sweep
 over hundreds of small scenes is a loop of solves whose launches each fill a small part
 of the GPU. ``BatchSolver`` keeps every scene's particles in one particle array (scene
-id per particle, scenes contiguous) and every scene's grid in one buffer (scene s owns
-blocks [s * B, (s + 1) * B)), so each phase of a substep is one launch over all scenes:
-the fused particle pass (block-sorted chunks never mix scenes inside an LDS tile), the
-grid update over one shared list of active blocks, and the per-scene collider rows.
-A tick of `substeps` substeps is captured once as a hipGraph and replayed.
+id per particle, scenes contiguous) and every scene's grid in one block-code space
+(scene s owns block codes [s * B, (s + 1) * B)), so each phase of a substep is one
+launch over all scenes: the fused particle pass (block-sorted chunks never mix scenes
+inside an LDS tile), the grid update over one shared list of active blocks, and the
+per-scene collider rows. With sparse stepping the node storage is a shared block pool:
+at every tick's rebuild the active blocks of all scenes get consecutive slots
+(mpm_pool_assign), so the grid takes memory in proportion to the active blocks, not to
+n_scenes * n^3. A tick of `substeps` substeps is captured once as a hipGraph and
+replayed.
 
 Per-scene parameters: material type parameters are shared across scenes (per-type
 arrays, as in the reference's model); yield stress, mass and the Lame parameters are
@@ -55,8 +59,11 @@ class BatchSolver:
 
     def __init__(self, grid: GridConfig | None = None, device: str = "auto", sparse: bool = True,
                  fused: bool = True, sort_interval: int = 1, guard_interval: int = 1,
-                 wrench_slots_per_scene: int = 2, deterministic: bool = True):
+                 wrench_slots_per_scene: int = 2, deterministic: bool = True, pool: bool = True):
         self.grid = grid or GridConfig()
+        # sparse storage: node arrays for the active blocks of all scenes only (with
+        # sparse stepping), instead of a dense n^3 grid per scene
+        self.pool = bool(pool) and bool(sparse)
         # 64-bit fixed-point scatter and wrench sums (bit-reproducible, and faster than
         # float atomics here); False = float atomics
         self.deterministic = bool(deterministic)
@@ -91,6 +98,8 @@ class BatchSolver:
         sim = self._sim
         sim.scene_colliders = SceneColliders(len(counts), self.grid.n_grid, sim.dx,
                                              self.slots_per_scene)
+        if self.pool:
+            sim.enable_pool()
         if self.deterministic:
             sim.enable_deterministic()
         torch = sim.torch
@@ -270,6 +279,9 @@ class BatchSolver:
             raise RuntimeError("internal: a solid material ran in the viscous-only particle pass")
         if flag & native.ERR_STENCIL:
             raise RuntimeError("a particle stencil left its scene's grid during the last tick")
+        if flag & native.ERR_POOL:
+            raise RuntimeError("a particle scattered into a block outside its scene's active set "
+                               "during the last tick (it moved more than a block within the tick)")
         if bad or lo_x.min() < 1.5 * dx or hi_x.max() > lim - 2.5 * dx:
             raise RuntimeError(
                 f"particles of some scene within 2 cells of the grid edge (x in "

@@ -23,7 +23,7 @@ VISCOUS_IDS = frozenset((6, 7, 8, 10, 12))
 GRID_ZERO_IN, GRID_FIXED_IN, GRID_WRENCH_DEFER = 1, 2, 4
 MAX_FIXED_ORDER_SLOTS = 64  # wrench slots reduced in fixed order; more use float atomics
 ERR_STENCIL, ERR_NONFINITE, ERR_SOLID_IN_VISCOUS, ERR_SCENE = 1, 2, 4, 8
-ERR_ADJ_MATERIAL, ERR_ADJ_COLLIDER, ERR_FIXED_RANGE, ERR_CHUNKS = 16, 32, 64, 256
+ERR_ADJ_MATERIAL, ERR_ADJ_COLLIDER, ERR_FIXED_RANGE, ERR_CHUNKS, ERR_POOL = 16, 32, 64, 256, 512
 COLLIDER_PLANE, COLLIDER_BOX, COLLIDER_CUP, COLLIDER_WALLS, COLLIDER_SDF, COLLIDER_POINTCLOUD = \
     1, 2, 3, 4, 5, 6
 
@@ -74,7 +74,8 @@ class Grid(C.Structure):
     _fields_ = [("node_in", _FP), ("node_out", _FP), ("n", C.c_int32 * 3),
                 ("nb", C.c_int32 * 3), ("dx", C.c_float), ("inv_dx", C.c_float),
                 ("error_flag", _FP), ("n_scenes", C.c_int32), ("pad_", C.c_int32),
-                ("cdf", Cdf), ("node_fix", _FP), ("fix_scale", C.c_float * 2), ("block_touch", _FP)]
+                ("cdf", Cdf), ("node_fix", _FP), ("fix_scale", C.c_float * 2), ("block_touch", _FP),
+                ("block_slot", _FP), ("pool_blocks", C.c_int32), ("pad2_", C.c_int32)]
 
 
 _L3 = (np.float32, (MAX_CDF, 3))
@@ -143,6 +144,8 @@ SIGNATURES = {
     "mpm_permute": (_I32, [_P, _P, _P, _P]),
     "mpm_sort_chunks_temp_bytes": (_I64, [_I32]),
     "mpm_sort_chunks": (_I32, [_P, _I32, _I32, _I32, _I32, _I32, _P, _P, _P, _I64, _P]),
+    "mpm_pool_temp_bytes": (_I64, [_I32]),
+    "mpm_pool_assign": (_I32, [_P, _P, _P, _P, _P, _P, _I64, _P]),
     "mpm_export_R": (_I32, [_P, _P, _P]),
     "mpm_export_cov": (_I32, [_P, _P, _P, _P]),
     "mpm_grid_to_dense": (_I32, [_P, _P, _P, _P, _P]),
@@ -167,7 +170,7 @@ SIGNATURES = {
 }
 
 _LIB = None
-ABI_VERSION = 7  # MPM_ABI_VERSION of include/mpm_hip.h this binding is written for
+ABI_VERSION = 8  # MPM_ABI_VERSION of include/mpm_hip.h this binding is written for
 
 
 class NativeError(RuntimeError):

@@ -209,6 +209,12 @@ class HipSimulator:
         self.chunk_start = None
         self.chunk_count = None
         self.chunk_capacity = 0
+        # block pool (enable_pool): node storage for the active blocks only
+        self.pool = False
+        self.block_slot = None
+        self.pool_list = None
+        self.pool_blocks = 0
+        self._pool_temp = None
         self._vmax = 0.0
         self._fix_exp = None
         self._structs()
@@ -240,6 +246,8 @@ class HipSimulator:
             self.cdf.fill(G)
         G.node_fix = native.ptr(self.node_fix)
         G.block_touch = native.ptr(self.block_touch)
+        G.block_slot = native.ptr(self.block_slot)
+        G.pool_blocks = self.pool_blocks
         if self._fix_exp is not None:
             G.fix_scale[0] = math.ldexp(1.0, self._fix_exp[0])
             G.fix_scale[1] = math.ldexp(1.0, self._fix_exp[1])
@@ -572,7 +580,8 @@ class HipSimulator:
             raise NotImplementedError("the deterministic scatter does not cover CDF colliders")
         self.deterministic = True
         if self.node_fix is None:
-            self.node_fix = self.torch.zeros(self.n_blocks * 64 * 4, dtype=self.torch.int64, device=self.device)
+            blocks = self.pool_blocks if self.pool else self.n_blocks
+            self.node_fix = self.torch.zeros(blocks * 64 * 4, dtype=self.torch.int64, device=self.device)
         self.grid_ws = None  # the fixed-point wrench needs a zeroed workspace
         self._graph = None
         self._structs()
@@ -598,6 +607,43 @@ class HipSimulator:
             self._fix_exp = exp
             self._structs()
 
+    # --------------------------------------------------------------- block pool
+    def enable_pool(self, initial_blocks: int = 64):
+        """Node storage for the active blocks only (mpm_pool_assign): node_in, node_out and
+        node_fix hold `pool_blocks` blocks, handed out again at every active-block rebuild
+        and grown (never shrunk) when the active set outgrows them. Needs the sparse
+        stepping (a rebuild before every tick)."""
+        if self.cdf is not None or self.n_rigid_bodies:
+            raise NotImplementedError("the block pool does not cover CPIC colliders or rigid bodies")
+        torch = self.torch
+        self.pool = True
+        self.block_slot = torch.full((self.n_blocks,), -1, dtype=torch.int32, device=self.device)
+        self.pool_list = torch.zeros(self.n_blocks, dtype=torch.int32, device=self.device)
+        tb = int(self.lib.mpm_pool_temp_bytes(self.n_blocks))
+        if tb < 0:
+            raise RuntimeError("mpm_pool_temp_bytes failed")
+        self._pool_temp = torch.empty(max(tb, 1), dtype=torch.uint8, device=self.device)
+        self._pool_resize(initial_blocks)
+        self.sparse_ready = False
+
+    def _pool_resize(self, blocks: int):
+        torch = self.torch
+        blocks = max(64, (int(blocks) + 63) // 64 * 64)
+        self.pool_blocks = blocks
+        self.node_in = torch.zeros(blocks * 64 * 4, dtype=torch.float32, device=self.device)
+        self.node_out = torch.zeros(blocks * 64 * 4, dtype=torch.float32, device=self.device)
+        if self.node_fix is not None:
+            self.node_fix = torch.zeros(blocks * 64 * 4, dtype=torch.int64, device=self.device)
+        self._graph = None
+        self._structs()
+
+    def grid_bytes(self) -> int:
+        """Device bytes of the grid storage: node arrays, block states and the pool table."""
+        ts = [self.node_in, self.node_out, self.node_fix, self.block_touch, self.block_slot, self.pool_list]
+        if self._blk is not None:
+            ts += list(self._blk.values())
+        return int(sum(t.numel() * t.element_size() for t in ts if t is not None))
+
     # --------------------------------------------------------------- sparse blocks
     def rebuild_active_blocks(self):
         """rebuild_active_blocks (mpm_solver_warp.py:989-1026), fully on device: the
@@ -618,9 +664,31 @@ class HipSimulator:
         native.check(L.mpm_blocks_mark(self.pP, self.pG, native.ptr(b["raw"]), s), "blocks_mark")
         native.check(L.mpm_blocks_dilate(self.pG, native.ptr(b["raw"]), native.ptr(b["cur"]), s),
                      "blocks_dilate")
+        if self.pool:
+            self._pool_assign(b, s)
+            return
         native.check(L.mpm_blocks_compact(self.pG, native.ptr(b["cur"]), native.ptr(b["prev"]),
                                           native.ptr(b["cur_list"]), native.ptr(b["union_list"]),
                                           native.ptr(b["counts"]), s), "blocks_compact")
+        self.sparse_ready = True
+
+    def _pool_assign(self, b, s):
+        """Slots for this tick's active blocks (in block-code order; the grid update walks
+        them as its list), the node arrays grown to hold them, and the slots in use
+        cleared: a slot may have held another block last tick."""
+        native.check(self.lib.mpm_pool_assign(self.pG, native.ptr(b["cur"]), native.ptr(self.block_slot),
+                                              native.ptr(self.pool_list), native.ptr(b["counts"]),
+                                              native.ptr(self._pool_temp), self._pool_temp.numel(), s),
+                     "pool_assign")
+        count = int(b["counts"][0].item())
+        if count > self.pool_blocks:
+            self._pool_resize(count + count // 4)
+        used = count * 64 * 4
+        self.node_in[:used].zero_()
+        self.node_out[:used].zero_()
+        if self.node_fix is not None:
+            self.node_fix[:used].zero_()
+        self.block_touch.zero_()
         self.sparse_ready = True
 
     def active_block_fraction(self) -> float:
@@ -761,8 +829,10 @@ class HipSimulator:
             ctl["box_lo"], ctl["box_hi"] = (0, 0, 0), (self.n_grid,) * 3
             b = self._blk
             zl, zc = b["union_list"], b["counts"][1:]
-            glist, gcount = b["cur_list"], b["counts"][0:]
+            glist, gcount = (self.pool_list if self.pool else b["cur_list"]), b["counts"][0:]
             self._prev_grid_box = None
+        elif self.pool:
+            raise RuntimeError("the block pool needs the sparse stepping (a block rebuild before every tick)")
         else:
             g, z = self._boxes()
             ctl["box_lo"] = g[0]
@@ -773,8 +843,9 @@ class HipSimulator:
         self._h2d_done.record()
         # clear the zero box (union of this tick's and the previous tick's box)
         if sparse:
-            native.check(L.mpm_grid_zero_blocks(self.pG, native.ptr(zl), native.ptr(zc),
-                                                self.n_blocks, 3, s), "zero_blocks")
+            if not self.pool:  # (the pool clears its slots at every rebuild)
+                native.check(L.mpm_grid_zero_blocks(self.pG, native.ptr(zl), native.ptr(zc),
+                                                    self.n_blocks, 3, s), "zero_blocks")
         else:
             lo = (C.c_int32 * 3)(*z[0])
             hi = (C.c_int32 * 3)(*[z[0][i] + z[1][i] for i in range(3)])
@@ -1461,6 +1532,9 @@ class Solver:
                                "(the state is far outside the expected momentum scale)")
         if flag & native.ERR_CHUNKS:
             raise RuntimeError("internal: the particle pass launched fewer chunks than the sort produced")
+        if flag & native.ERR_POOL:
+            raise RuntimeError("a particle scattered into a block outside the active set during the last "
+                               "tick (it moved more than a block within the tick)")
         if flag & 1:
             raise RuntimeError("a particle stencil left the grid during the last tick (out of "
                                "bounds P2G was skipped); enlarge grid_lim or add a wall collider")
