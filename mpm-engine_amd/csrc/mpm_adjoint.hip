@@ -13,8 +13,8 @@
 // the B-spline weights use the weight Hessian, so gradients with respect to positions
 // are exact for the discrete map (between cell crossings of the stencil base).
 //
-// Return-map adjoints (von Mises, material 1, and local mu(I), material 9, both with
-// Hencky elasticity): F_trial = U S V^T; the stress tau = U diag(h) U^T and the
+// Return-map adjoints (von Mises, material 1, with or without hardening, and local mu(I),
+// material 9, both with Hencky elasticity): F_trial = U S V^T; the stress tau = U diag(h) U^T and the
 // corrected F = U diag(g) V^T are isotropic functions of F_trial, and both maps scale
 // the deviatoric log-strain by one factor k, so they share the assembly below. Their
 // derivatives are assembled in the SVD frame from the diagonal Jacobians dh/dsigma, dg/dsigma and, per pair (a, b), the divided differences
@@ -252,6 +252,9 @@ MPM_DEV void adj_node(const mpm_grid_t& G, const mpm_model_t* md, const mpm_coll
                 }
             } else if (c.kind == MPM_COLLIDER_POINTCLOUD) {
                 if (live && c.occupancy[((size_t)gi * G.n[1] + gj) * G.n[2] + gk] == 1) uo = mk3(0.f, 0.f, 0.f);
+            } else if (c.kind == MPM_COLLIDER_CUP || c.kind == MPM_COLLIDER_SDF) {
+                SurfaceHit h;
+                if (live && surface_hit(c, G, gi, gj, gk, h)) uo = surface_apply(h, c.friction, uo);
             } else {
                 err |= kErrAdjCollider;
             }
@@ -316,6 +319,18 @@ MPM_DEV void adj_node(const mpm_grid_t& G, const mpm_model_t* md, const mpm_coll
             if ((gk < pad && ui.z < 0) || (gk >= G.n[2] - pad && ui.z > 0)) ub.z = 0.0f;
         } else if (c.kind == MPM_COLLIDER_POINTCLOUD) {
             if (live && c.occupancy[((size_t)gi * G.n[1] + gj) * G.n[2] + gk] == 1) ub = mk3(0.f, 0.f, 0.f);
+        } else if (c.kind == MPM_COLLIDER_CUP || c.kind == MPM_COLLIDER_SDF) {
+            SurfaceHit h;
+            if (!live || !surface_hit(c, G, gi, gj, gk, h)) continue;
+            // impulse m (u_in - u_new) into the force slot, r x impulse into the torque slot
+            v3 ib = mk3(0.f, 0.f, 0.f), tb = mk3(0.f, 0.f, 0.f);
+            if (c.slot >= 0 && c.slot < nslots) {
+                ib = mk3(impb[6 * c.slot], impb[6 * c.slot + 1], impb[6 * c.slot + 2]);
+                tb = mk3(impb[6 * c.slot + 3], impb[6 * c.slot + 4], impb[6 * c.slot + 5]);
+            }
+            const v3 pb = ib + cross(tb, h.rel);  // adjoint of the impulse
+            mb += dot(pb, ui - surface_apply(h, c.friction, ui));
+            ub = pb * m + surface_apply_T(h, c.friction, ui, ub - pb * m);
         }
     }
     if (damp < 1.0f) ub = ub * damp;
@@ -412,12 +427,16 @@ __global__ __launch_bounds__(kThreads) void k_adj_p2g_stress(mpm_particles_t P, 
         Ftb = adj_sand(ld9(P.F_trial, S, p), P.mu[p], P.lam[p], md->alpha[mat], Fb, tb, mub, lamb);
         A.mu[p] += mub;
         A.lam[p] += lamb;
-    } else if (mat == 1 && md->hardening[mat] <= 0.5f) {
+    } else if (mat == 1) {
+        // A.yield_stress holds the adjoint of the yield stress after this substep (hardened
+        // or passed on) and leaves with the adjoint of the one this substep read
         float mub = 0.f, lamb = 0.f, yb = 0.f;
-        Ftb = adj_von_mises(ld9(P.F_trial, S, p), P.mu[p], P.lam[p], P.yield_stress[p], Fb, tb, mub, lamb, yb);
+        const float xi = md->hardening[mat] > 0.5f ? md->xi[mat] : 0.0f;
+        Ftb = adj_von_mises(ld9(P.F_trial, S, p), P.mu[p], P.lam[p], P.yield_stress[p], xi, A.yield_stress[p], Fb,
+                            tb, mub, lamb, yb);
         A.mu[p] += mub;
         A.lam[p] += lamb;
-        A.yield_stress[p] += yb;
+        A.yield_stress[p] = yb;
     } else if (mat == 9) {
         float mub = 0.f, lamb = 0.f, pb[4] = {0.f, 0.f, 0.f, 0.f};
         Ftb = adj_mu_i(ld9(P.F_trial, S, p), md, mat, dt, P.mu[p], P.lam[p], Fb, tb, mub, lamb, pb);

@@ -1,7 +1,8 @@
 // Adjoints of the constitutive updates (return maps + Kirchhoff stress) used by the
 // reverse substep (mpm_adjoint.hip): fixed-corotated elasticity (material 0), von Mises
 // (material 1), Drucker-Prager sand (material 2), local mu(I) (material 9) and the
-// generalized-Newtonian fluid (material 10). Pure register math on the types of mpm_device.h; a host-side check compiles it
+// generalized-Newtonian fluid (material 10); and the contact maps of the surface colliders
+// (revolved cup, mesh SDF) with their transposed Jacobians. Pure register math on the types of mpm_device.h; a host-side check compiles it
 // with MPM_DEV = __host__ __device__ and compares it with float64 finite differences of
 // the forward maps.
 #pragma once
@@ -65,11 +66,14 @@ MPM_DEV void iso_adjoint(const m33& U, const m33& V, const float s[3], const flo
     }
 }
 
-// Adjoint of the von Mises return map + Hencky stress (mat 1, no hardening) at F_trial.
-// In: Fb = adjoint of the corrected F, tb = adjoint of tau. Out: adjoint of F_trial and
-// the per-particle parameter adjoints (added).
-MPM_DEV m33 adj_von_mises(const m33& Ft, float mu, float lam, float ys, const m33& Fb, const m33& tb_in,
-                          float& mub, float& lamb, float& yb) {
+// Adjoint of the von Mises return map + Hencky stress (mat 1) at F_trial. In: Fb = adjoint
+// of the corrected F, tb = adjoint of tau, ysb_next = adjoint of the yield stress the map
+// leaves behind, which with hardening (xi > 0; xi = 0 without) is ys + 2 mu xi dg when
+// yielding (mpm_utils.py:164-167) and ys otherwise. Out: adjoint of F_trial; the mu and
+// lam adjoints are added, and yb receives the adjoint of the yield stress the map read
+// (ysb_next carried through plus this substep's own dependence).
+MPM_DEV m33 adj_von_mises(const m33& Ft, float mu, float lam, float ys, float xi, float ysb_next, const m33& Fb,
+                          const m33& tb_in, float& mub, float& lamb, float& yb) {
     m33 U, V;
     v3 s3;
     svd3(Ft, U, s3, V);
@@ -96,6 +100,7 @@ MPM_DEV m33 adj_von_mises(const m33& Ft, float mu, float lam, float ys, const m3
     float enb[3];
     iso_adjoint(U, V, s, e, en, k, mu, lam, yield, Fb, tb_in, Al, enb, mub, lamb);
     float eb[3];
+    yb += ysb_next;  // the yield stress passes on (the whole of it below the yield surface)
     if (yield) {
         // en = me 1 + (ys / (2 mu hn)) eh with hn = |eh| + 1e-6
         const float ehn = hn - 1e-6f;
@@ -109,6 +114,16 @@ MPM_DEV m33 adj_von_mises(const m33& Ft, float mu, float lam, float ys, const m3
         for (int a = 0; a < 3; ++a) eb[a] = mb + kk * (tmp[a] - mt);
         yb += dotv / (2.0f * mu * hn);
         mub += -ys * dotv / (2.0f * mu * mu * hn);
+        if (xi != 0.0f && ysb_next != 0.0f) {
+            // hardening: ys' = ys + 2 mu xi dg, dg = |eh| + 1e-6 - ys / (2 mu), so
+            // d ys'/d ys = 1 - xi, d ys'/d mu = 2 xi dg + xi ys / mu and
+            // d ys'/d e = 2 mu xi eh / |eh| (eh is deviatoric: no mean to remove)
+            const float dg = hn - ys / (2.0f * mu);
+            yb -= xi * ysb_next;
+            mub += ysb_next * (2.0f * xi * dg + xi * ys / mu);
+            const float ce = ehn > 0.0f ? 2.0f * mu * xi * ysb_next / ehn : 0.0f;
+            for (int a = 0; a < 3; ++a) eb[a] += ce * eh[a];
+        }
     } else {
         for (int a = 0; a < 3; ++a) eb[a] = enb[a];
     }
@@ -366,4 +381,103 @@ MPM_DEV m33 adj_viscous(const m33& Ft, const m33& L, float bulk, float eta, floa
     const float Jcb = 3.0f * Jc * Jc * Jb + (Fb.a[0] + Fb.a[4] + Fb.a[8]);
     const float db = Jc != 0.0f ? Jcb / (3.0f * Jc * Jc) : 0.0f;
     return mscale(mcof(Ft), db);
+}
+
+// ------------------------------------------- surface colliders (revolved cup, mesh SDF)
+// The contact of a cup or mesh-SDF collider with one node, recomputed with the forward
+// rule's expressions (apply_collider of the grid update, mpm_solver_warp.py:2161-2225 and
+// :2680-2736) so that the branch taken is the forward's. The normal (central differences
+// of the cup SDF, trilinear SDF gradient of the mesh), the surface velocity v + w x r
+// and the lever arm r depend on the node position and the collider pose only: constants
+// of the substep for the state adjoint.
+struct SurfaceHit {
+    int mode;  // 1 grab (u = vs), 2 separable Coulomb on u - vs, 3 slip, 4 pass (cup, no normal)
+    bool cup;  // the cup's form of the Coulomb scaling (else the mesh collider's)
+    v3 vs, n, rel;
+};
+
+MPM_DEV bool surface_hit(const mpm_collider_t& c, const mpm_grid_t& G, int gi, int gj, int gk, SurfaceHit& h) {
+    const v3 xn = mk3((float)gi * G.dx, (float)gj * G.dx, (float)gk * G.dx);
+    h.rel = xn - mk3(c.point[0], c.point[1], c.point[2]);
+    h.vs = mk3(c.velocity[0], c.velocity[1], c.velocity[2]) + cross(mk3(c.omega[0], c.omega[1], c.omega[2]), h.rel);
+    h.n = mk3(0.f, 0.f, 0.f);
+    h.cup = c.kind == MPM_COLLIDER_CUP;
+    if (h.cup) {
+        m33 R;
+#pragma unroll
+        for (int q = 0; q < 9; ++q) R.a[q] = c.rot[q];
+        const v3 loc = mTvmul(R, h.rel);
+        const float sdf = glass_sdf(loc, c);
+        if (!(sdf < c.contact_band)) return false;
+        if (!(sdf > -c.sticky_depth)) {
+            h.mode = 1;
+            return true;
+        }
+        const float hd = 0.25f * G.dx;
+        const v3 nl = mk3(glass_sdf(loc + mk3(hd, 0, 0), c) - glass_sdf(loc - mk3(hd, 0, 0), c),
+                          glass_sdf(loc + mk3(0, hd, 0), c) - glass_sdf(loc - mk3(0, hd, 0), c),
+                          glass_sdf(loc + mk3(0, 0, hd), c) - glass_sdf(loc - mk3(0, 0, hd), c));
+        const float nlen = len(nl);
+        if (!(nlen > 1.0e-12f)) {
+            h.mode = 4;
+            return true;
+        }
+        h.n = mvmul(R, vdiv(nl, nlen));
+        h.mode = 2;
+        return true;
+    }
+    const v3 body = quat_rotate(c.quat, h.rel, true);
+    const v3 f = mk3((body.x - c.origin[0]) / c.cell, (body.y - c.origin[1]) / c.cell, (body.z - c.origin[2]) / c.cell);
+    const float rf = (float)c.res - 1.0f;
+    if (!(f.x >= 0.0f && f.y >= 0.0f && f.z >= 0.0f && f.x <= rf && f.y <= rf && f.z <= rf)) return false;
+    float sd;
+    trilerp(c.sdf_val, 1, c.res, f, &sd);
+    if (!(sd <= c.band)) return false;
+    float g[3];
+    trilerp(c.sdf_grad, 3, c.res, f, g);
+    const v3 gr = mk3(g[0], g[1], g[2]);
+    if (!(len(gr) > 1.0e-12f)) return false;
+    h.n = quat_rotate(c.quat, normalize(gr), false);
+    h.mode = c.surface == 0 ? 1 : (c.surface == 1 ? 3 : 2);
+    return true;
+}
+
+// the node velocity after the contact (forward)
+MPM_DEV v3 surface_apply(const SurfaceHit& h, float fr, v3 vin) {
+    if (h.mode == 1) return h.vs;
+    if (h.mode == 4) return vin;
+    const v3 vr = vin - h.vs;
+    const float vn = dot(vr, h.n);
+    if (h.mode == 3) return h.vs + (vr - h.n * vn);
+    if (!(vn < 0.0f)) return vin;
+    v3 vt = vr - h.n * vn;
+    if (h.cup) {
+        const float vtl = len(vt);
+        if (vtl > 1.0e-20f) vt = vdiv(vt, vtl) * fmaxf(vtl + fr * vn, 0.0f);
+    } else {
+        const float tl = len(vt);
+        if (tl > 1.0e-12f) vt = vt * (fmaxf(0.0f, tl + fr * vn) / tl);
+    }
+    return h.vs + vt;
+}
+
+// J^T g for J = d(velocity after the contact) / d(velocity before), at vin. Coulomb: with
+// vr = vin - vs, vn = vr . n, vt = vr - n vn, the result is vs + vt (1 + fr vn / |vt|), so
+// J^T g = (1 + fr vn / |vt|) P g + n fr (vt . g) / |vt| - vt fr vn (vt . g) / |vt|^3, P the
+// tangential projector; zero where the friction stops the tangential motion.
+MPM_DEV v3 surface_apply_T(const SurfaceHit& h, float fr, v3 vin, v3 g) {
+    if (h.mode == 1) return mk3(0.f, 0.f, 0.f);
+    if (h.mode == 4) return g;
+    const v3 vr = vin - h.vs;
+    const float vn = dot(vr, h.n);
+    const v3 Pg = g - h.n * dot(h.n, g);
+    if (h.mode == 3) return Pg;
+    if (!(vn < 0.0f)) return g;
+    const v3 vt = vr - h.n * vn;
+    const float tl = len(vt);
+    if (!(tl > (h.cup ? 1.0e-20f : 1.0e-12f))) return Pg;
+    if (!(tl + fr * vn > 0.0f)) return mk3(0.f, 0.f, 0.f);
+    const float sc = 1.0f + fr * vn / tl;
+    const float a = dot(vt, g);
+    return Pg * sc + h.n * (fr * a / tl) - vt * (fr * vn * a / (tl * tl * tl));
 }

@@ -459,6 +459,25 @@ MPM_DEV v3 quat_rotate(const float* q, v3 v, bool inverse) {
     return v * (2.0f * w * w - 1.0f) + qv * (2.0f * dot(qv, v)) + cross(qv, v) * ((inverse ? -w : w) * 2.0f);
 }
 
+// ------------------------------------------------------------------ collider SDFs
+// capped_cylinder_sdf / revolved_glass_sdf, mpm_solver_warp.py:29-57 (the revolved cup:
+// grid update and its adjoint)
+MPM_DEV float capped_cylinder_sdf(float r, float z, float radius, float z0, float z1) {
+    float zc = 0.5f * (z0 + z1), hh = 0.5f * (z1 - z0);
+    float dr = r - radius, dz = fabsf(z - zc) - hh;
+    float dro = fmaxf(dr, 0.0f), dzo = fmaxf(dz, 0.0f);
+    return sqrtf(dro * dro + dzo * dzo) + fminf(fmaxf(dr, dz), 0.0f);
+}
+
+MPM_DEV float glass_sdf(v3 l, const mpm_collider_t& c) {
+    float r = sqrtf(l.x * l.x + l.y * l.y);
+    float d_outer = capped_cylinder_sdf(r, l.z, c.outer_radius, -c.half_height, c.half_height);
+    float d_cav = capped_cylinder_sdf(r, l.z, c.inner_radius - c.fillet_radius,
+                                      c.inner_floor_z + c.fillet_radius, c.half_height + c.outer_radius) -
+                  c.fillet_radius;
+    return fmaxf(d_outer, -d_cav);
+}
+
 // ------------------------------------------------------------- layout helpers
 MPM_DEV int node_index(const mpm_grid_t& G, int i, int j, int k) {
     return ((((i >> 2) * G.nb[1] + (j >> 2)) * G.nb[2] + (k >> 2)) << 6) + ((i & 3) << 4) +

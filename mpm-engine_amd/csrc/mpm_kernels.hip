@@ -14,23 +14,6 @@ namespace {
 constexpr int kThreads = 256;
 
 // ------------------------------------------------------------------------ colliders
-// capped_cylinder_sdf / revolved_glass_sdf, mpm_solver_warp.py:29-57
-MPM_DEV float capped_cylinder_sdf(float r, float z, float radius, float z0, float z1) {
-    float zc = 0.5f * (z0 + z1), hh = 0.5f * (z1 - z0);
-    float dr = r - radius, dz = fabsf(z - zc) - hh;
-    float dro = fmaxf(dr, 0.0f), dzo = fmaxf(dz, 0.0f);
-    return sqrtf(dro * dro + dzo * dzo) + fminf(fmaxf(dr, dz), 0.0f);
-}
-
-MPM_DEV float glass_sdf(v3 l, const mpm_collider_t& c) {
-    float r = sqrtf(l.x * l.x + l.y * l.y);
-    float d_outer = capped_cylinder_sdf(r, l.z, c.outer_radius, -c.half_height, c.half_height);
-    float d_cav = capped_cylinder_sdf(r, l.z, c.inner_radius - c.fillet_radius,
-                                      c.inner_floor_z + c.fillet_radius, c.half_height + c.outer_radius) -
-                  c.fillet_radius;
-    return fmaxf(d_outer, -d_cav);
-}
-
 // Applies collider c to one node; returns the reaction impulse in imp (zero if untouched).
 MPM_DEV void apply_collider(const mpm_collider_t& c, const mpm_grid_t& G, int gi, int gj, int gk,
                             float m, v3& vo, v3& imp, v3& torque) {

@@ -10,12 +10,14 @@ final positions and the accumulated collider wrench, so a loss on the final stat
 the plate reaction forces back-propagates to Young's modulus, Poisson's ratio, the yield
 stress (through the per-particle parameters), the fluid's rheology, the granular
 friction law and to the initial particle state. Materials: fixed-corotated jelly (0),
-von Mises (1), Drucker-Prager sand (2), local mu(I) (9) and the generalized-Newtonian
-fluid (10).
+von Mises (1, with or without hardening: the hardened yield stress is carried as
+particle state, its adjoint substep to substep), Drucker-Prager sand (2), local mu(I)
+(9) and the generalized-Newtonian fluid (10).
 
 Forward: fused ticks of `checkpoint_every` substeps (the production substep), storing
 the particle state (x, v, C, F_trial and the velocity gradient L, which the viscous
-stress of the next substep reads) at every tick boundary. Backward, segment by
+stress of the next substep reads, and the yield stress, which hardening changes) at every
+tick boundary. Backward, segment by
 segment from the last: restore the checkpoint, recompute the segment substep by
 substep with the split passes while keeping every intermediate particle state and grid,
 then run the hand-written adjoint kernels of libmpm_hip.so (mpm_adjoint_g2p,
@@ -35,9 +37,11 @@ import numpy as np
 from .core import native
 
 _STATE_PLANES = 24   # x 3, v 3, C 9, F_trial 9
-_SLOT_PLANES = 51    # + F 9, stress 9, L 9
+_SLOT_PLANES = 52    # + F 9, stress 9, L 9, yield stress 1
 _L0 = 42             # first L plane of a slot
-_CKPT_PLANES = 33    # state + L
+_YS = 51             # yield-stress plane of a slot (von Mises hardening changes it per substep)
+_CKPT_PLANES = 34    # state + L + yield stress
+_CK_YS = 33
 FLUID_ID = 10        # the material whose parameters form the `fluid` input
 GRANULAR_ID = 9      # the material whose mu(I) coefficients form the `granular` input
 _FLUID_FIELDS = ("plastic_viscosity", "bulk", "hardening", "softening")
@@ -54,7 +58,7 @@ class Rollout:
 
     SUPPORTED_MATERIALS = frozenset((0, 1, 2, GRANULAR_ID, FLUID_ID))
     SUPPORTED_COLLIDERS = frozenset((native.COLLIDER_PLANE, native.COLLIDER_BOX, native.COLLIDER_WALLS,
-                                     native.COLLIDER_POINTCLOUD))
+                                     native.COLLIDER_POINTCLOUD, native.COLLIDER_CUP, native.COLLIDER_SDF))
 
     def __init__(self, solver, dt: float, substeps: int, checkpoint_every: int = 20):
         sim = solver._sim
@@ -65,12 +69,10 @@ class Rollout:
         if not sim._mat_ids <= self.SUPPORTED_MATERIALS:
             raise NotImplementedError(f"adjoint kernels cover material ids {sorted(self.SUPPORTED_MATERIALS)}; "
                                       f"the solver has {sorted(sim._mat_ids)}")
-        if 1 in sim._mat_ids and float(sim.model[0]["hardening"][1]) > 0.5:
-            # the hardened yield stress is particle state; its adjoint is not carried
-            raise NotImplementedError("von Mises with hardening (hardening_xi > 0) has no adjoint")
         kinds = {c.kind for c in sim.colliders.items}
         if not kinds <= self.SUPPORTED_COLLIDERS:
-            raise NotImplementedError("adjoint kernels cover plane, box, wall and point-cloud colliders")
+            raise NotImplementedError("adjoint kernels cover plane, box, wall, point-cloud, revolved-cup and "
+                                      "mesh-SDF colliders")
         if sim.model[0]["periodic_x"]:
             raise NotImplementedError("periodic domains have no adjoint")
         if sim.cdf is not None and sim.cdf.n > 0:
@@ -149,10 +151,13 @@ class Rollout:
         # graph serves every full segment)
         self.seg_sched = torch.zeros(max(L * self.row_bytes, 1), dtype=torch.uint8, device=dev)
         # ABI structs of every substep slot, built once
-        self._P_fwd = [self._particles(self.slots[k + 1], self.slots[k + 1]) for k in range(L)]
+        # (the recompute of substep k hardens slot k + 1's copy of the yield stress)
+        self._P_fwd = [self._particles(self.slots[k + 1], self.slots[k + 1], ys_slot=self.slots[k + 1])
+                       for k in range(L)]
         self._P_adj = [self._particles(self.slots[k], self.slots[k + 1]) for k in range(L)]
-        # the stress adjoint of substep k reads L_k, which slot k holds
-        self._P_st = [self._particles(self.slots[k], self.slots[k + 1], self.slots[k]) for k in range(L)]
+        # the stress adjoint of substep k reads L_k and the yield stress ys_k, which slot k holds
+        self._P_st = [self._particles(self.slots[k], self.slots[k + 1], self.slots[k], ys_slot=self.slots[k])
+                      for k in range(L)]
         self._G = [self._grid_struct(self.g_in[k], self.g_out[k]) for k in range(L)]
         self.use_graph = True
         self._graph = None
@@ -168,10 +173,10 @@ class Rollout:
         G.block_touch = None  # the simulator's block state belongs to its own grid
         return G
 
-    def _particles(self, state_slot, post_slot, l_slot=None):
+    def _particles(self, state_slot, post_slot, l_slot=None, ys_slot=None):
         """Particle struct: x, v, C, F_trial from state_slot's planes, F and stress from
-        post_slot's, L from l_slot's (default post_slot; shared per-particle arrays from
-        the simulator)."""
+        post_slot's, L from l_slot's (default post_slot), the yield stress from ys_slot's
+        (default the simulator's array); other per-particle arrays from the simulator."""
         P = native.Particles()
         C.memmove(C.byref(P), C.byref(self.sim.P), C.sizeof(P))
         n = self.n
@@ -180,6 +185,8 @@ class Rollout:
         lp = native.ptr(post_slot if l_slot is None else l_slot)
         P.x, P.v, P.C, P.F_trial = base, base + 3 * 4 * n, base + 6 * 4 * n, base + 15 * 4 * n
         P.F, P.stress, P.L = post + 24 * 4 * n, post + 33 * 4 * n, lp + _L0 * 4 * n
+        if ys_slot is not None:
+            P.yield_stress = native.ptr(ys_slot) + _YS * 4 * n
         P.cov = 0
         return P
 
@@ -260,6 +267,7 @@ class Rollout:
             ck[15:24].copy_(sim.F_trial)
             if self.viscous:
                 ck[24:33].copy_(sim.L)
+            ck[_CK_YS].copy_(sim.yield_stress)  # hardened in place by the passes
             native.check(lib.mpm_grid_zero_box(pG, zl, zh, 3, s), "zero_box")
             G = sim.G
             sim._pass(ST | P2G, dt, s)
@@ -282,7 +290,8 @@ class Rollout:
         """Checkpoint of segment `seg` into slot 0 and its collider rows into seg_sched."""
         self.slots[0, :_STATE_PLANES].copy_(self.ckpt[seg, :_STATE_PLANES])
         if self.viscous:
-            self.slots[0, _L0:].copy_(self.ckpt[seg, _STATE_PLANES:])
+            self.slots[0, _L0:_L0 + 9].copy_(self.ckpt[seg, _STATE_PLANES:_STATE_PLANES + 9])
+        self.slots[0, _YS].copy_(self.ckpt[seg, _CK_YS])
         if self.K:
             nb = S * self.row_bytes
             self.seg_sched[:nb].copy_(self.sched[k0 * self.row_bytes:k0 * self.row_bytes + nb])
@@ -301,7 +310,8 @@ class Rollout:
         for k in range(S):
             self.slots[k + 1, :_STATE_PLANES].copy_(self.slots[k, :_STATE_PLANES])
             if self.viscous:  # the recomputed stress of substep k reads L_k
-                self.slots[k + 1, _L0:].copy_(self.slots[k, _L0:])
+                self.slots[k + 1, _L0:_L0 + 9].copy_(self.slots[k, _L0:_L0 + 9])
+            self.slots[k + 1, _YS].copy_(self.slots[k, _YS])
             P, G = C.byref(self._P_fwd[k]), C.byref(self._G[k])
             native.check(lib.mpm_particle_pass(P, G, md, dt, native.PASS_STRESS | native.PASS_P2G, s),
                          "recompute p2g")

@@ -2,7 +2,8 @@
 // tests/test_adjoint_math.py with MPM_DEV = __host__ __device__; it makes no HIP call and
 // runs without a GPU). One case per input line:
 //   mat dt mu lam ys eta bulk K n mu_s dmu I0 d rho_s Ft[9] L[9] Fb[9] tb[9]
-// (for material 2 the ys slot carries the Drucker-Prager alpha)
+// (for material 2 the ys slot carries the Drucker-Prager alpha; for material 1 the eta and
+// bulk slots carry the hardening xi and the adjoint of the yield stress after the map)
 // One output line per case:
 //   Ftb[9] Lb[9] mu_b lam_b ys_b eta_b bulk_b K_b n_b mu_s_b dmu_b I0_b d_b
 #define MPM_DEV __host__ __device__ inline
@@ -42,9 +43,8 @@ int main() {
             Ftb = adj_fcr(Ft, mu, lam, Fb, tb, mub, lamb);
         } else if (mat == 2) {
             Ftb = adj_sand(Ft, mu, lam, ys, Fb, tb, mub, lamb);
-        } else if (mat == 1) {
-            md.hardening[mat] = 0.0f;
-            Ftb = adj_von_mises(Ft, mu, lam, ys, Fb, tb, mub, lamb, yb);
+        } else if (mat == 1) {  // eta / bulk slots: hardening xi, adjoint of the hardened ys
+            Ftb = adj_von_mises(Ft, mu, lam, ys, in[5], in[6], Fb, tb, mub, lamb, yb);
         } else if (mat == 9) {
             Ftb = adj_mu_i(Ft, &md, mat, dt, mu, lam, Fb, tb, mub, lamb, pb);
         } else if (mat == 10) {

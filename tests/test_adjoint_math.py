@@ -68,17 +68,22 @@ def _sand(Ft, mu, lam, alpha):
     return F, _hencky(F, mu, lam)
 
 
-def _von_mises(Ft, mu, lam, ys):
+def _von_mises(Ft, mu, lam, ys, xi=0.0):
+    """(F, tau, yield stress after the map): hardening ys + 2 mu xi dg when yielding
+    (mpm_utils.py:164-167)."""
     U, s, V = _svd(Ft)
     e = np.log(np.maximum(s, 0.01))
     t = 2 * mu * e + lam * e.sum()
+    ys_next = ys
     if np.linalg.norm(t - t.mean()) > ys:
         h = e - e.mean()
         hn = np.linalg.norm(h) + 1e-6
-        F = U @ np.diag(np.exp(e - (hn - ys / (2 * mu)) / hn * h)) @ V.T
+        dg = hn - ys / (2 * mu)
+        F = U @ np.diag(np.exp(e - dg / hn * h)) @ V.T
+        ys_next = ys + 2 * mu * xi * dg
     else:
         F = Ft
-    return F, _hencky(F, mu, lam)
+    return F, _hencky(F, mu, lam), ys_next
 
 
 def _mu_i(Ft, mu, lam, mu_s, dmu, I0, d, rho_s, dt):
@@ -166,9 +171,12 @@ def _cases(rng):
                 break
         out.append(dict(base, mat=2, mu=mu, lam=lam, ys=alpha,
                         Ft=_rot(rng) @ np.diag(np.exp(tr / 3 + dv)) @ _rot(rng).T))
-    # von Mises: E 2e5, nu 0.3, yield 1e3; elastic, plastic, near-repeated singular values
+    # von Mises: E 2e5, nu 0.3, yield 1e3; elastic, plastic, near-repeated singular values,
+    # with and without hardening (the eta and bulk slots carry xi and the adjoint of the
+    # yield stress after the map, which every case passes through)
     mu, lam = 2e5 / 2.6, 2e5 * 0.3 / (1.3 * 0.4)
-    for kind in ("elastic", "plastic", "plastic", "repeated", "repeated_plastic", "rotation"):
+    for kind in ("elastic", "plastic", "plastic", "repeated", "repeated_plastic", "rotation",
+                 "plastic_hard", "plastic_hard", "elastic_hard", "repeated_plastic_hard"):
         while True:
             if kind == "rotation":
                 e = np.zeros(3)
@@ -181,9 +189,11 @@ def _cases(rng):
                 e = rng.uniform(-0.02, 0.02, 3)
             t = 2 * mu * e + lam * e.sum()
             r = np.linalg.norm(t - t.mean()) / 1e3
-            if kind in ("rotation", "repeated") or (abs(r - 1) > 0.02 and (r > 1) == (kind != "elastic")):
+            if kind in ("rotation", "repeated") or (abs(r - 1) > 0.02 and (r > 1) == (not kind.startswith("elastic"))):
                 break
-        out.append(dict(base, mat=1, mu=mu, lam=lam, ys=1e3, Ft=_rot(rng) @ np.diag(np.exp(e)) @ _rot(rng).T))
+        xi = rng.choice([0.5, 1.0]) if kind.endswith("_hard") else 0.0
+        out.append(dict(base, mat=1, mu=mu, lam=lam, ys=1e3, eta=xi, bulk=rng.standard_normal() * 1e-3,
+                        Ft=_rot(rng) @ np.diag(np.exp(e)) @ _rot(rng).T))
     # mu(I): E 1e6, nu 0.3; elastic / yielding compression and tension
     mu, lam = 1e6 / 2.6, 1e6 * 0.3 / (1.3 * 0.4)
     for kind in ("elastic", "yield", "yield", "yield", "tension"):
@@ -205,7 +215,7 @@ def _cases(rng):
     for c in out:
         c.setdefault("L", np.zeros((3, 3)))
         c["Fb"] = rng.standard_normal((3, 3))
-        c["tb"] = rng.standard_normal((3, 3)) / max(c["mu"], c["bulk"], 1.0)
+        c["tb"] = rng.standard_normal((3, 3)) / max(c["mu"], c["bulk"] if c["mat"] != 1 else 0.0, 1.0)
         for k in ("Ft", "L", "Fb", "tb"):  # the kernel sees these in float32
             c[k] = c[k].astype(np.float32).astype(np.float64)
         for k in _NAMES + ("dt", "rho_s"):
@@ -220,7 +230,9 @@ def _objective(c, stress_only=False, **over):
     elif p["mat"] == 2:
         F, t = _sand(p["Ft"], p["mu"], p["lam"], p["ys"])
     elif p["mat"] == 1:
-        F, t = _von_mises(p["Ft"], p["mu"], p["lam"], p["ys"])
+        F, t, ys_next = _von_mises(p["Ft"], p["mu"], p["lam"], p["ys"], p["eta"])
+        if not stress_only:  # + (adjoint of the yield stress after the map) * that stress
+            return (F * p["Fb"]).sum() + (t * p["tb"]).sum() + p["bulk"] * ys_next
     elif p["mat"] == 9:
         F, t = _mu_i(p["Ft"], p["mu"], p["lam"], p["mu_s"], p["dmu"], p["I0"], p["d"], p["rho_s"], p["dt"])
     else:

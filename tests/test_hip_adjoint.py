@@ -240,17 +240,186 @@ def test_one_substep_material_directions():
         assert abs(ad - fd) <= 2e-3 * abs(fd), f"w {name}: adjoint {ad:.6e} vs central difference {fd:.6e}"
 
 
+def test_hardening_rollout_gradients_match_central_differences():
+    """Von Mises with linear hardening (xi = 0.5: each yielding substep raises the yield
+    stress by 2 mu xi dg, mpm_utils.py:164-167) in the squeeze, 60 substeps with
+    checkpoints every 20 at 0.5 m/s: the hardened yield stress is particle state, carried
+    through the checkpoints and recomputed slots, and its adjoint runs substep to substep
+    (d ys'/d ys = 1 - xi while yielding). d loss / d (E, nu, initial yield) and the
+    initial-velocity direction against central differences of the oracle rollout (the
+    loss linearised at the GPU trajectory, steps as in the rollout test above), within
+    1e-3 (E at a 3e-3 step, see below)."""
+    import torch
+
+    from mpm_amd.materials import vonmises
+
+    hard = vonmises(E=2.0e5, nu=0.3, yield_stress=1.0e3, density=1000.0, hardening_xi=0.5)
+    sc = _scene(plate_speed=0.5)
+    s = sc.build(Solver(grid=sc.grid, device="hip"))
+    s.set_material(hard)
+    n = s.n_particles
+    S = 60
+    ro = Rollout(s, DT, S, checkpoint_every=20)
+    dev = s._sim.device
+    E0, nu0, y0 = 2.0e5, 0.3, 1.0e3
+    f64 = dict(dtype=torch.float64, device=dev)
+    with torch.no_grad():
+        xt, wt = ro.run(torch.tensor(E0, **f64), torch.tensor(nu0, **f64), torch.tensor(1.1 * y0, **f64),
+                        torch.zeros(n, 3, device=dev))
+    x_star, w_star = xt.double(), wt[:, :3].double()
+    E = torch.tensor(E0, requires_grad=True, **f64)
+    nu = torch.tensor(nu0, requires_grad=True, **f64)
+    ys = torch.tensor(y0, requires_grad=True, **f64)
+    v0 = torch.zeros(n, 3, device=dev, requires_grad=True)
+    xT, w = ro.run(E, nu, ys, v0)
+    loss = ((xT.double() - x_star) ** 2).sum() + ((w[:, :3].double() - w_star) ** 2).sum()
+    loss.backward()
+    g = {"E": E.grad.item(), "nu": nu.grad.item(), "yield": ys.grad.item()}
+    gx = (2.0 * (xT.double() - x_star)).detach().cpu().numpy()
+    gw = (2.0 * (w[:, :3].double() - w_star)).detach().cpu().numpy()
+    # the hardening raised the yield stress of the yielding particles
+    assert float(s._sim.yield_stress.max().item()) > 1.001 * y0
+
+    def linearised(E_, nu_, y_, v_):
+        mu, lam = _lame(E_, nu_)
+        x, imp = _oracle_rollout(sc, S, np.float32(mu), np.float32(lam), np.float32(y_), v_, material=hard)
+        return (x * gx).sum() + (imp * gw).sum()
+
+    zero_v = np.zeros((n, 3), np.float32)
+    fd = {}
+    # E at 3e-3: this loss is small (~3e-5), and at 3e-4 and 1e-3 the oracle's differences
+    # scatter by 1.7e-3 and 1.6e-3 around values that settle within 2.4e-4 to 4e-4 of the
+    # adjoint from 3e-3 to 3e-2 (measured)
+    eps = {"E": 3e-3, "nu": 1e-3, "yield": 1e-2}
+    for i, name in enumerate(("E", "nu", "yield")):
+        base = [E0, nu0, y0]
+        h = eps[name] * base[i]
+        p, m = list(base), list(base)
+        p[i] += h
+        m[i] -= h
+        fd[name] = (linearised(*p, zero_v) - linearised(*m, zero_v)) / (2 * h)
+    rng = np.random.default_rng(3)
+    d = rng.choice([-1.0, 1.0], size=(n, 3)).astype(np.float32)
+    hv = 1e-3
+    fd_v = (linearised(E0, nu0, y0, hv * d) - linearised(E0, nu0, y0, -hv * d)) / (2 * hv)
+    report = {k: (g[k], fd[k]) for k in g}
+    report["v0.d"] = (float((v0.grad.double().cpu().numpy() * d).sum()), fd_v)
+    print("hardening: loss", loss.item(), report)
+    for k, (a, f) in report.items():
+        assert abs(a - f) <= 1e-3 * abs(f) + 1e-12, f"{k}: adjoint {a:.6e} vs central difference {f:.6e}"
+
+
+def _surface_scene(cls, eta=5.0, bulk=5.0e5, v0=None, **kw):
+    """The collider scene of the parity tests: a viscous fluid blob (eta 5, bulk 5e5) on a
+    slip floor with friction 0.2, a revolved cup moving at 0.02 m/s and turning at
+    (0.3, 0, 1) rad/s around it, a sphere mesh-SDF collider (separable, friction 0.4)
+    moving and spinning into it, a box plate and the domain walls."""
+    from test_hip_parity import PROF, _blob
+
+    from mpm_amd.geometry import sphere_sdf
+    from mpm_amd.materials import newtonian
+
+    grid, pts, vol = _blob(seed=3)
+    s = cls(grid=grid, **kw) if cls is Solver else cls(grid)
+    s.load_particles(pts, vol)
+    s.set_material(newtonian(eta=eta, density=1000.0, bulk_modulus=bulk))
+    if v0 is not None:
+        s._sim.v[:] = v0
+    s.add_plane((0, 0, 3 * grid.dx), (0, 0, 1), "slip", friction=0.2)
+    cup = s.add_cup(PROF, np.array([0.2, 0.2, 0.2]))
+    s.set_cup(cup, center=(0.2, 0.2, 0.2), quat=(1, 0, 0, 0), velocity=(0.02, 0, 0), omega=(0.3, 0.0, 1.0))
+    sdf = s.add_sdf_collider(sphere_sdf(0.03, res=24), center=(0.2, 0.26, 0.2), velocity=(0.0, -0.05, 0.0),
+                             omega=(0.0, 0.0, 2.0))
+    box = s.add_box((0.16, 0.2, 0.2), (0.02, 0.02, 0.06), velocity=(0.0, 0.0, -0.05))
+    s.add_domain_walls()
+    return s, (cup, sdf, box)
+
+
+@pytest.mark.parametrize("substeps,surfaces,log2_step", [(1, "cup+sphere", -13), (3, "cup", -11)])
+def test_cup_and_mesh_sdf_contact_directions(substeps, surfaces, log2_step):
+    """The revolved cup and the mesh-SDF collider in the adjoint: 1 and 3 substeps of 2e-4 s
+    of the parity tests' collider scene from a random velocity field (+-0.3 m/s per
+    component), so the fluid meets the cup's sticky layer and its walls (separating,
+    sliding and stopped by Coulomb friction) and the spinning sphere. Directional
+    derivatives along a random initial-velocity direction of a random linear functional of
+    the cup's and the sphere's accumulated force and torque (the contact adjoint: the
+    transposed contact map composed with the impulse m (u_in - u_new) and the torque
+    r x impulse), and along a position direction of a functional of x_T, against central
+    differences of the oracle, within 2e-3.
+
+    The base velocities are multiples of 2^-16 m/s and the steps powers of two, so the
+    perturbed states are exact in float32 (a rounded perturbation would add its own
+    relative error, 3e-4 at 1e-4 m/s). Over one substep the step is 2^-13 m/s: smaller
+    ones meet the fp32 rounding of the wrench sums, larger ones the contact switches (at
+    1e-3 m/s the differences move by 3 %). Over three substeps the rounding of the
+    rollout grows and the step is 2^-11 m/s, on the cup's wrench, whose differences
+    settle there (measured, 2 and 3 substeps: 6e-2 and 4e-2 off at 2^-17, 8e-4 and 6e-3
+    at 2^-13, 1.4e-4 at 2^-11); the sphere's smaller wrench does not settle beyond one
+    substep at any step (it stays 3e-3 to 1e-2 off). Position directions are checked
+    through x_T only: through the wrench
+    the same two limits leave no step where the oracle's differences settle (a position
+    step of 1e-6 m is 67 float32 ulps, and 1e-5 m already switches contacts); the
+    position path through P2G and G2P is the one the x_T functional pins."""
+    import torch
+
+    from test_hip_parity import _blob
+
+    dt = 2.0e-4
+    s, (cup, sdf, box) = _surface_scene(Solver, device="hip")
+    sim = s._sim
+    n = s.n_particles
+    rng = np.random.default_rng(5)
+    vbase = (np.round(rng.uniform(-0.3, 0.3, (n, 3)) * 2.0 ** 16) / 2.0 ** 16).astype(np.float32)
+    ro = Rollout(s, dt, substeps, checkpoint_every=substeps)
+    dev = sim.device
+    slots = [sim.colliders.items[h].slot for h in (cup, sdf, box)]
+    gw = np.zeros((ro.n_slots, 6))
+    gw[slots[0]] = rng.standard_normal(6)
+    gw[slots[1]] = rng.standard_normal(6) if surfaces == "cup+sphere" else 0.0
+    gx = rng.standard_normal((n, 3))
+    v = torch.from_numpy(vbase).to(dev).requires_grad_(True)
+    _, pts, _ = _blob(seed=3)
+    x0 = torch.from_numpy(pts.copy()).to(dev).requires_grad_(True)
+    xT, w = ro.run_fluid(torch.zeros(n, device=dev), v, x0=x0)
+    for sl in slots[:2]:  # both surface colliders carried load
+        assert float(w[sl, :3].abs().max()) > 0
+    lw = (w.double() * torch.from_numpy(gw).to(dev)).sum()
+    lx = (xT.double() * torch.from_numpy(gx).to(dev)).sum()
+    (gv,) = torch.autograd.grad(lw, [v], retain_graph=True)
+    (gxx,) = torch.autograd.grad(lx, [x0])
+
+    def oracle(vv, xx):
+        o, hs = _surface_scene(OracleSolver, v0=vv)
+        if xx is not None:
+            o._sim.x[:] = xx
+        o.step(dt, substeps)
+        imp = np.zeros((ro.n_slots, 6))
+        for h, sl in zip(hs[:2], slots[:2]):
+            wr = o.cup_wrench(h, 1.0)
+            imp[sl, :3], imp[sl, 3:] = wr["force"], wr["torque"]
+        return (imp * gw).sum(), (o.x().astype(np.float64) * gx).sum()
+
+    dv = rng.choice([-1.0, 1.0], (n, 3)).astype(np.float32)
+    hv = 2.0 ** log2_step
+    fd_w = (oracle(vbase + hv * dv, None)[0] - oracle(vbase - hv * dv, None)[0]) / (2 * hv)
+    ad_w = float((gv.double().cpu().numpy() * dv).sum())
+    dxd = rng.choice([-1.0, 1.0], (n, 3)).astype(np.float32)
+    hx = 1e-5
+    fd_x = (oracle(vbase, (pts + hx * dxd).astype(np.float32))[1]
+            - oracle(vbase, (pts - hx * dxd).astype(np.float32))[1]) / (2 * hx)
+    ad_x = float((gxx.double().cpu().numpy() * dxd).sum())
+    print(f"{substeps} substep(s): wrench . v0 adjoint {ad_w:.6e} oracle {fd_w:.6e}; x_T . x0 adjoint {ad_x:.6e} "
+          f"oracle {fd_x:.6e}")
+    assert abs(ad_w - fd_w) <= 2e-3 * abs(fd_w), f"wrench v0: adjoint {ad_w:.6e} vs central difference {fd_w:.6e}"
+    assert abs(ad_x - fd_x) <= 2e-3 * abs(fd_x), f"x_T x0: adjoint {ad_x:.6e} vs central difference {fd_x:.6e}"
+
+
 def test_unsupported_material_is_refused():
     sc = _scene()
     s = Solver(grid=sc.grid, device="hip")
     s.load_particles(sc.pos, sc.vol)
     s.set_material("plasticine", E=1.0e5, nu=0.3, density=1000.0)
     with pytest.raises(NotImplementedError):
-        Rollout(s, DT, 10)
-    from mpm_amd.materials import vonmises
-
-    s.set_material(vonmises(E=2.0e5, nu=0.3, yield_stress=1.0e3, hardening_xi=0.1))
-    with pytest.raises(NotImplementedError, match="hardening"):
         Rollout(s, DT, 10)
 
 
