@@ -335,8 +335,8 @@ def _surface_scene(cls, eta=5.0, bulk=5.0e5, v0=None, **kw):
     return s, (cup, sdf, box)
 
 
-@pytest.mark.parametrize("substeps,surfaces,log2_step", [(1, "cup+sphere", -13), (3, "cup", -11)])
-def test_cup_and_mesh_sdf_contact_directions(substeps, surfaces, log2_step):
+@pytest.mark.parametrize("substeps,surfaces,log2_step,bar", [(1, "cup+sphere", -13, 2e-3), (3, "cup", -11, 5e-3)])
+def test_cup_and_mesh_sdf_contact_directions(substeps, surfaces, log2_step, bar):
     """The revolved cup and the mesh-SDF collider in the adjoint: 1 and 3 substeps of 2e-4 s
     of the parity tests' collider scene from a random velocity field (+-0.3 m/s per
     component), so the fluid meets the cup's sticky layer and its walls (separating,
@@ -345,7 +345,7 @@ def test_cup_and_mesh_sdf_contact_directions(substeps, surfaces, log2_step):
     the cup's and the sphere's accumulated force and torque (the contact adjoint: the
     transposed contact map composed with the impulse m (u_in - u_new) and the torque
     r x impulse), and along a position direction of a functional of x_T, against central
-    differences of the oracle, within 2e-3.
+    differences of the oracle, within 2e-3 over one substep and 5e-3 over three.
 
     The base velocities are multiples of 2^-16 m/s and the steps powers of two, so the
     perturbed states are exact in float32 (a rounded perturbation would add its own
@@ -354,7 +354,8 @@ def test_cup_and_mesh_sdf_contact_directions(substeps, surfaces, log2_step):
     1e-3 m/s the differences move by 3 %). Over three substeps the rounding of the
     rollout grows and the step is 2^-11 m/s, on the cup's wrench, whose differences
     settle there (measured, 2 and 3 substeps: 6e-2 and 4e-2 off at 2^-17, 8e-4 and 6e-3
-    at 2^-13, 1.4e-4 at 2^-11); the sphere's smaller wrench does not settle beyond one
+    at 2^-13, 1.4e-4 at 2^-11 for one functional and 1.8e-3 for this test's: the 5e-3
+    bar); the sphere's smaller wrench does not settle beyond one
     substep at any step (it stays 3e-3 to 1e-2 off). Position directions are checked
     through x_T only: through the wrench
     the same two limits leave no step where the oracle's differences settle (a position
@@ -410,7 +411,7 @@ def test_cup_and_mesh_sdf_contact_directions(substeps, surfaces, log2_step):
     ad_x = float((gxx.double().cpu().numpy() * dxd).sum())
     print(f"{substeps} substep(s): wrench . v0 adjoint {ad_w:.6e} oracle {fd_w:.6e}; x_T . x0 adjoint {ad_x:.6e} "
           f"oracle {fd_x:.6e}")
-    assert abs(ad_w - fd_w) <= 2e-3 * abs(fd_w), f"wrench v0: adjoint {ad_w:.6e} vs central difference {fd_w:.6e}"
+    assert abs(ad_w - fd_w) <= bar * abs(fd_w), f"wrench v0: adjoint {ad_w:.6e} vs central difference {fd_w:.6e}"
     assert abs(ad_x - fd_x) <= 2e-3 * abs(fd_x), f"x_T x0: adjoint {ad_x:.6e} vs central difference {fd_x:.6e}"
 
 
