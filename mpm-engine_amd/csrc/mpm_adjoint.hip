@@ -146,8 +146,13 @@ __global__ __launch_bounds__(kAT) void k_adj_g2p(mpm_particles_t P, mpm_grid_t G
         const float c4 = 4.0f * inv_dx;
         const int o0 = st.base[0] - t.lo[0], o1 = st.base[1] - t.lo[1], o2 = st.base[2] - t.lo[2];
         float* __restrict__ gub = A.grid_v;
+        // Lane-rotated node order: step s visits node (s + rot) mod 27, so lanes whose
+        // particles share a cell (consecutive after the sort, same 27 targets) add to
+        // different LDS addresses in the same instruction instead of serialising on one.
+        const int rot = (int)(threadIdx.x % 27u);
 #pragma unroll 1
-        for (int n = 0; n < 27; ++n) {
+        for (int s = 0; s < 27; ++s) {
+            const int n = s + rot >= 27 ? s + rot - 27 : s + rot;
             const int i = n / 9, j = (n / 3) % 3, k = n % 3;
             const int gi = st.base[0] + i, gj = st.base[1] + j, gk = st.base[2] + k;
             const bool ing = in_grid(G, gi, gj, gk);
@@ -359,8 +364,11 @@ __global__ __launch_bounds__(kThreads) void k_adj_grid(mpm_grid_t G, const mpm_m
         if (gi >= G.n[0] || gj >= G.n[1] || gk >= G.n[2]) continue;
         const float4 in4 = *reinterpret_cast<const float4*>(G.node_in + o);
         float4 ub4 = *reinterpret_cast<const float4*>(A.grid_v + o);
-        float4 out;
-        adj_node(G, md, col, ncoll, impb, nslots, gi, gj, gk, in4, mk3(ub4.x, ub4.y, ub4.z), dt, out, err);
+        // An empty node (m = 0) has no collider acting and a zero adjoint: skip the
+        // chain (and its per-collider spill) there; whole empty wavefronts skip it.
+        float4 out = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (in4.w > 0.0f)
+            adj_node(G, md, col, ncoll, impb, nslots, gi, gj, gk, in4, mk3(ub4.x, ub4.y, ub4.z), dt, out, err);
         *reinterpret_cast<float4*>(A.grid_mv + o) = out;
         *reinterpret_cast<float4*>(A.grid_v + o) = make_float4(0.f, 0.f, 0.f, 0.f);  // consumed
     }

@@ -36,12 +36,14 @@ import numpy as np
 
 from .core import native
 
-_STATE_PLANES = 24   # x 3, v 3, C 9, F_trial 9
-_SLOT_PLANES = 52    # + F 9, stress 9, L 9, yield stress 1
-_L0 = 42             # first L plane of a slot
-_YS = 51             # yield-stress plane of a slot (von Mises hardening changes it per substep)
-_CKPT_PLANES = 34    # state + L + yield stress
-_CK_YS = 33
+# Planes of a substep slot: x 3, the yield stress 1 (von Mises hardening changes it per
+# substep), v 3, C 9, F_trial 9, F 9, stress 9, L 9. x and the yield stress lead, so the
+# recompute carries them to the next slot with one copy (the passes write the rest).
+_X, _YS, _V, _C, _FT, _F, _ST, _L0 = 0, 3, 4, 7, 16, 25, 34, 43
+_SLOT_PLANES = 52
+_CARRY = 4           # planes [0, 4): x and the yield stress
+_CKPT_PLANES = 34    # a checkpoint holds a slot's planes [0, 25) (x, yield, v, C, F_trial) and L
+_CK_L0 = 25
 FLUID_ID = 10        # the material whose parameters form the `fluid` input
 GRANULAR_ID = 9      # the material whose mu(I) coefficients form the `granular` input
 _FLUID_FIELDS = ("plastic_viscosity", "bulk", "hardening", "softening")
@@ -150,9 +152,13 @@ class Rollout:
         # collider rows of the segment being reversed (a fixed buffer, so one captured
         # graph serves every full segment)
         self.seg_sched = torch.zeros(max(L * self.row_bytes, 1), dtype=torch.uint8, device=dev)
-        # ABI structs of every substep slot, built once
-        # (the recompute of substep k hardens slot k + 1's copy of the yield stress)
-        self._P_fwd = [self._particles(self.slots[k + 1], self.slots[k + 1], ys_slot=self.slots[k + 1])
+        # ABI structs of every substep slot, built once. The recompute's stress + P2G pass
+        # of substep k reads slot k's state and L and writes F and stress into slot k + 1,
+        # hardening slot k + 1's copy of the yield stress; its G2P pass moves slot k + 1's
+        # copy of x and writes v, C, L and F_trial there.
+        self._P_p2g = [self._particles(self.slots[k], self.slots[k + 1], self.slots[k], ys_slot=self.slots[k + 1])
+                       for k in range(L)]
+        self._P_g2p = [self._particles(self.slots[k + 1], self.slots[k + 1], ys_slot=self.slots[k + 1])
                        for k in range(L)]
         self._P_adj = [self._particles(self.slots[k], self.slots[k + 1]) for k in range(L)]
         # the stress adjoint of substep k reads L_k and the yield stress ys_k, which slot k holds
@@ -183,8 +189,8 @@ class Rollout:
         base = native.ptr(state_slot)
         post = native.ptr(post_slot)
         lp = native.ptr(post_slot if l_slot is None else l_slot)
-        P.x, P.v, P.C, P.F_trial = base, base + 3 * 4 * n, base + 6 * 4 * n, base + 15 * 4 * n
-        P.F, P.stress, P.L = post + 24 * 4 * n, post + 33 * 4 * n, lp + _L0 * 4 * n
+        P.x, P.v, P.C, P.F_trial = base + _X * 4 * n, base + _V * 4 * n, base + _C * 4 * n, base + _FT * 4 * n
+        P.F, P.stress, P.L = post + _F * 4 * n, post + _ST * 4 * n, lp + _L0 * 4 * n
         if ys_slot is not None:
             P.yield_stress = native.ptr(ys_slot) + _YS * 4 * n
         P.cov = 0
@@ -261,13 +267,13 @@ class Rollout:
         for seg, k0 in enumerate(range(0, self.substeps, self.every)):
             S = min(self.every, self.substeps - k0)
             ck = self.ckpt[seg]
-            ck[0:3].copy_(sim.x)
-            ck[3:6].copy_(sim.v)
-            ck[6:15].copy_(sim.C)
-            ck[15:24].copy_(sim.F_trial)
+            ck[_X:_X + 3].copy_(sim.x)
+            ck[_YS].copy_(sim.yield_stress)  # hardened in place by the passes
+            ck[_V:_V + 3].copy_(sim.v)
+            ck[_C:_C + 9].copy_(sim.C)
+            ck[_FT:_FT + 9].copy_(sim.F_trial)
             if self.viscous:
-                ck[24:33].copy_(sim.L)
-            ck[_CK_YS].copy_(sim.yield_stress)  # hardened in place by the passes
+                ck[_CK_L0:_CK_L0 + 9].copy_(sim.L)
             native.check(lib.mpm_grid_zero_box(pG, zl, zh, 3, s), "zero_box")
             G = sim.G
             sim._pass(ST | P2G, dt, s)
@@ -288,10 +294,9 @@ class Rollout:
     # ------------------------------------------------------------------ backward
     def _load_segment(self, seg, k0, S):
         """Checkpoint of segment `seg` into slot 0 and its collider rows into seg_sched."""
-        self.slots[0, :_STATE_PLANES].copy_(self.ckpt[seg, :_STATE_PLANES])
+        self.slots[0, :_CK_L0].copy_(self.ckpt[seg, :_CK_L0])
         if self.viscous:
-            self.slots[0, _L0:_L0 + 9].copy_(self.ckpt[seg, _STATE_PLANES:_STATE_PLANES + 9])
-        self.slots[0, _YS].copy_(self.ckpt[seg, _CK_YS])
+            self.slots[0, _L0:_L0 + 9].copy_(self.ckpt[seg, _CK_L0:_CK_L0 + 9])
         if self.K:
             nb = S * self.row_bytes
             self.seg_sched[:nb].copy_(self.sched[k0 * self.row_bytes:k0 * self.row_bytes + nb])
@@ -304,22 +309,21 @@ class Rollout:
         s = sim.stream()
         dt = self.dt
         md = native.ptr(sim.model_dev)
+        # P2G accumulates into g_in[k] (zero first); the grid update writes every node of
+        # g_out[k] (the launch box is the whole grid), so g_out needs no clearing
         self.g_in[:S].zero_()
-        self.g_out[:S].zero_()
         sched0 = native.ptr(self.seg_sched)
         for k in range(S):
-            self.slots[k + 1, :_STATE_PLANES].copy_(self.slots[k, :_STATE_PLANES])
-            if self.viscous:  # the recomputed stress of substep k reads L_k
-                self.slots[k + 1, _L0:_L0 + 9].copy_(self.slots[k, _L0:_L0 + 9])
-            self.slots[k + 1, _YS].copy_(self.slots[k, _YS])
-            P, G = C.byref(self._P_fwd[k]), C.byref(self._G[k])
-            native.check(lib.mpm_particle_pass(P, G, md, dt, native.PASS_STRESS | native.PASS_P2G, s),
-                         "recompute p2g")
+            self.slots[k + 1, :_CARRY].copy_(self.slots[k, :_CARRY])
+            G = C.byref(self._G[k])
+            native.check(lib.mpm_particle_pass(C.byref(self._P_p2g[k]), G, md, dt,
+                                               native.PASS_STRESS | native.PASS_P2G, s), "recompute p2g")
             native.check(lib.mpm_grid_update(
                 G, md, native.ptr(self.ctl), native.ptr(self.blocks), native.ptr(self.nblocks),
                 sim.n_blocks, sched0 + k * self.row_bytes if self.K else None, self.K, None, 0, None,
                 dt, 0, s), "recompute grid")
-            native.check(lib.mpm_particle_pass(P, G, md, dt, native.PASS_G2P, s), "recompute g2p")
+            native.check(lib.mpm_particle_pass(C.byref(self._P_g2p[k]), G, md, dt, native.PASS_G2P, s),
+                         "recompute g2p")
         pA = C.byref(self.A)
         impb = native.ptr(self.impb)
         for k in reversed(range(S)):
