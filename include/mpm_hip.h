@@ -36,7 +36,7 @@ extern "C" {
 #endif
 
 #define MPM_MAX_MATERIALS 16
-#define MPM_ABI_VERSION 8
+#define MPM_ABI_VERSION 9
 
 typedef enum mpm_status {
     MPM_OK = 0,
@@ -62,6 +62,12 @@ typedef enum mpm_status {
    on the order atomics land in: runs are bit-reproducible, and identical across launch
    shapes and particle orders. Needs G->node_fix; no CDF colliders. */
 #define MPM_PASS_DETERMINISTIC 512
+/* paired interior pass: with MPM_PASS_DETERMINISTIC and the lean fused mode (G2P | STRESS |
+   P2G | LEAN, optionally KEEP_STRESS) only, P->chunk_start holding chunks of at most 512
+   particles (mpm_sort_chunks with chunk = 512) and 256-thread workgroups (no WG128 / PPT
+   bits): each lane takes two particles of its chunk, sharing one stencil tile, one scatter
+   arena and one flush. Bit-identical to the single-particle pass. */
+#define MPM_PASS_PAIR 1024
 
 /* grid-update flag bits (mpm_grid_update) */
 #define MPM_GRID_ZERO_IN 1      /* clear node_in after it is consumed */

@@ -22,7 +22,7 @@ SOURCES = ["mpm_kernels.hip", "mpm_particles.hip", "mpm_sort.hip", "mpm_adjoint.
            "mpm_coupling.hip", "mpm_geometry.hip", "mpm_rigid.hip",
            "mpm_modifiers.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
-FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-munsafe-fp-atomics",
+FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-munsafe-fp-atomics", "-ffp-contract=on",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
          "-I" + os.path.join(REPO, "include")]
 

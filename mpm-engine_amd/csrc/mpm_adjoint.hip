@@ -146,14 +146,19 @@ __global__ __launch_bounds__(kAT) void k_adj_g2p(mpm_particles_t P, mpm_grid_t G
         const float c4 = 4.0f * inv_dx;
         const int o0 = st.base[0] - t.lo[0], o1 = st.base[1] - t.lo[1], o2 = st.base[2] - t.lo[2];
         float* __restrict__ gub = A.grid_v;
-        // Lane-rotated node order: step s visits node (s + rot) mod 27, so lanes whose
-        // particles share a cell (consecutive after the sort, same 27 targets) add to
-        // different LDS addresses in the same instruction instead of serialising on one.
-        const int rot = (int)(threadIdx.x % 27u);
+        // Conflict-free node order: every stencil holds one node of each residue class
+        // (x, y, z mod 3); at step s the lane of rank rho = lane mod 27 visits class
+        // (s + rho) mod 27, so lanes of different rank add to different LDS addresses in
+        // the same instruction, whichever cells their particles sit in.
+        const int rho = (int)(threadIdx.x % 27u);
+        const int i0 = ((rho / 9 - st.base[0]) % 3 + 3) % 3, j0 = (((rho / 3) % 3 - st.base[1]) % 3 + 3) % 3,
+                  k0 = ((rho % 3 - st.base[2]) % 3 + 3) % 3;
 #pragma unroll 1
         for (int s = 0; s < 27; ++s) {
-            const int n = s + rot >= 27 ? s + rot - 27 : s + rot;
-            const int i = n / 9, j = (n / 3) % 3, k = n % 3;
+            int i = s / 9 + i0, j = (s / 3) % 3 + j0, k = s % 3 + k0;
+            i = i >= 3 ? i - 3 : i;
+            j = j >= 3 ? j - 3 : j;
+            k = k >= 3 ? k - 3 : k;
             const int gi = st.base[0] + i, gj = st.base[1] + j, gk = st.base[2] + k;
             const bool ing = in_grid(G, gi, gj, gk);
             int a = 0;
@@ -201,18 +206,25 @@ __global__ __launch_bounds__(kAT) void k_adj_g2p(mpm_particles_t P, mpm_grid_t G
 // --------------------------------------------------------------- grid adjoint
 // One node: recompute the forward chain (normalise + gravity, damping, colliders in
 // order), then transpose it. Plate / box impulses add their adjoint through `impb`.
+// kMaxC bounds ncoll: at kMaxC = 4 both collider loops unroll and the per-collider
+// velocities stay in registers; the 16-collider variant keeps them in scratch.
+template <int kMaxC>
 MPM_DEV void adj_node(const mpm_grid_t& G, const mpm_model_t* md, const mpm_collider_t* col, int ncoll,
                       const float* impb, int nslots, int gi, int gj, int gk, float4 in4, v3 ub, float dt, float4& out,
                       int& err) {
+    constexpr int kUnroll = kMaxC <= 4 ? kMaxC : 1;
+    const v3 xn = mk3((float)gi * G.dx, (float)gj * G.dx, (float)gk * G.dx);  // rounded, as the forward
     const float m = in4.w;
-    v3 us[kMaxAdjColliders + 1];
+    v3 us[kMaxC];
     v3 uo = mk3(0.f, 0.f, 0.f);
     const v3 mv = mk3(in4.x, in4.y, in4.z);
     if (m > 1e-15f) uo = mk3(mv.x / m + dt * md->gravity[0], mv.y / m + dt * md->gravity[1], mv.z / m + dt * md->gravity[2]);
     const float damp = md->grid_v_damping_scale;
     if (damp < 1.0f) uo = uo * damp;
-    bool act[kMaxAdjColliders];
-    for (int q = 0; q < ncoll; ++q) {
+    bool act[kMaxC];
+#pragma unroll kUnroll
+    for (int q = 0; q < kMaxC; ++q) {
+        if (q >= ncoll) break;
         const mpm_collider_t& c = col[q];
         us[q] = uo;
         act[q] = c.active && m > 0.0f && gi >= c.lo[0] && gi < c.hi[0] && gj >= c.lo[1] && gj < c.hi[1] &&
@@ -224,8 +236,7 @@ MPM_DEV void adj_node(const mpm_grid_t& G, const mpm_model_t* md, const mpm_coll
             if (c.kind == MPM_COLLIDER_BOX) {
                 const v3 vb = mk3(c.velocity[0], c.velocity[1], c.velocity[2]);
                 if (live) {
-                    const float ox = (float)gi * G.dx - c.point[0], oy = (float)gj * G.dx - c.point[1],
-                                oz = (float)gk * G.dx - c.point[2];
+                    const float ox = xn.x - c.point[0], oy = xn.y - c.point[1], oz = xn.z - c.point[2];
                     if (fabsf(ox) < c.size[0] && fabsf(oy) < c.size[1] && fabsf(oz) < c.size[2]) uo = vb;
                 } else if (c.reset == 1 && time < c.end_time + 15.0f * c.dt) {
                     uo = mk3(0.f, 0.f, 0.f);
@@ -233,7 +244,7 @@ MPM_DEV void adj_node(const mpm_grid_t& G, const mpm_model_t* md, const mpm_coll
             } else if (c.kind == MPM_COLLIDER_PLANE) {
                 if (live) {
                     const v3 n = mk3(c.normal[0], c.normal[1], c.normal[2]);
-                    const v3 off = mk3((float)gi * G.dx - c.point[0], (float)gj * G.dx - c.point[1], (float)gk * G.dx - c.point[2]);
+                    const v3 off = xn - mk3(c.point[0], c.point[1], c.point[2]);
                     if (dot(off, n) < 0.0f) {
                         if (c.surface == 0) uo = mk3(0.f, 0.f, 0.f);
                         else if (c.surface == 11) err |= kErrAdjCollider;
@@ -267,8 +278,9 @@ MPM_DEV void adj_node(const mpm_grid_t& G, const mpm_model_t* md, const mpm_coll
     }
     // reverse sweep
     float mb = 0.0f;
-    for (int q = ncoll - 1; q >= 0; --q) {
-        if (!act[q]) continue;
+#pragma unroll kUnroll
+    for (int q = kMaxC - 1; q >= 0; --q) {
+        if (q >= ncoll || !act[q]) continue;
         const mpm_collider_t& c = col[q];
         const v3 ui = us[q];
         const float time = c.time;
@@ -276,8 +288,7 @@ MPM_DEV void adj_node(const mpm_grid_t& G, const mpm_model_t* md, const mpm_coll
         if (c.kind == MPM_COLLIDER_BOX) {
             const v3 vb = mk3(c.velocity[0], c.velocity[1], c.velocity[2]);
             if (live) {
-                const float ox = (float)gi * G.dx - c.point[0], oy = (float)gj * G.dx - c.point[1],
-                            oz = (float)gk * G.dx - c.point[2];
+                const float ox = xn.x - c.point[0], oy = xn.y - c.point[1], oz = xn.z - c.point[2];
                 if (fabsf(ox) < c.size[0] && fabsf(oy) < c.size[1] && fabsf(oz) < c.size[2]) {
                     v3 ib = mk3(0.f, 0.f, 0.f);
                     if (c.slot >= 0 && c.slot < nslots)
@@ -291,7 +302,7 @@ MPM_DEV void adj_node(const mpm_grid_t& G, const mpm_model_t* md, const mpm_coll
         } else if (c.kind == MPM_COLLIDER_PLANE) {
             if (!live) continue;
             const v3 n = mk3(c.normal[0], c.normal[1], c.normal[2]);
-            const v3 off = mk3((float)gi * G.dx - c.point[0], (float)gj * G.dx - c.point[1], (float)gk * G.dx - c.point[2]);
+            const v3 off = xn - mk3(c.point[0], c.point[1], c.point[2]);
             if (!(dot(off, n) < 0.0f)) continue;
             if (c.surface == 0) {
                 ub = mk3(0.f, 0.f, 0.f);
@@ -349,6 +360,7 @@ MPM_DEV void adj_node(const mpm_grid_t& G, const mpm_model_t* md, const mpm_coll
     }
 }
 
+template <int kMaxC>
 __global__ __launch_bounds__(kThreads) void k_adj_grid(mpm_grid_t G, const mpm_model_t* __restrict__ md,
                                                        const mpm_collider_t* __restrict__ col, int ncoll,
                                                        const float* __restrict__ impb, int nslots, mpm_adjoint_t A,
@@ -368,7 +380,8 @@ __global__ __launch_bounds__(kThreads) void k_adj_grid(mpm_grid_t G, const mpm_m
         // chain (and its per-collider spill) there; whole empty wavefronts skip it.
         float4 out = make_float4(0.f, 0.f, 0.f, 0.f);
         if (in4.w > 0.0f)
-            adj_node(G, md, col, ncoll, impb, nslots, gi, gj, gk, in4, mk3(ub4.x, ub4.y, ub4.z), dt, out, err);
+            adj_node<kMaxC>(G, md, col, ncoll, impb, nslots, gi, gj, gk, in4, mk3(ub4.x, ub4.y, ub4.z), dt, out,
+                            err);
         *reinterpret_cast<float4*>(A.grid_mv + o) = out;
         *reinterpret_cast<float4*>(A.grid_v + o) = make_float4(0.f, 0.f, 0.f, 0.f);  // consumed
     }
@@ -510,8 +523,12 @@ int32_t mpm_adjoint_grid(const mpm_grid_t* G, const mpm_model_t* model, const mp
     if (ncoll > kMaxAdjColliders) return MPM_ERR_SIZE;
     const long nblk = (long)G->nb[0] * G->nb[1] * G->nb[2];
     const unsigned nwg = (unsigned)min((nblk + 3) / 4, 2048L);
-    hipLaunchKernelGGL(k_adj_grid, dim3(nwg), dim3(kThreads), 0, (hipStream_t)stream, *G, model, colliders, ncoll,
-                       impulse_adj, wrench_slots, *A, dt);
+    if (ncoll <= 4)
+        hipLaunchKernelGGL(k_adj_grid<4>, dim3(nwg), dim3(kThreads), 0, (hipStream_t)stream, *G, model, colliders,
+                           ncoll, impulse_adj, wrench_slots, *A, dt);
+    else
+        hipLaunchKernelGGL(k_adj_grid<kMaxAdjColliders>, dim3(nwg), dim3(kThreads), 0, (hipStream_t)stream, *G,
+                           model, colliders, ncoll, impulse_adj, wrench_slots, *A, dt);
     return launch_status();
 }
 

@@ -21,12 +21,14 @@ MPM_DEV void apply_collider(const mpm_collider_t& c, const mpm_grid_t& G, int gi
     torque = mk3(0.f, 0.f, 0.f);
     const float time = c.time;
     const bool live = time >= c.start_time && time < c.end_time;
+    // node position rounded first: offsets are xn - point (no multiply-add fused into the
+    // subtraction), the rounding the oracle's node-on-the-boundary tests see
     const v3 xn = mk3((float)gi * G.dx, (float)gj * G.dx, (float)gk * G.dx);
     switch (c.kind) {
         case MPM_COLLIDER_PLANE: {  // mpm_solver_warp.py:1929-1992
             if (!live) return;
             v3 n = mk3(c.normal[0], c.normal[1], c.normal[2]);
-            v3 off = mk3((float)gi * G.dx - c.point[0], (float)gj * G.dx - c.point[1], (float)gk * G.dx - c.point[2]);
+            v3 off = xn - mk3(c.point[0], c.point[1], c.point[2]);
             if (!(dot(off, n) < 0.0f)) return;
             if (c.surface == 0) {
                 vo = mk3(0.f, 0.f, 0.f);
@@ -44,7 +46,7 @@ MPM_DEV void apply_collider(const mpm_collider_t& c, const mpm_grid_t& G, int gi
         case MPM_COLLIDER_BOX: {  // mpm_solver_warp.py:2053-2088
             v3 vb = mk3(c.velocity[0], c.velocity[1], c.velocity[2]);
             if (live) {
-                v3 off = mk3((float)gi * G.dx - c.point[0], (float)gj * G.dx - c.point[1], (float)gk * G.dx - c.point[2]);
+                v3 off = xn - mk3(c.point[0], c.point[1], c.point[2]);
                 if (fabsf(off.x) < c.size[0] && fabsf(off.y) < c.size[1] && fabsf(off.z) < c.size[2]) {
                     imp = (vo - vb) * m;
                     vo = vb;

@@ -109,6 +109,8 @@ MPM_DEV void gather_tp(const Stencil& st, float inv_dx, Fetch fetch, v3& nv, m33
 // integer and the difference of the two bit patterns is that integer (for |y| < 2^51;
 // 2^12 such contributions to one node still fit an int64). Three instructions instead of
 // the generic float -> int64 conversion, 108 times per particle.
+MPM_DEV int mod3(int v) { return ((v % 3) + 3) % 3; }
+
 MPM_DEV unsigned long long fix_round(float y) {
     const double magic = 6755399441055744.0;  // 1.5 * 2^52
     return (unsigned long long)(__double_as_longlong((double)y + magic) - __double_as_longlong(magic));
@@ -146,12 +148,15 @@ MPM_DEV bool scatter_tp_fix(const Stencil& st, float dx, float inv_dx, float mas
     const float ms = mass * Sm;
     const float lim = 2251799813685248.0f;  // 2^51
     if (!(fmaxf(bnd.x, fmaxf(bnd.y, bnd.z)) < lim) || !(fabsf(ms) < lim)) return false;
-    // Lanes whose particles share a stencil cell (about 8 consecutive lanes after a cell
-    // sort) target the same 27 nodes; each lane starts its j and k loops at its own
-    // offset (rot % 3, rot / 3 % 3), so the same instruction of such lanes hits up to 9
-    // different nodes instead of one LDS address. The rotated weights are picked once per
-    // particle.
-    const int j0 = rot % 3, k0 = (rot / 3) % 3;
+    // Conflict-free node order. The 27 stencil nodes of any particle hold exactly one node
+    // of each residue class (x mod 3, y mod 3, z mod 3). At step (si, sj, sk) the lane of
+    // rank rho = rot mod 27 targets class ((si, sj, sk) + decode(rho)) mod 3, i.e. stencil
+    // offset ((s + decode(rho)) - base) mod 3 per axis: lanes of different rank hit
+    // different nodes in the same instruction whichever cells their particles sit in (only
+    // lanes 27 apart, particles about 3 cells apart after a cell sort, can meet). Integer
+    // sums, so the order changes no bit of the result.
+    const int rho = rot % 27;
+    const int i0 = mod3(rho / 9 - st.base[0]), j0 = mod3((rho / 3) % 3 - st.base[1]), k0 = mod3(rho % 3 - st.base[2]);
     int jr[3], kr[3];
     float wjr[3], djr[3], wkr[3], dkr[3];
 #pragma unroll
@@ -165,9 +170,10 @@ MPM_DEV bool scatter_tp_fix(const Stencil& st, float dx, float inv_dx, float mas
         dkr[t] = sel3(k, st.dw[2]);
     }
 #pragma unroll
-    for (int i = 0; i < 3; ++i) {
-        const float wi = st.w[0][i];
-        const v3 Pi = (a + b[0] * (float)i) * wi + g[0] * st.dw[0][i];
+    for (int ti = 0; ti < 3; ++ti) {
+        const int i = i0 + ti >= 3 ? i0 + ti - 3 : i0 + ti;
+        const float wi = sel3(i, st.w[0]);
+        const v3 Pi = (a + b[0] * (float)i) * wi + g[0] * sel3(i, st.dw[0]);
         const v3 Qi = b[1] * wi, Ri = g[1] * wi, Ti = b[2] * wi, Ui = g[2] * wi;
         const float mi = ms * wi;
 #pragma unroll
@@ -214,7 +220,8 @@ MPM_DEV void scatter_global(const mpm_grid_t& G, size_t soff, const Stencil& st,
     };
     if constexpr (kDet) {  // the same contributions as the LDS path, bit for bit
         unsigned long long* __restrict__ gfix = reinterpret_cast<unsigned long long*>(G.node_fix);
-        const bool ok = scatter_tp_fix(st, G.dx, G.inv_dx, mass, v, Cm, fs, G.fix_scale[0], G.fix_scale[1], 0,
+        const bool ok = scatter_tp_fix(st, G.dx, G.inv_dx, mass, v, Cm, fs, G.fix_scale[0], G.fix_scale[1],
+                                       (int)(threadIdx.x & 63u),
                                        [&](int i, int j, int k, unsigned long long q0, unsigned long long q1,
                                            unsigned long long q2, unsigned long long q3) {
             int ix = st.base[0] + i;
@@ -256,18 +263,38 @@ MPM_DEV void scatter_global(const mpm_grid_t& G, size_t soff, const Stencil& st,
 }
 
 // One particle's 27 scatter contributions (v + C dpos) w m + fs grad w (fs = -vol dt tau)
-// handed to add(i, j, k, value, w m). Lane-rotated node order: step s visits node
-// (s + rot) mod 27, so lanes whose particles share a cell (and therefore the same 27
-// target nodes) touch different LDS addresses in the same instruction instead of
-// serialising on one address.
-template <class Add>
+// handed to add(i, j, k, value, w m). kResidue: the conflict-free order of scatter_tp_fix
+// (the lane of rank rot mod 27 visits residue class (s + rot) mod 27 at step s, so lanes
+// of different rank never add to one LDS address in the same instruction). Otherwise step
+// s visits stencil node (s + rot) mod 27 (rot = 0: every lane in the same order, as the
+// segmented DPP sums need).
+template <bool kResidue = false, class Add>
 MPM_DEV void scatter_rotated(const Stencil& st, float dx, float inv_dx, float mass, v3 v, const m33& Cm,
                              const m33& fs, int rot, Add add) {
+    int i0 = 0, j0 = 0, k0 = 0;
+    if constexpr (kResidue) {
+        const int rho = rot % 27;
+        i0 = mod3(rho / 9 - st.base[0]);
+        j0 = mod3((rho / 3) % 3 - st.base[1]);
+        k0 = mod3(rho % 3 - st.base[2]);
+    }
 #pragma unroll 3
     for (int s = 0; s < 27; ++s) {
-        int n = s + rot;
-        n = n >= 27 ? n - 27 : n;
-        const int i = n / 9, j = (n / 3) % 3, k = n % 3;
+        int i, j, k;
+        if constexpr (kResidue) {
+            i = s / 9 + i0;
+            j = (s / 3) % 3 + j0;
+            k = s % 3 + k0;
+            i = i >= 3 ? i - 3 : i;
+            j = j >= 3 ? j - 3 : j;
+            k = k >= 3 ? k - 3 : k;
+        } else {
+            int n = s + rot;
+            n = n >= 27 ? n - 27 : n;
+            i = n / 9;
+            j = (n / 3) % 3;
+            k = n % 3;
+        }
         const float w0 = sel3(i, st.w[0]), w1 = sel3(j, st.w[1]), w2 = sel3(k, st.w[2]);
         const float e0 = sel3(i, st.dw[0]), e1 = sel3(j, st.dw[1]), e2 = sel3(k, st.dw[2]);
         const float wm = w0 * w1 * w2 * mass;
@@ -348,6 +375,17 @@ __global__ __launch_bounds__(kT) __attribute__((amdgpu_waves_per_eu(MPM_PASS_WAV
         const int p0 = chunked ? P.chunk_start[chunk] : chunk * kT;
         const int p1 = chunked ? P.chunk_start[chunk + 1] : min(p0 + kT, P.n);
         const int p = p0 + threadIdx.x;
+        if constexpr (kP2G && !kCdf) {
+            // The whole scatter arena is cleared up front (six 16-byte stores per lane), so
+            // the scatter box reduction's barrier orders the clearing before any scatter:
+            // no clearing phase and barrier of its own between the two.
+            for (int a = threadIdx.x; a < kArena; a += kT) {
+                s_scatter[0][a] = Acc(0);
+                s_scatter[1][a] = Acc(0);
+                s_scatter[2][a] = Acc(0);
+                s_scatter[3][a] = Acc(0);
+            }
+        }
         const bool sel = p < p1 && P.selection[p] == 0;
         const int scene = (sel && P.scene) ? P.scene[p] : 0;
         const bool alive = sel && (unsigned)scene < (unsigned)G.n_scenes;  // bad scene ids: error bit 8
@@ -516,21 +554,21 @@ __global__ __launch_bounds__(kT) __attribute__((amdgpu_waves_per_eu(MPM_PASS_WAV
                 hi[a] = alive ? st.base[a] + 3 : INT_MIN;
             }
             const Tile ts = make_tile<kT>(G, lo, hi, alive ? scene : INT_MAX, alive ? scene : INT_MIN, s_red1, periodic);
-            tile_for<kT>(ts, [&](int a, int, int, int) {
-                s_scatter[0][a] = Acc(0);
-                s_scatter[1][a] = Acc(0);
-                s_scatter[2][a] = Acc(0);
-                s_scatter[3][a] = Acc(0);
-            });
             int pc_s = 0;  // CPIC tag of the scatter half (mpm_utils.py:851-860)
             if constexpr (kCdf) {
+                tile_for<kT>(ts, [&](int a, int, int, int) {
+                    s_scatter[0][a] = Acc(0);
+                    s_scatter[1][a] = Acc(0);
+                    s_scatter[2][a] = Acc(0);
+                    s_scatter[3][a] = Acc(0);
+                });
                 if (threadIdx.x < 6 * MPM_MAX_CDF) s_react[threadIdx.x] = 0.f;
                 if (alive) {
                     const int old = (kG2P && mat != 7 && mat != 8) ? pc_g : P.cdf_tag[p];
                     pc_s = cdf_particle_tag(G, st, old, false);
                 }
+                __syncthreads();
             }
-            __syncthreads();
             // blocked deposits (mpm_utils.py:894-912): a contribution to an incompatible
             // node goes to its owner lane's reaction wrench instead of the grid
             auto mask = [&](int i, int j, int k, v3& add, float& wm) {
@@ -629,7 +667,7 @@ __global__ __launch_bounds__(kT) __attribute__((amdgpu_waves_per_eu(MPM_PASS_WAV
             } else if (alive) {
                 if (ts.lds) {
                     const int d1 = ts.d1, d2 = ts.d2, b0 = ts.base;
-                    scatter_rotated(st, G.dx, G.inv_dx, mass, v, Cm, fs, (int)(threadIdx.x % 27u),
+                    scatter_rotated<true>(st, G.dx, G.inv_dx, mass, v, Cm, fs, (int)(threadIdx.x % 27u),
                                     [&](int i, int j, int k, v3 add, float wm) {
                                         if (!mask(i, j, k, add, wm)) return;
                                         const int a = b0 + ((o0 + i) * d1 + (o1 + j)) * d2 + (o2 + k);
@@ -680,6 +718,297 @@ __global__ __launch_bounds__(kT) __attribute__((amdgpu_waves_per_eu(MPM_PASS_WAV
     }  // chunk loop
 }
 
+// Paired lean pass (MPM_PASS_PAIR, the deterministic interior substep): a workgroup takes
+// a group-aligned chunk of up to 512 sorted particles, two per lane (p0 + t and p0 + 256 +
+// t). The chunk's latency-bound phases (particle loads, the two box reductions with their
+// barriers, the tile load and the flush) are paid once for both particles, and the ~700
+// chunks of a 340k-particle pour fit one round of resident workgroups (3 per CU) instead
+// of two. The first particle's scatter inputs wait in an LDS stash while the second one
+// gathers; the gather tile and the scatter arena are never live together and share one
+// buffer, so LDS stays at ~50 KB per workgroup. The per-particle arithmetic is that of
+// k_particle_pass (the same helpers), and the fixed-point contributions are integers, so
+// the result is bit-identical to the single-particle pass.
+constexpr int kPairStash = 25;  // x 3, v 3, C 9, fs 9, mass 1
+template <bool kSolid, bool kKeepStress>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MPM_PASS_WAVES_PER_EU))) void k_particle_pass_pair(
+    mpm_particles_t P, mpm_grid_t G, const mpm_model_t* __restrict__ md, float dt, int xcd_q) {
+    constexpr int kT = 256;
+    constexpr int kArena = arena_of<kT>();
+    // the gather tile (kArena float4) lives in the first half of the scatter arena
+    __shared__ unsigned long long s_arena[4 * kArena];
+    __shared__ float s_stash[kPairStash][kT];
+    __shared__ int s_red0[kT / 64][8];
+    __shared__ int s_red1[kT / 64][8];
+    float4* s_gather = reinterpret_cast<float4*>(s_arena);
+
+    const int S = P.stride;
+    const int group = (int)(blockIdx.x % kXcds) * xcd_q + (int)(blockIdx.x / kXcds);
+    const int nchunks = *P.chunk_count;
+    if (blockIdx.x == 0 && threadIdx.x == 0 && nchunks > P.chunk_capacity && G.error_flag)
+        atomicOr(G.error_flag, 256);
+    if (group >= nchunks) return;  // uniform, before any barrier
+    const int p0 = P.chunk_start[group], p1 = P.chunk_start[group + 1];
+    const bool periodic = md->periodic_x != 0;
+    int err = 0;
+
+    int pp[2], mat[2], scene[2];
+    bool alive[2];
+    v3 x[2];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const int p = p0 + h * kT + (int)threadIdx.x;
+        const bool sel = p < p1 && P.selection[p] == 0;
+        scene[h] = (sel && P.scene) ? P.scene[p] : 0;
+        alive[h] = sel && (unsigned)scene[h] < (unsigned)G.n_scenes;  // bad scene ids: error bit 8
+        if (sel && !alive[h]) err |= 8;
+        mat[h] = alive[h] ? P.material[p] : 0;
+        x[h] = alive[h] ? ld3(P.x, S, p) : mk3(0.f, 0.f, 0.f);
+        pp[h] = p;
+    }
+
+    // gather tile over both particles' stencils
+    int lo[3], hi[3], slo = INT_MAX, shi = INT_MIN;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        lo[a] = INT_MAX;
+        hi[a] = INT_MIN;
+    }
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        if (alive[h] && mat[h] != 7 && mat[h] != 8) {
+            const Stencil st = make_stencil(x[h], G.inv_dx);
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                lo[a] = min(lo[a], st.base[a]);
+                hi[a] = max(hi[a], st.base[a] + 3);
+            }
+            slo = min(slo, scene[h]);
+            shi = max(shi, scene[h]);
+        }
+    }
+    const Tile tg = make_tile<kT>(G, lo, hi, slo, shi, s_red0, periodic);
+    {
+        const float4* __restrict__ out = reinterpret_cast<const float4*>(G.node_out);
+        tile_for<kT>(tg, [&](int a, int gi, int gj, int gk) {
+            const long o = in_grid(G, gi, gj, gk) ? node_addr(G, tg.soff, gi, gj, gk) : -1L;
+            s_gather[a] = o >= 0 ? out[o] : make_float4(0.f, 0.f, 0.f, 0.f);
+        });
+    }
+    __syncthreads();
+
+    // gather + return map + stress of one particle (k_particle_pass's lean fused mode);
+    // leaves its scatter inputs (new x, v, C after RPIC damping, fs = -vol dt tau, mass)
+    auto front = [&](int h, v3& v, m33& Cm, m33& fs, float& mass) __attribute__((always_inline)) {
+        const int p = pp[h];
+        v = mk3(0.f, 0.f, 0.f);
+        Cm = mzero();
+        fs = mzero();
+        mass = 0.0f;
+        if (!alive[h]) return;
+        const int mt = mat[h];
+        m33 Ft, L = mzero();
+        if (mt != 7 && mt != 8) {
+            const Stencil st = make_stencil(x[h], G.inv_dx);
+            if (!stencil_in_grid(G, st, periodic)) err |= 1;
+            v3 nv;
+            m33 nC, nF;
+            if (tg.lds) {
+                const int o0 = st.base[0] - tg.lo[0], o1 = st.base[1] - tg.lo[1], o2 = st.base[2] - tg.lo[2];
+                const int d1 = tg.d1, d2 = tg.d2, b0 = tg.base;
+                gather_tp(st, G.inv_dx,
+                          [&](int i, int j, int k) {
+                              const float4 g = s_gather[b0 + ((o0 + i) * d1 + (o1 + j)) * d2 + (o2 + k)];
+                              return mk3(g.x, g.y, g.z);
+                          },
+                          nv, nC, nF);
+            } else {
+                const float4* __restrict__ gout = reinterpret_cast<const float4*>(G.node_out);
+                const size_t soff = scene_base(G, scene[h]);
+                gather_tp<true>(st, G.inv_dx,
+                                [&](int i, int j, int k) {
+                                    int ix = st.base[0] + i;
+                                    if (periodic) ix = ((ix % G.n[0]) + G.n[0]) % G.n[0];
+                                    const int iy = st.base[1] + j, iz = st.base[2] + k;
+                                    const long na = in_grid(G, ix, iy, iz) ? node_addr(G, soff, ix, iy, iz) : -1L;
+                                    v3 g = mk3(0.f, 0.f, 0.f);
+                                    if (na >= 0) {
+                                        const float4 o4 = gout[na];
+                                        g = mk3(o4.x, o4.y, o4.z);
+                                    }
+                                    return g;
+                                },
+                                nv, nC, nF);
+            }
+            v = nv;
+            Cm = nC;
+            L = nF;
+            v3 xn = x[h] + v * dt;
+            if (periodic) {
+                const float lx = (float)G.n[0] * G.dx;
+                if (xn.x < 0.0f) xn.x = xn.x + lx;
+                if (xn.x >= lx) xn.x = xn.x - lx;
+            }
+            x[h] = xn;
+            m33 A = mscale(nF, dt);
+            A.a[0] += 1.f; A.a[4] += 1.f; A.a[8] += 1.f;
+            Ft = mmul(A, ld9(P.F, S, p));
+            if (md->update_cov_with_F && P.cov) {  // update_cov, mpm_utils.py:613-633
+                float c[6];
+#pragma unroll
+                for (int q = 0; q < 6; ++q) c[q] = P.cov[q * S + p];
+                m33 cn;
+                cn.a[0] = c[0]; cn.a[1] = c[1]; cn.a[2] = c[2];
+                cn.a[3] = c[1]; cn.a[4] = c[3]; cn.a[5] = c[4];
+                cn.a[6] = c[2]; cn.a[7] = c[4]; cn.a[8] = c[5];
+                const m33 n1 = madd(cn, mscale(madd(mmul(nF, cn), mmul(cn, mT(nF))), dt));
+                P.cov[p] = n1.a[0]; P.cov[S + p] = n1.a[1]; P.cov[2 * S + p] = n1.a[2];
+                P.cov[3 * S + p] = n1.a[4]; P.cov[4 * S + p] = n1.a[5]; P.cov[5 * S + p] = n1.a[8];
+            }
+            st3(P.x, S, p, xn);
+        } else {  // stationary / rigid ids: no gather; their stored v, C, F_trial scatter
+            v = ld3(P.v, S, p);
+            Cm = ld9(P.C, S, p);
+            Ft = ld9(P.F_trial, S, p);
+        }
+        PState ps;
+        ps.mu = P.mu[p];
+        ps.lam = P.lam[p];
+        ps.yield = P.yield_stress[p];
+        ps.Jp = P.Jp[p];
+        ps.dirty_mu = ps.dirty_yield = ps.dirty_Jp = false;
+        m33 F, stress;
+        if constexpr (kSolid) {
+            stress_update(Ft, L, mt, md, dt, ps, F, stress);
+        } else {
+            if (!stress_update_viscous(Ft, L, mt, md, ps, F, stress)) err |= 4;
+        }
+        st9(P.F, S, p, F);
+        if constexpr (kKeepStress) st9(P.stress, S, p, stress);
+        if (ps.dirty_yield) P.yield_stress[p] = ps.yield;
+        if (ps.dirty_mu) { P.mu[p] = ps.mu; P.lam[p] = ps.lam; }
+        if (ps.dirty_Jp) P.Jp[p] = ps.Jp;
+        const float r = md->rpic_damping;
+        if (r != 0.0f) {
+            const m33 Ct = mT(Cm);
+            Cm = madd(mscale(Cm, 1.0f - r), mscale(msub(Cm, Ct), r / 2.0f));
+            if (r < -0.001f) Cm = mzero();
+        }
+        fs = mscale(stress, -P.vol[p] * dt);
+        mass = P.mass[p];
+    };
+
+    const int t = (int)threadIdx.x;
+    {  // first particle: its scatter inputs wait in the stash (read back by this lane only)
+        v3 v;
+        m33 Cm, fs;
+        float mass;
+        front(0, v, Cm, fs, mass);
+        s_stash[0][t] = x[0].x; s_stash[1][t] = x[0].y; s_stash[2][t] = x[0].z;
+        s_stash[3][t] = v.x; s_stash[4][t] = v.y; s_stash[5][t] = v.z;
+#pragma unroll
+        for (int e = 0; e < 9; ++e) {
+            s_stash[6 + e][t] = Cm.a[e];
+            s_stash[15 + e][t] = fs.a[e];
+        }
+        s_stash[24][t] = mass;
+    }
+    v3 vB;
+    m33 CB, fsB;
+    float mB;
+    front(1, vB, CB, fsB, mB);
+
+    // scatter tile over both particles' new stencils (its barrier also ends every read of
+    // the gather tile, whose memory the arena reuses)
+    slo = INT_MAX;
+    shi = INT_MIN;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        lo[a] = INT_MAX;
+        hi[a] = INT_MIN;
+    }
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        if (alive[h]) {
+            const Stencil st = make_stencil(x[h], G.inv_dx);
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                lo[a] = min(lo[a], st.base[a]);
+                hi[a] = max(hi[a], st.base[a] + 3);
+            }
+            slo = min(slo, scene[h]);
+            shi = max(shi, scene[h]);
+        }
+    }
+    const Tile ts = make_tile<kT>(G, lo, hi, slo, shi, s_red1, periodic);
+    tile_for<kT>(ts, [&](int a, int, int, int) {
+        s_arena[a] = 0ull;
+        s_arena[kArena + a] = 0ull;
+        s_arena[2 * kArena + a] = 0ull;
+        s_arena[3 * kArena + a] = 0ull;
+    });
+    __syncthreads();
+    const int lane = t & 63;
+    auto scatter = [&](int h, v3 v, const m33& Cm, const m33& fs, float mass) __attribute__((always_inline)) {
+        if (!alive[h]) return;
+        const Stencil st = make_stencil(x[h], G.inv_dx);
+        if (!stencil_in_grid(G, st, periodic)) err |= 1;
+        if (ts.lds) {
+            const int o0 = st.base[0] - ts.lo[0], o1 = st.base[1] - ts.lo[1], o2 = st.base[2] - ts.lo[2];
+            const int d2 = ts.d2, d12 = ts.d1 * ts.d2;
+            const int r0 = ts.base + (o0 * ts.d1 + o1) * d2 + o2;
+            const bool ok = scatter_tp_fix(st, G.dx, G.inv_dx, mass, v, Cm, fs, G.fix_scale[0], G.fix_scale[1], lane,
+                                           [&](int i, int j, int k, unsigned long long q0, unsigned long long q1,
+                                               unsigned long long q2, unsigned long long q3) {
+                                               const int a = r0 + i * d12 + j * d2 + k;
+                                               atomicAdd(s_arena + a, q0);
+                                               atomicAdd(s_arena + kArena + a, q1);
+                                               atomicAdd(s_arena + 2 * kArena + a, q2);
+                                               atomicAdd(s_arena + 3 * kArena + a, q3);
+                                           });
+            if (!ok) err |= 64;
+        } else {
+            scatter_global<true>(G, scene_base(G, scene[h]), st, periodic, mass, v, Cm, fs,
+                                 [](int, int, int, v3&, float&) { return true; }, err);
+        }
+    };
+    scatter(1, vB, CB, fsB, mB);
+    {
+        v3 v = mk3(s_stash[3][t], s_stash[4][t], s_stash[5][t]);
+        m33 Cm, fs;
+#pragma unroll
+        for (int e = 0; e < 9; ++e) {
+            Cm.a[e] = s_stash[6 + e][t];
+            fs.a[e] = s_stash[15 + e][t];
+        }
+        scatter(0, v, Cm, fs, s_stash[24][t]);
+    }
+    __syncthreads();
+    {
+        unsigned long long* __restrict__ gfix = reinterpret_cast<unsigned long long*>(G.node_fix);
+        tile_for<kT>(ts, [&](int a, int gi, int gj, int gk) {
+            const unsigned long long a0 = s_arena[a], a1 = s_arena[kArena + a], a2 = s_arena[2 * kArena + a],
+                                     a3 = s_arena[3 * kArena + a];
+            if ((a0 | a1 | a2 | a3) == 0ull) return;
+            if (!in_grid(G, gi, gj, gk)) return;
+            const long na = node_addr(G, ts.soff, gi, gj, gk);
+            if (na < 0) {  // a block without a pool slot: the contribution is dropped
+                err |= 512;
+                return;
+            }
+            const size_t o = 4 * (size_t)na;
+            atomicAdd(gfix + o + 0, a0);
+            atomicAdd(gfix + o + 1, a1);
+            atomicAdd(gfix + o + 2, a2);
+            atomicAdd(gfix + o + 3, a3);
+        });
+        mark_tile_blocks<kT>(G, ts);
+    }
+#pragma unroll
+    for (int h = 0; h < 2; ++h)
+        if (alive[h] && !(x[h].x == x[h].x && x[h].y == x[h].y && x[h].z == x[h].z)) err |= 2;
+    if (err && G.error_flag) atomicOr(G.error_flag, err);
+}
+
 inline int32_t launch_status() { return hipGetLastError() == hipSuccess ? MPM_OK : MPM_ERR_HIP; }
 
 }  // namespace
@@ -701,6 +1030,27 @@ extern "C" int32_t mpm_particle_pass(const mpm_particles_t* P, const mpm_grid_t*
                 !G->cdf.dist_prev || !G->cdf.block || !G->cdf.lanes || !G->cdf.reaction || (mode & MPM_PASS_LEAN)))
         return MPM_ERR_ARG;
     if (P->n == 0) return MPM_OK;
+    if (mode & MPM_PASS_PAIR) {  // two particles per lane over chunks of up to 512
+        const int lean = MPM_PASS_G2P | MPM_PASS_STRESS | MPM_PASS_P2G | MPM_PASS_LEAN;
+        const int mm = mode & (lean | MPM_PASS_KEEP_STRESS);
+        if (!det || cdf || (mm != lean && mm != (lean | MPM_PASS_KEEP_STRESS)) ||
+            (mode & (MPM_PASS_WG128 | MPM_PASS_PPT2 | MPM_PASS_PPT4)) || !P->chunk_start || !P->chunk_count ||
+            P->chunk_capacity < 1)
+            return MPM_ERR_ARG;
+        const int q = (P->chunk_capacity + kXcds - 1) / kXcds;
+        const dim3 grid((unsigned)(q * kXcds)), block(256);
+        const bool solid = (mode & MPM_PASS_VISCOUS_ONLY) == 0, keep = (mm & MPM_PASS_KEEP_STRESS) != 0;
+        hipStream_t s = (hipStream_t)stream;
+        if (solid && keep)
+            hipLaunchKernelGGL((k_particle_pass_pair<true, true>), grid, block, 0, s, *P, *G, model, dt, q);
+        else if (solid)
+            hipLaunchKernelGGL((k_particle_pass_pair<true, false>), grid, block, 0, s, *P, *G, model, dt, q);
+        else if (keep)
+            hipLaunchKernelGGL((k_particle_pass_pair<false, true>), grid, block, 0, s, *P, *G, model, dt, q);
+        else
+            hipLaunchKernelGGL((k_particle_pass_pair<false, false>), grid, block, 0, s, *P, *G, model, dt, q);
+        return launch_status();
+    }
     const int wg = (!cdf && !det && (mode & MPM_PASS_WG128)) ? 128 : 256;
     const int ppt = (mode & MPM_PASS_PPT4) ? 4 : ((mode & MPM_PASS_PPT2) ? 2 : 1);
     hipStream_t s = (hipStream_t)stream;

@@ -18,8 +18,10 @@ arrays, as in the reference's model); yield stress, mass and the Lame parameters
 per particle, and each scene has its own colliders (``add_box(scene, ...)``).
 
 ``autotune`` picks the particle-pass launch shape (threads per workgroup, particles per
-thread) and the sort interval by timing candidates on the loaded batch from a snapshot
-of the state, and restores the snapshot afterwards.
+thread, or the paired pass: two particles per lane over chunks of up to 512 with one
+tile and one flush for both, deterministic scatter only) and the sort interval by timing
+candidates on the loaded batch from a snapshot of the state, and restores the snapshot
+afterwards.
 """
 from __future__ import annotations
 
@@ -34,8 +36,10 @@ from .core.solver import GridConfig, HipSimulator, _resolve_device
 
 f32 = np.float32
 
-# candidate particle-pass launch shapes: (threads per workgroup, particles per thread)
-LAUNCH_SHAPES = ((256, 1), (256, 2), (256, 4), (128, 1), (128, 2))
+# candidate particle-pass launch shapes: (threads per workgroup, particles per thread,
+# paired pass)
+LAUNCH_SHAPES = ((256, 1, False), (256, 2, False), (256, 4, False), (128, 1, False), (128, 2, False),
+                 (256, 1, True))
 
 
 def launch_bits(wg: int, ppt: int) -> int:
@@ -73,7 +77,7 @@ class BatchSolver:
         self.sort_interval = int(sort_interval)
         self.guard_interval = int(guard_interval)
         self.slots_per_scene = int(wrench_slots_per_scene)
-        self.launch = (256, 1)
+        self.launch = (256, 1, False)
         self._sim: HipSimulator | None = None
         self._tick = 0
         self._step = 0
@@ -257,9 +261,15 @@ class BatchSolver:
         return self
 
     # ------------------------------------------------------------------ stepping
-    def set_launch(self, wg: int = 256, ppt: int = 1) -> "BatchSolver":
+    def set_launch(self, wg: int = 256, ppt: int = 1, paired: bool = False) -> "BatchSolver":
+        """Particle-pass launch shape; paired (with wg 256, ppt 1) selects the paired
+        interior pass, which the deterministic scatter runs (float atomics keep the
+        single-particle pass)."""
+        if paired and (wg, ppt) != (256, 1):
+            raise ValueError("the paired pass runs 256-lane workgroups with one chunk each")
         self._sim.launch_bits = launch_bits(wg, ppt)
-        self.launch = (wg, ppt)
+        self._sim.pair_pass = bool(paired)
+        self.launch = (wg, ppt, bool(paired))
         return self
 
     def _sort(self) -> bool:
@@ -398,20 +408,22 @@ class BatchSolver:
         snap = self._snapshot()
         result = {"shapes": {}, "sort": {}}
         if sim.deterministic:  # the fixed-point scatter runs 256-lane workgroups only
-            shapes = tuple(sh for sh in shapes if sh[0] == 256) or ((256, 1),)
+            shapes = tuple(sh for sh in shapes if sh[0] == 256) or ((256, 1, False),)
+        else:  # the paired pass is a deterministic-scatter kernel
+            shapes = tuple(sh for sh in shapes if not sh[2]) or ((256, 1, False),)
         try:
             best = None
-            for wg, ppt in shapes:
+            for wg, ppt, paired in shapes:
                 self._restore(snap)
-                self.set_launch(wg, ppt)
+                self.set_launch(wg, ppt, paired)
                 self.sort_interval = 1
                 self._tick = 0
                 self._time_ticks(dt, substeps, 1)
                 t = float(np.mean(self._time_ticks(dt, substeps, ticks)))
-                result["shapes"][f"wg{wg}_ppt{ppt}"] = t
+                result["shapes"]["paired512" if paired else f"wg{wg}_ppt{ppt}"] = t
                 if best is None or t < best[0]:
-                    best = (t, wg, ppt)
-            self.set_launch(best[1], best[2])
+                    best = (t, wg, ppt, paired)
+            self.set_launch(best[1], best[2], best[3])
             # sort interval model
             self._restore(snap)
             torch.cuda.synchronize(sim.device)
@@ -439,5 +451,6 @@ class BatchSolver:
             sim.use_graph = use_graph
         result["chosen"] = {"threads_per_workgroup": self.launch[0],
                             "particles_per_thread": self.launch[1],
+                            "paired": self.launch[2],
                             "sort_interval": self.sort_interval}
         return result

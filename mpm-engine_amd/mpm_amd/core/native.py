@@ -19,6 +19,7 @@ MAX_MATERIALS = 16
 PASS_G2P, PASS_STRESS, PASS_P2G, PASS_LEAN, PASS_KEEP_STRESS, PASS_VISCOUS_ONLY = 1, 2, 4, 8, 16, 32
 PASS_WG128, PASS_PPT2, PASS_PPT4 = 64, 128, 256  # launch shape (results do not depend on it)
 PASS_DETERMINISTIC = 512
+PASS_PAIR = 1024  # two particles per lane over chunks of up to 512 (deterministic lean pass)
 VISCOUS_IDS = frozenset((6, 7, 8, 10, 12))
 GRID_ZERO_IN, GRID_FIXED_IN, GRID_WRENCH_DEFER = 1, 2, 4
 MAX_FIXED_ORDER_SLOTS = 64  # wrench slots reduced in fixed order; more use float atomics
@@ -170,7 +171,7 @@ SIGNATURES = {
 }
 
 _LIB = None
-ABI_VERSION = 8  # MPM_ABI_VERSION of include/mpm_hip.h this binding is written for
+ABI_VERSION = 9  # MPM_ABI_VERSION of include/mpm_hip.h this binding is written for
 
 
 class NativeError(RuntimeError):

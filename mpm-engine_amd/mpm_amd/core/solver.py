@@ -207,8 +207,13 @@ class HipSimulator:
         self.node_fix = None
         # particle-pass chunks (mpm_sort_chunks), computed at every sort; None until then
         self.chunk_start = None
-        self.chunk_count = None
+        self.chunk_count = None  # device int32 [2]: chunks of <= 256, of <= 512 particles
         self.chunk_capacity = 0
+        # paired interior pass (PASS_PAIR): two particles per lane over chunks of <= 512,
+        # for the deterministic lean substep when no launch shape is forced
+        self.pair_pass = False
+        self.chunk_start2 = None
+        self.chunk_capacity2 = 0
         # block pool (enable_pool): node storage for the active blocks only
         self.pool = False
         self.block_slot = None
@@ -235,6 +240,11 @@ class HipSimulator:
         P.chunk_start = native.ptr(self.chunk_start)
         P.chunk_count = native.ptr(self.chunk_count)
         P.chunk_capacity = self.chunk_capacity
+        P2 = native.Particles()  # the paired pass's view: the same arrays, 512-particle chunks
+        C.memmove(C.byref(P2), C.byref(P), C.sizeof(P))
+        P2.chunk_start = native.ptr(self.chunk_start2)
+        P2.chunk_count = (native.ptr(self.chunk_count) + 4) if self.chunk_count is not None else None
+        P2.chunk_capacity = self.chunk_capacity2
         G = native.Grid()
         G.node_in, G.node_out = native.ptr(self.node_in), native.ptr(self.node_out)
         G.n[:] = [self.n_grid] * 3
@@ -251,8 +261,8 @@ class HipSimulator:
         if self._fix_exp is not None:
             G.fix_scale[0] = math.ldexp(1.0, self._fix_exp[0])
             G.fix_scale[1] = math.ldexp(1.0, self._fix_exp[1])
-        self.P, self.G = P, G
-        self.pP, self.pG = C.byref(P), C.byref(G)
+        self.P, self.P2, self.G = P, P2, G
+        self.pP, self.pP2, self.pG = C.byref(P), C.byref(P2), C.byref(G)
 
     def stream(self) -> int:
         return self.torch.cuda.current_stream(self.device).cuda_stream
@@ -735,27 +745,33 @@ class HipSimulator:
 
     def _chunks(self, sorted_keys, cells: bool):
         """Particle-pass chunks over the sorted particles (mpm_sort_chunks): runs of at most
-        256 particles within 4 consecutive blocks of one block column. The chunk count is
-        read back (the sort already waits for its unsorted flag) to size the launch."""
+        256 particles within 4 consecutive blocks of one block column, and runs of at most
+        512 for the paired pass. The two chunk counts are read back in one copy (the sort
+        already waits for its unsorted flag) to size the launches."""
         torch = self.torch
         n = self.n_particles
         if self.chunk_start is None:
             self.chunk_start = torch.zeros(n + 1, dtype=torch.int32, device=self.device)
-            self.chunk_count = torch.zeros(1, dtype=torch.int32, device=self.device)
+            self.chunk_start2 = torch.zeros(n + 1, dtype=torch.int32, device=self.device)
+            self.chunk_count = torch.zeros(2, dtype=torch.int32, device=self.device)
         tb = int(self.lib.mpm_sort_chunks_temp_bytes(n))
         temp = torch.empty(max(tb, 1), dtype=torch.uint8, device=self.device)
         nb = self.n_grid // 4 + 2
-        native.check(self.lib.mpm_sort_chunks(native.ptr(sorted_keys), n, 6 if cells else 0, nb, 4, 256,
-                                              native.ptr(self.chunk_start), native.ptr(self.chunk_count),
-                                              native.ptr(temp), tb, self.stream()), "sort_chunks")
-        count = int(self.chunk_count.item())
-        # launch capacity in steps of 64 chunks, kept while it fits (a new capacity means a
-        # new captured graph)
-        cap = self.chunk_capacity
-        if count > cap or cap - count > 256:
-            cap = (count + 63) // 64 * 64
-        if cap != self.chunk_capacity:
-            self.chunk_capacity = cap
+        for size, starts, cnt in ((256, self.chunk_start, native.ptr(self.chunk_count)),
+                                  (512, self.chunk_start2, native.ptr(self.chunk_count) + 4)):
+            native.check(self.lib.mpm_sort_chunks(native.ptr(sorted_keys), n, 6 if cells else 0, nb, 4, size,
+                                                  native.ptr(starts), cnt, native.ptr(temp), tb, self.stream()),
+                         "sort_chunks")
+        counts = self.chunk_count.tolist()
+        # launch capacities in steps of 64 chunks, kept while they fit (a new capacity
+        # means a new captured graph)
+        caps = []
+        for count, cap in zip(counts, (self.chunk_capacity, self.chunk_capacity2)):
+            if count > cap or cap - count > 256:
+                cap = (count + 63) // 64 * 64
+            caps.append(cap)
+        if caps != [self.chunk_capacity, self.chunk_capacity2]:
+            self.chunk_capacity, self.chunk_capacity2 = caps
             self._graph = None
             self._structs()
 
@@ -867,7 +883,7 @@ class HipSimulator:
         elif self.use_graph and self._ticks > 0:
             key = (S, float(dt), bool(fused), K, lists, native.ptr(self.sched_dev),
                    self.update_cov_with_F, n_slots, native.ptr(self.grid_ws), native.ptr(self.wrench),
-                   self.chunk_capacity,
+                   self.chunk_capacity, self.chunk_capacity2, self.pair_pass,
                    frozenset(self._mat_ids) <= native.VISCOUS_IDS, self.launch_bits,
                    None if cdf is None else (cdf.n, int(cdf.mask_off), native.ptr(self.cdf_tag)),
                    self._fix_exp if self.deterministic else None,
@@ -902,12 +918,19 @@ class HipSimulator:
         return self.scene_colliders.schedule(self.time, dt, S)
 
     def _pass(self, mode, dt, s):
+        P = self.pP
+        fused = native.PASS_G2P | native.PASS_STRESS | native.PASS_P2G
+        if (self.pair_pass and self.deterministic and not self.launch_bits and self.chunk_start2 is not None
+                and (mode & native.PASS_LEAN) and (mode & fused) == fused
+                and (self.cdf is None or self.cdf.n == 0)):
+            P = self.pP2
+            mode |= native.PASS_PAIR
         mode |= self.launch_bits
         if self.deterministic:
             mode |= native.PASS_DETERMINISTIC
         if self._mat_ids <= native.VISCOUS_IDS:
             mode |= native.PASS_VISCOUS_ONLY
-        native.check(self.lib.mpm_particle_pass(self.pP, self.pG, native.ptr(self.model_dev), dt, mode, s),
+        native.check(self.lib.mpm_particle_pass(P, self.pG, native.ptr(self.model_dev), dt, mode, s),
                      "particle_pass")
 
     def _grid(self, dt, k_row, K, lists, s):

@@ -94,6 +94,20 @@ def test_launch_shapes_agree(shape):
         assert np.abs(b.x(s) - ref.x(s)).max() <= 1e-4 * sc.grid.dx
 
 
+def test_paired_pass_is_bit_identical_in_the_block_pool():
+    """The paired interior pass over the shared block pool (pool slots through node_addr,
+    per-scene tiles) against the single-particle pass: the fixed-point sums make every
+    position bitwise equal."""
+    sc = _scenes(6, seed=3)
+    a, b = _batch(sc), _batch(sc)
+    a.set_launch(256, 1, True)
+    _run(a)
+    _run(b)
+    assert a._sim.pool and a._sim.chunk_count.tolist()[1] > 0
+    for s in range(6):
+        assert np.array_equal(a.x(s), b.x(s))
+
+
 def test_dense_and_sparse_and_unsorted_agree():
     sc = _scenes(5, seed=4)
     ref = _run(_batch(sc))
@@ -157,8 +171,9 @@ def test_autotune_restores_state_and_picks_a_candidate():
     a = _batch(sc)
     _run(a, ticks=1)
     res = a.autotune(DT, 25, ticks=1, sort_intervals=(1, 2))
-    assert set(res["chosen"]) == {"threads_per_workgroup", "particles_per_thread", "sort_interval"}
-    assert (a.launch[0], a.launch[1]) in LAUNCH_SHAPES and a.sort_interval in (1, 2)
+    assert set(res["chosen"]) == {"threads_per_workgroup", "particles_per_thread", "paired", "sort_interval"}
+    assert a.launch in LAUNCH_SHAPES and a.sort_interval in (1, 2)
+    assert "paired512" in res["shapes"]  # the deterministic batch times the paired pass too
     ref = _run(_batch(sc), ticks=2)
     _run(a, ticks=1)
     T = 2 * 25 * DT
