@@ -24,6 +24,9 @@ MPM_DEV v3 operator-(v3 a, v3 b) { return mk3(a.x - b.x, a.y - b.y, a.z - b.z); 
 MPM_DEV v3 operator*(v3 a, float s) { return mk3(a.x * s, a.y * s, a.z * s); }
 MPM_DEV v3 operator*(float s, v3 a) { return mk3(a.x * s, a.y * s, a.z * s); }
 MPM_DEV v3 vdiv(v3 a, float s) { return mk3(a.x / s, a.y / s, a.z / s); }
+// a s + c with one rounding per component (fused multiply-add, stated in the source so the
+// rounding does not depend on what the compiler contracts)
+MPM_DEV v3 fma3(v3 a, float s, v3 c) { return mk3(fmaf(a.x, s, c.x), fmaf(a.y, s, c.y), fmaf(a.z, s, c.z)); }
 MPM_DEV float dot(v3 a, v3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
 MPM_DEV float len(v3 a) { return sqrtf(dot(a, a)); }
 MPM_DEV v3 cross(v3 a, v3 b) {

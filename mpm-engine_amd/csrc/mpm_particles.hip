@@ -72,26 +72,26 @@ MPM_DEV void gather_tp(const Stencil& st, float inv_dx, Fetch fetch, v3& nv, m33
 #pragma unroll
             for (int k = 0; k < 3; ++k) {
                 const v3 g = fetch(i, j, k);
-                A = A + g * st.w[2][k];
-                B = B + g * st.dw[2][k];
-                Z = Z + g * (st.w[2][k] * ((float)k - st.fx[2]));
+                A = fma3(g, st.w[2][k], A);
+                B = fma3(g, st.dw[2][k], B);
+                Z = fma3(g, st.w[2][k] * ((float)k - st.fx[2]), Z);
             }
             const float wj = st.w[1][j];
-            Ai = Ai + A * wj;
-            Ay = Ay + A * (wj * ((float)j - st.fx[1]));
-            Bi = Bi + B * wj;
-            Cz = Cz + Z * wj;
-            Dy = Dy + A * st.dw[1][j];
+            Ai = fma3(A, wj, Ai);
+            Ay = fma3(A, wj * ((float)j - st.fx[1]), Ay);
+            Bi = fma3(B, wj, Bi);
+            Cz = fma3(Z, wj, Cz);
+            Dy = fma3(A, st.dw[1][j], Dy);
         }
         const float wi = kRolled ? sel3(i, st.w[0]) : st.w[0][i];
         const float dwi = kRolled ? sel3(i, st.dw[0]) : st.dw[0][i];
-        nv = nv + Ai * wi;
-        cX = cX + Ai * (wi * ((float)i - st.fx[0]));
-        cY = cY + Ay * wi;
-        cZ = cZ + Cz * wi;
-        fX = fX + Ai * dwi;
-        fY = fY + Dy * wi;
-        fZ = fZ + Bi * wi;
+        nv = fma3(Ai, wi, nv);
+        cX = fma3(Ai, wi * ((float)i - st.fx[0]), cX);
+        cY = fma3(Ay, wi, cY);
+        cZ = fma3(Cz, wi, cZ);
+        fX = fma3(Ai, dwi, fX);
+        fY = fma3(Dy, wi, fY);
+        fZ = fma3(Bi, wi, fZ);
     }
     const float c4 = inv_dx * 4.0f;
     nC.a[0] = cX.x * c4; nC.a[1] = cY.x * c4; nC.a[2] = cZ.x * c4;
@@ -173,18 +173,18 @@ MPM_DEV bool scatter_tp_fix(const Stencil& st, float dx, float inv_dx, float mas
     for (int ti = 0; ti < 3; ++ti) {
         const int i = i0 + ti >= 3 ? i0 + ti - 3 : i0 + ti;
         const float wi = sel3(i, st.w[0]);
-        const v3 Pi = (a + b[0] * (float)i) * wi + g[0] * sel3(i, st.dw[0]);
+        const v3 Pi = fma3(fma3(b[0], (float)i, a), wi, g[0] * sel3(i, st.dw[0]));
         const v3 Qi = b[1] * wi, Ri = g[1] * wi, Ti = b[2] * wi, Ui = g[2] * wi;
         const float mi = ms * wi;
 #pragma unroll
         for (int tj = 0; tj < 3; ++tj) {
             const float wj = wjr[tj];
-            const v3 Pij = (Pi + Qi * (float)jr[tj]) * wj + Ri * djr[tj];
+            const v3 Pij = fma3(fma3(Qi, (float)jr[tj], Pi), wj, Ri * djr[tj]);
             const v3 Tij = Ti * wj, Uij = Ui * wj;
             const float mij = mi * wj;
 #pragma unroll
             for (int tk = 0; tk < 3; ++tk) {
-                const v3 mom = (Pij + Tij * (float)kr[tk]) * wkr[tk] + Uij * dkr[tk];
+                const v3 mom = fma3(fma3(Tij, (float)kr[tk], Pij), wkr[tk], Uij * dkr[tk]);
                 add(i, jr[tj], kr[tk], fix_round(mom.x), fix_round(mom.y), fix_round(mom.z),
                     fix_round(mij * wkr[tk]));
             }
