@@ -215,7 +215,7 @@ MPM_DEV m33 stress_viscous(float J, float bulk, const m33& L, float eta, float t
         eta_app = lerp_table(md->eta_table, md->eta_table_n, md->eta_table_smin, md->eta_table_smax,
                              logf(gd) * 0.43429448190325176f);
     else
-        eta_app = eta + tau_y / gd + pk * powf(gd, pn - 1.0f);
+        eta_app = eta + tau_y / gd + (pk != 0.0f ? pk * powf(gd, pn - 1.0f) : 0.0f);  // Newtonian: K = 0
     m33 c = mscale(D, 2.0f * eta_app);
     c.a[0] += pressure; c.a[4] += pressure; c.a[8] += pressure;
     return mscale(c, J);

@@ -40,21 +40,46 @@ MPM_DEV bool fits(const int lo[3], const int hi[3], int cap, int& d0, int& d1, i
     return d0 * d1 * d2 <= cap;
 }
 
+// Wavefront min / max with DPP (all 64 lanes active): row shifts by 1, 2, 4, 8 leave the
+// row's extreme in its lane 15 (min and max are idempotent, so overlapping windows are
+// fine; lanes shifted in from outside the row keep the identity), row_bcast:15 and
+// row_bcast:31 carry rows 0-2 into lane 63, which every lane then reads.
+template <bool kMax>
+MPM_DEV int wave_extreme(int x) {
+    const int id = kMax ? INT_MIN : INT_MAX;
+    auto op = [](int a, int b) { return kMax ? max(a, b) : min(a, b); };
+    x = op(x, __builtin_amdgcn_update_dpp(id, x, 0x111, 0xf, 0xf, false));  // row_shr:1
+    x = op(x, __builtin_amdgcn_update_dpp(id, x, 0x112, 0xf, 0xf, false));  // row_shr:2
+    x = op(x, __builtin_amdgcn_update_dpp(id, x, 0x114, 0xf, 0xf, false));  // row_shr:4
+    x = op(x, __builtin_amdgcn_update_dpp(id, x, 0x118, 0xf, 0xf, false));  // row_shr:8
+    x = op(x, __builtin_amdgcn_update_dpp(id, x, 0x142, 0xa, 0xf, false));  // row_bcast:15 -> rows 1, 3
+    x = op(x, __builtin_amdgcn_update_dpp(id, x, 0x143, 0xc, 0xf, false));  // row_bcast:31 -> rows 2, 3
+    return __builtin_amdgcn_readlane(x, 63);
+}
+
+// n = (i * d1 + j) * d2 + k -> (i, j, k) for n < 2^16 and d1, d2 <= 2^10 without integer
+// division: floor((n + 1/2) / d) from a float reciprocal is exact there (the half keeps
+// the product of an exact multiple above the integer, and the rounding stays far below
+// 1/(2d) of the next one)
+MPM_DEV void split3(int n, float inv_d1, float inv_d2, int d1, int d2, int& i, int& j, int& k) {
+    const int r = (int)(((float)n + 0.5f) * inv_d2);
+    k = n - r * d2;
+    i = (int)(((float)r + 0.5f) * inv_d1);
+    j = r - i * d1;
+}
+
 // per-thread stencil box [lo, hi) and scene range [slo, shi] (sentinels INT_MAX / INT_MIN
 // when inactive) -> tile; contains one __syncthreads, so every thread of the workgroup
 // must call it
 template <int kT>
 MPM_DEV Tile make_tile(const mpm_grid_t& G, int lo[3], int hi[3], int slo, int shi, int (*red)[8], bool periodic) {
 #pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-#pragma unroll
-        for (int a = 0; a < 3; ++a) {
-            lo[a] = min(lo[a], __shfl_xor(lo[a], off, 64));
-            hi[a] = max(hi[a], __shfl_xor(hi[a], off, 64));
-        }
-        slo = min(slo, __shfl_xor(slo, off, 64));
-        shi = max(shi, __shfl_xor(shi, off, 64));
+    for (int a = 0; a < 3; ++a) {
+        lo[a] = wave_extreme<false>(lo[a]);
+        hi[a] = wave_extreme<true>(hi[a]);
     }
+    slo = wave_extreme<false>(slo);
+    shi = wave_extreme<true>(shi);
     const int wave = threadIdx.x >> 6;
     if ((threadIdx.x & 63) == 0) {
         for (int a = 0; a < 3; ++a) {
@@ -109,9 +134,10 @@ MPM_DEV void tile_for(const Tile& t, F f) {
     if (!t.lds) return;
     const int start = t.wg ? (int)threadIdx.x : (int)(threadIdx.x & 63);
     const int step = t.wg ? kT : 64;
+    const float inv1 = 1.0f / (float)t.d1, inv2 = 1.0f / (float)t.d2;
     for (int n = start; n < t.vol; n += step) {
-        const int k = n % t.d2, r = n / t.d2;
-        const int j = r % t.d1, i = r / t.d1;
+        int i, j, k;
+        split3(n, inv1, inv2, t.d1, t.d2, i, j, k);
         f(t.base + n, t.lo[0] + i, t.lo[1] + j, t.lo[2] + k);
     }
 }
@@ -135,9 +161,10 @@ MPM_DEV void mark_tile_blocks(const mpm_grid_t& G, const Tile& t) {
     const size_t boff = t.soff >> 6;
     const int start = t.wg ? (int)threadIdx.x : (int)(threadIdx.x & 63);
     const int step = t.wg ? kT : 64;
+    const float inv1 = 1.0f / (float)nbk[1], inv2 = 1.0f / (float)nbk[2];
     for (int n = start; n < total; n += step) {
-        const int k = n % nbk[2], r = n / nbk[2];
-        const int j = r % nbk[1], i = r / nbk[1];
+        int i, j, k;
+        split3(n, inv1, inv2, nbk[1], nbk[2], i, j, k);
         G.block_touch[boff + ((size_t)(b0[0] + i) * G.nb[1] + (b0[1] + j)) * G.nb[2] + (b0[2] + k)] = 3;
     }
 }
