@@ -129,9 +129,11 @@ MPM_DEV unsigned long long fix_round(float y) {
 // |dw| <= 1 every contribution is below |a| + 2 sum_c |b_c| + sum_c |g_c| per component.
 // Beyond 2^51 units (or NaN) the particle adds nothing and false is returned (error bit
 // 64). add(i, j, k, q_momentum[3], q_mass) receives the rounded integers.
+// si, sj: the caller's node strides along i and j (k has stride 1); add also receives the
+// node's offset i si + j sj + k, built from per-particle row offsets (two adds per node).
 template <class Add>
 MPM_DEV bool scatter_tp_fix(const Stencil& st, float dx, float inv_dx, float mass, v3 v, const m33& Cm,
-                            const m33& fs, float Sm, float Sp, int rot, Add add) {
+                            const m33& fs, float Sm, float Sp, int rot, int si, int sj, Add add) {
     const float cb = mass * dx * Sp, cg = inv_dx * Sp;
     v3 b[3], g[3];
 #pragma unroll
@@ -157,13 +159,14 @@ MPM_DEV bool scatter_tp_fix(const Stencil& st, float dx, float inv_dx, float mas
     // sums, so the order changes no bit of the result.
     const int rho = rot % 27;
     const int i0 = mod3(rho / 9 - st.base[0]), j0 = mod3((rho / 3) % 3 - st.base[1]), k0 = mod3(rho % 3 - st.base[2]);
-    int jr[3], kr[3];
+    int jr[3], kr[3], jo[3];
     float wjr[3], djr[3], wkr[3], dkr[3];
 #pragma unroll
     for (int t = 0; t < 3; ++t) {
         const int j = j0 + t >= 3 ? j0 + t - 3 : j0 + t, k = k0 + t >= 3 ? k0 + t - 3 : k0 + t;
         jr[t] = j;
         kr[t] = k;
+        jo[t] = j * sj;
         wjr[t] = sel3(j, st.w[1]);
         djr[t] = sel3(j, st.dw[1]);
         wkr[t] = sel3(k, st.w[2]);
@@ -172,6 +175,7 @@ MPM_DEV bool scatter_tp_fix(const Stencil& st, float dx, float inv_dx, float mas
 #pragma unroll
     for (int ti = 0; ti < 3; ++ti) {
         const int i = i0 + ti >= 3 ? i0 + ti - 3 : i0 + ti;
+        const int io = i * si;
         const float wi = sel3(i, st.w[0]);
         const v3 Pi = fma3(fma3(b[0], (float)i, a), wi, g[0] * sel3(i, st.dw[0]));
         const v3 Qi = b[1] * wi, Ri = g[1] * wi, Ti = b[2] * wi, Ui = g[2] * wi;
@@ -182,10 +186,11 @@ MPM_DEV bool scatter_tp_fix(const Stencil& st, float dx, float inv_dx, float mas
             const v3 Pij = fma3(fma3(Qi, (float)jr[tj], Pi), wj, Ri * djr[tj]);
             const v3 Tij = Ti * wj, Uij = Ui * wj;
             const float mij = mi * wj;
+            const int ijo = io + jo[tj];
 #pragma unroll
             for (int tk = 0; tk < 3; ++tk) {
                 const v3 mom = fma3(fma3(Tij, (float)kr[tk], Pij), wkr[tk], Uij * dkr[tk]);
-                add(i, jr[tj], kr[tk], fix_round(mom.x), fix_round(mom.y), fix_round(mom.z),
+                add(i, jr[tj], kr[tk], ijo + kr[tk], fix_round(mom.x), fix_round(mom.y), fix_round(mom.z),
                     fix_round(mij * wkr[tk]));
             }
         }
@@ -221,8 +226,8 @@ MPM_DEV void scatter_global(const mpm_grid_t& G, size_t soff, const Stencil& st,
     if constexpr (kDet) {  // the same contributions as the LDS path, bit for bit
         unsigned long long* __restrict__ gfix = reinterpret_cast<unsigned long long*>(G.node_fix);
         const bool ok = scatter_tp_fix(st, G.dx, G.inv_dx, mass, v, Cm, fs, G.fix_scale[0], G.fix_scale[1],
-                                       (int)(threadIdx.x & 63u),
-                                       [&](int i, int j, int k, unsigned long long q0, unsigned long long q1,
+                                       (int)(threadIdx.x & 63u), 0, 0,
+                                       [&](int i, int j, int k, int, unsigned long long q0, unsigned long long q1,
                                            unsigned long long q2, unsigned long long q3) {
             int ix = st.base[0] + i;
             if (periodic) ix = ((ix % G.n[0]) + G.n[0]) % G.n[0];
@@ -654,9 +659,9 @@ __global__ __launch_bounds__(kT) __attribute__((amdgpu_waves_per_eu(MPM_PASS_WAV
                 const int d2 = ts.d2, d12 = ts.d1 * ts.d2;
                 const int r0 = ts.base + (o0 * ts.d1 + o1) * d2 + o2;
                 const bool ok = scatter_tp_fix(st, G.dx, G.inv_dx, mass, v, Cm, fs, G.fix_scale[0], G.fix_scale[1],
-                                               lane, [&](int i, int j, int k, unsigned long long q0, unsigned long long q1,
-                                                   unsigned long long q2, unsigned long long q3) {
-                    const int a = r0 + i * d12 + j * d2 + k;
+                                               lane, d12, d2, [&](int, int, int, int off, unsigned long long q0,
+                                                   unsigned long long q1, unsigned long long q2, unsigned long long q3) {
+                    const int a = r0 + off;
                     unsigned long long* sc = reinterpret_cast<unsigned long long*>(&s_scatter[0][0]);
                     atomicAdd(sc + a, q0);
                     atomicAdd(sc + kArena + a, q1);
@@ -957,9 +962,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MPM_PASS_WA
             const int d2 = ts.d2, d12 = ts.d1 * ts.d2;
             const int r0 = ts.base + (o0 * ts.d1 + o1) * d2 + o2;
             const bool ok = scatter_tp_fix(st, G.dx, G.inv_dx, mass, v, Cm, fs, G.fix_scale[0], G.fix_scale[1], lane,
-                                           [&](int i, int j, int k, unsigned long long q0, unsigned long long q1,
-                                               unsigned long long q2, unsigned long long q3) {
-                                               const int a = r0 + i * d12 + j * d2 + k;
+                                           d12, d2, [&](int, int, int, int off, unsigned long long q0,
+                                                        unsigned long long q1, unsigned long long q2,
+                                                        unsigned long long q3) {
+                                               const int a = r0 + off;
                                                atomicAdd(s_arena + a, q0);
                                                atomicAdd(s_arena + kArena + a, q1);
                                                atomicAdd(s_arena + 2 * kArena + a, q2);
