@@ -110,7 +110,8 @@ typedef struct mpm_particles {
                                    at most 256 of them; NULL: chunks of 256 consecutive particles
                                    (also with MPM_PASS_WG128, which uses 128) */
     const int32_t *chunk_count; /* device int: chunks in chunk_start */
-    int32_t chunk_capacity;     /* chunks the pass launches, >= *chunk_count (with chunk_start) */
+    int32_t chunk_capacity;     /* chunks one sweep of the launch covers (with chunk_start): a
+                                   size hint, more chunks are taken by further sweeps */
     int32_t pad_;
 } mpm_particles_t;
 
@@ -166,8 +167,7 @@ typedef struct mpm_grid {
                              met a MPM_PASS_VISCOUS_ONLY pass, 8 = a scene id outside
                              [0, n_scenes) (that particle is skipped), 64 = a fixed-point
                              contribution beyond 2^51 units (deterministic scatter: the
-                             scale is too fine for this state), 256 = more chunks than
-                             P->chunk_capacity (the chunks beyond were not processed),
+                             scale is too fine for this state),
                              512 = a scatter into a block without a pool slot (that
                              contribution was dropped) */
     int32_t n_scenes;     /* independent scenes sharing the buffers (>= 1): scene s owns

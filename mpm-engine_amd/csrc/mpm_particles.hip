@@ -371,10 +371,12 @@ __global__ __launch_bounds__(kT) __attribute__((amdgpu_waves_per_eu(MPM_PASS_WAV
     // chunks: group-aligned runs of sorted particles (chunk_start), or kT consecutive ones
     const bool chunked = P.chunk_start != nullptr;
     const int nchunks = chunked ? *P.chunk_count : (P.n + kT - 1) / kT;
-    if (chunked && blockIdx.x == 0 && threadIdx.x == 0 && nchunks > P.chunk_capacity && G.error_flag)
-        atomicOr(G.error_flag, 256);
-    for (int it = 0; it < ppt; ++it) {
-        const int chunk = group * ppt + it;
+    // Chunks beyond one sweep of the launch (more than chunk_capacity of them: the host
+    // sizes the launch from an earlier count without waiting for this one) are taken by
+    // further sweeps, so the capacity sizes the launch and never bounds the work.
+    const int sweep = (int)gridDim.x * ppt;
+    for (int it = 0;; ++it) {
+        const int chunk = (it / ppt) * sweep + group * ppt + it % ppt;
         if (chunk >= nchunks) return;  // past the last chunk (uniform exit before any barrier)
         if (it > 0) __syncthreads();    // the previous chunk's LDS reads are done
         const int p0 = chunked ? P.chunk_start[chunk] : chunk * kT;
@@ -749,10 +751,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MPM_PASS_WA
     const int S = P.stride;
     const int group = (int)(blockIdx.x % kXcds) * xcd_q + (int)(blockIdx.x / kXcds);
     const int nchunks = *P.chunk_count;
-    if (blockIdx.x == 0 && threadIdx.x == 0 && nchunks > P.chunk_capacity && G.error_flag)
-        atomicOr(G.error_flag, 256);
-    if (group >= nchunks) return;  // uniform, before any barrier
-    const int p0 = P.chunk_start[group], p1 = P.chunk_start[group + 1];
+    // further sweeps take the chunks beyond the launch (see k_particle_pass)
+    for (int chunk = group; chunk < nchunks; chunk += (int)gridDim.x) {
+    if (chunk != group) __syncthreads();  // the previous chunk's LDS reads are done
+    const int p0 = P.chunk_start[chunk], p1 = P.chunk_start[chunk + 1];
     const bool periodic = md->periodic_x != 0;
     int err = 0;
 
@@ -1013,6 +1015,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MPM_PASS_WA
     for (int h = 0; h < 2; ++h)
         if (alive[h] && !(x[h].x == x[h].x && x[h].y == x[h].y && x[h].z == x[h].z)) err |= 2;
     if (err && G.error_flag) atomicOr(G.error_flag, err);
+    }  // chunk sweep
 }
 
 inline int32_t launch_status() { return hipGetLastError() == hipSuccess ? MPM_OK : MPM_ERR_HIP; }

@@ -303,12 +303,18 @@ class BatchSolver:
         """One control tick of `substeps` substeps for every scene: guard, optional block
         sort, active-block rebuild, then the tick (a hipGraph replay after the first)."""
         sim = self._sim
-        if self._tick % max(1, self.guard_interval) == 0:
-            self._guard(dt, substeps)
-        if self.sort_interval and self._tick % self.sort_interval == 0:
-            self._sort()
-        if self.sparse:
-            sim.rebuild_active_blocks()
+        # collider rows first: the host builds them while the GPU finishes the last tick
+        sim.prepare_tick(dt, substeps)
+        try:
+            if self._tick % max(1, self.guard_interval) == 0:
+                self._guard(dt, substeps)
+            if self.sort_interval and self._tick % self.sort_interval == 0:
+                self._sort()
+            if self.sparse:
+                sim.rebuild_active_blocks()
+        except BaseException:
+            sim.drop_prepared()
+            raise
         self._tick += 1
         sim.run_tick(dt, substeps, fused=self.fused, sparse=self.sparse)
         self._step += substeps

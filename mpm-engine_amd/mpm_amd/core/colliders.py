@@ -127,6 +127,14 @@ class ColliderSet:
         self.items.append(c)
         return len(self.items) - 1
 
+    def poses(self):
+        """Snapshot of the pose state a schedule advances (point, rotation, quaternion)."""
+        return [(list(c.point), np.array(c.rot, copy=True), list(c.quat)) for c in self.items]
+
+    def set_poses(self, snap):
+        for c, (p, r, q) in zip(self.items, snap):
+            c.point, c.rot, c.quat = p, r, q
+
     # ------------------------------------------------------------------ boxes
     def aabb(self, c: Collider):
         """Current restricted launch box of c, None when it misses the grid, or the
@@ -373,6 +381,13 @@ class SceneColliders:
 
     def _signature(self):
         return tuple((len(cs.items), cs.restrict_bc) for cs in self.sets)
+
+    def poses(self):
+        return [cs.poses() for cs in self.sets]
+
+    def set_poses(self, snap):
+        for cs, sn in zip(self.sets, snap):
+            cs.set_poses(sn)
 
     def _build_template(self, K):
         ns = len(self.sets)

@@ -24,7 +24,7 @@ VISCOUS_IDS = frozenset((6, 7, 8, 10, 12))
 GRID_ZERO_IN, GRID_FIXED_IN, GRID_WRENCH_DEFER = 1, 2, 4
 MAX_FIXED_ORDER_SLOTS = 64  # wrench slots reduced in fixed order; more use float atomics
 ERR_STENCIL, ERR_NONFINITE, ERR_SOLID_IN_VISCOUS, ERR_SCENE = 1, 2, 4, 8
-ERR_ADJ_MATERIAL, ERR_ADJ_COLLIDER, ERR_FIXED_RANGE, ERR_CHUNKS, ERR_POOL = 16, 32, 64, 256, 512
+ERR_ADJ_MATERIAL, ERR_ADJ_COLLIDER, ERR_FIXED_RANGE, ERR_POOL = 16, 32, 64, 512
 COLLIDER_PLANE, COLLIDER_BOX, COLLIDER_CUP, COLLIDER_WALLS, COLLIDER_SDF, COLLIDER_POINTCLOUD = \
     1, 2, 3, 4, 5, 6
 

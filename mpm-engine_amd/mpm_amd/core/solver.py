@@ -134,7 +134,7 @@ class HipSimulator:
         # per-block scatter / write state: the grid update skips blocks nothing reached
         self.block_touch = t(self.n_blocks, dt=torch.int32)
         self.error_flag = t(1, dt=torch.int32)
-        self.bounds = t(8)
+        self.bounds = t(9)  # mpm_bounds' 8 floats, then the error flag's bits (one readback)
         # block lists and device tick control
         self.box_list = t(self.n_blocks, dt=torch.int32)
         self.ctl = t(native.TICK_CTL_DTYPE.itemsize // 4, dt=torch.int32)
@@ -205,6 +205,8 @@ class HipSimulator:
         # deterministic scatter (fixed-point integer accumulation), off by default
         self.deterministic = False
         self.node_fix = None
+        self._prepared = None  # prepare_tick's collider rows for the coming tick
+        self._sort_buf = None  # sort_by_block's work buffers
         # particle-pass chunks (mpm_sort_chunks), computed at every sort; None until then
         self.chunk_start = None
         self.chunk_count = None  # device int32 [2]: chunks of <= 256, of <= 512 particles
@@ -577,8 +579,9 @@ class HipSimulator:
         """(min xyz, max xyz, max |v|, non-finite count, error flag) via a device
         reduction; one small device->host copy."""
         native.check(self.lib.mpm_bounds(self.pP, native.ptr(self.bounds), self.stream()), "bounds")
+        self.bounds[8:9].copy_(self.error_flag.view(self.torch.float32))
         b = self.bounds.cpu().numpy()
-        flag = int(self.error_flag.item())
+        flag = int(b[8:9].view(np.int32)[0])
         self._vmax = float(b[6])
         return b[0:3].copy(), b[3:6].copy(), float(b[6]), int(b[7]), flag
 
@@ -709,66 +712,84 @@ class HipSimulator:
     # --------------------------------------------------------------- sort
     def sort_by_block(self, cells: bool = False) -> bool:
         """Stable block-key sort on device (solver.py:454-467, mpm_solver_warp.py:
-        1548-1619); returns False (no permutation) when already ordered. With `cells`
-        the key is refined by the stencil base cell inside the block."""
+        1548-1619); returns False (no permutation) when already ordered, read back from a
+        device flag right after the key kernel. With `cells` the key is refined by the
+        stencil base cell inside the block. The work buffers are allocated once."""
         torch = self.torch
         n = self.n_particles
         if n < 2:
             return False
         dev = self.device
-        keys = torch.empty(n, dtype=torch.int32, device=dev)
-        vals = torch.empty(n, dtype=torch.int32, device=dev)
-        flag = torch.zeros(1, dtype=torch.int32, device=dev)
+        if self._sort_buf is None:
+            tb = int(self.lib.mpm_sort_temp_bytes(n))
+            i32 = dict(dtype=torch.int32, device=dev)
+            self._sort_buf = {"keys": torch.empty(n, **i32), "vals": torch.empty(n, **i32),
+                              "flag": torch.zeros(1, **i32), "keys_o": torch.empty(n, **i32),
+                              "perm": torch.empty(n, **i32), "tb": tb,
+                              "temp": torch.empty(max(tb, 1), dtype=torch.uint8, device=dev),
+                              "scratch": torch.empty(9 * n, dtype=torch.float32, device=dev),
+                              "density": torch.empty_like(self.density)}
+        b = self._sort_buf
         s = self.stream()
         keyfn = self.lib.mpm_sort_keys_cells if cells else self.lib.mpm_sort_keys
-        native.check(keyfn(self.pP, self.pG, native.ptr(keys), native.ptr(vals), native.ptr(flag), s),
+        b["flag"].zero_()
+        native.check(keyfn(self.pP, self.pG, native.ptr(b["keys"]), native.ptr(b["vals"]), native.ptr(b["flag"]), s),
                      "sort_keys")
-        if int(flag.item()) == 0:
-            self._chunks(keys, cells)
+        if int(b["flag"].item()) == 0:
+            self._chunks(b["keys"], cells)
             return False
-        tb = int(self.lib.mpm_sort_temp_bytes(n))
-        temp = torch.empty(max(tb, 1), dtype=torch.uint8, device=dev)
-        keys_o = torch.empty_like(keys)
-        perm = torch.empty_like(vals)
-        native.check(self.lib.mpm_sort_pairs(native.ptr(keys), native.ptr(keys_o), native.ptr(vals),
-                                             native.ptr(perm), n, native.ptr(temp), tb, s),
+        native.check(self.lib.mpm_sort_pairs(native.ptr(b["keys"]), native.ptr(b["keys_o"]), native.ptr(b["vals"]),
+                                             native.ptr(b["perm"]), n, native.ptr(b["temp"]), b["tb"], s),
                      "sort_pairs")
-        scratch = torch.empty(9 * n, dtype=torch.float32, device=dev)
-        native.check(self.lib.mpm_permute(self.pP, native.ptr(perm), native.ptr(scratch), s), "permute")
-        pl = perm.long()
-        self.density.copy_(self.density[pl])
+        native.check(self.lib.mpm_permute(self.pP, native.ptr(b["perm"]), native.ptr(b["scratch"]), s), "permute")
+        pl = b["perm"].long()
+        torch.index_select(self.density, 0, pl, out=b["density"])
+        self.density, b["density"] = b["density"], self.density
         if self.init_cov is not None:
             self.init_cov.copy_(self.init_cov[pl])
         self._last_perm = pl
-        self._chunks(keys_o, cells)
+        self._chunks(b["keys_o"], cells)
         return True
 
     def _chunks(self, sorted_keys, cells: bool):
         """Particle-pass chunks over the sorted particles (mpm_sort_chunks): runs of at most
         256 particles within 4 consecutive blocks of one block column, and runs of at most
-        512 for the paired pass. The two chunk counts are read back in one copy (the sort
-        already waits for its unsorted flag) to size the launches."""
+        512 for the paired pass. The launch sizes come from the counts of an earlier sort,
+        copied back without waiting (the pass sweeps its launch again for any chunks
+        beyond it); only the first sort waits for its counts."""
         torch = self.torch
         n = self.n_particles
         if self.chunk_start is None:
             self.chunk_start = torch.zeros(n + 1, dtype=torch.int32, device=self.device)
             self.chunk_start2 = torch.zeros(n + 1, dtype=torch.int32, device=self.device)
             self.chunk_count = torch.zeros(2, dtype=torch.int32, device=self.device)
-        tb = int(self.lib.mpm_sort_chunks_temp_bytes(n))
-        temp = torch.empty(max(tb, 1), dtype=torch.uint8, device=self.device)
+            self._chunk_temp_bytes = int(self.lib.mpm_sort_chunks_temp_bytes(n))
+            self._chunk_temp = torch.empty(max(self._chunk_temp_bytes, 1), dtype=torch.uint8, device=self.device)
+            self._count_host = torch.zeros(2, dtype=torch.int32).pin_memory()
+            self._count_ev = None
         nb = self.n_grid // 4 + 2
         for size, starts, cnt in ((256, self.chunk_start, native.ptr(self.chunk_count)),
                                   (512, self.chunk_start2, native.ptr(self.chunk_count) + 4)):
             native.check(self.lib.mpm_sort_chunks(native.ptr(sorted_keys), n, 6 if cells else 0, nb, 4, size,
-                                                  native.ptr(starts), cnt, native.ptr(temp), tb, self.stream()),
-                         "sort_chunks")
-        counts = self.chunk_count.tolist()
-        # launch capacities in steps of 64 chunks, kept while they fit (a new capacity
-        # means a new captured graph)
+                                                  native.ptr(starts), cnt, native.ptr(self._chunk_temp),
+                                                  self._chunk_temp_bytes, self.stream()), "sort_chunks")
+        if self._count_ev is None:  # first sort: wait for the counts
+            counts = self.chunk_count.tolist()
+        elif self._count_ev.query():  # the previous sort's counts have arrived
+            counts = self._count_host.tolist()
+        else:
+            counts = None
+        self._count_host.copy_(self.chunk_count, non_blocking=True)
+        self._count_ev = torch.cuda.Event()
+        self._count_ev.record()
+        if counts is None:
+            return
+        # launch capacities in steps of 64 chunks with 64 to spare, kept while the count
+        # stays within them (a new capacity means a new captured graph)
         caps = []
         for count, cap in zip(counts, (self.chunk_capacity, self.chunk_capacity2)):
             if count > cap or cap - count > 256:
-                cap = (count + 63) // 64 * 64
+                cap = (count + 127) // 64 * 64
             caps.append(cap)
         if caps != [self.chunk_capacity, self.chunk_capacity2]:
             self.chunk_capacity, self.chunk_capacity2 = caps
@@ -793,10 +814,29 @@ class HipSimulator:
             self._graph = None
         return self.cdf
 
+    def prepare_tick(self, dt: float, substeps: int):
+        """Build the coming tick's collider rows now: host work that, done before the
+        guard's readback, overlaps the GPU still running the previous tick. run_tick uses
+        them when it is called for the same start time, dt and substep count; otherwise (or
+        on drop_prepared) the collider poses the schedule advanced are restored."""
+        self.drop_prepared()
+        S = int(substeps)
+        if S <= 0:
+            return
+        cs = self.scene_colliders if self.scene_colliders is not None else self.colliders
+        snap = cs.poses()
+        self._prepared = (self.time, float(dt), S, self._schedule(dt, S), cs, snap)
+
+    def drop_prepared(self):
+        p, self._prepared = self._prepared, None
+        if p is not None:
+            p[4].set_poses(p[5])
+
     def run_tick(self, dt: float, substeps: int, fused: bool, sparse: bool):
         torch = self.torch
         S = int(substeps)
         if S <= 0:
+            self.drop_prepared()
             return
         self._upload_model()
         L = self.lib
@@ -822,7 +862,13 @@ class HipSimulator:
                 for j, (t0, t1) in enumerate(wins):
                     act[:, col] |= (((times >= t0) & (times < t1)).astype(np.uint32) << np.uint32(j))
             self.substep_active[:S].copy_(torch.from_numpy(act.view(np.int32)))
-        tab, t_end, K, n_slots = self._schedule(dt, S)
+        p, self._prepared = self._prepared, None
+        if p is not None and p[:3] == (self.time, float(dt), S):
+            tab, t_end, K, n_slots = p[3]
+        else:
+            if p is not None:
+                p[4].set_poses(p[5])
+            tab, t_end, K, n_slots = self._schedule(dt, S)
         self.n_slots = n_slots
         if 6 * n_slots > self.wrench.numel():  # grow, keeping the accumulated values
             w = torch.zeros(6 * n_slots, dtype=torch.float32, device=self.device)
@@ -1186,7 +1232,6 @@ class Solver:
     _sim: Any = field(default=None, init=False, repr=False)
     _step: int = field(default=0, init=False, repr=False)
     _tick: int = field(default=0, init=False, repr=False)
-    _vol0: Any = field(default=None, init=False, repr=False)
 
     def __post_init__(self) -> None:
         if self.inversion_policy not in {"warn", "raise", "nan"}:
@@ -1204,7 +1249,6 @@ class Solver:
         if np.asarray(vol).reshape(-1).shape[0] != len(pos):
             raise ValueError(f"load_particles: {np.asarray(vol).size} volumes for {len(pos)} particles")
         self.device = _resolve_device(self.device)
-        self._vol0 = np.asarray(vol, np.float32).copy()
         cov6 = None
         if cov is not None:
             if cov_mode not in ("step", "from_F"):
@@ -1525,12 +1569,18 @@ class Solver:
         """solver.py:429-452: guard + launch box, optional block sort, optional sparse
         rebuild, then one control tick of `substeps` substeps."""
         sim = self._sim
-        if self._tick % max(1, self.guard_interval) == 0:
-            self._update_grid_box(dt, substeps)
-        if self.sort_interval and self._tick % self.sort_interval == 0:
-            self._sort_particles()
-        if self.sparse:
-            sim.rebuild_active_blocks()
+        # collider rows first: the host builds them while the GPU finishes the last tick
+        sim.prepare_tick(dt, substeps)
+        try:
+            if self._tick % max(1, self.guard_interval) == 0:
+                self._update_grid_box(dt, substeps)
+            if self.sort_interval and self._tick % self.sort_interval == 0:
+                self._sort_particles()
+            if self.sparse:
+                sim.rebuild_active_blocks()
+        except BaseException:
+            sim.drop_prepared()
+            raise
         sim.profile = self.profile
         self._tick += 1
         sim.run_tick(dt, substeps, fused=self.fused, sparse=self.sparse)
@@ -1538,10 +1588,7 @@ class Solver:
         return self
 
     def _sort_particles(self) -> bool:
-        moved = self._sim.sort_by_block(cells=self.sort_key == "cell")
-        if moved:
-            self._vol0 = self._vol0[self._sim._last_perm.cpu().numpy()]
-        return moved
+        return self._sim.sort_by_block(cells=self.sort_key == "cell")
 
     def _update_grid_box(self, dt: float, substeps: int) -> None:
         """solver.py:485-518 on device-reduced bounds."""
@@ -1553,8 +1600,6 @@ class Solver:
         if flag & native.ERR_FIXED_RANGE:
             raise RuntimeError("deterministic scatter: a contribution exceeded the fixed-point range "
                                "(the state is far outside the expected momentum scale)")
-        if flag & native.ERR_CHUNKS:
-            raise RuntimeError("internal: the particle pass launched fewer chunks than the sort produced")
         if flag & native.ERR_POOL:
             raise RuntimeError("a particle scattered into a block outside the active set during the last "
                                "tick (it moved more than a block within the tick)")
@@ -1641,6 +1686,13 @@ class Solver:
         J = J.copy()
         J[bad] = np.nan
         return J
+
+    @property
+    def _vol0(self) -> np.ndarray:
+        """Rest volumes in the current particle order (the reference's host copy, which it
+        permutes at every sort, solver.py:466): read from the simulator's volume array,
+        which the sort permutes on the device."""
+        return self._sim.vol.cpu().numpy()
 
     def vol(self) -> np.ndarray:
         return self._vol0 * self._checked_jacobian()
