@@ -206,7 +206,10 @@ class HipSimulator:
         self.deterministic = False
         self.node_fix = None
         self._prepared = None  # prepare_tick's collider rows for the coming tick
+        self.chunked = True  # group-aligned chunks at every sort (False: fixed runs of 256)
         self._sort_buf = None  # sort_by_block's work buffers
+        self._sort_graph = None  # its captured permutation (key: buffers and flags)
+        self._sort_graph_key = None
         # particle-pass chunks (mpm_sort_chunks), computed at every sort; None until then
         self.chunk_start = None
         self.chunk_count = None  # device int32 [2]: chunks of <= 256, of <= 512 particles
@@ -714,7 +717,9 @@ class HipSimulator:
         """Stable block-key sort on device (solver.py:454-467, mpm_solver_warp.py:
         1548-1619); returns False (no permutation) when already ordered, read back from a
         device flag right after the key kernel. With `cells` the key is refined by the
-        stencil base cell inside the block. The work buffers are allocated once."""
+        stencil base cell inside the block. The work buffers are allocated once, and the
+        work after the flag (radix sort, permutation, chunk boundaries: ~80 launches) is
+        replayed as one captured hipGraph."""
         torch = self.torch
         n = self.n_particles
         if n < 2:
@@ -726,37 +731,59 @@ class HipSimulator:
             self._sort_buf = {"keys": torch.empty(n, **i32), "vals": torch.empty(n, **i32),
                               "flag": torch.zeros(1, **i32), "keys_o": torch.empty(n, **i32),
                               "perm": torch.empty(n, **i32), "tb": tb,
+                              "perm_l": torch.empty(n, dtype=torch.int64, device=dev),
                               "temp": torch.empty(max(tb, 1), dtype=torch.uint8, device=dev),
                               "scratch": torch.empty(9 * n, dtype=torch.float32, device=dev),
                               "density": torch.empty_like(self.density)}
         b = self._sort_buf
+        if self.chunked:
+            self._chunk_buffers()
         s = self.stream()
         keyfn = self.lib.mpm_sort_keys_cells if cells else self.lib.mpm_sort_keys
         b["flag"].zero_()
         native.check(keyfn(self.pP, self.pG, native.ptr(b["keys"]), native.ptr(b["vals"]), native.ptr(b["flag"]), s),
                      "sort_keys")
         if int(b["flag"].item()) == 0:
-            self._chunks(b["keys"], cells)
+            if self.chunked:
+                self._chunk_kernels(b["keys"], cells, s)
+                self._chunk_counts()
             return False
+        if self.use_graph:
+            key = (bool(cells), n, self.chunked, C.string_at(C.addressof(self.P), C.sizeof(self.P)),
+                   native.ptr(self.density), native.ptr(self.init_cov))
+            if self._sort_graph is None or self._sort_graph_key != key:
+                g = torch.cuda.CUDAGraph()
+                torch.cuda.synchronize(dev)
+                with torch.cuda.graph(g):
+                    self._permute_sorted(cells, self.stream())
+                self._sort_graph, self._sort_graph_key = g, key
+            self._sort_graph.replay()
+        else:
+            self._permute_sorted(cells, s)
+        self._last_perm = b["perm_l"]  # valid until the next sort
+        if self.chunked:
+            self._chunk_counts()
+        return True
+
+    def _permute_sorted(self, cells: bool, s):
+        """Radix sort of the keys, every particle array permuted in place, the host-side
+        per-particle arrays alongside, then the chunk boundaries of the sorted order."""
+        b = self._sort_buf
+        n = self.n_particles
         native.check(self.lib.mpm_sort_pairs(native.ptr(b["keys"]), native.ptr(b["keys_o"]), native.ptr(b["vals"]),
                                              native.ptr(b["perm"]), n, native.ptr(b["temp"]), b["tb"], s),
                      "sort_pairs")
         native.check(self.lib.mpm_permute(self.pP, native.ptr(b["perm"]), native.ptr(b["scratch"]), s), "permute")
-        pl = b["perm"].long()
-        torch.index_select(self.density, 0, pl, out=b["density"])
-        self.density, b["density"] = b["density"], self.density
+        pl = b["perm_l"]
+        pl.copy_(b["perm"])
+        self.torch.index_select(self.density, 0, pl, out=b["density"])
+        self.density.copy_(b["density"])
         if self.init_cov is not None:
             self.init_cov.copy_(self.init_cov[pl])
-        self._last_perm = pl
-        self._chunks(b["keys_o"], cells)
-        return True
+        if self.chunked:
+            self._chunk_kernels(b["keys_o"], cells, s)
 
-    def _chunks(self, sorted_keys, cells: bool):
-        """Particle-pass chunks over the sorted particles (mpm_sort_chunks): runs of at most
-        256 particles within 4 consecutive blocks of one block column, and runs of at most
-        512 for the paired pass. The launch sizes come from the counts of an earlier sort,
-        copied back without waiting (the pass sweeps its launch again for any chunks
-        beyond it); only the first sort waits for its counts."""
+    def _chunk_buffers(self):
         torch = self.torch
         n = self.n_particles
         if self.chunk_start is None:
@@ -767,12 +794,24 @@ class HipSimulator:
             self._chunk_temp = torch.empty(max(self._chunk_temp_bytes, 1), dtype=torch.uint8, device=self.device)
             self._count_host = torch.zeros(2, dtype=torch.int32).pin_memory()
             self._count_ev = None
+
+    def _chunk_kernels(self, sorted_keys, cells: bool, s):
+        """Particle-pass chunks over the sorted particles (mpm_sort_chunks): runs of at most
+        256 particles within 4 consecutive blocks of one block column, and runs of at most
+        512 for the paired pass."""
         nb = self.n_grid // 4 + 2
+        n = self.n_particles
         for size, starts, cnt in ((256, self.chunk_start, native.ptr(self.chunk_count)),
                                   (512, self.chunk_start2, native.ptr(self.chunk_count) + 4)):
             native.check(self.lib.mpm_sort_chunks(native.ptr(sorted_keys), n, 6 if cells else 0, nb, 4, size,
                                                   native.ptr(starts), cnt, native.ptr(self._chunk_temp),
-                                                  self._chunk_temp_bytes, self.stream()), "sort_chunks")
+                                                  self._chunk_temp_bytes, s), "sort_chunks")
+
+    def _chunk_counts(self):
+        """Launch sizes from the chunk counts of an earlier sort, copied back without
+        waiting (the pass sweeps its launch again for any chunks beyond it); only the
+        first sort waits for its counts."""
+        torch = self.torch
         if self._count_ev is None:  # first sort: wait for the counts
             counts = self.chunk_count.tolist()
         elif self._count_ev.query():  # the previous sort's counts have arrived
