@@ -333,12 +333,15 @@ MPM_DEV bool stencil_in_grid(const mpm_grid_t& G, const Stencil& st, bool period
     return ok;
 }
 
-// Workgroups are dispatched round-robin over the 8 XCDs (each with its own L2). Group g
-// of `ppt` consecutive chunks of block-sorted particles is handed to workgroup b with
-// g = (b % 8) * q + b / 8, so every XCD sweeps a contiguous range of chunks: spatial
-// neighbours, whose scatter flushes overlap on the same grid nodes, share one L2 instead
-// of bouncing lines between L2s. The workgroup processes its ppt chunks one after the
-// other (one particle per thread per chunk).
+// Workgroups are dispatched round-robin over the 8 XCDs (each with its own L2). With G
+// groups of `ppt` consecutive chunks of block-sorted particles (G from the device's chunk
+// count) and q = ceil(G / 8), group g = (b % 8) * q + y is handed to workgroup b for
+// y = b / 8, b / 8 + Q, ... < q (Q = launch / 8), so every XCD sweeps one contiguous eighth
+// of the chunks: spatial neighbours, whose scatter flushes overlap on the same grid
+// nodes, share one L2 instead of bouncing lines between L2s, and the XCDs stay balanced
+// whatever the launch size (a launch larger than the count leaves workgroups that exit at
+// once, a smaller one takes further sweeps). A workgroup processes the ppt chunks of a
+// group one after the other (one particle per thread per chunk).
 constexpr int kXcds = 8;
 
 // amdgpu_waves_per_eu(3): keep the allocation within 168 VGPRs (3 wavefronts per SIMD; left
@@ -351,7 +354,7 @@ constexpr int kXcds = 8;
 template <int MODE, bool kSolid, int kT, bool kCdf, bool kDet>
 __global__ __launch_bounds__(kT) __attribute__((amdgpu_waves_per_eu(MPM_PASS_WAVES_PER_EU))) void k_particle_pass(mpm_particles_t P, mpm_grid_t G,
                                                       const mpm_model_t* __restrict__ md, float dt,
-                                                      int xcd_q, int ppt) {
+                                                      int ppt) {
     constexpr int kArena = arena_of<kT>();
     constexpr bool kG2P = (MODE & MPM_PASS_G2P) != 0;
     constexpr bool kStress = (MODE & MPM_PASS_STRESS) != 0;
@@ -367,18 +370,19 @@ __global__ __launch_bounds__(kT) __attribute__((amdgpu_waves_per_eu(MPM_PASS_WAV
     __shared__ float s_react[kCdf && kP2G ? 6 * MPM_MAX_CDF : 1];  // blocked deposits per lane
 
     const int S = P.stride;
-    const int group = (int)(blockIdx.x % kXcds) * xcd_q + (int)(blockIdx.x / kXcds);
     // chunks: group-aligned runs of sorted particles (chunk_start), or kT consecutive ones
     const bool chunked = P.chunk_start != nullptr;
     const int nchunks = chunked ? *P.chunk_count : (P.n + kT - 1) / kT;
-    // Chunks beyond one sweep of the launch (more than chunk_capacity of them: the host
-    // sizes the launch from an earlier count without waiting for this one) are taken by
-    // further sweeps, so the capacity sizes the launch and never bounds the work.
-    const int sweep = (int)gridDim.x * ppt;
+    const int xcd = (int)(blockIdx.x % kXcds), Q = (int)(gridDim.x / kXcds);
+    const int qe = ((nchunks + ppt - 1) / ppt + kXcds - 1) / kXcds;  // groups per XCD
+    bool first = true;
     for (int it = 0;; ++it) {
-        const int chunk = (it / ppt) * sweep + group * ppt + it % ppt;
-        if (chunk >= nchunks) return;  // past the last chunk (uniform exit before any barrier)
-        if (it > 0) __syncthreads();    // the previous chunk's LDS reads are done
+        const int y = (int)(blockIdx.x / kXcds) + (it / ppt) * Q;
+        if (y >= qe) return;  // this workgroup's share is done (uniform, before any barrier)
+        const int chunk = (xcd * qe + y) * ppt + it % ppt;
+        if (chunk >= nchunks) continue;  // the last XCD's short tail (uniform)
+        if (!first) __syncthreads();     // the previous chunk's LDS reads are done
+        first = false;
         const int p0 = chunked ? P.chunk_start[chunk] : chunk * kT;
         const int p1 = chunked ? P.chunk_start[chunk + 1] : min(p0 + kT, P.n);
         const int p = p0 + threadIdx.x;
@@ -738,7 +742,7 @@ __global__ __launch_bounds__(kT) __attribute__((amdgpu_waves_per_eu(MPM_PASS_WAV
 constexpr int kPairStash = 25;  // x 3, v 3, C 9, fs 9, mass 1
 template <bool kSolid, bool kKeepStress>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MPM_PASS_WAVES_PER_EU))) void k_particle_pass_pair(
-    mpm_particles_t P, mpm_grid_t G, const mpm_model_t* __restrict__ md, float dt, int xcd_q) {
+    mpm_particles_t P, mpm_grid_t G, const mpm_model_t* __restrict__ md, float dt) {
     constexpr int kT = 256;
     constexpr int kArena = arena_of<kT>();
     // the gather tile (kArena float4) lives in the first half of the scatter arena
@@ -749,11 +753,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MPM_PASS_WA
     float4* s_gather = reinterpret_cast<float4*>(s_arena);
 
     const int S = P.stride;
-    const int group = (int)(blockIdx.x % kXcds) * xcd_q + (int)(blockIdx.x / kXcds);
     const int nchunks = *P.chunk_count;
-    // further sweeps take the chunks beyond the launch (see k_particle_pass)
-    for (int chunk = group; chunk < nchunks; chunk += (int)gridDim.x) {
-    if (chunk != group) __syncthreads();  // the previous chunk's LDS reads are done
+    // XCD-contiguous shares of the chunks, swept as in k_particle_pass
+    const int xcd = (int)(blockIdx.x % kXcds), Q = (int)(gridDim.x / kXcds);
+    const int qe = (nchunks + kXcds - 1) / kXcds;
+    for (int y = (int)(blockIdx.x / kXcds); y < qe; y += Q) {
+    const int chunk = xcd * qe + y;
+    if (chunk >= nchunks) break;  // the last XCD's short tail (uniform)
+    if (y != (int)(blockIdx.x / kXcds)) __syncthreads();  // the previous chunk's LDS reads are done
     const int p0 = P.chunk_start[chunk], p1 = P.chunk_start[chunk + 1];
     const bool periodic = md->periodic_x != 0;
     int err = 0;
@@ -1051,13 +1058,13 @@ extern "C" int32_t mpm_particle_pass(const mpm_particles_t* P, const mpm_grid_t*
         const bool solid = (mode & MPM_PASS_VISCOUS_ONLY) == 0, keep = (mm & MPM_PASS_KEEP_STRESS) != 0;
         hipStream_t s = (hipStream_t)stream;
         if (solid && keep)
-            hipLaunchKernelGGL((k_particle_pass_pair<true, true>), grid, block, 0, s, *P, *G, model, dt, q);
+            hipLaunchKernelGGL((k_particle_pass_pair<true, true>), grid, block, 0, s, *P, *G, model, dt);
         else if (solid)
-            hipLaunchKernelGGL((k_particle_pass_pair<true, false>), grid, block, 0, s, *P, *G, model, dt, q);
+            hipLaunchKernelGGL((k_particle_pass_pair<true, false>), grid, block, 0, s, *P, *G, model, dt);
         else if (keep)
-            hipLaunchKernelGGL((k_particle_pass_pair<false, true>), grid, block, 0, s, *P, *G, model, dt, q);
+            hipLaunchKernelGGL((k_particle_pass_pair<false, true>), grid, block, 0, s, *P, *G, model, dt);
         else
-            hipLaunchKernelGGL((k_particle_pass_pair<false, false>), grid, block, 0, s, *P, *G, model, dt, q);
+            hipLaunchKernelGGL((k_particle_pass_pair<false, false>), grid, block, 0, s, *P, *G, model, dt);
         return launch_status();
     }
     const int wg = (!cdf && !det && (mode & MPM_PASS_WG128)) ? 128 : 256;
@@ -1077,9 +1084,9 @@ extern "C" int32_t mpm_particle_pass(const mpm_particles_t* P, const mpm_grid_t*
     const int m = mode & (MPM_PASS_G2P | MPM_PASS_STRESS | MPM_PASS_P2G | MPM_PASS_LEAN | MPM_PASS_KEEP_STRESS);
 #define LAUNCH(M, T, C, D)                                                                                     \
     if (solid)                                                                                                 \
-        hipLaunchKernelGGL((k_particle_pass<(M), true, T, C, D>), grid, block, 0, s, Pl, *G, model, dt, q, ppt); \
+        hipLaunchKernelGGL((k_particle_pass<(M), true, T, C, D>), grid, block, 0, s, Pl, *G, model, dt, ppt); \
     else                                                                                                       \
-        hipLaunchKernelGGL((k_particle_pass<(M), false, T, C, D>), grid, block, 0, s, Pl, *G, model, dt, q, ppt);
+        hipLaunchKernelGGL((k_particle_pass<(M), false, T, C, D>), grid, block, 0, s, Pl, *G, model, dt, ppt);
 #define PLAIN(M)                         \
     if (wg == 128) {                     \
         LAUNCH(M, 128, false, false)     \

@@ -91,6 +91,7 @@ class Rollout:
         self.perm = sim.torch.arange(n0, device=sim.device)
         if solver._sort_particles():
             self.perm = sim._last_perm.clone()
+        sim.sync_chunk_capacity()  # the launches below keep this order for the whole rollout
         self.dt = float(dt)
         self.substeps = int(substeps)
         self.every = int(checkpoint_every)

@@ -783,6 +783,12 @@ class HipSimulator:
         if self.chunked:
             self._chunk_kernels(b["keys_o"], cells, s)
 
+    def sync_chunk_capacity(self):
+        """Size the particle-pass launches from the current chunk counts (waits for them):
+        for callers that sort once and then run many substeps on that order."""
+        if self.chunk_start is not None:
+            self._chunk_counts(wait=True)
+
     def _chunk_buffers(self):
         torch = self.torch
         n = self.n_particles
@@ -807,12 +813,14 @@ class HipSimulator:
                                                   native.ptr(starts), cnt, native.ptr(self._chunk_temp),
                                                   self._chunk_temp_bytes, s), "sort_chunks")
 
-    def _chunk_counts(self):
+    def _chunk_counts(self, wait: bool = False):
         """Launch sizes from the chunk counts of an earlier sort, copied back without
-        waiting (the pass sweeps its launch again for any chunks beyond it); only the
-        first sort waits for its counts."""
+        waiting (the pass sweeps its launch again for any chunks beyond it, which costs a
+        whole second chain of the pass, so the launch keeps a quarter of the count plus 64
+        chunks to spare; surplus workgroups exit at once). The first sort, and `wait`,
+        wait for the current counts."""
         torch = self.torch
-        if self._count_ev is None:  # first sort: wait for the counts
+        if self._count_ev is None or wait:  # wait for the counts
             counts = self.chunk_count.tolist()
         elif self._count_ev.query():  # the previous sort's counts have arrived
             counts = self._count_host.tolist()
@@ -823,12 +831,14 @@ class HipSimulator:
         self._count_ev.record()
         if counts is None:
             return
-        # launch capacities in steps of 64 chunks with 64 to spare, kept while the count
-        # stays within them (a new capacity means a new captured graph)
+        # launch capacities in steps of 64 chunks, kept while the count stays within them
+        # and they hold no more than twice the spare (a new capacity means a new captured
+        # graph)
         caps = []
         for count, cap in zip(counts, (self.chunk_capacity, self.chunk_capacity2)):
-            if count > cap or cap - count > 256:
-                cap = (count + 127) // 64 * 64
+            spare = count // 4 + 64
+            if count > cap or cap - count > 2 * spare + 64:
+                cap = (count + spare + 63) // 64 * 64
             caps.append(cap)
         if caps != [self.chunk_capacity, self.chunk_capacity2]:
             self.chunk_capacity, self.chunk_capacity2 = caps
