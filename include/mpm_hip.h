@@ -36,7 +36,7 @@ extern "C" {
 #endif
 
 #define MPM_MAX_MATERIALS 16
-#define MPM_ABI_VERSION 9
+#define MPM_ABI_VERSION 10
 
 typedef enum mpm_status {
     MPM_OK = 0,
@@ -450,6 +450,14 @@ int32_t mpm_blocks_compact(const mpm_grid_t *G, const int32_t *cur, const int32_
 /* out (device, 8 floats): min x, y, z, max x, y, z, max |v| component, non-finite count.
  * Works on all particles, like the reference's host readback. */
 int32_t mpm_bounds(const mpm_particles_t *P, float *out, void *stream);
+/* Launch box on the device from mpm_bounds' output (device): the particle box widened by
+ * pad + travel * (max |v| component) metres, as the node box [floor(lo / dx),
+ * ceil(hi / dx) + 1) clamped to the grid, written to ctl->box_lo / box_hi (the whole grid
+ * when a position is not finite). No counterpart in the reference, whose host computes
+ * the box from a readback (solver.py:485-518): for callers that keep the tick on the
+ * device (the differentiable rollout's per-segment boxes). */
+int32_t mpm_box_from_bounds(const float *bounds, const mpm_grid_t *G, float pad, float travel,
+                            mpm_tick_ctl_t *ctl, void *stream);
 
 /* ---- block sort (solver.py:454-467, mpm_solver_warp.py:1548-1619) ------------------- */
 /* keys[p] = scene*nb^3 + ((bx*nb + by)*nb + bz) with b = floor(x/dx - 0.5) >> 2,
@@ -582,10 +590,13 @@ typedef struct mpm_adjoint {
  * of x_k (partial) into A->x and of F_k into A->F. */
 int32_t mpm_adjoint_g2p(const mpm_particles_t *P, const mpm_grid_t *G, const mpm_adjoint_t *A,
                         const mpm_model_t *model, float dt, void *stream);
-/* Grid adjoint over every node: the colliders (rows of this substep, ncoll <= 16) and
- * the normalisation transposed; impulse_adj (float[6 * wrench_slots]) is the adjoint of
- * the accumulated wrench. */
-int32_t mpm_adjoint_grid(const mpm_grid_t *G, const mpm_model_t *model, const mpm_collider_t *colliders,
+/* Grid adjoint over the listed blocks (block_list / *block_count as mpm_blocks_from_box
+ * writes them, at most `capacity`; NULL: every block): the colliders (rows of this
+ * substep, ncoll <= 16) and the normalisation transposed; impulse_adj (float[6 *
+ * wrench_slots]) is the adjoint of the accumulated wrench. Nodes outside the listed
+ * blocks must hold no mass and no velocity adjoint. */
+int32_t mpm_adjoint_grid(const mpm_grid_t *G, const mpm_model_t *model, const int32_t *block_list,
+                         const int32_t *block_count, int32_t capacity, const mpm_collider_t *colliders,
                          int32_t ncoll, const float *impulse_adj, int32_t wrench_slots,
                          const mpm_adjoint_t *A, float dt, void *stream);
 /* P2G adjoint (gather) fused with the return-map adjoint: completes the adjoint of x_k,

@@ -364,11 +364,15 @@ template <int kMaxC>
 __global__ __launch_bounds__(kThreads) void k_adj_grid(mpm_grid_t G, const mpm_model_t* __restrict__ md,
                                                        const mpm_collider_t* __restrict__ col, int ncoll,
                                                        const float* __restrict__ impb, int nslots, mpm_adjoint_t A,
-                                                       float dt) {
-    const int nblk = G.nb[0] * G.nb[1] * G.nb[2];
+                                                       const int32_t* __restrict__ list,
+                                                       const int32_t* __restrict__ count, float dt) {
+    const int nall = G.nb[0] * G.nb[1] * G.nb[2];
+    const int nblk = list ? *count : nall;
     const int lane = threadIdx.x & 63;
     int err = 0;
-    for (int b = blockIdx.x * (kThreads / 64) + (threadIdx.x >> 6); b < nblk; b += gridDim.x * (kThreads / 64)) {
+    for (int e = blockIdx.x * (kThreads / 64) + (threadIdx.x >> 6); e < nblk; e += gridDim.x * (kThreads / 64)) {
+        const int b = list ? list[e] : e;
+        if ((unsigned)b >= (unsigned)nall) continue;
         int bi, bj, bk;
         decode_block(G, b, bi, bj, bk);
         const int gi = bi * 4 + (lane >> 4), gj = bj * 4 + ((lane >> 2) & 3), gk = bk * 4 + (lane & 3);
@@ -514,21 +518,22 @@ int32_t mpm_adjoint_g2p(const mpm_particles_t* P, const mpm_grid_t* G, const mpm
     return launch_status();
 }
 
-int32_t mpm_adjoint_grid(const mpm_grid_t* G, const mpm_model_t* model, const mpm_collider_t* colliders,
+int32_t mpm_adjoint_grid(const mpm_grid_t* G, const mpm_model_t* model, const int32_t* block_list,
+                         const int32_t* block_count, int32_t capacity, const mpm_collider_t* colliders,
                          int32_t ncoll, const float* impulse_adj, int32_t wrench_slots, const mpm_adjoint_t* A,
                          float dt, void* stream) {
     if (!grid_ok1(G) || !model || !adj_ok(A) || ncoll < 0 || (ncoll > 0 && !colliders) || wrench_slots < 0 ||
-        (wrench_slots > 0 && !impulse_adj))
+        (wrench_slots > 0 && !impulse_adj) || (block_list && (!block_count || capacity < 1)))
         return MPM_ERR_ARG;
     if (ncoll > kMaxAdjColliders) return MPM_ERR_SIZE;
-    const long nblk = (long)G->nb[0] * G->nb[1] * G->nb[2];
+    const long nblk = block_list ? (long)capacity : (long)G->nb[0] * G->nb[1] * G->nb[2];
     const unsigned nwg = (unsigned)min((nblk + 3) / 4, 2048L);
     if (ncoll <= 4)
         hipLaunchKernelGGL(k_adj_grid<4>, dim3(nwg), dim3(kThreads), 0, (hipStream_t)stream, *G, model, colliders,
-                           ncoll, impulse_adj, wrench_slots, *A, dt);
+                           ncoll, impulse_adj, wrench_slots, *A, block_list, block_count, dt);
     else
         hipLaunchKernelGGL(k_adj_grid<kMaxAdjColliders>, dim3(nwg), dim3(kThreads), 0, (hipStream_t)stream, *G,
-                           model, colliders, ncoll, impulse_adj, wrench_slots, *A, dt);
+                           model, colliders, ncoll, impulse_adj, wrench_slots, *A, block_list, block_count, dt);
     return launch_status();
 }
 

@@ -412,6 +412,24 @@ __global__ void k_blocks_from_box(mpm_grid_t G, mpm_tick_ctl_t* ctl, int32_t* li
     list[t] = scene * (G.nb[0] * G.nb[1] * G.nb[2]) + (bi * G.nb[1] + bj) * G.nb[2] + bk;
 }
 
+// Launch box on the device from mpm_bounds' output: the particle box widened by
+// pad + travel * max |v|, as nodes [floor(lo / dx), ceil(hi / dx) + 1) clamped to the grid
+// (the host's grid_box with no halo); the whole grid when a position is not finite.
+__global__ void k_box_from_bounds(const float* __restrict__ b, mpm_grid_t G, float pad, float travel,
+                                  mpm_tick_ctl_t* ctl) {
+    if (threadIdx.x != 0) return;
+    const float w = pad + travel * b[6];
+    const bool ok = b[7] == 0.0f && w == w;
+    for (int a = 0; a < 3; ++a) {
+        int lo = ok ? (int)floorf((b[a] - w) / G.dx) : 0;
+        int hi = ok ? (int)ceilf((b[3 + a] + w) / G.dx) + 1 : G.n[a];
+        lo = max(0, min(lo, G.n[a]));
+        hi = max(lo, min(hi, G.n[a]));
+        ctl->box_lo[a] = lo;
+        ctl->box_hi[a] = hi;
+    }
+}
+
 // ------------------------------------------------------------------ sparse blocks
 // blocks_mark, mpm_utils.py:1506-1513 (int() truncation toward zero, >> 2, clamp)
 __global__ void k_blocks_mark(mpm_particles_t P, mpm_grid_t G, int32_t* raw) {
@@ -723,6 +741,13 @@ int32_t mpm_blocks_from_box(const mpm_grid_t* G, mpm_tick_ctl_t* ctl, int32_t* l
     long total = (long)G->nb[0] * G->nb[1] * G->nb[2] * G->n_scenes;
     hipLaunchKernelGGL(k_blocks_from_box, dim3(blocks_for(total)), dim3(kThreads), 0, (hipStream_t)stream, *G, ctl,
                        list);
+    return launch_status();
+}
+
+int32_t mpm_box_from_bounds(const float* bounds, const mpm_grid_t* G, float pad, float travel, mpm_tick_ctl_t* ctl,
+                            void* stream) {
+    if (!bounds || !dense_grid_ok(G) || !ctl || !(pad >= 0.0f) || !(travel >= 0.0f)) return MPM_ERR_ARG;
+    hipLaunchKernelGGL(k_box_from_bounds, dim3(1), dim3(64), 0, (hipStream_t)stream, bounds, *G, pad, travel, ctl);
     return launch_status();
 }
 

@@ -119,19 +119,21 @@ class Rollout:
         self.n_slots = sim.colliders.n_slots
         self.sched = torch.from_numpy(tab.view(np.uint8).reshape(-1).copy()).to(dev)
         self.row_bytes = self.K * native.COLLIDER_DTYPE.itemsize
-        # whole-grid launch box (single scene)
+        # launch box of the grid kernels, per checkpoint segment: the particle box at the
+        # segment's start widened by 3 cells plus 1.5x the segment's travel at the largest
+        # speed (the guard of Solver.step), computed on the device (mpm_bounds ->
+        # mpm_box_from_bounds -> mpm_blocks_from_box) and kept for the backward
         nb = sim.n_blocks
         self.ctl = torch.zeros(native.TICK_CTL_DTYPE.itemsize // 4, dtype=torch.int32, device=dev)
-        ctl = np.zeros(1, dtype=native.TICK_CTL_DTYPE)
-        ctl["box_lo"], ctl["box_hi"] = (0, 0, 0), (sim.n_grid,) * 3
-        self.ctl.copy_(torch.from_numpy(ctl.view(np.int32).reshape(-1).copy()))
-        self.blocks = torch.arange(nb, dtype=torch.int32, device=dev)
-        self.nblocks = torch.tensor([nb], dtype=torch.int32, device=dev)
+        self.blocks = torch.zeros(nb, dtype=torch.int32, device=dev)  # the box's block list
+        self.bnd = torch.zeros(9, dtype=torch.float32, device=dev)
+        self.box_pad = 3.0 * float(sim.dx)
         self.ws = torch.zeros(max(int(sim.lib.mpm_grid_workspace_floats(nb, self.n_slots)), 1),
                               dtype=torch.float32, device=dev)
         self.wrench = torch.zeros(max(6 * self.n_slots, 6), dtype=torch.float32, device=dev)
         nseg = (self.substeps + self.every - 1) // self.every
         self.ckpt = torch.empty(nseg, _CKPT_PLANES, n, dtype=torch.float32, device=dev)
+        self.seg_ctl = torch.zeros(nseg, self.ctl.numel(), dtype=torch.int32, device=dev)
         # one segment of recomputed substeps: particle slots and grids
         L = min(self.every, self.substeps)
         self.slots = torch.zeros(L + 1, _SLOT_PLANES, n, dtype=torch.float32, device=dev)
@@ -232,7 +234,7 @@ class Rollout:
         sched = native.ptr(self.sched) + row * self.row_bytes
         native.check(sim.lib.mpm_grid_update(
             C.byref(G), native.ptr(sim.model_dev), native.ptr(self.ctl), native.ptr(self.blocks),
-            native.ptr(self.nblocks), sim.n_blocks, sched if self.K else None, self.K,
+            native.ptr(self.ctl) + 24, sim.n_blocks, sched if self.K else None, self.K,
             native.ptr(wrench) if slots else None, slots, native.ptr(self.ws) if slots else None,
             dt, flags, stream), "grid_update")
 
@@ -276,6 +278,7 @@ class Rollout:
             if self.viscous:
                 ck[_CK_L0:_CK_L0 + 9].copy_(sim.L)
             native.check(lib.mpm_grid_zero_box(pG, zl, zh, 3, s), "zero_box")
+            self._segment_box(seg, S, s)
             G = sim.G
             sim._pass(ST | P2G, dt, s)
             self._grid_update(G, k0, dt, s, self.wrench, self.n_slots, native.GRID_ZERO_IN)
@@ -292,15 +295,31 @@ class Rollout:
         wrench = self.wrench[:6 * self.n_slots].clone().view(-1, 6)
         return self._to_user(sim.x.t()), wrench
 
+    def _segment_box(self, seg, S, s):
+        """The segment's launch box from the current particle state, on the device; kept
+        for the segment's recompute in the backward."""
+        sim, lib = self.sim, self.sim.lib
+        native.check(lib.mpm_bounds(sim.pP, native.ptr(self.bnd), s), "bounds")
+        native.check(lib.mpm_box_from_bounds(native.ptr(self.bnd), sim.pG, self.box_pad, 1.5 * self.dt * S,
+                                             native.ptr(self.ctl), s), "box_from_bounds")
+        native.check(lib.mpm_blocks_from_box(sim.pG, native.ptr(self.ctl), native.ptr(self.blocks), s),
+                     "blocks_from_box")
+        self.seg_ctl[seg].copy_(self.ctl)
+
     # ------------------------------------------------------------------ backward
     def _load_segment(self, seg, k0, S):
-        """Checkpoint of segment `seg` into slot 0 and its collider rows into seg_sched."""
+        """Checkpoint of segment `seg` into slot 0, its collider rows into seg_sched and its
+        launch box (and block list) into ctl."""
         self.slots[0, :_CK_L0].copy_(self.ckpt[seg, :_CK_L0])
         if self.viscous:
             self.slots[0, _L0:_L0 + 9].copy_(self.ckpt[seg, _CK_L0:_CK_L0 + 9])
         if self.K:
             nb = S * self.row_bytes
             self.seg_sched[:nb].copy_(self.sched[k0 * self.row_bytes:k0 * self.row_bytes + nb])
+        self.ctl.copy_(self.seg_ctl[seg])
+        sim = self.sim
+        native.check(sim.lib.mpm_blocks_from_box(sim.pG, native.ptr(self.ctl), native.ptr(self.blocks),
+                                                 sim.stream()), "blocks_from_box")
 
     def _segment_body(self, S):
         """Recompute S substeps from slot 0 with the split passes, keeping every state and
@@ -320,7 +339,7 @@ class Rollout:
             native.check(lib.mpm_particle_pass(C.byref(self._P_p2g[k]), G, md, dt,
                                                native.PASS_STRESS | native.PASS_P2G, s), "recompute p2g")
             native.check(lib.mpm_grid_update(
-                G, md, native.ptr(self.ctl), native.ptr(self.blocks), native.ptr(self.nblocks),
+                G, md, native.ptr(self.ctl), native.ptr(self.blocks), native.ptr(self.ctl) + 24,
                 sim.n_blocks, sched0 + k * self.row_bytes if self.K else None, self.K, None, 0, None,
                 dt, 0, s), "recompute grid")
             native.check(lib.mpm_particle_pass(C.byref(self._P_g2p[k]), G, md, dt, native.PASS_G2P, s),
@@ -330,7 +349,8 @@ class Rollout:
         for k in reversed(range(S)):
             P, G = C.byref(self._P_adj[k]), C.byref(self._G[k])
             native.check(lib.mpm_adjoint_g2p(P, G, pA, md, dt, s), "adjoint g2p")
-            native.check(lib.mpm_adjoint_grid(G, md, sched0 + k * self.row_bytes if self.K else None, self.K,
+            native.check(lib.mpm_adjoint_grid(G, md, native.ptr(self.blocks), native.ptr(self.ctl) + 24,
+                                              sim.n_blocks, sched0 + k * self.row_bytes if self.K else None, self.K,
                                               impb, self.n_slots, pA, dt, s), "adjoint grid")
             native.check(lib.mpm_adjoint_p2g_stress(C.byref(self._P_st[k]), G, pA, md, dt, s),
                          "adjoint p2g+stress")

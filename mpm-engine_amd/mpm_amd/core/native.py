@@ -138,6 +138,7 @@ SIGNATURES = {
     "mpm_blocks_dilate": (_I32, [_P, _P, _P, _P]),
     "mpm_blocks_compact": (_I32, [_P, _P, _P, _P, _P, _P, _P]),
     "mpm_bounds": (_I32, [_P, _P, _P]),
+    "mpm_box_from_bounds": (_I32, [_P, _P, _F, _F, _P, _P]),
     "mpm_sort_keys": (_I32, [_P, _P, _P, _P, _P, _P]),
     "mpm_sort_keys_cells": (_I32, [_P, _P, _P, _P, _P, _P]),
     "mpm_sort_temp_bytes": (_I64, [_I32]),
@@ -166,12 +167,12 @@ SIGNATURES = {
     "mpm_segment_distance": (_I32, [_P, _I32, _P, _F, _I32, _P, _P]),
     "mpm_grid_update_noalloc": (_I32, [_P, _P, _P, _P, _P, _I32, _P, _I32, _P, _I32, _F, _I32, _P]),
     "mpm_adjoint_g2p": (_I32, [_P, _P, _P, _P, _F, _P]),
-    "mpm_adjoint_grid": (_I32, [_P, _P, _P, _I32, _P, _I32, _P, _F, _P]),
+    "mpm_adjoint_grid": (_I32, [_P, _P, _P, _P, _I32, _P, _I32, _P, _I32, _P, _F, _P]),
     "mpm_adjoint_p2g_stress": (_I32, [_P, _P, _P, _P, _F, _P]),
 }
 
 _LIB = None
-ABI_VERSION = 9  # MPM_ABI_VERSION of include/mpm_hip.h this binding is written for
+ABI_VERSION = 10  # MPM_ABI_VERSION of include/mpm_hip.h this binding is written for
 
 
 class NativeError(RuntimeError):
