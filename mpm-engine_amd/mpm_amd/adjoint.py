@@ -14,16 +14,24 @@ von Mises (1, with or without hardening: the hardened yield stress is carried as
 particle state, its adjoint substep to substep), Drucker-Prager sand (2), local mu(I)
 (9) and the generalized-Newtonian fluid (10).
 
-Forward: fused ticks of `checkpoint_every` substeps (the production substep), storing
-the particle state (x, v, C, F_trial and the velocity gradient L, which the viscous
-stress of the next substep reads, and the yield stress, which hardening changes) at every
-tick boundary. Backward, segment by
-segment from the last: restore the checkpoint, recompute the segment substep by
-substep with the split passes while keeping every intermediate particle state and grid,
-then run the hand-written adjoint kernels of libmpm_hip.so (mpm_adjoint_g2p,
-mpm_adjoint_grid, mpm_adjoint_p2g_stress) backwards through the segment. Memory is
-O(checkpoints + one segment), time about one forward for the recompute plus the
-adjoint kernels.
+Two ways to keep what the backward reads, chosen by the memory they need:
+
+* tape (the default when it fits in half the free device memory): the forward runs the
+  split passes (stress + P2G, grid update, G2P) and keeps every substep's particle state
+  and grids; the backward is the hand-written adjoint kernels of libmpm_hip.so
+  (mpm_adjoint_g2p, mpm_adjoint_grid, mpm_adjoint_p2g_stress) run backwards through the
+  tape. Memory O(substeps x (particles + grid)), time one forward plus the adjoint kernels.
+* checkpoints: the forward runs fused ticks of `checkpoint_every` substeps (the
+  production substep), storing the particle state (x, v, C, F_trial, the velocity
+  gradient L, which the viscous stress of the next substep reads, and the yield stress,
+  which hardening changes) at every tick boundary. Backward, segment by segment from the
+  last: restore the checkpoint, recompute the segment with the split passes while keeping
+  every intermediate particle state and grid, then run the adjoint kernels backwards
+  through the segment. Memory O(checkpoints + one segment), time about one forward for
+  the recompute plus the adjoint kernels.
+
+In both the grid kernels of a segment of `checkpoint_every` substeps cover a launch box
+computed on the device from the particle bounds at the segment's start.
 
 The reference solver has no gradient path; this module is the differentiation form
 built on the same kernels and collider schedule as the forward Solver.
@@ -62,7 +70,7 @@ class Rollout:
     SUPPORTED_COLLIDERS = frozenset((native.COLLIDER_PLANE, native.COLLIDER_BOX, native.COLLIDER_WALLS,
                                      native.COLLIDER_POINTCLOUD, native.COLLIDER_CUP, native.COLLIDER_SDF))
 
-    def __init__(self, solver, dt: float, substeps: int, checkpoint_every: int = 20):
+    def __init__(self, solver, dt: float, substeps: int, checkpoint_every: int = 20, tape: bool | None = None):
         sim = solver._sim
         if sim is None:
             raise ValueError("load particles and set a material before building a Rollout")
@@ -132,12 +140,16 @@ class Rollout:
                               dtype=torch.float32, device=dev)
         self.wrench = torch.zeros(max(6 * self.n_slots, 6), dtype=torch.float32, device=dev)
         nseg = (self.substeps + self.every - 1) // self.every
-        self.ckpt = torch.empty(nseg, _CKPT_PLANES, n, dtype=torch.float32, device=dev)
         self.seg_ctl = torch.zeros(nseg, self.ctl.numel(), dtype=torch.int32, device=dev)
-        # one segment of recomputed substeps: particle slots and grids
-        L = min(self.every, self.substeps)
-        self.slots = torch.zeros(L + 1, _SLOT_PLANES, n, dtype=torch.float32, device=dev)
         node_floats = nb * 64 * 4
+        if tape is None:  # the whole tape when it takes at most half the free device memory
+            tape_bytes = 4 * ((self.substeps + 1) * _SLOT_PLANES * n + 2 * self.substeps * node_floats)
+            tape = tape_bytes <= torch.cuda.mem_get_info(dev)[0] // 2
+        self.tape = bool(tape)
+        # particle slots and grids of every substep (tape) or of one segment (checkpoints)
+        L = self.substeps if self.tape else min(self.every, self.substeps)
+        self.ckpt = None if self.tape else torch.empty(nseg, _CKPT_PLANES, n, dtype=torch.float32, device=dev)
+        self.slots = torch.zeros(L + 1, _SLOT_PLANES, n, dtype=torch.float32, device=dev)
         self.g_in = torch.zeros(L, node_floats, dtype=torch.float32, device=dev)
         self.g_out = torch.zeros(L, node_floats, dtype=torch.float32, device=dev)
         # adjoint buffers
@@ -153,8 +165,9 @@ class Rollout:
         self.A = A
         self.impb = z(max(6 * self.n_slots, 6))
         # collider rows of the segment being reversed (a fixed buffer, so one captured
-        # graph serves every full segment)
-        self.seg_sched = torch.zeros(max(L * self.row_bytes, 1), dtype=torch.uint8, device=dev)
+        # graph serves every full segment; the tape reads the whole schedule)
+        self.seg_sched = None if self.tape else torch.zeros(max(L * self.row_bytes, 1), dtype=torch.uint8,
+                                                             device=dev)
         # ABI structs of every substep slot, built once. The recompute's stress + P2G pass
         # of substep k reads slot k's state and L and writes F and stress into slot k + 1,
         # hardening slot k + 1's copy of the yield stress; its G2P pass moves slot k + 1's
@@ -169,7 +182,8 @@ class Rollout:
                       for k in range(L)]
         self._G = [self._grid_struct(self.g_in[k], self.g_out[k]) for k in range(L)]
         self.use_graph = True
-        self._graph = None
+        self._graph = None     # checkpoints: one full segment's backward; tape: the whole backward
+        self._fwd_graph = None  # tape: the forward's substep loop
         # checkpoints, slots and model fields hold the most recent forward only: each
         # forward gets a serial number and a backward refuses to run on another's data
         self.forward_count = 0
@@ -249,6 +263,8 @@ class Rollout:
         if was_det:
             sim.disable_deterministic()
         try:
+            if self.tape:
+                return self._forward_tape(mu, lam, ys, v0, x0, fluid, granular)
             return self._forward(mu, lam, ys, v0, x0, fluid, granular)
         finally:
             self.forward_count += 1
@@ -295,11 +311,89 @@ class Rollout:
         wrench = self.wrench[:6 * self.n_slots].clone().view(-1, 6)
         return self._to_user(sim.x.t()), wrench
 
-    def _segment_box(self, seg, S, s):
-        """The segment's launch box from the current particle state, on the device; kept
-        for the segment's recompute in the backward."""
+    def _forward_tape(self, mu, lam, ys, v0, x0, fluid, granular):
+        sim = self.sim
+        torch = self.torch
+        self._set_state(mu, lam, ys, v0, x0, fluid, granular)
+        self.wrench.zero_()
+        sim.error_flag.zero_()
+        first = self.slots[0]
+        first[_X:_X + 3].copy_(sim.x)
+        first[_YS].copy_(sim.yield_stress)
+        first[_V:_V + 3].copy_(sim.v)
+        first[_C:_C + 9].copy_(sim.C)
+        first[_FT:_FT + 9].copy_(sim.F_trial)
+        first[_L0:_L0 + 9].copy_(sim.L)
+        self.g_in.zero_()  # every P2G of the tape accumulates into its own grid
+        if self.use_graph:
+            if self._fwd_graph is None:
+                g = torch.cuda.CUDAGraph()
+                torch.cuda.synchronize(sim.device)
+                with torch.cuda.graph(g):
+                    self._tape_forward_body()
+                self._fwd_graph = g
+            self._fwd_graph.replay()
+        else:
+            self._tape_forward_body()
+        # the simulator ends in the final state, as after the fused forward
+        last = self.slots[self.substeps]
+        sim.x.copy_(last[_X:_X + 3])
+        sim.yield_stress.copy_(last[_YS])
+        sim.v.copy_(last[_V:_V + 3])
+        sim.C.copy_(last[_C:_C + 9])
+        sim.F_trial.copy_(last[_FT:_FT + 9])
+        sim.F.copy_(last[_F:_F + 9])
+        sim.stress.copy_(last[_ST:_ST + 9])
+        sim.L.copy_(last[_L0:_L0 + 9])
+        flag = int(sim.error_flag.item())
+        if flag & (native.ERR_STENCIL | native.ERR_NONFINITE):
+            raise RuntimeError(f"forward rollout left the grid or produced non-finite state (flag {flag})")
+        wrench = self.wrench[:6 * self.n_slots].clone().view(-1, 6)
+        return self._to_user(sim.x.t()), wrench
+
+    def _tape_forward_body(self):
+        """Every substep with the split passes into its own slot and grids, segment by
+        segment (each with its launch box from the positions at its start)."""
+        sim = self.sim
+        s = sim.stream()
+        md = native.ptr(sim.model_dev)
+        sched = native.ptr(self.sched)
+        for seg, k0 in enumerate(range(0, self.substeps, self.every)):
+            S = min(self.every, self.substeps - k0)
+            self._segment_box(seg, S, s, C.byref(self._P_adj[k0]))
+            for k in range(k0, k0 + S):
+                self._substep(k, sched + k * self.row_bytes, s, md, True)
+
+    def _substep(self, k, row, s, md, wrench):
+        """Substep k with the split passes: slot k -> slot k + 1 through grids k (the
+        collider rows at device address `row`; the wrench accumulated when `wrench`)."""
+        sim, lib, dt = self.sim, self.sim.lib, self.dt
+        self.slots[k + 1, :_CARRY].copy_(self.slots[k, :_CARRY])
+        G = C.byref(self._G[k])
+        native.check(lib.mpm_particle_pass(C.byref(self._P_p2g[k]), G, md, dt,
+                                           native.PASS_STRESS | native.PASS_P2G, s), "stress + p2g")
+        slots = self.n_slots if wrench else 0
+        native.check(lib.mpm_grid_update(
+            G, md, native.ptr(self.ctl), native.ptr(self.blocks), native.ptr(self.ctl) + 24,
+            sim.n_blocks, row if self.K else None, self.K, native.ptr(self.wrench) if slots else None, slots,
+            native.ptr(self.ws) if slots else None, dt, 0, s), "grid update")
+        native.check(lib.mpm_particle_pass(C.byref(self._P_g2p[k]), G, md, dt, native.PASS_G2P, s), "g2p")
+
+    def _reverse_substep(self, k, row, s, md):
+        """The adjoint of substep k, from slots k, k + 1 and grids k."""
+        lib, dt = self.sim.lib, self.dt
+        P, G, pA = C.byref(self._P_adj[k]), C.byref(self._G[k]), C.byref(self.A)
+        native.check(lib.mpm_adjoint_g2p(P, G, pA, md, dt, s), "adjoint g2p")
+        native.check(lib.mpm_adjoint_grid(G, md, native.ptr(self.blocks), native.ptr(self.ctl) + 24,
+                                          self.sim.n_blocks, row if self.K else None, self.K,
+                                          native.ptr(self.impb), self.n_slots, pA, dt, s), "adjoint grid")
+        native.check(lib.mpm_adjoint_p2g_stress(C.byref(self._P_st[k]), G, pA, md, dt, s), "adjoint p2g+stress")
+
+    def _segment_box(self, seg, S, s, pP=None):
+        """The segment's launch box from the particle positions (the simulator's, or those
+        of the particle struct pP), on the device; kept for the segment's backward."""
         sim, lib = self.sim, self.sim.lib
-        native.check(lib.mpm_bounds(sim.pP, native.ptr(self.bnd), s), "bounds")
+        native.check(lib.mpm_bounds(sim.pP if pP is None else pP, native.ptr(self.bnd), s), "bounds")
         native.check(lib.mpm_box_from_bounds(native.ptr(self.bnd), sim.pG, self.box_pad, 1.5 * self.dt * S,
                                              native.ptr(self.ctl), s), "box_from_bounds")
         native.check(lib.mpm_blocks_from_box(sim.pG, native.ptr(self.ctl), native.ptr(self.blocks), s),
@@ -325,35 +419,31 @@ class Rollout:
         """Recompute S substeps from slot 0 with the split passes, keeping every state and
         grid, then run the adjoint kernels backwards through them."""
         sim = self.sim
-        lib = sim.lib
         s = sim.stream()
-        dt = self.dt
         md = native.ptr(sim.model_dev)
-        # P2G accumulates into g_in[k] (zero first); the grid update writes every node of
-        # g_out[k] (the launch box is the whole grid), so g_out needs no clearing
+        # P2G accumulates into g_in[k] (zero first); the grid update writes g_out[k] over
+        # the segment's launch box, where every particle of the segment gathers
         self.g_in[:S].zero_()
         sched0 = native.ptr(self.seg_sched)
         for k in range(S):
-            self.slots[k + 1, :_CARRY].copy_(self.slots[k, :_CARRY])
-            G = C.byref(self._G[k])
-            native.check(lib.mpm_particle_pass(C.byref(self._P_p2g[k]), G, md, dt,
-                                               native.PASS_STRESS | native.PASS_P2G, s), "recompute p2g")
-            native.check(lib.mpm_grid_update(
-                G, md, native.ptr(self.ctl), native.ptr(self.blocks), native.ptr(self.ctl) + 24,
-                sim.n_blocks, sched0 + k * self.row_bytes if self.K else None, self.K, None, 0, None,
-                dt, 0, s), "recompute grid")
-            native.check(lib.mpm_particle_pass(C.byref(self._P_g2p[k]), G, md, dt, native.PASS_G2P, s),
-                         "recompute g2p")
-        pA = C.byref(self.A)
-        impb = native.ptr(self.impb)
+            self._substep(k, sched0 + k * self.row_bytes, s, md, False)
         for k in reversed(range(S)):
-            P, G = C.byref(self._P_adj[k]), C.byref(self._G[k])
-            native.check(lib.mpm_adjoint_g2p(P, G, pA, md, dt, s), "adjoint g2p")
-            native.check(lib.mpm_adjoint_grid(G, md, native.ptr(self.blocks), native.ptr(self.ctl) + 24,
-                                              sim.n_blocks, sched0 + k * self.row_bytes if self.K else None, self.K,
-                                              impb, self.n_slots, pA, dt, s), "adjoint grid")
-            native.check(lib.mpm_adjoint_p2g_stress(C.byref(self._P_st[k]), G, pA, md, dt, s),
-                         "adjoint p2g+stress")
+            self._reverse_substep(k, sched0 + k * self.row_bytes, s, md)
+
+    def _tape_backward_body(self):
+        """The adjoint kernels backwards through the whole tape, segment by segment (each
+        with the launch box its forward used)."""
+        sim = self.sim
+        s = sim.stream()
+        md = native.ptr(sim.model_dev)
+        sched = native.ptr(self.sched)
+        for seg, k0 in reversed(list(enumerate(range(0, self.substeps, self.every)))):
+            S = min(self.every, self.substeps - k0)
+            self.ctl.copy_(self.seg_ctl[seg])
+            native.check(sim.lib.mpm_blocks_from_box(sim.pG, native.ptr(self.ctl), native.ptr(self.blocks), s),
+                         "blocks_from_box")
+            for k in reversed(range(k0, k0 + S)):
+                self._reverse_substep(k, sched + k * self.row_bytes, s, md)
 
     def backward(self, x_bar, wrench_bar, forward_id=None):
         """Adjoint of the last forward: returns (mu_bar, lam_bar, ys_bar, v0_bar, x0_bar,
@@ -384,20 +474,31 @@ class Rollout:
         if wrench_bar is not None and self.n_slots:
             self.impb[:6 * self.n_slots].copy_(wrench_bar.reshape(-1))
         sim._upload_model()
-        segs = list(enumerate(range(0, self.substeps, self.every)))
-        for seg, k0 in reversed(segs):
-            S = min(self.every, self.substeps - k0)
-            self._load_segment(seg, k0, S)
-            if self.use_graph and S == self.every:
-                if self._graph is None:  # one capture serves every full segment
+        if self.tape:
+            if not self.use_graph:
+                self._tape_backward_body()
+            else:
+                if self._graph is None:  # one capture of the whole backward
                     g = torch.cuda.CUDAGraph()
                     torch.cuda.synchronize(sim.device)
                     with torch.cuda.graph(g):
-                        self._segment_body(S)
+                        self._tape_backward_body()
                     self._graph = g
                 self._graph.replay()
-            else:
-                self._segment_body(S)
+        else:
+            for seg, k0 in reversed(list(enumerate(range(0, self.substeps, self.every)))):
+                S = min(self.every, self.substeps - k0)
+                self._load_segment(seg, k0, S)
+                if self.use_graph and S == self.every:
+                    if self._graph is None:  # one capture serves every full segment
+                        g = torch.cuda.CUDAGraph()
+                        torch.cuda.synchronize(sim.device)
+                        with torch.cuda.graph(g):
+                            self._segment_body(S)
+                        self._graph = g
+                    self._graph.replay()
+                else:
+                    self._segment_body(S)
         flag = int(sim.error_flag.item())
         if flag & (native.ERR_ADJ_MATERIAL | native.ERR_ADJ_COLLIDER):
             raise NotImplementedError(f"a material or collider without an adjoint was met (flag {flag})")
