@@ -178,6 +178,16 @@ MPM_DEV unsigned long long wrench_q(float x, float S, int* error_flag) {
 // atomics: the wrench readout is bit-reproducible for a given block list. With more
 // slots (one or more wrench colliders per scene of a large batch) lane 0 adds the
 // wavefront sums to `wrench` with float atomics instead.
+// kStage (at most kStageRows colliders per scene, 16-byte aligned rows): each wavefront
+// copies its block's scene row of colliders into LDS with one round of parallel vector
+// loads (again only when the scene changes), and the per-node collider loop reads them
+// from there, instead of a chain of dependent scalar loads, one per collider, that miss
+// the scalar cache together at the start of the launch.
+constexpr int kStageRows = 12;
+static_assert(sizeof(mpm_collider_t) % 16 == 0, "collider rows are copied as float4");
+constexpr int kRowVec = (int)(sizeof(mpm_collider_t) / 16);
+
+template <bool kStage>
 __global__ __launch_bounds__(kThreads) void k_grid_update(mpm_grid_t G, const mpm_model_t* __restrict__ md,
                                                           const mpm_tick_ctl_t* __restrict__ ctl,
                                                           const int32_t* __restrict__ list,
@@ -187,7 +197,9 @@ __global__ __launch_bounds__(kThreads) void k_grid_update(mpm_grid_t G, const mp
                                                           float* __restrict__ wrench, float dt, int flags) {
     // per-wavefront wrench rows: float sums, or fixed-point integer sums (deterministic)
     __shared__ unsigned long long s_wraw[kThreads / 64][kMaxSlots * 6];
+    __shared__ float4 s_rows[kStage ? kThreads / 64 : 1][kStage ? kStageRows * kRowVec : 1];
     float(*s_w)[kMaxSlots * 6] = reinterpret_cast<float(*)[kMaxSlots * 6]>(&s_wraw[0][0]);
+    int staged = -1;  // scene whose collider row this wavefront holds in s_rows
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
     const bool lds_w = nslots <= kMaxSlots;
@@ -226,6 +238,17 @@ __global__ __launch_bounds__(kThreads) void k_grid_update(mpm_grid_t G, const mp
         const int code = __shfl(my_code, l, 64);
         const int scene = code / bps;
         const mpm_collider_t* __restrict__ cs = col + (size_t)scene * ncoll;
+        if constexpr (kStage) {
+            if (scene != staged) {  // wavefront-uniform
+                const float4* __restrict__ src = reinterpret_cast<const float4*>(cs);
+                for (int t = lane; t < ncoll * kRowVec; t += 64) s_rows[wave][t] = src[t];
+                staged = scene;
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            }
+            cs = reinterpret_cast<const mpm_collider_t*>(&s_rows[wave][0]);
+        }
         int bi, bj, bk;
         decode_block(G, code - scene * bps, bi, bj, bk);
         const int gi = bi * 4 + (lane >> 4), gj = bj * 4 + ((lane >> 2) & 3), gk = bk * 4 + (lane & 3);
@@ -567,13 +590,28 @@ __global__ void k_sort_keys(mpm_particles_t P, mpm_grid_t G, int32_t* keys, int3
     if (p + 1 < P.n && key_of(p + 1) < k) atomicOr(unsorted, 1);
 }
 
-template <typename T>
-__global__ void k_gather(const T* __restrict__ src, const int32_t* __restrict__ perm, T* __restrict__ dst, int n,
-                         int stride, int planes) {
-    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (long)n * planes) return;
-    int q = (int)(t / n), i = (int)(t % n);
-    dst[(size_t)q * stride + i] = src[(size_t)q * stride + perm[i]];
+// Up to kGroupPlanes 4-byte planes of n particles (any fields), gathered by perm into
+// scratch and copied back: one launch each way for the group instead of one per field and
+// one copy per plane.
+constexpr int kGroupPlanes = 9;
+struct PlaneGroup {
+    uint32_t* p[kGroupPlanes];
+    int count;
+};
+
+__global__ void k_gather_planes(PlaneGroup g, const int32_t* __restrict__ perm, uint32_t* __restrict__ scratch, int n,
+                                int stride) {
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long)n * g.count) return;
+    const int q = (int)(t / n), i = (int)(t - (long)q * n);
+    scratch[(size_t)q * stride + i] = g.p[q][perm[i]];
+}
+
+__global__ void k_copy_planes(PlaneGroup g, const uint32_t* __restrict__ scratch, int n, int stride) {
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long)n * g.count) return;
+    const int q = (int)(t / n), i = (int)(t - (long)q * n);
+    g.p[q][i] = scratch[(size_t)q * stride + i];
 }
 
 // ------------------------------------------------------------------------ exports
@@ -771,8 +809,9 @@ int32_t mpm_grid_update(const mpm_grid_t* G, const mpm_model_t* model, const mpm
     if (capacity == 0) return MPM_OK;
     const unsigned nwg = grid_update_wgs(capacity);
     hipStream_t s = (hipStream_t)stream;
-    hipLaunchKernelGGL(k_grid_update, dim3(nwg), dim3(kThreads), 0, s, *G, model, ctl, list, count, colliders, ncoll,
-                       wrench_slots, workspace, wrench, dt, flags);
+    const bool stage = ncoll > 0 && ncoll <= kStageRows && ((uintptr_t)colliders & 15) == 0;
+    hipLaunchKernelGGL(stage ? k_grid_update<true> : k_grid_update<false>, dim3(nwg), dim3(kThreads), 0, s, *G,
+                       model, ctl, list, count, colliders, ncoll, wrench_slots, workspace, wrench, dt, flags);
     if (wrench_slots > 0 && fixed && !(flags & MPM_GRID_WRENCH_DEFER))
         hipLaunchKernelGGL(k_wrench_fix_flush, dim3(wrench_slots * 6), dim3(kThreads), 0, s, *G,
                            reinterpret_cast<unsigned long long*>(workspace), fixed_order ? (int)nwg : 1,
@@ -883,34 +922,31 @@ int32_t mpm_permute(const mpm_particles_t* P, const int32_t* perm, float* scratc
     if (P->n == 0) return MPM_OK;
     hipStream_t s = (hipStream_t)stream;
     const int n = P->n, S = P->stride;
-    auto one = [&](float* a, int planes) -> int {
-        hipLaunchKernelGGL(k_gather<float>, dim3(blocks_for((long)n * planes)), dim3(kThreads), 0, s, a, perm, scratch,
-                           n, S, planes);
-        for (int q = 0; q < planes; ++q)
-            if (hipMemcpyAsync(a + (size_t)q * S, scratch + (size_t)q * S, sizeof(float) * n, hipMemcpyDeviceToDevice,
-                               s) != hipSuccess)
-                return MPM_ERR_HIP;
-        return MPM_OK;
+    uint32_t* sc = reinterpret_cast<uint32_t*>(scratch);
+    PlaneGroup g{};
+    auto flush = [&]() {
+        if (g.count == 0) return;
+        const unsigned nb = blocks_for((long)n * g.count);
+        hipLaunchKernelGGL(k_gather_planes, dim3(nb), dim3(kThreads), 0, s, g, perm, sc, n, S);
+        hipLaunchKernelGGL(k_copy_planes, dim3(nb), dim3(kThreads), 0, s, g, sc, n, S);
+        g.count = 0;
     };
-    auto onei = [&](int32_t* a) -> int {
-        int32_t* sc = reinterpret_cast<int32_t*>(scratch);
-        hipLaunchKernelGGL(k_gather<int32_t>, dim3(blocks_for(n)), dim3(kThreads), 0, s, a, perm, sc, n, S, 1);
-        return hipMemcpyAsync(a, sc, sizeof(int32_t) * n, hipMemcpyDeviceToDevice, s) == hipSuccess ? MPM_OK
-                                                                                                    : MPM_ERR_HIP;
+    auto add = [&](void* a, int planes) {  // a field of `planes` planes of 4-byte values
+        if (!a) return;
+        for (int q = 0; q < planes; ++q) {
+            g.p[g.count++] = reinterpret_cast<uint32_t*>(a) + (size_t)q * S;
+            if (g.count == kGroupPlanes) flush();
+        }
     };
-    int r = MPM_OK;
-    r |= one(P->x, 3); r |= one(P->v, 3);
-    r |= one(P->F, 9); r |= one(P->F_trial, 9); r |= one(P->C, 9); r |= one(P->stress, 9); r |= one(P->L, 9);
-    r |= one(P->vol, 1); r |= one(P->mass, 1); r |= one(P->mu, 1); r |= one(P->lam, 1);
-    r |= one(P->yield_stress, 1); r |= one(P->Jp, 1);
-    if (P->cov) r |= one(P->cov, 6);
-    r |= onei(P->material); r |= onei(P->selection);
-    if (P->scene) r |= onei(P->scene);
-    if (P->cdf_tag) r |= onei(P->cdf_tag);
-    if (P->rigid_id) r |= onei(P->rigid_id);
-    if (P->x_ref) r |= one(P->x_ref, 3);
-    if (P->modifier_mask) r |= onei(P->modifier_mask);
-    if (r != MPM_OK) return MPM_ERR_HIP;
+    add(P->x, 3); add(P->v, 3);
+    add(P->F, 9); add(P->F_trial, 9); add(P->C, 9); add(P->stress, 9); add(P->L, 9);
+    add(P->vol, 1); add(P->mass, 1); add(P->mu, 1); add(P->lam, 1);
+    add(P->yield_stress, 1); add(P->Jp, 1);
+    add(P->cov, 6);
+    add(P->material, 1); add(P->selection, 1);
+    add(P->scene, 1); add(P->cdf_tag, 1); add(P->rigid_id, 1);
+    add(P->x_ref, 3); add(P->modifier_mask, 1);
+    flush();
     return launch_status();
 }
 
