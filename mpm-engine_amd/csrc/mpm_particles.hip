@@ -397,15 +397,42 @@ __global__ __launch_bounds__(kT) __attribute__((amdgpu_waves_per_eu(MPM_PASS_WAV
                 s_scatter[3][a] = Acc(0);
             }
         }
-        const bool sel = p < p1 && P.selection[p] == 0;
-        const int scene = (sel && P.scene) ? P.scene[p] : 0;
+        // The particle's selection, scene, material and position, and the inputs the pass
+        // reads after the gather (F, the per-particle material state, mass and volume), in
+        // one round of loads at the chunk's start: not one round per condition that
+        // depends on the last, and not one after each of the chain's later barriers. (The
+        // CPIC pass, which has less register room, loads the latter where it uses them.)
+        constexpr bool kEarly = !kCdf;
+        int sel_v = 1, scene_v = 0, mat_v = 0;
+        v3 x_v = mk3(0.f, 0.f, 0.f);
+        m33 F_v = mzero();
+        float mass_v = 0.f, vol_v = 0.f, mu_v = 0.f, lam_v = 0.f, ys_v = 0.f, jp_v = 0.f;
+        if (p < p1) {
+            sel_v = P.selection[p];
+            if (P.scene) scene_v = P.scene[p];
+            mat_v = P.material[p];
+            x_v = ld3(P.x, S, p);
+            if constexpr (kEarly && kG2P) F_v = ld9(P.F, S, p);
+            if constexpr (kEarly && kStress) {
+                mu_v = P.mu[p];
+                lam_v = P.lam[p];
+                ys_v = P.yield_stress[p];
+                jp_v = P.Jp[p];
+            }
+            if constexpr (kEarly && kP2G) {
+                mass_v = P.mass[p];
+                vol_v = P.vol[p];
+            }
+        }
+        const bool sel = p < p1 && sel_v == 0;
+        const int scene = sel ? scene_v : 0;
         const bool alive = sel && (unsigned)scene < (unsigned)G.n_scenes;  // bad scene ids: error bit 8
-        const int mat = alive ? P.material[p] : 0;
+        const int mat = alive ? mat_v : 0;
         const size_t soff = scene_base(G, alive ? scene : 0);
         const bool periodic = md->periodic_x != 0;
         int err = (sel && !alive) ? 8 : 0;
 
-        v3 x = alive ? ld3(P.x, S, p) : mk3(0.f, 0.f, 0.f);
+        v3 x = alive ? x_v : mk3(0.f, 0.f, 0.f);
         v3 v = mk3(0.f, 0.f, 0.f);
         m33 Cm = mzero(), L = mzero(), Ft = mzero();
 
@@ -490,7 +517,7 @@ __global__ __launch_bounds__(kT) __attribute__((amdgpu_waves_per_eu(MPM_PASS_WAV
                 }
                 m33 A = mscale(nF, dt);
                 A.a[0] += 1.f; A.a[4] += 1.f; A.a[8] += 1.f;
-                Ft = mmul(A, ld9(P.F, S, p));
+                Ft = mmul(A, kEarly ? F_v : ld9(P.F, S, p));
                 if (md->update_cov_with_F && P.cov) {  // update_cov, mpm_utils.py:613-633
                     float c[6];
     #pragma unroll
@@ -535,10 +562,10 @@ __global__ __launch_bounds__(kT) __attribute__((amdgpu_waves_per_eu(MPM_PASS_WAV
         if constexpr (kStress) {
             if (alive) {
                 PState ps;
-                ps.mu = P.mu[p];
-                ps.lam = P.lam[p];
-                ps.yield = P.yield_stress[p];
-                ps.Jp = P.Jp[p];
+                ps.mu = kEarly ? mu_v : P.mu[p];
+                ps.lam = kEarly ? lam_v : P.lam[p];
+                ps.yield = kEarly ? ys_v : P.yield_stress[p];
+                ps.Jp = kEarly ? jp_v : P.Jp[p];
                 ps.dirty_mu = ps.dirty_yield = ps.dirty_Jp = false;
                 m33 F;
                 if constexpr (kSolid) {
@@ -618,8 +645,8 @@ __global__ __launch_bounds__(kT) __attribute__((amdgpu_waves_per_eu(MPM_PASS_WAV
                     Cm = madd(mscale(Cm, 1.0f - r), mscale(msub(Cm, Ct), r / 2.0f));
                     if (r < -0.001f) Cm = mzero();
                 }
-                fs = mscale(stress, -P.vol[p] * dt);
-                mass = P.mass[p];
+                fs = mscale(stress, -(kEarly ? vol_v : P.vol[p]) * dt);
+                mass = kEarly ? mass_v : P.mass[p];
                 if (!stencil_in_grid(G, st, periodic)) err |= 1;
             }
             // Lanes whose particles share a stencil base cell (consecutive after a cell
@@ -771,12 +798,20 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MPM_PASS_WA
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
         const int p = p0 + h * kT + (int)threadIdx.x;
-        const bool sel = p < p1 && P.selection[p] == 0;
-        scene[h] = (sel && P.scene) ? P.scene[p] : 0;
+        int sel_v = 1, scene_v = 0, mat_v = 0;  // one round of loads, as in the single pass
+        v3 x_v = mk3(0.f, 0.f, 0.f);
+        if (p < p1) {
+            sel_v = P.selection[p];
+            if (P.scene) scene_v = P.scene[p];
+            mat_v = P.material[p];
+            x_v = ld3(P.x, S, p);
+        }
+        const bool sel = p < p1 && sel_v == 0;
+        scene[h] = sel ? scene_v : 0;
         alive[h] = sel && (unsigned)scene[h] < (unsigned)G.n_scenes;  // bad scene ids: error bit 8
         if (sel && !alive[h]) err |= 8;
-        mat[h] = alive[h] ? P.material[p] : 0;
-        x[h] = alive[h] ? ld3(P.x, S, p) : mk3(0.f, 0.f, 0.f);
+        mat[h] = alive[h] ? mat_v : 0;
+        x[h] = alive[h] ? x_v : mk3(0.f, 0.f, 0.f);
         pp[h] = p;
     }
 
