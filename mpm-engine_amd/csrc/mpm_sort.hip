@@ -1,19 +1,33 @@
 // Stable device radix sort of (block key, particle index) pairs for the block sort
-// (Solver._sort_particles, src/warpmpm/core/solver.py:454-467). rocPRIM's radix sort
-// (through hipCUB) is stable, so equal keys keep their index order exactly like the
-// reference's np.argsort(kind="stable").
+// (Solver._sort_particles, src/warpmpm/core/solver.py:454-467). rocPRIM's radix sort is
+// stable, so equal keys keep their index order exactly like the reference's
+// np.argsort(kind="stable").
+//
+// The sort is called inside captured hipGraphs (the simulators replay the whole
+// permutation as one graph), so it must launch kernels only. rocPRIM's default radix sort
+// switches above 2^20 items to its onesweep algorithm, which clears its digit histogram,
+// lookback states and block counter with hipMemsetAsync: memset nodes in a captured graph,
+// which this platform does not replay reliably (a stale lookback state sends keys to
+// wrong offsets). The configuration below keeps rocPRIM's block-sort + merge path, which
+// launches only kernels, for every size.
 #include <hip/hip_runtime.h>
 #include <hipcub/hipcub.hpp>
+#include <rocprim/device/device_radix_sort.hpp>
 
 #include "../../include/mpm_hip.h"
+
+namespace {
+using SortConfig = rocprim::radix_sort_config<rocprim::default_config, rocprim::default_config,
+                                              rocprim::default_config, (size_t)1 << 32>;
+}
 
 extern "C" {
 
 int64_t mpm_sort_temp_bytes(int32_t n) {
     size_t bytes = 0;
     if (n <= 0) return 0;
-    if (hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, (const int32_t*)nullptr, (int32_t*)nullptr,
-                                           (const int32_t*)nullptr, (int32_t*)nullptr, n) != hipSuccess)
+    if (rocprim::radix_sort_pairs<SortConfig>(nullptr, bytes, (const int32_t*)nullptr, (int32_t*)nullptr,
+                                              (const int32_t*)nullptr, (int32_t*)nullptr, (size_t)n) != hipSuccess)
         return -1;
     return (int64_t)bytes;
 }
@@ -23,8 +37,8 @@ int32_t mpm_sort_pairs(const int32_t* keys, int32_t* keys_out, const int32_t* va
     if (n < 0 || (n > 0 && (!keys || !keys_out || !vals || !vals_out || !temp))) return MPM_ERR_ARG;
     if (n == 0) return MPM_OK;
     size_t bytes = (size_t)temp_bytes;
-    hipError_t e = hipcub::DeviceRadixSort::SortPairs(temp, bytes, keys, keys_out, vals, vals_out, n, 0,
-                                                      (int)(sizeof(int32_t) * 8), (hipStream_t)stream);
+    hipError_t e = rocprim::radix_sort_pairs<SortConfig>(temp, bytes, keys, keys_out, vals, vals_out, (size_t)n, 0,
+                                                         (unsigned)(sizeof(int32_t) * 8), (hipStream_t)stream);
     return e == hipSuccess ? MPM_OK : MPM_ERR_HIP;
 }
 

@@ -445,6 +445,41 @@ def test_graph_replayed_sorts_match_eager_bitwise():
     assert np.array_equal(xg, xe) and np.array_equal(fg, fe) and np.array_equal(dg, de)
 
 
+
+def test_graph_replayed_sorts_above_a_million_particles():
+    """The permutation graph replays the radix sort as well. Above 2^20 keys rocPRIM's
+    default radix sort clears its digit histogram and lookback states with memsets, which
+    captured graphs do not replay reliably here (a stale lookback state sends keys to wrong
+    offsets); the library keeps rocPRIM's kernel-only merge path at every size. 1.2M
+    particles, re-shuffled before each of five sorts so that every sort permutes: each
+    graph replay leaves the particle arrays bit for bit as the same sorts launched eagerly,
+    in block-and-cell key order."""
+    import torch
+
+    rng = np.random.default_rng(13)
+    grid = GridConfig(n_grid=64, grid_lim=1.0)
+    n = 1_200_000
+    pts = rng.uniform(0.2, 0.8, (n, 3)).astype(np.float32)
+    vol = np.full(n, 1.0e-7, np.float32)
+
+    def make(graph):
+        s = Solver(grid=grid, device="hip").load_particles(pts, vol)
+        s.set_material(newtonian(eta=1.0, density=1000.0, bulk_modulus=1.0e4))
+        s._sim.use_graph = graph
+        return s._sim
+
+    g, e = make(True), make(False)
+    for it in range(5):
+        shuffle = torch.from_numpy(rng.permutation(n)).to(g.x.device)
+        for sim in (g, e):
+            sim.x.copy_(sim.x[:, shuffle])
+        assert g.sort_by_block(cells=True) and e.sort_by_block(cells=True)
+        assert (g._sort_graph is not None) and (e._sort_graph is None)
+        assert torch.equal(g.x, e.x) and torch.equal(g._last_perm, e._last_perm), f"sort {it}"
+        # the key kernel finds the result in order (no second permutation)
+        assert not g.sort_by_block(cells=True), f"sort {it}: keys out of order"
+    assert torch.equal(torch.sort(g.x[0])[0], torch.sort(torch.from_numpy(pts[:, 0]).to(g.x.device))[0])
+
 def test_graph_replay_matches_eager():
     def run(graph):
         grid, pts, vol = _blob(seed=6)
