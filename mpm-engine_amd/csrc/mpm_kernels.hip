@@ -301,7 +301,7 @@ __global__ __launch_bounds__(kThreads) void k_grid_update(mpm_grid_t G, const mp
             vo = mk3(in4.x * inv + dt * g0, in4.y * inv + dt * g1, in4.z * inv + dt * g2);
         }
         if (damp < 1.0f) vo = vo * damp;
-        for (int q = 0; q < ncoll; ++q) {
+        auto collider_step = [&](int q) {
             const mpm_collider_t& c = cs[q];
             v3 imp = mk3(0.f, 0.f, 0.f), tq = mk3(0.f, 0.f, 0.f);
             const bool act = inbox && c.active && m > 0.0f && gi >= c.lo[0] && gi < c.hi[0] && gj >= c.lo[1] &&
@@ -310,22 +310,25 @@ __global__ __launch_bounds__(kThreads) void k_grid_update(mpm_grid_t G, const mp
             if (c.slot >= 0 && c.slot < nslots && __any(act)) {
                 const float s0 = wave_sum(imp.x), s1 = wave_sum(imp.y), s2 = wave_sum(imp.z);
                 const float s3 = wave_sum(tq.x), s4 = wave_sum(tq.y), s5 = wave_sum(tq.z);
+                // the wavefront's own LDS row takes adds that do not return the old value
+                // (one lane, so the order is fixed)
                 if (lane == 0 && fixed) {  // order-independent: integer sums of fixed point
                     int* ef = G.error_flag;
-                    const unsigned long long q[6] = {wrench_q(s0, sw_f, ef), wrench_q(s1, sw_f, ef),
-                                                     wrench_q(s2, sw_f, ef), wrench_q(s3, sw_t, ef),
-                                                     wrench_q(s4, sw_t, ef), wrench_q(s5, sw_t, ef)};
+                    const unsigned long long q6[6] = {wrench_q(s0, sw_f, ef), wrench_q(s1, sw_f, ef),
+                                                      wrench_q(s2, sw_f, ef), wrench_q(s3, sw_t, ef),
+                                                      wrench_q(s4, sw_t, ef), wrench_q(s5, sw_t, ef)};
                     if (lds_w) {
                         unsigned long long* w = &s_wraw[wave][6 * c.slot];
-                        for (int e = 0; e < 6; ++e) w[e] += q[e];
+                        for (int e = 0; e < 6; ++e) atomicAdd(w + e, q6[e]);
                     } else {
                         unsigned long long* w = reinterpret_cast<unsigned long long*>(partials) + 6 * (size_t)c.slot;
-                        for (int e = 0; e < 6; ++e) atomicAdd(w + e, q[e]);
+                        for (int e = 0; e < 6; ++e) atomicAdd(w + e, q6[e]);
                     }
                 } else if (lane == 0) {
                     if (lds_w) {
                         float* w = &s_w[wave][6 * c.slot];
-                        w[0] += s0; w[1] += s1; w[2] += s2; w[3] += s3; w[4] += s4; w[5] += s5;
+                        atomicAdd(w + 0, s0); atomicAdd(w + 1, s1); atomicAdd(w + 2, s2);
+                        atomicAdd(w + 3, s3); atomicAdd(w + 4, s4); atomicAdd(w + 5, s5);
                     } else {
                         float* w = wrench + 6 * (size_t)c.slot;
                         atomicAdd(w + 0, s0); atomicAdd(w + 1, s1); atomicAdd(w + 2, s2);
@@ -333,6 +336,24 @@ __global__ __launch_bounds__(kThreads) void k_grid_update(mpm_grid_t G, const mp
                     }
                 }
             }
+        };
+        if constexpr (kStage) {
+            // colliders whose box misses this block act on none of its nodes: one round of
+            // LDS reads marks the others, and only those run, in registration order
+            unsigned hit = 0;
+            const int gi0 = bi * 4, gj0 = bj * 4, gk0 = bk * 4;
+#pragma unroll
+            for (int q = 0; q < kStageRows; ++q) {
+                if (q < ncoll) {
+                    const mpm_collider_t& c = cs[q];
+                    if (c.active && gi0 < c.hi[0] && gi0 + 4 > c.lo[0] && gj0 < c.hi[1] && gj0 + 4 > c.lo[1] &&
+                        gk0 < c.hi[2] && gk0 + 4 > c.lo[2])
+                        hit |= 1u << q;
+                }
+            }
+            for (; hit; hit &= hit - 1) collider_step(__ffs(hit) - 1);
+        } else {
+            for (int q = 0; q < ncoll; ++q) collider_step(q);
         }
         if (inbox) *reinterpret_cast<float4*>(G.node_out + o) = make_float4(vo.x, vo.y, vo.z, 0.0f);
         if (valid && !fixed && (flags & MPM_GRID_ZERO_IN) &&
