@@ -556,19 +556,22 @@ __global__ void k_bounds_init(int* out) {
 
 __global__ __launch_bounds__(kThreads) void k_bounds(mpm_particles_t P, int* out) {
     __shared__ int sh[8][kThreads / 64];
-    int p = blockIdx.x * blockDim.x + threadIdx.x;
     int kmin[3] = {0x7fffffff, 0x7fffffff, 0x7fffffff};
     int kmax[3] = {(int)0x80000000, (int)0x80000000, (int)0x80000000};
     int kv = 0, bad = 0;
-    if (p < P.n) {
+    // grid-stride: a bounded launch keeps the same-address global atomics at the end few
+    // (one set per workgroup, all on the same seven words)
+    for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < P.n; p += gridDim.x * blockDim.x) {
+        int nonfinite = 0;
         for (int a = 0; a < 3; ++a) {
             float x = P.x[a * P.stride + p];
-            if (!isfinite(x)) bad = 1;
+            if (!isfinite(x)) nonfinite = 1;
             int k = ord_key(x);
-            kmin[a] = k; kmax[a] = k;
+            kmin[a] = min(kmin[a], k); kmax[a] = max(kmax[a], k);
             float av = fabsf(P.v[a * P.stride + p]);
             kv = max(kv, __float_as_int(av));
         }
+        bad += nonfinite;  // particles with a non-finite position
     }
     for (int off = 32; off > 0; off >>= 1) {
         for (int a = 0; a < 3; ++a) {
@@ -930,7 +933,7 @@ int32_t mpm_bounds(const mpm_particles_t* P, float* out, void* stream) {
     hipStream_t s = (hipStream_t)stream;
     int* o = reinterpret_cast<int*>(out);
     hipLaunchKernelGGL(k_bounds_init, dim3(1), dim3(64), 0, s, o);
-    if (P->n > 0) hipLaunchKernelGGL(k_bounds, dim3(blocks_for(P->n)), dim3(kThreads), 0, s, *P, o);
+    if (P->n > 0) hipLaunchKernelGGL(k_bounds, dim3(min(blocks_for(P->n), 512u)), dim3(kThreads), 0, s, *P, o);
     hipLaunchKernelGGL(k_bounds_finish, dim3(1), dim3(64), 0, s, o);
     return launch_status();
 }
