@@ -796,6 +796,27 @@ def test_tape_and_checkpoints_agree():
     assert np.abs(va - vb).max() <= 2e-3 * np.abs(vb).max()
 
 
+
+def test_tape_falls_back_to_checkpoints_when_memory_is_short(monkeypatch):
+    """tape=None keeps the whole tape only when it takes at most half the free device
+    memory: with (reported) free memory below that the Rollout checkpoints instead, and
+    still differentiates."""
+    import torch
+
+    sc = _scene(plate_speed=0.5)
+    s = sc.build(Solver(grid=sc.grid, device="hip"))
+    n = s.n_particles
+    assert Rollout(s, DT, 10, checkpoint_every=5).tape
+    monkeypatch.setattr(torch.cuda, "mem_get_info", lambda *a, **k: (1 << 20, 1 << 40))
+    ro = Rollout(s, DT, 10, checkpoint_every=5)
+    assert not ro.tape and ro.ckpt is not None
+    dev = s._sim.device
+    E = torch.tensor(2.0e5, dtype=torch.float64, device=dev, requires_grad=True)
+    xT, _ = ro.run(E, torch.tensor(0.3, dtype=torch.float64, device=dev),
+                   torch.tensor(1.0e3, dtype=torch.float64, device=dev), torch.zeros(n, 3, device=dev))
+    xT.double().sum().backward()
+    assert np.isfinite(E.grad.item())
+
 def _granular():
     from mpm_amd.materials import granular
 
