@@ -153,7 +153,7 @@ __global__ __launch_bounds__(kAT) void k_adj_g2p(mpm_particles_t P, mpm_grid_t G
         const int rho = (int)(threadIdx.x % 27u);
         const int i0 = ((rho / 9 - st.base[0]) % 3 + 3) % 3, j0 = (((rho / 3) % 3 - st.base[1]) % 3 + 3) % 3,
                   k0 = ((rho % 3 - st.base[2]) % 3 + 3) % 3;
-#pragma unroll 1
+#pragma unroll 3
         for (int s = 0; s < 27; ++s) {
             int i = s / 9 + i0, j = (s / 3) % 3 + j0, k = s % 3 + k0;
             i = i >= 3 ? i - 3 : i;
@@ -415,7 +415,7 @@ __global__ __launch_bounds__(kThreads) void k_adj_p2g_stress(mpm_particles_t P, 
     const float4* __restrict__ gmv = reinterpret_cast<const float4*>(A.grid_mv);
     v3 vb = mk3(0.f, 0.f, 0.f), xb = ld3(A.x, S, p), wa = vb;
     m33 Cb = mzero(), fsb = mzero();
-#pragma unroll 1
+#pragma unroll
     for (int n = 0; n < 27; ++n) {
         int i, j, k;
         size_t o;
