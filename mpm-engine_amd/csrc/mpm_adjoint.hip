@@ -94,12 +94,30 @@ MPM_DEV m33 outer(v3 a, v3 b) {
 // through global memory. L_k comes from the forward's G2P (P.L).
 constexpr int kAT = 256;
 
+// G2P adjoint of one particle over its 27 stencil nodes, in tensor-product form. With
+// per-axis weights w_a, derivatives d_a, second derivatives h_a and offsets q_a (node
+// index minus fx) of the node (i, j, k), the forward gather's weight is w = w0 w1 w2, its
+// gradient dw = (d0 w1 w2, w0 d1 w2, w0 w1 d2) and Hessian H (H00 = h0 w1 w2, H01 = d0 d1 w2,
+// ...); the adjoint of the node velocity u and of the particle position are
+//   ub  = w (vt + c4 Cb q) + Lb dw / dx,
+//   xb += dw (u.vt + c4 u.(Cb q)) / dx - c4 / dx Cb^T (sum_nodes w u) + H (Lb^T u) / dx^2,
+// where the second term, linear in u, is summed over the nodes first and multiplied by
+// Cb^T once, and the products shared by a row (i, j) of nodes are formed once per row.
+// Nodes go in the residue-class order (at step (ti, tj, tk) the lane of rank rho visits
+// class ((ti, tj, tk) + decode(rho)) mod 3), so lanes of different rank add to
+// different LDS addresses in the same instruction. The LDS sums of ub are 64-bit fixed
+// point (ds_add_u64) in a unit chosen per workgroup from a bound on its particles' ub:
+// float atomic adds on LDS cost about as much as the rest of the node loop three times
+// over, the integer adds a small part of it, and the workgroup's sums no longer depend on
+// the order of its adds. A workgroup whose bound is not finite (a NaN or infinity in
+// the adjoints) adds in float through global memory, so the NaN reaches the grid.
 __global__ __launch_bounds__(kAT) void k_adj_g2p(mpm_particles_t P, mpm_grid_t G, mpm_adjoint_t A,
                                                  const mpm_model_t* __restrict__ md, float dt) {
     constexpr int kArena = arena_of<kAT>();
     __shared__ float4 s_u[kArena];
-    __shared__ float s_ub[3][kArena];
+    __shared__ unsigned long long s_ub[3][kArena];
     __shared__ int s_red[kAT / 64][8];
+    __shared__ int s_bound[kAT / 64];
     (void)md;
     const int p = blockIdx.x * kAT + threadIdx.x;
     const int S = P.stride;
@@ -120,86 +138,155 @@ __global__ __launch_bounds__(kAT) void k_adj_g2p(mpm_particles_t P, mpm_grid_t G
         lo[a] = moving ? st.base[a] : INT_MAX;
         hi[a] = moving ? st.base[a] + 3 : INT_MIN;
     }
-    const Tile t = make_tile<kAT>(G, lo, hi, moving ? 0 : INT_MAX, moving ? 0 : INT_MIN, s_red, false);
-    const float4* __restrict__ uo = reinterpret_cast<const float4*>(G.node_out);
-    tile_for<kAT>(t, [&](int a, int gi, int gj, int gk) {
-        s_u[a] = in_grid(G, gi, gj, gk) ? uo[node_index(G, gi, gj, gk)] : make_float4(0.f, 0.f, 0.f, 0.f);
-        s_ub[0][a] = 0.f;
-        s_ub[1][a] = 0.f;
-        s_ub[2][a] = 0.f;
-    });
-    __syncthreads();
+    const Tile tl = make_tile<kAT>(G, lo, hi, moving ? 0 : INT_MAX, moving ? 0 : INT_MIN, s_red, false);
+    const float c4 = 4.0f * inv_dx;
+    m33 Cb1 = mzero(), Lb = mzero(), Fb = mzero();
+    v3 xb1 = mk3(0.f, 0.f, 0.f), vt = mk3(0.f, 0.f, 0.f);
+    float bound = 0.f;
     if (moving) {
         const m33 F = ld9(P.F, S, p);
         const m33 L = ld9(P.L, S, p);
-        const v3 xb1 = ld3(A.x, S, p);
+        xb1 = ld3(A.x, S, p);
         const v3 vb1 = ld3(A.v, S, p);
-        const m33 Cb1 = ld9(A.C, S, p);
+        Cb1 = ld9(A.C, S, p);
         const m33 Ftb1 = ld9(A.F_trial, S, p);
-        const v3 vt = vb1 + xb1 * dt;                   // x' = x + dt v'
+        vt = vb1 + xb1 * dt;  // x' = x + dt v'
         // F_trial' = (I + dt L) F, plus the adjoint of L from the next substep's stress
-        const m33 Lb = madd(mscale(mmul(Ftb1, mT(F)), dt), ld9(A.L, S, p));
+        Lb = madd(mscale(mmul(Ftb1, mT(F)), dt), ld9(A.L, S, p));
         m33 IL = mscale(L, dt);
         IL.a[0] += 1.f; IL.a[4] += 1.f; IL.a[8] += 1.f;
-        const m33 Fb = mmul(mT(IL), Ftb1);
-        v3 xb = xb1;
-        const float c4 = 4.0f * inv_dx;
-        const int o0 = st.base[0] - t.lo[0], o1 = st.base[1] - t.lo[1], o2 = st.base[2] - t.lo[2];
+        Fb = mmul(mT(IL), Ftb1);
+        // every component of every ub of this particle is below this (w, |dw| <= 1, |q| <= 1.5)
+        float sc = 0.f, sl = 0.f;
+#pragma unroll
+        for (int e = 0; e < 9; ++e) {
+            sc += fabsf(Cb1.a[e]);
+            sl += fabsf(Lb.a[e]);
+        }
+        bound = fabsf(vt.x) + fabsf(vt.y) + fabsf(vt.z) + 1.5f * c4 * sc + inv_dx * sl;
+    }
+    // the workgroup's largest bound, as float bits (order-preserving for values >= 0; NaN
+    // and infinity sort above every finite value)
+    const int wmax = wave_extreme<true>(__float_as_int(bound));
+    if ((threadIdx.x & 63u) == 0) s_bound[threadIdx.x >> 6] = wmax;
+    const float4* __restrict__ uo = reinterpret_cast<const float4*>(G.node_out);
+    tile_for<kAT>(tl, [&](int a, int gi, int gj, int gk) {
+        s_u[a] = in_grid(G, gi, gj, gk) ? uo[node_index(G, gi, gj, gk)] : make_float4(0.f, 0.f, 0.f, 0.f);
+        s_ub[0][a] = 0ull;
+        s_ub[1][a] = 0ull;
+        s_ub[2][a] = 0ull;
+    });
+    __syncthreads();
+    int bmax = s_bound[0];
+#pragma unroll
+    for (int w = 1; w < kAT / 64; ++w) bmax = max(bmax, s_bound[w]);
+    // fixed-point unit 2^(51 - e) with bound < 2^e: each contribution stays below 2^51 units
+    // and 256 particles' sum on one node below 2^59
+    const int ex = max(((bmax >> 23) & 0xff) - 126, -75);
+    const float unit = __int_as_float((178 - ex) << 23);
+    const double inv_unit = __longlong_as_double((long long)(972 + ex) << 52);
+    const bool fixed = tl.lds && bmax < 0x7f800000;
+    if (moving) {
+        const float k1 = inv_dx, k2 = c4 * inv_dx;
+        const m33 Lbs = mscale(Lb, inv_dx), Lb2 = mscale(Lb, inv_dx * inv_dx);
         float* __restrict__ gub = A.grid_v;
-        // Conflict-free node order: every stencil holds one node of each residue class
-        // (x, y, z mod 3); at step s the lane of rank rho = lane mod 27 visits class
-        // (s + rho) mod 27, so lanes of different rank add to different LDS addresses in
-        // the same instruction, whichever cells their particles sit in.
+        // rotated per-axis stencil values: slot t of axis a is stencil index (t + r0_a) mod 3
         const int rho = (int)(threadIdx.x % 27u);
-        const int i0 = ((rho / 9 - st.base[0]) % 3 + 3) % 3, j0 = (((rho / 3) % 3 - st.base[1]) % 3 + 3) % 3,
-                  k0 = ((rho % 3 - st.base[2]) % 3 + 3) % 3;
-#pragma unroll 3
-        for (int s = 0; s < 27; ++s) {
-            int i = s / 9 + i0, j = (s / 3) % 3 + j0, k = s % 3 + k0;
-            i = i >= 3 ? i - 3 : i;
-            j = j >= 3 ? j - 3 : j;
-            k = k >= 3 ? k - 3 : k;
-            const int gi = st.base[0] + i, gj = st.base[1] + j, gk = st.base[2] + k;
-            const bool ing = in_grid(G, gi, gj, gk);
-            int a = 0;
-            float4 u4;
-            if (t.lds) {
-                a = t.base + ((o0 + i) * t.d1 + (o1 + j)) * t.d2 + (o2 + k);
-                u4 = s_u[a];
-            } else {
-                if (!ing) continue;
-                u4 = uo[node_index(G, gi, gj, gk)];
-            }
-            const v3 u = mk3(u4.x, u4.y, u4.z);
-            const NodeW nw = node_w(st, i, j, k);
-            const v3 Cq = mvmul(Cb1, nw.q);
-            // position adjoint through the weights, offsets and weight gradients
-            xb = xb + nw.dw * (inv_dx * dot(u, vt) + c4 * inv_dx * dot(u, Cq)) -
-                 mTvmul(Cb1, u) * (c4 * inv_dx * nw.w) + mvmul(nw.H, mTvmul(Lb, u)) * (inv_dx * inv_dx);
-            const v3 ub = vt * nw.w + Cq * (c4 * nw.w) + mvmul(Lb, nw.dw) * inv_dx;
-            if (t.lds) {
-                atomicAdd(&s_ub[0][a], ub.x);
-                atomicAdd(&s_ub[1][a], ub.y);
-                atomicAdd(&s_ub[2][a], ub.z);
-            } else {
-                const size_t o = 4 * (size_t)node_index(G, gi, gj, gk);
-                atomicAdd(gub + o + 0, ub.x);
-                atomicAdd(gub + o + 1, ub.y);
-                atomicAdd(gub + o + 2, ub.z);
+        const int r0[3] = {((rho / 9 - st.base[0]) % 3 + 3) % 3, (((rho / 3) % 3 - st.base[1]) % 3 + 3) % 3,
+                           ((rho % 3 - st.base[2]) % 3 + 3) % 3};
+        int ix[3][3];
+        float wv[3][3], dv[3][3], hv[3][3], qv[3][3];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+#pragma unroll
+            for (int t = 0; t < 3; ++t) {
+                const int r = r0[a] + t >= 3 ? r0[a] + t - 3 : r0[a] + t;
+                ix[a][t] = r;
+                wv[a][t] = sel3f(r, st.w[a]);
+                dv[a][t] = sel3f(r, st.dw[a]);
+                hv[a][t] = r == 1 ? -2.0f : 1.0f;
+                qv[a][t] = (float)r - st.fx[a];
             }
         }
-        st3(A.x, S, p, xb);
+        v3 xb = mk3(0.f, 0.f, 0.f), su = mk3(0.f, 0.f, 0.f);
+        auto nodes = [&](auto lds_tag) {
+            constexpr bool kLds = decltype(lds_tag)::value;
+            const int o0 = st.base[0] - tl.lo[0], o1 = st.base[1] - tl.lo[1], o2 = st.base[2] - tl.lo[2];
+#pragma unroll
+            for (int ti = 0; ti < 3; ++ti) {
+                const int i = ix[0][ti];
+                const float w0 = wv[0][ti], d0 = dv[0][ti], h0 = hv[0][ti];
+                const v3 Cq0 = mk3(Cb1.a[0], Cb1.a[3], Cb1.a[6]) * qv[0][ti];
+#pragma unroll
+                for (int tj = 0; tj < 3; ++tj) {
+                    const int j = ix[1][tj];
+                    const float w1 = wv[1][tj], d1 = dv[1][tj], h1 = hv[1][tj];
+                    const float w01 = w0 * w1, d0w1 = d0 * w1, w0d1 = w0 * d1, d0d1 = d0 * d1, h0w1 = h0 * w1,
+                                w0h1 = w0 * h1;
+                    const v3 Cq01 = fma3(mk3(Cb1.a[1], Cb1.a[4], Cb1.a[7]), qv[1][tj], Cq0);
+                    // Lb dw / dx = (Lbs e0 d0w1 + Lbs e1 w0d1) w2 + Lbs e2 w01 d2
+                    const v3 La = mk3(Lbs.a[0], Lbs.a[3], Lbs.a[6]) * d0w1 + mk3(Lbs.a[1], Lbs.a[4], Lbs.a[7]) * w0d1;
+                    const v3 Lc = mk3(Lbs.a[2], Lbs.a[5], Lbs.a[8]) * w01;
+                    const int row = kLds ? tl.base + ((o0 + i) * tl.d1 + (o1 + j)) * tl.d2 + o2 : 0;
+#pragma unroll
+                    for (int tk = 0; tk < 3; ++tk) {
+                        const int k = ix[2][tk];
+                        const float w2 = wv[2][tk], d2 = dv[2][tk], h2 = hv[2][tk];
+                        float4 u4;
+                        int a = 0;
+                        size_t o = 0;
+                        if constexpr (kLds) {
+                            a = row + k;
+                            u4 = s_u[a];
+                        } else {
+                            const int gi = st.base[0] + i, gj = st.base[1] + j, gk = st.base[2] + k;
+                            if (!in_grid(G, gi, gj, gk)) continue;
+                            o = 4 * (size_t)node_index(G, gi, gj, gk);
+                            u4 = uo[o / 4];
+                        }
+                        const v3 u = mk3(u4.x, u4.y, u4.z);
+                        const float w = w01 * w2;
+                        const v3 dw = mk3(d0w1 * w2, w0d1 * w2, w01 * d2);
+                        const v3 Cq = fma3(mk3(Cb1.a[2], Cb1.a[5], Cb1.a[8]), qv[2][tk], Cq01);
+                        const float T = k1 * dot(u, vt) + k2 * dot(u, Cq);
+                        xb = fma3(dw, T, xb);
+                        su = fma3(u, w, su);
+                        // H (Lb^T u) / dx^2, H symmetric
+                        const v3 lv = mTvmul(Lb2, u);
+                        const float H00 = h0w1 * w2, H11 = w0h1 * w2, H22 = w01 * h2;
+                        const float H01 = d0d1 * w2, H02 = d0w1 * d2, H12 = w0d1 * d2;
+                        xb = xb + mk3(H00 * lv.x + H01 * lv.y + H02 * lv.z, H01 * lv.x + H11 * lv.y + H12 * lv.z,
+                                      H02 * lv.x + H12 * lv.y + H22 * lv.z);
+                        const v3 ub = fma3(La, w2, Lc * d2) + fma3(Cq, c4, vt) * w;
+                        if constexpr (kLds) {
+                            atomicAdd(&s_ub[0][a], fix_round(ub.x * unit));
+                            atomicAdd(&s_ub[1][a], fix_round(ub.y * unit));
+                            atomicAdd(&s_ub[2][a], fix_round(ub.z * unit));
+                        } else {
+                            atomicAdd(gub + o + 0, ub.x);
+                            atomicAdd(gub + o + 1, ub.y);
+                            atomicAdd(gub + o + 2, ub.z);
+                        }
+                    }
+                }
+            }
+        };
+        if (fixed) nodes(std::true_type{});
+        else nodes(std::false_type{});
+        // the weights' term linear in u, summed first: - c4 / dx Cb^T (sum w u)
+        xb = xb - mTvmul(Cb1, su) * k2;
+        st3(A.x, S, p, xb1 + xb);
         st9(A.F, S, p, Fb);
     }
     __syncthreads();
     float* __restrict__ gub = A.grid_v;
-    tile_for<kAT>(t, [&](int a, int gi, int gj, int gk) {
-        const float b0 = s_ub[0][a], b1 = s_ub[1][a], b2 = s_ub[2][a];
-        if ((b0 == 0.f && b1 == 0.f && b2 == 0.f) || !in_grid(G, gi, gj, gk)) return;
+    tile_for<kAT>(tl, [&](int a, int gi, int gj, int gk) {
+        const long long q0 = (long long)s_ub[0][a], q1 = (long long)s_ub[1][a], q2 = (long long)s_ub[2][a];
+        if ((q0 | q1 | q2) == 0 || !in_grid(G, gi, gj, gk)) return;
         const size_t o = 4 * (size_t)node_index(G, gi, gj, gk);
-        atomicAdd(gub + o + 0, b0);
-        atomicAdd(gub + o + 1, b1);
-        atomicAdd(gub + o + 2, b2);
+        atomicAdd(gub + o + 0, (float)((double)q0 * inv_unit));
+        atomicAdd(gub + o + 1, (float)((double)q1 * inv_unit));
+        atomicAdd(gub + o + 2, (float)((double)q2 * inv_unit));
     });
 }
 

@@ -102,20 +102,6 @@ MPM_DEV void gather_tp(const Stencil& st, float inv_dx, Fetch fetch, v3& nv, m33
     nF.a[6] = fX.z * inv_dx; nF.a[7] = fY.z * inv_dx; nF.a[8] = fZ.z * inv_dx;
 }
 
-// Deterministic scatter: every contribution in fixed point of scale S (a power of two),
-// rounded to the nearest unit (ties to even). Integer sums do not depend on the order the
-// atomics land in. The rounding goes through float64: y is exact there, and y + 1.5 *
-// 2^52 lies in the binade [2^52, 2^53), whose spacing is 1, so the add rounds y to an
-// integer and the difference of the two bit patterns is that integer (for |y| < 2^51;
-// 2^12 such contributions to one node still fit an int64). Three instructions instead of
-// the generic float -> int64 conversion, 108 times per particle.
-MPM_DEV int mod3(int v) { return ((v % 3) + 3) % 3; }
-
-MPM_DEV unsigned long long fix_round(float y) {
-    const double magic = 6755399441055744.0;  // 1.5 * 2^52
-    return (unsigned long long)(__double_as_longlong((double)y + magic) - __double_as_longlong(magic));
-}
-
 // Fixed-point scatter of one particle in tensor-product form. The reference's momentum
 // contribution to stencil node (i, j, k) (p2g_particle, mpm_utils.py:880-892),
 //   w m (v + C dpos) + fs grad w,   w = w_i w_j w_k,  dpos = ((i, j, k) - f) dx,
