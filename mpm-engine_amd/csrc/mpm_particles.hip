@@ -594,7 +594,22 @@ __global__ __launch_bounds__(kT) __attribute__((amdgpu_waves_per_eu(MPM_PASS_WAV
                 __syncthreads();
             }
             // blocked deposits (mpm_utils.py:894-912): a contribution to an incompatible
-            // node goes to its owner lane's reaction wrench instead of the grid
+            // node goes to its owner lane's reaction wrench instead of the grid. Each lane
+            // sums its own in registers (force rf, torque rt, for owner r_own; a change of
+            // owner, rare, adds the running sum to LDS first), and after the scatter each
+            // wavefront adds one DPP-summed wrench per owner: every lane adding to the same
+            // six LDS words per node serialised the wavefront's float atomics.
+            int r_own = -1;
+            v3 rf = mk3(0.f, 0.f, 0.f), rt = mk3(0.f, 0.f, 0.f);
+            auto react_to_lds = [&]() {
+                float* r = s_react + 6 * r_own;
+                atomicAdd(r + 0, rf.x);
+                atomicAdd(r + 1, rf.y);
+                atomicAdd(r + 2, rf.z);
+                atomicAdd(r + 3, rt.x);
+                atomicAdd(r + 4, rt.y);
+                atomicAdd(r + 5, rt.z);
+            };
             auto mask = [&](int i, int j, int k, v3& add, float& wm) {
                 if constexpr (kCdf) {
                     if (pc_s != 0) {
@@ -606,13 +621,14 @@ __global__ __launch_bounds__(kT) __attribute__((amdgpu_waves_per_eu(MPM_PASS_WAV
                                 const float* c = G.cdf.lanes->center[own];
                                 const v3 xw = mk3((float)gi * G.dx, (float)gj * G.dx, (float)gk * G.dx);
                                 const v3 tq = cross(xw - mk3(c[0], c[1], c[2]), add);
-                                float* r = s_react + 6 * own;
-                                atomicAdd(r + 0, add.x);
-                                atomicAdd(r + 1, add.y);
-                                atomicAdd(r + 2, add.z);
-                                atomicAdd(r + 3, tq.x);
-                                atomicAdd(r + 4, tq.y);
-                                atomicAdd(r + 5, tq.z);
+                                if (own != r_own) {
+                                    if (r_own >= 0) react_to_lds();
+                                    r_own = own;
+                                    rf = mk3(0.f, 0.f, 0.f);
+                                    rt = mk3(0.f, 0.f, 0.f);
+                                }
+                                rf = rf + add;
+                                rt = rt + tq;
                                 add = mk3(0.f, 0.f, 0.f);
                                 wm = 0.f;
                                 return false;
@@ -702,6 +718,22 @@ __global__ __launch_bounds__(kT) __attribute__((amdgpu_waves_per_eu(MPM_PASS_WAV
                                     });
                 } else {
                     scatter_global<kDet>(G, soff, st, periodic, mass, v, Cm, fs, mask, err);
+                }
+            }
+            if constexpr (kCdf) {  // all lanes converged: the wavefront's wrench per owner
+                if (__ballot(r_own >= 0)) {
+#pragma unroll 1
+                    for (int o = 0; o < MPM_MAX_CDF; ++o) {
+                        const bool mine = r_own == o;
+                        if (!__ballot(mine)) continue;
+                        const float w6[6] = {wave_sum(mine ? rf.x : 0.f), wave_sum(mine ? rf.y : 0.f),
+                                             wave_sum(mine ? rf.z : 0.f), wave_sum(mine ? rt.x : 0.f),
+                                             wave_sum(mine ? rt.y : 0.f), wave_sum(mine ? rt.z : 0.f)};
+                        if ((threadIdx.x & 63u) == 0) {
+#pragma unroll
+                            for (int q = 0; q < 6; ++q) atomicAdd(s_react + 6 * o + q, w6[q]);
+                        }
+                    }
                 }
             }
             __syncthreads();

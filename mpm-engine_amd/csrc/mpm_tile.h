@@ -55,6 +55,26 @@ MPM_DEV bool fits(const int lo[3], const int hi[3], int cap, int& d0, int& d1, i
     return d0 * d1 * d2 <= cap;
 }
 
+// Wavefront sum with DPP (all 64 lanes active): row shifts by 1, 2, 4, 8 leave each row's
+// inclusive prefix sum in its lanes (lanes shifted in from outside the row add 0), so
+// lane 15 holds the row sum; row_bcast:15 adds rows 0 and 2 into rows 1 and 3 and
+// row_bcast:31 rows 0-1 into rows 2 and 3: lane 63 ends with the total, which every lane
+// reads. A fixed order, so the sum is the same in every run.
+template <int kCtrl, int kRowMask>
+MPM_DEV float dpp_add(float x) {
+    const int y = __builtin_amdgcn_update_dpp(0, __float_as_int(x), kCtrl, kRowMask, 0xf, false);
+    return x + __int_as_float(y);
+}
+MPM_DEV float wave_sum(float v) {
+    v = dpp_add<0x111, 0xf>(v);  // row_shr:1
+    v = dpp_add<0x112, 0xf>(v);  // row_shr:2
+    v = dpp_add<0x114, 0xf>(v);  // row_shr:4
+    v = dpp_add<0x118, 0xf>(v);  // row_shr:8
+    v = dpp_add<0x142, 0xa>(v);  // row_bcast:15
+    v = dpp_add<0x143, 0xc>(v);  // row_bcast:31
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
+}
+
 // Wavefront min / max with DPP (all 64 lanes active): row shifts by 1, 2, 4, 8 leave the
 // row's extreme in its lane 15 (min and max are idempotent, so overlapping windows are
 // fine; lanes shifted in from outside the row keep the identity), row_bcast:15 and
