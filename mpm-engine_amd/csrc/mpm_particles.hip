@@ -348,9 +348,11 @@ __global__ __launch_bounds__(kT) __attribute__((amdgpu_waves_per_eu(MPM_PASS_WAV
     constexpr bool kLean = (MODE & MPM_PASS_LEAN) != 0;
     constexpr bool kKeepStress = (MODE & MPM_PASS_KEEP_STRESS) != 0;
     __shared__ float4 s_gather[kG2P ? kArena : 1];
-    // scatter arena: float sums, or fixed-point integer sums in the deterministic mode
-    using Acc = typename std::conditional<kDet, unsigned long long, float>::type;
+    // scatter arena: 64-bit fixed-point sums, in the grid's global unit in the deterministic
+    // mode, in a per-workgroup unit otherwise (whose flush adds floats to the grid)
+    using Acc = unsigned long long;
     __shared__ Acc s_scatter[kP2G ? 4 : 1][kP2G ? kArena : 1];
+    __shared__ int s_bnd[kDet ? 1 : kT / 64][2];
     __shared__ int s_red0[kT / 64][8];
     __shared__ int s_red1[kT / 64][8];
     __shared__ float s_react[kCdf && kP2G ? 6 * MPM_MAX_CDF : 1];  // blocked deposits per lane
@@ -651,6 +653,48 @@ __global__ __launch_bounds__(kT) __attribute__((amdgpu_waves_per_eu(MPM_PASS_WAV
                 mass = kEarly ? mass_v : P.mass[p];
                 if (!stencil_in_grid(G, st, periodic)) err |= 1;
             }
+            // Float mode: the LDS sums are fixed point too (float atomic adds on LDS are
+            // several times slower than 64-bit integer ones), in units 2^(46 - e) chosen per
+            // workgroup from bounds below 2^e on its particles' contributions, one for
+            // momentum and one for mass: a contribution stays below 2^46 units, a 16-lane
+            // segment sum below 2^50 and the workgroup's sum on a node below 2^54. With
+            // |w|, |dpos| / (1.5 dx), |grad w| dx all at most 1, every momentum component
+            // is below m (|v|_1 + 1.5 dx sum|C|) + sum|fs| / dx. A workgroup with a bound
+            // that is not finite scatters in float through global memory.
+            float unit_p = 0.f, unit_m = 0.f;
+            double inv_p = 0.0, inv_m = 0.0;
+            bool fin = true;
+            if constexpr (!kDet) {
+                float bp = 0.f, bm = 0.f;
+                if (alive) {
+                    float sc = 0.f, sf = 0.f;
+#pragma unroll
+                    for (int e = 0; e < 9; ++e) {
+                        sc += fabsf(Cm.a[e]);
+                        sf += fabsf(fs.a[e]);
+                    }
+                    bm = fabsf(mass);
+                    bp = bm * (fabsf(v.x) + fabsf(v.y) + fabsf(v.z) + 1.5f * G.dx * sc) + G.inv_dx * sf;
+                }
+                const int wp = wave_extreme<true>(__float_as_int(bp)), wq = wave_extreme<true>(__float_as_int(bm));
+                if ((threadIdx.x & 63u) == 0) {
+                    s_bnd[threadIdx.x >> 6][0] = wp;
+                    s_bnd[threadIdx.x >> 6][1] = wq;
+                }
+                __syncthreads();
+                int mp = s_bnd[0][0], mq = s_bnd[0][1];
+#pragma unroll
+                for (int w = 1; w < kT / 64; ++w) {
+                    mp = max(mp, s_bnd[w][0]);
+                    mq = max(mq, s_bnd[w][1]);
+                }
+                fin = mp < 0x7f800000 && mq < 0x7f800000;
+                const int ep = max(((mp >> 23) & 0xff) - 126, -80), em = max(((mq >> 23) & 0xff) - 126, -80);
+                unit_p = __int_as_float((173 - ep) << 23);
+                unit_m = __int_as_float((173 - em) << 23);
+                inv_p = __longlong_as_double((long long)(977 + ep) << 52);
+                inv_m = __longlong_as_double((long long)(977 + em) << 52);
+            }
             // Lanes whose particles share a stencil base cell (consecutive after a cell
             // sort) form segments; when a wavefront has enough of them, each segment is
             // summed with DPP row shifts and only its last lane (per 16-lane row) adds
@@ -662,7 +706,7 @@ __global__ __launch_bounds__(kT) __attribute__((amdgpu_waves_per_eu(MPM_PASS_WAV
             const bool head = lane == 0 || prevkey != cellkey;
             const unsigned long long heads = __ballot(head);
             // (never in the deterministic mode: a float pre-sum would depend on the grouping)
-            const bool segmented = !kDet && ts.lds && __popcll(~heads) >= 16;  // wavefront-uniform
+            const bool segmented = !kDet && ts.lds && fin && __popcll(~heads) >= 16;  // wavefront-uniform
             if (segmented) {
                 const unsigned long long upto = lane == 63 ? ~0ull : ((1ull << (lane + 1)) - 1ull);
                 const int seg0 = 63 - __clzll(heads & upto);
@@ -682,10 +726,10 @@ __global__ __launch_bounds__(kT) __attribute__((amdgpu_waves_per_eu(MPM_PASS_WAV
                     if (tail && alive) {
                         const int a = b0 + ((o0 + i) * d1 + (o1 + j)) * d2 + (o2 + k);
                         if constexpr (!kDet) {
-                            atomicAdd(&s_scatter[0][a], s0);
-                            atomicAdd(&s_scatter[1][a], s1);
-                            atomicAdd(&s_scatter[2][a], s2);
-                            atomicAdd(&s_scatter[3][a], s3);
+                            atomicAdd(&s_scatter[0][a], fix_round(s0 * unit_p));
+                            atomicAdd(&s_scatter[1][a], fix_round(s1 * unit_p));
+                            atomicAdd(&s_scatter[2][a], fix_round(s2 * unit_p));
+                            atomicAdd(&s_scatter[3][a], fix_round(s3 * unit_m));
                         }
                     }
                 });
@@ -697,7 +741,7 @@ __global__ __launch_bounds__(kT) __attribute__((amdgpu_waves_per_eu(MPM_PASS_WAV
                                                lane, d12, d2, [&](int, int, int, int off, unsigned long long q0,
                                                    unsigned long long q1, unsigned long long q2, unsigned long long q3) {
                     const int a = r0 + off;
-                    unsigned long long* sc = reinterpret_cast<unsigned long long*>(&s_scatter[0][0]);
+                    Acc* sc = &s_scatter[0][0];
                     atomicAdd(sc + a, q0);
                     atomicAdd(sc + kArena + a, q1);
                     atomicAdd(sc + 2 * kArena + a, q2);
@@ -705,16 +749,16 @@ __global__ __launch_bounds__(kT) __attribute__((amdgpu_waves_per_eu(MPM_PASS_WAV
                 });
                 if (!ok) err |= 64;
             } else if (alive) {
-                if (ts.lds) {
+                if (ts.lds && fin) {
                     const int d1 = ts.d1, d2 = ts.d2, b0 = ts.base;
                     scatter_rotated<true>(st, G.dx, G.inv_dx, mass, v, Cm, fs, (int)(threadIdx.x % 27u),
                                     [&](int i, int j, int k, v3 add, float wm) {
                                         if (!mask(i, j, k, add, wm)) return;
                                         const int a = b0 + ((o0 + i) * d1 + (o1 + j)) * d2 + (o2 + k);
-                                        atomicAdd(&s_scatter[0][a], add.x);
-                                        atomicAdd(&s_scatter[1][a], add.y);
-                                        atomicAdd(&s_scatter[2][a], add.z);
-                                        atomicAdd(&s_scatter[3][a], wm);
+                                        atomicAdd(&s_scatter[0][a], fix_round(add.x * unit_p));
+                                        atomicAdd(&s_scatter[1][a], fix_round(add.y * unit_p));
+                                        atomicAdd(&s_scatter[2][a], fix_round(add.z * unit_p));
+                                        atomicAdd(&s_scatter[3][a], fix_round(wm * unit_m));
                                     });
                 } else {
                     scatter_global<kDet>(G, soff, st, periodic, mass, v, Cm, fs, mask, err);
@@ -756,10 +800,10 @@ __global__ __launch_bounds__(kT) __attribute__((amdgpu_waves_per_eu(MPM_PASS_WAV
                         atomicAdd(gfix + o + 2, a2);
                         atomicAdd(gfix + o + 3, a3);
                     } else {
-                        atomicAdd(gin + o + 0, a0);
-                        atomicAdd(gin + o + 1, a1);
-                        atomicAdd(gin + o + 2, a2);
-                        atomicAdd(gin + o + 3, a3);
+                        atomicAdd(gin + o + 0, (float)((double)(long long)a0 * inv_p));
+                        atomicAdd(gin + o + 1, (float)((double)(long long)a1 * inv_p));
+                        atomicAdd(gin + o + 2, (float)((double)(long long)a2 * inv_p));
+                        atomicAdd(gin + o + 3, (float)((double)(long long)a3 * inv_m));
                     }
                 });
                 mark_tile_blocks<kT>(G, ts);
