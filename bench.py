@@ -12,7 +12,8 @@ workload. Two more legs ride in the same line:
              100-substep hipGraph chunks, autotuned launch shape and sort interval, against
              the same scenes solved one after another with the plain Solver;
   "cdf_pour" the same pour with both glasses as CPIC thin-boundary sheets;
-  "float_atomics" the pour with the reference's float-atomic scatter instead of the
+  "float_atomics" the pour with the float scatter (float atomics into the grid, as the
+             reference accumulates; per-workgroup fixed-point sums in LDS) instead of the
              default fixed-point (bit-reproducible) one;
   "adjoint"  the squeeze identification (50k von Mises particles, 400 substeps): gradient
              of a loss on final positions and plate forces, backward time / forward time.
@@ -570,8 +571,9 @@ def cdf_leg(args):
 
 
 def float_leg(args):
-    """The headline pour with Solver(deterministic=False): the float-atomic scatter (the
-    reference's accumulation) instead of the default 64-bit fixed-point integer one.
+    """The headline pour with Solver(deterministic=False): the float scatter (float atomics
+    into the grid, the reference's accumulation, from per-workgroup fixed-point LDS sums)
+    instead of the default 64-bit fixed-point integer one.
     Frame time and fused-pass time for comparison with the headline."""
     import numpy as np
     import torch

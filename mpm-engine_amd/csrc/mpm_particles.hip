@@ -6,8 +6,9 @@
 // LDS staging (claymore-style, on block-sorted particles): the workgroup reduces the
 // stencil bounding box of its particles; when the box holds at most kArena nodes it
 // loads the gather box of node_out into LDS once (one 16-byte read per node instead of
-// 27 per particle) and scatters into an LDS arena with ds_add_f32, then flushes each
-// touched arena node to node_in with four global float atomics. Chunks whose box is
+// 27 per particle) and scatters into an LDS arena of 64-bit fixed-point sums
+// (ds_add_u64), then flushes each touched arena node with four global atomics: integer
+// ones to node_fix in the deterministic mode, float ones to node_in otherwise. Chunks whose box is
 // larger (unsorted or scattered particles) fall back to direct global gathers and
 // atomics, so results never depend on the particle order. Everything that decides a
 // barrier is reduced first, so every branch around __syncthreads is workgroup-uniform.
